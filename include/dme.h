@@ -1,0 +1,263 @@
+/*
+ * dme.h -- C ABI of the MI355X (gfx950) motion-estimation engine for the
+ * Dirac encoder.
+ *
+ * The reference exposes this path only as C++ classes in libdirac_motionest
+ * (there is no plugin registry or FFI in the reference).  Each entry point
+ * below replaces one of those class methods; the citation on each names the
+ * method and the file:line of its declaration in the reference tree.  All
+ * array layouts are those of the reference's MEData (libdirac_common/motion.h)
+ * flattened row-major, because TwoDArray is contiguous (arrays.h:502-542):
+ * a caller can pass &array[0][0] of its own MEData and get the results
+ * written in place.
+ *
+ * Conventions
+ *   - every function returns an int status: DME_OK (0) or a negative code;
+ *     dme_last_error() returns a message for the calling thread's last error.
+ *   - no C++ exceptions cross this boundary.
+ *   - one dme_ctx per host thread; a context is bound to one HIP device.
+ *   - results are deterministic and bit-exact against the reference for the
+ *     same inputs and parameters (MMX build semantics by default).
+ *   - picture samples are int16, 8-bit video stored minus 128, as the
+ *     reference stores them (libdirac_common/pic_io.cpp:396-401).
+ */
+#ifndef DME_H
+#define DME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DME_OK               0
+#define DME_ERR_INVALID     -1  /* bad argument or unsupported parameter set  */
+#define DME_ERR_HIP         -2  /* a HIP runtime call failed                  */
+#define DME_ERR_NOMEM       -3  /* device or host allocation failed           */
+#define DME_ERR_STATE       -4  /* call out of order (e.g. subpel before pel)  */
+#define DME_ERR_NO_DEVICE   -5  /* no HIP device / kernels missing            */
+
+/* ---- prediction modes: values of enum PredMode (common.h:84) ------------ */
+#define DME_INTRA      0
+#define DME_REF1_ONLY  1
+#define DME_REF2_ONLY  2
+#define DME_REF1AND2   3
+
+/* ---- float semantics of the sub-pel cost functions ---------------------- */
+/* The reference's default build defines HAVE_MMX (configure.ac:303-326); its
+ * in-bounds up-sampled SAD adds the integer SAD to the start cost once
+ * (me_utils_mmx.cpp:270-278).  The plain C build accumulates per pixel in
+ * float (me_utils.cpp:532-541).  Both are supported.                        */
+#define DME_FP_MMX  0
+#define DME_FP_C    1
+
+/* MVector (motion.h:57-183, MotionVector<int>) */
+typedef struct dme_mv {
+    int32_t x, y;
+} dme_mv;
+
+/* MvCostData (motion.h:192-211) */
+typedef struct dme_cost {
+    float SAD, mvcost, total;
+} dme_cost;
+
+/* Motion-estimation parameters.  Block parameters are the raw level-2 values
+ * handed to PicturePredParams::SetBlockSizes (common.cpp:207-320), which
+ * rationalises them exactly as the reference does; level-1/0 parameters are
+ * derived from them there.                                                   */
+typedef struct dme_params {
+    int32_t xblen, yblen, xbsep, ybsep;
+    int32_t mv_precision;  /* 0 pel, 1 half, 2 quarter (MVPrecisionType)     */
+    int32_t depth;         /* pyramid depth; < 0 selects the reference's own
+                              formula floor(min(log2(W/12), log2(H/12)))
+                              (pixel_match.cpp:94-95)                          */
+    float   lambda;        /* ME Lagrangian (L1MELambda / L2MELambda)        */
+    int32_t fp_semantics;  /* DME_FP_MMX (default) or DME_FP_C               */
+    int32_t chroma_format; /* 0 = 4:2:0 (only value supported)               */
+    int32_t reserved[7];   /* must be zero                                   */
+} dme_params;
+
+/* Derived geometry for a picture of a given size (SetMotionParameters,
+ * seq_compress.cpp:107-133; MakeMEDataHierarchy, pixel_match.cpp:185-220).   */
+#define DME_MAX_DEPTH 8
+typedef struct dme_geometry {
+    int32_t width, height, chroma_width, chroma_height;
+    int32_t xnum_sb, ynum_sb, xnum_blocks, ynum_blocks;
+    int32_t depth;
+    int32_t level_width[DME_MAX_DEPTH + 1];   /* pyramid picture sizes        */
+    int32_t level_height[DME_MAX_DEPTH + 1];
+    int32_t level_xblocks[DME_MAX_DEPTH + 1]; /* pel-search block grid       */
+    int32_t level_yblocks[DME_MAX_DEPTH + 1];
+    int32_t luma_blen[3][2], luma_bsep[3][2]; /* [level][x/y] after rationalising */
+    int32_t chroma_blen[3][2], chroma_bsep[3][2];
+    float   level_factor[3];                  /* ModeDecider level factors   */
+} dme_geometry;
+
+/* MEData of one picture (motion.h:219-410), caller-owned, any pointer may be
+ * NULL to skip that array.  Shapes: B = ynum_blocks*xnum_blocks elements in
+ * raster order, S = ynum_sb*xnum_sb.  Index [r] is reference r+1 (the
+ * reference indexes Vectors/PredCosts with Range(1,num_refs)).               */
+typedef struct dme_medata {
+    dme_mv   *vectors[2];      /* B  Vectors(1), Vectors(2)                   */
+    dme_cost *pred_costs[2];   /* B  PredCosts(1), PredCosts(2)               */
+    int32_t  *mode;            /* B  Mode (PredMode values)                   */
+    float    *intra_costs;     /* B  IntraCosts                               */
+    dme_cost *bipred_costs;    /* B  BiPredCosts                              */
+    int16_t  *dc[3];           /* B  DC(Y_COMP), DC(U_COMP), DC(V_COMP)       */
+    int32_t  *sb_split;        /* S  SBSplit                                  */
+    float    *sb_costs;        /* S  SBCosts                                  */
+    float    *lambda_map;      /* B  LambdaMap                                */
+} dme_medata;
+
+typedef struct dme_ctx dme_ctx;
+
+/* ---- host-only helpers (no GPU needed) --------------------------------- */
+
+/* Returns the message of the calling thread's last failed call ("" if none). */
+const char *dme_last_error(void);
+
+/* Library/ABI version: major*10000 + minor*100 + patch. */
+int dme_version(void);
+
+/* Geometry for a width x height 4:2:0 picture under params.
+ * Replaces SequenceCompressor::SetMotionParameters (seq_compress.cpp:107-133)
+ * + PicturePredParams::SetBlockSizes (common.cpp:207-320)
+ * + PixelMatcher::MakeMEDataHierarchy (pixel_match.cpp:185-220).           */
+int dme_geometry_for(const dme_params *params, int width, int height,
+                     dme_geometry *out);
+
+/* Host restatement of MEData::SetLambdaMap(num_refs, lambda) with
+ * FindTransitions (motion.cpp:128-171, 201-333), computed in x87 long double
+ * exactly as the reference does.  vectors[r] are the pel-search vectors of
+ * reference r+1 (B elements each), out_lambda_map receives B floats.        */
+int dme_lambda_map_host(int xnum_blocks, int ynum_blocks, int num_refs,
+                        const dme_mv *vectors1, const dme_mv *vectors2,
+                        float lambda, float *out_lambda_map);
+
+/* ---- device context ---------------------------------------------------- */
+
+/* Creates a context on HIP device `device`, with its own stream. */
+int dme_create(int device, dme_ctx **out);
+int dme_destroy(dme_ctx *ctx);
+
+/* Sets the parameters for subsequent calls (PixelMatcher/SubpelRefine/
+ * ModeDecider constructors take EncoderParams, pixel_match.h:71,
+ * me_subpel.h:66, me_mode_decn.h:79).                                       */
+int dme_set_params(dme_ctx *ctx, const dme_params *params);
+
+/* Registers picture `pic_num` (EncQueue::PushPicture + SetOrigData,
+ * enc_queue.cpp:179, enc_picture.cpp:100).  Planes are host pointers with
+ * stride = width (luma) and width/2 (chroma, 4:2:0); u and v may be NULL if
+ * the picture is only ever used as a reference.  The planes are copied to
+ * device memory.                                                            */
+int dme_upload_picture(dme_ctx *ctx, int pic_num, int width, int height,
+                       const int16_t *y, const int16_t *u, const int16_t *v);
+
+/* As dme_upload_picture, but the planes are already resident in device
+ * memory of the context's device and are used in place (not copied).  They
+ * must stay valid and unchanged until dme_drop_picture.                     */
+int dme_bind_picture_device(dme_ctx *ctx, int pic_num, int width, int height,
+                            const int16_t *d_y, const int16_t *d_u,
+                            const int16_t *d_v);
+
+/* Releases a picture and its cached pyramid / up-converted planes
+ * (EncQueue::ClearAll / EncPicture dtor).                                   */
+int dme_drop_picture(dme_ctx *ctx, int pic_num);
+
+/* PixelMatcher::DoSearch(EncQueue&, pic_num)  (pixel_match.h:81,
+ * pixel_match.cpp:59-157): hierarchical integer search of pic_num against
+ * refs[0..nrefs-1] (nrefs 1 or 2).  Fills out->vectors / out->pred_costs
+ * (pel units) if non-NULL.  Also initialises the picture's MEData
+ * (EncPicture::InitMEData, enc_picture.cpp:106-112).                       */
+int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int nrefs,
+                     dme_medata *out);
+
+/* MEData::SetLambdaMap(num_refs, lambda) (motion.h:362, motion.cpp:128-171),
+ * computed on the host from the pel-search vectors and uploaded.  If
+ * lambda_map is non-NULL it is used as given instead (B floats).            */
+int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda_map,
+                       dme_medata *out);
+
+/* SubpelRefine::DoSubpel(EncQueue&, pic_num) (me_subpel.h:77,
+ * me_subpel.cpp:65-99), or the pixel-precision vector doubling of
+ * PictureCompressor::SubPixelME (picture_compress.cpp:240-259).            */
+int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out);
+
+/* ModeDecider::DoModeDecn(EncQueue&, pic_num) (me_mode_decn.h:93,
+ * me_mode_decn.cpp:66-177) including SetDC, and the pixel-precision undo of
+ * PictureCompressor::ModeDecisionME (picture_compress.cpp:274-296).  Writes
+ * every non-NULL array of out.                                              */
+int dme_mode_decide(dme_ctx *ctx, int pic_num, dme_medata *out);
+
+/* The whole live ME sequence of one inter picture, in the order
+ * PictureCompressor runs it (picture_compress.cpp:76-80, 208-298):
+ * PixelME -> SetLambdaMap -> SubPixelME -> ModeDecisionME.  This is what the
+ * reference's (dead) MotionEstimator::DoME (motion_estimate.h:79) declares.  */
+int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, int nrefs,
+                        dme_medata *out);
+
+/* Batched form: runs the live sequence for njobs pictures in one pass, all
+ * stages of all pictures sharing launches.  Pictures may differ in size,
+ * reference count and block parameters (per-job params).  outs[i] receives
+ * job i's MEData.  No counterpart in the reference (which handles one
+ * picture at a time, seq_compress.cpp:256-310).                              */
+typedef struct dme_job {
+    int32_t pic_num;
+    int32_t nrefs;
+    int32_t refs[2];
+    dme_params params;
+} dme_job;
+int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs,
+                  dme_medata *outs);
+
+/* Waits for all work queued on the context's stream. */
+int dme_synchronize(dme_ctx *ctx);
+
+/* Counters of the last completed pipeline call on this context. */
+typedef struct dme_stats {
+    double  sad_terms;        /* algorithmic SAD terms of the last call       */
+    double  kernel_ms;        /* summed kernel time of the last call (events) */
+    int32_t launches;         /* kernel launches of the last call             */
+    int32_t reserved[7];
+} dme_stats;
+int dme_get_stats(dme_ctx *ctx, dme_stats *out);
+
+/* ---- synthetic input generator (deterministic, own PRNG) --------------- */
+
+/* Texture of Gaussian-filtered uniform noise plus flat-with-noise rectangles,
+ * 8-bit values stored minus 128.  Writes width*height luma samples. */
+int dme_synth_texture(uint64_t seed, int width, int height, int nrects,
+                      int16_t *out_y);
+
+/* Motion field description for dme_synth_warp.  Vectors are in pel units
+ * (fractions allowed, quantised to quarter pel by the warp).                 */
+typedef struct dme_synth_object {
+    float x0, y0, x1, y1;   /* object rectangle in the *current* picture     */
+    float dx, dy;           /* displacement of the object                    */
+} dme_synth_object;
+typedef struct dme_synth_motion {
+    float zoom;             /* affine zoom about the picture centre (1 = none) */
+    float pan_x, pan_y;     /* global translation                            */
+    int32_t nobjects;
+    const dme_synth_object *objects;
+    float noise_sigma;      /* additive Gaussian noise after warping         */
+    uint64_t noise_seed;
+} dme_synth_motion;
+
+/* ref(x,y) = cur(x + d(x,y)) sampled bilinearly on the quarter-pel grid with
+ * clamp-to-edge, plus noise; d is the motion field.  8-bit range, minus 128. */
+int dme_synth_warp(const int16_t *cur, int width, int height,
+                   const dme_synth_motion *motion, int16_t *out_ref);
+
+/* 4:2:0 chroma planes derived from a luma plane (2x2 mean plus a seeded
+ * offset), width/2 x height/2 each.                                          */
+int dme_synth_chroma(const int16_t *y, int width, int height, uint64_t seed,
+                     int16_t *out_u, int16_t *out_v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DME_H */

@@ -1,0 +1,332 @@
+// ref_harness.cpp -- drives the reference encoder's own motion-estimation
+// code (compiled from the sources under /root/reference by oracle/Makefile)
+// so the oracle's restatement can be checked against it.
+//
+// TEST INFRASTRUCTURE ONLY.  The result, oracle/_ref/libdirac_ref.so, is
+// loaded by tests/ and by tests/golden/make_golden.py to write fixtures, and
+// may serve as bench.py's CPU baseline.  No reference source is copied here:
+// this file only calls the reference's public classes.
+//
+// Two drivers are provided:
+//   * depth < 0: the reference's own PixelMatcher::DoSearch, whose pyramid
+//     depth is fixed by pixel_match.cpp:94-95;
+//   * depth > 0: the DoSearch sequence re-driven here with an explicit depth
+//     from the same public pieces DoSearch uses (DownConverter, MEData,
+//     BlockMatcher, AddNewVlist, MvMedian/MvMean, BChk), following
+//     pixel_match.cpp:59-380.  With depth equal to the reference's formula
+//     both drivers must agree, which the tests check.
+// The later stages are always the reference's own classes: MEData::
+// SetLambdaMap, SubpelRefine::DoSubpel and ModeDecider::DoModeDecn, called
+// in the order of PictureCompressor (picture_compress.cpp:76-298).
+
+#include <libdirac_common/common.h>
+#include <libdirac_common/motion.h>
+#include <libdirac_common/upconvert.h>
+#include <libdirac_encoder/enc_picture.h>
+#include <libdirac_encoder/enc_queue.h>
+#include <libdirac_motionest/block_match.h>
+#include <libdirac_motionest/downconvert.h>
+#include <libdirac_motionest/me_mode_decn.h>
+#include <libdirac_motionest/me_subpel.h>
+#include <libdirac_motionest/pixel_match.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "cpu_ref.h"
+
+using namespace dirac;
+
+namespace {
+
+void fill(PicArray &a, const int16_t *src) {
+    for (int y = 0; y < a.LengthY(); ++y)
+        for (int x = 0; x < a.LengthX(); ++x) a[y][x] = src[(size_t)y * a.LengthX() + x];
+}
+
+void dump(const PicArray &a, int16_t *dst) {
+    for (int y = 0; y < a.LengthY(); ++y)
+        for (int x = 0; x < a.LengthX(); ++x) dst[(size_t)y * a.LengthX() + x] = a[y][x];
+}
+
+void dump_mv(const MvArray &a, dme_mv *dst) {
+    if (!dst) return;
+    for (int y = 0; y < a.LengthY(); ++y)
+        for (int x = 0; x < a.LengthX(); ++x) {
+            dst[(size_t)y * a.LengthX() + x].x = a[y][x].x;
+            dst[(size_t)y * a.LengthX() + x].y = a[y][x].y;
+        }
+}
+
+void dump_cost(const TwoDArray<MvCostData> &a, dme_cost *dst) {
+    if (!dst) return;
+    for (int y = 0; y < a.LengthY(); ++y)
+        for (int x = 0; x < a.LengthX(); ++x) {
+            dme_cost &d = dst[(size_t)y * a.LengthX() + x];
+            d.SAD = a[y][x].SAD;
+            d.mvcost = a[y][x].mvcost;
+            d.total = a[y][x].total;
+        }
+}
+
+template <class T, class U>
+void dump_arr(const TwoDArray<T> &a, U *dst) {
+    if (!dst) return;
+    for (int y = 0; y < a.LengthY(); ++y)
+        for (int x = 0; x < a.LengthX(); ++x) dst[(size_t)y * a.LengthX() + x] = (U)a[y][x];
+}
+
+void dump_medata(const MEData &me, int nrefs, dme_medata *out) {
+    if (!out) return;
+    for (int r = 0; r < nrefs; ++r) {
+        dump_mv(me.Vectors(r + 1), out->vectors[r]);
+        dump_cost(me.PredCosts(r + 1), out->pred_costs[r]);
+    }
+    dump_arr(me.Mode(), out->mode);
+    dump_arr(me.IntraCosts(), out->intra_costs);
+    dump_cost(me.BiPredCosts(), out->bipred_costs);
+    dump_arr(me.DC(Y_COMP), out->dc[0]);
+    dump_arr(me.DC(U_COMP), out->dc[1]);
+    dump_arr(me.DC(V_COMP), out->dc[2]);
+    dump_arr(me.SBSplit(), out->sb_split);
+    dump_arr(me.SBCosts(), out->sb_costs);
+    dump_arr(me.LambdaMap(), out->lambda_map);
+}
+
+// PixelMatcher::MatchPic re-driven with the public pieces
+// (pixel_match.cpp:242-380).
+void match_pic(const PicArray &pic, const PicArray &ref, MEData &me, const MvData *guide, int ref_id,
+               int level, const PicturePredParams &pp) {
+    const int xr = std::min(level + 1, 5), yr = xr;
+    MvArray &mv = me.Vectors(ref_id);
+    TwoDArray<MvCostData> &pc = me.PredCosts(ref_id);
+    for (int y = 0; y < mv.LengthY(); ++y)
+        for (int x = 0; x < mv.LengthX(); ++x) {
+            mv[y][x].x = 0;
+            mv[y][x].y = 0;
+            pc[y][x].total = 10000000.0f;
+        }
+    BlockMatcher bm(pic, ref, pp.LumaBParams(2), pp.MVPrecision(), mv, pc);
+    CandidateList cl;
+    AddNewVlist(cl, MVector(0, 0), xr, yr);
+    struct Blk {
+        static void run(int x, int y, const MVector &pred, const MvData *guide, int ref_id, int xr, int yr,
+                        CandidateList &cl, BlockMatcher &bm) {
+            if (guide) {
+                const MvArray &ga = guide->Vectors(ref_id);
+                const int xd = BChk(x >> 1, ga.LengthX()), yd = BChk(y >> 1, ga.LengthY());
+                AddNewVlist(cl, ga[yd][xd] * 2, xr, yr);
+            }
+            AddNewVlist(cl, pred, xr, yr);
+            bm.FindBestMatchPel(x, y, cl, pred, 0);
+            cl.erase(cl.begin() + 1, cl.end());
+        }
+    };
+    Blk::run(0, 0, MVector(0, 0), guide, ref_id, xr, yr, cl, bm);
+    for (int x = 1; x < mv.LengthX(); ++x) Blk::run(x, 0, mv[0][x - 1], guide, ref_id, xr, yr, cl, bm);
+    for (int y = 1; y < mv.LengthY(); ++y) {
+        Blk::run(0, y, mv[y - 1][0], guide, ref_id, xr, yr, cl, bm);
+        for (int x = 1; x < mv.LastX(); ++x)
+            Blk::run(x, y, MvMedian(mv[y][x - 1], mv[y - 1][x], mv[y - 1][x + 1]), guide, ref_id, xr, yr, cl, bm);
+        Blk::run(mv.LastX(), y, MvMean(mv[y - 1][mv.LastX()], mv[y][mv.LastX() - 1]), guide, ref_id, xr, yr, cl,
+                 bm);
+    }
+}
+
+// PixelMatcher::DoSearch with an explicit depth (pixel_match.cpp:59-157).
+void pel_search_depth(EncQueue &q, int pic_num, int depth, oracle_capture *cap) {
+    EncPicture &P = q.GetPicture(pic_num);
+    MEData &me = P.GetMEData();
+    const PicturePredParams &pp = me.GetPicPredParams();
+    const PicArray &pic = P.DataForME(false);
+    const std::vector<int> &refs = P.GetPparams().Refs();
+    const int ref1 = refs[0], ref2 = refs.size() > 1 ? refs[1] : refs[0];
+    const PicArray &r1 = q.GetPicture(ref1).DataForME(false);
+    const PicArray &r2 = q.GetPicture(ref2).DataForME(false);
+
+    std::vector<PicArray *> pd(depth + 1), d1(depth + 1), d2(depth + 1);
+    DownConverter dcon;
+    int sf = 1;
+    for (int i = 1; i <= depth; ++i) {
+        sf *= 2;
+        pd[i] = new PicArray(pic.LengthY() / sf, pic.LengthX() / sf);
+        d1[i] = new PicArray(pic.LengthY() / sf, pic.LengthX() / sf);
+        d2[i] = new PicArray(pic.LengthY() / sf, pic.LengthX() / sf);
+    }
+    for (int i = 1; i <= depth; ++i) {
+        dcon.DoDownConvert(i == 1 ? pic : *pd[i - 1], *pd[i]);
+        dcon.DoDownConvert(i == 1 ? r1 : *d1[i - 1], *d1[i]);
+        if (ref1 != ref2) dcon.DoDownConvert(i == 1 ? r2 : *d2[i - 1], *d2[i]);
+        if (cap && cap->cur_pyramid[i]) dump(*pd[i], cap->cur_pyramid[i]);
+    }
+    // MakeMEDataHierarchy (pixel_match.cpp:185-220)
+    std::vector<MEData *> ms(depth + 1, (MEData *)0);
+    const OLBParams bp = pp.LumaBParams(2);
+    int xn = pp.XNumBlocks(), yn = pp.YNumBlocks();
+    PicturePredParams lp = pp;
+    lp.SetXNumSB(0);
+    lp.SetYNumSB(0);
+    for (int i = 1; i <= depth; ++i) {
+        xn >>= 1;
+        yn >>= 1;
+        if (pd[i]->LengthX() % bp.Xbsep() != 0) xn++;
+        if (pd[i]->LengthY() % bp.Ybsep() != 0) yn++;
+        lp.SetXNumBlocks(xn);
+        lp.SetYNumBlocks(yn);
+        ms[i] = new MEData(lp, 2);
+    }
+    for (int l = depth; l >= 0; --l) {
+        const PicArray &pl = l ? *pd[l] : pic;
+        MEData &ml = l ? *ms[l] : me;
+        const MvData *guide = l < depth ? ms[l + 1] : (const MvData *)0;
+        match_pic(pl, l ? *d1[l] : r1, ml, guide, 1, l, pp);
+        if (ref1 != ref2) match_pic(pl, l ? *d2[l] : r2, ml, guide, 2, l, pp);
+        if (cap)
+            for (int r = 0; r < (ref1 != ref2 ? 2 : 1); ++r) {
+                dump_mv(ml.Vectors(r + 1), cap->level_vectors[r][l]);
+                dump_cost(ml.PredCosts(r + 1), cap->level_costs[r][l]);
+            }
+    }
+    for (int i = 1; i <= depth; ++i) {
+        delete pd[i];
+        delete d1[i];
+        delete d2[i];
+        delete ms[i];
+    }
+}
+
+}  // namespace
+
+extern "C" int refh_downconvert(const int16_t *in, int in_w, int in_h, int16_t *out, int out_w, int out_h) {
+    PicArray a(in_h, in_w), b(out_h, out_w);
+    fill(a, in);
+    DownConverter d;
+    d.DoDownConvert(a, b);
+    dump(b, out);
+    return 0;
+}
+
+extern "C" int refh_upconvert(const int16_t *in, int w, int h, int16_t *out) {
+    PicArray a(h, w), b(2 * h, 2 * w);
+    fill(a, in);
+    UpConverter u(-128, 127, w, h);
+    u.DoUpConverter(a, b);
+    dump(b, out);
+    return 0;
+}
+
+extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
+                       int stop_after, dme_medata *out, oracle_capture *cap) {
+    const int W = cur->width, H = cur->height;
+    EncoderParams encp(VIDEO_FORMAT_CUSTOM, INTER_PICTURE, 2, false);
+    encp.SetFullSearch(false);
+    encp.SetCombinedME(false);
+    encp.SetXRangeME(32);
+    encp.SetYRangeME(32);
+
+    EncQueue q;
+    // references first, as intra reference pictures
+    for (int r = 0; r < nrefs; ++r) {
+        PictureParams pp(format420, W, H, 8, 8);
+        pp.SetPictureNum(refs[r].pic_num);
+        pp.SetPicSort(PictureSort::IntraRefPictureSort());
+        q.PushPicture(pp);
+        EncPicture &e = q.GetPicture(refs[r].pic_num);
+        fill(e.Data(Y_COMP), refs[r].y);
+        if (refs[r].u) fill(e.Data(U_COMP), refs[r].u);
+        if (refs[r].v) fill(e.Data(V_COMP), refs[r].v);
+        e.SetOrigData();
+    }
+    PictureParams cp(format420, W, H, 8, 8);
+    cp.SetPictureNum(cur->pic_num);
+    cp.SetPicSort(PictureSort::InterRefPictureSort());
+    for (int r = 0; r < nrefs; ++r) cp.Refs().push_back(refs[r].pic_num);
+    q.PushPicture(cp);
+    EncPicture &E = q.GetPicture(cur->pic_num);
+    fill(E.Data(Y_COMP), cur->y);
+    if (cur->u) fill(E.Data(U_COMP), cur->u);
+    if (cur->v) fill(E.Data(V_COMP), cur->v);
+    E.SetOrigData();
+
+    // Block parameters and MEData as SequenceCompressor sets them
+    // (seq_compress.cpp:107-133, 266)
+    PicturePredParams pp;
+    pp.SetBlockSizes(OLBParams(p->xblen, p->yblen, p->xbsep, p->ybsep), format420);
+    pp.SetMVPrecision(MVPrecisionType(p->mv_precision));
+    pp.SetXNumSB((W + pp.LumaBParams(0).Xbsep() - 1) / pp.LumaBParams(0).Xbsep());
+    pp.SetYNumSB((H + pp.LumaBParams(0).Ybsep() - 1) / pp.LumaBParams(0).Ybsep());
+    pp.SetXNumBlocks(4 * pp.XNumSB());
+    pp.SetYNumBlocks(4 * pp.YNumSB());
+    E.InitMEData(pp, nrefs);
+    MEData &me = E.GetMEData();
+
+    // PixelME (picture_compress.cpp:76-80)
+    if (p->depth < 0) {
+        PixelMatcher pm(encp);
+        pm.DoSearch(q, cur->pic_num);
+    } else {
+        pel_search_depth(q, cur->pic_num, p->depth, cap);
+    }
+    if (stop_after > ORACLE_STOP_PEL) {
+        // SubPixelME (picture_compress.cpp:208-261)
+        me.SetLambdaMap(nrefs, p->lambda);
+        if (stop_after > ORACLE_STOP_LAMBDA) {
+            const MVPrecisionType orig = me.GetPicPredParams().MVPrecision();
+            if (orig != MV_PRECISION_PIXEL) {
+                SubpelRefine sr(encp);
+                sr.DoSubpel(q, cur->pic_num);
+            } else {
+                for (int r = 1; r <= nrefs; ++r) {
+                    MvArray &a = me.Vectors(r);
+                    for (int j = 0; j < a.LengthY(); ++j)
+                        for (int i = 0; i < a.LengthX(); ++i) a[j][i] = a[j][i] << 1;
+                }
+                me.GetPicPredParams().SetMVPrecision(MV_PRECISION_HALF_PIXEL);
+            }
+            if (stop_after > ORACLE_STOP_SUBPEL) {
+                // ModeDecisionME (picture_compress.cpp:263-298)
+                {
+                    ModeDecider md(encp);
+                    md.DoModeDecn(q, cur->pic_num);
+                }
+                if (orig == MV_PRECISION_PIXEL) {
+                    for (int r = 1; r <= nrefs; ++r) {
+                        MvArray &a = me.Vectors(r);
+                        for (int j = 0; j < a.LengthY(); ++j)
+                            for (int i = 0; i < a.LengthX(); ++i) a[j][i] = a[j][i] >> 1;
+                    }
+                    me.GetPicPredParams().SetMVPrecision(MV_PRECISION_PIXEL);
+                }
+            }
+        }
+    }
+    dump_medata(me, nrefs, out);
+    if (cap)
+        for (int r = 0; r < nrefs; ++r)
+            if (cap->ref_up[r]) dump(q.GetPicture(refs[r].pic_num).UpDataForME(false), cap->ref_up[r]);
+    return 0;
+}
+
+extern "C" int refh_lambda_map(int xnb, int ynb, int nrefs, const dme_mv *v1, const dme_mv *v2, float lambda,
+                               float *out) {
+    PicturePredParams pp;
+    pp.SetBlockSizes(OLBParams(12, 12, 8, 8), format420);
+    pp.SetXNumBlocks(xnb);
+    pp.SetYNumBlocks(ynb);
+    pp.SetXNumSB(xnb / 4);
+    pp.SetYNumSB(ynb / 4);
+    MEData me(pp, nrefs);
+    for (int r = 1; r <= nrefs; ++r) {
+        const dme_mv *v = r == 1 ? v1 : v2;
+        MvArray &a = me.Vectors(r);
+        for (int j = 0; j < ynb; ++j)
+            for (int i = 0; i < xnb; ++i) {
+                a[j][i].x = v[(size_t)j * xnb + i].x;
+                a[j][i].y = v[(size_t)j * xnb + i].y;
+            }
+    }
+    me.SetLambdaMap(nrefs, lambda);
+    dump_arr(me.LambdaMap(), out);
+    return 0;
+}
