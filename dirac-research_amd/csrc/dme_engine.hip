@@ -1,0 +1,739 @@
+// dme_engine.hip -- host orchestration and the C ABI (include/dme.h).
+//
+// A context owns one HIP stream on one device, the registered pictures (with
+// their lazily built pyramid levels and up-converted planes, the device
+// counterpart of EncPicture's mutable caches, enc_picture.h:146-148) and the
+// per-picture MEData being estimated.  Every stage runs as batched launches
+// over all jobs of a call; the single-picture entry points are batches of one.
+//
+// Stage order follows PictureCompressor (picture_compress.cpp:76-298):
+//   pel search (coarse to fine, bulk + wavefront per level)
+//   -> lambda map (host, long double)
+//   -> sub-pel refinement (wavefront) or the pixel-precision doubling
+//   -> mode decision (bulk level 2 + superblock wavefront) -> SetDC.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dme_device.h"
+#include "dme_internal.h"
+
+namespace dme {
+
+hipError_t launch_downconvert(const int16_t *in, int in_w, int16_t *out, int ow, int oh, hipStream_t st);
+hipError_t launch_upconvert(const int16_t *in, int W, int H, int16_t *up, hipStream_t st);
+hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
+hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int threads, hipStream_t st);
+hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int threads, hipStream_t st);
+hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
+                                hipStream_t st);
+hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st);
+hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_error(DME_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define DME_TRY(expr)              \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != DME_OK) return rc_; \
+    } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    DevBuf() {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t bytes) {
+        if (bytes <= n) return DME_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return set_error(DME_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        n = bytes;
+        return DME_OK;
+    }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+struct Picture {
+    int num = 0, W = 0, H = 0;
+    const int16_t *y = nullptr, *u = nullptr, *v = nullptr;  // device planes (owned or bound)
+    DevBuf own[3];
+    DevBuf pyr[DME_MAX_DEPTH + 1];  // levels 1..
+    int pyr_levels = 0;             // levels built so far
+    DevBuf up;
+    bool up_ready = false;
+};
+
+struct JobState {
+    int pic_num = -1;
+    int nrefs = 0, refs[2] = {0, 0};
+    dme_params p;
+    Geometry g;
+    int stage = 0;  // 1 pel, 2 lambda, 3 subpel, 4 mode
+    DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3];
+    DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1], bsad[2], bmv[2];
+    JobDesc desc;
+    std::vector<float> h_lam, h_lam1, h_lam0;
+    std::vector<dme_mv> h_vec[2];
+};
+
+}  // namespace dme
+
+struct dme_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dme_params params;
+    bool params_set = false;
+    std::map<int, std::unique_ptr<dme::Picture>> pics;
+    std::map<int, std::unique_ptr<dme::JobState>> jobs;
+    dme::DevBuf d_jobs, d_tasks;
+    std::vector<dme::Task> h_tasks;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    dme_stats stats;
+    int wave_threads = 512;
+};
+
+namespace dme {
+
+static int set_device(dme_ctx *ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return DME_OK;
+}
+
+// Builds pyramid levels 1..levels of a picture (PixelMatcher::
+// MakePicHierarchy, pixel_match.cpp:159-183: each level from the previous,
+// sizes from full resolution).
+static int ensure_pyramid(dme_ctx *ctx, Picture &P, int levels, int &launches) {
+    for (int i = P.pyr_levels + 1; i <= levels; ++i) {
+        const int ow = P.W >> i, oh = P.H >> i;
+        const int iw = P.W >> (i - 1);
+        DME_TRY(P.pyr[i].ensure(sizeof(int16_t) * (size_t)ow * oh));
+        const int16_t *in = i == 1 ? P.y : P.pyr[i - 1].as<int16_t>();
+        HIP_TRY(launch_downconvert(in, iw, P.pyr[i].as<int16_t>(), ow, oh, ctx->stream));
+        ++launches;
+        P.pyr_levels = i;
+    }
+    return DME_OK;
+}
+
+// EncPicture::UpOrigData (enc_picture.cpp:131-156), cached per picture.
+static int ensure_up(dme_ctx *ctx, Picture &P, int &launches) {
+    if (P.up_ready) return DME_OK;
+    DME_TRY(P.up.ensure(sizeof(int16_t) * 4 * (size_t)P.W * P.H));
+    HIP_TRY(launch_upconvert(P.y, P.W, P.H, P.up.as<int16_t>(), ctx->stream));
+    ++launches;
+    P.up_ready = true;
+    return DME_OK;
+}
+
+static Picture *find_pic(dme_ctx *ctx, int num) {
+    auto it = ctx->pics.find(num);
+    return it == ctx->pics.end() ? nullptr : it->second.get();
+}
+
+// Allocates a job's MEData and scratch, fills its descriptor.
+static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs) {
+    // buffers below may be reallocated: nothing queued may still use them
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    Picture *P = find_pic(ctx, pic_num);
+    if (!P) return set_error(DME_ERR_INVALID, "picture " + std::to_string(pic_num) + " is not registered");
+    if (nrefs < 1 || nrefs > 2 || !refs) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
+    if (nrefs == 2 && refs[0] == refs[1]) return set_error(DME_ERR_INVALID, "the two references must differ");
+    Picture *R[2] = {nullptr, nullptr};
+    for (int r = 0; r < nrefs; ++r) {
+        R[r] = find_pic(ctx, refs[r]);
+        if (!R[r]) return set_error(DME_ERR_INVALID, "reference " + std::to_string(refs[r]) + " is not registered");
+        if (R[r]->W != P->W || R[r]->H != P->H) return set_error(DME_ERR_INVALID, "reference size differs from picture");
+        if (refs[r] == pic_num) return set_error(DME_ERR_INVALID, "a picture cannot reference itself");
+    }
+    DME_TRY(make_geometry(p, P->W, P->H, J.g));
+    const Geometry &g = J.g;
+    J.pic_num = pic_num;
+    J.nrefs = nrefs;
+    J.refs[0] = refs[0];
+    J.refs[1] = nrefs > 1 ? refs[1] : refs[0];
+    J.p = p;
+    J.stage = 0;
+    const size_t B = (size_t)g.xnb * g.ynb, S = (size_t)g.xsb * g.ysb;
+    for (int r = 0; r < 2; ++r) {
+        DME_TRY(J.vec[r].ensure(sizeof(dme_mv) * B));
+        DME_TRY(J.pc[r].ensure(sizeof(dme_cost) * B));
+        size_t maxl = 1;
+        for (int l = 1; l <= g.depth; ++l) {
+            const size_t n = (size_t)g.lxb[l] * g.lyb[l];
+            DME_TRY(J.lvec[r][l].ensure(sizeof(dme_mv) * n));
+            DME_TRY(J.lpc[r][l].ensure(sizeof(dme_cost) * n));
+            maxl = std::max(maxl, n);
+        }
+        maxl = std::max(maxl, B);
+        DME_TRY(J.bsad[r].ensure(sizeof(int) * maxl));
+        DME_TRY(J.bmv[r].ensure(sizeof(dme_mv) * maxl));
+    }
+    DME_TRY(J.mode.ensure(sizeof(int32_t) * B));
+    DME_TRY(J.intra.ensure(sizeof(float) * B));
+    DME_TRY(J.bi.ensure(sizeof(dme_cost) * B));
+    for (int c = 0; c < 3; ++c) {
+        DME_TRY(J.dc[c].ensure(sizeof(int16_t) * B));
+        DME_TRY(J.dct[c].ensure(sizeof(int16_t) * B));
+    }
+    DME_TRY(J.split.ensure(sizeof(int32_t) * S));
+    DME_TRY(J.sbc.ensure(sizeof(float) * S));
+    DME_TRY(J.lam.ensure(sizeof(float) * B));
+    DME_TRY(J.lam1.ensure(sizeof(float) * B / 4));
+    DME_TRY(J.lam0.ensure(sizeof(float) * B / 16));
+    DME_TRY(J.ucost.ensure(sizeof(float) * B));
+
+    JobDesc &d = J.desc;
+    std::memset(&d, 0, sizeof(d));
+    d.W = g.W; d.H = g.H; d.CW = g.CW; d.CH = g.CH;
+    d.xnb = g.xnb; d.ynb = g.ynb; d.xsb = g.xsb; d.ysb = g.ysb; d.depth = g.depth; d.nrefs = nrefs;
+    d.prec = p.mv_precision;
+    d.step = p.mv_precision == 0 ? 1 : p.mv_precision;
+    d.fp_sem = p.fp_semantics;
+    for (int l = 0; l <= g.depth; ++l) {
+        d.lw[l] = g.lw[l]; d.lh[l] = g.lh[l]; d.lxb[l] = g.lxb[l]; d.lyb[l] = g.lyb[l];
+    }
+    for (int l = 0; l < 3; ++l)
+        for (int a = 0; a < 2; ++a) {
+            d.blen[l][a] = g.blen[l][a]; d.bsep[l][a] = g.bsep[l][a]; d.boff[l][a] = g.boff[l][a];
+        }
+    for (int a = 0; a < 2; ++a) { d.cblen[a] = g.cblen[a]; d.cbsep[a] = g.cbsep[a]; d.cboff[a] = g.cboff[a]; }
+    for (int l = 0; l < 3; ++l) d.lf[l] = g.level_factor[l];
+    d.pic[0] = P->y;
+    d.cu = P->u;
+    d.cv = P->v;
+    for (int r = 0; r < 2; ++r) {
+        d.vec[r] = J.vec[r].as<dme_mv>();
+        d.pc[r] = J.pc[r].as<dme_cost>();
+        d.lvec[r][0] = d.vec[r];
+        d.lpc[r][0] = d.pc[r];
+        for (int l = 1; l <= g.depth; ++l) {
+            d.lvec[r][l] = J.lvec[r][l].as<dme_mv>();
+            d.lpc[r][l] = J.lpc[r][l].as<dme_cost>();
+        }
+        d.bsad[r] = J.bsad[r].as<int>();
+        d.bmv[r] = J.bmv[r].as<dme_mv>();
+    }
+    for (int r = 0; r < nrefs; ++r) d.ref[r][0] = R[r]->y;
+    d.mode = J.mode.as<int32_t>();
+    d.intra = J.intra.as<float>();
+    d.bi = J.bi.as<dme_cost>();
+    for (int c = 0; c < 3; ++c) {
+        d.dc[c] = J.dc[c].as<int16_t>();
+        d.dct[c] = J.dct[c].as<int16_t>();
+    }
+    d.split = J.split.as<int32_t>();
+    d.sbc = J.sbc.as<float>();
+    d.lam = J.lam.as<float>();
+    d.lam1 = J.lam1.as<float>();
+    d.lam0 = J.lam0.as<float>();
+    d.ucost = J.ucost.as<float>();
+    return DME_OK;
+}
+
+// Fills the descriptor's plane pointers once the caches exist.
+static int bind_planes(dme_ctx *ctx, JobState &J, bool need_up, int &launches) {
+    Picture *P = find_pic(ctx, J.pic_num);
+    if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
+    DME_TRY(ensure_pyramid(ctx, *P, J.g.depth, launches));
+    J.desc.pic[0] = P->y;
+    for (int l = 1; l <= J.g.depth; ++l) J.desc.pic[l] = P->pyr[l].as<int16_t>();
+    for (int r = 0; r < J.nrefs; ++r) {
+        Picture *R = find_pic(ctx, J.refs[r]);
+        if (!R) return set_error(DME_ERR_STATE, "reference was dropped");
+        DME_TRY(ensure_pyramid(ctx, *R, J.g.depth, launches));
+        J.desc.ref[r][0] = R->y;
+        for (int l = 1; l <= J.g.depth; ++l) J.desc.ref[r][l] = R->pyr[l].as<int16_t>();
+        if (need_up) {
+            DME_TRY(ensure_up(ctx, *R, launches));
+            J.desc.up[r] = R->up.as<int16_t>();
+        }
+    }
+    return DME_OK;
+}
+
+// Uploads the descriptors of a batch.
+static int upload_jobs(dme_ctx *ctx, const std::vector<JobState *> &js) {
+    std::vector<JobDesc> h(js.size());
+    for (size_t i = 0; i < js.size(); ++i) h[i] = js[i]->desc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    DME_TRY(ctx->d_jobs.ensure(sizeof(JobDesc) * h.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_jobs.p, h.data(), sizeof(JobDesc) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    // the host vector dies at return: wait for the copy to leave it
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DME_OK;
+}
+
+// Task table builder: groups of tasks uploaded in one copy per stage.
+struct TaskTable {
+    std::vector<Task> all;
+    struct Range { size_t first; int n; int total; };
+    Range add(const std::vector<Task> &t) {
+        Range r;
+        r.first = all.size();
+        r.n = (int)t.size();
+        r.total = t.empty() ? 0 : t.back().offset + t.back().count;
+        all.insert(all.end(), t.begin(), t.end());
+        return r;
+    }
+};
+
+static int upload_tasks(dme_ctx *ctx, const TaskTable &tt) {
+    if (tt.all.empty()) return DME_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    DME_TRY(ctx->d_tasks.ensure(sizeof(Task) * tt.all.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_tasks.p, tt.all.data(), sizeof(Task) * tt.all.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DME_OK;
+}
+
+static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->d_tasks.as<Task>() + r.first; }
+
+// ---- stages ---------------------------------------------------------------
+
+static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    int maxd = 0;
+    for (auto *j : js) {
+        DME_TRY(bind_planes(ctx, *j, false, launches));
+        maxd = std::max(maxd, j->g.depth);
+    }
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<TaskTable::Range> bulk(maxd + 1);
+    for (int l = maxd; l >= 0; --l) {
+        std::vector<Task> t;
+        int off = 0;
+        for (size_t i = 0; i < js.size(); ++i)
+            for (int r = 0; r < js[i]->nrefs; ++r) {
+                if (l > js[i]->g.depth) continue;
+                const int cnt = js[i]->g.lxb[l] * js[i]->g.lyb[l];
+                t.push_back(Task{(int)i, r, cnt, off});
+                off += cnt;
+            }
+        bulk[l] = tt.add(t);
+    }
+    std::vector<Task> wt;
+    for (size_t i = 0; i < js.size(); ++i)
+        for (int r = 0; r < js[i]->nrefs; ++r) wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
+    const TaskTable::Range wave = tt.add(wt);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    for (int l = maxd; l >= 0; --l) {
+        HIP_TRY(launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, ctx->stream));
+        HIP_TRY(launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, ctx->wave_threads, ctx->stream));
+        launches += 2;
+    }
+    for (auto *j : js) j->stage = 1;
+    return DME_OK;
+}
+
+// Lambda maps: host computation from the level-0 pel vectors, then upload of
+// the level-2 map and the level-1/0 maps of ModeDecider.
+static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *overrides) {
+    for (auto *j : js) {
+        const size_t B = (size_t)j->g.xnb * j->g.ynb;
+        for (int r = 0; r < j->nrefs; ++r) {
+            j->h_vec[r].resize(B);
+            HIP_TRY(hipMemcpyAsync(j->h_vec[r].data(), j->vec[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < js.size(); ++i) {
+        JobState &j = *js[i];
+        const Geometry &g = j.g;
+        const size_t B = (size_t)g.xnb * g.ynb;
+        j.h_lam.resize(B);
+        if (overrides && overrides[i])
+            std::memcpy(j.h_lam.data(), overrides[i], sizeof(float) * B);
+        else
+            lambda_map_l2(g.xnb, g.ynb, j.nrefs, j.h_vec[0].data(), j.nrefs > 1 ? j.h_vec[1].data() : nullptr,
+                          j.p.lambda, j.h_lam.data());
+        j.h_lam1.resize(B / 4);
+        j.h_lam0.resize(B / 16);
+        lambda_map_level(j.h_lam.data(), g.xnb, 1, (float)(1.0 / g.level_factor[1]), g.xnb / 2, g.ynb / 2,
+                         j.h_lam1.data());
+        lambda_map_level(j.h_lam.data(), g.xnb, 0, (float)(1.0 / g.level_factor[0]), g.xnb / 4, g.ynb / 4,
+                         j.h_lam0.data());
+        HIP_TRY(hipMemcpyAsync(j.lam.p, j.h_lam.data(), sizeof(float) * B, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(j.lam1.p, j.h_lam1.data(), sizeof(float) * B / 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(j.lam0.p, j.h_lam0.data(), sizeof(float) * B / 16, hipMemcpyHostToDevice, ctx->stream));
+        j.stage = 2;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DME_OK;
+}
+
+static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    for (auto *j : js) DME_TRY(bind_planes(ctx, *j, true, launches));
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<Task> wt, sh;
+    int off = 0;
+    for (size_t i = 0; i < js.size(); ++i)
+        for (int r = 0; r < js[i]->nrefs; ++r) {
+            wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
+            if (js[i]->p.mv_precision == 0) {
+                const int cnt = js[i]->g.xnb * js[i]->g.ynb;
+                sh.push_back(Task{(int)i, r, cnt, off});
+                off += cnt;
+            }
+        }
+    const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    HIP_TRY(launch_subpel_wave(dj, dtask(ctx, wave), wave.n, ctx->wave_threads, ctx->stream));
+    ++launches;
+    if (shift.n) {
+        HIP_TRY(launch_shift_vectors(dj, dtask(ctx, shift), shift.n, shift.total, 1, ctx->stream));
+        ++launches;
+    }
+    for (auto *j : js) j->stage = 3;
+    return DME_OK;
+}
+
+static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    for (auto *j : js) {
+        DME_TRY(bind_planes(ctx, *j, true, launches));
+        const size_t B = (size_t)j->g.xnb * j->g.ynb;
+        // zero-initialised MEData arrays (motion.cpp:55-126)
+        HIP_TRY(hipMemsetAsync(j->intra.p, 0, sizeof(float) * B, ctx->stream));
+        HIP_TRY(hipMemsetAsync(j->bi.p, 0, sizeof(dme_cost) * B, ctx->stream));
+        for (int c = 0; c < 3; ++c) HIP_TRY(hipMemsetAsync(j->dc[c].p, 0, sizeof(int16_t) * B, ctx->stream));
+        if (j->nrefs < 2) {
+            HIP_TRY(hipMemsetAsync(j->vec[1].p, 0, sizeof(dme_mv) * B, ctx->stream));
+            HIP_TRY(hipMemsetAsync(j->pc[1].p, 0, sizeof(dme_cost) * B, ctx->stream));
+        }
+    }
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<Task> l2, dc, sh;
+    int off = 0, offd = 0, offs = 0;
+    for (size_t i = 0; i < js.size(); ++i) {
+        const int cnt = js[i]->g.xnb * js[i]->g.ynb;
+        l2.push_back(Task{(int)i, 0, cnt, off});
+        off += cnt;
+        for (int c = 0; c < 3; ++c) {
+            dc.push_back(Task{(int)i, c, cnt, offd});
+            offd += cnt;
+        }
+        if (js[i]->p.mv_precision == 0)
+            for (int r = 0; r < js[i]->nrefs; ++r) {
+                sh.push_back(Task{(int)i, r, cnt, offs});
+                offs += cnt;
+            }
+    }
+    const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rsh = tt.add(sh);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    HIP_TRY(launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, ctx->stream));
+    HIP_TRY(launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, ctx->stream));
+    HIP_TRY(launch_block_dc(dj, dtask(ctx, rdc), rdc.n, rdc.total, ctx->stream));
+    HIP_TRY(launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, ctx->stream));
+    launches += 4;
+    if (rsh.n) {
+        HIP_TRY(launch_shift_vectors(dj, dtask(ctx, rsh), rsh.n, rsh.total, 0, ctx->stream));
+        ++launches;
+    }
+    for (auto *j : js) j->stage = 4;
+    return DME_OK;
+}
+
+static int export_job(dme_ctx *ctx, JobState &J, dme_medata *out) {
+    if (!out) return DME_OK;
+    const size_t B = (size_t)J.g.xnb * J.g.ynb, S = (size_t)J.g.xsb * J.g.ysb;
+    hipStream_t st = ctx->stream;
+    for (int r = 0; r < J.nrefs; ++r) {
+        if (out->vectors[r]) HIP_TRY(hipMemcpyAsync(out->vectors[r], J.vec[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost, st));
+        if (out->pred_costs[r])
+            HIP_TRY(hipMemcpyAsync(out->pred_costs[r], J.pc[r].p, sizeof(dme_cost) * B, hipMemcpyDeviceToHost, st));
+    }
+    if (J.stage >= 2 && out->lambda_map)
+        HIP_TRY(hipMemcpyAsync(out->lambda_map, J.lam.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+    if (J.stage >= 4) {
+        if (out->mode) HIP_TRY(hipMemcpyAsync(out->mode, J.mode.p, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+        if (out->intra_costs) HIP_TRY(hipMemcpyAsync(out->intra_costs, J.intra.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+        if (out->bipred_costs) HIP_TRY(hipMemcpyAsync(out->bipred_costs, J.bi.p, sizeof(dme_cost) * B, hipMemcpyDeviceToHost, st));
+        for (int c = 0; c < 3; ++c)
+            if (out->dc[c]) HIP_TRY(hipMemcpyAsync(out->dc[c], J.dc[c].p, sizeof(int16_t) * B, hipMemcpyDeviceToHost, st));
+        if (out->sb_split) HIP_TRY(hipMemcpyAsync(out->sb_split, J.split.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+        if (out->sb_costs) HIP_TRY(hipMemcpyAsync(out->sb_costs, J.sbc.p, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return DME_OK;
+}
+
+static JobState *job_for(dme_ctx *ctx, int pic_num) {
+    auto &slot = ctx->jobs[pic_num];
+    if (!slot) slot.reset(new JobState());
+    return slot.get();
+}
+
+static int begin_timing(dme_ctx *ctx) {
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    return DME_OK;
+}
+
+static int end_timing(dme_ctx *ctx, int launches) {
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.kernel_ms = ms;
+    ctx->stats.launches = launches;
+    return DME_OK;
+}
+
+// The whole live sequence for a batch of jobs.
+static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
+    int launches = 0;
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(stage_pel(ctx, js, launches));
+    DME_TRY(stage_lambda(ctx, js, lam_over));
+    DME_TRY(stage_subpel(ctx, js, launches));
+    DME_TRY(stage_mode(ctx, js, launches));
+    DME_TRY(end_timing(ctx, launches));
+    return DME_OK;
+}
+
+}  // namespace dme
+
+using namespace dme;
+
+#define CTX_CHECK(ctx)                                                             \
+    do {                                                                           \
+        if (!(ctx)) return set_error(DME_ERR_INVALID, "NULL context");            \
+        DME_TRY(set_device(ctx));                                                  \
+    } while (0)
+
+extern "C" int dme_create(int device, dme_ctx **out) {
+    if (!out) return set_error(DME_ERR_INVALID, "NULL out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return set_error(DME_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return set_error(DME_ERR_INVALID, "device index out of range");
+    std::unique_ptr<dme_ctx> c(new dme_ctx());
+    c->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    std::memset(&c->params, 0, sizeof(c->params));
+    *out = c.release();
+    return DME_OK;
+}
+
+extern "C" int dme_destroy(dme_ctx *ctx) {
+    if (!ctx) return DME_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->jobs.clear();
+    ctx->pics.clear();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DME_OK;
+}
+
+extern "C" int dme_set_params(dme_ctx *ctx, const dme_params *params) {
+    if (!ctx || !params) return set_error(DME_ERR_INVALID, "NULL argument");
+    // validate everything but the size-dependent depth on a large picture
+    Geometry g;
+    dme_params q = *params;
+    q.depth = -1;
+    DME_TRY(make_geometry(q, 3840, 2160, g));
+    if (params->depth > DME_MAX_DEPTH) return set_error(DME_ERR_INVALID, "pyramid depth must be in [1, 8]");
+    ctx->params = *params;
+    ctx->params_set = true;
+    return DME_OK;
+}
+
+static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *y, const int16_t *u,
+                            const int16_t *v, bool device) {
+    if (!y) return set_error(DME_ERR_INVALID, "luma plane is NULL");
+    if (width < 24 || height < 24 || (width & 1) || (height & 1) || width > 16384 || height > 16384)
+        return set_error(DME_ERR_INVALID, "picture size must be even and within [24, 16384]");
+    if ((u == nullptr) != (v == nullptr)) return set_error(DME_ERR_INVALID, "give both chroma planes or neither");
+    std::unique_ptr<Picture> P(new Picture());
+    P->num = pic_num;
+    P->W = width;
+    P->H = height;
+    const size_t n = (size_t)width * height, cn = (size_t)(width / 2) * (height / 2);
+    if (device) {
+        P->y = y;
+        P->u = u;
+        P->v = v;
+    } else {
+        DME_TRY(P->own[0].ensure(sizeof(int16_t) * n));
+        HIP_TRY(hipMemcpy(P->own[0].p, y, sizeof(int16_t) * n, hipMemcpyHostToDevice));
+        P->y = P->own[0].as<int16_t>();
+        if (u) {
+            DME_TRY(P->own[1].ensure(sizeof(int16_t) * cn));
+            DME_TRY(P->own[2].ensure(sizeof(int16_t) * cn));
+            HIP_TRY(hipMemcpy(P->own[1].p, u, sizeof(int16_t) * cn, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(P->own[2].p, v, sizeof(int16_t) * cn, hipMemcpyHostToDevice));
+            P->u = P->own[1].as<int16_t>();
+            P->v = P->own[2].as<int16_t>();
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->pics[pic_num] = std::move(P);
+    return DME_OK;
+}
+
+extern "C" int dme_upload_picture(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *y,
+                                  const int16_t *u, const int16_t *v) {
+    CTX_CHECK(ctx);
+    return register_picture(ctx, pic_num, width, height, y, u, v, false);
+}
+
+extern "C" int dme_bind_picture_device(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *d_y,
+                                       const int16_t *d_u, const int16_t *d_v) {
+    CTX_CHECK(ctx);
+    return register_picture(ctx, pic_num, width, height, d_y, d_u, d_v, true);
+}
+
+extern "C" int dme_drop_picture(dme_ctx *ctx, int pic_num) {
+    CTX_CHECK(ctx);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->pics.erase(pic_num);
+    ctx->jobs.erase(pic_num);
+    return DME_OK;
+}
+
+extern "C" int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+    CTX_CHECK(ctx);
+    if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
+    JobState *J = job_for(ctx, pic_num);
+    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    std::vector<JobState *> js{J};
+    int launches = 0;
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(stage_pel(ctx, js, launches));
+    DME_TRY(end_timing(ctx, launches));
+    return export_job(ctx, *J, out);
+}
+
+extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda_map, dme_medata *out) {
+    CTX_CHECK(ctx);
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 1) return set_error(DME_ERR_STATE, "run dme_pixel_search first");
+    std::vector<JobState *> js{it->second.get()};
+    const float *ov[1] = {lambda_map};
+    DME_TRY(stage_lambda(ctx, js, ov));
+    return export_job(ctx, *js[0], out);
+}
+
+extern "C" int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out) {
+    CTX_CHECK(ctx);
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 2) return set_error(DME_ERR_STATE, "set the lambda map first");
+    std::vector<JobState *> js{it->second.get()};
+    int launches = 0;
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(stage_subpel(ctx, js, launches));
+    DME_TRY(end_timing(ctx, launches));
+    return export_job(ctx, *js[0], out);
+}
+
+extern "C" int dme_mode_decide(dme_ctx *ctx, int pic_num, dme_medata *out) {
+    CTX_CHECK(ctx);
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 3) return set_error(DME_ERR_STATE, "run dme_subpel first");
+    std::vector<JobState *> js{it->second.get()};
+    int launches = 0;
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(stage_mode(ctx, js, launches));
+    DME_TRY(end_timing(ctx, launches));
+    return export_job(ctx, *js[0], out);
+}
+
+extern "C" int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+    CTX_CHECK(ctx);
+    if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
+    JobState *J = job_for(ctx, pic_num);
+    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    std::vector<JobState *> js{J};
+    DME_TRY(run_jobs(ctx, js, nullptr));
+    return export_job(ctx, *J, out);
+}
+
+extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) {
+    CTX_CHECK(ctx);
+    if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
+    std::vector<JobState *> js;
+    for (int i = 0; i < njobs; ++i) {
+        for (int k = 0; k < i; ++k)
+            if (jobs[k].pic_num == jobs[i].pic_num) return set_error(DME_ERR_INVALID, "a picture appears twice in a batch");
+        JobState *J = job_for(ctx, jobs[i].pic_num);
+        DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs));
+        js.push_back(J);
+    }
+    DME_TRY(run_jobs(ctx, js, nullptr));
+    if (outs)
+        for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
+    return DME_OK;
+}
+
+extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int16_t *out, size_t out_elems) {
+    CTX_CHECK(ctx);
+    Picture *P = find_pic(ctx, pic_num);
+    if (!P || !out) return set_error(DME_ERR_INVALID, "unknown picture or NULL output");
+    const void *src = nullptr;
+    size_t n = 0;
+    if (kind == 0) {
+        if (level < 1 || level > P->pyr_levels) return set_error(DME_ERR_STATE, "pyramid level not built");
+        src = P->pyr[level].p;
+        n = (size_t)(P->W >> level) * (P->H >> level);
+    } else if (kind == 1) {
+        if (!P->up_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
+        src = P->up.p;
+        n = 4 * (size_t)P->W * P->H;
+    } else {
+        return set_error(DME_ERR_INVALID, "kind must be 0 or 1");
+    }
+    if (out_elems < n) return set_error(DME_ERR_INVALID, "output buffer too small");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(out, src, sizeof(int16_t) * n, hipMemcpyDeviceToHost));
+    return DME_OK;
+}
+
+extern "C" int dme_synchronize(dme_ctx *ctx) {
+    CTX_CHECK(ctx);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) {
+    if (!ctx || !out) return set_error(DME_ERR_INVALID, "NULL argument");
+    *out = ctx->stats;
+    return DME_OK;
+}

@@ -1,0 +1,184 @@
+"""Python host mirror of the MI355X Dirac motion-estimation engine.
+
+A thin layer over the C ABI of libdme.so (include/dme.h) used by the tests,
+smoke() and bench.py.  ``Context`` mirrors the reference's libdirac_motionest
+surface: ``pixel_search`` (PixelMatcher::DoSearch, pixel_match.h:81),
+``set_lambda_map`` (MEData::SetLambdaMap, motion.h:362), ``subpel``
+(SubpelRefine::DoSubpel, me_subpel.h:77), ``mode_decide``
+(ModeDecider::DoModeDecn, me_mode_decn.h:93) and ``motion_estimate`` (the
+live sequence of PictureCompressor, picture_compress.cpp:76-298).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import (COST_DTYPE, DME_FP_C, DME_FP_MMX, DME_INTRA, DME_REF1_ONLY, DME_REF2_ONLY,  # noqa: F401
+                  DME_REF1AND2, MV_DTYPE, DmeError, check, lib, ptr)
+
+
+def make_params(xblen=12, yblen=12, xbsep=8, ybsep=8, precision=2, depth=-1, lam=10.0, fp=DME_FP_MMX):
+    p = abi.dme_params()
+    p.xblen, p.yblen, p.xbsep, p.ybsep = xblen, yblen, xbsep, ybsep
+    p.mv_precision = precision
+    p.depth = depth
+    p.lambda_ = lam
+    p.fp_semantics = fp
+    p.chroma_format = 0
+    return p
+
+
+def geometry(params, width, height):
+    g = abi.dme_geometry()
+    check(lib().dme_geometry_for(C.byref(params), width, height, C.byref(g)))
+    return g
+
+
+class MEData:
+    """Host MEData arrays (motion.h:219-410) in the C-ABI layout."""
+
+    FIELDS = ("vectors", "pred_costs", "mode", "intra_costs", "bipred_costs", "dc", "sb_split", "sb_costs",
+              "lambda_map")
+
+    def __init__(self, geom):
+        Y, X = geom.ynum_blocks, geom.xnum_blocks
+        YS, XS = geom.ynum_sb, geom.xnum_sb
+        self.vectors = [np.zeros((Y, X), MV_DTYPE) for _ in range(2)]
+        self.pred_costs = [np.zeros((Y, X), COST_DTYPE) for _ in range(2)]
+        self.mode = np.zeros((Y, X), np.int32)
+        self.intra_costs = np.zeros((Y, X), np.float32)
+        self.bipred_costs = np.zeros((Y, X), COST_DTYPE)
+        self.dc = [np.zeros((Y, X), np.int16) for _ in range(3)]
+        self.sb_split = np.zeros((YS, XS), np.int32)
+        self.sb_costs = np.zeros((YS, XS), np.float32)
+        self.lambda_map = np.zeros((Y, X), np.float32)
+
+    def struct(self):
+        m = abi.dme_medata()
+        for r in range(2):
+            m.vectors[r] = self.vectors[r].ctypes.data
+            m.pred_costs[r] = self.pred_costs[r].ctypes.data
+        m.mode = self.mode.ctypes.data
+        m.intra_costs = self.intra_costs.ctypes.data
+        m.bipred_costs = self.bipred_costs.ctypes.data
+        for c in range(3):
+            m.dc[c] = self.dc[c].ctypes.data
+        m.sb_split = self.sb_split.ctypes.data
+        m.sb_costs = self.sb_costs.ctypes.data
+        m.lambda_map = self.lambda_map.ctypes.data
+        return m
+
+    def arrays(self, nrefs=2):
+        """Flat dict of named arrays (for comparisons)."""
+        d = {}
+        for r in range(nrefs):
+            d["vectors%d" % (r + 1)] = self.vectors[r]
+            d["pred_costs%d" % (r + 1)] = self.pred_costs[r]
+        d.update(mode=self.mode, intra_costs=self.intra_costs, bipred_costs=self.bipred_costs, dc_y=self.dc[0],
+                 dc_u=self.dc[1], dc_v=self.dc[2], sb_split=self.sb_split, sb_costs=self.sb_costs,
+                 lambda_map=self.lambda_map)
+        return d
+
+
+class Context:
+    """A device context (dme_create / dme_destroy)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().dme_create(device, C.byref(h)))
+        self.h = h
+        self.params = None
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib().dme_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        check(lib().dme_set_params(self.h, C.byref(params)))
+        self.params = params
+
+    def upload(self, pic_num, y, u=None, v=None):
+        y = np.ascontiguousarray(y, np.int16)
+        H, W = y.shape
+        u = None if u is None else np.ascontiguousarray(u, np.int16)
+        v = None if v is None else np.ascontiguousarray(v, np.int16)
+        check(lib().dme_upload_picture(self.h, pic_num, W, H, ptr(y), ptr(u), ptr(v)))
+
+    def bind_device(self, pic_num, width, height, d_y, d_u=None, d_v=None):
+        check(lib().dme_bind_picture_device(self.h, pic_num, width, height, d_y, d_u, d_v))
+
+    def drop(self, pic_num):
+        check(lib().dme_drop_picture(self.h, pic_num))
+
+    def _refs(self, refs):
+        a = (C.c_int * 2)(*(list(refs) + [0] * (2 - len(refs))))
+        return a, len(refs)
+
+    def pixel_search(self, pic_num, refs, out=None):
+        a, n = self._refs(refs)
+        check(lib().dme_pixel_search(self.h, pic_num, a, n, C.byref(out.struct()) if out else None))
+        return out
+
+    def set_lambda_map(self, pic_num, lambda_map=None, out=None):
+        lm = None if lambda_map is None else np.ascontiguousarray(lambda_map, np.float32)
+        check(lib().dme_set_lambda_map(self.h, pic_num, ptr(lm), C.byref(out.struct()) if out else None))
+        return out
+
+    def subpel(self, pic_num, out=None):
+        check(lib().dme_subpel(self.h, pic_num, C.byref(out.struct()) if out else None))
+        return out
+
+    def mode_decide(self, pic_num, out=None):
+        check(lib().dme_mode_decide(self.h, pic_num, C.byref(out.struct()) if out else None))
+        return out
+
+    def motion_estimate(self, pic_num, refs, out=None):
+        a, n = self._refs(refs)
+        check(lib().dme_motion_estimate(self.h, pic_num, a, n, C.byref(out.struct()) if out else None))
+        return out
+
+    def run_batch(self, jobs, outs=None):
+        """jobs: list of (pic_num, refs, params)."""
+        arr = (abi.dme_job * len(jobs))()
+        for i, (pn, refs, prm) in enumerate(jobs):
+            arr[i].pic_num = pn
+            arr[i].nrefs = len(refs)
+            for k, r in enumerate(refs):
+                arr[i].refs[k] = r
+            arr[i].params = prm
+        if outs is not None:
+            os_ = (abi.dme_medata * len(jobs))(*[o.struct() for o in outs])
+            check(lib().dme_run_batch(self.h, arr, len(jobs), os_))
+        else:
+            check(lib().dme_run_batch(self.h, arr, len(jobs), None))
+        return outs
+
+    def get_plane(self, pic_num, kind, level, shape):
+        out = np.zeros(shape, np.int16)
+        check(lib().dme_get_plane(self.h, pic_num, kind, level, ptr(out), out.size))
+        return out
+
+    def synchronize(self):
+        check(lib().dme_synchronize(self.h))
+
+    def stats(self):
+        s = abi.dme_stats()
+        check(lib().dme_get_stats(self.h, C.byref(s)))
+        return s
+
+
+def lambda_map_host(vectors1, vectors2, lam, nrefs):
+    v1 = np.ascontiguousarray(vectors1, MV_DTYPE)
+    Y, X = v1.shape
+    v2 = None if vectors2 is None else np.ascontiguousarray(vectors2, MV_DTYPE)
+    out = np.zeros((Y, X), np.float32)
+    check(lib().dme_lambda_map_host(X, Y, nrefs, ptr(v1), ptr(v2), lam, ptr(out)))
+    return out

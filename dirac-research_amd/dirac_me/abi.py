@@ -1,0 +1,145 @@
+"""ctypes mirror of include/dme.h and the loader of libdme.so.
+
+The library is built in-tree (``dirac-research_amd/libdme.so``) by
+``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libdme.so")
+
+DME_OK = 0
+DME_ERR_INVALID = -1
+DME_ERR_HIP = -2
+DME_ERR_NOMEM = -3
+DME_ERR_STATE = -4
+DME_ERR_NO_DEVICE = -5
+
+DME_INTRA, DME_REF1_ONLY, DME_REF2_ONLY, DME_REF1AND2 = 0, 1, 2, 3
+DME_FP_MMX, DME_FP_C = 0, 1
+DME_MAX_DEPTH = 8
+
+MV_DTYPE = np.dtype([("x", np.int32), ("y", np.int32)])
+COST_DTYPE = np.dtype([("SAD", np.float32), ("mvcost", np.float32), ("total", np.float32)])
+
+
+class dme_mv(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32)]
+
+
+class dme_cost(C.Structure):
+    _fields_ = [("SAD", C.c_float), ("mvcost", C.c_float), ("total", C.c_float)]
+
+
+class dme_params(C.Structure):
+    _fields_ = [
+        ("xblen", C.c_int32), ("yblen", C.c_int32), ("xbsep", C.c_int32), ("ybsep", C.c_int32),
+        ("mv_precision", C.c_int32), ("depth", C.c_int32), ("lambda_", C.c_float),
+        ("fp_semantics", C.c_int32), ("chroma_format", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class dme_geometry(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("chroma_width", C.c_int32), ("chroma_height", C.c_int32),
+        ("xnum_sb", C.c_int32), ("ynum_sb", C.c_int32), ("xnum_blocks", C.c_int32), ("ynum_blocks", C.c_int32),
+        ("depth", C.c_int32),
+        ("level_width", C.c_int32 * (DME_MAX_DEPTH + 1)), ("level_height", C.c_int32 * (DME_MAX_DEPTH + 1)),
+        ("level_xblocks", C.c_int32 * (DME_MAX_DEPTH + 1)), ("level_yblocks", C.c_int32 * (DME_MAX_DEPTH + 1)),
+        ("luma_blen", (C.c_int32 * 2) * 3), ("luma_bsep", (C.c_int32 * 2) * 3),
+        ("chroma_blen", (C.c_int32 * 2) * 3), ("chroma_bsep", (C.c_int32 * 2) * 3),
+        ("level_factor", C.c_float * 3),
+    ]
+
+
+class dme_medata(C.Structure):
+    _fields_ = [
+        ("vectors", C.c_void_p * 2), ("pred_costs", C.c_void_p * 2), ("mode", C.c_void_p),
+        ("intra_costs", C.c_void_p), ("bipred_costs", C.c_void_p), ("dc", C.c_void_p * 3),
+        ("sb_split", C.c_void_p), ("sb_costs", C.c_void_p), ("lambda_map", C.c_void_p),
+    ]
+
+
+class dme_job(C.Structure):
+    _fields_ = [("pic_num", C.c_int32), ("nrefs", C.c_int32), ("refs", C.c_int32 * 2), ("params", dme_params)]
+
+
+class dme_stats(C.Structure):
+    _fields_ = [("sad_terms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+
+class dme_synth_object(C.Structure):
+    _fields_ = [("x0", C.c_float), ("y0", C.c_float), ("x1", C.c_float), ("y1", C.c_float),
+                ("dx", C.c_float), ("dy", C.c_float)]
+
+
+class dme_synth_motion(C.Structure):
+    _fields_ = [("zoom", C.c_float), ("pan_x", C.c_float), ("pan_y", C.c_float), ("nobjects", C.c_int32),
+                ("objects", C.POINTER(dme_synth_object)), ("noise_sigma", C.c_float), ("noise_seed", C.c_uint64)]
+
+
+# Every symbol the header declares, with its argument types.
+_P = C.c_void_p
+_I = C.c_int
+SIGNATURES = {
+    "dme_last_error": ([], C.c_char_p),
+    "dme_version": ([], _I),
+    "dme_geometry_for": ([C.POINTER(dme_params), _I, _I, C.POINTER(dme_geometry)], _I),
+    "dme_lambda_map_host": ([_I, _I, _I, _P, _P, C.c_float, _P], _I),
+    "dme_create": ([_I, C.POINTER(_P)], _I),
+    "dme_destroy": ([_P], _I),
+    "dme_set_params": ([_P, C.POINTER(dme_params)], _I),
+    "dme_upload_picture": ([_P, _I, _I, _I, _P, _P, _P], _I),
+    "dme_bind_picture_device": ([_P, _I, _I, _I, _P, _P, _P], _I),
+    "dme_drop_picture": ([_P, _I], _I),
+    "dme_pixel_search": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
+    "dme_set_lambda_map": ([_P, _I, _P, C.POINTER(dme_medata)], _I),
+    "dme_subpel": ([_P, _I, C.POINTER(dme_medata)], _I),
+    "dme_mode_decide": ([_P, _I, C.POINTER(dme_medata)], _I),
+    "dme_motion_estimate": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
+    "dme_run_batch": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_get_plane": ([_P, _I, _I, _I, _P, C.c_size_t], _I),
+    "dme_synchronize": ([_P], _I),
+    "dme_get_stats": ([_P, C.POINTER(dme_stats)], _I),
+    "dme_synth_texture": ([C.c_uint64, _I, _I, _I, _P], _I),
+    "dme_synth_warp": ([_P, _I, _I, C.POINTER(dme_synth_motion), _P], _I),
+    "dme_synth_chroma": ([_P, _I, _I, C.c_uint64, _P, _P], _I),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libdme.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libdme.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        l = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = l
+    return _lib
+
+
+class DmeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dme error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != DME_OK:
+        raise DmeError(rc, lib().dme_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

@@ -212,6 +212,16 @@ typedef struct dme_job {
 int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs,
                   dme_medata *outs);
 
+/* Diagnostics: copies a cached device plane of picture pic_num to host.
+ * kind 0 = pyramid level `level` (1..depth built so far), (W>>level) x
+ * (H>>level) samples (PixelMatcher::MakePicHierarchy's down_data,
+ * pixel_match.cpp:159-183, which the reference keeps private);
+ * kind 1 = the 2W x 2H up-converted plane (EncPicture::UpOrigData,
+ * enc_picture.cpp:131-156), built once the picture served as a reference.
+ * out_elems must be at least the plane's sample count.                    */
+int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level,
+                  int16_t *out, size_t out_elems);
+
 /* Waits for all work queued on the context's stream. */
 int dme_synchronize(dme_ctx *ctx);
 

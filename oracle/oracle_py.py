@@ -1,0 +1,191 @@
+"""ctypes access to the oracle (liboracle.so) and, where it was built, to the
+reference harness (_ref/libdirac_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, smoke() and bench.py's
+cpu_baseline leg.  Struct layouts come from the product ABI mirror.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "dirac-research_amd"))
+
+from dirac_me import abi  # noqa: E402
+from dirac_me import MEData  # noqa: E402
+
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libdirac_ref.so")
+
+STOP_PEL, STOP_LAMBDA, STOP_SUBPEL, STOP_MODE = 1, 2, 3, 4
+
+
+class oracle_picture(C.Structure):
+    _fields_ = [("pic_num", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p)]
+
+
+class oracle_capture(C.Structure):
+    _fields_ = [("level_vectors", (C.c_void_p * (abi.DME_MAX_DEPTH + 1)) * 2),
+                ("level_costs", (C.c_void_p * (abi.DME_MAX_DEPTH + 1)) * 2),
+                ("cur_pyramid", C.c_void_p * (abi.DME_MAX_DEPTH + 1)),
+                ("ref_up", C.c_void_p * 2)]
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        l = _load(ORACLE_SO)
+        if l is None:
+            raise RuntimeError("liboracle.so is not built; run __graft_entry__.build()")
+        l.oracle_me.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture), C.POINTER(oracle_picture),
+                                C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.dme_medata),
+                                C.POINTER(oracle_capture), C.POINTER(C.c_double)]
+        l.oracle_downconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+        l.oracle_upconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        l.oracle_lambda_map.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+        l.oracle_geometry.argtypes = [C.POINTER(abi.dme_params), C.c_int, C.c_int, C.POINTER(abi.dme_geometry)]
+        l.oracle_last_error.restype = C.c_char_p
+        _oracle = l
+    return _oracle
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        l = _load(REF_SO)
+        if l is None:
+            raise RuntimeError("oracle/_ref/libdirac_ref.so is not built (needs the reference tree)")
+        l.refh_me.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture), C.POINTER(oracle_picture),
+                              C.c_int, C.c_int, C.POINTER(abi.dme_medata), C.POINTER(oracle_capture)]
+        l.refh_downconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+        l.refh_upconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        l.refh_lambda_map.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+        _ref = l
+    return _ref
+
+
+def _chk(rc, lib_=None):
+    if rc != 0:
+        msg = oracle().oracle_last_error().decode() if lib_ is None else ""
+        raise RuntimeError("oracle error %d %s" % (rc, msg))
+
+
+def geometry(params, W, H):
+    g = abi.dme_geometry()
+    _chk(oracle().oracle_geometry(C.byref(params), W, H, C.byref(g)))
+    return g
+
+
+def _pic(num, y, u=None, v=None):
+    H, W = y.shape
+    return oracle_picture(num, W, H, y.ctypes.data, None if u is None else u.ctypes.data,
+                          None if v is None else v.ctypes.data)
+
+
+class Capture:
+    """Per-level pel-search state and pyramid pictures."""
+
+    def __init__(self, geom, nrefs):
+        self.level_vectors = [[None] * (geom.depth + 1) for _ in range(2)]
+        self.level_costs = [[None] * (geom.depth + 1) for _ in range(2)]
+        self.pyramid = [None] * (geom.depth + 1)
+        self.ref_up = [np.zeros((2 * geom.height, 2 * geom.width), np.int16) for _ in range(nrefs)]
+        self.s = oracle_capture()
+        for l in range(geom.depth + 1):
+            shp = (geom.level_yblocks[l], geom.level_xblocks[l])
+            for r in range(nrefs):
+                self.level_vectors[r][l] = np.zeros(shp, abi.MV_DTYPE)
+                self.level_costs[r][l] = np.zeros(shp, abi.COST_DTYPE)
+                self.s.level_vectors[r][l] = self.level_vectors[r][l].ctypes.data
+                self.s.level_costs[r][l] = self.level_costs[r][l].ctypes.data
+            if l:
+                self.pyramid[l] = np.zeros((geom.level_height[l], geom.level_width[l]), np.int16)
+                self.s.cur_pyramid[l] = self.pyramid[l].ctypes.data
+        for r in range(nrefs):
+            self.s.ref_up[r] = self.ref_up[r].ctypes.data
+
+
+def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=()):
+    cur = np.ascontiguousarray(cur, np.int16)
+    refs = [np.ascontiguousarray(r, np.int16) for r in refs]
+    u = None if u is None else np.ascontiguousarray(u, np.int16)
+    v = None if v is None else np.ascontiguousarray(v, np.int16)
+    H, W = cur.shape
+    geom = geometry(params, W, H)
+    out = MEData(geom)
+    nums = [0, 2] if len(refs) == 2 else [0]
+    cp = _pic(1, cur, u, v)
+    rp = (oracle_picture * 2)(*[_pic(nums[i], refs[i]) for i in range(len(refs))])
+    cap = Capture(geom, len(refs)) if with_cap else None
+    ms = out.struct()
+    rc = fn(C.byref(params), C.byref(cp), rp, len(refs), stop, *extra, C.byref(ms),
+            C.byref(cap.s) if cap else None)
+    return rc, out, cap, geom
+
+
+def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, lambda_override=None):
+    terms = C.c_double(0)
+    lo = None if lambda_override is None else np.ascontiguousarray(lambda_override, np.float32)
+    o = oracle()
+
+    def fn(pp, cp, rp, n, stop_, ms, cap):
+        return o.oracle_me(pp, cp, rp, n, stop_, None if lo is None else lo.ctypes.data, ms, cap, C.byref(terms))
+
+    rc, out, cap, geom = _run(fn, params, cur, refs, u, v, stop, capture)
+    _chk(rc)
+    out.sad_terms = terms.value
+    return out, cap, geom
+
+
+def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False):
+    rc, out, cap, geom = _run(ref().refh_me, params, cur, refs, u, v, stop, capture)
+    if rc != 0:
+        raise RuntimeError("reference harness error %d" % rc)
+    return out, cap, geom
+
+
+def downconvert(fn_lib, arr, ow, oh, name):
+    arr = np.ascontiguousarray(arr, np.int16)
+    H, W = arr.shape
+    out = np.zeros((oh, ow), np.int16)
+    rc = getattr(fn_lib, name)(arr.ctypes.data, W, H, out.ctypes.data, ow, oh)
+    assert rc == 0
+    return out
+
+
+def upconvert(fn_lib, arr, name):
+    arr = np.ascontiguousarray(arr, np.int16)
+    H, W = arr.shape
+    out = np.zeros((2 * H, 2 * W), np.int16)
+    rc = getattr(fn_lib, name)(arr.ctypes.data, W, H, out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+def lambda_map(fn_lib, name, v1, v2, lam, nrefs):
+    v1 = np.ascontiguousarray(v1, abi.MV_DTYPE)
+    Y, X = v1.shape
+    v2 = None if v2 is None else np.ascontiguousarray(v2, abi.MV_DTYPE)
+    out = np.zeros((Y, X), np.float32)
+    rc = getattr(fn_lib, name)(X, Y, nrefs, v1.ctypes.data, None if v2 is None else v2.ctypes.data, lam,
+                               out.ctypes.data)
+    assert rc == 0
+    return out
