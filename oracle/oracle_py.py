@@ -19,6 +19,7 @@ from dirac_me import MEData  # noqa: E402
 
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libdirac_ref.so")
+REF_C_SO = os.path.join(HERE, "_ref", "libdirac_ref_c.so")
 
 STOP_PEL, STOP_LAMBDA, STOP_SUBPEL, STOP_MODE = 1, 2, 3, 4
 
@@ -42,7 +43,7 @@ def _load(path):
 
 
 _oracle = None
-_ref = None
+_ref = {}
 
 
 def oracle():
@@ -63,23 +64,23 @@ def oracle():
     return _oracle
 
 
-def ref_available():
-    return os.path.exists(REF_SO)
+def ref_available(variant="mmx"):
+    return os.path.exists(REF_SO if variant == "mmx" else REF_C_SO)
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        l = _load(REF_SO)
+def ref(variant="mmx"):
+    """The reference harness: its default MMX build, or the plain C build."""
+    if variant not in _ref:
+        l = _load(REF_SO if variant == "mmx" else REF_C_SO)
         if l is None:
-            raise RuntimeError("oracle/_ref/libdirac_ref.so is not built (needs the reference tree)")
+            raise RuntimeError("oracle/_ref reference build missing (needs the reference tree)")
         l.refh_me.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture), C.POINTER(oracle_picture),
                               C.c_int, C.c_int, C.POINTER(abi.dme_medata), C.POINTER(oracle_capture)]
         l.refh_downconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
         l.refh_upconvert.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         l.refh_lambda_map.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
-        _ref = l
-    return _ref
+        _ref[variant] = l
+    return _ref[variant]
 
 
 def _chk(rc, lib_=None):
@@ -155,8 +156,8 @@ def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False,
     return out, cap, geom
 
 
-def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False):
-    rc, out, cap, geom = _run(ref().refh_me, params, cur, refs, u, v, stop, capture)
+def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, variant="mmx"):
+    rc, out, cap, geom = _run(ref(variant).refh_me, params, cur, refs, u, v, stop, capture)
     if rc != 0:
         raise RuntimeError("reference harness error %d" % rc)
     return out, cap, geom
