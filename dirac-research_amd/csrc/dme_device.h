@@ -33,6 +33,10 @@ struct JobDesc {
     dme_cost *lpc[2][DME_MAX_DEPTH + 1];
     int *bsad[2];                               // zero/guide-window best SAD
     dme_mv *bmv[2];                             // ... and its vector
+    int *ptab[2];                               // pel SAD tables (see k_pel_bulk)
+    int *qtab[2];                               // sub-pel SAD tables (k_subpel_bulk)
+    int *m1tab;                                 // level-1 unit SAD tables (k_mode_tables)
+    int *m0tab;                                 // level-0 unit SAD tables
     // MEData of the picture
     dme_mv *vec[2];
     dme_cost *pc[2];

@@ -29,8 +29,11 @@ namespace dme {
 hipError_t launch_downconvert(const int16_t *in, int in_w, int16_t *out, int ow, int oh, hipStream_t st);
 hipError_t launch_upconvert(const int16_t *in, int W, int H, int16_t *up, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
-hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int threads, hipStream_t st);
-hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int threads, hipStream_t st);
+hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows,
+                           hipStream_t st);
+hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
+hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
                                 hipStream_t st);
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
@@ -97,6 +100,7 @@ struct JobState {
     int stage = 0;  // 1 pel, 2 lambda, 3 subpel, 4 mode
     DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3];
     DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1], bsad[2], bmv[2];
+    DevBuf ptab[2], qtab[2], m1tab, m0tab;
     JobDesc desc;
     std::vector<float> h_lam, h_lam1, h_lam0;
     std::vector<dme_mv> h_vec[2];
@@ -115,10 +119,44 @@ struct dme_ctx {
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dme_stats stats;
-    int wave_threads = 512;
+    int wave_threads = 1024;
+    // per-launch timing while profiling is enabled
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;  // (kernel kind, pool index of the start event)
 };
 
 namespace dme {
+
+enum KernelKind {
+    KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
+    KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_COUNT
+};
+static const char *const kKernelNames[DME_NUM_KERNELS] = {
+    "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
+    "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "", "", "", ""};
+
+// Runs a launch; with profiling on, brackets it with a pair of events.
+template <class F>
+static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
+    size_t idx = 0;
+    if (ctx->profiling) {
+        idx = ctx->ev_used.size() * 2;
+        while (ctx->ev_pool.size() < idx + 2) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ctx->ev_pool.push_back(e);
+        }
+        HIP_TRY(hipEventRecord(ctx->ev_pool[idx], ctx->stream));
+    }
+    HIP_TRY(f());
+    if (ctx->profiling) {
+        HIP_TRY(hipEventRecord(ctx->ev_pool[idx + 1], ctx->stream));
+        ctx->ev_used.push_back(std::make_pair(kind, (int)idx));
+    }
+    ++launches;
+    return DME_OK;
+}
 
 static int set_device(dme_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
@@ -134,8 +172,9 @@ static int ensure_pyramid(dme_ctx *ctx, Picture &P, int levels, int &launches) {
         const int iw = P.W >> (i - 1);
         DME_TRY(P.pyr[i].ensure(sizeof(int16_t) * (size_t)ow * oh));
         const int16_t *in = i == 1 ? P.y : P.pyr[i - 1].as<int16_t>();
-        HIP_TRY(launch_downconvert(in, iw, P.pyr[i].as<int16_t>(), ow, oh, ctx->stream));
-        ++launches;
+        DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches, [&] {
+            return launch_downconvert(in, iw, P.pyr[i].as<int16_t>(), ow, oh, ctx->stream);
+        }));
         P.pyr_levels = i;
     }
     return DME_OK;
@@ -145,8 +184,8 @@ static int ensure_pyramid(dme_ctx *ctx, Picture &P, int levels, int &launches) {
 static int ensure_up(dme_ctx *ctx, Picture &P, int &launches) {
     if (P.up_ready) return DME_OK;
     DME_TRY(P.up.ensure(sizeof(int16_t) * 4 * (size_t)P.W * P.H));
-    HIP_TRY(launch_upconvert(P.y, P.W, P.H, P.up.as<int16_t>(), ctx->stream));
-    ++launches;
+    DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
+                         [&] { return launch_upconvert(P.y, P.W, P.H, P.up.as<int16_t>(), ctx->stream); }));
     P.up_ready = true;
     return DME_OK;
 }
@@ -193,7 +232,17 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
         maxl = std::max(maxl, B);
         DME_TRY(J.bsad[r].ensure(sizeof(int) * maxl));
         DME_TRY(J.bmv[r].ensure(sizeof(dme_mv) * maxl));
+        // pel tables: two (2R+1)^2 tables per block, R = min(l+1,5) + 1
+        size_t maxt = 0;
+        for (int l = 0; l <= g.depth; ++l) {
+            const size_t R = (size_t)std::min(l + 1, 5) + 1;
+            maxt = std::max(maxt, (size_t)g.lxb[l] * g.lyb[l] * 2 * (2 * R + 1) * (2 * R + 1));
+        }
+        DME_TRY(J.ptab[r].ensure(sizeof(int) * maxt));
+        DME_TRY(J.qtab[r].ensure(sizeof(int) * 49 * B));
     }
+    DME_TRY(J.m1tab.ensure(sizeof(int) * 24 * (B / 4)));
+    DME_TRY(J.m0tab.ensure(sizeof(int) * 32 * (B / 16)));
     DME_TRY(J.mode.ensure(sizeof(int32_t) * B));
     DME_TRY(J.intra.ensure(sizeof(float) * B));
     DME_TRY(J.bi.ensure(sizeof(dme_cost) * B));
@@ -238,7 +287,11 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
         }
         d.bsad[r] = J.bsad[r].as<int>();
         d.bmv[r] = J.bmv[r].as<dme_mv>();
+        d.ptab[r] = J.ptab[r].as<int>();
+        d.qtab[r] = J.qtab[r].as<int>();
     }
+    d.m1tab = J.m1tab.as<int>();
+    d.m0tab = J.m0tab.as<int>();
     for (int r = 0; r < nrefs; ++r) d.ref[r][0] = R[r]->y;
     d.mode = J.mode.as<int32_t>();
     d.intra = J.intra.as<float>();
@@ -345,9 +398,14 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
     for (int l = maxd; l >= 0; --l) {
-        HIP_TRY(launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, ctx->stream));
-        HIP_TRY(launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, ctx->wave_threads, ctx->stream));
-        launches += 2;
+        int rows = 1;
+        for (auto *j : js)
+            if (l <= j->g.depth) rows = std::max(rows, j->g.lyb[l]);
+        DME_TRY(timed_launch(ctx, KK_PEL_BULK, launches, [&] {
+            return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, ctx->stream);
+        }));
+        DME_TRY(timed_launch(ctx, KK_PEL_WAVE, launches,
+                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, ctx->stream); }));
     }
     for (auto *j : js) j->stage = 1;
     return DME_OK;
@@ -394,25 +452,35 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
     for (auto *j : js) DME_TRY(bind_planes(ctx, *j, true, launches));
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
-    std::vector<Task> wt, sh;
-    int off = 0;
+    std::vector<Task> wt, sh, bt;
+    int off = 0, offb = 0, rows = 1;
     for (size_t i = 0; i < js.size(); ++i)
         for (int r = 0; r < js[i]->nrefs; ++r) {
-            wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
+            const int cnt = js[i]->g.xnb * js[i]->g.ynb;
             if (js[i]->p.mv_precision == 0) {
-                const int cnt = js[i]->g.xnb * js[i]->g.ynb;
                 sh.push_back(Task{(int)i, r, cnt, off});
                 off += cnt;
+            } else {
+                wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
+                bt.push_back(Task{(int)i, r, cnt, offb});
+                offb += cnt;
+                rows = std::max(rows, js[i]->g.ynb);
             }
         }
-    const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh);
+    const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
-    HIP_TRY(launch_subpel_wave(dj, dtask(ctx, wave), wave.n, ctx->wave_threads, ctx->stream));
-    ++launches;
+    if (bulk.n) {
+        DME_TRY(timed_launch(ctx, KK_SUBPEL_BULK, launches, [&] {
+            return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, ctx->stream);
+        }));
+        DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,
+                             [&] { return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows, ctx->stream); }));
+    }
     if (shift.n) {
-        HIP_TRY(launch_shift_vectors(dj, dtask(ctx, shift), shift.n, shift.total, 1, ctx->stream));
-        ++launches;
+        DME_TRY(timed_launch(ctx, KK_SHIFT, launches, [&] {
+            return launch_shift_vectors(dj, dtask(ctx, shift), shift.n, shift.total, 1, ctx->stream);
+        }));
     }
     for (auto *j : js) j->stage = 3;
     return DME_OK;
@@ -433,12 +501,16 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     }
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
-    std::vector<Task> l2, dc, sh;
-    int off = 0, offd = 0, offs = 0;
+    std::vector<Task> l2, dc, sh, mt;
+    int off = 0, offd = 0, offs = 0, offm = 0;
     for (size_t i = 0; i < js.size(); ++i) {
         const int cnt = js[i]->g.xnb * js[i]->g.ynb;
         l2.push_back(Task{(int)i, 0, cnt, off});
         off += cnt;
+        mt.push_back(Task{(int)i, 1, cnt / 4, offm});
+        offm += cnt / 4;
+        mt.push_back(Task{(int)i, 0, cnt / 16, offm});
+        offm += cnt / 16;
         for (int c = 0; c < 3; ++c) {
             dc.push_back(Task{(int)i, c, cnt, offd});
             offd += cnt;
@@ -449,17 +521,24 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
                 offs += cnt;
             }
     }
-    const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rsh = tt.add(sh);
+    const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rsh = tt.add(sh), rmt = tt.add(mt);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
-    HIP_TRY(launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, ctx->stream));
-    HIP_TRY(launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, ctx->stream));
-    HIP_TRY(launch_block_dc(dj, dtask(ctx, rdc), rdc.n, rdc.total, ctx->stream));
-    HIP_TRY(launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, ctx->stream));
-    launches += 4;
+    hipStream_t st = ctx->stream;
+    DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches,
+                         [&] { return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
+                         [&] { return launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, st); }));
+    DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,
+                         [&] { return launch_block_dc(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
+                         [&] { return launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
     if (rsh.n) {
-        HIP_TRY(launch_shift_vectors(dj, dtask(ctx, rsh), rsh.n, rsh.total, 0, ctx->stream));
-        ++launches;
+        DME_TRY(timed_launch(ctx, KK_SHIFT, launches, [&] {
+            return launch_shift_vectors(dj, dtask(ctx, rsh), rsh.n, rsh.total, 0, st);
+        }));
     }
     for (auto *j : js) j->stage = 4;
     return DME_OK;
@@ -496,6 +575,7 @@ static JobState *job_for(dme_ctx *ctx, int pic_num) {
 }
 
 static int begin_timing(dme_ctx *ctx) {
+    ctx->ev_used.clear();
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     return DME_OK;
 }
@@ -507,6 +587,17 @@ static int end_timing(dme_ctx *ctx, int launches) {
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms = ms;
     ctx->stats.launches = launches;
+    for (int k = 0; k < DME_NUM_KERNELS; ++k) {
+        ctx->stats.kernel_ms_by[k] = 0.0;
+        ctx->stats.launches_by[k] = 0;
+    }
+    for (auto &u : ctx->ev_used) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev_pool[u.second], ctx->ev_pool[u.second + 1]));
+        ctx->stats.kernel_ms_by[u.first] += t;
+        ctx->stats.launches_by[u.first] += 1;
+    }
+    ctx->ev_used.clear();
     return DME_OK;
 }
 
@@ -558,6 +649,7 @@ extern "C" int dme_destroy(dme_ctx *ctx) {
     ctx->pics.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DME_OK;
@@ -730,6 +822,27 @@ extern "C" int dme_synchronize(dme_ctx *ctx) {
     CTX_CHECK(ctx);
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return DME_OK;
+}
+
+extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
+    CTX_CHECK(ctx);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &kv : ctx->pics)
+        if (pic_num < 0 || kv.first == pic_num) {
+            kv.second->pyr_levels = 0;
+            kv.second->up_ready = false;
+        }
+    return DME_OK;
+}
+
+extern "C" int dme_set_profiling(dme_ctx *ctx, int enable) {
+    if (!ctx) return set_error(DME_ERR_INVALID, "NULL context");
+    ctx->profiling = enable != 0;
+    return DME_OK;
+}
+
+extern "C" const char *dme_kernel_name(int k) {
+    return (k >= 0 && k < DME_NUM_KERNELS) ? kKernelNames[k] : "";
 }
 
 extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) {

@@ -51,7 +51,7 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
     if (ybsep > yblen) yblen = ybsep + 4;
     if (2 * xbsep < xblen) xblen = xbsep + 4;
     if (2 * ybsep < yblen) yblen = ybsep + 4;
-    if (xblen > 64 || yblen > 64) return set_error(DME_ERR_INVALID, "block length above 64 is not supported");
+    if (xblen > 32 || yblen > 32) return set_error(DME_ERR_INVALID, "block lengths above 32 are not supported");
 
     g.W = width; g.H = height; g.CW = width >> 1; g.CH = height >> 1;
     g.blen[2][0] = xblen; g.blen[2][1] = yblen; g.bsep[2][0] = xbsep; g.bsep[2][1] = ybsep;

@@ -235,61 +235,117 @@ __global__ __launch_bounds__(256) void k_upconvert(const int16_t *__restrict__ i
 }
 
 // ---------------------------------------------------------------------------
-// Pel search, bulk part: per block, the first minimum over the zero window
-// and the guide window (the lists that do not depend on the prediction;
-// pixel_match.cpp:304, 351-364).  One wave per block, four per workgroup.
+// Pel search, bulk part.  For every block of a level, two SAD tables of
+// radius R = r + 1 (r = min(level+1, 5), pixel_match.cpp:258-259): around
+// the zero vector (T_Z) and around the guide vector (T_G, levels below the
+// coarsest).  The zero and guide candidate lists of the reference
+// (pixel_match.cpp:304, 351-364) are the radius-r cores of these tables, so
+// their first minimum A (in enumeration order) is taken here; the wavefront
+// then only has to look up the prediction window, which falls inside one of
+// the tables for most blocks.  One wave per block; picture block and both
+// reference regions are staged in LDS.
+//
+// Table layout: ptab[r][(block * 2 + t) * D*D + (dy + R) * D + (dx + R)],
+// D = 2R + 1, t = 0 zero-centred, 1 guide-centred; INT_MAX where absent.
 // ---------------------------------------------------------------------------
+#define PEL_MAXB 32                        // largest block side (host-checked)
+#define PEL_MAXR 6
+#define PEL_REG (PEL_MAXB + 2 * PEL_MAXR)  // 44
+#define PEL_MAXTS ((2 * PEL_MAXR + 1) * (2 * PEL_MAXR + 1))
+
+__device__ __forceinline__ int pel_radius(int level) { return min(level + 1, 5); }
+
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total, int level) {
-    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (gi >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    __shared__ int16_t s_pic[4][PEL_MAXB * PEL_MAXB];
+    __shared__ int16_t s_reg[4][2][PEL_REG * PEL_REG];
+    __shared__ int s_tab[4][2 * PEL_MAXTS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * 4 + wv;
+    const bool valid = gi < total;
+    const Task tk = tasks[find_task(tasks, ntasks, valid ? gi : total - 1)];
     const JobDesc &J = jobs[tk.job];
-    const int r = tk.ref, local = gi - tk.offset;
+    const int r = tk.ref, local = (valid ? gi : total - 1) - tk.offset;
     const int X = J.lxb[level];
     const int xb = local % X, yb = local / X;
     const int W = J.lw[level], H = J.lh[level];
+    const int16_t *__restrict__ pic = J.pic[level];
+    const int16_t *__restrict__ ref = J.ref[r][level];
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
+    const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
+    const bool has_guide = level < J.depth;
+    dme_mv g;
+    g.x = 0;
+    g.y = 0;
+    if (has_guide) {
+        const int GX = J.lxb[level + 1], GY = J.lyb[level + 1];
+        const dme_mv gv = J.lvec[r][level + 1][(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
+        g.x = gv.x * 2;
+        g.y = gv.y * 2;
+    }
+    const bool work = valid && !b.empty();
+    const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
+    if (work) {
+        for (int i = lane; i < b.xl * b.yl; i += WAVE) {
+            const int dy = i / b.xl, dx = i - dy * b.xl;
+            s_pic[wv][dy * PEL_MAXB + dx] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
+        }
+        for (int t = 0; t < (has_guide ? 2 : 1); ++t) {
+            const int cx = t ? g.x : 0, cy = t ? g.y : 0;
+            for (int i = lane; i < RW * RH; i += WAVE) {
+                const int ry = i / RW, rx = i - ry * RW;
+                s_reg[wv][t][i] = ref[(size_t)clampi(b.yp + cy - R + ry, 0, H - 1) * W +
+                                      clampi(b.xp + cx - R + rx, 0, W - 1)];
+            }
+        }
+    }
+    __syncthreads();
+    if (work) {
+        for (int c = lane; c < 2 * TS; c += WAVE) {
+            const int t = c >= TS ? 1 : 0, e = c - t * TS;
+            int s = INT_MAX;
+            if (t == 0 || has_guide) {
+                const int ox = e % D, oy = e / D;  // region offset of the candidate
+                const int16_t *rg = &s_reg[wv][t][oy * RW + ox];
+                s = 0;
+                for (int y = 0; y < b.yl; ++y) {
+                    const int16_t *pr = &s_pic[wv][y * PEL_MAXB];
+                    const int16_t *rr = rg + y * RW;
+                    for (int x = 0; x < b.xl; ++x) s += abs((int)pr[x] - (int)rr[x]);
+                }
+            }
+            s_tab[wv][c] = s;
+            J.ptab[r][(size_t)local * 2 * TS + c] = s;
+        }
+    }
+    __syncthreads();
     int best_sad = 10000000;
     dme_mv best_mv;
     best_mv.x = 0;
     best_mv.y = 0;
-    if (!b.empty()) {
-        const int rad = min(level + 1, 5);
+    if (work) {
+        // first minimum over the zero list then the guide list (radius rad)
         const int n = (2 * rad + 1) * (2 * rad + 1);
-        dme_mv g;
-        g.x = 0;
-        g.y = 0;
-        int nc = n;
-        if (level < J.depth) {
-            const int GX = J.lxb[level + 1], GY = J.lyb[level + 1];
-            const dme_mv gv = J.lvec[r][level + 1][(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
-            g.x = gv.x * 2;
-            g.y = gv.y * 2;
-            nc = 2 * n;
+        const int nc = has_guide ? 2 * n : n;
+        unsigned long long key = ~0ull;
+        for (int k = lane; k < nc; k += WAVE) {
+            const int t = k >= n ? 1 : 0;
+            const dme_mv o = win_off(k - t * n, rad);
+            const int s = s_tab[wv][t * TS + (o.y + R) * D + (o.x + R)];
+            const unsigned long long kk = ((unsigned long long)(unsigned)s << 32) | (unsigned)k;
+            key = kk < key ? kk : key;
         }
-        const unsigned long long key = wave_pel_argmin(J.pic[level], J.ref[r][level], W, H, b, nc, [&](int c) {
-            dme_mv m;
-            if (c < n) {
-                m = win_off(c, rad);
-            } else {
-                m = win_off(c - n, rad);
-                m.x += g.x;
-                m.y += g.y;
-            }
-            return m;
-        });
-        const int c = (int)(key & 0xffffffffu);
+        key = wave_min_u64(key);
+        const int k = (int)(key & 0xffffffffu);
         best_sad = (int)(key >> 32);
-        if (c < n) {
-            best_mv = win_off(c, rad);
-        } else {
-            best_mv = win_off(c - n, rad);
+        const int t = k >= n ? 1 : 0;
+        best_mv = win_off(k - t * n, rad);
+        if (t) {
             best_mv.x += g.x;
             best_mv.y += g.y;
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (valid && lane == 0) {
         J.bsad[r][local] = best_sad;
         J.bmv[r][local] = best_mv;
     }
@@ -298,67 +354,143 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
 // ---------------------------------------------------------------------------
 // Pel search, wavefront part (PixelMatcher::MatchPic/DoBlock,
 // pixel_match.cpp:242-380; BlockMatcher::FindBestMatchPel,
-// block_match.cpp:242-275).  One workgroup per (picture, reference).
+// block_match.cpp:242-275).  One workgroup per (picture, reference), one
+// lane per block row: at step t row y works on block x = t - 2y, reading the
+// left, top and top-right results from a 3-slot LDS ring per row.  The
+// prediction window is looked up in a bulk table when it lies inside one;
+// otherwise the lane's wave computes its SADs together.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                    int level) {
+    extern __shared__ dme_mv s_ring[];  // [Y][3]
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (level > J.depth || tk.ref >= J.nrefs) return;  // uniform per workgroup
     const int r = tk.ref;
     const int X = J.lxb[level], Y = J.lyb[level];
     const int W = J.lw[level], H = J.lh[level];
-    const int rad = min(level + 1, 5);
+    const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
     const int n = (2 * rad + 1) * (2 * rad + 1);
+    const bool has_guide = level < J.depth;
+    const int GX = has_guide ? J.lxb[level + 1] : 1, GY = has_guide ? J.lyb[level + 1] : 1;
+    const dme_mv *__restrict__ G = has_guide ? J.lvec[r][level + 1] : nullptr;
+    const int16_t *__restrict__ pic = J.pic[level];
+    const int16_t *__restrict__ ref = J.ref[r][level];
+    const int *__restrict__ tab = J.ptab[r];
+    const int *__restrict__ bsad = J.bsad[r];
+    const dme_mv *__restrict__ bmv = J.bmv[r];
     dme_mv *__restrict__ V = J.lvec[r][level];
     dme_cost *__restrict__ C = J.lpc[r][level];
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
+    int boff[2] = {J.boff[2][0], J.boff[2][1]};
     const int lane = threadIdx.x & 63;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = 0; t <= T; ++t) {
-        const int ylo = t - (X - 1) > 0 ? (t - (X - 1) + 1) >> 1 : 0;
-        const int yhi = min(Y - 1, t >> 1);
-        for (int yb = ylo + wave; yb <= yhi; yb += nwaves) {
+        for (int base = 0; base < Y; base += blockDim.x) {
+            const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
-            const size_t k = (size_t)yb * X + xb;
-            // prediction rules (pixel_match.cpp:314-347)
-            dme_mv pred;
-            if (yb == 0) {
-                if (xb == 0) { pred.x = 0; pred.y = 0; }
-                else pred = V[xb - 1];
-            } else if (xb == 0) {
-                pred = V[k - X];
-            } else if (xb < X - 1) {
-                pred = med3(V[k - 1], V[k - X], V[k - X + 1]);
-            } else {
-                pred = mean2(V[k - X], V[k - 1]);
-            }
-            const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
-            int best_sad = J.bsad[r][k];
-            dme_mv best = J.bmv[r][k];
-            if (!b.empty()) {
-                const unsigned long long key =
-                    wave_pel_argmin(J.pic[level], J.ref[r][level], W, H, b, n, [&](int c) {
-                        dme_mv m = win_off(c, rad);
-                        m.x += pred.x;
-                        m.y += pred.y;
-                        return m;
-                    });
-                const int s = (int)(key >> 32);
-                if (s < best_sad) {
-                    best_sad = s;
-                    best = win_off((int)(key & 0xffffffffu), rad);
-                    best.x += pred.x;
-                    best.y += pred.y;
+            const bool active = yb < Y && xb >= 0 && xb < X;
+            dme_mv pred, best;
+            pred.x = pred.y = 0;
+            best = pred;
+            int best_sad = 10000000;
+            bool miss = false;
+            Blk b;
+            b.xp = b.yp = 0;
+            b.xl = b.yl = 0;
+            if (active) {
+                const size_t k = (size_t)yb * X + xb;
+                // prediction rules (pixel_match.cpp:314-347)
+                if (yb == 0) {
+                    if (xb > 0) pred = s_ring[yb * 3 + (t - 1) % 3];
+                } else if (xb == 0) {
+                    pred = s_ring[(yb - 1) * 3 + (t - 2) % 3];
+                } else if (xb < X - 1) {
+                    pred = med3(s_ring[yb * 3 + (t - 1) % 3], s_ring[(yb - 1) * 3 + (t - 2) % 3],
+                                s_ring[(yb - 1) * 3 + (t - 1) % 3]);
+                } else {
+                    pred = mean2(s_ring[(yb - 1) * 3 + (t - 2) % 3], s_ring[yb * 3 + (t - 1) % 3]);
+                }
+                best_sad = bsad[k];
+                best = bmv[k];
+                b = block_limits(blen, bsep, boff, W, H, xb, yb);
+                if (!b.empty()) {
+                    dme_mv g;
+                    g.x = g.y = 0;
+                    if (has_guide) {
+                        const dme_mv gv = G[(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
+                        g.x = gv.x * 2;
+                        g.y = gv.y * 2;
+                    }
+                    int tsel = -1;
+                    dme_mv cen;
+                    cen.x = cen.y = 0;
+                    if (has_guide && abs(pred.x - g.x) <= 1 && abs(pred.y - g.y) <= 1) {
+                        tsel = 1;
+                        cen = g;
+                    } else if (abs(pred.x) <= 1 && abs(pred.y) <= 1) {
+                        tsel = 0;
+                    }
+                    if (tsel >= 0) {
+                        const int *tb = tab + ((size_t)k * 2 + tsel) * TS;
+                        int ps = INT_MAX, pk = 0;
+                        for (int q = 0; q < n; ++q) {
+                            const dme_mv o = win_off(q, rad);
+                            const int s = tb[(pred.y + o.y - cen.y + R) * D + (pred.x + o.x - cen.x + R)];
+                            if (s < ps) {
+                                ps = s;
+                                pk = q;
+                            }
+                        }
+                        if (ps < best_sad) {
+                            best_sad = ps;
+                            best = win_off(pk, rad);
+                            best.x += pred.x;
+                            best.y += pred.y;
+                        }
+                    } else {
+                        miss = true;
+                    }
                 }
             }
-            if (lane == 0) {
+            // prediction windows outside the tables: computed by the whole wave
+            unsigned long long mm = __ballot(miss);
+            while (mm) {
+                const int src = __ffsll((long long)mm) - 1;
+                mm &= mm - 1;
+                Blk bm;
+                bm.xp = __shfl(b.xp, src, WAVE);
+                bm.yp = __shfl(b.yp, src, WAVE);
+                bm.xl = __shfl(b.xl, src, WAVE);
+                bm.yl = __shfl(b.yl, src, WAVE);
+                dme_mv pm;
+                pm.x = __shfl(pred.x, src, WAVE);
+                pm.y = __shfl(pred.y, src, WAVE);
+                const unsigned long long key = wave_pel_argmin(pic, ref, W, H, bm, n, [&](int c) {
+                    dme_mv m = win_off(c, rad);
+                    m.x += pm.x;
+                    m.y += pm.y;
+                    return m;
+                });
+                if (lane == src) {
+                    const int s = (int)(key >> 32);
+                    if (s < best_sad) {
+                        best_sad = s;
+                        best = win_off((int)(key & 0xffffffffu), rad);
+                        best.x += pred.x;
+                        best.y += pred.y;
+                    }
+                }
+            }
+            if (active) {
+                const size_t k = (size_t)yb * X + xb;
                 V[k] = best;
                 dme_cost c;
                 c.SAD = (float)best_sad;
                 c.mvcost = (float)(int16_t)norm1(best.x - pred.x, best.y - pred.y);
                 c.total = c.SAD + 0.0f * c.mvcost;  // SetTotal(0.0)
                 C[k] = c;
+                s_ring[yb * 3 + t % 3] = best;
             }
         }
         __syncthreads();
@@ -437,27 +569,6 @@ __device__ float chain_total(float start, const int16_t *__restrict__ pic, int W
     return sum;
 }
 
-// The bail-out Diff with a start cost: update best when the candidate's
-// total is strictly lower.  MMX semantics add the integer SAD to the start
-// once (me_utils_mmx.cpp:270-278).
-__device__ __forceinline__ void try_candidate(const JobDesc &J, const int16_t *up, int UW, int UH, const Blk &b,
-                                              dme_mv mv, int step, int isad, float mvcost, float lambda,
-                                              dme_cost &best, dme_mv &best_mv) {
-    const float start = mvcost * lambda;
-    const UpStart st = up_start(b, mv, step, UW, UH);
-    float total;
-    if (st.bounds || J.fp_sem == DME_FP_C)
-        total = chain_total(start, J.pic[0], J.W, up, UW, UH, b, st);
-    else
-        total = (float)isad + start;
-    if (total < best.total) {
-        best_mv = mv;
-        best.total = total;
-        best.mvcost = mvcost;
-        best.SAD = total - start;
-    }
-}
-
 // GetVarUp (block_match.cpp:233-240) through its short return type
 __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
     const int a = norm1(m.x - pred.x, m.y - pred.y), c = norm1(m.x, m.y);
@@ -465,72 +576,222 @@ __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
 }
 
 // ---------------------------------------------------------------------------
-// Sub-pel refinement (SubpelRefine::MatchPic/DoBlock/GetPred,
+// Sub-pel refinement, bulk part.  RefineMatchSubp (block_match.cpp:327-467)
+// only ever evaluates positions within 3 quarter pels (1 half pel at half
+// precision) of the pel vector: stage 1 moves at most one half pel per axis,
+// stage 2 one quarter pel more.  Their integer SADs do not depend on the
+// prediction, so all (2*3+1)^2 of them are tabulated per block here, from an
+// LDS copy of the up-converted reference region; the wavefront then only
+// adds costs and compares.  Layout: qtab[r][block * QS + (dy+rad)*Q + dx+rad].
+// ---------------------------------------------------------------------------
+#define SUB_MAXREG (2 * PEL_MAXB + 3)  // 67
+
+__device__ __forceinline__ int sub_radius(int prec) { return prec == 2 ? 3 : 1; }
+
+__global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                     int ntasks, int total) {
+    __shared__ int16_t s_pic[4][PEL_MAXB * PEL_MAXB];
+    __shared__ int16_t s_up[4][SUB_MAXREG * SUB_MAXREG];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * 4 + wv;
+    const bool valid = gi < total;
+    const Task tk = tasks[find_task(tasks, ntasks, valid ? gi : total - 1)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, k = (valid ? gi : total - 1) - tk.offset;
+    const int X = J.xnb, W = J.W, H = J.H, UW = 2 * W, UH = 2 * H;
+    const int xb = k % X, yb = k / X;
+    const int prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
+    const bool work = valid && !b.empty() && prec > 0;
+    const dme_mv pel = J.vec[r][k];
+    dme_mv c;
+    c.x = pel.x << prec;
+    c.y = pel.y << prec;
+    // up-plane region covering every candidate's samples
+    int x0, y0, RW, RH;
+    if (prec == 2) {
+        x0 = 2 * b.xp + ((c.x - rad) >> 1);
+        y0 = 2 * b.yp + ((c.y - rad) >> 1);
+        RW = ((c.x + rad) >> 1) - ((c.x - rad) >> 1) + 2 * b.xl;
+        RH = ((c.y + rad) >> 1) - ((c.y - rad) >> 1) + 2 * b.yl;
+    } else {
+        x0 = 2 * b.xp + c.x - rad;
+        y0 = 2 * b.yp + c.y - rad;
+        RW = 2 * rad + 2 * b.xl - 1;
+        RH = 2 * rad + 2 * b.yl - 1;
+    }
+    const int16_t *__restrict__ up = J.up[r];
+    if (work) {
+        for (int i = lane; i < b.xl * b.yl; i += WAVE) {
+            const int dy = i / b.xl, dx = i - dy * b.xl;
+            s_pic[wv][dy * PEL_MAXB + dx] = J.pic[0][(size_t)(b.yp + dy) * W + b.xp + dx];
+        }
+        for (int i = lane; i < RW * RH; i += WAVE) {
+            const int ry = i / RW, rx = i - ry * RW;
+            s_up[wv][ry * SUB_MAXREG + rx] = up[(size_t)clampi(y0 + ry, 0, UH - 1) * UW + clampi(x0 + rx, 0, UW - 1)];
+        }
+    }
+    __syncthreads();
+    if (work && lane < QS) {
+        const int dx = lane % Q - rad, dy = lane / Q - rad;
+        const int mx = c.x + dx, my = c.y + dy;
+        int sx, sy, rx, ry;
+        if (prec == 2) {
+            sx = 2 * b.xp + (mx >> 1) - x0; sy = 2 * b.yp + (my >> 1) - y0; rx = mx & 1; ry = my & 1;
+        } else {
+            sx = 2 * b.xp + mx - x0; sy = 2 * b.yp + my - y0; rx = 0; ry = 0;
+        }
+        const int w0 = (2 - rx) * (2 - ry), w1 = rx * (2 - ry), w2 = (2 - rx) * ry, w3 = rx * ry;
+        int s = 0;
+        for (int y = 0; y < b.yl; ++y) {
+            const int16_t *r0 = &s_up[wv][(sy + 2 * y) * SUB_MAXREG + sx];
+            const int16_t *r1 = r0 + SUB_MAXREG;
+            const int16_t *pr = &s_pic[wv][y * PEL_MAXB];
+            for (int x = 0; x < b.xl; ++x) {
+                const int v = (w0 * r0[2 * x] + w1 * r0[2 * x + 1] + w2 * r1[2 * x] + w3 * r1[2 * x + 1] + 2) >> 2;
+                s += abs(v - (int)pr[x]);
+            }
+        }
+        J.qtab[r][(size_t)k * QS + lane] = s;
+    }
+}
+
+// float total of a bail-out Diff (me_utils.cpp:481-571, 686-867): MMX
+// semantics add the integer SAD once to the start cost in bounds
+// (me_utils_mmx.cpp:270-278); the bounds-checked path and the plain C build
+// add per pixel in float.  That per-pixel sum equals the single addition
+// whenever every partial sum is representable, which holds when the start is
+// a multiple of the float spacing at the final magnitude; otherwise it is
+// replayed pixel by pixel.
+__device__ __forceinline__ bool chain_is_exact(float start, int isad) {
+    const double m = (double)start + (double)isad;
+    if (m < 1.0) return start == 0.0f;
+    int e;
+    frexp(m, &e);                        // m in [2^(e-1), 2^e)
+    const double u = ldexp(1.0, e - 24);  // float spacing in that binade
+    return u <= 1.0 && fmod((double)start, u) == 0.0;
+}
+
+__device__ float cand_total(const JobDesc &J, const int16_t *up, int UW, int UH, const Blk &b, dme_mv mv, int step,
+                            int isad, float start) {
+    const UpStart st = up_start(b, mv, step, UW, UH);
+    if ((st.bounds || J.fp_sem == DME_FP_C) && !chain_is_exact(start, isad))
+        return chain_total(start, J.pic[0], J.W, up, UW, UH, b, st);
+    return (float)isad + start;
+}
+
+// ---------------------------------------------------------------------------
+// Sub-pel refinement, wavefront part (SubpelRefine::MatchPic/DoBlock/GetPred,
 // me_subpel.cpp:101-176; BlockMatcher::RefineMatchSubp,
-// block_match.cpp:327-467).  One workgroup per (picture, reference),
-// wavefront t = x + 2y.
+// block_match.cpp:327-467).  One lane per block row, wavefront t = x + 2y;
+// the left, top-left, top and top-right refined vectors come from a 4-slot
+// LDS ring per row; each lane copies its block's SAD table into its own LDS
+// slot.  Predictor SADs outside the table are computed by the whole wave.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks) {
+    extern __shared__ int s_dyn[];  // ring [Y][4] dme_mv, then tables [Y][QS] int
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (tk.ref >= J.nrefs || J.prec == 0) return;
-    const int r = tk.ref, prec = J.prec;
+    const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
     const int X = J.xnb, Y = J.ynb, W = J.W, H = J.H;
     const int UW = 2 * W, UH = 2 * H;
-    const int16_t *up = J.up[r];
+    dme_mv *s_ring = reinterpret_cast<dme_mv *>(s_dyn);
+    int *s_tab = s_dyn + 2 * 4 * Y;
+    const int16_t *__restrict__ up = J.up[r];
+    const int *__restrict__ qtab = J.qtab[r];
+    const float *__restrict__ lamv = J.lam;
     dme_mv *__restrict__ V = J.vec[r];
     dme_cost *__restrict__ C = J.pc[r];
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
+    int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
+    int boff[2] = {J.boff[2][0], J.boff[2][1]};
+    const int lane = threadIdx.x & 63;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = 0; t <= T; ++t) {
-        const int ylo = t - (X - 1) > 0 ? (t - (X - 1) + 1) >> 1 : 0;
-        const int yhi = min(Y - 1, t >> 1);
-        for (int yb = ylo + wave; yb <= yhi; yb += nwaves) {
+        for (int base = 0; base < Y; base += blockDim.x) {
+            const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
-            const size_t k = (size_t)yb * X + xb;
-            // GetPred: median of the available left, top-left, top, top-right
-            dme_mv nb[4];
-            int nn = 0;
-            if (xb > 0) nb[nn++] = V[k - 1];
-            if (xb > 0 && yb > 0) nb[nn++] = V[k - X - 1];
-            if (yb > 0) nb[nn++] = V[k - X];
-            if (yb > 0 && xb + 1 < X) nb[nn++] = V[k - X + 1];
-            dme_mv pred;
-            if (nn == 0) { pred.x = 0; pred.y = 0; }
-            else if (nn == 1) pred = nb[0];
-            else if (nn == 2) pred = mean2(nb[0], nb[1]);
-            else if (nn == 3) pred = med3(nb[0], nb[1], nb[2]);
-            else {
-                int sx = 0, sy = 0, mxx = nb[0].x, mxy = nb[0].y, mnx = nb[0].x, mny = nb[0].y;
-                for (int i = 0; i < 4; ++i) {
-                    sx += nb[i].x; sy += nb[i].y;
-                    mxx = max(mxx, nb[i].x); mxy = max(mxy, nb[i].y);
-                    mnx = min(mnx, nb[i].x); mny = min(mny, nb[i].y);
-                }
-                pred.x = (sx - (mxx + mnx) + 1) >> 1;
-                pred.y = (sy - (mxy + mny) + 1) >> 1;
-            }
-            const float lambda = J.lam[k];
-            const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
-            const dme_mv pel = V[k];
-            dme_cost cost = C[k];
-            dme_mv pel_up;
-            pel_up.x = pel.x << prec;
-            pel_up.y = pel.y << prec;
-            cost.mvcost = var_up(pred, pel_up);
-            cost.total = cost.SAD + lambda * cost.mvcost;
-            const float lim = (float)(2 * b.xl * b.yl);
+            const bool active = yb < Y && xb >= 0 && xb < X;
+            const size_t k = active ? (size_t)yb * X + xb : 0;
+            dme_mv pred, pel;
+            pred.x = pred.y = pel.x = pel.y = 0;
+            dme_cost cost;
+            cost.SAD = cost.mvcost = cost.total = 0.0f;
+            float lambda = 0.0f, lim = 0.0f;
+            Blk b;
+            b.xp = b.yp = b.xl = b.yl = 0;
+            int *mytab = s_tab + (active ? yb : 0) * QS;
+            bool done = true, miss = false;
             dme_mv out_mv;
             dme_cost out_cost;
-            if (cost.SAD < lim) {
-                out_mv = pel_up;
-                out_cost = cost;
-            } else {
+            out_mv.x = out_mv.y = 0;
+            out_cost = cost;
+            int psad = 0;
+            if (active) {
+                // GetPred: median of the available left, top-left, top, top-right
+                dme_mv nb[4];
+                int nn = 0;
+                if (xb > 0) nb[nn++] = s_ring[yb * 4 + (t - 1) % 4];
+                if (xb > 0 && yb > 0) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 3) % 4];
+                if (yb > 0) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 2) % 4];
+                if (yb > 0 && xb + 1 < X) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 1) % 4];
+                if (nn == 1) pred = nb[0];
+                else if (nn == 2) pred = mean2(nb[0], nb[1]);
+                else if (nn == 3) pred = med3(nb[0], nb[1], nb[2]);
+                else if (nn == 4) {
+                    int sx = 0, sy = 0, mxx = nb[0].x, mxy = nb[0].y, mnx = nb[0].x, mny = nb[0].y;
+                    for (int i = 0; i < 4; ++i) {
+                        sx += nb[i].x; sy += nb[i].y;
+                        mxx = max(mxx, nb[i].x); mxy = max(mxy, nb[i].y);
+                        mnx = min(mnx, nb[i].x); mny = min(mny, nb[i].y);
+                    }
+                    pred.x = (sx - (mxx + mnx) + 1) >> 1;
+                    pred.y = (sy - (mxy + mny) + 1) >> 1;
+                }
+                lambda = lamv[k];
+                b = block_limits(blen, bsep, boff, W, H, xb, yb);
+                pel = V[k];
+                cost = C[k];
+                dme_mv pel_up;
+                pel_up.x = pel.x << prec;
+                pel_up.y = pel.y << prec;
+                cost.mvcost = var_up(pred, pel_up);
+                cost.total = cost.SAD + lambda * cost.mvcost;
+                lim = (float)(2 * b.xl * b.yl);
+                if (cost.SAD < lim) {
+                    out_mv = pel_up;
+                    out_cost = cost;
+                } else {
+                    done = false;
+                    if (!b.empty()) {
+                        for (int q = 0; q < QS; ++q) mytab[q] = qtab[k * QS + q];
+                        const int dx = pred.x - pel_up.x, dy = pred.y - pel_up.y;
+                        if (abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
+                        else miss = true;
+                    }
+                }
+            }
+            // predictor SADs outside the table: the wave computes them together
+            unsigned long long mm = __ballot(miss);
+            while (mm) {
+                const int src = __ffsll((long long)mm) - 1;
+                mm &= mm - 1;
+                Blk bm;
+                bm.xp = __shfl(b.xp, src, WAVE);
+                bm.yp = __shfl(b.yp, src, WAVE);
+                bm.xl = __shfl(b.xl, src, WAVE);
+                bm.yl = __shfl(b.yl, src, WAVE);
+                dme_mv pm;
+                pm.x = __shfl(pred.x, src, WAVE);
+                pm.y = __shfl(pred.y, src, WAVE);
+                int s = 0;
+                wave_up_sads(J.pic[0], W, up, UW, UH, bm, prec, &pm, 1, &s);
+                if (lane == src) psad = s;
+            }
+            if (active && !done) {
                 dme_cost pcost;
                 pcost.mvcost = 0.0f;
-                int ps = 0;
-                if (!b.empty()) wave_up_sads(J.pic[0], W, up, UW, UH, b, prec, &pred, 1, &ps);
-                pcost.SAD = (float)ps;
+                pcost.SAD = (float)psad;
                 pcost.total = pcost.SAD;
                 if (pcost.SAD < lim) {
                     out_mv = pred;
@@ -539,43 +800,44 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                     dme_cost best = cost;
                     dme_mv bmv = pel;
                     bool bailed = false;
+                    const int cx = pel.x << prec, cy = pel.y << prec;
                     for (int i = 1; i <= prec; ++i) {
                         bmv.x <<= 1;
                         bmv.y <<= 1;
                         dme_mv tb = bmv;
                         const int sh = prec - i;
                         if (!b.empty()) {
-                            dme_mv cd[4];
-                            cd[0].x = bmv.x - 1; cd[0].y = bmv.y;
-                            cd[1].x = bmv.x + 1; cd[1].y = bmv.y;
-                            cd[2].x = bmv.x; cd[2].y = bmv.y - 1;
-                            cd[3].x = bmv.x; cd[3].y = bmv.y + 1;
-                            int sd[4];
-                            wave_up_sads(J.pic[0], W, up, UW, UH, b, i, cd, 4, sd);
-                            for (int q = 0; q < 4; ++q) {
-                                dme_mv cs;
-                                cs.x = cd[q].x << sh;
-                                cs.y = cd[q].y << sh;
-                                try_candidate(J, up, UW, UH, b, cd[q], i, sd[q], var_up(pred, cs), lambda, best, tb);
-                            }
-                            int n2 = 0;
-                            if (tb.x != bmv.x) {
-                                cd[0].x = tb.x; cd[0].y = tb.y - 1;
-                                cd[1].x = tb.x; cd[1].y = tb.y + 1;
-                                n2 = 2;
-                            } else if (tb.y != bmv.y) {
-                                cd[0].x = tb.x - 1; cd[0].y = tb.y;
-                                cd[1].x = tb.x + 1; cd[1].y = tb.y;
-                                n2 = 2;
-                            }
-                            if (n2) {
-                                wave_up_sads(J.pic[0], W, up, UW, UH, b, i, cd, 2, sd);
-                                for (int q = 0; q < 2; ++q) {
-                                    dme_mv cs;
-                                    cs.x = cd[q].x << sh;
-                                    cs.y = cd[q].y << sh;
-                                    try_candidate(J, up, UW, UH, b, cd[q], i, sd[q], var_up(pred, cs), lambda, best, tb);
+                            // candidates in units of this stage; table index in final units
+                            auto test = [&](int mx, int my) {
+                                dme_mv m;
+                                m.x = mx;
+                                m.y = my;
+                                dme_mv mf;
+                                mf.x = mx << sh;
+                                mf.y = my << sh;
+                                const int s = mytab[(mf.y - cy + rad) * Q + (mf.x - cx + rad)];
+                                const float mvc = var_up(pred, mf);
+                                const float start = mvc * lambda;
+                                const float total = cand_total(J, up, UW, UH, b, m, i, s, start);
+                                if (total < best.total) {
+                                    tb = m;
+                                    best.total = total;
+                                    best.mvcost = mvc;
+                                    best.SAD = total - start;
                                 }
+                            };
+                            test(bmv.x - 1, bmv.y);
+                            test(bmv.x + 1, bmv.y);
+                            test(bmv.x, bmv.y - 1);
+                            test(bmv.x, bmv.y + 1);
+                            if (tb.x != bmv.x) {
+                                const dme_mv nb2 = tb;
+                                test(nb2.x, nb2.y - 1);
+                                test(nb2.x, nb2.y + 1);
+                            } else if (tb.y != bmv.y) {
+                                const dme_mv nb2 = tb;
+                                test(nb2.x - 1, nb2.y);
+                                test(nb2.x + 1, nb2.y);
                             }
                         }
                         bmv = tb;
@@ -593,16 +855,16 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                     }
                 }
             }
-            if (lane == 0) {
+            if (active) {
                 V[k] = out_mv;
                 C[k] = out_cost;
+                s_ring[yb * 4 + t % 4] = out_mv;
             }
         }
         __syncthreads();
     }
 }
 
-// Pixel-precision path: vectors doubled before mode decision and halved
 // after it (picture_compress.cpp:240-259, 274-296).
 __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                        int ntasks, int total, int left) {
@@ -730,6 +992,8 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
 
 // Prediction of ModeDecider::DoME from the level-2 vectors
 // (me_mode_decn.cpp:308-321).
+// Prediction of ModeDecider::DoME from the level-2 vectors
+// (me_mode_decn.cpp:308-321).
 __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X2, int xb, int yb) {
     dme_mv p;
     const size_t k = (size_t)yb * X2 + xb;
@@ -740,18 +1004,103 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
     return p;
 }
 
-// ModeDecider::DoME for one unit and reference (me_mode_decn.cpp:279-364,
-// BlockMatcher::FindBestMatchSubp, block_match.cpp:277-325).  cands are the
-// four guide vectors; duplicates are dropped as AddNewVlist does.
-__device__ void unit_me(const JobDesc &J, int level, int xu, int yu, int r, const dme_mv g[4], float lambda,
-                        dme_mv &out_mv, dme_cost &out_cost) {
-    dme_mv cd[4];
-    int nc = 0;
-    for (int q = 0; q < 4; ++q) {
-        bool dup = false;
-        for (int s = 0; s < nc; ++s) dup = dup || (cd[s].x == g[q].x && cd[s].y == g[q].y);
-        if (!dup) cd[nc++] = g[q];
+// ---------------------------------------------------------------------------
+// Mode decision tables.  The candidates of a sub-superblock (level 1) are the
+// level-2 vectors of its four blocks, fixed before any superblock decision
+// (me_mode_decn.cpp:296-300); those of a superblock (level 0) are level-1
+// results, hence level-2 vectors of its sixteen blocks.  Their SADs, and the
+// bi-prediction SADs of every level-1 candidate pair, do not depend on the
+// predictions, so they are computed here in parallel:
+//   m1tab[unit * 24 + r*4 + q]      unit candidate q of reference r
+//   m1tab[unit * 24 + 8 + q1*4 + q2] bi-prediction of candidates q1, q2
+//   m0tab[sb * 32 + r*16 + q]       superblock candidate q (block q of the SB)
+// Duplicate vectors are evaluated once.  One wave per unit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int up_sample(const int16_t *__restrict__ up, int UW, int UH, const Blk &b, dme_mv mv,
+                                         int step, int dy, int dx) {
+    const UpStart s = up_start(b, mv, step, UW, UH);
+    return up_val(up, UW, UH, s, dy, dx);
+}
+
+__global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                     int ntasks, int total) {
+    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int lane = threadIdx.x & 63;
+    const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
+    const int u = gi - tk.offset;
+    const int X2 = J.xnb, W = J.W, H = J.H, UW = 2 * W, UH = 2 * H, step = J.step, nr = J.nrefs;
+    const int XU = level ? J.xnb / 2 : J.xsb;
+    const int xu = u % XU, yu = u / XU;
+    const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], W, H, xu, yu);
+    const int nq = level ? 4 : 16, side = level ? 2 : 4;
+    const int16_t *__restrict__ pic = J.pic[0];
+    int *out = level ? J.m1tab + (size_t)u * 24 : J.m0tab + (size_t)u * 32;
+    if (b.empty()) {
+        if (lane < (level ? 24 : 32)) out[lane] = 0;
+        return;
     }
+    // candidate vectors and their first occurrence
+    dme_mv cv[2][16];
+    int first[2][16];
+    for (int r = 0; r < nr; ++r)
+        for (int q = 0; q < nq; ++q) {
+            const int bx = xu * side + (q % side), by = yu * side + (q / side);
+            cv[r][q] = J.vec[r][(size_t)by * X2 + bx];
+            first[r][q] = q;
+            for (int p = 0; p < q; ++p)
+                if (first[r][q] == q && cv[r][p].x == cv[r][q].x && cv[r][p].y == cv[r][q].y) first[r][q] = p;
+        }
+    const int n = b.xl * b.yl;
+    for (int r = 0; r < nr; ++r) {
+        for (int q = 0; q < nq; ++q) {
+            if (first[r][q] != q) continue;
+            int s = 0;
+            for (int i = lane; i < n; i += WAVE) {
+                const int dy = i / b.xl, dx = i - dy * b.xl;
+                s += abs(up_sample(J.up[r], UW, UH, b, cv[r][q], step, dy, dx) -
+                         (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+            }
+            s = wave_sum(s);
+            for (int p = q; p < nq; ++p)
+                if (first[r][p] == q && lane == 0) out[(level ? 4 : 16) * r + p] = s;
+        }
+    }
+    if (level == 1 && nr > 1) {
+        // bi-prediction SADs for the distinct candidate pairs
+        for (int q1 = 0; q1 < 4; ++q1) {
+            if (first[0][q1] != q1) continue;
+            for (int q2 = 0; q2 < 4; ++q2) {
+                if (first[1][q2] != q2) continue;
+                const UpStart s1 = up_start(b, cv[0][q1], step, UW, UH);
+                const UpStart s2 = up_start(b, cv[1][q2], step, UW, UH);
+                int s = 0;
+                for (int i = lane; i < n; i += WAVE) {
+                    const int dy = i / b.xl, dx = i - dy * b.xl;
+                    const int p = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
+                    const int16_t diff = (int16_t)((p << 1) - up_val(J.up[0], UW, UH, s1, dy, dx));
+                    const int16_t temp = (int16_t)(((int)diff - up_val(J.up[1], UW, UH, s2, dy, dx)) >> 1);
+                    s += abs((int)temp);
+                }
+                s = wave_sum(s);
+                if (lane == 0)
+                    for (int a = q1; a < 4; ++a)
+                        for (int c = q2; c < 4; ++c)
+                            if (first[0][a] == q1 && first[1][c] == q2) out[8 + a * 4 + c] = s;
+            }
+        }
+    }
+}
+
+// ModeDecider::DoME for one unit and reference (me_mode_decn.cpp:279-364;
+// BlockMatcher::FindBestMatchSubp, block_match.cpp:277-325) from tabulated
+// SADs.  g are the four guide vectors with their SADs; duplicates are skipped
+// as AddNewVlist does (they cannot win a strict-< comparison anyway).
+// Returns the index of the chosen guide (-1 when nothing was chosen).
+__device__ int unit_me(const JobDesc &J, int level, int xu, int yu, int r, const dme_mv g[4], const int gs[4],
+                       float lambda, dme_mv &out_mv, dme_cost &out_cost) {
     const dme_mv pred = mode_pred(J.vec[r], J.xnb, xu << (2 - level), yu << (2 - level));
     const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], J.W, J.H, xu, yu);
     dme_cost best;
@@ -761,20 +1110,32 @@ __device__ void unit_me(const JobDesc &J, int level, int xu, int yu, int r, cons
     dme_mv bmv;
     bmv.x = 0;
     bmv.y = 0;
+    int bq = -1;
     if (!b.empty()) {
         const int UW = 2 * J.W, UH = 2 * J.H;
-        int sd[4];
-        wave_up_sads(J.pic[0], J.W, J.up[r], UW, UH, b, J.step, cd, nc, sd);
-        for (int q = 0; q < nc; ++q)
-            try_candidate(J, J.up[r], UW, UH, b, cd[q], J.step, sd[q], var_up(pred, cd[q]), lambda, best, bmv);
+        for (int q = 0; q < 4; ++q) {
+            bool dup = false;
+            for (int p = 0; p < q; ++p) dup = dup || (g[p].x == g[q].x && g[p].y == g[q].y);
+            if (dup) continue;
+            const float mvc = var_up(pred, g[q]);
+            const float start = mvc * lambda;
+            const float total = cand_total(J, J.up[r], UW, UH, b, g[q], J.step, gs[q], start);
+            if (total < best.total) {
+                bmv = g[q];
+                best.total = total;
+                best.mvcost = mvc;
+                best.SAD = total - start;
+                bq = q;
+            }
+        }
     }
     out_mv = bmv;
     out_cost = best;
+    return bq;
 }
 
 // DoUnitDecn for levels 1 and 0 (me_mode_decn.cpp:368-441; no intra).
-__device__ float unit_decn(const JobDesc &J, int level, int xu, int yu, float loc_lambda, dme_mv v[2], dme_cost c[2],
-                           int &mode) {
+__device__ float unit_decn(const JobDesc &J, int level, float loc_lambda, dme_cost c[2], int bisad, int &mode) {
     const float lf = J.lf[level];
     const float mode_cost = 0.0f;
     mode = DME_REF1_ONLY;
@@ -787,10 +1148,9 @@ __device__ float unit_decn(const JobDesc &J, int level, int xu, int yu, float lo
             mode = DME_REF2_ONLY;
             min_cost = uc;
         }
-        const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], J.W, J.H, xu, yu);
         dme_cost bc;
         bc.mvcost = c[0].mvcost + c[1].mvcost;
-        bc.SAD = b.empty() ? 0.0f : (float)wave_bi_sad(J, b, v[0], v[1]);
+        bc.SAD = (float)bisad;
         bc.total = bc.SAD + loc_lambda * bc.mvcost;
         bc.total *= lf;
         uc = bc.total + mode_cost;
@@ -827,19 +1187,27 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
             int split = 2;
             // level 1
             dme_mv v1[4][2];
+            int q1[4][2];
             int m1[4];
             float sb1 = 0.0f;
             for (int u = 0; u < 4; ++u) {
                 const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
-                const float lam = J.lam1[(size_t)yu * X1 + xu];
+                const size_t ku = (size_t)yu * X1 + xu;
+                const float lam = J.lam1[ku];
+                const int *tab = J.m1tab + ku * 24;
                 dme_cost c[2];
                 for (int r = 0; r < nr; ++r) {
                     dme_mv g[4];
-                    for (int q = 0; q < 4; ++q)
+                    int gs[4];
+                    for (int q = 0; q < 4; ++q) {
                         g[q] = J.vec[r][(size_t)(2 * yu + (q >> 1)) * X2 + 2 * xu + (q & 1)];
-                    unit_me(J, 1, xu, yu, r, g, lam, v1[u][r], c[r]);
+                        gs[q] = tab[r * 4 + q];
+                    }
+                    q1[u][r] = unit_me(J, 1, xu, yu, r, g, gs, lam, v1[u][r], c[r]);
                 }
-                sb1 += unit_decn(J, 1, xu, yu, lam, v1[u], c, m1[u]);
+                int bis = 0;
+                if (nr > 1 && q1[u][0] >= 0 && q1[u][1] >= 0) bis = tab[8 + q1[u][0] * 4 + q1[u][1]];
+                sb1 += unit_decn(J, 1, lam, c, bis, m1[u]);
             }
             if (sb1 <= sbc) {
                 sbc = sb1;
@@ -853,18 +1221,34 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
                     if (nr > 1) J.vec[1][d] = v1[u][1];
                 }
             }
-            // level 0
+            // level 0: guides are the level-1 results; their SADs are those of
+            // the blocks they came from (unchosen units fall back to (0,0))
             {
                 const float lam = J.lam0[s];
+                const int *tab = J.m0tab + s * 32;
+                const Blk b0 = block_limits(J.blen[0], J.bsep[0], J.boff[0], J.W, J.H, xs, ys);
                 dme_mv v0[2];
                 dme_cost c[2];
                 for (int r = 0; r < nr; ++r) {
                     dme_mv g[4];
-                    for (int q = 0; q < 4; ++q) g[q] = v1[q][r];
-                    unit_me(J, 0, xs, ys, r, g, lam, v0[r], c[r]);
+                    int gs[4];
+                    for (int q = 0; q < 4; ++q) {
+                        g[q] = v1[q][r];
+                        const int bq = q1[q][r];
+                        if (bq >= 0) {
+                            gs[q] = tab[r * 16 + ((2 * (q >> 1) + (bq >> 1)) * 4 + 2 * (q & 1) + (bq & 1))];
+                        } else {
+                            int sd = 0;
+                            if (!b0.empty()) wave_up_sads(J.pic[0], J.W, J.up[r], 2 * J.W, 2 * J.H, b0, J.step, &g[q], 1, &sd);
+                            gs[q] = sd;
+                        }
+                    }
+                    unit_me(J, 0, xs, ys, r, g, gs, lam, v0[r], c[r]);
                 }
+                int bis = 0;
+                if (nr > 1 && !b0.empty()) bis = wave_bi_sad(J, b0, v0[0], v0[1]);
                 int m0;
-                const float sb0 = unit_decn(J, 0, xs, ys, lam, v0, c, m0);
+                const float sb0 = unit_decn(J, 0, lam, c, bis, m0);
                 if (sb0 <= sbc) {
                     sbc = sb0;
                     split = 0;
@@ -955,13 +1339,26 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     return hipGetLastError();
 }
 
-hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int threads, hipStream_t st) {
-    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), 0, st, jobs, tasks, level);
+hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows,
+                           hipStream_t st) {
+    if (ntasks <= 0) return hipSuccess;
+    const int threads = min(1024, ((max_rows + 63) / 64) * 64);
+    const size_t lds = sizeof(dme_mv) * 3 * (size_t)max_rows;
+    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, level);
     return hipGetLastError();
 }
 
-hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int threads, hipStream_t st) {
-    hipLaunchKernelGGL(k_subpel_wave, dim3(ntasks), dim3(threads), 0, st, jobs, tasks);
+hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_subpel_bulk, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st) {
+    if (ntasks <= 0) return hipSuccess;
+    const int threads = min(1024, ((max_rows + 63) / 64) * 64);
+    const size_t lds = (sizeof(dme_mv) * 4 + sizeof(int) * 49) * (size_t)max_rows;
+    hipLaunchKernelGGL(k_subpel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks);
     return hipGetLastError();
 }
 
@@ -975,6 +1372,12 @@ hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntas
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mode_l2, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mode_tables, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 

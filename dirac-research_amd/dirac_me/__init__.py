@@ -174,6 +174,22 @@ class Context:
         check(lib().dme_get_stats(self.h, C.byref(s)))
         return s
 
+    def kernel_times(self):
+        """{kernel name: (ms, launches)} of the last call (profiling on)."""
+        s = self.stats()
+        out = {}
+        for k in range(abi.DME_NUM_KERNELS):
+            name = lib().dme_kernel_name(k).decode()
+            if name and s.launches_by[k]:
+                out[name] = (s.kernel_ms_by[k], s.launches_by[k])
+        return out
+
+    def set_profiling(self, on):
+        check(lib().dme_set_profiling(self.h, 1 if on else 0))
+
+    def invalidate(self, pic_num=-1):
+        check(lib().dme_invalidate_caches(self.h, pic_num))
+
 
 def lambda_map_host(vectors1, vectors2, lam, nrefs):
     v1 = np.ascontiguousarray(vectors1, MV_DTYPE)

@@ -68,9 +68,13 @@ class dme_job(C.Structure):
     _fields_ = [("pic_num", C.c_int32), ("nrefs", C.c_int32), ("refs", C.c_int32 * 2), ("params", dme_params)]
 
 
+DME_NUM_KERNELS = 16
+
+
 class dme_stats(C.Structure):
     _fields_ = [("sad_terms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("reserved", C.c_int32 * 7), ("kernel_ms_by", C.c_double * DME_NUM_KERNELS),
+                ("launches_by", C.c_int32 * DME_NUM_KERNELS)]
 
 
 class dme_synth_object(C.Structure):
@@ -106,6 +110,9 @@ SIGNATURES = {
     "dme_get_plane": ([_P, _I, _I, _I, _P, C.c_size_t], _I),
     "dme_synchronize": ([_P], _I),
     "dme_get_stats": ([_P, C.POINTER(dme_stats)], _I),
+    "dme_invalidate_caches": ([_P, _I], _I),
+    "dme_set_profiling": ([_P, _I], _I),
+    "dme_kernel_name": ([_I], C.c_char_p),
     "dme_synth_texture": ([C.c_uint64, _I, _I, _I, _P], _I),
     "dme_synth_warp": ([_P, _I, _I, C.POINTER(dme_synth_motion), _P], _I),
     "dme_synth_chroma": ([_P, _I, _I, C.c_uint64, _P, _P], _I),

@@ -225,14 +225,27 @@ int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level,
 /* Waits for all work queued on the context's stream. */
 int dme_synchronize(dme_ctx *ctx);
 
-/* Counters of the last completed pipeline call on this context. */
+/* Marks the cached pyramid levels and up-converted plane of pic_num stale
+ * (its samples changed in place, e.g. a recycled device frame buffer); they
+ * are rebuilt into the same buffers on next use.  pic_num < 0: all pictures. */
+int dme_invalidate_caches(dme_ctx *ctx, int pic_num);
+
+/* Counters of the last completed pipeline call on this context.  Per-kernel
+ * times are recorded only while profiling is enabled (events around every
+ * launch on the context's stream).                                          */
+#define DME_NUM_KERNELS 16
 typedef struct dme_stats {
     double  sad_terms;        /* algorithmic SAD terms of the last call       */
-    double  kernel_ms;        /* summed kernel time of the last call (events) */
+    double  kernel_ms;        /* stream time of the last call (events)        */
     int32_t launches;         /* kernel launches of the last call             */
     int32_t reserved[7];
+    double  kernel_ms_by[DME_NUM_KERNELS];  /* summed time per kernel kind   */
+    int32_t launches_by[DME_NUM_KERNELS];   /* launches per kernel kind      */
 } dme_stats;
 int dme_get_stats(dme_ctx *ctx, dme_stats *out);
+int dme_set_profiling(dme_ctx *ctx, int enable);
+/* Name of kernel kind k (0 <= k < DME_NUM_KERNELS), "" if unused. */
+const char *dme_kernel_name(int k);
 
 /* ---- synthetic input generator (deterministic, own PRNG) --------------- */
 

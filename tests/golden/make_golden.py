@@ -1,0 +1,106 @@
+"""Writes the golden fixtures under tests/golden/ from the reference encoder's
+own ME code, compiled from its sources by oracle/Makefile (target `ref`).
+
+Run in a container that has the reference tree:  python tests/golden/make_golden.py
+
+Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
+seed by the deterministic generator in libdme.so), the parameters, and the
+reference's outputs: full MEData arrays for CIF pictures, per-array SHA-256
+digests plus the small arrays for a 1080p picture, pyramid / up-converted
+planes of small random pictures, and lambda maps of random vector fields.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "dirac-research_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle_py as O  # noqa: E402
+from dirac_me import DME_FP_C, DME_FP_MMX, make_params, synth  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def inputs_digest(d):
+    return sha(np.concatenate([d["cur"].ravel()] + [r.ravel() for r in d["refs"]] + [d["u"].ravel(), d["v"].ravel()]))
+
+
+# (name, config function, seed, precision, depth(-1 = reference formula), lambda, fp, variant, full arrays)
+CASES = [
+    ("cif_s0_d3_q", "config0", 0, 2, 3, 10.0, DME_FP_MMX, "mmx", True),
+    ("cif_s1_d3_q", "config0", 1, 2, 3, 10.0, DME_FP_MMX, "mmx", True),
+    ("cif_s2_dref_q", "config0", 2, 2, -1, 10.0, DME_FP_MMX, "mmx", True),
+    ("cif_s0_d3_h", "config0", 0, 1, 3, 10.0, DME_FP_MMX, "mmx", True),
+    ("cif_s0_d3_p", "config0", 0, 0, 3, 10.0, DME_FP_MMX, "mmx", True),
+    ("cif_s3_d3_q_l10.7", "config0", 3, 2, 3, 10.7, DME_FP_MMX, "mmx", True),
+    ("cif_s3_d3_q_l10.7_c", "config0", 3, 2, 3, 10.7, DME_FP_C, "c", True),
+    ("hd_s0_d4_q", "config1", 0, 2, 4, 10.0, DME_FP_MMX, "mmx", False),
+    ("hd_s0_d4_q_l10.7_c", "config1", 0, 2, 4, 10.7, DME_FP_C, "c", False),
+]
+
+
+def main():
+    out_dir = HERE
+    index = {}
+    for name, cfg, seed, prec, depth, lam, fp, variant, full in CASES:
+        d = getattr(synth, cfg)(seed)
+        p = make_params(*d["block"], precision=prec, depth=depth, lam=lam, fp=fp)
+        res, cap, g = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], capture=depth > 0, variant=variant)
+        arrs = res.arrays(d["nrefs"])
+        meta = dict(config=cfg, seed=seed, precision=prec, depth=depth, lam=lam, fp=fp, variant=variant,
+                    nrefs=d["nrefs"], inputs_sha256=inputs_digest(d), sha256={k: sha(v) for k, v in arrs.items()})
+        save = {}
+        if full:
+            save.update(arrs)
+            if cap is not None:
+                for r in range(d["nrefs"]):
+                    for l in range(1, g.depth + 1):
+                        save["level%d_vectors%d" % (l, r + 1)] = cap.level_vectors[r][l]
+        else:
+            for k in ("mode", "sb_split", "sb_costs"):
+                save[k] = arrs[k]
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=json.dumps(meta), **save)
+        index[name] = meta
+        print("wrote", name)
+
+    # pyramid and up-converted planes of small random pictures
+    rng = np.random.default_rng(20261203)
+    planes = {}
+    for i, (w, h) in enumerate(((64, 48), (50, 38), (96, 70))):
+        a = rng.integers(-128, 128, size=(h, w)).astype(np.int16)
+        planes["in%d" % i] = a
+        planes["down%d" % i] = O.downconvert(O.ref(), a, w // 2, h // 2, "refh_downconvert")
+        planes["up%d" % i] = O.upconvert(O.ref(), a, "refh_upconvert")
+    np.savez_compressed(os.path.join(out_dir, "planes.npz"), **planes)
+    print("wrote planes")
+
+    # lambda maps of random vector fields, one and two references
+    lam = {}
+    for i, (X, Y) in enumerate(((44, 36), (160, 92))):
+        for r in range(2):
+            v = np.zeros((Y, X), O.abi.MV_DTYPE)
+            v["x"] = rng.integers(-6, 7, size=(Y, X))
+            v["y"] = rng.integers(-6, 7, size=(Y, X))
+            # a few smooth regions and outliers
+            v["x"][: Y // 2, : X // 2] = 3
+            v["y"][Y // 3:, X // 3:] = -2
+            lam["v%d_%d" % (i, r)] = v
+        for nrefs in (1, 2):
+            lam["lam%d_n%d" % (i, nrefs)] = O.lambda_map(O.ref(), "refh_lambda_map", lam["v%d_0" % i],
+                                                         lam["v%d_1" % i], 10.7, nrefs)
+    np.savez_compressed(os.path.join(out_dir, "lambda.npz"), **lam)
+    print("wrote lambda")
+    with open(os.path.join(out_dir, "index.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

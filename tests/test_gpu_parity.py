@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, bit-exact
+on every MEData array, at the workload sizes and at the edges the reference
+handles (overhanging block grids, pictures that do not move, one and two
+references, every MV precision, both float semantics)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import dirac_me
+from dirac_me import DME_FP_C, DME_FP_MMX, make_params, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(ctx, d):
+    nums = [0, 2][: len(d["refs"])]
+    for n, r in zip(nums, d["refs"]):
+        ctx.upload(n, r)
+    ctx.upload(1, d["cur"], d["u"], d["v"])
+    return nums
+
+
+def _gpu(ctx, p, d):
+    ctx.set_params(p)
+    nums = _upload(ctx, d)
+    H, W = d["cur"].shape
+    out = dirac_me.MEData(dirac_me.geometry(p, W, H))
+    ctx.motion_estimate(1, nums, out)
+    return out
+
+
+def _assert_same(want, got, nrefs, tag=""):
+    for k, v in want.arrays(nrefs).items():
+        g = got.arrays(nrefs)[k]
+        if v.tobytes() != g.tobytes():
+            n = int((v.view(np.uint8) != g.view(np.uint8)).sum())
+            raise AssertionError("%s %s differs (%d bytes)" % (tag, k, n))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("prec", [2, 1, 0])
+def test_cif_p_picture(ctx, seed, prec):
+    d = synth.config0(seed)
+    p = make_params(*d["block"], precision=prec, depth=d["depth"], lam=10.0)
+    print("case cif seed", seed, "prec", prec, flush=True)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 1)
+
+
+@pytest.mark.parametrize("lam,fp", [(10.7, DME_FP_MMX), (10.7, DME_FP_C), (3.3, DME_FP_C)])
+def test_cif_float_semantics(ctx, lam, fp):
+    d = synth.config0(3)
+    p = make_params(*d["block"], precision=2, depth=3, lam=lam, fp=fp)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 1)
+
+
+def test_cif_reference_depth(ctx):
+    d = synth.config0(2)
+    p = make_params(*d["block"], precision=2, depth=-1, lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 1)
+
+
+@pytest.mark.parametrize("lam,fp", [(10.0, DME_FP_MMX), (10.7, DME_FP_C)])
+def test_1080p_b_picture(ctx, lam, fp):
+    d = synth.config1(0)
+    p = make_params(*d["block"], precision=2, depth=4, lam=lam, fp=fp)
+    print("case 1080p lam", lam, "fp", fp, flush=True)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 2)
+
+
+@pytest.mark.parametrize("size,nrefs,seed", [("720p", 2, 21), ("720p", 1, 22), ("cif", 2, 23)])
+def test_other_sizes(ctx, size, nrefs, seed):
+    d = synth.sized(seed, size, nrefs)
+    p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), nrefs)
+
+
+def test_overhanging_grid_and_odd_blocks(ctx):
+    # 360x240 with 10x14 blocks: rationalised to 12/16 lengths, block grid
+    # overhangs on both axes, blocks entirely outside the picture
+    d = synth.picture_set(31, 360, 240, 2, 6.0, 3, 10.0)
+    p = make_params(10, 14, 7, 9, precision=2, depth=2, lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 2)
+
+
+def test_static_picture(ctx):
+    cur = synth.texture(40, 352, 288, 3)
+    u, v = synth.chroma(cur, 1)
+    d = dict(cur=cur, u=u, v=v, refs=[cur.copy(), cur.copy()])
+    p = make_params(12, 12, 8, 8, precision=2, depth=3, lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    got = _gpu(ctx, p, d)
+    _assert_same(want, got, 2)
+
+
+def test_stagewise_entry_points(ctx):
+    d = synth.config0(7)
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    g = dirac_me.geometry(p, 352, 288)
+    ctx.set_params(p)
+    nums = _upload(ctx, d)
+    for stop, call in ((O.STOP_PEL, lambda o: ctx.pixel_search(1, nums, o)),
+                       (O.STOP_LAMBDA, lambda o: ctx.set_lambda_map(1, None, o)),
+                       (O.STOP_SUBPEL, lambda o: ctx.subpel(1, o)),
+                       (O.STOP_MODE, lambda o: ctx.mode_decide(1, o))):
+        got = dirac_me.MEData(g)
+        call(got)
+        want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], stop=stop)
+        keys = ["vectors1", "pred_costs1"] + (["lambda_map"] if stop >= O.STOP_LAMBDA else []) + \
+            (list(want.arrays(1)) if stop == O.STOP_MODE else [])
+        for k in keys:
+            assert want.arrays(1)[k].tobytes() == got.arrays(1)[k].tobytes(), (stop, k)
+
+
+def test_lambda_override(ctx):
+    d = synth.config0(8)
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    g = dirac_me.geometry(p, 352, 288)
+    lam = np.random.default_rng(3).choice(np.float32([0.0, 1.7, 10.0, 3.25]), size=(g.ynum_blocks, g.xnum_blocks))
+    ctx.set_params(p)
+    nums = _upload(ctx, d)
+    got = dirac_me.MEData(g)
+    ctx.pixel_search(1, nums)
+    ctx.set_lambda_map(1, lam)
+    ctx.subpel(1)
+    ctx.mode_decide(1, got)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], lambda_override=lam)
+    _assert_same(want, got, 1)
+
+
+def test_pyramid_and_upconverted_planes(ctx):
+    d = synth.config1(1)
+    p = make_params(*d["block"], precision=2, depth=4, lam=10.0)
+    _gpu(ctx, p, d)
+    g = dirac_me.geometry(p, 1920, 1080)
+    _, cap, _ = O.run_oracle(p, d["cur"], d["refs"], stop=O.STOP_SUBPEL, capture=True)
+    for l in range(1, 5):
+        got = ctx.get_plane(1, 0, l, (g.level_height[l], g.level_width[l]))
+        assert np.array_equal(got, cap.pyramid[l]), l
+    assert np.array_equal(ctx.get_plane(0, 1, 0, (2160, 3840)), cap.ref_up[0])
+    assert np.array_equal(ctx.get_plane(2, 1, 0, (2160, 3840)), cap.ref_up[1])
+
+
+def test_batch_of_mixed_pictures(ctx):
+    # one batched call over pictures of different sizes, reference counts and
+    # block parameters; each must equal its own single-picture oracle result
+    cases = [("cif", 1, 51), ("720p", 2, 52), ("cif", 2, 53), ("720p", 1, 54)]
+    jobs, outs, wants = [], [], []
+    for i, (size, nrefs, seed) in enumerate(cases):
+        d = synth.sized(seed, size, nrefs)
+        p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+        base = 10 * (i + 1)
+        nums = [base, base + 2][:nrefs]
+        for n, r in zip(nums, d["refs"]):
+            ctx.upload(n, r)
+        ctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        H, W = d["cur"].shape
+        jobs.append((base + 1, nums, p))
+        outs.append(dirac_me.MEData(dirac_me.geometry(p, W, H)))
+        wants.append((O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])[0], nrefs))
+    ctx.run_batch(jobs, outs)
+    for (w, nrefs), o in zip(wants, outs):
+        _assert_same(w, o, nrefs, "batch")
+
+
+def test_errors_are_reported(ctx):
+    p = make_params(12, 12, 8, 8, precision=2, depth=3)
+    ctx.set_params(p)
+    with pytest.raises(dirac_me.DmeError):
+        ctx.motion_estimate(999, [0])  # unregistered picture
+    with pytest.raises(dirac_me.DmeError):
+        ctx.subpel(12345)  # no pel search done

@@ -1,0 +1,230 @@
+"""Pins of the CPU oracle (oracle/cpu_ref.cpp).
+
+1. Against the golden fixtures written from the reference encoder's own code
+   (tests/golden/make_golden.py): bit-exact, every MEData array.
+2. Against the reference build itself, live, where oracle/_ref exists.
+3. Against what the mathematics fixes, independently of the oracle's code:
+   constant pictures through the filters, brute-force SADs of tiny blocks,
+   exact recovery of integer translations, zero motion for identical
+   pictures.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from dirac_me import DME_FP_C, DME_FP_MMX, MV_DTYPE, make_params, synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _fixture(name):
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    return json.loads(str(z["meta"])), z
+
+
+def _inputs(meta):
+    d = getattr(synth, meta["config"])(meta["seed"])
+    digest = _sha(np.concatenate([d["cur"].ravel()] + [r.ravel() for r in d["refs"]] + [d["u"].ravel(), d["v"].ravel()]))
+    assert digest == meta["inputs_sha256"], "synthetic generator changed: regenerate the fixtures"
+    return d
+
+
+def _params(meta, d):
+    return make_params(*d["block"], precision=meta["precision"], depth=meta["depth"], lam=meta["lam"], fp=meta["fp"])
+
+
+CIF_CASES = ["cif_s0_d3_q", "cif_s1_d3_q", "cif_s2_dref_q", "cif_s0_d3_h", "cif_s0_d3_p", "cif_s3_d3_q_l10.7",
+             "cif_s3_d3_q_l10.7_c"]
+
+
+@pytest.mark.parametrize("name", CIF_CASES)
+def test_oracle_matches_reference_fixture_cif(name):
+    meta, z = _fixture(name)
+    d = _inputs(meta)
+    p = _params(meta, d)
+    got, cap, g = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], capture=meta["depth"] > 0)
+    for k, v in got.arrays(meta["nrefs"]).items():
+        assert np.array_equal(v.view(np.uint8), z[k].view(np.uint8)), k
+        assert _sha(v) == meta["sha256"][k], k
+    if meta["depth"] > 0:
+        for r in range(meta["nrefs"]):
+            for l in range(1, g.depth + 1):
+                assert np.array_equal(cap.level_vectors[r][l], z["level%d_vectors%d" % (l, r + 1)]), (r, l)
+
+
+@pytest.mark.parametrize("name", ["hd_s0_d4_q", "hd_s0_d4_q_l10.7_c"])
+def test_oracle_matches_reference_fixture_1080p(name):
+    meta, z = _fixture(name)
+    d = _inputs(meta)
+    got, _, _ = O.run_oracle(_params(meta, d), d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in got.arrays(meta["nrefs"]).items():
+        assert _sha(v) == meta["sha256"][k], k
+
+
+def test_oracle_planes_match_reference_fixture():
+    z = np.load(os.path.join(GOLD, "planes.npz"))
+    for i in range(3):
+        a = z["in%d" % i]
+        h, w = a.shape
+        assert np.array_equal(O.downconvert(O.oracle(), a, w // 2, h // 2, "oracle_downconvert"), z["down%d" % i])
+        assert np.array_equal(O.upconvert(O.oracle(), a, "oracle_upconvert"), z["up%d" % i])
+
+
+def test_oracle_lambda_matches_reference_fixture():
+    z = np.load(os.path.join(GOLD, "lambda.npz"))
+    for i in range(2):
+        for n in (1, 2):
+            got = O.lambda_map(O.oracle(), "oracle_lambda_map", z["v%d_0" % i], z["v%d_1" % i], 10.7, n)
+            assert np.array_equal(got.view(np.uint32), z["lam%d_n%d" % (i, n)].view(np.uint32))
+
+
+# ---- live comparison with the reference build -----------------------------
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [5, 6, 7])
+@pytest.mark.parametrize("prec,depth", [(2, 3), (2, -1), (1, 2), (0, 3)])
+def test_oracle_matches_reference_live_cif(seed, prec, depth):
+    d = synth.config0(seed)
+    p = make_params(*d["block"], precision=prec, depth=depth, lam=10.0)
+    want, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"])
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in want.arrays(1).items():
+        assert np.array_equal(v.view(np.uint8), got.arrays(1)[k].view(np.uint8)), k
+
+
+@needs_ref
+@pytest.mark.parametrize("size,nrefs,seed", [("720p", 2, 11), ("720p", 1, 12)])
+def test_oracle_matches_reference_live_sizes(size, nrefs, seed):
+    d = synth.sized(seed, size, nrefs)
+    p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    want, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"])
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in want.arrays(nrefs).items():
+        assert np.array_equal(v.view(np.uint8), got.arrays(nrefs)[k].view(np.uint8)), k
+
+
+@pytest.mark.skipif(not O.ref_available("c"), reason="plain-C reference build not present")
+@pytest.mark.parametrize("lam", [10.0, 7.3])
+def test_oracle_c_semantics_match_plain_c_reference(lam):
+    d = synth.config0(9)
+    p = make_params(*d["block"], precision=2, depth=3, lam=lam, fp=DME_FP_C)
+    want, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], variant="c")
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in want.arrays(1).items():
+        assert np.array_equal(v.view(np.uint8), got.arrays(1)[k].view(np.uint8)), k
+
+
+@needs_ref
+def test_reference_drivers_agree_at_reference_depth():
+    # the harness's explicit-depth driver must equal PixelMatcher::DoSearch
+    d = synth.config0(4)
+    g = O.geometry(make_params(*d["block"]), 352, 288)
+    a, _, _ = O.run_ref(make_params(*d["block"], depth=-1), d["cur"], d["refs"], d["u"], d["v"])
+    b, _, _ = O.run_ref(make_params(*d["block"], depth=g.depth), d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in a.arrays(1).items():
+        assert np.array_equal(v.view(np.uint8), b.arrays(1)[k].view(np.uint8)), k
+
+
+# ---- closed forms and invariants -------------------------------------------
+
+@pytest.mark.parametrize("c", [-128, -37, 0, 5, 127])
+def test_filters_preserve_constants(c):
+    # the half-band taps sum to 256 and the up-sampling taps to 32: a flat
+    # picture stays flat at every pyramid level and half-pel position
+    a = np.full((40, 56), c, np.int16)
+    assert np.all(O.downconvert(O.oracle(), a, 28, 20, "oracle_downconvert") == c)
+    assert np.all(O.upconvert(O.oracle(), a, "oracle_upconvert") == c)
+
+
+def test_upconvert_keeps_even_samples_and_clips():
+    rng = np.random.default_rng(1)
+    a = rng.integers(-128, 128, size=(30, 34)).astype(np.int16)
+    u = O.upconvert(O.oracle(), a, "oracle_upconvert")
+    assert np.array_equal(u[::2, ::2], a)
+    assert u.min() >= -128 and u.max() <= 127
+
+
+def test_downconvert_of_a_linear_ramp_is_the_ramp_at_half_rate():
+    # interior samples of a horizontal ramp x are reproduced exactly: the
+    # symmetric filter at a half-sample phase returns the midpoint 2cx+0.5
+    x = np.arange(64)
+    a = np.tile((x - 32).astype(np.int16), (32, 1))
+    d = O.downconvert(O.oracle(), a, 32, 16, "oracle_downconvert")
+    inner = d[:, 6:26]
+    want = np.floor((2 * np.arange(6, 26) + 0.5 - 32) + 0.5).astype(np.int16)
+    assert np.array_equal(inner, np.tile(want, (16, 1)))
+
+
+def _pel_only(cur, ref, block=(12, 12, 8, 8), depth=2):
+    p = make_params(*block, precision=2, depth=depth, lam=10.0)
+    out, _, g = O.run_oracle(p, cur, [ref], stop=O.STOP_PEL)
+    return out, g, p
+
+
+def test_identical_pictures_give_zero_vectors_and_zero_sad():
+    cur = synth.texture(3, 96, 80, 0)
+    out, g, _ = _pel_only(cur, cur.copy())
+    assert np.all(out.vectors[0]["x"] == 0) and np.all(out.vectors[0]["y"] == 0)
+    inside = out.pred_costs[0]["SAD"][out.pred_costs[0]["SAD"] < 1e7]
+    assert np.all(inside == 0)
+
+
+@pytest.mark.parametrize("dx,dy", [(3, -2), (-5, 4), (0, 7)])
+def test_integer_translation_is_recovered_exactly(dx, dy):
+    # ref(x) = cur(x - d): blocks away from the borders find mv = d with SAD 0
+    big = synth.texture(11, 160, 144, 0)
+    cur = big[16:16 + 112, 16:16 + 128]
+    ref = big[16 - dy:16 - dy + 112, 16 - dx:16 - dx + 128]
+    out, g, p = _pel_only(np.ascontiguousarray(cur), np.ascontiguousarray(ref))
+    v = out.vectors[0]
+    s = out.pred_costs[0]["SAD"]
+    inner = (slice(3, 10), slice(3, 12))
+    assert np.all(v["x"][inner] == dx) and np.all(v["y"][inner] == dy)
+    assert np.all(s[inner] == 0)
+
+
+def test_pel_sad_brute_force_on_tiny_blocks():
+    # the pel-search SAD of the chosen vector equals a direct sum over the
+    # block with clamp-to-edge reference reads (me_utils.cpp:138-282)
+    rng = np.random.default_rng(5)
+    cur = rng.integers(-128, 128, size=(48, 64)).astype(np.int16)
+    ref = rng.integers(-128, 128, size=(48, 64)).astype(np.int16)
+    out, g, p = _pel_only(cur, ref, depth=1)
+    xbsep = ybsep = 8
+    off = 2
+    for yb in range(g.ynum_blocks):
+        for xb in range(g.xnum_blocks):
+            x0, y0 = xb * xbsep - off, yb * ybsep - off
+            xs, ys = max(x0, 0), max(y0, 0)
+            xe, ye = min(x0 + 12, 64), min(y0 + 12, 48)
+            if xe <= xs or ye <= ys:
+                assert out.pred_costs[0]["SAD"][yb, xb] == 1e7
+                continue
+            mv = out.vectors[0][yb, xb]
+            yy = np.clip(np.arange(ys, ye) + mv["y"], 0, 47)
+            xx = np.clip(np.arange(xs, xe) + mv["x"], 0, 63)
+            direct = np.abs(cur[ys:ye, xs:xe].astype(int) - ref[np.ix_(yy, xx)].astype(int)).sum()
+            assert out.pred_costs[0]["SAD"][yb, xb] == direct
+
+
+def test_sad_term_count_is_reported():
+    d = synth.config0(0)
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    out, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    assert 1e6 < out.sad_terms < 1e9
+
+
+def test_mv_dtype_layout():
+    assert MV_DTYPE.itemsize == 8
+    assert DME_FP_MMX == 0
