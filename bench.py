@@ -1,0 +1,184 @@
+"""Benchmark: plain GPU motion estimation of 1080p B pictures with two
+references (16x16 blocks at separation 12, 4-level pyramid, quarter pel,
+lambda 10), batched, on one MI355X.
+
+A step is one pass of the whole live ME sequence (pyramids and up-converted
+references rebuilt, pel search, lambda map, sub-pel refinement, mode
+decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
+one JSON line; the algorithmic byte and SAD-term counts behind the roofline
+figures are in _geometry_terms().
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dirac-research_amd"))
+
+METRIC = "1080p 2-ref ME frames/s: plain GPU vs learned-proposal at ≤1% mean cost gap"
+HBM_PEAK_GBS = 8000.0
+INT_LANE_OPS_PEAK = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes per clock x 2.4 GHz
+
+
+def _geometry_terms(geom, nrefs):
+    """Algorithmic HBM bytes and SAD terms per picture for the main kernels."""
+    import numpy as np  # noqa: F401
+
+    xl, yl = geom.luma_blen[2][0], geom.luma_blen[2][1]
+    B = geom.xnum_blocks * geom.ynum_blocks
+    out = {}
+    # pel search bulk: picture block + two reference regions read, two SAD
+    # tables and the zero/guide winner written, per block and level
+    b_bytes = t_terms = w_bytes = 0.0
+    for l in range(geom.depth + 1):
+        nb = geom.level_xblocks[l] * geom.level_yblocks[l]
+        r = min(l + 1, 5)
+        R = r + 1
+        TS = (2 * R + 1) ** 2
+        ntab = 2 if l < geom.depth else 1
+        b_bytes += nb * nrefs * (2 * xl * yl + ntab * 2 * (xl + 2 * R) * (yl + 2 * R) + ntab * TS * 4 + 12)
+        t_terms += nb * nrefs * ntab * TS * xl * yl
+        w_bytes += nb * nrefs * ((2 * r + 1) ** 2 * 4 + 12 + 8 + 20)
+    out["k_pel_bulk"] = (b_bytes, t_terms)
+    out["k_pel_wave"] = (w_bytes, 0.0)
+    # sub-pel bulk: picture block + up-converted region read, 49 SADs written
+    out["k_subpel_bulk"] = (B * nrefs * (2 * xl * yl + 2 * (2 * xl + 3) * (2 * yl + 3) + 49 * 4), B * nrefs * 49 * xl * yl)
+    out["k_subpel_wave"] = (B * nrefs * (49 * 4 + 4 + 8 + 12 + 20), 0.0)
+    # mode decision: level-1 and level-0 unit tables, superblock wavefront
+    l1 = geom.luma_blen[1][0] * geom.luma_blen[1][1]
+    l0 = geom.luma_blen[0][0] * geom.luma_blen[0][1]
+    nu1, nsb = B // 4, B // 16
+    out["k_mode_tables"] = (nu1 * (2 * l1 + nrefs * 4 * 2 * l1 + 24 * 4) + nsb * (2 * l0 + nrefs * 16 * 2 * l0 + 32 * 4),
+                            nu1 * (nrefs * 4 + 16) * l1 + nsb * nrefs * 16 * l0)
+    out["k_mode_wave"] = (nsb * (16 * 4 + 4 * 24 * 4 + 32 * 4 + 2 * l0 * 3 + 16 * 20), nsb * l0)
+    out["k_upconvert"] = (nrefs * geom.width * geom.height * 2 * 5, 0.0)
+    out["k_downconvert"] = (sum((1 + nrefs) * (geom.level_width[l] * geom.level_height[l] * 2 * 5)
+                                for l in range(1, geom.depth + 1)), 0.0)
+    return out
+
+
+def cpu_baseline(frames, params, budget_s):
+    """Reference encoder ME (oracle/_ref, single thread) or the oracle, on a
+    bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    kind = "reference" if O.ref_available() else "port"
+    run = O.run_ref if kind == "reference" else O.run_oracle
+    n = 0
+    t0 = time.time()
+    while True:
+        d = frames[n % len(frames)]
+        run(params, d["cur"], d["refs"], d["u"], d["v"])
+        n += 1
+        if time.time() - t0 >= budget_s or n >= 200:
+            break
+    dt = time.time() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n / dt, "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": "%d 1080p B pictures (2 refs) of the same synthetic workload, %s %s, %.1f s on %s" % (
+                n, "reference encoder ME (MMX build, compiled from its sources)" if kind == "reference" else "oracle port",
+                "single thread", dt, cpu)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the motion-estimation path runs one picture batch per GPU and "
+              "there is no multi-GPU launcher (pictures would shard by picture with no exchange)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import dirac_me
+    from dirac_me import synth
+
+    B = args.batch
+    frames = [synth.config1(s) for s in range(min(B, 8))]
+    d0 = frames[0]
+    params = dirac_me.make_params(*d0["block"], precision=2, depth=4, lam=10.0)
+    geom = dirac_me.geometry(params, 1920, 1080)
+    ctx = dirac_me.Context(0)
+    ctx.set_params(params)
+    jobs = []
+    for j in range(B):
+        d = frames[j % len(frames)]
+        base = 3 * j
+        ctx.upload(base, d["refs"][0])
+        ctx.upload(base + 2, d["refs"][1])
+        ctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        jobs.append((base + 1, [base, base + 2], params))
+    print("bench: %d pictures uploaded; warmup %d, steps %d" % (B, args.warmup, args.steps), file=sys.stderr, flush=True)
+
+    def step():
+        ctx.invalidate(-1)
+        ctx.run_batch(jobs)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    fps = B * args.steps / dt
+    print("bench: timed %.3f s" % dt, file=sys.stderr, flush=True)
+
+    # one more step with per-kernel events, for the roofline figures
+    ctx.set_profiling(True)
+    step()
+    ctx.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    dom = max(kt, key=lambda k: kt[k][0])
+    terms = _geometry_terms(geom, 2)
+    alg_bytes, alg_terms = terms.get(dom, (0.0, 0.0))
+    ms, nl = kt[dom]
+    per_launch_ms = ms / nl
+    achieved = alg_bytes * B / nl / (per_launch_ms * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": per_launch_ms,
+            "launches_per_step": nl}
+    valu = {}
+    for k in ("k_pel_bulk", "k_subpel_bulk", "k_mode_tables"):
+        if k in kt:
+            ops = 3.0 * terms[k][1] * B
+            valu[k] = {"sad_terms_per_s": terms[k][1] * B / (kt[k][0] * 1e-3),
+                       "int_ops_frac": ops / (kt[k][0] * 1e-3) / INT_LANE_OPS_PEAK}
+    out = {
+        "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "plain GPU ME, 1920x1080 4:2:0 B pictures with 2 refs, blocks 16/12, 4-level "
+                               "pyramid, quarter pel, lambda 10; batch of %d pictures per step" % B,
+                   "batch": B, "width": 1920, "height": 1080, "refs": 2, "depth": 4, "precision": "quarter"},
+        "block_decisions_per_s": fps * geom.xnum_blocks * geom.ynum_blocks * 2,
+        "roofline": roof, "valu": valu,
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(frames, params, args.cpu_budget)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
