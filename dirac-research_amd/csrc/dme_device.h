@@ -7,6 +7,15 @@
 
 #include "../../include/dme.h"
 
+// Device code reads every descriptor pointer as a global-address-space
+// pointer, so its loads are global (not flat) loads, which need no ordering
+// against LDS accesses.  Same 64-bit representation on host and device.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DME_G __attribute__((address_space(1)))
+#else
+#define DME_G
+#endif
+
 namespace dme {
 
 // Everything one picture's motion estimation needs on the device.  Built on
@@ -24,35 +33,44 @@ struct JobDesc {
     int cblen[2], cbsep[2], cboff[2];
     float lf[3];
     // planes
-    const int16_t *pic[DME_MAX_DEPTH + 1];      // [0] = Y of the picture
-    const int16_t *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
-    const int16_t *up[2];                       // 2W x 2H up-converted refs
-    const int16_t *cu, *cv;                     // chroma of the picture
+    const int16_t DME_G *pic[DME_MAX_DEPTH + 1];      // [0] = Y of the picture
+    const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
+    const int16_t DME_G *up[2];                       // 2W x 2H up-converted refs
+    const int16_t DME_G *cu, *cv;                     // chroma of the picture
     // pel search, per reference and level ([0] aliases the MEData arrays)
-    dme_mv *lvec[2][DME_MAX_DEPTH + 1];
-    dme_cost *lpc[2][DME_MAX_DEPTH + 1];
-    int *bsad[2];                               // zero/guide-window best SAD
-    dme_mv *bmv[2];                             // ... and its vector
-    int *ptab[2];                               // pel SAD tables (see k_pel_bulk)
-    int *qtab[2];                               // sub-pel SAD tables (k_subpel_bulk)
-    int *m1tab;                                 // level-1 unit SAD tables (k_mode_tables)
-    int *m0tab;                                 // level-0 unit SAD tables
+    dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];
+    dme_cost DME_G *lpc[2][DME_MAX_DEPTH + 1];
+    int DME_G *bsad[2];                               // zero/guide-window best SAD
+    dme_mv DME_G *bmv[2];                             // ... and its vector
+    int DME_G *ptab[2];                               // pel SAD tables (see k_pel_bulk)
+    int DME_G *qtab[2];                               // sub-pel SAD tables (k_subpel_bulk)
+    int DME_G *m1tab;                                 // level-1 unit SAD tables (k_mode_tables)
+    int DME_G *m0tab;                                 // level-0 unit SAD tables
     // MEData of the picture
-    dme_mv *vec[2];
-    dme_cost *pc[2];
-    int32_t *mode;
-    float *intra;
-    dme_cost *bi;
-    int16_t *dc[3];
-    int32_t *split;
-    float *sbc;
-    float *lam;
+    dme_mv DME_G *vec[2];
+    dme_cost DME_G *pc[2];
+    int32_t DME_G *mode;
+    float DME_G *intra;
+    dme_cost DME_G *bi;
+    int16_t DME_G *dc[3];
+    int32_t DME_G *split;
+    float DME_G *sbc;
+    float DME_G *lam;
     // mode decision temporaries (levels 1 and 0)
-    float *lam1, *lam0;
-    dme_mv *v1[2], *v0[2];
-    dme_cost *c1[2], *c0[2];
-    float *ucost;                               // level-2 unit costs
-    int16_t *dct[3];                            // block DCs before smoothing
+    float DME_G *lam1, *lam0;
+    dme_mv DME_G *v1[2], *v0[2];
+    dme_cost DME_G *c1[2], *c0[2];
+    float DME_G *ucost;                               // level-2 unit costs
+    int16_t DME_G *dct[3];                            // block DCs before smoothing
+};
+
+// A plane conversion of a batched pyramid / up-conversion launch: `in` of
+// width in_w, `out` of ow x oh samples (the source size for up-conversion),
+// covering tiles [offset, offset + tiles) of the launch.
+struct PlaneTask {
+    const int16_t DME_G *in;
+    int16_t DME_G *out;
+    int in_w, ow, oh, offset;
 };
 
 // One unit of a batched launch: a (job, reference) pair covering `count`

@@ -12,7 +12,7 @@
 //   -> sub-pel refinement (wavefront) or the pixel-precision doubling
 //   -> mode decision (bulk level 2 + superblock wavefront) -> SetDC.
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <cstring>
@@ -26,8 +26,8 @@
 
 namespace dme {
 
-hipError_t launch_downconvert(const int16_t *in, int in_w, int16_t *out, int ow, int oh, hipStream_t st);
-hipError_t launch_upconvert(const int16_t *in, int W, int H, int16_t *up, hipStream_t st);
+hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows,
                            hipStream_t st);
@@ -115,7 +115,7 @@ struct dme_ctx {
     bool params_set = false;
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
-    dme::DevBuf d_jobs, d_tasks;
+    dme::DevBuf d_jobs, d_tasks, d_planes;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dme_stats stats;
@@ -163,30 +163,77 @@ static int set_device(dme_ctx *ctx) {
     return DME_OK;
 }
 
-// Builds pyramid levels 1..levels of a picture (PixelMatcher::
-// MakePicHierarchy, pixel_match.cpp:159-183: each level from the previous,
-// sizes from full resolution).
-static int ensure_pyramid(dme_ctx *ctx, Picture &P, int levels, int &launches) {
-    for (int i = P.pyr_levels + 1; i <= levels; ++i) {
-        const int ow = P.W >> i, oh = P.H >> i;
-        const int iw = P.W >> (i - 1);
-        DME_TRY(P.pyr[i].ensure(sizeof(int16_t) * (size_t)ow * oh));
-        const int16_t *in = i == 1 ? P.y : P.pyr[i - 1].as<int16_t>();
-        DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches, [&] {
-            return launch_downconvert(in, iw, P.pyr[i].as<int16_t>(), ow, oh, ctx->stream);
-        }));
-        P.pyr_levels = i;
+// Builds the missing pyramid levels (PixelMatcher::MakePicHierarchy,
+// pixel_match.cpp:159-183: each level from the previous, sizes from full
+// resolution) and up-converted planes (EncPicture::UpOrigData,
+// enc_picture.cpp:131-156) of a set of pictures, one launch per level.
+static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, const std::vector<Picture *> &ups,
+                        int &launches) {
+    int maxl = 0;
+    for (auto &kv : levels) maxl = std::max(maxl, kv.second);
+    std::vector<PlaneTask> all;
+    std::vector<std::pair<size_t, int>> ranges;  // (first, count) per launch, levels 1..maxl then up
+    for (int l = 1; l <= maxl; ++l) {
+        const size_t first = all.size();
+        int off = 0;
+        for (auto &kv : levels) {
+            Picture &P = *kv.first;
+            if (P.pyr_levels >= l || kv.second < l) continue;
+            const int ow = P.W >> l, oh = P.H >> l;
+            DME_TRY(P.pyr[l].ensure(sizeof(int16_t) * (size_t)ow * oh));
+            PlaneTask t;
+            t.in = l == 1 ? P.y : P.pyr[l - 1].as<int16_t>();
+            t.out = P.pyr[l].as<int16_t>();
+            t.in_w = P.W >> (l - 1);
+            t.ow = ow;
+            t.oh = oh;
+            t.offset = off;
+            off += ((ow + 63) / 64) * ((oh + 3) / 4);
+            all.push_back(t);
+        }
+        ranges.push_back(std::make_pair(first, off));
     }
-    return DME_OK;
-}
-
-// EncPicture::UpOrigData (enc_picture.cpp:131-156), cached per picture.
-static int ensure_up(dme_ctx *ctx, Picture &P, int &launches) {
-    if (P.up_ready) return DME_OK;
-    DME_TRY(P.up.ensure(sizeof(int16_t) * 4 * (size_t)P.W * P.H));
-    DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
-                         [&] { return launch_upconvert(P.y, P.W, P.H, P.up.as<int16_t>(), ctx->stream); }));
-    P.up_ready = true;
+    {
+        const size_t first = all.size();
+        int off = 0;
+        for (Picture *P : ups) {
+            if (P->up_ready) continue;
+            DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
+            PlaneTask t;
+            t.in = P->y;
+            t.out = P->up.as<int16_t>();
+            t.in_w = P->W;
+            t.ow = P->W;
+            t.oh = P->H;
+            t.offset = off;
+            off += ((P->W + 63) / 64) * ((P->H + 3) / 4);
+            all.push_back(t);
+        }
+        ranges.push_back(std::make_pair(first, off));
+    }
+    if (all.empty()) return DME_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    DME_TRY(ctx->d_planes.ensure(sizeof(PlaneTask) * all.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_planes.p, all.data(), sizeof(PlaneTask) * all.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    const PlaneTask *dp = ctx->d_planes.as<PlaneTask>();
+    for (int l = 1; l <= maxl; ++l) {
+        const auto &rg = ranges[l - 1];
+        const int n = (int)((l < maxl ? ranges[l].first : ranges.back().first) - rg.first);
+        if (rg.second == 0) continue;
+        DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches,
+                             [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->stream); }));
+    }
+    const auto &ru = ranges.back();
+    if (ru.second) {
+        const int n = (int)(all.size() - ru.first);
+        DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
+                             [&] { return launch_upconvert(dp + ru.first, n, ru.second, ctx->stream); }));
+    }
+    // the task array must outlive the copy
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
+    for (Picture *P : ups) P->up_ready = true;
     return DME_OK;
 }
 
@@ -309,22 +356,32 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     return DME_OK;
 }
 
-// Fills the descriptor's plane pointers once the caches exist.
-static int bind_planes(dme_ctx *ctx, JobState &J, bool need_up, int &launches) {
-    Picture *P = find_pic(ctx, J.pic_num);
-    if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
-    DME_TRY(ensure_pyramid(ctx, *P, J.g.depth, launches));
-    J.desc.pic[0] = P->y;
-    for (int l = 1; l <= J.g.depth; ++l) J.desc.pic[l] = P->pyr[l].as<int16_t>();
-    for (int r = 0; r < J.nrefs; ++r) {
-        Picture *R = find_pic(ctx, J.refs[r]);
-        if (!R) return set_error(DME_ERR_STATE, "reference was dropped");
-        DME_TRY(ensure_pyramid(ctx, *R, J.g.depth, launches));
-        J.desc.ref[r][0] = R->y;
-        for (int l = 1; l <= J.g.depth; ++l) J.desc.ref[r][l] = R->pyr[l].as<int16_t>();
-        if (need_up) {
-            DME_TRY(ensure_up(ctx, *R, launches));
-            J.desc.up[r] = R->up.as<int16_t>();
+// Builds what a batch's pictures need, then fills the descriptors' plane
+// pointers.
+static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, int &launches) {
+    std::map<Picture *, int> levels;
+    std::vector<Picture *> ups;
+    for (auto *j : js) {
+        Picture *P = find_pic(ctx, j->pic_num);
+        if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
+        levels[P] = std::max(levels[P], j->g.depth);
+        for (int r = 0; r < j->nrefs; ++r) {
+            Picture *R = find_pic(ctx, j->refs[r]);
+            if (!R) return set_error(DME_ERR_STATE, "reference was dropped");
+            levels[R] = std::max(levels[R], j->g.depth);
+            if (need_up && std::find(ups.begin(), ups.end(), R) == ups.end()) ups.push_back(R);
+        }
+    }
+    DME_TRY(build_planes(ctx, levels, ups, launches));
+    for (auto *j : js) {
+        Picture *P = find_pic(ctx, j->pic_num);
+        j->desc.pic[0] = P->y;
+        for (int l = 1; l <= j->g.depth; ++l) j->desc.pic[l] = P->pyr[l].as<int16_t>();
+        for (int r = 0; r < j->nrefs; ++r) {
+            Picture *R = find_pic(ctx, j->refs[r]);
+            j->desc.ref[r][0] = R->y;
+            for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
+            if (need_up) j->desc.up[r] = R->up.as<int16_t>();
         }
     }
     return DME_OK;
@@ -372,10 +429,10 @@ static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->
 
 static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     int maxd = 0;
-    for (auto *j : js) {
-        DME_TRY(bind_planes(ctx, *j, false, launches));
-        maxd = std::max(maxd, j->g.depth);
-    }
+    // the up-converted planes are built with the pyramids: one pass over the
+    // pictures, and the sub-pel stage then finds them ready
+    DME_TRY(bind_planes(ctx, js, true, launches));
+    for (auto *j : js) maxd = std::max(maxd, j->g.depth);
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
     std::vector<TaskTable::Range> bulk(maxd + 1);
@@ -449,7 +506,7 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
 }
 
 static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
-    for (auto *j : js) DME_TRY(bind_planes(ctx, *j, true, launches));
+    DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
     std::vector<Task> wt, sh, bt;
@@ -487,8 +544,8 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
 }
 
 static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    DME_TRY(bind_planes(ctx, js, true, launches));
     for (auto *j : js) {
-        DME_TRY(bind_planes(ctx, *j, true, launches));
         const size_t B = (size_t)j->g.xnb * j->g.ynb;
         // zero-initialised MEData arrays (motion.cpp:55-126)
         HIP_TRY(hipMemsetAsync(j->intra.p, 0, sizeof(float) * B, ctx->stream));

@@ -169,10 +169,25 @@ __device__ unsigned long long wave_pel_argmin(const int16_t *__restrict__ pic, c
 // ---------------------------------------------------------------------------
 __constant__ int c_down_taps[6] = {86, 46, 4, -8, -4, 4};
 
-__global__ __launch_bounds__(256) void k_downconvert(const int16_t *__restrict__ in, int in_w,
-                                                     int16_t *__restrict__ out, int ow, int oh) {
-    const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
+// One launch serves every plane of a batch: tiles of 64 x 4 output samples
+// are mapped to planes through the offset table.
+__global__ __launch_bounds__(256) void k_downconvert(const PlaneTask *__restrict__ pt, int npt, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    int lo = 0, hi = npt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pt[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneTask p = pt[lo];
+    const int local = tile - p.offset;
+    const int tx = (p.ow + 63) / 64;
+    const int cx = (local % tx) * 64 + (threadIdx.x & 63);
+    const int cy = (local / tx) * 4 + (threadIdx.x >> 6);
+    const int ow = p.ow, oh = p.oh, in_w = p.in_w;
+    const int16_t *__restrict__ in = p.in;
+    int16_t *__restrict__ out = p.out;
     if (cx >= ow || cy >= oh) return;
     const int xlen = 2 * ow, ylen = 2 * oh;
     const int y = 2 * cy, x = 2 * cx;
@@ -209,10 +224,23 @@ __device__ __forceinline__ int up_tap8(const int s[8]) {
     return clampi(sum >> 5, -128, 127);
 }
 
-__global__ __launch_bounds__(256) void k_upconvert(const int16_t *__restrict__ in, int W, int H,
-                                                   int16_t *__restrict__ up) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__ pt, int npt, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    int lo = 0, hi = npt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pt[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneTask p = pt[lo];
+    const int local = tile - p.offset;
+    const int W = p.ow, H = p.oh;
+    const int tx = (W + 63) / 64;
+    const int x = (local % tx) * 64 + (threadIdx.x & 63);
+    const int y = (local / tx) * 4 + (threadIdx.x >> 6);
+    const int16_t *__restrict__ in = p.in;
+    int16_t *__restrict__ up = p.out;
     if (x >= W || y >= H) return;
     int rowi[8];
 #pragma unroll
@@ -232,6 +260,51 @@ __global__ __launch_bounds__(256) void k_upconvert(const int16_t *__restrict__ i
     up[(2 * y) * UW + 2 * x + 1] = (int16_t)up_tap8(ev);
     up[(2 * y + 1) * UW + 2 * x] = (int16_t)vt[3];
     up[(2 * y + 1) * UW + 2 * x + 1] = (int16_t)up_tap8(vt);
+}
+
+// ---------------------------------------------------------------------------
+// Memory-level parallelism.  A wavefront step is a short chain of dependent
+// operations, so its global loads must be issued together, not one per loop
+// iteration: the copies below keep 32 loads in flight per lane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void copy_ints(int *__restrict__ dst, const int *__restrict__ src, int n) {
+    for (int q0 = 0; q0 < n; q0 += 32) {
+        int v[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) v[i] = (q0 + i < n) ? src[q0 + i] : 0;
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (q0 + i < n) dst[q0 + i] = v[i];
+    }
+}
+
+// Wave-cooperative copy of a clamped rectangle of a plane into LDS:
+// dst[ry * pitch + rx] = src[clamp(y0 + ry)][clamp(x0 + rx)], 16 loads in
+// flight per lane.
+__device__ __forceinline__ void wave_copy_rect(int16_t *__restrict__ dst, int pitch, const int16_t *__restrict__ src,
+                                               int W, int H, int x0, int y0, int RW, int RH) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int n = RW * RH;
+    for (int b0 = 0; b0 < n; b0 += 16 * WAVE) {
+        int16_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            v[k] = 0;
+            if (i < n) {
+                const int ry = i / RW, rx = i - ry * RW;
+                v[k] = src[(size_t)clampi(y0 + ry, 0, H - 1) * W + clampi(x0 + rx, 0, W - 1)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            if (i < n) {
+                const int ry = i / RW;
+                dst[ry * pitch + i - ry * RW] = v[k];
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -286,18 +359,9 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     const bool work = valid && !b.empty();
     const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
     if (work) {
-        for (int i = lane; i < b.xl * b.yl; i += WAVE) {
-            const int dy = i / b.xl, dx = i - dy * b.xl;
-            s_pic[wv][dy * PEL_MAXB + dx] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-        }
-        for (int t = 0; t < (has_guide ? 2 : 1); ++t) {
-            const int cx = t ? g.x : 0, cy = t ? g.y : 0;
-            for (int i = lane; i < RW * RH; i += WAVE) {
-                const int ry = i / RW, rx = i - ry * RW;
-                s_reg[wv][t][i] = ref[(size_t)clampi(b.yp + cy - R + ry, 0, H - 1) * W +
-                                      clampi(b.xp + cx - R + rx, 0, W - 1)];
-            }
-        }
+        wave_copy_rect(s_pic[wv], PEL_MAXB, pic, W, H, b.xp, b.yp, b.xl, b.yl);
+        wave_copy_rect(s_reg[wv][0], RW, ref, W, H, b.xp - R, b.yp - R, RW, RH);
+        if (has_guide) wave_copy_rect(s_reg[wv][1], RW, ref, W, H, b.xp + g.x - R, b.yp + g.y - R, RW, RH);
     }
     __syncthreads();
     if (work) {
@@ -356,13 +420,19 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
 // pixel_match.cpp:242-380; BlockMatcher::FindBestMatchPel,
 // block_match.cpp:242-275).  One workgroup per (picture, reference), one
 // lane per block row: at step t row y works on block x = t - 2y, reading the
-// left, top and top-right results from a 3-slot LDS ring per row.  The
-// prediction window is looked up in a bulk table when it lies inside one;
-// otherwise the lane's wave computes its SADs together.
+// left, top and top-right results from a 3-slot LDS ring per row.  While it
+// works on block x, the lane prefetches block x+1's zero/guide winner, guide
+// and SAD tables into an LDS slot, so a step's lookups never wait on global
+// memory.  A prediction window outside both tables is computed by the lane's
+// wave from an LDS copy of the picture block and reference window.
 // ---------------------------------------------------------------------------
+#define PEL_MISS_BUF 3072  // int16 per wave: block (<= 32x32), window (<= 42x42 at r = 5), 128 sums
+
+__host__ __device__ __forceinline__ int pel_slot_ints(int TS) { return 8 + 2 * TS; }
+
 __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                   int level) {
-    extern __shared__ dme_mv s_ring[];  // [Y][3]
+                                                   int level, int max_rows) {
+    extern __shared__ int s_dyn[];
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (level > J.depth || tk.ref >= J.nrefs) return;  // uniform per workgroup
@@ -383,12 +453,36 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
     dme_cost *__restrict__ C = J.lpc[r][level];
     int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
     int boff[2] = {J.boff[2][0], J.boff[2][1]};
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int SL = pel_slot_ints(TS);
+    // LDS: ring [max_rows][3] mv | slots [2][max_rows][SL] | miss buffers [waves][PEL_MISS_BUF] int16
+    dme_mv *ring = reinterpret_cast<dme_mv *>(s_dyn);
+    int *slots = s_dyn + 6 * max_rows;
+    int16_t *missbuf = reinterpret_cast<int16_t *>(slots + 2 * max_rows * SL) + wv * PEL_MISS_BUF;
     const int T = (X - 1) + 2 * (Y - 1);
-    for (int t = 0; t <= T; ++t) {
+    for (int t = -1; t <= T; ++t) {
         for (int base = 0; base < Y; base += blockDim.x) {
             const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
+            // prefetch block xb+1 of this row
+            if (yb < Y && xb + 1 >= 0 && xb + 1 < X) {
+                const int xn = xb + 1;
+                const size_t kn = (size_t)yb * X + xn;
+                int *sl = slots + ((xn & 1) * max_rows + yb) * SL;
+                sl[0] = bsad[kn];
+                const dme_mv a = bmv[kn];
+                sl[1] = a.x;
+                sl[2] = a.y;
+                int gx = 0, gy = 0;
+                if (has_guide) {
+                    const dme_mv gv = G[(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xn >> 1, 0, GX - 1)];
+                    gx = gv.x * 2;
+                    gy = gv.y * 2;
+                }
+                sl[3] = gx;
+                sl[4] = gy;
+                copy_ints(sl + 8, tab + kn * 2 * TS, has_guide ? 2 * TS : TS);
+            }
             const bool active = yb < Y && xb >= 0 && xb < X;
             dme_mv pred, best;
             pred.x = pred.y = 0;
@@ -399,44 +493,38 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
             b.xp = b.yp = 0;
             b.xl = b.yl = 0;
             if (active) {
-                const size_t k = (size_t)yb * X + xb;
+                const int *sl = slots + ((xb & 1) * max_rows + yb) * SL;
                 // prediction rules (pixel_match.cpp:314-347)
                 if (yb == 0) {
-                    if (xb > 0) pred = s_ring[yb * 3 + (t - 1) % 3];
+                    if (xb > 0) pred = ring[yb * 3 + (t - 1) % 3];
                 } else if (xb == 0) {
-                    pred = s_ring[(yb - 1) * 3 + (t - 2) % 3];
+                    pred = ring[(yb - 1) * 3 + (t - 2) % 3];
                 } else if (xb < X - 1) {
-                    pred = med3(s_ring[yb * 3 + (t - 1) % 3], s_ring[(yb - 1) * 3 + (t - 2) % 3],
-                                s_ring[(yb - 1) * 3 + (t - 1) % 3]);
+                    pred = med3(ring[yb * 3 + (t - 1) % 3], ring[(yb - 1) * 3 + (t - 2) % 3],
+                                ring[(yb - 1) * 3 + (t - 1) % 3]);
                 } else {
-                    pred = mean2(s_ring[(yb - 1) * 3 + (t - 2) % 3], s_ring[yb * 3 + (t - 1) % 3]);
+                    pred = mean2(ring[(yb - 1) * 3 + (t - 2) % 3], ring[yb * 3 + (t - 1) % 3]);
                 }
-                best_sad = bsad[k];
-                best = bmv[k];
+                best_sad = sl[0];
+                best.x = sl[1];
+                best.y = sl[2];
                 b = block_limits(blen, bsep, boff, W, H, xb, yb);
                 if (!b.empty()) {
-                    dme_mv g;
-                    g.x = g.y = 0;
-                    if (has_guide) {
-                        const dme_mv gv = G[(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
-                        g.x = gv.x * 2;
-                        g.y = gv.y * 2;
-                    }
-                    int tsel = -1;
-                    dme_mv cen;
-                    cen.x = cen.y = 0;
-                    if (has_guide && abs(pred.x - g.x) <= 1 && abs(pred.y - g.y) <= 1) {
-                        tsel = 1;
-                        cen = g;
+                    const int gx = sl[3], gy = sl[4];
+                    const int *tb = nullptr;
+                    int cx = 0, cy = 0;
+                    if (has_guide && abs(pred.x - gx) <= 1 && abs(pred.y - gy) <= 1) {
+                        tb = sl + 8 + TS;
+                        cx = gx;
+                        cy = gy;
                     } else if (abs(pred.x) <= 1 && abs(pred.y) <= 1) {
-                        tsel = 0;
+                        tb = sl + 8;
                     }
-                    if (tsel >= 0) {
-                        const int *tb = tab + ((size_t)k * 2 + tsel) * TS;
+                    if (tb) {
                         int ps = INT_MAX, pk = 0;
                         for (int q = 0; q < n; ++q) {
                             const dme_mv o = win_off(q, rad);
-                            const int s = tb[(pred.y + o.y - cen.y + R) * D + (pred.x + o.x - cen.x + R)];
+                            const int s = tb[(pred.y + o.y - cy + R) * D + (pred.x + o.x - cx + R)];
                             if (s < ps) {
                                 ps = s;
                                 pk = q;
@@ -453,25 +541,46 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                     }
                 }
             }
-            // prediction windows outside the tables: computed by the whole wave
+            // prediction windows outside the tables: the wave stages the block
+            // and the window in LDS and evaluates all candidates together
             unsigned long long mm = __ballot(miss);
             while (mm) {
                 const int src = __ffsll((long long)mm) - 1;
                 mm &= mm - 1;
-                Blk bm;
-                bm.xp = __shfl(b.xp, src, WAVE);
-                bm.yp = __shfl(b.yp, src, WAVE);
-                bm.xl = __shfl(b.xl, src, WAVE);
-                bm.yl = __shfl(b.yl, src, WAVE);
-                dme_mv pm;
-                pm.x = __shfl(pred.x, src, WAVE);
-                pm.y = __shfl(pred.y, src, WAVE);
-                const unsigned long long key = wave_pel_argmin(pic, ref, W, H, bm, n, [&](int c) {
-                    dme_mv m = win_off(c, rad);
-                    m.x += pm.x;
-                    m.y += pm.y;
-                    return m;
-                });
+                const int xp = __shfl(b.xp, src, WAVE), yp = __shfl(b.yp, src, WAVE);
+                const int xl = __shfl(b.xl, src, WAVE), yl = __shfl(b.yl, src, WAVE);
+                const int px = __shfl(pred.x, src, WAVE), py = __shfl(pred.y, src, WAVE);
+                const int RW = xl + 2 * rad, RH = yl + 2 * rad;
+                int16_t *bp = missbuf, *br = missbuf + xl * yl;
+                int *msad = reinterpret_cast<int *>(missbuf + PEL_MISS_BUF) - 128;  // n <= 121 sums
+                wave_copy_rect(bp, xl, pic, W, H, xp, yp, xl, yl);
+                wave_copy_rect(br, RW, ref, W, H, xp + px - rad, yp + py - rad, RW, RH);
+                for (int c = lane; c < n; c += WAVE) msad[c] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                // (candidate, row) items spread over the lanes
+                for (int it = lane; it < n * yl; it += WAVE) {
+                    const int c = it / yl, y = it - c * yl;
+                    const dme_mv o = win_off(c, rad);
+                    const int16_t *rg = br + (y + o.y + rad) * RW + (o.x + rad);
+                    const int16_t *pr = bp + y * xl;
+                    int s = 0;
+                    for (int x = 0; x < xl; ++x) s += abs((int)pr[x] - (int)rg[x]);
+                    atomicAdd(&msad[c], s);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                unsigned long long key = ~0ull;
+                for (int c = lane; c < n; c += WAVE) {
+                    const unsigned long long kk = ((unsigned long long)(unsigned)msad[c] << 32) | (unsigned)c;
+                    key = kk < key ? kk : key;
+                }
+                key = wave_min_u64(key);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 if (lane == src) {
                     const int s = (int)(key >> 32);
                     if (s < best_sad) {
@@ -490,7 +599,7 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 c.mvcost = (float)(int16_t)norm1(best.x - pred.x, best.y - pred.y);
                 c.total = c.SAD + 0.0f * c.mvcost;  // SetTotal(0.0)
                 C[k] = c;
-                s_ring[yb * 3 + t % 3] = best;
+                ring[yb * 3 + t % 3] = best;
             }
         }
         __syncthreads();
@@ -533,27 +642,56 @@ __device__ __forceinline__ int up_val(const int16_t *__restrict__ up, int UW, in
 }
 
 // Integer SADs of up to 4 candidates against the up plane; wave-uniform.
+// (candidate, pixel) items are spread over the lanes, 8 per lane at a time
+// with all their loads issued together.
 __device__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up, int UW, int UH,
                              const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int R = pick_group(nc, b.yl);
-    const int per = WAVE / R, g = lane / R, sub = lane - g * R;
-    for (int base = 0; base < nc; base += per) {
-        const int c = base + g;
-        int s = 0;
-        if (c < nc) {
-            const UpStart st = up_start(b, cands[c], step, UW, UH);
-            for (int row = sub; row < b.yl; row += R) {
-                const int16_t *pr = pic + (size_t)(b.yp + row) * W + b.xp;
-                for (int x = 0; x < b.xl; ++x) s += abs(up_val(up, UW, UH, st, row, x) - (int)pr[x]);
+    UpStart st[4];
+    for (int c = 0; c < 4; ++c) st[c] = up_start(b, cands[c < nc ? c : 0], step, UW, UH);
+    const int np = b.xl * b.yl, n = nc * np;
+    int acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+    for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
+        int pv[8], a[8], bb[8], cc[8], dd[8], w[8][4], cid[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int it = b0 + k * WAVE + lane;
+            cid[k] = -1;
+            pv[k] = a[k] = bb[k] = cc[k] = dd[k] = 0;
+            w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
+            if (it < n) {
+                const int c = it / np, p = it - c * np;
+                const int dy = p / b.xl, dx = p - dy * b.xl;
+                const UpStart &s0 = c == 0 ? st[0] : (c == 1 ? st[1] : (c == 2 ? st[2] : st[3]));
+                const int uy = s0.rsy + 2 * dy, ux = s0.rsx + 2 * dx;
+                const int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
+                const int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
+                cid[k] = c;
+                pv[k] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
+                a[k] = up[(size_t)by * UW + bx];
+                bb[k] = up[(size_t)by * UW + bx1];
+                cc[k] = up[(size_t)by1 * UW + bx];
+                dd[k] = up[(size_t)by1 * UW + bx1];
+                w[k][0] = (2 - s0.rx) * (2 - s0.ry);
+                w[k][1] = s0.rx * (2 - s0.ry);
+                w[k][2] = (2 - s0.rx) * s0.ry;
+                w[k][3] = s0.rx * s0.ry;
             }
         }
-        for (int m = 1; m < R; m <<= 1) s += __shfl_xor(s, m, WAVE);
-        for (int q = 0; q < per && base + q < nc; ++q) {
-            const int v = __shfl(s, q * R, WAVE);
-            out[base + q] = v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = (w[k][0] * a[k] + w[k][1] * bb[k] + w[k][2] * cc[k] + w[k][3] * dd[k] + 2) >> 2;
+            const int d = abs(v - pv[k]);
+            acc0 += cid[k] == 0 ? d : 0;
+            acc1 += cid[k] == 1 ? d : 0;
+            acc2 += cid[k] == 2 ? d : 0;
+            acc3 += cid[k] == 3 ? d : 0;
         }
     }
+    if (nc > 0) out[0] = wave_sum(acc0);
+    if (nc > 1) out[1] = wave_sum(acc1);
+    if (nc > 2) out[2] = wave_sum(acc2);
+    if (nc > 3) out[3] = wave_sum(acc3);
 }
 
 // Per-pixel float accumulation of the bounds-checked path (and of the plain
@@ -622,14 +760,8 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
     }
     const int16_t *__restrict__ up = J.up[r];
     if (work) {
-        for (int i = lane; i < b.xl * b.yl; i += WAVE) {
-            const int dy = i / b.xl, dx = i - dy * b.xl;
-            s_pic[wv][dy * PEL_MAXB + dx] = J.pic[0][(size_t)(b.yp + dy) * W + b.xp + dx];
-        }
-        for (int i = lane; i < RW * RH; i += WAVE) {
-            const int ry = i / RW, rx = i - ry * RW;
-            s_up[wv][ry * SUB_MAXREG + rx] = up[(size_t)clampi(y0 + ry, 0, UH - 1) * UW + clampi(x0 + rx, 0, UW - 1)];
-        }
+        wave_copy_rect(s_pic[wv], PEL_MAXB, J.pic[0], W, H, b.xp, b.yp, b.xl, b.yl);
+        wave_copy_rect(s_up[wv], SUB_MAXREG, up, UW, UH, x0, y0, RW, RH);
     }
     __syncthreads();
     if (work && lane < QS) {
@@ -688,16 +820,19 @@ __device__ float cand_total(const JobDesc &J, const int16_t *up, int UW, int UH,
 // LDS ring per row; each lane copies its block's SAD table into its own LDS
 // slot.  Predictor SADs outside the table are computed by the whole wave.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks) {
-    extern __shared__ int s_dyn[];  // ring [Y][4] dme_mv, then tables [Y][QS] int
+#define SUB_SLOT 56  // 49 SADs, lambda, pel vector, pel costs
+
+__global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                      int max_rows) {
+    extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT]
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (tk.ref >= J.nrefs || J.prec == 0) return;
     const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
     const int X = J.xnb, Y = J.ynb, W = J.W, H = J.H;
     const int UW = 2 * W, UH = 2 * H;
-    dme_mv *s_ring = reinterpret_cast<dme_mv *>(s_dyn);
-    int *s_tab = s_dyn + 2 * 4 * Y;
+    dme_mv *ring = reinterpret_cast<dme_mv *>(s_dyn);
+    int *slots = s_dyn + 8 * max_rows;
     const int16_t *__restrict__ up = J.up[r];
     const int *__restrict__ qtab = J.qtab[r];
     const float *__restrict__ lamv = J.lam;
@@ -707,12 +842,27 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
     int boff[2] = {J.boff[2][0], J.boff[2][1]};
     const int lane = threadIdx.x & 63;
     const int T = (X - 1) + 2 * (Y - 1);
-    for (int t = 0; t <= T; ++t) {
+    for (int t = -1; t <= T; ++t) {
         for (int base = 0; base < Y; base += blockDim.x) {
             const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
+            // prefetch block xb+1 of this row: its SAD table, lambda, pel result
+            if (yb < Y && xb + 1 >= 0 && xb + 1 < X) {
+                const size_t kn = (size_t)yb * X + xb + 1;
+                int *sl = slots + (((xb + 1) & 1) * max_rows + yb) * SUB_SLOT;
+                copy_ints(sl, qtab + kn * QS, QS);
+                sl[49] = __float_as_int(lamv[kn]);
+                const dme_mv pv = V[kn];
+                sl[50] = pv.x;
+                sl[51] = pv.y;
+                const dme_cost pc = C[kn];
+                sl[52] = __float_as_int(pc.SAD);
+                sl[53] = __float_as_int(pc.mvcost);
+                sl[54] = __float_as_int(pc.total);
+            }
             const bool active = yb < Y && xb >= 0 && xb < X;
             const size_t k = active ? (size_t)yb * X + xb : 0;
+            const int *mytab = slots + (((xb & 1) & 1) * max_rows + (active ? yb : 0)) * SUB_SLOT;
             dme_mv pred, pel;
             pred.x = pred.y = pel.x = pel.y = 0;
             dme_cost cost;
@@ -720,7 +870,6 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
             float lambda = 0.0f, lim = 0.0f;
             Blk b;
             b.xp = b.yp = b.xl = b.yl = 0;
-            int *mytab = s_tab + (active ? yb : 0) * QS;
             bool done = true, miss = false;
             dme_mv out_mv;
             dme_cost out_cost;
@@ -729,29 +878,39 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
             int psad = 0;
             if (active) {
                 // GetPred: median of the available left, top-left, top, top-right
-                dme_mv nb[4];
+                dme_mv nb0, nb1, nb2, nb3;
+                nb0 = nb1 = nb2 = nb3 = pred;
                 int nn = 0;
-                if (xb > 0) nb[nn++] = s_ring[yb * 4 + (t - 1) % 4];
-                if (xb > 0 && yb > 0) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 3) % 4];
-                if (yb > 0) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 2) % 4];
-                if (yb > 0 && xb + 1 < X) nb[nn++] = s_ring[(yb - 1) * 4 + (t - 1) % 4];
-                if (nn == 1) pred = nb[0];
-                else if (nn == 2) pred = mean2(nb[0], nb[1]);
-                else if (nn == 3) pred = med3(nb[0], nb[1], nb[2]);
+                auto push = [&](dme_mv v) {
+                    if (nn == 0) nb0 = v;
+                    else if (nn == 1) nb1 = v;
+                    else if (nn == 2) nb2 = v;
+                    else nb3 = v;
+                    ++nn;
+                };
+                if (xb > 0) push(ring[yb * 4 + (t - 1) % 4]);
+                if (xb > 0 && yb > 0) push(ring[(yb - 1) * 4 + (t - 3) % 4]);
+                if (yb > 0) push(ring[(yb - 1) * 4 + (t - 2) % 4]);
+                if (yb > 0 && xb + 1 < X) push(ring[(yb - 1) * 4 + (t - 1) % 4]);
+                if (nn == 1) pred = nb0;
+                else if (nn == 2) pred = mean2(nb0, nb1);
+                else if (nn == 3) pred = med3(nb0, nb1, nb2);
                 else if (nn == 4) {
-                    int sx = 0, sy = 0, mxx = nb[0].x, mxy = nb[0].y, mnx = nb[0].x, mny = nb[0].y;
-                    for (int i = 0; i < 4; ++i) {
-                        sx += nb[i].x; sy += nb[i].y;
-                        mxx = max(mxx, nb[i].x); mxy = max(mxy, nb[i].y);
-                        mnx = min(mnx, nb[i].x); mny = min(mny, nb[i].y);
-                    }
+                    const int sx = nb0.x + nb1.x + nb2.x + nb3.x, sy = nb0.y + nb1.y + nb2.y + nb3.y;
+                    const int mxx = max(max(nb0.x, nb1.x), max(nb2.x, nb3.x));
+                    const int mnx = min(min(nb0.x, nb1.x), min(nb2.x, nb3.x));
+                    const int mxy = max(max(nb0.y, nb1.y), max(nb2.y, nb3.y));
+                    const int mny = min(min(nb0.y, nb1.y), min(nb2.y, nb3.y));
                     pred.x = (sx - (mxx + mnx) + 1) >> 1;
                     pred.y = (sy - (mxy + mny) + 1) >> 1;
                 }
-                lambda = lamv[k];
+                lambda = __int_as_float(mytab[49]);
                 b = block_limits(blen, bsep, boff, W, H, xb, yb);
-                pel = V[k];
-                cost = C[k];
+                pel.x = mytab[50];
+                pel.y = mytab[51];
+                cost.SAD = __int_as_float(mytab[52]);
+                cost.mvcost = __int_as_float(mytab[53]);
+                cost.total = __int_as_float(mytab[54]);
                 dme_mv pel_up;
                 pel_up.x = pel.x << prec;
                 pel_up.y = pel.y << prec;
@@ -764,7 +923,6 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                 } else {
                     done = false;
                     if (!b.empty()) {
-                        for (int q = 0; q < QS; ++q) mytab[q] = qtab[k * QS + q];
                         const int dx = pred.x - pel_up.x, dy = pred.y - pel_up.y;
                         if (abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
                         else miss = true;
@@ -807,7 +965,6 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                         dme_mv tb = bmv;
                         const int sh = prec - i;
                         if (!b.empty()) {
-                            // candidates in units of this stage; table index in final units
                             auto test = [&](int mx, int my) {
                                 dme_mv m;
                                 m.x = mx;
@@ -858,13 +1015,14 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
             if (active) {
                 V[k] = out_mv;
                 C[k] = out_cost;
-                s_ring[yb * 4 + t % 4] = out_mv;
+                ring[yb * 4 + t % 4] = out_mv;
             }
         }
         __syncthreads();
     }
 }
 
+// Pixel-precision path: vectors doubled before mode decision and halved
 // after it (picture_compress.cpp:240-259, 274-296).
 __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                        int ntasks, int total, int left) {
@@ -888,14 +1046,49 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
     const int UW = 2 * J.W, UH = 2 * J.H;
     const UpStart s1 = up_start(b, m1, J.step, UW, UH);
     const UpStart s2 = up_start(b, m2, J.step, UW, UH);
+    const int16_t *__restrict__ pic = J.pic[0];
+    const int16_t *__restrict__ u1 = J.up[0];
+    const int16_t *__restrict__ u2 = J.up[1];
+    const int w10 = (2 - s1.rx) * (2 - s1.ry), w11 = s1.rx * (2 - s1.ry), w12 = (2 - s1.rx) * s1.ry, w13 = s1.rx * s1.ry;
+    const int w20 = (2 - s2.rx) * (2 - s2.ry), w21 = s2.rx * (2 - s2.ry), w22 = (2 - s2.rx) * s2.ry, w23 = s2.rx * s2.ry;
     int s = 0;
     const int n = b.xl * b.yl;
-    for (int i = lane; i < n; i += WAVE) {
-        const int dy = i / b.xl, dx = i - dy * b.xl;
-        const int p = J.pic[0][(size_t)(b.yp + dy) * J.W + b.xp + dx];
-        const int16_t diff = (int16_t)((p << 1) - up_val(J.up[0], UW, UH, s1, dy, dx));
-        const int16_t temp = (int16_t)(((int)diff - up_val(J.up[1], UW, UH, s2, dy, dx)) >> 1);
-        s += abs((int)temp);
+    for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
+        int p[8], a[8][4], c[8][4];
+        bool ok[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            ok[k] = i < n;
+            const int ii = ok[k] ? i : 0;
+            const int dy = ii / b.xl, dx = ii - dy * b.xl;
+            p[k] = pic[(size_t)(b.yp + dy) * J.W + b.xp + dx];
+            int uy = s1.rsy + 2 * dy, ux = s1.rsx + 2 * dx;
+            int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
+            int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
+            a[k][0] = u1[(size_t)by * UW + bx];
+            a[k][1] = u1[(size_t)by * UW + bx1];
+            a[k][2] = u1[(size_t)by1 * UW + bx];
+            a[k][3] = u1[(size_t)by1 * UW + bx1];
+            uy = s2.rsy + 2 * dy;
+            ux = s2.rsx + 2 * dx;
+            by = clampi(uy, 0, UH - 1);
+            bx = clampi(ux, 0, UW - 1);
+            by1 = clampi(uy + 1, 0, UH - 1);
+            bx1 = clampi(ux + 1, 0, UW - 1);
+            c[k][0] = u2[(size_t)by * UW + bx];
+            c[k][1] = u2[(size_t)by * UW + bx1];
+            c[k][2] = u2[(size_t)by1 * UW + bx];
+            c[k][3] = u2[(size_t)by1 * UW + bx1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int r1 = (w10 * a[k][0] + w11 * a[k][1] + w12 * a[k][2] + w13 * a[k][3] + 2) >> 2;
+            const int r2 = (w20 * c[k][0] + w21 * c[k][1] + w22 * c[k][2] + w23 * c[k][3] + 2) >> 2;
+            const int16_t diff = (int16_t)((p[k] << 1) - r1);
+            const int16_t temp = (int16_t)(((int)diff - r2) >> 1);
+            s += ok[k] ? abs((int)temp) : 0;
+        }
     }
     return wave_sum(s);
 }
@@ -1168,6 +1361,20 @@ __device__ float unit_decn(const JobDesc &J, int level, float loc_lambda, dme_co
 // superblocks t = X + Y, one wave per superblock.  Level 0 always runs: the
 // superblock cost after level 1 never exceeds the level-2 cost.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ dme_mv shfl_mv(dme_mv v, int src) {
+    dme_mv o;
+    o.x = __shfl(v.x, src, WAVE);
+    o.y = __shfl(v.y, src, WAVE);
+    return o;
+}
+__device__ __forceinline__ dme_cost shfl_cost(dme_cost c, int src) {
+    dme_cost o;
+    o.SAD = __shfl(c.SAD, src, WAVE);
+    o.mvcost = __shfl(c.mvcost, src, WAVE);
+    o.total = __shfl(c.total, src, WAVE);
+    return o;
+}
+
 __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ jobs) {
     const JobDesc &J = jobs[blockIdx.x];
     const int XS = J.xsb, YS = J.ysb, X2 = J.xnb;
@@ -1182,32 +1389,53 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
             const size_t s = (size_t)ys * XS + xs;
             // level 2: sum of the block costs in raster order
             float sbc = 0.0f;
+#pragma unroll
             for (int j = 0; j < 4; ++j)
+#pragma unroll
                 for (int i = 0; i < 4; ++i) sbc += J.ucost[(size_t)(4 * ys + j) * X2 + 4 * xs + i];
             int split = 2;
-            // level 1
+            // level 1: the eight (unit, reference) searches are independent;
+            // lane 2u+r runs unit u against reference r
+            dme_mv lv;
+            lv.x = lv.y = 0;
+            dme_cost lc;
+            lc.SAD = lc.mvcost = 0.0f;
+            lc.total = 100000000.0f;
+            int lq = -1;
+            if (lane < 8 && (lane & 1) < nr) {
+                const int u = lane >> 1, r = lane & 1;
+                const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
+                const size_t ku = (size_t)yu * X1 + xu;
+                dme_mv g[4];
+                int gs[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    g[q] = J.vec[r][(size_t)(2 * yu + (q >> 1)) * X2 + 2 * xu + (q & 1)];
+                    gs[q] = J.m1tab[ku * 24 + r * 4 + q];
+                }
+                lq = unit_me(J, 1, xu, yu, r, g, gs, J.lam1[ku], lv, lc);
+            }
             dme_mv v1[4][2];
+            dme_cost c1[4][2];
             int q1[4][2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    v1[u][r] = shfl_mv(lv, 2 * u + r);
+                    c1[u][r] = shfl_cost(lc, 2 * u + r);
+                    q1[u][r] = __shfl(lq, 2 * u + r, WAVE);
+                }
             int m1[4];
             float sb1 = 0.0f;
+#pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
                 const size_t ku = (size_t)yu * X1 + xu;
-                const float lam = J.lam1[ku];
-                const int *tab = J.m1tab + ku * 24;
-                dme_cost c[2];
-                for (int r = 0; r < nr; ++r) {
-                    dme_mv g[4];
-                    int gs[4];
-                    for (int q = 0; q < 4; ++q) {
-                        g[q] = J.vec[r][(size_t)(2 * yu + (q >> 1)) * X2 + 2 * xu + (q & 1)];
-                        gs[q] = tab[r * 4 + q];
-                    }
-                    q1[u][r] = unit_me(J, 1, xu, yu, r, g, gs, lam, v1[u][r], c[r]);
-                }
                 int bis = 0;
-                if (nr > 1 && q1[u][0] >= 0 && q1[u][1] >= 0) bis = tab[8 + q1[u][0] * 4 + q1[u][1]];
-                sb1 += unit_decn(J, 1, lam, c, bis, m1[u]);
+                if (nr > 1 && q1[u][0] >= 0 && q1[u][1] >= 0) bis = J.m1tab[ku * 24 + 8 + q1[u][0] * 4 + q1[u][1]];
+                dme_cost c[2] = {c1[u][0], c1[u][1]};
+                sb1 += unit_decn(J, 1, J.lam1[ku], c, bis, m1[u]);
             }
             if (sb1 <= sbc) {
                 sbc = sb1;
@@ -1215,36 +1443,59 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
                 if (lane < 16) {
                     const int u = ((lane >> 3) << 1) | ((lane >> 1) & 1);
                     const size_t d = (size_t)(4 * ys + (lane >> 2)) * X2 + 4 * xs + (lane & 3);
-                    J.mode[d] = m1[u];
+                    int mu = m1[0];
+                    dme_mv a = v1[0][0], bb = v1[0][1];
+                    if (u == 1) { mu = m1[1]; a = v1[1][0]; bb = v1[1][1]; }
+                    if (u == 2) { mu = m1[2]; a = v1[2][0]; bb = v1[2][1]; }
+                    if (u == 3) { mu = m1[3]; a = v1[3][0]; bb = v1[3][1]; }
+                    J.mode[d] = mu;
                     J.dc[0][d] = 0;
-                    J.vec[0][d] = v1[u][0];
-                    if (nr > 1) J.vec[1][d] = v1[u][1];
+                    J.vec[0][d] = a;
+                    if (nr > 1) J.vec[1][d] = bb;
                 }
             }
-            // level 0: guides are the level-1 results; their SADs are those of
-            // the blocks they came from (unchosen units fall back to (0,0))
+            // level 0: guides are the level-1 results; their SADs are those
+            // of the blocks they came from (a unit with no result offers (0,0),
+            // whose SAD the wave computes)
             {
                 const float lam = J.lam0[s];
                 const int *tab = J.m0tab + s * 32;
                 const Blk b0 = block_limits(J.blen[0], J.bsep[0], J.boff[0], J.W, J.H, xs, ys);
-                dme_mv v0[2];
-                dme_cost c[2];
-                for (int r = 0; r < nr; ++r) {
-                    dme_mv g[4];
-                    int gs[4];
+                int gs0[2][4];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        g[q] = v1[q][r];
-                        const int bq = q1[q][r];
-                        if (bq >= 0) {
-                            gs[q] = tab[r * 16 + ((2 * (q >> 1) + (bq >> 1)) * 4 + 2 * (q & 1) + (bq & 1))];
-                        } else {
-                            int sd = 0;
-                            if (!b0.empty()) wave_up_sads(J.pic[0], J.W, J.up[r], 2 * J.W, 2 * J.H, b0, J.step, &g[q], 1, &sd);
-                            gs[q] = sd;
+                        gs0[r][q] = 0;
+                        if (r < nr) {
+                            const int bq = q1[q][r];
+                            if (bq >= 0) {
+                                gs0[r][q] = tab[r * 16 + ((2 * (q >> 1) + (bq >> 1)) * 4 + 2 * (q & 1) + (bq & 1))];
+                            } else if (!b0.empty()) {
+                                int sd = 0;
+                                wave_up_sads(J.pic[0], J.W, J.up[r], 2 * J.W, 2 * J.H, b0, J.step, &v1[q][r], 1, &sd);
+                                gs0[r][q] = sd;
+                            }
                         }
                     }
-                    unit_me(J, 0, xs, ys, r, g, gs, lam, v0[r], c[r]);
+                dme_mv mv0;
+                mv0.x = mv0.y = 0;
+                dme_cost mc0;
+                mc0.SAD = mc0.mvcost = 0.0f;
+                mc0.total = 100000000.0f;
+                if (lane < nr) {
+                    const int r = lane;
+                    dme_mv g[4];
+                    int gs[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        g[q] = r ? v1[q][1] : v1[q][0];
+                        gs[q] = r ? gs0[1][q] : gs0[0][q];
+                    }
+                    unit_me(J, 0, xs, ys, r, g, gs, lam, mv0, mc0);
                 }
+                dme_mv v0[2] = {shfl_mv(mv0, 0), shfl_mv(mv0, 1)};
+                dme_cost c[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
                 int bis = 0;
                 if (nr > 1 && !b0.empty()) bis = wave_bi_sad(J, b0, v0[0], v0[1]);
                 int m0;
@@ -1321,15 +1572,15 @@ __global__ __launch_bounds__(256) void k_dc_smooth(const JobDesc *__restrict__ j
 // Launch wrappers (host side, same translation unit as the kernels)
 // ---------------------------------------------------------------------------
 
-hipError_t launch_downconvert(const int16_t *in, int in_w, int16_t *out, int ow, int oh, hipStream_t st) {
-    dim3 grid((ow + 63) / 64, (oh + 3) / 4);
-    hipLaunchKernelGGL(k_downconvert, grid, dim3(256), 0, st, in, in_w, out, ow, oh);
+hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_downconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
     return hipGetLastError();
 }
 
-hipError_t launch_upconvert(const int16_t *in, int W, int H, int16_t *up, hipStream_t st) {
-    dim3 grid((W + 63) / 64, (H + 3) / 4);
-    hipLaunchKernelGGL(k_upconvert, grid, dim3(256), 0, st, in, W, H, up);
+hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
     return hipGetLastError();
 }
 
@@ -1343,8 +1594,18 @@ hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, i
                            hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
     const int threads = min(1024, ((max_rows + 63) / 64) * 64);
-    const size_t lds = sizeof(dme_mv) * 3 * (size_t)max_rows;
-    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, level);
+    const int R = min(level + 1, 5) + 1, TS = (2 * R + 1) * (2 * R + 1);
+    const size_t lds = sizeof(int) * (6 * (size_t)max_rows + 2 * (size_t)max_rows * pel_slot_ints(TS)) +
+                       sizeof(int16_t) * PEL_MISS_BUF * (threads / 64);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, level, max_rows);
     return hipGetLastError();
 }
 
@@ -1357,8 +1618,16 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
     const int threads = min(1024, ((max_rows + 63) / 64) * 64);
-    const size_t lds = (sizeof(dme_mv) * 4 + sizeof(int) * 49) * (size_t)max_rows;
-    hipLaunchKernelGGL(k_subpel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks);
+    const size_t lds = sizeof(int) * (8 + 2 * SUB_SLOT) * (size_t)max_rows;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_subpel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_subpel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
     return hipGetLastError();
 }
 
