@@ -644,7 +644,7 @@ __device__ __forceinline__ int up_val(const int16_t *__restrict__ up, int UW, in
 // Integer SADs of up to 4 candidates against the up plane; wave-uniform.
 // (candidate, pixel) items are spread over the lanes, 8 per lane at a time
 // with all their loads issued together.
-__device__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up, int UW, int UH,
+__device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up, int UW, int UH,
                              const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
     const int lane = threadIdx.x & (WAVE - 1);
     UpStart st[4];
@@ -662,7 +662,10 @@ __device__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const int16
             if (it < n) {
                 const int c = it / np, p = it - c * np;
                 const int dy = p / b.xl, dx = p - dy * b.xl;
-                const UpStart &s0 = c == 0 ? st[0] : (c == 1 ? st[1] : (c == 2 ? st[2] : st[3]));
+                UpStart s0 = st[0];
+#pragma unroll
+                for (int e = 1; e < 4; ++e)
+                    if (c == e) s0 = st[e];
                 const int uy = s0.rsy + 2 * dy, ux = s0.rsx + 2 * dx;
                 const int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
                 const int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
@@ -705,6 +708,69 @@ __device__ float chain_total(float start, const int16_t *__restrict__ pic, int W
         for (int dx = 0; dx < b.xl; ++dx) sum += (float)abs(up_val(up, UW, UH, st, dy, dx) - (int)pr[dx]);
     }
     return sum;
+}
+
+// The same float chain computed by the whole wave (uniform arguments).  For
+// 0 <= s < 2^24 and an integer d >= 0, s + d is exact while it stays below
+// the power of two above s: both are multiples of the spacing of s's binade.
+// So the chain only rounds where a partial sum enters a higher binade.  The
+// wave takes 64 differences at a time, forms their prefix sums, adds the runs
+// between crossings in one exact step each and performs the crossing
+// additions one by one.  Sums from 2^24 up are added one pixel at a time.
+__device__ __forceinline__ float binade_limit(float s) {
+    const unsigned e = __float_as_uint(s) >> 23;  // s >= 0
+    return __uint_as_float((e + 1) << 23);
+}
+
+__device__ __noinline__ float wave_chain(float start, const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up,
+                            int UW, int UH, const Blk &b, const UpStart &st) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int n = b.xl * b.yl;
+    float s = start;
+    for (int b0 = 0; b0 < n; b0 += 4 * WAVE) {
+        int d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            const int ii = i < n ? i : n - 1;
+            const int dy = ii / b.xl, dx = ii - dy * b.xl;
+            const int v = abs(up_val(up, UW, UH, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+            d[k] = i < n ? v : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (b0 + k * WAVE >= n) break;
+            int x = d[k];
+#pragma unroll
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const int y = __shfl_up(x, off, WAVE);
+                if (lane >= off) x += y;
+            }
+            const int total = __builtin_amdgcn_readlane(x, WAVE - 1);
+            int consumed = 0, first = 0;
+            while (first < WAVE) {
+                if (s >= 16777216.0f) {
+                    for (int j = first; j < WAVE; ++j)
+                        s += (float)(__builtin_amdgcn_readlane(x, j) - (j ? __builtin_amdgcn_readlane(x, j - 1) : 0));
+                    break;
+                }
+                const double D = (double)binade_limit(s) - (double)s;
+                const unsigned long long m = __ballot(lane >= first && (double)(x - consumed) >= D);
+                if (!m) {
+                    s += (float)(total - consumed);
+                    break;
+                }
+                const int j = __ffsll((long long)m) - 1;
+                const int before = j ? __builtin_amdgcn_readlane(x, j - 1) : 0;
+                const int at = __builtin_amdgcn_readlane(x, j);
+                s += (float)(before - consumed);
+                s += (float)(at - before);
+                consumed = at;
+                first = j + 1;
+            }
+        }
+    }
+    return s;
 }
 
 // GetVarUp (block_match.cpp:233-240) through its short return type
@@ -1041,6 +1107,19 @@ __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict
 
 // BiBlock{Half,Quarter}Pel::Diff (me_utils.cpp:1172-1543): sum of
 // |((2 pic - r1) - r2) >> 1| with short intermediates.
+__device__ __forceinline__ int up_interp(const int16_t *__restrict__ up, int UW, int UH, const UpStart &s, int dy,
+                                         int dx) {
+    const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
+    const unsigned y0 = clampi(uy, 0, UH - 1), x0 = clampi(ux, 0, UW - 1);
+    const unsigned y1 = clampi(uy + 1, 0, UH - 1), x1 = clampi(ux + 1, 0, UW - 1);
+    const int a = up[y0 * UW + x0];
+    const int bq = s.rx ? (int)up[y0 * UW + x1] : 0;
+    const int c = s.ry ? (int)up[y1 * UW + x0] : 0;
+    const int d = (s.rx & s.ry) ? (int)up[y1 * UW + x1] : 0;
+    return ((2 - s.rx) * (2 - s.ry) * a + s.rx * (2 - s.ry) * bq + (2 - s.rx) * s.ry * c + s.rx * s.ry * d + 2) >> 2;
+}
+
+template <int NB = 8>
 __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int UW = 2 * J.W, UH = 2 * J.H;
@@ -1049,44 +1128,25 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
     const int16_t *__restrict__ pic = J.pic[0];
     const int16_t *__restrict__ u1 = J.up[0];
     const int16_t *__restrict__ u2 = J.up[1];
-    const int w10 = (2 - s1.rx) * (2 - s1.ry), w11 = s1.rx * (2 - s1.ry), w12 = (2 - s1.rx) * s1.ry, w13 = s1.rx * s1.ry;
-    const int w20 = (2 - s2.rx) * (2 - s2.ry), w21 = s2.rx * (2 - s2.ry), w22 = (2 - s2.rx) * s2.ry, w23 = s2.rx * s2.ry;
     int s = 0;
     const int n = b.xl * b.yl;
-    for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
-        int p[8], a[8][4], c[8][4];
-        bool ok[8];
+    for (int b0 = 0; b0 < n; b0 += NB * WAVE) {
+        int p[NB], r1[NB], r2[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < NB; ++k) {
             const int i = b0 + k * WAVE + lane;
             ok[k] = i < n;
             const int ii = ok[k] ? i : 0;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
-            p[k] = pic[(size_t)(b.yp + dy) * J.W + b.xp + dx];
-            int uy = s1.rsy + 2 * dy, ux = s1.rsx + 2 * dx;
-            int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
-            int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
-            a[k][0] = u1[(size_t)by * UW + bx];
-            a[k][1] = u1[(size_t)by * UW + bx1];
-            a[k][2] = u1[(size_t)by1 * UW + bx];
-            a[k][3] = u1[(size_t)by1 * UW + bx1];
-            uy = s2.rsy + 2 * dy;
-            ux = s2.rsx + 2 * dx;
-            by = clampi(uy, 0, UH - 1);
-            bx = clampi(ux, 0, UW - 1);
-            by1 = clampi(uy + 1, 0, UH - 1);
-            bx1 = clampi(ux + 1, 0, UW - 1);
-            c[k][0] = u2[(size_t)by * UW + bx];
-            c[k][1] = u2[(size_t)by * UW + bx1];
-            c[k][2] = u2[(size_t)by1 * UW + bx];
-            c[k][3] = u2[(size_t)by1 * UW + bx1];
+            p[k] = pic[(unsigned)((b.yp + dy) * J.W + b.xp + dx)];
+            r1[k] = up_interp(u1, UW, UH, s1, dy, dx);
+            r2[k] = up_interp(u2, UW, UH, s2, dy, dx);
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int r1 = (w10 * a[k][0] + w11 * a[k][1] + w12 * a[k][2] + w13 * a[k][3] + 2) >> 2;
-            const int r2 = (w20 * c[k][0] + w21 * c[k][1] + w22 * c[k][2] + w23 * c[k][3] + 2) >> 2;
-            const int16_t diff = (int16_t)((p[k] << 1) - r1);
-            const int16_t temp = (int16_t)(((int)diff - r2) >> 1);
+        for (int k = 0; k < NB; ++k) {
+            const int16_t diff = (int16_t)((p[k] << 1) - r1[k]);
+            const int16_t temp = (int16_t)(((int)diff - r2[k]) >> 1);
             s += ok[k] ? abs((int)temp) : 0;
         }
     }
@@ -1289,13 +1349,50 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
 
 // ModeDecider::DoME for one unit and reference (me_mode_decn.cpp:279-364;
 // BlockMatcher::FindBestMatchSubp, block_match.cpp:277-325) from tabulated
-// SADs.  g are the four guide vectors with their SADs; duplicates are skipped
-// as AddNewVlist does (they cannot win a strict-< comparison anyway).
+// SADs, one (unit, reference) per lane.  g are the four guide vectors with
+// their SADs; duplicates are skipped as AddNewVlist does (they cannot win a
+// strict-< comparison anyway).  Totals that need the per-pixel float chain
+// are computed by the whole wave, so every lane of the wave calls this.
 // Returns the index of the chosen guide (-1 when nothing was chosen).
-__device__ int unit_me(const JobDesc &J, int level, int xu, int yu, int r, const dme_mv g[4], const int gs[4],
-                       float lambda, dme_mv &out_mv, dme_cost &out_cost) {
-    const dme_mv pred = mode_pred(J.vec[r], J.xnb, xu << (2 - level), yu << (2 - level));
-    const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], J.W, J.H, xu, yu);
+__device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pred, const dme_mv g[4],
+                       const int gs[4], float lambda, dme_mv &out_mv, dme_cost &out_cost) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int UW = 2 * J.W, UH = 2 * J.H;
+    float tot[4], mvc[4], st[4];
+    bool cand[4], need[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bool dup = false;
+#pragma unroll
+        for (int p = 0; p < q; ++p) dup = dup || (g[p].x == g[q].x && g[p].y == g[q].y);
+        cand[q] = on && !b.empty() && !dup;
+        mvc[q] = var_up(pred, g[q]);
+        st[q] = mvc[q] * lambda;
+        tot[q] = (float)gs[q] + st[q];
+        const UpStart us = up_start(b, g[q], J.step, UW, UH);
+        need[q] = cand[q] && (us.bounds || J.fp_sem == DME_FP_C) && !chain_is_exact(st[q], gs[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        unsigned long long mm = __ballot(need[q]);
+        while (mm) {
+            const int src = __ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            Blk bs;
+            bs.xp = __builtin_amdgcn_readlane(b.xp, src);
+            bs.yp = __builtin_amdgcn_readlane(b.yp, src);
+            bs.xl = __builtin_amdgcn_readlane(b.xl, src);
+            bs.yl = __builtin_amdgcn_readlane(b.yl, src);
+            dme_mv m;
+            m.x = __builtin_amdgcn_readlane(g[q].x, src);
+            m.y = __builtin_amdgcn_readlane(g[q].y, src);
+            const float start = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(st[q]), src));
+            const int rs = __builtin_amdgcn_readlane(r, src);
+            const UpStart us = up_start(bs, m, J.step, UW, UH);
+            const float v = wave_chain(start, J.pic[0], J.W, rs ? J.up[1] : J.up[0], UW, UH, bs, us);
+            if (lane == src) tot[q] = v;
+        }
+    }
     dme_cost best;
     best.SAD = 0.0f;
     best.mvcost = 0.0f;
@@ -1304,22 +1401,14 @@ __device__ int unit_me(const JobDesc &J, int level, int xu, int yu, int r, const
     bmv.x = 0;
     bmv.y = 0;
     int bq = -1;
-    if (!b.empty()) {
-        const int UW = 2 * J.W, UH = 2 * J.H;
-        for (int q = 0; q < 4; ++q) {
-            bool dup = false;
-            for (int p = 0; p < q; ++p) dup = dup || (g[p].x == g[q].x && g[p].y == g[q].y);
-            if (dup) continue;
-            const float mvc = var_up(pred, g[q]);
-            const float start = mvc * lambda;
-            const float total = cand_total(J, J.up[r], UW, UH, b, g[q], J.step, gs[q], start);
-            if (total < best.total) {
-                bmv = g[q];
-                best.total = total;
-                best.mvcost = mvc;
-                best.SAD = total - start;
-                bq = q;
-            }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (cand[q] && tot[q] < best.total) {
+            bmv = g[q];
+            best.total = tot[q];
+            best.mvcost = mvc[q];
+            best.SAD = tot[q] - st[q];
+            bq = q;
         }
     }
     out_mv = bmv;
@@ -1359,7 +1448,11 @@ __device__ float unit_decn(const JobDesc &J, int level, float loc_lambda, dme_co
 // Superblock decisions (ModeDecider::DoSBDecn / DoLevelDecn,
 // me_mode_decn.cpp:161-276).  One workgroup per picture, wavefront over
 // superblocks t = X + Y, one wave per superblock.  Level 0 always runs: the
-// superblock cost after level 1 never exceeds the level-2 cost.
+// superblock cost after level 1 never exceeds the level-2 cost.  Everything a
+// superblock reads from memory is fetched in one round at the start (the
+// vectors its predictions use belong to superblocks of earlier steps, or are
+// its own level-2 vectors, which it overwrites only after reading them);
+// values then move between lanes by shuffles.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ dme_mv shfl_mv(dme_mv v, int src) {
     dme_mv o;
@@ -1374,142 +1467,159 @@ __device__ __forceinline__ dme_cost shfl_cost(dme_cost c, int src) {
     o.total = __shfl(c.total, src, WAVE);
     return o;
 }
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
+#ifndef MODE_NB
+#define MODE_NB 4
+#endif
 __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ jobs) {
     const JobDesc &J = jobs[blockIdx.x];
-    const int XS = J.xsb, YS = J.ysb, X2 = J.xnb;
+    const int XS = J.xsb, YS = J.ysb, X2 = J.xnb, X1 = J.xnb / 2;
     const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
     const int nr = J.nrefs;
-    const int X1 = J.xnb / 2;
+    const dme_mv *__restrict__ V0 = J.vec[0];
+    const dme_mv *__restrict__ V1 = J.vec[1];
     for (int t = 0; t <= (XS - 1) + (YS - 1); ++t) {
         const int ylo = t - (XS - 1) > 0 ? t - (XS - 1) : 0;
         const int yhi = min(YS - 1, t);
         for (int ys = ylo + wave; ys <= yhi; ys += nwaves) {
             const int xs = t - ys;
             const size_t s = (size_t)ys * XS + xs;
-            // level 2: sum of the block costs in raster order
-            float sbc = 0.0f;
+            // ---- loads.  lanes 0..15: level-2 costs of the 16 blocks;
+            // lanes 0..7: unit u = lane/2 against reference r = lane%2 at level 1;
+            // every lane: one level-1 bi-prediction SAD (unit lane/16, pair lane%16);
+            // lanes 0..31: level-0 candidate SADs; lanes 0..1: level-0 prediction
+            float ucl = 0.0f;
+            if (lane < 16) ucl = J.ucost[(size_t)(4 * ys + (lane >> 2)) * X2 + 4 * xs + (lane & 3)];
+            const int u = (lane >> 1) & 3, r1 = lane & 1;
+            const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
+            const size_t ku = (size_t)yu * X1 + xu;
+            const bool on1 = lane < 8 && r1 < nr;
+            dme_mv g[4];
+            int gs[4];
+            float lam1 = 0.0f;
+            dme_mv pred1;
+            pred1.x = pred1.y = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) sbc += J.ucost[(size_t)(4 * ys + j) * X2 + 4 * xs + i];
-            int split = 2;
-            // level 1: the eight (unit, reference) searches are independent;
-            // lane 2u+r runs unit u against reference r
-            dme_mv lv;
-            lv.x = lv.y = 0;
-            dme_cost lc;
-            lc.SAD = lc.mvcost = 0.0f;
-            lc.total = 100000000.0f;
-            int lq = -1;
-            if (lane < 8 && (lane & 1) < nr) {
-                const int u = lane >> 1, r = lane & 1;
-                const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
-                const size_t ku = (size_t)yu * X1 + xu;
-                dme_mv g[4];
-                int gs[4];
+            for (int q = 0; q < 4; ++q) {
+                g[q].x = g[q].y = 0;
+                gs[q] = 0;
+            }
+            if (on1) {
+                const dme_mv *__restrict__ V = r1 ? V1 : V0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    g[q] = J.vec[r][(size_t)(2 * yu + (q >> 1)) * X2 + 2 * xu + (q & 1)];
-                    gs[q] = J.m1tab[ku * 24 + r * 4 + q];
+                    g[q] = V[(size_t)(2 * yu + (q >> 1)) * X2 + 2 * xu + (q & 1)];
+                    gs[q] = J.m1tab[ku * 24 + r1 * 4 + q];
                 }
-                lq = unit_me(J, 1, xu, yu, r, g, gs, J.lam1[ku], lv, lc);
+                lam1 = J.lam1[ku];
+                pred1 = mode_pred(V, X2, 2 * xu, 2 * yu);
             }
-            dme_mv v1[4][2];
-            dme_cost c1[4][2];
-            int q1[4][2];
+            int bitab = 0;
+            if (nr > 1) {
+                const int ub = lane >> 4;
+                const size_t kb = (size_t)(2 * ys + (ub >> 1)) * X1 + 2 * xs + (ub & 1);
+                bitab = J.m1tab[kb * 24 + 8 + (lane & 15)];
+            }
+            const int m0v = lane < 32 ? J.m0tab[s * 32 + lane] : 0;
+            const float lam0 = J.lam0[s];
+            dme_mv pred0;
+            pred0.x = pred0.y = 0;
+            if (lane < nr) pred0 = mode_pred(lane ? V1 : V0, X2, 4 * xs, 4 * ys);
+
+            // ---- level 2: sum of the block costs in raster order
+            float sbc = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    v1[u][r] = shfl_mv(lv, 2 * u + r);
-                    c1[u][r] = shfl_cost(lc, 2 * u + r);
-                    q1[u][r] = __shfl(lq, 2 * u + r, WAVE);
-                }
-            int m1[4];
+            for (int i = 0; i < 16; ++i) sbc += readlane_f(ucl, i);
+            int split = 2;
+
+            // ---- level 1: the eight (unit, reference) searches
+            const Blk b1 = block_limits(J.blen[1], J.bsep[1], J.boff[1], J.W, J.H, xu, yu);
+            dme_mv lv;
+            dme_cost lc;
+            const int lq = unit_me(J, on1, b1, r1, pred1, g, gs, lam1, lv, lc);
+            // unit decisions, lane u for unit u
+            const int sa = (2 * lane) & 63, sb = (2 * lane + 1) & 63;
+            dme_cost cu[2] = {shfl_cost(lc, sa), shfl_cost(lc, sb)};
+            const int qa = __shfl(lq, sa, WAVE), qb = __shfl(lq, sb, WAVE);
+            const float lamu = __shfl(lam1, sa, WAVE);
+            const int bsrc = ((lane & 3) * 16 + (qa > 0 ? qa : 0) * 4 + (qb > 0 ? qb : 0)) & 63;
+            const int bv = __shfl(bitab, bsrc, WAVE);
+            const int bis = (nr > 1 && qa >= 0 && qb >= 0) ? bv : 0;
+            int mu;
+            const float ucost1 = unit_decn(J, 1, lamu, cu, bis, mu);
             float sb1 = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int xu = 2 * xs + (u & 1), yu = 2 * ys + (u >> 1);
-                const size_t ku = (size_t)yu * X1 + xu;
-                int bis = 0;
-                if (nr > 1 && q1[u][0] >= 0 && q1[u][1] >= 0) bis = J.m1tab[ku * 24 + 8 + q1[u][0] * 4 + q1[u][1]];
-                dme_cost c[2] = {c1[u][0], c1[u][1]};
-                sb1 += unit_decn(J, 1, J.lam1[ku], c, bis, m1[u]);
-            }
-            if (sb1 <= sbc) {
-                sbc = sb1;
-                split = 1;
-                if (lane < 16) {
-                    const int u = ((lane >> 3) << 1) | ((lane >> 1) & 1);
-                    const size_t d = (size_t)(4 * ys + (lane >> 2)) * X2 + 4 * xs + (lane & 3);
-                    int mu = m1[0];
-                    dme_mv a = v1[0][0], bb = v1[0][1];
-                    if (u == 1) { mu = m1[1]; a = v1[1][0]; bb = v1[1][1]; }
-                    if (u == 2) { mu = m1[2]; a = v1[2][0]; bb = v1[2][1]; }
-                    if (u == 3) { mu = m1[3]; a = v1[3][0]; bb = v1[3][1]; }
-                    J.mode[d] = mu;
-                    J.dc[0][d] = 0;
-                    J.vec[0][d] = a;
-                    if (nr > 1) J.vec[1][d] = bb;
-                }
-            }
-            // level 0: guides are the level-1 results; their SADs are those
-            // of the blocks they came from (a unit with no result offers (0,0),
-            // whose SAD the wave computes)
+            for (int i = 0; i < 4; ++i) sb1 += readlane_f(ucost1, i);
             {
-                const float lam = J.lam0[s];
-                const int *tab = J.m0tab + s * 32;
-                const Blk b0 = block_limits(J.blen[0], J.bsep[0], J.boff[0], J.W, J.H, xs, ys);
-                int gs0[2][4];
-#pragma unroll
-                for (int r = 0; r < 2; ++r)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        gs0[r][q] = 0;
-                        if (r < nr) {
-                            const int bq = q1[q][r];
-                            if (bq >= 0) {
-                                gs0[r][q] = tab[r * 16 + ((2 * (q >> 1) + (bq >> 1)) * 4 + 2 * (q & 1) + (bq & 1))];
-                            } else if (!b0.empty()) {
-                                int sd = 0;
-                                wave_up_sads(J.pic[0], J.W, J.up[r], 2 * J.W, 2 * J.H, b0, J.step, &v1[q][r], 1, &sd);
-                                gs0[r][q] = sd;
-                            }
-                        }
-                    }
-                dme_mv mv0;
-                mv0.x = mv0.y = 0;
-                dme_cost mc0;
-                mc0.SAD = mc0.mvcost = 0.0f;
-                mc0.total = 100000000.0f;
-                if (lane < nr) {
-                    const int r = lane;
-                    dme_mv g[4];
-                    int gs[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        g[q] = r ? v1[q][1] : v1[q][0];
-                        gs[q] = r ? gs0[1][q] : gs0[0][q];
-                    }
-                    unit_me(J, 0, xs, ys, r, g, gs, lam, mv0, mc0);
-                }
-                dme_mv v0[2] = {shfl_mv(mv0, 0), shfl_mv(mv0, 1)};
-                dme_cost c[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
-                int bis = 0;
-                if (nr > 1 && !b0.empty()) bis = wave_bi_sad(J, b0, v0[0], v0[1]);
-                int m0;
-                const float sb0 = unit_decn(J, 0, lam, c, bis, m0);
-                if (sb0 <= sbc) {
-                    sbc = sb0;
-                    split = 0;
+                // block lane of the superblock: its unit and that unit's result
+                const int bu = ((lane >> 3) << 1) | ((lane >> 1) & 1);
+                const int mb = __shfl(mu, bu & 3, WAVE);
+                const dme_mv va = shfl_mv(lv, (2 * bu) & 63), vb = shfl_mv(lv, (2 * bu + 1) & 63);
+                if (sb1 <= sbc) {
+                    sbc = sb1;
+                    split = 1;
                     if (lane < 16) {
                         const size_t d = (size_t)(4 * ys + (lane >> 2)) * X2 + 4 * xs + (lane & 3);
-                        J.mode[d] = m0;
+                        J.mode[d] = mb;
                         J.dc[0][d] = 0;
-                        J.vec[0][d] = v0[0];
-                        if (nr > 1) J.vec[1][d] = v0[1];
+                        J.vec[0][d] = va;
+                        if (nr > 1) J.vec[1][d] = vb;
                     }
+                }
+            }
+
+            // ---- level 0: guides are the level-1 results; their SADs are
+            // those of the blocks they came from (a unit with no result offers
+            // (0,0), whose SAD the wave computes)
+            const Blk b0 = block_limits(J.blen[0], J.bsep[0], J.boff[0], J.W, J.H, xs, ys);
+            const int r0 = lane & 1;
+            dme_mv g0[4];
+            int gs0[4];
+            bool miss[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                g0[q] = shfl_mv(lv, (2 * q + r0) & 63);
+                const int bq = __shfl(lq, (2 * q + r0) & 63, WAVE);
+                const int e = r0 * 16 + (2 * (q >> 1) + (bq > 0 ? bq >> 1 : 0)) * 4 + 2 * (q & 1) + (bq > 0 ? bq & 1 : 0);
+                const int tv = __shfl(m0v, e & 63, WAVE);
+                gs0[q] = bq >= 0 ? tv : 0;
+                miss[q] = lane < nr && bq < 0 && !b0.empty();
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                unsigned long long mm = __ballot(miss[q]);
+                while (mm) {
+                    const int src = __ffsll((long long)mm) - 1;
+                    mm &= mm - 1;
+                    dme_mv m;
+                    m.x = __builtin_amdgcn_readlane(g0[q].x, src);
+                    m.y = __builtin_amdgcn_readlane(g0[q].y, src);
+                    int sd = 0;
+                    wave_up_sads(J.pic[0], J.W, src ? J.up[1] : J.up[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
+                    if (lane == src) gs0[q] = sd;
+                }
+            }
+            dme_mv mv0;
+            dme_cost mc0;
+            unit_me(J, lane < nr, b0, r0, pred0, g0, gs0, lam0, mv0, mc0);
+            const dme_mv va0 = shfl_mv(mv0, 0), vb0 = shfl_mv(mv0, 1);
+            dme_cost c0[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
+            int bis0 = 0;
+            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<MODE_NB>(J, b0, va0, vb0);
+            int m0;
+            const float sb0 = unit_decn(J, 0, lam0, c0, bis0, m0);
+            if (sb0 <= sbc) {
+                sbc = sb0;
+                split = 0;
+                if (lane < 16) {
+                    const size_t d = (size_t)(4 * ys + (lane >> 2)) * X2 + 4 * xs + (lane & 3);
+                    J.mode[d] = m0;
+                    J.dc[0][d] = 0;
+                    J.vec[0][d] = va0;
+                    if (nr > 1) J.vec[1][d] = vb0;
                 }
             }
             if (lane == 0) {
