@@ -40,10 +40,8 @@ struct JobDesc {
     // pel search, per reference and level ([0] aliases the MEData arrays)
     dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];
     dme_cost DME_G *lpc[2][DME_MAX_DEPTH + 1];
-    int DME_G *bsad[2];                               // zero/guide-window best SAD
-    dme_mv DME_G *bmv[2];                             // ... and its vector
-    int DME_G *ptab[2];                               // pel SAD tables (see k_pel_bulk)
-    int DME_G *qtab[2];                               // sub-pel SAD tables (k_subpel_bulk)
+    int DME_G *ptab[2];                               // pel block records, wavefront order (k_pel_bulk)
+    int DME_G *qtab[2];                               // sub-pel block records, wavefront order (k_subpel_bulk)
     int DME_G *m1tab;                                 // level-1 unit SAD tables (k_mode_tables)
     int DME_G *m0tab;                                 // level-0 unit SAD tables
     // MEData of the picture

@@ -29,8 +29,8 @@ namespace dme {
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
-hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows,
-                           hipStream_t st);
+hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
+                           int max_blen_y, hipStream_t st);
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
@@ -99,7 +99,7 @@ struct JobState {
     Geometry g;
     int stage = 0;  // 1 pel, 2 lambda, 3 subpel, 4 mode
     DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3];
-    DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1], bsad[2], bmv[2];
+    DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1];
     DevBuf ptab[2], qtab[2], m1tab, m0tab;
     JobDesc desc;
     std::vector<float> h_lam, h_lam1, h_lam0;
@@ -269,24 +269,18 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     for (int r = 0; r < 2; ++r) {
         DME_TRY(J.vec[r].ensure(sizeof(dme_mv) * B));
         DME_TRY(J.pc[r].ensure(sizeof(dme_cost) * B));
-        size_t maxl = 1;
         for (int l = 1; l <= g.depth; ++l) {
             const size_t n = (size_t)g.lxb[l] * g.lyb[l];
             DME_TRY(J.lvec[r][l].ensure(sizeof(dme_mv) * n));
             DME_TRY(J.lpc[r][l].ensure(sizeof(dme_cost) * n));
-            maxl = std::max(maxl, n);
         }
-        maxl = std::max(maxl, B);
-        DME_TRY(J.bsad[r].ensure(sizeof(int) * maxl));
-        DME_TRY(J.bmv[r].ensure(sizeof(dme_mv) * maxl));
-        // pel tables: two (2R+1)^2 tables per block, R = min(l+1,5) + 1
+        // pel records: 28 ints per block of a level; sub-pel records: 56
         size_t maxt = 0;
         for (int l = 0; l <= g.depth; ++l) {
-            const size_t R = (size_t)std::min(l + 1, 5) + 1;
-            maxt = std::max(maxt, (size_t)g.lxb[l] * g.lyb[l] * 2 * (2 * R + 1) * (2 * R + 1));
+            maxt = std::max(maxt, (size_t)g.lxb[l] * g.lyb[l] * 28);
         }
         DME_TRY(J.ptab[r].ensure(sizeof(int) * maxt));
-        DME_TRY(J.qtab[r].ensure(sizeof(int) * 49 * B));
+        DME_TRY(J.qtab[r].ensure(sizeof(int) * 56 * B));
     }
     DME_TRY(J.m1tab.ensure(sizeof(int) * 24 * (B / 4)));
     DME_TRY(J.m0tab.ensure(sizeof(int) * 32 * (B / 16)));
@@ -332,8 +326,6 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
             d.lvec[r][l] = J.lvec[r][l].as<dme_mv>();
             d.lpc[r][l] = J.lpc[r][l].as<dme_cost>();
         }
-        d.bsad[r] = J.bsad[r].as<int>();
-        d.bmv[r] = J.bmv[r].as<dme_mv>();
         d.ptab[r] = J.ptab[r].as<int>();
         d.qtab[r] = J.qtab[r].as<int>();
     }
@@ -455,14 +447,18 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
     for (int l = maxd; l >= 0; --l) {
-        int rows = 1;
+        int rows = 1, bx = 1, by = 1;
         for (auto *j : js)
-            if (l <= j->g.depth) rows = std::max(rows, j->g.lyb[l]);
+            if (l <= j->g.depth) {
+                rows = std::max(rows, j->g.lyb[l]);
+                bx = std::max(bx, j->g.blen[2][0]);
+                by = std::max(by, j->g.blen[2][1]);
+            }
         DME_TRY(timed_launch(ctx, KK_PEL_BULK, launches, [&] {
             return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, ctx->stream);
         }));
         DME_TRY(timed_launch(ctx, KK_PEL_WAVE, launches,
-                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, ctx->stream); }));
+                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by, ctx->stream); }));
     }
     for (auto *j : js) j->stage = 1;
     return DME_OK;

@@ -263,6 +263,71 @@ __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Wavefront order.  The wavefront kernels visit block (x, y) of an X x Y grid
+// at step t = x + 2y, so the bulk kernels store each block's record at its
+// rank in the order (t, y): the records a step needs are then one contiguous
+// run, which the workgroup copies into LDS with coalesced loads.
+// ---------------------------------------------------------------------------
+// number of blocks with x + 2y < t: the sum over rows of clamp(t - 2y, 0, X)
+__host__ __device__ __forceinline__ int wf_before(int t, int X, int Y) {
+    if (t <= 0) return 0;
+    const int a = t >= X ? min((t - X) / 2 + 1, Y) : 0;  // rows entirely before t
+    const int c = min((t + 1) / 2, Y);                   // rows with some block before t
+    const int m = c - a;
+    return a * X + m * t - m * (a + c - 1);
+}
+// first row that has a block on diagonal t
+__host__ __device__ __forceinline__ int wf_ylo(int t, int X) { return t - X + 1 > 0 ? (t - X + 2) / 2 : 0; }
+__host__ __device__ __forceinline__ int wf_pos(int x, int y, int X, int Y) {
+    const int t = x + 2 * y;
+    return wf_before(t, X, Y) + y - wf_ylo(t, X);
+}
+
+// Workgroup copy of diagonal tn's records (rec ints each) from src into the
+// LDS slots of bank tn & 1: the first PF loads per thread are issued by
+// wf_issue before a step's work and stored by wf_finish after it.
+#define WF_PF 16
+struct WfCopy {
+    int v[WF_PF];
+    int len;
+    const int *src;
+    int *dst;
+};
+
+__device__ __forceinline__ void wf_issue(WfCopy &c, const int *__restrict__ tab, int *slots, int rec, int max_rows,
+                                         int tn, int X, int Y) {
+    const int r0 = wf_before(tn, X, Y);
+    c.len = (wf_before(tn + 1, X, Y) - r0) * rec;
+    c.src = tab + (size_t)r0 * rec;
+    c.dst = slots + ((tn & 1) * max_rows + wf_ylo(tn, X)) * rec;
+#pragma unroll
+    for (int k = 0; k < WF_PF; ++k) {
+        const int i = k * blockDim.x + threadIdx.x;
+        c.v[k] = i < c.len ? c.src[i] : 0;
+    }
+}
+
+__device__ __forceinline__ void wf_finish(WfCopy &c) {
+#pragma unroll
+    for (int k = 0; k < WF_PF; ++k) {
+        const int i = k * blockDim.x + threadIdx.x;
+        if (i < c.len) c.dst[i] = c.v[k];
+    }
+    for (int b0 = WF_PF * blockDim.x; b0 < c.len; b0 += WF_PF * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < WF_PF; ++k) {
+            const int i = b0 + k * blockDim.x + threadIdx.x;
+            c.v[k] = i < c.len ? c.src[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < WF_PF; ++k) {
+            const int i = b0 + k * blockDim.x + threadIdx.x;
+            if (i < c.len) c.dst[i] = c.v[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Memory-level parallelism.  A wavefront step is a short chain of dependent
 // operations, so its global loads must be issued together, not one per loop
 // iteration: the copies below keep 32 loads in flight per lane.
@@ -308,18 +373,16 @@ __device__ __forceinline__ void wave_copy_rect(int16_t *__restrict__ dst, int pi
 }
 
 // ---------------------------------------------------------------------------
-// Pel search, bulk part.  For every block of a level, two SAD tables of
-// radius R = r + 1 (r = min(level+1, 5), pixel_match.cpp:258-259): around
-// the zero vector (T_Z) and around the guide vector (T_G, levels below the
-// coarsest).  The zero and guide candidate lists of the reference
-// (pixel_match.cpp:304, 351-364) are the radius-r cores of these tables, so
-// their first minimum A (in enumeration order) is taken here; the wavefront
-// then only has to look up the prediction window, which falls inside one of
-// the tables for most blocks.  One wave per block; picture block and both
+// Pel search, bulk part.  For every block of a level, the SADs of every
+// candidate within R = r + 1 (r = min(level+1, 5), pixel_match.cpp:258-259)
+// of the zero vector and of the guide vector (levels below the coarsest) are
+// computed in LDS.  The zero and guide candidate lists of the reference
+// (pixel_match.cpp:304, 351-364) are the radius-r windows at the two centres,
+// so their first minimum (in enumeration order) is taken here.  So is the
+// first minimum of every r-window centred within one pel of zero or guide:
+// the prediction list of most blocks is one of them, and the wavefront then
+// answers it with one lookup.  One wave per block; picture block and both
 // reference regions are staged in LDS.
-//
-// Table layout: ptab[r][(block * 2 + t) * D*D + (dy + R) * D + (dx + R)],
-// D = 2R + 1, t = 0 zero-centred, 1 guide-centred; INT_MAX where absent.
 // ---------------------------------------------------------------------------
 #define PEL_MAXB 32                        // largest block side (host-checked)
 #define PEL_MAXR 6
@@ -327,12 +390,21 @@ __device__ __forceinline__ void wave_copy_rect(int16_t *__restrict__ dst, int pi
 #define PEL_MAXTS ((2 * PEL_MAXR + 1) * (2 * PEL_MAXR + 1))
 
 __device__ __forceinline__ int pel_radius(int level) { return min(level + 1, 5); }
+// Pel record of a block (PEL_REC ints, blocks in wavefront order):
+//   [0] zero/guide-window best SAD, [1..2] its vector, [3..4] guide vector,
+//   [8 + t*9 + (dy+1)*3 + dx+1] key of the prediction window centred at
+//   (dx, dy) from the centre of table t (0: zero, 1: guide), for the nine
+//   predictions within one pel of the centre.  A key is SAD * 128 + k with k
+//   the candidate's index in AddNewVlist order, so the smallest key is the
+//   first strict-< minimum of FindBestMatchPel.
+#define PEL_REC 28
 
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total, int level) {
     __shared__ int16_t s_pic[4][PEL_MAXB * PEL_MAXB];
     __shared__ int16_t s_reg[4][2][PEL_REG * PEL_REG];
     __shared__ int s_tab[4][2 * PEL_MAXTS];
+    __shared__ int s_key[4][18];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gi = blockIdx.x * 4 + wv;
     const bool valid = gi < total;
@@ -358,6 +430,7 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     }
     const bool work = valid && !b.empty();
     const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
+    int *__restrict__ rec = J.ptab[r] + (size_t)wf_pos(xb, yb, X, J.lyb[level]) * PEL_REC;
     if (work) {
         wave_copy_rect(s_pic[wv], PEL_MAXB, pic, W, H, b.xp, b.yp, b.xl, b.yl);
         wave_copy_rect(s_reg[wv][0], RW, ref, W, H, b.xp - R, b.yp - R, RW, RH);
@@ -379,7 +452,19 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
                 }
             }
             s_tab[wv][c] = s;
-            J.ptab[r][(size_t)local * 2 * TS + c] = s;
+        }
+    }
+    if (lane < 18) s_key[wv][lane] = INT_MAX;
+    __syncthreads();
+    const int n = (2 * rad + 1) * (2 * rad + 1);
+    if (work) {
+        const int nw = (has_guide ? 2 : 1) * 9 * n;
+        for (int i = lane; i < nw; i += WAVE) {
+            const int tw = i / n, c = i - tw * n;  // tw = table * 9 + window
+            const int t = tw >= 9 ? 1 : 0, w = tw - 9 * t;
+            const int ddx = w % 3 - 1, ddy = w / 3 - 1;
+            const dme_mv o = win_off(c, rad);
+            atomicMin(&s_key[wv][tw], s_tab[wv][t * TS + (ddy + o.y + R) * D + (ddx + o.x + R)] * 128 + c);
         }
     }
     __syncthreads();
@@ -388,30 +473,27 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     best_mv.x = 0;
     best_mv.y = 0;
     if (work) {
-        // first minimum over the zero list then the guide list (radius rad)
-        const int n = (2 * rad + 1) * (2 * rad + 1);
-        const int nc = has_guide ? 2 * n : n;
-        unsigned long long key = ~0ull;
-        for (int k = lane; k < nc; k += WAVE) {
-            const int t = k >= n ? 1 : 0;
-            const dme_mv o = win_off(k - t * n, rad);
-            const int s = s_tab[wv][t * TS + (o.y + R) * D + (o.x + R)];
-            const unsigned long long kk = ((unsigned long long)(unsigned)s << 32) | (unsigned)k;
-            key = kk < key ? kk : key;
-        }
-        key = wave_min_u64(key);
-        const int k = (int)(key & 0xffffffffu);
-        best_sad = (int)(key >> 32);
-        const int t = k >= n ? 1 : 0;
-        best_mv = win_off(k - t * n, rad);
-        if (t) {
-            best_mv.x += g.x;
-            best_mv.y += g.y;
+        // zero list, then guide list (radius rad, centred windows)
+        const int k0 = s_key[wv][4];
+        best_sad = k0 >> 7;
+        best_mv = win_off(k0 & 127, rad);
+        if (has_guide) {
+            const int k1 = s_key[wv][13];
+            if ((k1 >> 7) < best_sad) {
+                best_sad = k1 >> 7;
+                best_mv = win_off(k1 & 127, rad);
+                best_mv.x += g.x;
+                best_mv.y += g.y;
+            }
         }
     }
+    if (valid && lane < 18) rec[8 + lane] = s_key[wv][lane];
     if (valid && lane == 0) {
-        J.bsad[r][local] = best_sad;
-        J.bmv[r][local] = best_mv;
+        rec[0] = best_sad;
+        rec[1] = best_mv.x;
+        rec[2] = best_mv.y;
+        rec[3] = g.x;
+        rec[4] = g.y;
     }
 }
 
@@ -419,81 +501,67 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
 // Pel search, wavefront part (PixelMatcher::MatchPic/DoBlock,
 // pixel_match.cpp:242-380; BlockMatcher::FindBestMatchPel,
 // block_match.cpp:242-275).  One workgroup per (picture, reference), one
-// lane per block row: at step t row y works on block x = t - 2y, reading the
-// left, top and top-right results from a 3-slot LDS ring per row.  While it
-// works on block x, the lane prefetches block x+1's zero/guide winner, guide
-// and SAD tables into an LDS slot, so a step's lookups never wait on global
-// memory.  A prediction window outside both tables is computed by the lane's
-// wave from an LDS copy of the picture block and reference window.
+// thread per block row: at step t row y works on block x = t - 2y, reading
+// the left, top and top-right results from a 3-slot LDS ring per row.  While
+// the step runs, the workgroup copies the records of diagonal t + 1 into the
+// other LDS bank.  A prediction within one pel of zero or of the guide is
+// answered by one lookup of its window key.  The other predictions of the
+// step ("misses") are queued in LDS and the whole workgroup computes their
+// windows together: it stages each block and its reference window, sums row
+// SADs per candidate with LDS atomics and takes the smallest key.
 // ---------------------------------------------------------------------------
-#define PEL_MISS_BUF 3072  // int16 per wave: block (<= 32x32), window (<= 42x42 at r = 5), 128 sums
-
-__host__ __device__ __forceinline__ int pel_slot_ints(int TS) { return 8 + 2 * TS; }
+struct PelMiss {
+    int row, xp, yp, xl, yl, px, py, key;
+};
 
 __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                   int level, int max_rows) {
+                                                   int level, int max_rows, int miss_cap, int stage_elems) {
     extern __shared__ int s_dyn[];
+    __shared__ int s_nmiss;
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (level > J.depth || tk.ref >= J.nrefs) return;  // uniform per workgroup
     const int r = tk.ref;
     const int X = J.lxb[level], Y = J.lyb[level];
     const int W = J.lw[level], H = J.lh[level];
-    const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
+    const int rad = pel_radius(level);
     const int n = (2 * rad + 1) * (2 * rad + 1);
     const bool has_guide = level < J.depth;
-    const int GX = has_guide ? J.lxb[level + 1] : 1, GY = has_guide ? J.lyb[level + 1] : 1;
-    const dme_mv *__restrict__ G = has_guide ? J.lvec[r][level + 1] : nullptr;
     const int16_t *__restrict__ pic = J.pic[level];
     const int16_t *__restrict__ ref = J.ref[r][level];
     const int *__restrict__ tab = J.ptab[r];
-    const int *__restrict__ bsad = J.bsad[r];
-    const dme_mv *__restrict__ bmv = J.bmv[r];
     dme_mv *__restrict__ V = J.lvec[r][level];
     dme_cost *__restrict__ C = J.lpc[r][level];
     int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
     int boff[2] = {J.boff[2][0], J.boff[2][1]};
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int SL = pel_slot_ints(TS);
-    // LDS: ring [max_rows][3] mv | slots [2][max_rows][SL] | miss buffers [waves][PEL_MISS_BUF] int16
+    const int YL = blen[1];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    // LDS: ring [max_rows][3] mv | slots [2][max_rows][PEL_REC] | queue [miss_cap] |
+    //      sums [miss_cap][n] | staging [miss_cap][stage_elems] int16
     dme_mv *ring = reinterpret_cast<dme_mv *>(s_dyn);
     int *slots = s_dyn + 6 * max_rows;
-    int16_t *missbuf = reinterpret_cast<int16_t *>(slots + 2 * max_rows * SL) + wv * PEL_MISS_BUF;
+    PelMiss *queue = reinterpret_cast<PelMiss *>(slots + 2 * max_rows * PEL_REC);
+    int *msum = reinterpret_cast<int *>(queue + miss_cap);
+    int16_t *mstage = reinterpret_cast<int16_t *>(msum + miss_cap * n);
+    if (tid == 0) s_nmiss = 0;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = -1; t <= T; ++t) {
-        for (int base = 0; base < Y; base += blockDim.x) {
-            const int yb = base + threadIdx.x;
+        WfCopy cp;
+        wf_issue(cp, tab, slots, PEL_REC, max_rows, t + 1, X, Y);
+        for (int base = 0; base < Y; base += nth) {
+            const int yb = base + tid;
             const int xb = t - 2 * yb;
-            // prefetch block xb+1 of this row
-            if (yb < Y && xb + 1 >= 0 && xb + 1 < X) {
-                const int xn = xb + 1;
-                const size_t kn = (size_t)yb * X + xn;
-                int *sl = slots + ((xn & 1) * max_rows + yb) * SL;
-                sl[0] = bsad[kn];
-                const dme_mv a = bmv[kn];
-                sl[1] = a.x;
-                sl[2] = a.y;
-                int gx = 0, gy = 0;
-                if (has_guide) {
-                    const dme_mv gv = G[(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xn >> 1, 0, GX - 1)];
-                    gx = gv.x * 2;
-                    gy = gv.y * 2;
-                }
-                sl[3] = gx;
-                sl[4] = gy;
-                copy_ints(sl + 8, tab + kn * 2 * TS, has_guide ? 2 * TS : TS);
-            }
             const bool active = yb < Y && xb >= 0 && xb < X;
             dme_mv pred, best;
             pred.x = pred.y = 0;
             best = pred;
-            int best_sad = 10000000;
-            bool miss = false;
+            int best_sad = 10000000, slot = -1;
             Blk b;
             b.xp = b.yp = 0;
             b.xl = b.yl = 0;
+            __syncthreads();  // the queue counter is reset; ring and bank writes are visible
             if (active) {
-                const int *sl = slots + ((xb & 1) * max_rows + yb) * SL;
+                const int *sl = slots + ((xb & 1) * max_rows + yb) * PEL_REC;
                 // prediction rules (pixel_match.cpp:314-347)
                 if (yb == 0) {
                     if (xb > 0) pred = ring[yb * 3 + (t - 1) % 3];
@@ -511,85 +579,102 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 b = block_limits(blen, bsep, boff, W, H, xb, yb);
                 if (!b.empty()) {
                     const int gx = sl[3], gy = sl[4];
-                    const int *tb = nullptr;
-                    int cx = 0, cy = 0;
-                    if (has_guide && abs(pred.x - gx) <= 1 && abs(pred.y - gy) <= 1) {
-                        tb = sl + 8 + TS;
-                        cx = gx;
-                        cy = gy;
-                    } else if (abs(pred.x) <= 1 && abs(pred.y) <= 1) {
-                        tb = sl + 8;
-                    }
-                    if (tb) {
-                        int ps = INT_MAX, pk = 0;
-                        for (int q = 0; q < n; ++q) {
-                            const dme_mv o = win_off(q, rad);
-                            const int s = tb[(pred.y + o.y - cy + R) * D + (pred.x + o.x - cx + R)];
-                            if (s < ps) {
-                                ps = s;
-                                pk = q;
-                            }
-                        }
-                        if (ps < best_sad) {
-                            best_sad = ps;
-                            best = win_off(pk, rad);
+                    int key = -1;
+                    if (has_guide && abs(pred.x - gx) <= 1 && abs(pred.y - gy) <= 1)
+                        key = sl[17 + (pred.y - gy + 1) * 3 + (pred.x - gx + 1)];
+                    else if (abs(pred.x) <= 1 && abs(pred.y) <= 1)
+                        key = sl[8 + (pred.y + 1) * 3 + (pred.x + 1)];
+                    if (key >= 0) {
+                        if ((key >> 7) < best_sad) {
+                            best_sad = key >> 7;
+                            best = win_off(key & 127, rad);
                             best.x += pred.x;
                             best.y += pred.y;
                         }
                     } else {
-                        miss = true;
+                        slot = atomicAdd(&s_nmiss, 1);
                     }
                 }
             }
-            // prediction windows outside the tables: the wave stages the block
-            // and the window in LDS and evaluates all candidates together
-            unsigned long long mm = __ballot(miss);
-            while (mm) {
-                const int src = __ffsll((long long)mm) - 1;
-                mm &= mm - 1;
-                const int xp = __shfl(b.xp, src, WAVE), yp = __shfl(b.yp, src, WAVE);
-                const int xl = __shfl(b.xl, src, WAVE), yl = __shfl(b.yl, src, WAVE);
-                const int px = __shfl(pred.x, src, WAVE), py = __shfl(pred.y, src, WAVE);
-                const int RW = xl + 2 * rad, RH = yl + 2 * rad;
-                int16_t *bp = missbuf, *br = missbuf + xl * yl;
-                int *msad = reinterpret_cast<int *>(missbuf + PEL_MISS_BUF) - 128;  // n <= 121 sums
-                wave_copy_rect(bp, xl, pic, W, H, xp, yp, xl, yl);
-                wave_copy_rect(br, RW, ref, W, H, xp + px - rad, yp + py - rad, RW, RH);
-                for (int c = lane; c < n; c += WAVE) msad[c] = 0;
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                // (candidate, row) items spread over the lanes
-                for (int it = lane; it < n * yl; it += WAVE) {
-                    const int c = it / yl, y = it - c * yl;
-                    const dme_mv o = win_off(c, rad);
-                    const int16_t *rg = br + (y + o.y + rad) * RW + (o.x + rad);
-                    const int16_t *pr = bp + y * xl;
-                    int s = 0;
-                    for (int x = 0; x < xl; ++x) s += abs((int)pr[x] - (int)rg[x]);
-                    atomicAdd(&msad[c], s);
+            __syncthreads();
+            const int nmiss = s_nmiss;
+            __syncthreads();
+            if (tid == 0) s_nmiss = 0;
+            for (int m0 = 0; m0 < nmiss; m0 += miss_cap) {
+                const int nm = min(miss_cap, nmiss - m0);
+                if (slot >= m0 && slot < m0 + nm) {
+                    PelMiss e;
+                    e.row = yb;
+                    e.xp = b.xp;
+                    e.yp = b.yp;
+                    e.xl = b.xl;
+                    e.yl = b.yl;
+                    e.px = pred.x;
+                    e.py = pred.y;
+                    e.key = INT_MAX;
+                    queue[slot - m0] = e;
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                unsigned long long key = ~0ull;
-                for (int c = lane; c < n; c += WAVE) {
-                    const unsigned long long kk = ((unsigned long long)(unsigned)msad[c] << 32) | (unsigned)c;
-                    key = kk < key ? kk : key;
+                __syncthreads();
+                // stage each block (pitch xl) and its window (pitch xl + 2 rad)
+                for (int i0 = 0; i0 < nm * stage_elems; i0 += 8 * nth) {
+                    int16_t v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int i = i0 + k * nth + tid;
+                        v[k] = 0;
+                        if (i < nm * stage_elems) {
+                            const int e = i / stage_elems, j = i - e * stage_elems;
+                            const PelMiss q = queue[e];
+                            const int nb = q.xl * q.yl, RW = q.xl + 2 * rad, RH = q.yl + 2 * rad;
+                            if (j < nb) {
+                                const int dy = j / q.xl;
+                                v[k] = pic[(size_t)(q.yp + dy) * W + q.xp + j - dy * q.xl];
+                            } else if (j - nb < RW * RH) {
+                                const int j2 = j - nb, ry = j2 / RW;
+                                v[k] = ref[(size_t)clampi(q.yp + q.py - rad + ry, 0, H - 1) * W +
+                                           clampi(q.xp + q.px - rad + j2 - ry * RW, 0, W - 1)];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int i = i0 + k * nth + tid;
+                        if (i < nm * stage_elems) mstage[i] = v[k];
+                    }
                 }
-                key = wave_min_u64(key);
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                if (lane == src) {
-                    const int s = (int)(key >> 32);
-                    if (s < best_sad) {
-                        best_sad = s;
-                        best = win_off((int)(key & 0xffffffffu), rad);
+                for (int i = tid; i < nm * n; i += nth) msum[i] = 0;
+                __syncthreads();
+                // row SADs of every (miss, candidate, row)
+                for (int i = tid; i < nm * n * YL; i += nth) {
+                    const int e = i / (n * YL), rem = i - e * n * YL;
+                    const int c = rem / YL, y = rem - c * YL;
+                    const PelMiss q = queue[e];
+                    if (y < q.yl) {
+                        const dme_mv o = win_off(c, rad);
+                        const int RW = q.xl + 2 * rad;
+                        const int16_t *bp = mstage + e * stage_elems + y * q.xl;
+                        const int16_t *rp = mstage + e * stage_elems + q.xl * q.yl + (y + o.y + rad) * RW + o.x + rad;
+                        int sd = 0;
+                        for (int x = 0; x < q.xl; ++x) sd += abs((int)bp[x] - (int)rp[x]);
+                        atomicAdd(&msum[e * n + c], sd);
+                    }
+                }
+                __syncthreads();
+                for (int i = tid; i < nm * n; i += nth) {
+                    const int e = i / n, c = i - e * n;
+                    atomicMin(&queue[e].key, msum[i] * 128 + c);
+                }
+                __syncthreads();
+                if (slot >= m0 && slot < m0 + nm) {
+                    const int key = queue[slot - m0].key;
+                    if ((key >> 7) < best_sad) {
+                        best_sad = key >> 7;
+                        best = win_off(key & 127, rad);
                         best.x += pred.x;
                         best.y += pred.y;
                     }
                 }
+                __syncthreads();
             }
             if (active) {
                 const size_t k = (size_t)yb * X + xb;
@@ -602,6 +687,7 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 ring[yb * 3 + t % 3] = best;
             }
         }
+        wf_finish(cp);
         __syncthreads();
     }
 }
@@ -786,9 +872,12 @@ __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
 // stage 2 one quarter pel more.  Their integer SADs do not depend on the
 // prediction, so all (2*3+1)^2 of them are tabulated per block here, from an
 // LDS copy of the up-converted reference region; the wavefront then only
-// adds costs and compares.  Layout: qtab[r][block * QS + (dy+rad)*Q + dx+rad].
+// adds costs and compares.  Record layout (SUB_SLOT ints per block, blocks in
+// wavefront order): SAD (dy+rad)*Q + dx+rad, then lambda, pel vector and pel
+// costs at 49..54.
 // ---------------------------------------------------------------------------
 #define SUB_MAXREG (2 * PEL_MAXB + 3)  // 67
+#define SUB_SLOT 56                    // 49 SADs, lambda, pel vector, pel costs
 
 __device__ __forceinline__ int sub_radius(int prec) { return prec == 2 ? 3 : 1; }
 
@@ -808,6 +897,7 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
     const bool work = valid && !b.empty() && prec > 0;
     const dme_mv pel = J.vec[r][k];
+    int *__restrict__ rec = J.qtab[r] + (size_t)wf_pos(xb, yb, X, J.ynb) * SUB_SLOT;
     dme_mv c;
     c.x = pel.x << prec;
     c.y = pel.y << prec;
@@ -850,7 +940,16 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
                 s += abs(v - (int)pr[x]);
             }
         }
-        J.qtab[r][(size_t)k * QS + lane] = s;
+        rec[lane] = s;
+    }
+    if (valid && lane == 0) {
+        rec[49] = __float_as_int(J.lam[k]);
+        rec[50] = pel.x;
+        rec[51] = pel.y;
+        const dme_cost pc = J.pc[r][k];
+        rec[52] = __float_as_int(pc.SAD);
+        rec[53] = __float_as_int(pc.mvcost);
+        rec[54] = __float_as_int(pc.total);
     }
 }
 
@@ -883,11 +982,10 @@ __device__ float cand_total(const JobDesc &J, const int16_t *up, int UW, int UH,
 // me_subpel.cpp:101-176; BlockMatcher::RefineMatchSubp,
 // block_match.cpp:327-467).  One lane per block row, wavefront t = x + 2y;
 // the left, top-left, top and top-right refined vectors come from a 4-slot
-// LDS ring per row; each lane copies its block's SAD table into its own LDS
-// slot.  Predictor SADs outside the table are computed by the whole wave.
+// LDS ring per row; the workgroup copies the records of diagonal t + 1 into
+// the other LDS bank while step t runs.  Predictor SADs outside the table are
+// computed by the whole wave.
 // ---------------------------------------------------------------------------
-#define SUB_SLOT 56  // 49 SADs, lambda, pel vector, pel costs
-
 __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                       int max_rows) {
     extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT]
@@ -901,7 +999,6 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
     int *slots = s_dyn + 8 * max_rows;
     const int16_t *__restrict__ up = J.up[r];
     const int *__restrict__ qtab = J.qtab[r];
-    const float *__restrict__ lamv = J.lam;
     dme_mv *__restrict__ V = J.vec[r];
     dme_cost *__restrict__ C = J.pc[r];
     int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
@@ -909,23 +1006,11 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
     const int lane = threadIdx.x & 63;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = -1; t <= T; ++t) {
+        WfCopy cp;
+        wf_issue(cp, qtab, slots, SUB_SLOT, max_rows, t + 1, X, Y);
         for (int base = 0; base < Y; base += blockDim.x) {
             const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
-            // prefetch block xb+1 of this row: its SAD table, lambda, pel result
-            if (yb < Y && xb + 1 >= 0 && xb + 1 < X) {
-                const size_t kn = (size_t)yb * X + xb + 1;
-                int *sl = slots + (((xb + 1) & 1) * max_rows + yb) * SUB_SLOT;
-                copy_ints(sl, qtab + kn * QS, QS);
-                sl[49] = __float_as_int(lamv[kn]);
-                const dme_mv pv = V[kn];
-                sl[50] = pv.x;
-                sl[51] = pv.y;
-                const dme_cost pc = C[kn];
-                sl[52] = __float_as_int(pc.SAD);
-                sl[53] = __float_as_int(pc.mvcost);
-                sl[54] = __float_as_int(pc.total);
-            }
             const bool active = yb < Y && xb >= 0 && xb < X;
             const size_t k = active ? (size_t)yb * X + xb : 0;
             const int *mytab = slots + (((xb & 1) & 1) * max_rows + (active ? yb : 0)) * SUB_SLOT;
@@ -1084,6 +1169,7 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                 ring[yb * 4 + t % 4] = out_mv;
             }
         }
+        wf_finish(cp);
         __syncthreads();
     }
 }
@@ -1700,22 +1786,27 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     return hipGetLastError();
 }
 
-hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows,
-                           hipStream_t st) {
+hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
+                           int max_blen_y, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
-    const int threads = min(1024, ((max_rows + 63) / 64) * 64);
-    const int R = min(level + 1, 5) + 1, TS = (2 * R + 1) * (2 * R + 1);
-    const size_t lds = sizeof(int) * (6 * (size_t)max_rows + 2 * (size_t)max_rows * pel_slot_ints(TS)) +
-                       sizeof(int16_t) * PEL_MISS_BUF * (threads / 64);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int threads = 1024;
+    const int rad = min(level + 1, 5), n = (2 * rad + 1) * (2 * rad + 1);
+    const int stage_elems = (max_blen_x * max_blen_y + (max_blen_x + 2 * rad) * (max_blen_y + 2 * rad) + 1) & ~1;
+    const size_t fixed = sizeof(int) * (6 * (size_t)max_rows + 2 * (size_t)max_rows * PEL_REC);
+    const size_t per_miss = sizeof(PelMiss) + sizeof(int) * n + sizeof(int16_t) * stage_elems;
+    const size_t budget = 160 * 1024 - 1024;  // static LDS of the kernel stays below 1 KiB
+    if (fixed + per_miss > budget) return hipErrorInvalidValue;
+    const int miss_cap = (int)min((size_t)256, (budget - fixed) / per_miss);
+    const size_t lds = fixed + per_miss * miss_cap;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)k_pel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
+                                           (int)budget);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, level, max_rows);
+    hipLaunchKernelGGL(k_pel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, level, max_rows, miss_cap,
+                       stage_elems);
     return hipGetLastError();
 }
 
