@@ -286,41 +286,43 @@ __host__ __device__ __forceinline__ int wf_pos(int x, int y, int X, int Y) {
 // Workgroup copy of diagonal tn's records (rec ints each) from src into the
 // LDS slots of bank tn & 1: the first PF loads per thread are issued by
 // wf_issue before a step's work and stored by wf_finish after it.
-#define WF_PF 16
+template <int PF>
 struct WfCopy {
-    int v[WF_PF];
+    int v[PF];
     int len;
     const int *src;
     int *dst;
 };
 
-__device__ __forceinline__ void wf_issue(WfCopy &c, const int *__restrict__ tab, int *slots, int rec, int max_rows,
+template <int PF>
+__device__ __forceinline__ void wf_issue(WfCopy<PF> &c, const int *__restrict__ tab, int *slots, int rec, int max_rows,
                                          int tn, int X, int Y) {
     const int r0 = wf_before(tn, X, Y);
     c.len = (wf_before(tn + 1, X, Y) - r0) * rec;
     c.src = tab + (size_t)r0 * rec;
     c.dst = slots + ((tn & 1) * max_rows + wf_ylo(tn, X)) * rec;
 #pragma unroll
-    for (int k = 0; k < WF_PF; ++k) {
+    for (int k = 0; k < PF; ++k) {
         const int i = k * blockDim.x + threadIdx.x;
         c.v[k] = i < c.len ? c.src[i] : 0;
     }
 }
 
-__device__ __forceinline__ void wf_finish(WfCopy &c) {
+template <int PF>
+__device__ __forceinline__ void wf_finish(WfCopy<PF> &c) {
 #pragma unroll
-    for (int k = 0; k < WF_PF; ++k) {
+    for (int k = 0; k < PF; ++k) {
         const int i = k * blockDim.x + threadIdx.x;
         if (i < c.len) c.dst[i] = c.v[k];
     }
-    for (int b0 = WF_PF * blockDim.x; b0 < c.len; b0 += WF_PF * blockDim.x) {
+    for (int b0 = PF * blockDim.x; b0 < c.len; b0 += PF * blockDim.x) {
 #pragma unroll
-        for (int k = 0; k < WF_PF; ++k) {
+        for (int k = 0; k < PF; ++k) {
             const int i = b0 + k * blockDim.x + threadIdx.x;
             c.v[k] = i < c.len ? c.src[i] : 0;
         }
 #pragma unroll
-        for (int k = 0; k < WF_PF; ++k) {
+        for (int k = 0; k < PF; ++k) {
             const int i = b0 + k * blockDim.x + threadIdx.x;
             if (i < c.len) c.dst[i] = c.v[k];
         }
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
     if (tid == 0) s_nmiss = 0;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = -1; t <= T; ++t) {
-        WfCopy cp;
+        WfCopy<4> cp;
         wf_issue(cp, tab, slots, PEL_REC, max_rows, t + 1, X, Y);
         for (int base = 0; base < Y; base += nth) {
             const int yb = base + tid;
@@ -597,9 +599,7 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 }
             }
             __syncthreads();
-            const int nmiss = s_nmiss;
-            __syncthreads();
-            if (tid == 0) s_nmiss = 0;
+            const int nmiss = s_nmiss;  // reset after the rounds; the next pass starts with a barrier
             for (int m0 = 0; m0 < nmiss; m0 += miss_cap) {
                 const int nm = min(miss_cap, nmiss - m0);
                 if (slot >= m0 && slot < m0 + nm) {
@@ -676,6 +676,7 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 }
                 __syncthreads();
             }
+            if (tid == 0 && nmiss) s_nmiss = 0;
             if (active) {
                 const size_t k = (size_t)yb * X + xb;
                 V[k] = best;
@@ -980,40 +981,115 @@ __device__ float cand_total(const JobDesc &J, const int16_t *up, int UW, int UH,
 // ---------------------------------------------------------------------------
 // Sub-pel refinement, wavefront part (SubpelRefine::MatchPic/DoBlock/GetPred,
 // me_subpel.cpp:101-176; BlockMatcher::RefineMatchSubp,
-// block_match.cpp:327-467).  One lane per block row, wavefront t = x + 2y;
+// block_match.cpp:327-467).  One thread per block row, wavefront t = x + 2y;
 // the left, top-left, top and top-right refined vectors come from a 4-slot
-// LDS ring per row; the workgroup copies the records of diagonal t + 1 into
-// the other LDS bank while step t runs.  Predictor SADs outside the table are
-// computed by the whole wave.
+// LDS ring per row, and the workgroup copies the records of diagonal t + 1
+// into the other LDS bank while step t runs.  Candidate SADs come from the
+// record's table.  What the table cannot answer is queued in LDS and taken
+// by the workgroup's waves, one entry per wave: predictor SADs outside the
+// table, and totals that need the per-pixel float chain.  The four neighbour
+// tests of a stage do not depend on each other's outcome, so their chains are
+// queued together, then those of the up-to-two second-round tests.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+struct SubJob {
+    int xp, yp, xl, yl;  // block
+    int mx, my, step;    // vector and its units (1: half pel, 2: quarter pel)
+    int ires;            // integer SAD (predictor SADs)
+    float start, fres;   // chain start and result
+    int pad[2];
+};
+#define SUB_QCAP 256
+
+// One queue round trip: every thread with want set gets a slot, the waves
+// serve the entries, each thread gets its result back.  Uniform call.  Round
+// trips alternate between two counters: a counter is read after the first
+// barrier of its round trip and reset (when it was used) before the next
+// round trip's first barrier, which every thread passes before the round trip
+// after that touches the counter again; an empty round trip costs one barrier.
+template <bool CHAIN>
+__device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, const JobDesc &J,
+                                          const int16_t *__restrict__ up, const SubJob (&mine)[4],
+                                          const bool (&want)[4], int (&ires)[4], float (&fres)[4]) {
+    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wv = tid >> 6, nwv = nth >> 6;
+    int *cnt = cnts + phase;
+    phase ^= 1;
+    int slot[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) slot[j] = want[j] ? atomicAdd(cnt, 1) : -1;
+    __syncthreads();
+    const int nq = *cnt;
+    if (nq == 0) return;  // uniform
+    for (int m0 = 0; m0 < nq; m0 += SUB_QCAP) {
+        const int nm = min(SUB_QCAP, nq - m0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (slot[j] >= m0 && slot[j] < m0 + nm) queue[slot[j] - m0] = mine[j];
+        __syncthreads();
+        const int UW = 2 * J.W, UH = 2 * J.H;
+        for (int e = wv; e < nm; e += nwv) {
+            const SubJob q = queue[e];
+            Blk b;
+            b.xp = q.xp;
+            b.yp = q.yp;
+            b.xl = q.xl;
+            b.yl = q.yl;
+            dme_mv m;
+            m.x = q.mx;
+            m.y = q.my;
+            if (CHAIN) {
+                const UpStart us = up_start(b, m, q.step, UW, UH);
+                const float v = wave_chain(q.start, J.pic[0], J.W, up, UW, UH, b, us);
+                if (lane == 0) queue[e].fres = v;
+            } else {
+                int v = 0;
+                wave_up_sads(J.pic[0], J.W, up, UW, UH, b, q.step, &m, 1, &v);
+                if (lane == 0) queue[e].ires = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (slot[j] >= m0 && slot[j] < m0 + nm) {
+                ires[j] = queue[slot[j] - m0].ires;
+                fres[j] = queue[slot[j] - m0].fres;
+            }
+        __syncthreads();
+    }
+    if (tid == 0) *cnt = 0;
+}
+
+__global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                       int max_rows) {
-    extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT]
+    extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT] | queue [SUB_QCAP]
+    __shared__ int s_cnt[2];
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (tk.ref >= J.nrefs || J.prec == 0) return;
-    const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
+    const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1;
     const int X = J.xnb, Y = J.ynb, W = J.W, H = J.H;
     const int UW = 2 * W, UH = 2 * H;
     dme_mv *ring = reinterpret_cast<dme_mv *>(s_dyn);
     int *slots = s_dyn + 8 * max_rows;
+    SubJob *queue = reinterpret_cast<SubJob *>(slots + 2 * max_rows * SUB_SLOT);
     const int16_t *__restrict__ up = J.up[r];
     const int *__restrict__ qtab = J.qtab[r];
     dme_mv *__restrict__ V = J.vec[r];
     dme_cost *__restrict__ C = J.pc[r];
     int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
     int boff[2] = {J.boff[2][0], J.boff[2][1]};
-    const int lane = threadIdx.x & 63;
+    const bool c_sem = J.fp_sem == DME_FP_C;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    int phase = 0;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = -1; t <= T; ++t) {
-        WfCopy cp;
+        WfCopy<8> cp;
         wf_issue(cp, qtab, slots, SUB_SLOT, max_rows, t + 1, X, Y);
         for (int base = 0; base < Y; base += blockDim.x) {
+            __syncthreads();  // ring, bank and counter writes are visible
             const int yb = base + threadIdx.x;
             const int xb = t - 2 * yb;
             const bool active = yb < Y && xb >= 0 && xb < X;
-            const size_t k = active ? (size_t)yb * X + xb : 0;
-            const int *mytab = slots + (((xb & 1) & 1) * max_rows + (active ? yb : 0)) * SUB_SLOT;
+            const int *mytab = slots + ((xb & 1) * max_rows + (active ? yb : 0)) * SUB_SLOT;
             dme_mv pred, pel;
             pred.x = pred.y = pel.x = pel.y = 0;
             dme_cost cost;
@@ -1021,7 +1097,7 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
             float lambda = 0.0f, lim = 0.0f;
             Blk b;
             b.xp = b.yp = b.xl = b.yl = 0;
-            bool done = true, miss = false;
+            bool refine = false, miss = false;
             dme_mv out_mv;
             dme_cost out_cost;
             out_mv.x = out_mv.y = 0;
@@ -1072,7 +1148,7 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                     out_mv = pel_up;
                     out_cost = cost;
                 } else {
-                    done = false;
+                    refine = true;
                     if (!b.empty()) {
                         const int dx = pred.x - pel_up.x, dy = pred.y - pel_up.y;
                         if (abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
@@ -1080,90 +1156,125 @@ __global__ __launch_bounds__(1024) void k_subpel_wave(const JobDesc *__restrict_
                     }
                 }
             }
-            // predictor SADs outside the table: the wave computes them together
-            unsigned long long mm = __ballot(miss);
-            while (mm) {
-                const int src = __ffsll((long long)mm) - 1;
-                mm &= mm - 1;
-                Blk bm;
-                bm.xp = __shfl(b.xp, src, WAVE);
-                bm.yp = __shfl(b.yp, src, WAVE);
-                bm.xl = __shfl(b.xl, src, WAVE);
-                bm.yl = __shfl(b.yl, src, WAVE);
-                dme_mv pm;
-                pm.x = __shfl(pred.x, src, WAVE);
-                pm.y = __shfl(pred.y, src, WAVE);
-                int s = 0;
-                wave_up_sads(J.pic[0], W, up, UW, UH, bm, prec, &pm, 1, &s);
-                if (lane == src) psad = s;
+            // predictor SADs outside the table
+            {
+                SubJob mine[4];
+                bool want[4] = {miss, false, false, false};
+                int ir[4] = {psad, 0, 0, 0};
+                float fr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                mine[0].xp = b.xp;
+                mine[0].yp = b.yp;
+                mine[0].xl = b.xl;
+                mine[0].yl = b.yl;
+                mine[0].mx = pred.x;
+                mine[0].my = pred.y;
+                mine[0].step = prec;
+                mine[0].ires = 0;
+                mine[0].start = mine[0].fres = 0.0f;
+                mine[1] = mine[2] = mine[3] = mine[0];
+                sub_serve<false>(s_cnt, phase, queue, J, up, mine, want, ir, fr);
+                psad = ir[0];
             }
-            if (active && !done) {
-                dme_cost pcost;
-                pcost.mvcost = 0.0f;
-                pcost.SAD = (float)psad;
-                pcost.total = pcost.SAD;
-                if (pcost.SAD < lim) {
-                    out_mv = pred;
-                    out_cost = pcost;
-                } else {
-                    dme_cost best = cost;
-                    dme_mv bmv = pel;
-                    bool bailed = false;
-                    const int cx = pel.x << prec, cy = pel.y << prec;
-                    for (int i = 1; i <= prec; ++i) {
-                        bmv.x <<= 1;
-                        bmv.y <<= 1;
-                        dme_mv tb = bmv;
-                        const int sh = prec - i;
-                        if (!b.empty()) {
-                            auto test = [&](int mx, int my) {
-                                dme_mv m;
-                                m.x = mx;
-                                m.y = my;
-                                dme_mv mf;
-                                mf.x = mx << sh;
-                                mf.y = my << sh;
-                                const int s = mytab[(mf.y - cy + rad) * Q + (mf.x - cx + rad)];
-                                const float mvc = var_up(pred, mf);
-                                const float start = mvc * lambda;
-                                const float total = cand_total(J, up, UW, UH, b, m, i, s, start);
-                                if (total < best.total) {
-                                    tb = m;
-                                    best.total = total;
-                                    best.mvcost = mvc;
-                                    best.SAD = total - start;
-                                }
-                            };
-                            test(bmv.x - 1, bmv.y);
-                            test(bmv.x + 1, bmv.y);
-                            test(bmv.x, bmv.y - 1);
-                            test(bmv.x, bmv.y + 1);
-                            if (tb.x != bmv.x) {
-                                const dme_mv nb2 = tb;
-                                test(nb2.x, nb2.y - 1);
-                                test(nb2.x, nb2.y + 1);
-                            } else if (tb.y != bmv.y) {
-                                const dme_mv nb2 = tb;
-                                test(nb2.x - 1, nb2.y);
-                                test(nb2.x + 1, nb2.y);
-                            }
-                        }
-                        bmv = tb;
-                        if ((double)best.total > 1.1 * (double)pcost.total) {
-                            bailed = true;
-                            break;
+            dme_cost pcost;
+            pcost.mvcost = 0.0f;
+            pcost.SAD = (float)psad;
+            pcost.total = pcost.SAD;
+            if (refine && pcost.SAD < lim) {
+                out_mv = pred;
+                out_cost = pcost;
+                refine = false;
+            }
+            // RefineMatchSubp stages; every thread takes part in the queue
+            // round trips, the refining ones with their candidates
+            dme_cost best = cost;
+            dme_mv bmv = pel;
+            bool live = refine, bailed = false;
+            const bool tests = !b.empty();
+            const int cx = pel.x << prec, cy = pel.y << prec;
+            for (int i = 1; i <= prec; ++i) {
+                if (live) {
+                    bmv.x <<= 1;
+                    bmv.y <<= 1;
+                }
+                dme_mv tb = bmv;
+                const int sh = prec - i;
+                for (int round = 0; round < 2; ++round) {
+                    dme_mv m[4];
+                    bool on[4];
+                    if (round == 0) {
+                        m[0].x = bmv.x - 1; m[0].y = bmv.y;
+                        m[1].x = bmv.x + 1; m[1].y = bmv.y;
+                        m[2].x = bmv.x; m[2].y = bmv.y - 1;
+                        m[3].x = bmv.x; m[3].y = bmv.y + 1;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) on[j] = live && tests;
+                    } else {
+                        const bool mx = tb.x != bmv.x, my = !mx && tb.y != bmv.y;
+                        m[0].x = mx ? tb.x : tb.x - 1; m[0].y = mx ? tb.y - 1 : tb.y;
+                        m[1].x = mx ? tb.x : tb.x + 1; m[1].y = mx ? tb.y + 1 : tb.y;
+                        m[2] = m[3] = tb;
+                        on[0] = on[1] = live && tests && (mx || my);
+                        on[2] = on[3] = false;
+                    }
+                    int sd[4];
+                    float mvc[4], st[4], tot[4];
+                    bool need[4];
+                    SubJob mine[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        dme_mv mf;
+                        mf.x = m[j].x << sh;
+                        mf.y = m[j].y << sh;
+                        const int ti = (mf.y - cy + rad) * Q + (mf.x - cx + rad);
+                        sd[j] = on[j] ? mytab[ti] : 0;
+                        mvc[j] = var_up(pred, mf);
+                        st[j] = mvc[j] * lambda;
+                        tot[j] = (float)sd[j] + st[j];
+                        const UpStart us = up_start(b, m[j], i, UW, UH);
+                        need[j] = on[j] && (us.bounds || c_sem) && !chain_is_exact(st[j], sd[j]);
+                        mine[j].xp = b.xp;
+                        mine[j].yp = b.yp;
+                        mine[j].xl = b.xl;
+                        mine[j].yl = b.yl;
+                        mine[j].mx = m[j].x;
+                        mine[j].my = m[j].y;
+                        mine[j].step = i;
+                        mine[j].ires = 0;
+                        mine[j].start = st[j];
+                        mine[j].fres = 0.0f;
+                    }
+                    int ir[4] = {0, 0, 0, 0};
+                    float fr[4] = {tot[0], tot[1], tot[2], tot[3]};
+                    sub_serve<true>(s_cnt, phase, queue, J, up, mine, need, ir, fr);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (on[j] && fr[j] < best.total) {
+                            tb = m[j];
+                            best.total = fr[j];
+                            best.mvcost = mvc[j];
+                            best.SAD = fr[j] - st[j];
                         }
                     }
-                    if (bailed) {
-                        out_mv = pred;
-                        out_cost = pcost;
-                    } else {
-                        out_mv = bmv;
-                        out_cost = best;
+                }
+                if (live) {
+                    bmv = tb;
+                    if ((double)best.total > 1.1 * (double)pcost.total) {
+                        bailed = true;
+                        live = false;
                     }
                 }
             }
+            if (refine) {
+                if (bailed) {
+                    out_mv = pred;
+                    out_cost = pcost;
+                } else {
+                    out_mv = bmv;
+                    out_cost = best;
+                }
+            }
             if (active) {
+                const size_t k = (size_t)yb * X + xb;
                 V[k] = out_mv;
                 C[k] = out_cost;
                 ring[yb * 4 + t % 4] = out_mv;
@@ -1818,13 +1929,13 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
 
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
-    const int threads = min(1024, ((max_rows + 63) / 64) * 64);
-    const size_t lds = sizeof(int) * (8 + 2 * SUB_SLOT) * (size_t)max_rows;
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int threads = 512;
+    const size_t lds = sizeof(int) * (8 + 2 * SUB_SLOT) * (size_t)max_rows + sizeof(SubJob) * SUB_QCAP;
+    if (lds > 159 * 1024) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)k_subpel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
+                                           159 * 1024);
         if (e != hipSuccess) return e;
         attr = true;
     }
