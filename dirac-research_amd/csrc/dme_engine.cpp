@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dme_device.h"
@@ -102,8 +103,25 @@ struct JobState {
     DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1];
     DevBuf ptab[2], qtab[2], m1tab, m0tab;
     JobDesc desc;
-    std::vector<float> h_lam, h_lam1, h_lam0;
-    std::vector<dme_mv> h_vec[2];
+};
+
+// Pinned host staging that grows on demand (freed with the context).
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n) {
+        if (n <= bytes) return DME_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess)
+            return set_error(DME_ERR_NOMEM, "pinned host allocation of " + std::to_string(n) + " bytes failed");
+        bytes = n;
+        return DME_OK;
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
 };
 
 }  // namespace dme
@@ -116,6 +134,7 @@ struct dme_ctx {
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
     dme::DevBuf d_jobs, d_tasks, d_planes;
+    dme::PinnedBuf pin;  // lambda-stage staging: level-0 vectors in, lambda maps out
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dme_stats stats;
@@ -188,7 +207,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
             t.ow = ow;
             t.oh = oh;
             t.offset = off;
-            off += ((ow + 63) / 64) * ((oh + 3) / 4);
+            off += ((ow + 63) / 64) * ((oh + 7) / 8);  // 64 x 8 output tiles
             all.push_back(t);
         }
         ranges.push_back(std::make_pair(first, off));
@@ -467,36 +486,59 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
 // Lambda maps: host computation from the level-0 pel vectors, then upload of
 // the level-2 map and the level-1/0 maps of ModeDecider.
 static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *overrides) {
-    for (auto *j : js) {
-        const size_t B = (size_t)j->g.xnb * j->g.ynb;
-        for (int r = 0; r < j->nrefs; ++r) {
-            j->h_vec[r].resize(B);
-            HIP_TRY(hipMemcpyAsync(j->h_vec[r].data(), j->vec[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        }
+    // staging layout per job: vectors of each reference, then the level-2,
+    // level-1 and level-0 lambda maps
+    std::vector<size_t> off(js.size() + 1, 0);
+    for (size_t i = 0; i < js.size(); ++i) {
+        const size_t B = (size_t)js[i]->g.xnb * js[i]->g.ynb;
+        off[i + 1] = off[i] + sizeof(dme_mv) * B * js[i]->nrefs + sizeof(float) * (B + B / 4 + B / 16);
+    }
+    DME_TRY(ctx->pin.ensure(off.back()));
+    char *base = static_cast<char *>(ctx->pin.p);
+    for (size_t i = 0; i < js.size(); ++i) {
+        const size_t B = (size_t)js[i]->g.xnb * js[i]->g.ynb;
+        for (int r = 0; r < js[i]->nrefs; ++r)
+            HIP_TRY(hipMemcpyAsync(base + off[i] + sizeof(dme_mv) * B * r, js[i]->vec[r].p, sizeof(dme_mv) * B,
+                                   hipMemcpyDeviceToHost, ctx->stream));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (size_t i = 0; i < js.size(); ++i) {
+    // MEData::SetLambdaMap per picture, pictures spread over host threads
+    auto work = [&](size_t i) {
         JobState &j = *js[i];
         const Geometry &g = j.g;
         const size_t B = (size_t)g.xnb * g.ynb;
-        j.h_lam.resize(B);
+        const dme_mv *v = reinterpret_cast<const dme_mv *>(base + off[i]);
+        float *lam = reinterpret_cast<float *>(base + off[i] + sizeof(dme_mv) * B * j.nrefs);
         if (overrides && overrides[i])
-            std::memcpy(j.h_lam.data(), overrides[i], sizeof(float) * B);
+            std::memcpy(lam, overrides[i], sizeof(float) * B);
         else
-            lambda_map_l2(g.xnb, g.ynb, j.nrefs, j.h_vec[0].data(), j.nrefs > 1 ? j.h_vec[1].data() : nullptr,
-                          j.p.lambda, j.h_lam.data());
-        j.h_lam1.resize(B / 4);
-        j.h_lam0.resize(B / 16);
-        lambda_map_level(j.h_lam.data(), g.xnb, 1, (float)(1.0 / g.level_factor[1]), g.xnb / 2, g.ynb / 2,
-                         j.h_lam1.data());
-        lambda_map_level(j.h_lam.data(), g.xnb, 0, (float)(1.0 / g.level_factor[0]), g.xnb / 4, g.ynb / 4,
-                         j.h_lam0.data());
-        HIP_TRY(hipMemcpyAsync(j.lam.p, j.h_lam.data(), sizeof(float) * B, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(j.lam1.p, j.h_lam1.data(), sizeof(float) * B / 4, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(j.lam0.p, j.h_lam0.data(), sizeof(float) * B / 16, hipMemcpyHostToDevice, ctx->stream));
+            lambda_map_l2(g.xnb, g.ynb, j.nrefs, v, j.nrefs > 1 ? v + B : nullptr, j.p.lambda, lam);
+        lambda_map_level(lam, g.xnb, 1, (float)(1.0 / g.level_factor[1]), g.xnb / 2, g.ynb / 2, lam + B);
+        lambda_map_level(lam, g.xnb, 0, (float)(1.0 / g.level_factor[0]), g.xnb / 4, g.ynb / 4, lam + B + B / 4);
+    };
+    const size_t nt = std::min<size_t>(js.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        for (size_t i = 0; i < js.size(); ++i) work(i);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t i = t; i < js.size(); i += nt) work(i);
+            });
+        for (auto &x : th) x.join();
+    }
+    for (size_t i = 0; i < js.size(); ++i) {
+        JobState &j = *js[i];
+        const size_t B = (size_t)j.g.xnb * j.g.ynb;
+        const char *lam = base + off[i] + sizeof(dme_mv) * B * j.nrefs;
+        HIP_TRY(hipMemcpyAsync(j.lam.p, lam, sizeof(float) * B, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(j.lam1.p, lam + sizeof(float) * B, sizeof(float) * B / 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipMemcpyAsync(j.lam0.p, lam + sizeof(float) * (B + B / 4), sizeof(float) * B / 16,
+                               hipMemcpyHostToDevice, ctx->stream));
         j.stage = 2;
     }
+    // the staging buffer is reused by the next batch: the copies must have left it
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return DME_OK;
 }

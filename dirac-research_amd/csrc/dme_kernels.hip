@@ -163,15 +163,24 @@ __device__ unsigned long long wave_pel_argmin(const int16_t *__restrict__ pic, c
 
 // ---------------------------------------------------------------------------
 // K1: DownConverter::DoDownConvert + RowLoop (downconvert.cpp:51-204).
-// One thread per output sample; the 12 short row-buffer values it needs are
-// recomputed from the column taps.  Index clamping to [0, 2*len-1] is the
-// reference's edge handling for len >= 12 (host-checked).
+// Separable: the vertical 12-tap pass fills the short row buffer, then the
+// horizontal pass filters it; both round (+128) >> 8 and the row buffer is
+// short.  Index clamping to [0, 2*len-1] is the reference's edge handling for
+// len >= 12 (host-checked).  A workgroup makes a tile of 64 x 8 output
+// samples from a 138 x 26 input tile staged in LDS.
 // ---------------------------------------------------------------------------
 __constant__ int c_down_taps[6] = {86, 46, 4, -8, -4, 4};
 
-// One launch serves every plane of a batch: tiles of 64 x 4 output samples
-// are mapped to planes through the offset table.
+#define DC_TW 64                  // output tile width
+#define DC_TH 8                   // output tile height
+#define DC_IW (2 * DC_TW + 10)    // input tile columns (138)
+#define DC_IH (2 * DC_TH + 10)    // input tile rows (26)
+
+// One launch serves every plane of a batch: tiles are mapped to planes
+// through the offset table.
 __global__ __launch_bounds__(256) void k_downconvert(const PlaneTask *__restrict__ pt, int npt, int total) {
+    __shared__ int16_t s_in[DC_IH][DC_IW];
+    __shared__ int16_t s_v[DC_TH][DC_IW];
     const int tile = blockIdx.x;
     if (tile >= total) return;
     int lo = 0, hi = npt - 1;
@@ -182,35 +191,49 @@ __global__ __launch_bounds__(256) void k_downconvert(const PlaneTask *__restrict
     }
     const PlaneTask p = pt[lo];
     const int local = tile - p.offset;
-    const int tx = (p.ow + 63) / 64;
-    const int cx = (local % tx) * 64 + (threadIdx.x & 63);
-    const int cy = (local / tx) * 4 + (threadIdx.x >> 6);
+    const int tx = (p.ow + DC_TW - 1) / DC_TW;
+    const int cx0 = (local % tx) * DC_TW, cy0 = (local / tx) * DC_TH;
     const int ow = p.ow, oh = p.oh, in_w = p.in_w;
+    const int xlen = 2 * ow, ylen = 2 * oh;
     const int16_t *__restrict__ in = p.in;
     int16_t *__restrict__ out = p.out;
-    if (cx >= ow || cy >= oh) return;
-    const int xlen = 2 * ow, ylen = 2 * oh;
-    const int y = 2 * cy, x = 2 * cx;
-    int rows[12];
-    for (int k = 0; k < 6; ++k) {
-        rows[5 - k] = clampi(y - k, 0, ylen - 1);
-        rows[6 + k] = clampi(y + 1 + k, 0, ylen - 1);
-    }
-    int rb[12];
+    const int tid = threadIdx.x;
+    // input tile: rows 2*cy0 - 5 + r, columns 2*cx0 - 5 + c, clamped
+    for (int i0 = 0; i0 < DC_IH * DC_IW; i0 += 8 * 256) {
+        int16_t v[8];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) {
-        const int col = clampi(x - 5 + q, 0, xlen - 1);
+        for (int k = 0; k < 8; ++k) {
+            const int i = i0 + k * 256 + tid;
+            const int r = i / DC_IW, c = i - r * DC_IW;
+            v[k] = 0;
+            if (i < DC_IH * DC_IW)
+                v[k] = in[(size_t)clampi(2 * cy0 - 5 + r, 0, ylen - 1) * in_w + clampi(2 * cx0 - 5 + c, 0, xlen - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = i0 + k * 256 + tid;
+            if (i < DC_IH * DC_IW) (&s_in[0][0])[i] = v[k];
+        }
+    }
+    __syncthreads();
+    // vertical pass: output row o uses input rows 2o .. 2o + 11 of the tile
+    for (int i = tid; i < DC_TH * DC_IW; i += 256) {
+        const int o = i / DC_IW, c = i - o * DC_IW;
         int sum = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            sum += ((int)in[(size_t)rows[5 - k] * in_w + col] + (int)in[(size_t)rows[6 + k] * in_w + col]) *
-                   c_down_taps[k];
-        rb[q] = (int16_t)((sum + 128) >> 8);
+        for (int k = 0; k < 6; ++k) sum += ((int)s_in[2 * o + 5 - k][c] + (int)s_in[2 * o + 6 + k][c]) * c_down_taps[k];
+        s_v[o][c] = (int16_t)((sum + 128) >> 8);
     }
-    int sum = 0;
+    __syncthreads();
+    for (int i = tid; i < DC_TH * DC_TW; i += 256) {
+        const int o = i / DC_TW, c = i - o * DC_TW;
+        const int cx = cx0 + c, cy = cy0 + o;
+        if (cx >= ow || cy >= oh) continue;
+        int sum = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sum += (rb[5 - k] + rb[6 + k]) * c_down_taps[k];
-    out[(size_t)cy * ow + cx] = (int16_t)((sum + 128) >> 8);
+        for (int k = 0; k < 6; ++k) sum += ((int)s_v[o][2 * c + 5 - k] + (int)s_v[o][2 * c + 6 + k]) * c_down_taps[k];
+        out[(size_t)cy * ow + cx] = (int16_t)((sum + 128) >> 8);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -375,6 +398,81 @@ __device__ __forceinline__ void wave_copy_rect(int16_t *__restrict__ dst, int pi
 }
 
 // ---------------------------------------------------------------------------
+// Packed SADs.  Samples are staged in LDS as v + 32768, two per dword, so that
+// v_sad_u16 adds two exact absolute differences per instruction; a candidate
+// row that starts at an odd sample is realigned with v_alignbyte.
+// ---------------------------------------------------------------------------
+// Wave copy of a clamped rectangle into LDS as biased 16-bit samples, rows
+// `pitch` samples apart; samples [RW, padw) of each row are set to zero.
+// Each lane keeps one column of a strip of at most 64 columns and steps over
+// rows, 16 loads in flight.
+__device__ __forceinline__ void wave_stage_u16(uint16_t *__restrict__ dst, int pitch, const int16_t *__restrict__ src,
+                                               int W, int H, int x0, int y0, int RW, int RH, int padw) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int c0 = 0; c0 < padw; c0 += WAVE) {
+        const int cw = min(WAVE, padw - c0);
+        const int rpi = WAVE / cw;  // rows per pass of the wave
+        const int col = c0 + lane % cw, row0 = lane / cw;
+        const bool on = row0 < rpi;
+        const int sx = clampi(x0 + col, 0, W - 1);
+        const bool inside = col < RW;
+        for (int r0 = 0; r0 < RH; r0 += 16 * rpi) {
+            uint16_t v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int ry = r0 + k * rpi + row0;
+                v[k] = 0;
+                if (on && ry < RH && inside)
+                    v[k] = (uint16_t)(src[(size_t)clampi(y0 + ry, 0, H - 1) * W + sx] + 32768);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int ry = r0 + k * rpi + row0;
+                if (on && ry < RH) dst[ry * pitch + col] = v[k];
+            }
+        }
+    }
+}
+
+// SAD of an xl x yl block (packed rows of pp dwords, zero beyond xl) against
+// the packed region rows of rp dwords starting at sample (ox, oy).
+__device__ __forceinline__ int packed_sad(const uint32_t *__restrict__ pic, int pp, const uint32_t *__restrict__ reg,
+                                          int rp, int ox, int oy, int xl, int yl) {
+    const int nd = (xl + 1) >> 1, nch = (nd + 3) >> 2;
+    const int full = xl >> 1;  // dwords holding two samples of the block
+    uint32_t m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gi = 4 * (nch - 1) + i;
+        m[i] = gi < full ? 0xffffffffu : (gi == full && (xl & 1) ? 0x0000ffffu : 0u);
+    }
+    const uint32_t *rb = reg + oy * rp + (ox >> 1);
+    const unsigned sh = (ox & 1) * 2;
+    uint32_t acc = 0;
+    for (int y = 0; y < yl; ++y) {
+        const uint32_t *pr = pic + y * pp;
+        const uint32_t *rr = rb + y * rp;
+        for (int c = 0; c < nch; ++c) {
+            const uint32_t r0 = rr[4 * c], r1 = rr[4 * c + 1], r2 = rr[4 * c + 2], r3 = rr[4 * c + 3],
+                           r4 = rr[4 * c + 4];
+            uint32_t a0 = __builtin_amdgcn_alignbyte(r1, r0, sh), a1 = __builtin_amdgcn_alignbyte(r2, r1, sh);
+            uint32_t a2 = __builtin_amdgcn_alignbyte(r3, r2, sh), a3 = __builtin_amdgcn_alignbyte(r4, r3, sh);
+            if (c == nch - 1) {
+                a0 &= m[0];
+                a1 &= m[1];
+                a2 &= m[2];
+                a3 &= m[3];
+            }
+            acc = __builtin_amdgcn_sad_u16(pr[4 * c], a0, acc);
+            acc = __builtin_amdgcn_sad_u16(pr[4 * c + 1], a1, acc);
+            acc = __builtin_amdgcn_sad_u16(pr[4 * c + 2], a2, acc);
+            acc = __builtin_amdgcn_sad_u16(pr[4 * c + 3], a3, acc);
+        }
+    }
+    return (int)acc;
+}
+
+// ---------------------------------------------------------------------------
 // Pel search, bulk part.  For every block of a level, the SADs of every
 // candidate within R = r + 1 (r = min(level+1, 5), pixel_match.cpp:258-259)
 // of the zero vector and of the guide vector (levels below the coarsest) are
@@ -390,6 +488,8 @@ __device__ __forceinline__ void wave_copy_rect(int16_t *__restrict__ dst, int pi
 #define PEL_MAXR 6
 #define PEL_REG (PEL_MAXB + 2 * PEL_MAXR)  // 44
 #define PEL_MAXTS ((2 * PEL_MAXR + 1) * (2 * PEL_MAXR + 1))
+#define PEL_PP (PEL_MAXB / 2)        // packed block row: 16 dwords
+#define PEL_RP 25                    // packed region row in dwords: odd, so rows use different LDS banks
 
 __device__ __forceinline__ int pel_radius(int level) { return min(level + 1, 5); }
 // Pel record of a block (PEL_REC ints, blocks in wavefront order):
@@ -401,18 +501,30 @@ __device__ __forceinline__ int pel_radius(int level) { return min(level + 1, 5);
 //   first strict-< minimum of FindBestMatchPel.
 #define PEL_REC 28
 
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, WAVE));
+    return v;
+}
+
+#define WAVE_SYNC()                                             \
+    do {                                                        \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  \
+        __builtin_amdgcn_wave_barrier();                        \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  \
+    } while (0)
+
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total, int level) {
-    __shared__ int16_t s_pic[4][PEL_MAXB * PEL_MAXB];
-    __shared__ int16_t s_reg[4][2][PEL_REG * PEL_REG];
+    // packed staging: block rows of PEL_PP dwords, region rows of PEL_RP dwords
+    __shared__ uint32_t s_pic[4][PEL_MAXB * PEL_PP];
+    __shared__ uint32_t s_reg[4][2][PEL_REG * PEL_RP];
     __shared__ int s_tab[4][2 * PEL_MAXTS];
-    __shared__ int s_key[4][18];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gi = blockIdx.x * 4 + wv;
-    const bool valid = gi < total;
-    const Task tk = tasks[find_task(tasks, ntasks, valid ? gi : total - 1)];
+    if (gi >= total) return;  // each wave works alone
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
-    const int r = tk.ref, local = (valid ? gi : total - 1) - tk.offset;
+    const int r = tk.ref, local = gi - tk.offset;
     const int X = J.lxb[level];
     const int xb = local % X, yb = local / X;
     const int W = J.lw[level], H = J.lh[level];
@@ -420,7 +532,9 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     const int16_t *__restrict__ ref = J.ref[r][level];
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
     const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
+    const int n = (2 * rad + 1) * (2 * rad + 1);
     const bool has_guide = level < J.depth;
+    const int ntab = has_guide ? 2 : 1;
     dme_mv g;
     g.x = 0;
     g.y = 0;
@@ -430,67 +544,55 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
         g.x = gv.x * 2;
         g.y = gv.y * 2;
     }
-    const bool work = valid && !b.empty();
-    const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
     int *__restrict__ rec = J.ptab[r] + (size_t)wf_pos(xb, yb, X, J.lyb[level]) * PEL_REC;
-    if (work) {
-        wave_copy_rect(s_pic[wv], PEL_MAXB, pic, W, H, b.xp, b.yp, b.xl, b.yl);
-        wave_copy_rect(s_reg[wv][0], RW, ref, W, H, b.xp - R, b.yp - R, RW, RH);
-        if (has_guide) wave_copy_rect(s_reg[wv][1], RW, ref, W, H, b.xp + g.x - R, b.yp + g.y - R, RW, RH);
-    }
-    __syncthreads();
-    if (work) {
-        for (int c = lane; c < 2 * TS; c += WAVE) {
-            const int t = c >= TS ? 1 : 0, e = c - t * TS;
-            int s = INT_MAX;
-            if (t == 0 || has_guide) {
-                const int ox = e % D, oy = e / D;  // region offset of the candidate
-                const int16_t *rg = &s_reg[wv][t][oy * RW + ox];
-                s = 0;
-                for (int y = 0; y < b.yl; ++y) {
-                    const int16_t *pr = &s_pic[wv][y * PEL_MAXB];
-                    const int16_t *rr = rg + y * RW;
-                    for (int x = 0; x < b.xl; ++x) s += abs((int)pr[x] - (int)rr[x]);
-                }
-            }
-            s_tab[wv][c] = s;
-        }
-    }
-    if (lane < 18) s_key[wv][lane] = INT_MAX;
-    __syncthreads();
-    const int n = (2 * rad + 1) * (2 * rad + 1);
-    if (work) {
-        const int nw = (has_guide ? 2 : 1) * 9 * n;
-        for (int i = lane; i < nw; i += WAVE) {
-            const int tw = i / n, c = i - tw * n;  // tw = table * 9 + window
-            const int t = tw >= 9 ? 1 : 0, w = tw - 9 * t;
-            const int ddx = w % 3 - 1, ddy = w / 3 - 1;
-            const dme_mv o = win_off(c, rad);
-            atomicMin(&s_key[wv][tw], s_tab[wv][t * TS + (ddy + o.y + R) * D + (ddx + o.x + R)] * 128 + c);
-        }
-    }
-    __syncthreads();
     int best_sad = 10000000;
     dme_mv best_mv;
     best_mv.x = 0;
     best_mv.y = 0;
-    if (work) {
+    int mykey = INT_MAX;  // lane t*9 + w keeps the key of window w of table t
+    if (!b.empty()) {
+        const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
+        wave_stage_u16(reinterpret_cast<uint16_t *>(s_pic[wv]), 2 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
+                       (b.xl + 7) & ~7);
+        wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][0]), 2 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW, RH,
+                       RW);
+        if (has_guide)
+            wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][1]), 2 * PEL_RP, ref, W, H, b.xp + g.x - R,
+                           b.yp + g.y - R, RW, RH, RW);
+        WAVE_SYNC();
+        for (int c = lane; c < ntab * TS; c += WAVE) {
+            const int t = c >= TS ? 1 : 0, e = c - t * TS;
+            const int ox = e % D, oy = e / D;  // region offset of the candidate
+            s_tab[wv][c] = packed_sad(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl);
+        }
+        WAVE_SYNC();
+        // first minima (key = SAD * 128 + enumeration index) of the nine
+        // windows around each centre; lane c holds candidates c and c + 64
+        const dme_mv o0 = win_off(lane < n ? lane : 0, rad), o1 = win_off(lane + 64 < n ? lane + 64 : 0, rad);
+        for (int t = 0; t < ntab; ++t)
+#pragma unroll
+            for (int w = 0; w < 9; ++w) {
+                const int ddx = w % 3 - 1 + R, ddy = w / 3 - 1 + R;
+                const int *tb = s_tab[wv] + t * TS;
+                int v = INT_MAX;
+                if (lane < n) v = tb[(ddy + o0.y) * D + ddx + o0.x] * 128 + lane;
+                if (lane + 64 < n) v = min(v, tb[(ddy + o1.y) * D + ddx + o1.x] * 128 + lane + 64);
+                v = wave_min_i(v);
+                if (lane == t * 9 + w) mykey = v;
+            }
         // zero list, then guide list (radius rad, centred windows)
-        const int k0 = s_key[wv][4];
+        const int k0 = __shfl(mykey, 4, WAVE), k1 = __shfl(mykey, 13, WAVE);
         best_sad = k0 >> 7;
         best_mv = win_off(k0 & 127, rad);
-        if (has_guide) {
-            const int k1 = s_key[wv][13];
-            if ((k1 >> 7) < best_sad) {
-                best_sad = k1 >> 7;
-                best_mv = win_off(k1 & 127, rad);
-                best_mv.x += g.x;
-                best_mv.y += g.y;
-            }
+        if (has_guide && (k1 >> 7) < best_sad) {
+            best_sad = k1 >> 7;
+            best_mv = win_off(k1 & 127, rad);
+            best_mv.x += g.x;
+            best_mv.y += g.y;
         }
     }
-    if (valid && lane < 18) rec[8 + lane] = s_key[wv][lane];
-    if (valid && lane == 0) {
+    if (lane < 18) rec[8 + lane] = mykey;
+    if (lane == 0) {
         rec[0] = best_sad;
         rec[1] = best_mv.x;
         rec[2] = best_mv.y;
