@@ -36,6 +36,7 @@ struct JobDesc {
     const int16_t DME_G *pic[DME_MAX_DEPTH + 1];      // [0] = Y of the picture
     const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
     const int16_t DME_G *up[2];                       // 2W x 2H up-converted refs
+    const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
     // pel search, per reference and level ([0] aliases the MEData arrays)
     dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];

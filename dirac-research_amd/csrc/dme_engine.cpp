@@ -32,7 +32,10 @@ hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
                            int max_blen_y, hipStream_t st);
-hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                              int max_blen_y, hipStream_t st);
+hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st);
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
@@ -89,7 +92,7 @@ struct Picture {
     DevBuf own[3];
     DevBuf pyr[DME_MAX_DEPTH + 1];  // levels 1..
     int pyr_levels = 0;             // levels built so far
-    DevBuf up;
+    DevBuf up, qp;  // up-converted plane and its quarter-pel planes
     bool up_ready = false;
 };
 
@@ -133,7 +136,7 @@ struct dme_ctx {
     bool params_set = false;
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
-    dme::DevBuf d_jobs, d_tasks, d_planes;
+    dme::DevBuf d_jobs, d_tasks, d_planes, d_flag;
     dme::PinnedBuf pin;  // lambda-stage staging: level-0 vectors in, lambda maps out
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -149,11 +152,12 @@ namespace dme {
 
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
-    KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_COUNT
+    KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
-    "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "", "", "", ""};
+    "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp", "", "",
+    ""};
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
 template <class F>
@@ -230,6 +234,25 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         }
         ranges.push_back(std::make_pair(first, off));
     }
+    {
+        const size_t first = all.size();
+        int off = 0;
+        for (Picture *P : ups) {
+            if (P->up_ready) continue;
+            const size_t n = (size_t)(P->W + 2) * 4 * (P->H + 2);  // dwords
+            DME_TRY(P->qp.ensure(sizeof(uint32_t) * n));
+            PlaneTask t;
+            t.in = P->up.as<int16_t>();
+            t.out = P->qp.as<int16_t>();
+            t.in_w = 2 * P->W;
+            t.ow = P->W;
+            t.oh = P->H;
+            t.offset = off;
+            off += (int)((n + 255) / 256);
+            all.push_back(t);
+        }
+        ranges.push_back(std::make_pair(first, off));
+    }
     if (all.empty()) return DME_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     DME_TRY(ctx->d_planes.ensure(sizeof(PlaneTask) * all.size()));
@@ -238,16 +261,21 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
     const PlaneTask *dp = ctx->d_planes.as<PlaneTask>();
     for (int l = 1; l <= maxl; ++l) {
         const auto &rg = ranges[l - 1];
-        const int n = (int)((l < maxl ? ranges[l].first : ranges.back().first) - rg.first);
+        const int n = (int)(ranges[l].first - rg.first);  // the next range starts where this one ends
         if (rg.second == 0) continue;
         DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches,
                              [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->stream); }));
     }
-    const auto &ru = ranges.back();
+    const auto &ru = ranges[ranges.size() - 2], &rq = ranges.back();
     if (ru.second) {
-        const int n = (int)(all.size() - ru.first);
+        const int n = (int)(rq.first - ru.first);
         DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
                              [&] { return launch_upconvert(dp + ru.first, n, ru.second, ctx->stream); }));
+    }
+    if (rq.second) {
+        const int n = (int)(all.size() - rq.first);
+        DME_TRY(timed_launch(ctx, KK_BUILD_QP, launches,
+                             [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->stream); }));
     }
     // the task array must outlive the copy
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -392,7 +420,10 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
             Picture *R = find_pic(ctx, j->refs[r]);
             j->desc.ref[r][0] = R->y;
             for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
-            if (need_up) j->desc.up[r] = R->up.as<int16_t>();
+            if (need_up) {
+                j->desc.up[r] = R->up.as<int16_t>();
+                j->desc.qp[r] = R->qp.as<uint32_t>();
+            }
         }
     }
     return DME_OK;
@@ -548,7 +579,7 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
     std::vector<Task> wt, sh, bt;
-    int off = 0, offb = 0, rows = 1;
+    int off = 0, offb = 0, rows = 1, bx = 1, by = 1;
     for (size_t i = 0; i < js.size(); ++i)
         for (int r = 0; r < js[i]->nrefs; ++r) {
             const int cnt = js[i]->g.xnb * js[i]->g.ynb;
@@ -560,6 +591,8 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
                 bt.push_back(Task{(int)i, r, cnt, offb});
                 offb += cnt;
                 rows = std::max(rows, js[i]->g.ynb);
+                bx = std::max(bx, js[i]->g.blen[2][0]);
+                by = std::max(by, js[i]->g.blen[2][1]);
             }
         }
     const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt);
@@ -567,7 +600,7 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
     if (bulk.n) {
         DME_TRY(timed_launch(ctx, KK_SUBPEL_BULK, launches, [&] {
-            return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, ctx->stream);
+            return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, bx, by, ctx->stream);
         }));
         DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,
                              [&] { return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows, ctx->stream); }));
@@ -791,7 +824,14 @@ static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, co
             P->v = P->own[2].as<int16_t>();
         }
     }
+    // luma must be 8-bit samples minus 128
+    DME_TRY(ctx->d_flag.ensure(sizeof(int)));
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(ctx->d_flag.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_check_range(P->y, n, ctx->d_flag.as<int>(), ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (bad) return set_error(DME_ERR_INVALID, "luma samples must lie in [-128, 127] (8-bit samples minus 128)");
     ctx->pics[pic_num] = std::move(P);
     return DME_OK;
 }

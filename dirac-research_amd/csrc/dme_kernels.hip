@@ -25,6 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 
@@ -350,6 +351,126 @@ __device__ __forceinline__ void wf_finish(WfCopy<PF> &c) {
             if (i < c.len) c.dst[i] = c.v[k];
         }
     }
+}
+
+// Luma samples are 8-bit values minus 128 (pic_io.cpp:396-401); the packed
+// byte SADs rely on that range, so pictures are checked when registered.
+__global__ __launch_bounds__(256) void k_check_range(const int16_t *__restrict__ y, size_t n, int *flag) {
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        bad = bad || y[i] < -128 || y[i] > 127;
+    if (bad) *flag = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Quarter-pel planes.  Every value a sub-pel Diff can read from the
+// up-converted reference (BlockDiffHalfPel / QuarterPel, me_utils.cpp:410-867)
+// is the sample at a quarter-pel position (4X + fx, 4Y + fy): the rounded
+// average of the one, two or four up-plane samples around it, with clamped
+// coordinates.  They are tabulated once per reference as biased bytes
+// (v + 128; up-plane values lie in [-128, 127]) in the layout
+//   qp[((Y + 1) * 4 + fy) * (W + 2) + X + 1] = dword of the four fx bytes,
+// for X in [-1, W], Y in [-1, H]; positions further out repeat the border
+// (both clamped corners coincide there), so reads clamp X and Y to that range.
+// A half-pel vector m reads the even phases: position 2m in quarter pels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ pt, int npt, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    int lo = 0, hi = npt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pt[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneTask p = pt[lo];
+    const int W = p.ow, H = p.oh, UW = 2 * W, UH = 2 * H, QW = W + 2;
+    const int i = (tile - p.offset) * 256 + threadIdx.x;
+    if (i >= QW * 4 * (H + 2)) return;
+    const int row = i / QW, X = i - row * QW - 1;
+    const int Y = (row >> 2) - 1, fy = row & 3;
+    const int16_t *__restrict__ up = p.in;
+    const int hy = 2 * Y + (fy >> 1), ry = fy & 1;
+    const int y0 = clampi(hy, 0, UH - 1), y1 = clampi(hy + 1, 0, UH - 1);
+    uint32_t out = 0;
+#pragma unroll
+    for (int fx = 0; fx < 4; ++fx) {
+        const int hx = 2 * X + (fx >> 1), rx = fx & 1;
+        const int x0 = clampi(hx, 0, UW - 1), x1 = clampi(hx + 1, 0, UW - 1);
+        const int a = up[(size_t)y0 * UW + x0], b = up[(size_t)y0 * UW + x1];
+        const int c = up[(size_t)y1 * UW + x0], d = up[(size_t)y1 * UW + x1];
+        const int v = ((2 - rx) * (2 - ry) * a + rx * (2 - ry) * b + (2 - rx) * ry * c + rx * ry * d + 2) >> 2;
+        out |= (uint32_t)(v + 128) << (8 * fx);
+    }
+    reinterpret_cast<uint32_t *>(p.out)[i] = out;
+}
+
+// Wave copy of a clamped rectangle of an 8-bit-range plane into LDS as
+// biased bytes (v + 128), rows `pitch` bytes apart; bytes [RW, padw) of each
+// row are set to zero.  Lanes keep a column of a strip of at most 64.
+__device__ __forceinline__ void wave_stage_u8(uint8_t *__restrict__ dst, int pitch, const int16_t *__restrict__ src,
+                                              int W, int H, int x0, int y0, int RW, int RH, int padw) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int c0 = 0; c0 < padw; c0 += WAVE) {
+        const int cw = min(WAVE, padw - c0);
+        const int rpi = WAVE / cw;
+        const int col = c0 + lane % cw, row0 = lane / cw;
+        const bool on = row0 < rpi;
+        const int sx = clampi(x0 + col, 0, W - 1);
+        const bool inside = col < RW;
+        for (int r0 = 0; r0 < RH; r0 += 16 * rpi) {
+            uint8_t v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int ry = r0 + k * rpi + row0;
+                v[k] = 0;
+                if (on && ry < RH && inside)
+                    v[k] = (uint8_t)(src[(size_t)clampi(y0 + ry, 0, H - 1) * W + sx] + 128);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int ry = r0 + k * rpi + row0;
+                if (on && ry < RH) dst[ry * pitch + col] = v[k];
+            }
+        }
+    }
+}
+
+// SAD of an xl x yl block (packed bytes, rows of pp dwords, zero beyond xl)
+// against packed byte rows of rp dwords starting at byte bo of row row0.
+__device__ __forceinline__ int packed_sad_u8(const uint32_t *__restrict__ pic, int pp, const uint32_t *__restrict__ reg,
+                                             int rp, int bo, int row0, int xl, int yl) {
+    const int nd = (xl + 3) >> 2, nch = (nd + 3) >> 2;
+    uint32_t m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int valid = min(max(xl - 4 * (4 * (nch - 1) + i), 0), 4);
+        m[i] = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+    }
+    const uint32_t *rb = reg + row0 * rp + (bo >> 2);
+    const unsigned sh = bo & 3;
+    uint32_t acc = 0;
+    for (int y = 0; y < yl; ++y) {
+        const uint32_t *pr = pic + y * pp;
+        const uint32_t *rr = rb + y * rp;
+        for (int c = 0; c < nch; ++c) {
+            const uint32_t r0 = rr[4 * c], r1 = rr[4 * c + 1], r2 = rr[4 * c + 2], r3 = rr[4 * c + 3],
+                           r4 = rr[4 * c + 4];
+            uint32_t a0 = __builtin_amdgcn_alignbyte(r1, r0, sh), a1 = __builtin_amdgcn_alignbyte(r2, r1, sh);
+            uint32_t a2 = __builtin_amdgcn_alignbyte(r3, r2, sh), a3 = __builtin_amdgcn_alignbyte(r4, r3, sh);
+            if (c == nch - 1) {
+                a0 &= m[0];
+                a1 &= m[1];
+                a2 &= m[2];
+                a3 &= m[3];
+            }
+            acc = __builtin_amdgcn_sad_u8(pr[4 * c], a0, acc);
+            acc = __builtin_amdgcn_sad_u8(pr[4 * c + 1], a1, acc);
+            acc = __builtin_amdgcn_sad_u8(pr[4 * c + 2], a2, acc);
+            acc = __builtin_amdgcn_sad_u8(pr[4 * c + 3], a3, acc);
+        }
+    }
+    return (int)acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -819,35 +940,32 @@ __device__ __forceinline__ UpStart up_start(const Blk &b, dme_mv mv, int step, i
     return s;
 }
 
-__device__ __forceinline__ int up_val(const int16_t *__restrict__ up, int UW, int UH, const UpStart &s, int dy, int dx) {
-    const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
-    const int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
-    const int a = up[(size_t)by * UW + bx];
-    if (!(s.rx | s.ry)) return a;
-    const int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
-    const int w0 = (2 - s.rx) * (2 - s.ry), w1 = s.rx * (2 - s.ry), w2 = (2 - s.rx) * s.ry, w3 = s.rx * s.ry;
-    return (w0 * a + w1 * up[(size_t)by * UW + bx1] + w2 * up[(size_t)by1 * UW + bx] + w3 * up[(size_t)by1 * UW + bx1] +
-            2) >> 2;
+// Up-plane sample of pixel (dy, dx) for start s, read from the quarter-pel
+// planes: quarter-pel position (2 rsx + rx + 4 dx, 2 rsy + ry + 4 dy).
+__device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int H, const UpStart &s, int dy, int dx) {
+    const int Xq = 2 * s.rsx + s.rx + 4 * dx, Yq = 2 * s.rsy + s.ry + 4 * dy;
+    const int X = clampi(Xq >> 2, -1, W), Y = clampi(Yq >> 2, -1, H);
+    const uint32_t d = qp[(size_t)((Y + 1) * 4 + (Yq & 3)) * (W + 2) + X + 1];
+    return (int)((d >> (8 * (Xq & 3))) & 0xffu) - 128;
 }
 
-// Integer SADs of up to 4 candidates against the up plane; wave-uniform.
-// (candidate, pixel) items are spread over the lanes, 8 per lane at a time
-// with all their loads issued together.
-__device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up, int UW, int UH,
-                             const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
+// Integer SADs of up to 4 candidates against the up-converted reference;
+// wave-uniform.  (candidate, pixel) items are spread over the lanes, 8 per
+// lane at a time with all their loads issued together.
+__device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const uint32_t *__restrict__ qp, int UW,
+                                          int UH, const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
     const int lane = threadIdx.x & (WAVE - 1);
     UpStart st[4];
     for (int c = 0; c < 4; ++c) st[c] = up_start(b, cands[c < nc ? c : 0], step, UW, UH);
     const int np = b.xl * b.yl, n = nc * np;
     int acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
     for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
-        int pv[8], a[8], bb[8], cc[8], dd[8], w[8][4], cid[8];
+        int pv[8], rv[8], cid[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int it = b0 + k * WAVE + lane;
             cid[k] = -1;
-            pv[k] = a[k] = bb[k] = cc[k] = dd[k] = 0;
-            w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
+            pv[k] = rv[k] = 0;
             if (it < n) {
                 const int c = it / np, p = it - c * np;
                 const int dy = p / b.xl, dx = p - dy * b.xl;
@@ -855,25 +973,14 @@ __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W
 #pragma unroll
                 for (int e = 1; e < 4; ++e)
                     if (c == e) s0 = st[e];
-                const int uy = s0.rsy + 2 * dy, ux = s0.rsx + 2 * dx;
-                const int by = clampi(uy, 0, UH - 1), bx = clampi(ux, 0, UW - 1);
-                const int by1 = clampi(uy + 1, 0, UH - 1), bx1 = clampi(ux + 1, 0, UW - 1);
                 cid[k] = c;
                 pv[k] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-                a[k] = up[(size_t)by * UW + bx];
-                bb[k] = up[(size_t)by * UW + bx1];
-                cc[k] = up[(size_t)by1 * UW + bx];
-                dd[k] = up[(size_t)by1 * UW + bx1];
-                w[k][0] = (2 - s0.rx) * (2 - s0.ry);
-                w[k][1] = s0.rx * (2 - s0.ry);
-                w[k][2] = (2 - s0.rx) * s0.ry;
-                w[k][3] = s0.rx * s0.ry;
+                rv[k] = qp_at(qp, UW >> 1, UH >> 1, s0, dy, dx);
             }
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int v = (w[k][0] * a[k] + w[k][1] * bb[k] + w[k][2] * cc[k] + w[k][3] * dd[k] + 2) >> 2;
-            const int d = abs(v - pv[k]);
+            const int d = abs(rv[k] - pv[k]);
             acc0 += cid[k] == 0 ? d : 0;
             acc1 += cid[k] == 1 ? d : 0;
             acc2 += cid[k] == 2 ? d : 0;
@@ -887,19 +994,8 @@ __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W
 }
 
 // Per-pixel float accumulation of the bounds-checked path (and of the plain
-// C build), raster order (me_utils.cpp:532-541, 843-867).  Every lane runs it
-// on the same addresses, so the loads are broadcast.
-__device__ float chain_total(float start, const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up,
-                             int UW, int UH, const Blk &b, const UpStart &st) {
-    float sum = start;
-    for (int dy = 0; dy < b.yl; ++dy) {
-        const int16_t *pr = pic + (size_t)(b.yp + dy) * W + b.xp;
-        for (int dx = 0; dx < b.xl; ++dx) sum += (float)abs(up_val(up, UW, UH, st, dy, dx) - (int)pr[dx]);
-    }
-    return sum;
-}
-
-// The same float chain computed by the whole wave (uniform arguments).  For
+// C build), raster order (me_utils.cpp:532-541, 843-867), computed by the
+// whole wave (uniform arguments).  For
 // 0 <= s < 2^24 and an integer d >= 0, s + d is exact while it stays below
 // the power of two above s: both are multiples of the spacing of s's binade.
 // So the chain only rounds where a partial sum enters a higher binade.  The
@@ -911,8 +1007,8 @@ __device__ __forceinline__ float binade_limit(float s) {
     return __uint_as_float((e + 1) << 23);
 }
 
-__device__ __noinline__ float wave_chain(float start, const int16_t *__restrict__ pic, int W, const int16_t *__restrict__ up,
-                            int UW, int UH, const Blk &b, const UpStart &st) {
+__device__ __noinline__ float wave_chain(float start, const int16_t *__restrict__ pic, int W,
+                                        const uint32_t *__restrict__ qp, int UW, int UH, const Blk &b, const UpStart &st) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int n = b.xl * b.yl;
     float s = start;
@@ -923,7 +1019,7 @@ __device__ __noinline__ float wave_chain(float start, const int16_t *__restrict_
             const int i = b0 + k * WAVE + lane;
             const int ii = i < n ? i : n - 1;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
-            const int v = abs(up_val(up, UW, UH, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+            const int v = abs(qp_at(qp, UW >> 1, UH >> 1, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
             d[k] = i < n ? v : 0;
         }
 #pragma unroll
@@ -973,79 +1069,85 @@ __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
 // only ever evaluates positions within 3 quarter pels (1 half pel at half
 // precision) of the pel vector: stage 1 moves at most one half pel per axis,
 // stage 2 one quarter pel more.  Their integer SADs do not depend on the
-// prediction, so all (2*3+1)^2 of them are tabulated per block here, from an
-// LDS copy of the up-converted reference region; the wavefront then only
-// adds costs and compares.  Record layout (SUB_SLOT ints per block, blocks in
-// wavefront order): SAD (dy+rad)*Q + dx+rad, then lambda, pel vector and pel
-// costs at 49..54.
+// prediction, so all (2*3+1)^2 of them are tabulated per block here; the
+// wavefront then only adds costs and compares.  One wave per block: the
+// picture block and, from the quarter-pel planes, the region of each of the
+// 16 phases that the candidates touch are staged in LDS as bytes (one dword
+// load brings the four horizontal phases of a position); each lane then
+// takes one candidate with packed v_sad_u8.  Record layout (SUB_SLOT ints
+// per block, blocks in wavefront order): SAD (dy+rad)*Q + dx+rad, then
+// lambda, pel vector and pel costs at 49..54.
 // ---------------------------------------------------------------------------
 #define SUB_MAXREG (2 * PEL_MAXB + 3)  // 67
 #define SUB_SLOT 56                    // 49 SADs, lambda, pel vector, pel costs
+#define SUB_PP 8                       // packed picture row: 32 bytes
 
 __device__ __forceinline__ int sub_radius(int prec) { return prec == 2 ? 3 : 1; }
 
+// qrows: rows of one phase region; qd: dwords per region row
 __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                     int ntasks, int total) {
-    __shared__ int16_t s_pic[4][PEL_MAXB * PEL_MAXB];
-    __shared__ int16_t s_up[4][SUB_MAXREG * SUB_MAXREG];
+                                                     int ntasks, int total, int qrows, int qd) {
+    extern __shared__ uint32_t s_sub[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gi = blockIdx.x * 4 + wv;
-    const bool valid = gi < total;
-    const Task tk = tasks[find_task(tasks, ntasks, valid ? gi : total - 1)];
+    if (gi >= total) return;  // each wave works alone
+    const int per_wave = PEL_MAXB * SUB_PP + 16 * qrows * qd;
+    uint32_t *s_pic = s_sub + wv * per_wave;
+    uint32_t *s_q = s_pic + PEL_MAXB * SUB_PP;  // [fy * 4 + fx][row][qd]
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
-    const int r = tk.ref, k = (valid ? gi : total - 1) - tk.offset;
-    const int X = J.xnb, W = J.W, H = J.H, UW = 2 * W, UH = 2 * H;
+    const int r = tk.ref, k = gi - tk.offset;
+    const int X = J.xnb, W = J.W, H = J.H, QW = W + 2;
     const int xb = k % X, yb = k / X;
     const int prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
-    const bool work = valid && !b.empty() && prec > 0;
     const dme_mv pel = J.vec[r][k];
     int *__restrict__ rec = J.qtab[r] + (size_t)wf_pos(xb, yb, X, J.ynb) * SUB_SLOT;
-    dme_mv c;
-    c.x = pel.x << prec;
-    c.y = pel.y << prec;
-    // up-plane region covering every candidate's samples
-    int x0, y0, RW, RH;
-    if (prec == 2) {
-        x0 = 2 * b.xp + ((c.x - rad) >> 1);
-        y0 = 2 * b.yp + ((c.y - rad) >> 1);
-        RW = ((c.x + rad) >> 1) - ((c.x - rad) >> 1) + 2 * b.xl;
-        RH = ((c.y + rad) >> 1) - ((c.y - rad) >> 1) + 2 * b.yl;
-    } else {
-        x0 = 2 * b.xp + c.x - rad;
-        y0 = 2 * b.yp + c.y - rad;
-        RW = 2 * rad + 2 * b.xl - 1;
-        RH = 2 * rad + 2 * b.yl - 1;
-    }
-    const int16_t *__restrict__ up = J.up[r];
-    if (work) {
-        wave_copy_rect(s_pic[wv], PEL_MAXB, J.pic[0], W, H, b.xp, b.yp, b.xl, b.yl);
-        wave_copy_rect(s_up[wv], SUB_MAXREG, up, UW, UH, x0, y0, RW, RH);
-    }
-    __syncthreads();
-    if (work && lane < QS) {
-        const int dx = lane % Q - rad, dy = lane / Q - rad;
-        const int mx = c.x + dx, my = c.y + dy;
-        int sx, sy, rx, ry;
-        if (prec == 2) {
-            sx = 2 * b.xp + (mx >> 1) - x0; sy = 2 * b.yp + (my >> 1) - y0; rx = mx & 1; ry = my & 1;
-        } else {
-            sx = 2 * b.xp + mx - x0; sy = 2 * b.yp + my - y0; rx = 0; ry = 0;
-        }
-        const int w0 = (2 - rx) * (2 - ry), w1 = rx * (2 - ry), w2 = (2 - rx) * ry, w3 = rx * ry;
-        int s = 0;
-        for (int y = 0; y < b.yl; ++y) {
-            const int16_t *r0 = &s_up[wv][(sy + 2 * y) * SUB_MAXREG + sx];
-            const int16_t *r1 = r0 + SUB_MAXREG;
-            const int16_t *pr = &s_pic[wv][y * PEL_MAXB];
-            for (int x = 0; x < b.xl; ++x) {
-                const int v = (w0 * r0[2 * x] + w1 * r0[2 * x + 1] + w2 * r1[2 * x] + w3 * r1[2 * x + 1] + 2) >> 2;
-                s += abs(v - (int)pr[x]);
+    if (!b.empty() && prec > 0) {
+        // candidate range in quarter pels and the integer offsets it spans
+        const int sq = 2 - prec;  // quarter pels per candidate unit
+        const int cqx = (pel.x << prec) << sq, cqy = (pel.y << prec) << sq, rq = rad << sq;
+        const int qx0 = (cqx - rq) >> 2, qy0 = (cqy - rq) >> 2;
+        const int RW = b.xl + ((cqx + rq) >> 2) - qx0, RH = b.yl + ((cqy + rq) >> 2) - qy0;
+        wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic), 4 * SUB_PP, J.pic[0], W, H, b.xp, b.yp, b.xl, b.yl,
+                      (b.xl + 15) & ~15);
+        // phase regions: one dword per (phase row fy, position) holds the four fx
+        const uint32_t *__restrict__ qp = J.qp[r];
+        const int n = 4 * RH * RW;
+        for (int i0 = 0; i0 < n; i0 += 8 * WAVE) {
+            uint32_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int i = i0 + e * WAVE + lane;
+                v[e] = 0;
+                if (i < n) {
+                    const int fy = i / (RH * RW), rem = i - fy * RH * RW;
+                    const int ry = rem / RW, rx = rem - ry * RW;
+                    const int Yq = clampi(b.yp + qy0 + ry, -1, H), Xq = clampi(b.xp + qx0 + rx, -1, W);
+                    v[e] = qp[(size_t)((Yq + 1) * 4 + fy) * QW + Xq + 1];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int i = i0 + e * WAVE + lane;
+                if (i < n) {
+                    const int fy = i / (RH * RW), rem = i - fy * RH * RW;
+                    const int ry = rem / RW, rx = rem - ry * RW;
+                    uint8_t *row = reinterpret_cast<uint8_t *>(s_q + (fy * 4) * qrows * qd + ry * qd) + rx;
+#pragma unroll
+                    for (int fx = 0; fx < 4; ++fx) row[fx * qrows * qd * 4] = (uint8_t)(v[e] >> (8 * fx));
+                }
             }
         }
-        rec[lane] = s;
+        WAVE_SYNC();
+        if (lane < QS) {
+            const int mx = (cqx >> 0) + ((lane % Q - rad) << sq), my = cqy + ((lane / Q - rad) << sq);
+            const int fx = mx & 3, fy = my & 3;
+            const uint32_t *plane = s_q + (fy * 4 + fx) * qrows * qd;
+            rec[lane] = packed_sad_u8(s_pic, SUB_PP, plane, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl);
+        }
     }
-    if (valid && lane == 0) {
+    if (lane == 0) {
         rec[49] = __float_as_int(J.lam[k]);
         rec[50] = pel.x;
         rec[51] = pel.y;
@@ -1072,13 +1174,6 @@ __device__ __forceinline__ bool chain_is_exact(float start, int isad) {
     return u <= 1.0 && fmod((double)start, u) == 0.0;
 }
 
-__device__ float cand_total(const JobDesc &J, const int16_t *up, int UW, int UH, const Blk &b, dme_mv mv, int step,
-                            int isad, float start) {
-    const UpStart st = up_start(b, mv, step, UW, UH);
-    if ((st.bounds || J.fp_sem == DME_FP_C) && !chain_is_exact(start, isad))
-        return chain_total(start, J.pic[0], J.W, up, UW, UH, b, st);
-    return (float)isad + start;
-}
 
 // ---------------------------------------------------------------------------
 // Sub-pel refinement, wavefront part (SubpelRefine::MatchPic/DoBlock/GetPred,
@@ -1110,7 +1205,7 @@ struct SubJob {
 // after that touches the counter again; an empty round trip costs one barrier.
 template <bool CHAIN>
 __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, const JobDesc &J,
-                                          const int16_t *__restrict__ up, const SubJob (&mine)[4],
+                                          const uint32_t *__restrict__ qp, const SubJob (&mine)[4],
                                           const bool (&want)[4], int (&ires)[4], float (&fres)[4]) {
     const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wv = tid >> 6, nwv = nth >> 6;
     int *cnt = cnts + phase;
@@ -1140,11 +1235,11 @@ __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, 
             m.y = q.my;
             if (CHAIN) {
                 const UpStart us = up_start(b, m, q.step, UW, UH);
-                const float v = wave_chain(q.start, J.pic[0], J.W, up, UW, UH, b, us);
+                const float v = wave_chain(q.start, J.pic[0], J.W, qp, UW, UH, b, us);
                 if (lane == 0) queue[e].fres = v;
             } else {
                 int v = 0;
-                wave_up_sads(J.pic[0], J.W, up, UW, UH, b, q.step, &m, 1, &v);
+                wave_up_sads(J.pic[0], J.W, qp, UW, UH, b, q.step, &m, 1, &v);
                 if (lane == 0) queue[e].ires = v;
             }
         }
@@ -1173,7 +1268,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
     dme_mv *ring = reinterpret_cast<dme_mv *>(s_dyn);
     int *slots = s_dyn + 8 * max_rows;
     SubJob *queue = reinterpret_cast<SubJob *>(slots + 2 * max_rows * SUB_SLOT);
-    const int16_t *__restrict__ up = J.up[r];
+    const uint32_t *__restrict__ qp = J.qp[r];
     const int *__restrict__ qtab = J.qtab[r];
     dme_mv *__restrict__ V = J.vec[r];
     dme_cost *__restrict__ C = J.pc[r];
@@ -1274,7 +1369,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                 mine[0].ires = 0;
                 mine[0].start = mine[0].fres = 0.0f;
                 mine[1] = mine[2] = mine[3] = mine[0];
-                sub_serve<false>(s_cnt, phase, queue, J, up, mine, want, ir, fr);
+                sub_serve<false>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
                 psad = ir[0];
             }
             dme_cost pcost;
@@ -1347,7 +1442,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     }
                     int ir[4] = {0, 0, 0, 0};
                     float fr[4] = {tot[0], tot[1], tot[2], tot[3]};
-                    sub_serve<true>(s_cnt, phase, queue, J, up, mine, need, ir, fr);
+                    sub_serve<true>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (on[j] && fr[j] < best.total) {
@@ -1406,18 +1501,6 @@ __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict
 
 // BiBlock{Half,Quarter}Pel::Diff (me_utils.cpp:1172-1543): sum of
 // |((2 pic - r1) - r2) >> 1| with short intermediates.
-__device__ __forceinline__ int up_interp(const int16_t *__restrict__ up, int UW, int UH, const UpStart &s, int dy,
-                                         int dx) {
-    const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
-    const unsigned y0 = clampi(uy, 0, UH - 1), x0 = clampi(ux, 0, UW - 1);
-    const unsigned y1 = clampi(uy + 1, 0, UH - 1), x1 = clampi(ux + 1, 0, UW - 1);
-    const int a = up[y0 * UW + x0];
-    const int bq = s.rx ? (int)up[y0 * UW + x1] : 0;
-    const int c = s.ry ? (int)up[y1 * UW + x0] : 0;
-    const int d = (s.rx & s.ry) ? (int)up[y1 * UW + x1] : 0;
-    return ((2 - s.rx) * (2 - s.ry) * a + s.rx * (2 - s.ry) * bq + (2 - s.rx) * s.ry * c + s.rx * s.ry * d + 2) >> 2;
-}
-
 template <int NB = 8>
 __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1425,8 +1508,8 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
     const UpStart s1 = up_start(b, m1, J.step, UW, UH);
     const UpStart s2 = up_start(b, m2, J.step, UW, UH);
     const int16_t *__restrict__ pic = J.pic[0];
-    const int16_t *__restrict__ u1 = J.up[0];
-    const int16_t *__restrict__ u2 = J.up[1];
+    const uint32_t *__restrict__ q1 = J.qp[0];
+    const uint32_t *__restrict__ q2 = J.qp[1];
     int s = 0;
     const int n = b.xl * b.yl;
     for (int b0 = 0; b0 < n; b0 += NB * WAVE) {
@@ -1439,8 +1522,8 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
             const int ii = ok[k] ? i : 0;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
             p[k] = pic[(unsigned)((b.yp + dy) * J.W + b.xp + dx)];
-            r1[k] = up_interp(u1, UW, UH, s1, dy, dx);
-            r2[k] = up_interp(u2, UW, UH, s2, dy, dx);
+            r1[k] = qp_at(q1, J.W, J.H, s1, dy, dx);
+            r2[k] = qp_at(q2, J.W, J.H, s2, dy, dx);
         }
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -1568,11 +1651,6 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
 //   m0tab[sb * 32 + r*16 + q]       superblock candidate q (block q of the SB)
 // Duplicate vectors are evaluated once.  One wave per unit.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int up_sample(const int16_t *__restrict__ up, int UW, int UH, const Blk &b, dme_mv mv,
-                                         int step, int dy, int dx) {
-    const UpStart s = up_start(b, mv, step, UW, UH);
-    return up_val(up, UW, UH, s, dy, dx);
-}
 
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                      int ntasks, int total) {
@@ -1612,7 +1690,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
             int s = 0;
             for (int i = lane; i < n; i += WAVE) {
                 const int dy = i / b.xl, dx = i - dy * b.xl;
-                s += abs(up_sample(J.up[r], UW, UH, b, cv[r][q], step, dy, dx) -
+                s += abs(qp_at(J.qp[r], W, H, up_start(b, cv[r][q], step, UW, UH), dy, dx) -
                          (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
             }
             s = wave_sum(s);
@@ -1632,8 +1710,8 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
                 for (int i = lane; i < n; i += WAVE) {
                     const int dy = i / b.xl, dx = i - dy * b.xl;
                     const int p = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-                    const int16_t diff = (int16_t)((p << 1) - up_val(J.up[0], UW, UH, s1, dy, dx));
-                    const int16_t temp = (int16_t)(((int)diff - up_val(J.up[1], UW, UH, s2, dy, dx)) >> 1);
+                    const int16_t diff = (int16_t)((p << 1) - qp_at(J.qp[0], W, H, s1, dy, dx));
+                    const int16_t temp = (int16_t)(((int)diff - qp_at(J.qp[1], W, H, s2, dy, dx)) >> 1);
                     s += abs((int)temp);
                 }
                 s = wave_sum(s);
@@ -1688,7 +1766,7 @@ __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pr
             const float start = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(st[q]), src));
             const int rs = __builtin_amdgcn_readlane(r, src);
             const UpStart us = up_start(bs, m, J.step, UW, UH);
-            const float v = wave_chain(start, J.pic[0], J.W, rs ? J.up[1] : J.up[0], UW, UH, bs, us);
+            const float v = wave_chain(start, J.pic[0], J.W, rs ? J.qp[1] : J.qp[0], UW, UH, bs, us);
             if (lane == src) tot[q] = v;
         }
     }
@@ -1897,7 +1975,7 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
                     m.x = __builtin_amdgcn_readlane(g0[q].x, src);
                     m.y = __builtin_amdgcn_readlane(g0[q].y, src);
                     int sd = 0;
-                    wave_up_sads(J.pic[0], J.W, src ? J.up[1] : J.up[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
+                    wave_up_sads(J.pic[0], J.W, src ? J.qp[1] : J.qp[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
                     if (lane == src) gs0[q] = sd;
                 }
             }
@@ -2023,9 +2101,34 @@ hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, i
     return hipGetLastError();
 }
 
-hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                              int max_blen_y, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_subpel_bulk, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    // a phase region spans the block plus two positions per axis
+    const int qrows = max_blen_y + 2, qd = 4 * (((max_blen_x + 3) / 4 + 3) / 4) + 1;  // reads reach 4 * chunks + 1
+    const size_t lds = sizeof(uint32_t) * 4 * ((size_t)PEL_MAXB * SUB_PP + 16 * (size_t)qrows * qd);
+    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_subpel_bulk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           159 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_subpel_bulk, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, qrows, qd);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 1023) / 1024, 4096);
+    hipLaunchKernelGGL(k_check_range, dim3(blocks), dim3(256), 0, st, y, n, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_qp, dim3(total), dim3(256), 0, st, pt, npt, total);
     return hipGetLastError();
 }
 

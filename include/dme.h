@@ -151,7 +151,8 @@ int dme_set_params(dme_ctx *ctx, const dme_params *params);
  * enc_queue.cpp:179, enc_picture.cpp:100).  Planes are host pointers with
  * stride = width (luma) and width/2 (chroma, 4:2:0); u and v may be NULL if
  * the picture is only ever used as a reference.  The planes are copied to
- * device memory.                                                            */
+ * device memory.  Luma samples must lie in [-128, 127] (8-bit video stored
+ * minus 128, pic_io.cpp:396-401); other values give DME_ERR_INVALID.       */
 int dme_upload_picture(dme_ctx *ctx, int pic_num, int width, int height,
                        const int16_t *y, const int16_t *u, const int16_t *v);
 
