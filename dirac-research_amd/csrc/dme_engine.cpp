@@ -241,6 +241,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
             if (P->up_ready) continue;
             const size_t n = (size_t)(P->W + 2) * 4 * (P->H + 2);  // dwords
             DME_TRY(P->qp.ensure(sizeof(uint32_t) * n));
+            const size_t npos = (size_t)(P->W + 2) * (P->H + 2);   // one thread per position
             PlaneTask t;
             t.in = P->up.as<int16_t>();
             t.out = P->qp.as<int16_t>();
@@ -248,7 +249,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
             t.ow = P->W;
             t.oh = P->H;
             t.offset = off;
-            off += (int)((n + 255) / 256);
+            off += (int)((npos + 255) / 256);
             all.push_back(t);
         }
         ranges.push_back(std::make_pair(first, off));

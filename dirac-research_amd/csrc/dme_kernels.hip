@@ -385,24 +385,32 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
     }
     const PlaneTask p = pt[lo];
     const int W = p.ow, H = p.oh, UW = 2 * W, UH = 2 * H, QW = W + 2;
+    // one thread per position (X, Y): its 16 phases use up-plane rows and
+    // columns 2Y .. 2Y + 2 and 2X .. 2X + 2
     const int i = (tile - p.offset) * 256 + threadIdx.x;
-    if (i >= QW * 4 * (H + 2)) return;
-    const int row = i / QW, X = i - row * QW - 1;
-    const int Y = (row >> 2) - 1, fy = row & 3;
+    if (i >= QW * (H + 2)) return;
+    const int yy = i / QW, X = i - yy * QW - 1, Y = yy - 1;
     const int16_t *__restrict__ up = p.in;
-    const int hy = 2 * Y + (fy >> 1), ry = fy & 1;
-    const int y0 = clampi(hy, 0, UH - 1), y1 = clampi(hy + 1, 0, UH - 1);
-    uint32_t out = 0;
+    int u[3][3];
 #pragma unroll
-    for (int fx = 0; fx < 4; ++fx) {
-        const int hx = 2 * X + (fx >> 1), rx = fx & 1;
-        const int x0 = clampi(hx, 0, UW - 1), x1 = clampi(hx + 1, 0, UW - 1);
-        const int a = up[(size_t)y0 * UW + x0], b = up[(size_t)y0 * UW + x1];
-        const int c = up[(size_t)y1 * UW + x0], d = up[(size_t)y1 * UW + x1];
-        const int v = ((2 - rx) * (2 - ry) * a + rx * (2 - ry) * b + (2 - rx) * ry * c + rx * ry * d + 2) >> 2;
-        out |= (uint32_t)(v + 128) << (8 * fx);
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            u[a][c] = up[(size_t)clampi(2 * Y + a, 0, UH - 1) * UW + clampi(2 * X + c, 0, UW - 1)];
+    uint32_t *__restrict__ out = reinterpret_cast<uint32_t *>(p.out);
+#pragma unroll
+    for (int fy = 0; fy < 4; ++fy) {
+        const int oy = fy >> 1, ry = fy & 1;
+        uint32_t d = 0;
+#pragma unroll
+        for (int fx = 0; fx < 4; ++fx) {
+            const int ox = fx >> 1, rx = fx & 1;
+            const int v = ((2 - rx) * (2 - ry) * u[oy][ox] + rx * (2 - ry) * u[oy][ox + 1] +
+                           (2 - rx) * ry * u[oy + 1][ox] + rx * ry * u[oy + 1][ox + 1] + 2) >> 2;
+            d |= (uint32_t)(v + 128) << (8 * fx);
+        }
+        out[(size_t)(yy * 4 + fy) * QW + X + 1] = d;
     }
-    reinterpret_cast<uint32_t *>(p.out)[i] = out;
 }
 
 // Wave copy of a clamped rectangle of an 8-bit-range plane into LDS as
@@ -634,6 +642,10 @@ __device__ __forceinline__ int wave_min_i(int v) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  \
     } while (0)
 
+// U8: the level's planes hold 8-bit-range samples (level 0: the pictures
+// themselves), staged as bytes for v_sad_u8; otherwise (pyramid levels, whose
+// 12-tap filter can overshoot that range) as 16-bit samples for v_sad_u16.
+template <bool U8>
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total, int level) {
     // packed staging: block rows of PEL_PP dwords, region rows of PEL_RP dwords
@@ -673,34 +685,50 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     int mykey = INT_MAX;  // lane t*9 + w keeps the key of window w of table t
     if (!b.empty()) {
         const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
-        wave_stage_u16(reinterpret_cast<uint16_t *>(s_pic[wv]), 2 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
-                       (b.xl + 7) & ~7);
-        wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][0]), 2 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW, RH,
-                       RW);
-        if (has_guide)
-            wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][1]), 2 * PEL_RP, ref, W, H, b.xp + g.x - R,
-                           b.yp + g.y - R, RW, RH, RW);
+        if (U8) {
+            wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic[wv]), 4 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
+                          (b.xl + 15) & ~15);
+            wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW, RH,
+                          RW);
+            if (has_guide)
+                wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][1]), 4 * PEL_RP, ref, W, H, b.xp + g.x - R,
+                              b.yp + g.y - R, RW, RH, RW);
+        } else {
+            wave_stage_u16(reinterpret_cast<uint16_t *>(s_pic[wv]), 2 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
+                           (b.xl + 7) & ~7);
+            wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][0]), 2 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW,
+                           RH, RW);
+            if (has_guide)
+                wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][1]), 2 * PEL_RP, ref, W, H, b.xp + g.x - R,
+                               b.yp + g.y - R, RW, RH, RW);
+        }
         WAVE_SYNC();
         for (int c = lane; c < ntab * TS; c += WAVE) {
             const int t = c >= TS ? 1 : 0, e = c - t * TS;
             const int ox = e % D, oy = e / D;  // region offset of the candidate
-            s_tab[wv][c] = packed_sad(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl);
+            s_tab[wv][c] = U8 ? packed_sad_u8(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl)
+                              : packed_sad(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl);
         }
         WAVE_SYNC();
         // first minima (key = SAD * 128 + enumeration index) of the nine
-        // windows around each centre; lane c holds candidates c and c + 64
-        const dme_mv o0 = win_off(lane < n ? lane : 0, rad), o1 = win_off(lane + 64 < n ? lane + 64 : 0, rad);
-        for (int t = 0; t < ntab; ++t)
-#pragma unroll
-            for (int w = 0; w < 9; ++w) {
-                const int ddx = w % 3 - 1 + R, ddy = w / 3 - 1 + R;
-                const int *tb = s_tab[wv] + t * TS;
-                int v = INT_MAX;
-                if (lane < n) v = tb[(ddy + o0.y) * D + ddx + o0.x] * 128 + lane;
-                if (lane + 64 < n) v = min(v, tb[(ddy + o1.y) * D + ddx + o1.x] * 128 + lane + 64);
-                v = wave_min_i(v);
-                if (lane == t * 9 + w) mykey = v;
-            }
+        // windows around each centre: the keys of every (window, candidate)
+        // go to LDS (over the dead region staging), then lane t*9 + w scans
+        // its window's n keys
+        int *kb = reinterpret_cast<int *>(s_reg[wv][0]);
+        const int nk = ntab * 9 * n;
+        for (int i = lane; i < nk; i += WAVE) {
+            const int tw = i / n, c = i - tw * n;
+            const int t = tw >= 9 ? 1 : 0, w = tw - 9 * t;
+            const dme_mv o = win_off(c, rad);
+            kb[i] = s_tab[wv][t * TS + (w / 3 - 1 + R + o.y) * D + (w % 3 - 1 + R + o.x)] * 128 + c;
+        }
+        WAVE_SYNC();
+        if (lane < ntab * 9) {
+            const int *kw = kb + lane * n;
+            int v = kw[0];
+            for (int c = 1; c < n; ++c) v = min(v, kw[c]);
+            mykey = v;
+        }
         // zero list, then guide list (radius rad, centred windows)
         const int k0 = __shfl(mykey, 4, WAVE), k1 = __shfl(mykey, 13, WAVE);
         best_sad = k0 >> 7;
@@ -1113,26 +1141,24 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
                       (b.xl + 15) & ~15);
         // phase regions: one dword per (phase row fy, position) holds the four fx
         const uint32_t *__restrict__ qp = J.qp[r];
-        const int n = 4 * RH * RW;
-        for (int i0 = 0; i0 < n; i0 += 8 * WAVE) {
+        // lanes keep a column; rows run over (fy, ry), RW <= 34
+        const int rpi = WAVE / RW, rx = lane % RW, row0 = lane / RW;
+        const bool on = row0 < rpi;
+        const int Xq = clampi(b.xp + qx0 + rx, -1, W) + 1, NR = 4 * RH;
+        for (int R0 = 0; R0 < NR; R0 += 8 * rpi) {
             uint32_t v[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int i = i0 + e * WAVE + lane;
+                const int R = R0 + e * rpi + row0;
+                const int fy = (R >= RH) + (R >= 2 * RH) + (R >= 3 * RH), ry = R - fy * RH;
                 v[e] = 0;
-                if (i < n) {
-                    const int fy = i / (RH * RW), rem = i - fy * RH * RW;
-                    const int ry = rem / RW, rx = rem - ry * RW;
-                    const int Yq = clampi(b.yp + qy0 + ry, -1, H), Xq = clampi(b.xp + qx0 + rx, -1, W);
-                    v[e] = qp[(size_t)((Yq + 1) * 4 + fy) * QW + Xq + 1];
-                }
+                if (on && R < NR) v[e] = qp[(size_t)((clampi(b.yp + qy0 + ry, -1, H) + 1) * 4 + fy) * QW + Xq];
             }
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int i = i0 + e * WAVE + lane;
-                if (i < n) {
-                    const int fy = i / (RH * RW), rem = i - fy * RH * RW;
-                    const int ry = rem / RW, rx = rem - ry * RW;
+                const int R = R0 + e * rpi + row0;
+                const int fy = (R >= RH) + (R >= 2 * RH) + (R >= 3 * RH), ry = R - fy * RH;
+                if (on && R < NR) {
                     uint8_t *row = reinterpret_cast<uint8_t *>(s_q + (fy * 4) * qrows * qd + ry * qd) + rx;
 #pragma unroll
                     for (int fx = 0; fx < 4; ++fx) row[fx * qrows * qd * 4] = (uint8_t)(v[e] >> (8 * fx));
@@ -2073,7 +2099,11 @@ hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t
 
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pel_bulk, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total, level);
+    if (level == 0)  // the pictures themselves: samples checked to 8-bit range at registration
+        hipLaunchKernelGGL(k_pel_bulk<true>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total, level);
+    else
+        hipLaunchKernelGGL(k_pel_bulk<false>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total,
+                           level);
     return hipGetLastError();
 }
 

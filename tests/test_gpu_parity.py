@@ -175,3 +175,18 @@ def test_errors_are_reported(ctx):
         ctx.motion_estimate(999, [0])  # unregistered picture
     with pytest.raises(dirac_me.DmeError):
         ctx.subpel(12345)  # no pel search done
+
+
+@pytest.mark.gpu
+def test_luma_outside_8bit_range_is_rejected(ctx):
+    # samples are 8-bit values minus 128 (pic_io.cpp:396-401); the packed byte
+    # SADs depend on it, so registration checks the range
+    y = np.zeros((48, 64), np.int16)
+    ctx.upload(500, y)  # in range: accepted
+    y[7, 9] = 128
+    with pytest.raises(dirac_me.DmeError):
+        ctx.upload(501, y)
+    y[7, 9] = -129
+    with pytest.raises(dirac_me.DmeError):
+        ctx.upload(502, y)
+    ctx.drop(500)
