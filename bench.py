@@ -24,14 +24,17 @@ INT_LANE_OPS_PEAK = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes per clock x 2.4 GHz
 
 
 def _geometry_terms(geom, nrefs):
-    """Algorithmic HBM bytes and SAD terms per picture for the main kernels."""
-    import numpy as np  # noqa: F401
+    """Algorithmic HBM bytes and SAD terms per picture for the main kernels.
 
+    Bytes count what a kernel reads and writes per work unit with no reuse
+    between units (int16 samples, int32 table entries); terms count the
+    absolute differences of the SADs it evaluates.
+    """
     xl, yl = geom.luma_blen[2][0], geom.luma_blen[2][1]
     B = geom.xnum_blocks * geom.ynum_blocks
     out = {}
-    # pel search bulk: picture block + two reference regions read, two SAD
-    # tables and the zero/guide winner written, per block and level
+    # pel tables: picture block + one or two reference regions of radius
+    # R = r + 1 read, a 28-int record written, per block, level and reference
     b_bytes = t_terms = w_bytes = 0.0
     for l in range(geom.depth + 1):
         nb = geom.level_xblocks[l] * geom.level_yblocks[l]
@@ -39,25 +42,62 @@ def _geometry_terms(geom, nrefs):
         R = r + 1
         TS = (2 * R + 1) ** 2
         ntab = 2 if l < geom.depth else 1
-        b_bytes += nb * nrefs * (2 * xl * yl + ntab * 2 * (xl + 2 * R) * (yl + 2 * R) + ntab * TS * 4 + 12)
+        b_bytes += nb * nrefs * (2 * xl * yl + ntab * 2 * (xl + 2 * R) * (yl + 2 * R) + 8 + 28 * 4)
         t_terms += nb * nrefs * ntab * TS * xl * yl
-        w_bytes += nb * nrefs * ((2 * r + 1) ** 2 * 4 + 12 + 8 + 20)
+        w_bytes += nb * nrefs * (28 * 4 + 8 + 12)
     out["k_pel_bulk"] = (b_bytes, t_terms)
     out["k_pel_wave"] = (w_bytes, 0.0)
-    # sub-pel bulk: picture block + up-converted region read, 49 SADs written
-    out["k_subpel_bulk"] = (B * nrefs * (2 * xl * yl + 2 * (2 * xl + 3) * (2 * yl + 3) + 49 * 4), B * nrefs * 49 * xl * yl)
-    out["k_subpel_wave"] = (B * nrefs * (49 * 4 + 4 + 8 + 12 + 20), 0.0)
+    # sub-pel tables: picture block + 16 quarter-pel phase regions of
+    # (xl + 2) x (yl + 2) bytes read, a 56-int record written
+    out["k_subpel_bulk"] = (B * nrefs * (2 * xl * yl + 16 * (xl + 2) * (yl + 2) + 8 + 12 + 4 + 56 * 4),
+                            B * nrefs * 49 * xl * yl)
+    out["k_subpel_wave"] = (B * nrefs * (56 * 4 + 8 + 12), 0.0)
     # mode decision: level-1 and level-0 unit tables, superblock wavefront
     l1 = geom.luma_blen[1][0] * geom.luma_blen[1][1]
     l0 = geom.luma_blen[0][0] * geom.luma_blen[0][1]
     nu1, nsb = B // 4, B // 16
-    out["k_mode_tables"] = (nu1 * (2 * l1 + nrefs * 4 * 2 * l1 + 24 * 4) + nsb * (2 * l0 + nrefs * 16 * 2 * l0 + 32 * 4),
+    out["k_mode_tables"] = (nu1 * (2 * l1 + nrefs * 4 * 4 * l1 + 24 * 4) + nsb * (2 * l0 + nrefs * 16 * 4 * l0 + 32 * 4),
                             nu1 * (nrefs * 4 + 16) * l1 + nsb * nrefs * 16 * l0)
-    out["k_mode_wave"] = (nsb * (16 * 4 + 4 * 24 * 4 + 32 * 4 + 2 * l0 * 3 + 16 * 20), nsb * l0)
+    out["k_mode_wave"] = (nsb * (16 * 4 + 4 * 24 * 4 + 32 * 4 + 2 * l0 * 9 + 16 * 20), nsb * l0)
     out["k_upconvert"] = (nrefs * geom.width * geom.height * 2 * 5, 0.0)
+    out["k_build_qp"] = (nrefs * (geom.width * geom.height * 4 * 2 + (geom.width + 2) * (geom.height + 2) * 16), 0.0)
     out["k_downconvert"] = (sum((1 + nrefs) * (geom.level_width[l] * geom.level_height[l] * 2 * 5)
                                 for l in range(1, geom.depth + 1)), 0.0)
     return out
+
+
+def _profiled_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters on
+    this workload, from the summary kept in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "pel_bulk_1080p_2ref_batch64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    _, rest = line.split(":", 1)
+                    fields = dict(kv.strip().split("=") for kv in rest.split(","))
+                    if fields.get("kernel") == kernel and int(fields.get("batch", -1)) == batch:
+                        return float(fields["bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
+
+
+def _copy_calibration(mb):
+    """Device-to-device copy of `mb` MiB (the HIP runtime's copy kernel), so
+    that a counter run reads FETCH_SIZE / WRITE_SIZE of a kernel whose traffic
+    is known."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(frames, params, budget_s):
@@ -97,9 +137,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the motion-estimation path runs one picture batch per GPU and "
@@ -155,7 +197,8 @@ def main():
     per_launch_ms = ms / nl
     achieved = alg_bytes * B / nl / (per_launch_ms * 1e-3) / 1e9
     roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": per_launch_ms,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": _profiled_traffic(dom, B),
+            "algorithmic_bytes_per_launch": alg_bytes * B / nl, "avg_launch_ms": per_launch_ms,
             "launches_per_step": nl}
     valu = {}
     for k in ("k_pel_bulk", "k_subpel_bulk", "k_mode_tables"):
@@ -177,6 +220,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(frames, params, args.cpu_budget)
     ctx.close()
+    if args.copy_calibration_mb:
+        _copy_calibration(args.copy_calibration_mb)
     print(json.dumps(out))
 
 

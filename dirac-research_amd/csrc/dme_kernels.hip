@@ -37,6 +37,16 @@ namespace dme {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin, and each XCD has its
+// own L2.  xcd_block maps the dispatch index to a logical index so that each
+// XCD receives one contiguous range of the grid: neighbouring blocks, whose
+// reference regions overlap, then share an L2.
+__device__ __forceinline__ int xcd_block() {
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 __device__ __forceinline__ int find_task(const Task *t, int n, int gi) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -653,7 +663,7 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     __shared__ uint32_t s_reg[4][2][PEL_REG * PEL_RP];
     __shared__ int s_tab[4][2 * PEL_MAXTS];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gi = blockIdx.x * 4 + wv;
+    const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;  // each wave works alone
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -1117,7 +1127,7 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
                                                      int ntasks, int total, int qrows, int qd) {
     extern __shared__ uint32_t s_sub[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gi = blockIdx.x * 4 + wv;
+    const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;  // each wave works alone
     const int per_wave = PEL_MAXB * SUB_PP + 16 * qrows * qd;
     uint32_t *s_pic = s_sub + wv * per_wave;
@@ -1589,7 +1599,7 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 // parallel: it only reads the block's own refined vectors and costs.
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
-    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -1680,7 +1690,7 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
 
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                      int ntasks, int total) {
-    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -2041,7 +2051,7 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total) {
-    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
