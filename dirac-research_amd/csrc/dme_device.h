@@ -1,5 +1,6 @@
 // dme_device.h -- device-side job descriptors shared by the kernels
-// (dme_kernels.hip) and the host orchestration (dme_engine.hip).
+// (dme_kernels.hip, dme_learned.hip) and the host orchestration
+// (dme_engine.cpp).
 #ifndef DME_DEVICE_H
 #define DME_DEVICE_H
 
@@ -38,6 +39,7 @@ struct JobDesc {
     const int16_t DME_G *up[2];                       // 2W x 2H up-converted refs
     const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
+    int lp_off;                                       // first position of the picture in the proposal tensors
     // pel search, per reference and level ([0] aliases the MEData arrays)
     dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];
     dme_cost DME_G *lpc[2][DME_MAX_DEPTH + 1];

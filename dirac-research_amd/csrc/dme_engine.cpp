@@ -15,6 +15,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -22,6 +23,7 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/dme_learned.h"
 #include "dme_device.h"
 #include "dme_internal.h"
 
@@ -36,6 +38,22 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
                               int max_blen_y, hipStream_t st);
 hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st);
+hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks, int total, uint16_t *feat, float *lab,
+                              hipStream_t st);
+hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4, int *stats,
+                            hipStream_t st);
+hipError_t launch_lp_pel_costs(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_lp_frags(const float *w, uint16_t *fwd, uint16_t *bwd, hipStream_t st);
+hipError_t launch_lp_conv(const uint16_t *in, int npos, int H, int W, const uint16_t *frag, const float *bias, int relu,
+                          uint16_t *out, float *out_f32, hipStream_t st);
+hipError_t launch_lp_wgrad(const uint16_t *in, const uint16_t *g, int npos, int H, int W, float *dw, float *db,
+                           hipStream_t st);
+hipError_t launch_lp_relu_bwd(const uint16_t *act, const float *g, size_t count, uint16_t *out, hipStream_t st);
+hipError_t launch_lp_adam(float *p, const float *g, float *m, float *v, int count, float lr, float b1, float b2,
+                          float eps, float bc1, float bc2, hipStream_t st);
+hipError_t launch_lp_wta(const float *out, const float *lab, int npos, float alpha, float beta, float scale,
+                         float *dout, float *loss, hipStream_t st);
+hipError_t launch_lp_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
@@ -138,6 +156,11 @@ struct dme_ctx {
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
     dme::DevBuf d_jobs, d_tasks, d_planes, d_flag;
     dme::PinnedBuf pin;  // lambda-stage staging: level-0 vectors in, lambda maps out
+    // learned proposal: weights (f32 master), BF16 fragments per layer,
+    // activations of a batch, per-job counters
+    bool lp_ready = false;
+    dme::DevBuf lp_w, lp_frag, lp_act[4], lp_out, lp_stats;
+    long long lp_blocks = 0, lp_fallbacks = 0;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dme_stats stats;
@@ -152,12 +175,13 @@ namespace dme {
 
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
-    KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_COUNT
+    KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
+    KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
-    "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp", "", "",
-    ""};
+    "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
+    "k_lp_net", "k_lp_refine", ""};
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
 template <class F>
@@ -470,7 +494,8 @@ static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->
 
 // ---- stages ---------------------------------------------------------------
 
-static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+// Plain pel search of levels depth .. stop (stop 0: the whole search).
+static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, int stop = 0) {
     int maxd = 0;
     // the up-converted planes are built with the pyramids: one pass over the
     // pictures, and the sub-pel stage then finds them ready
@@ -479,7 +504,7 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
     std::vector<TaskTable::Range> bulk(maxd + 1);
-    for (int l = maxd; l >= 0; --l) {
+    for (int l = maxd; l >= stop; --l) {
         std::vector<Task> t;
         int off = 0;
         for (size_t i = 0; i < js.size(); ++i)
@@ -497,7 +522,7 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     const TaskTable::Range wave = tt.add(wt);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
-    for (int l = maxd; l >= 0; --l) {
+    for (int l = maxd; l >= stop; --l) {
         int rows = 1, bx = 1, by = 1;
         for (auto *j : js)
             if (l <= j->g.depth) {
@@ -984,5 +1009,268 @@ extern "C" const char *dme_kernel_name(int k) {
 extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) {
     if (!ctx || !out) return set_error(DME_ERR_INVALID, "NULL argument");
     *out = ctx->stats;
+    return DME_OK;
+}
+
+// ---- learned proposal --------------------------------------------------------
+
+namespace dme {
+
+// Pel stage with learned proposals: the plain search down to level 2, then
+// the network over every picture's level-0 block grid (pictures with equal
+// grids form one NHWC tensor), exact refinement and the pel costs.
+static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    if (!ctx->lp_ready) return set_error(DME_ERR_STATE, "dme_lp_set_weights was not called");
+    for (auto *j : js)
+        if (j->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
+    DME_TRY(stage_pel(ctx, js, launches, 2));
+    // group pictures by block grid; offsets in the proposal tensors
+    std::vector<size_t> order(js.size());
+    for (size_t i = 0; i < js.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return std::make_pair(js[a]->g.xnb, js[a]->g.ynb) < std::make_pair(js[b]->g.xnb, js[b]->g.ynb);
+    });
+    struct Group { int off, n, W, H; };
+    std::vector<Group> groups;
+    int P = 0;
+    for (size_t o = 0; o < order.size(); ++o) {
+        JobState *j = js[order[o]];
+        if (groups.empty() || groups.back().W != j->g.xnb || groups.back().H != j->g.ynb)
+            groups.push_back(Group{P, 0, j->g.xnb, j->g.ynb});
+        groups.back().n++;
+        j->desc.lp_off = P;
+        P += j->g.xnb * j->g.ynb;
+    }
+    for (int a = 0; a < 4; ++a) DME_TRY(ctx->lp_act[a].ensure(sizeof(uint16_t) * 32 * (size_t)P));
+    DME_TRY(ctx->lp_out.ensure(sizeof(float) * 32 * (size_t)P));
+    DME_TRY(ctx->lp_stats.ensure(sizeof(int) * 2 * js.size()));
+    DME_TRY(upload_jobs(ctx, js));
+    HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(int) * 2 * js.size(), ctx->stream));
+    TaskTable tt;
+    std::vector<Task> fb, rb;
+    int offf = 0, offr = 0;
+    for (size_t i = 0; i < js.size(); ++i) {
+        const int cnt = js[i]->g.xnb * js[i]->g.ynb;
+        fb.push_back(Task{(int)i, 0, cnt, offf});
+        offf += cnt;
+        for (int r = 0; r < js[i]->nrefs; ++r) {
+            rb.push_back(Task{(int)i, r, cnt, offr});
+            offr += cnt;
+        }
+    }
+    const TaskTable::Range feat = tt.add(fb), refine = tt.add(rb);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    uint16_t *act[4];
+    for (int a = 0; a < 4; ++a) act[a] = ctx->lp_act[a].as<uint16_t>();
+    const uint16_t *frag = ctx->lp_frag.as<uint16_t>();
+    const float *w = ctx->lp_w.as<float>();
+    DME_TRY(timed_launch(ctx, KK_LP_NET, launches, [&]() -> hipError_t {
+        hipError_t e = launch_lp_features(dj, dtask(ctx, feat), feat.n, feat.total, act[0], nullptr, ctx->stream);
+        for (const Group &g : groups) {
+            const size_t o = (size_t)g.off * 32;
+            const int npos = g.n * g.W * g.H;
+            for (int layer = 0; layer < 4 && e == hipSuccess; ++layer) {
+                const uint16_t *in = act[layer] + o;
+                const uint16_t *fr = frag + (size_t)layer * 2 * DME_LP_FRAG_ELEMS;  // forward fragments
+                const float *bias = w + (size_t)layer * DME_LP_LAYER_PARAMS + 9 * 32 * 32;
+                if (layer < 3)
+                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 1, act[layer + 1] + o, nullptr, ctx->stream);
+                else
+                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 0, nullptr, ctx->lp_out.as<float>() + o,
+                                       ctx->stream);
+            }
+        }
+        return e;
+    }));
+    DME_TRY(timed_launch(ctx, KK_LP_REFINE, launches, [&]() -> hipError_t {
+        hipError_t e = launch_lp_refine(dj, dtask(ctx, refine), refine.n, refine.total, ctx->lp_out.as<float>(),
+                                        ctx->lp_stats.as<int>(), ctx->stream);
+        if (e == hipSuccess) e = launch_lp_pel_costs(dj, dtask(ctx, refine), refine.n, refine.total, ctx->stream);
+        return e;
+    }));
+    std::vector<int> st(2 * js.size());
+    HIP_TRY(hipMemcpyAsync(st.data(), ctx->lp_stats.p, sizeof(int) * st.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < js.size(); ++i) {
+        ctx->lp_blocks += st[2 * i];
+        ctx->lp_fallbacks += st[2 * i + 1];
+    }
+    for (auto *j : js) j->stage = 1;
+    return DME_OK;
+}
+
+static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
+    int launches = 0;
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(stage_pel_learned(ctx, js, launches));
+    DME_TRY(stage_lambda(ctx, js, nullptr));
+    DME_TRY(stage_subpel(ctx, js, launches));
+    DME_TRY(stage_mode(ctx, js, launches));
+    DME_TRY(end_timing(ctx, launches));
+    return DME_OK;
+}
+
+}  // namespace dme
+
+extern "C" int dme_lp_set_weights(dme_ctx *ctx, const float *params, size_t count) {
+    CTX_CHECK(ctx);
+    if (!params || count != (size_t)DME_LP_PARAMS)
+        return set_error(DME_ERR_INVALID, "the network has " + std::to_string(DME_LP_PARAMS) + " parameters");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(params[i])) return set_error(DME_ERR_INVALID, "non-finite network parameter");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    DME_TRY(ctx->lp_w.ensure(sizeof(float) * count));
+    DME_TRY(ctx->lp_frag.ensure(sizeof(uint16_t) * 4 * 2 * DME_LP_FRAG_ELEMS));
+    HIP_TRY(hipMemcpy(ctx->lp_w.p, params, sizeof(float) * count, hipMemcpyHostToDevice));
+    for (int layer = 0; layer < 4; ++layer) {
+        uint16_t *f = ctx->lp_frag.as<uint16_t>() + (size_t)layer * 2 * DME_LP_FRAG_ELEMS;
+        HIP_TRY(launch_lp_frags(ctx->lp_w.as<float>() + (size_t)layer * DME_LP_LAYER_PARAMS, f, f + DME_LP_FRAG_ELEMS,
+                                ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->lp_ready = true;
+    return DME_OK;
+}
+
+extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+    CTX_CHECK(ctx);
+    if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
+    JobState *J = job_for(ctx, pic_num);
+    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    std::vector<JobState *> js{J};
+    DME_TRY(run_jobs_learned(ctx, js));
+    return export_job(ctx, *J, out);
+}
+
+extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) {
+    CTX_CHECK(ctx);
+    if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
+    std::vector<JobState *> js;
+    for (int i = 0; i < njobs; ++i) {
+        for (int k = 0; k < i; ++k)
+            if (jobs[k].pic_num == jobs[i].pic_num) return set_error(DME_ERR_INVALID, "a picture appears twice in a batch");
+        JobState *J = job_for(ctx, jobs[i].pic_num);
+        DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs));
+        js.push_back(J);
+    }
+    DME_TRY(run_jobs_learned(ctx, js));
+    if (outs)
+        for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks) {
+    CTX_CHECK(ctx);
+    if (blocks) *blocks = ctx->lp_blocks;
+    if (fallbacks) *fallbacks = ctx->lp_fallbacks;
+    return DME_OK;
+}
+
+extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_features, float *d_labels) {
+    CTX_CHECK(ctx);
+    if (!d_features || !d_labels) return set_error(DME_ERR_INVALID, "NULL output");
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 1)
+        return set_error(DME_ERR_STATE, "no pixel search has run for picture " + std::to_string(pic_num));
+    JobState *J = it->second.get();
+    if (J->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
+    std::vector<JobState *> js{J};
+    int launches = 0;
+    DME_TRY(bind_planes(ctx, js, true, launches));
+    J->desc.lp_off = 0;
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    const int cnt = J->g.xnb * J->g.ynb;
+    const TaskTable::Range feat = tt.add(std::vector<Task>{Task{0, 0, cnt, 0}});
+    DME_TRY(upload_tasks(ctx, tt));
+    HIP_TRY(launch_lp_features(ctx->d_jobs.as<JobDesc>(), dtask(ctx, feat), feat.n, feat.total, d_features, d_labels,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_prepare(dme_ctx *ctx, const float *d_w, uint16_t *d_frag_fwd, uint16_t *d_frag_bwd) {
+    CTX_CHECK(ctx);
+    if (!d_w || !d_frag_fwd || !d_frag_bwd) return set_error(DME_ERR_INVALID, "NULL argument");
+    HIP_TRY(launch_lp_frags(d_w, d_frag_fwd, d_frag_bwd, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_conv(dme_ctx *ctx, const uint16_t *d_in, int n, int h, int w, const uint16_t *d_frag,
+                           const float *d_bias, int relu, uint16_t *d_out, float *d_out_f32) {
+    CTX_CHECK(ctx);
+    if (!d_in || !d_frag || (!d_out && !d_out_f32) || n <= 0 || h <= 0 || w <= 0)
+        return set_error(DME_ERR_INVALID, "bad convolution arguments");
+    HIP_TRY(launch_lp_conv(d_in, n * h * w, h, w, d_frag, d_bias, relu, d_out, d_out_f32, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_wgrad(dme_ctx *ctx, const uint16_t *d_in, const uint16_t *d_g, int n, int h, int w, float *d_dw,
+                            float *d_db) {
+    CTX_CHECK(ctx);
+    if (!d_in || !d_g || !d_dw || !d_db || n <= 0 || h <= 0 || w <= 0)
+        return set_error(DME_ERR_INVALID, "bad gradient arguments");
+    HIP_TRY(launch_lp_wgrad(d_in, d_g, n * h * w, h, w, d_dw, d_db, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_relu_backward(dme_ctx *ctx, const uint16_t *d_act, const float *d_g, size_t count,
+                                    uint16_t *d_out) {
+    CTX_CHECK(ctx);
+    if (!d_g || !d_out) return set_error(DME_ERR_INVALID, "NULL argument");
+    HIP_TRY(launch_lp_relu_bwd(d_act, d_g, count, d_out, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_wta_loss(dme_ctx *ctx, const float *d_out, const float *d_labels, int npos, float alpha,
+                               float beta, float scale, float *d_dout, float *d_loss) {
+    CTX_CHECK(ctx);
+    if (!d_out || !d_labels || !d_dout || !d_loss || npos <= 0) return set_error(DME_ERR_INVALID, "bad loss arguments");
+    HIP_TRY(launch_lp_wta(d_out, d_labels, npos, alpha, beta, scale, d_dout, d_loss, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_adam(dme_ctx *ctx, float *d_param, const float *d_grad, float *d_m, float *d_v, int count,
+                           float lr, float beta1, float beta2, float eps, int t) {
+    CTX_CHECK(ctx);
+    if (!d_param || !d_grad || !d_m || !d_v || count <= 0 || t < 1) return set_error(DME_ERR_INVALID, "bad Adam arguments");
+    const float bc1 = 1.0f - std::pow(beta1, (float)t), bc2 = 1.0f - std::pow(beta2, (float)t);
+    HIP_TRY(launch_lp_adam(d_param, d_grad, d_m, d_v, count, lr, beta1, beta2, eps, bc1, bc2, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_zero(dme_ctx *ctx, void *d_ptr, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!d_ptr || (bytes & 3)) return set_error(DME_ERR_INVALID, "zeroing needs a pointer and a multiple of 4 bytes");
+    HIP_TRY(launch_lp_zero(d_ptr, bytes, ctx->stream));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_alloc(dme_ctx *ctx, size_t bytes, void **d_ptr) {
+    CTX_CHECK(ctx);
+    if (!d_ptr || bytes == 0) return set_error(DME_ERR_INVALID, "bad allocation arguments");
+    *d_ptr = nullptr;
+    if (hipMalloc(d_ptr, bytes) != hipSuccess) {
+        *d_ptr = nullptr;
+        (void)hipGetLastError();
+        return set_error(DME_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return DME_OK;
+}
+
+extern "C" int dme_lp_free(dme_ctx *ctx, void *d_ptr) {
+    CTX_CHECK(ctx);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (d_ptr) HIP_TRY(hipFree(d_ptr));
+    return DME_OK;
+}
+
+extern "C" int dme_lp_copy(dme_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) {
+    CTX_CHECK(ctx);
+    if (!dst || !src || kind < 1 || kind > 3) return set_error(DME_ERR_INVALID, "bad copy arguments");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return DME_OK;
 }

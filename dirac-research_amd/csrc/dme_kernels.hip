@@ -2045,6 +2045,223 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Learned proposal stage (no counterpart in the reference).  The plain pel
+// search runs down to pyramid level 2 (quarter resolution); a small CNN on
+// the level-0 block grid (dme_learned.hip) turns the level-2 vectors, their
+// costs and the picture's texture into four candidate vectors per block and
+// reference; exact integer SADs over a +-2 pel window around each candidate
+// pick the level-0 pel vector, with a wider exact search where the result
+// costs more than 1.2 times the network's predicted SAD.  The vectors then go
+// through the unchanged sub-pel and mode-decision stages.
+// ---------------------------------------------------------------------------
+#define LP_SCALE_PEL 8.0f  // pel per unit of the network's offset outputs
+#define LP_RP 11           // packed region row, dwords (regions up to 40 bytes wide)
+#define LP_RROWS 40        // region rows
+
+__device__ __forceinline__ uint16_t f2bf_rne(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// level-2 vector and SAD covering level-0 block (x, y)
+__device__ __forceinline__ void lp_guide(const JobDesc &J, int r, int x, int y, dme_mv &v2, float &s2) {
+    const int X2 = J.lxb[2], Y2 = J.lyb[2];
+    const size_t k2 = (size_t)min(y >> 2, Y2 - 1) * X2 + min(x >> 2, X2 - 1);
+    v2 = J.lvec[r][2][k2];
+    s2 = J.lpc[r][2][k2].SAD;
+}
+
+// Input features of the network, one position per level-0 block:
+//   3r + 0, 1: level-2 vector in pel / 32; 3r + 2: log(1 + level-2 SAD) / 8;
+//   6, 7: mean absolute horizontal / vertical gradient of the level-2
+//   picture over the 4 x 4 samples under the block, / 32; the rest zero.
+// With labels: target offset of the plain level-0 vector from 4 x the
+// level-2 vector, log(1 + its SAD), valid.
+__global__ __launch_bounds__(256) void k_lp_features(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                     int ntasks, int total, uint16_t *__restrict__ feat,
+                                                     float *__restrict__ lab) {
+    const int gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
+    const size_t p = (size_t)J.lp_off + k;
+    float f[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) f[c] = 0.0f;
+    dme_mv v2[2];
+    v2[0].x = v2[0].y = v2[1].x = v2[1].y = 0;
+    for (int r = 0; r < J.nrefs; ++r) {
+        float s2;
+        lp_guide(J, r, x, y, v2[r], s2);
+        f[3 * r] = 4.0f * v2[r].x / 32.0f;
+        f[3 * r + 1] = 4.0f * v2[r].y / 32.0f;
+        f[3 * r + 2] = log1pf(s2) / 8.0f;
+    }
+    {
+        const int W2 = J.lw[2], H2 = J.lh[2];
+        const int cx = (x * J.bsep[2][0] + J.bsep[2][0] / 2) >> 2, cy = (y * J.bsep[2][1] + J.bsep[2][1] / 2) >> 2;
+        const int16_t *__restrict__ pl = J.pic[2];
+        int gx = 0, gy = 0;
+        for (int dy = -2; dy < 2; ++dy)
+            for (int dx = -2; dx < 2; ++dx) {
+                const int yy = clampi(cy + dy, 0, H2 - 1), xx = clampi(cx + dx, 0, W2 - 1);
+                const int v = pl[(size_t)yy * W2 + xx];
+                gx += abs(pl[(size_t)yy * W2 + clampi(xx + 1, 0, W2 - 1)] - v);
+                gy += abs(pl[(size_t)clampi(yy + 1, 0, H2 - 1) * W2 + xx] - v);
+            }
+        f[6] = gx / (16.0f * 32.0f);
+        f[7] = gy / (16.0f * 32.0f);
+    }
+    uint16_t *fo = feat + p * 32;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) fo[c] = c < 8 ? f2bf_rne(f[c]) : (uint16_t)0;
+    if (lab) {
+        const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
+        float *lo = lab + p * 8;
+        for (int r = 0; r < 2; ++r) {
+            const bool valid = r < J.nrefs && !b.empty();
+            const dme_mv v0 = valid ? J.vec[r][k] : v2[r];
+            lo[4 * r] = (float)(v0.x - 4 * v2[r].x);
+            lo[4 * r + 1] = (float)(v0.y - 4 * v2[r].y);
+            lo[4 * r + 2] = valid ? log1pf(J.pc[r][k].SAD) : 0.0f;
+            lo[4 * r + 3] = valid ? 1.0f : 0.0f;
+        }
+    }
+}
+
+// Exact refinement of the proposals: one wave per (block, reference).
+// Candidates in order: for each of the four hypotheses the +-2 window around
+// it in AddNewVlist order; if the best SAD exceeds 1.2 x the predicted SAD,
+// also the +-4 window around the level-2 guide and the +-2 window around
+// zero.  Keys SAD * 256 + index keep the first minimum.  Writes the level-0
+// vector and SAD; stats[job * 2 + {0, 1}] count blocks and fallbacks.
+__global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                   int ntasks, int total, const float *__restrict__ out4,
+                                                   int *__restrict__ stats) {
+    __shared__ uint32_t s_pic[4][PEL_MAXB * SUB_PP];
+    __shared__ uint32_t s_reg[4][4][LP_RROWS * LP_RP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = xcd_block() * 4 + wv;
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
+    dme_mv *__restrict__ V = J.vec[r];
+    dme_cost *__restrict__ C = J.pc[r];
+    if (b.empty()) {  // out of the picture: no update (pixel_match.cpp:278-281)
+        if (lane == 0) {
+            V[k].x = V[k].y = 0;
+            C[k].SAD = 10000000.0f;
+        }
+        return;
+    }
+    const int16_t *__restrict__ pic = J.pic[0];
+    const int16_t *__restrict__ ref = J.ref[r][0];
+    const int W = J.W, H = J.H;
+    dme_mv g;
+    float s2;
+    lp_guide(J, r, x, y, g, s2);
+    g.x *= 4;
+    g.y *= 4;
+    const float *o = out4 + ((size_t)J.lp_off + k) * 32;
+    dme_mv hyp[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        hyp[h].x = g.x + (int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f));
+        hyp[h].y = g.y + (int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f));
+    }
+    const float pred_sad = expm1f(fminf(o[16 + r], 20.0f));
+    wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic[wv]), 4 * SUB_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
+                  (b.xl + 15) & ~15);
+    const int RW = b.xl + 4, RH = b.yl + 4;
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][h]), 4 * LP_RP, ref, W, H, b.xp + hyp[h].x - 2,
+                      b.yp + hyp[h].y - 2, RW, RH, RW);
+    WAVE_SYNC();
+    int key = INT_MAX;
+    for (int i = lane; i < 100; i += WAVE) {
+        const int h = i / 25;
+        const dme_mv w = win_off(i - 25 * h, 2);
+        const int s = packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][h], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
+        key = min(key, s * 256 + i);
+    }
+    key = wave_min_i(key);
+    int best_sad = key >> 8;
+    dme_mv best = win_off((key & 255) % 25, 2);
+    {
+        const int h = (key & 255) / 25;
+        best.x += hyp[h].x;
+        best.y += hyp[h].y;
+    }
+    const bool fallback = (float)best_sad > 1.2f * pred_sad;
+    if (fallback) {  // uniform
+        WAVE_SYNC();
+        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * LP_RP, ref, W, H, b.xp + g.x - 4, b.yp + g.y - 4,
+                      b.xl + 8, b.yl + 8, b.xl + 8);
+        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][1]), 4 * LP_RP, ref, W, H, b.xp - 2, b.yp - 2, RW, RH, RW);
+        WAVE_SYNC();
+        int key2 = INT_MAX;
+        for (int i = lane; i < 106; i += WAVE) {
+            const bool gw = i < 81;
+            const dme_mv w = gw ? win_off(i, 4) : win_off(i - 81, 2);
+            const int s = gw ? packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][0], LP_RP, w.x + 4, w.y + 4, b.xl, b.yl)
+                             : packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][1], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
+            key2 = min(key2, s * 256 + 100 + i);
+        }
+        key2 = wave_min_i(key2);
+        if (key2 < key) {
+            const int i = (key2 & 255) - 100;
+            best_sad = key2 >> 8;
+            if (i < 81) {
+                best = win_off(i, 4);
+                best.x += g.x;
+                best.y += g.y;
+            } else {
+                best = win_off(i - 81, 2);
+            }
+        }
+    }
+    if (lane == 0) {
+        V[k] = best;
+        C[k].SAD = (float)best_sad;
+        atomicAdd(&stats[2 * tk.job], 1);
+        if (fallback) atomicAdd(&stats[2 * tk.job + 1], 1);
+    }
+}
+
+// Pel costs of the proposed vectors: the prediction rules of the pel search
+// (pixel_match.cpp:314-347) over the final level-0 vectors, mvcost =
+// Norm1(mv - pred) through its short, total = SAD (SetTotal(0.0)).
+__global__ __launch_bounds__(256) void k_lp_pel_costs(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                      int ntasks, int total) {
+    const int gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int k = gi - tk.offset, X = J.xnb, xb = k % X, yb = k / X;
+    const dme_mv *__restrict__ V = J.vec[tk.ref];
+    dme_mv pred;
+    pred.x = pred.y = 0;
+    if (yb == 0) {
+        if (xb > 0) pred = V[k - 1];
+    } else if (xb == 0) {
+        pred = V[k - X];
+    } else if (xb < X - 1) {
+        pred = med3(V[k - 1], V[k - X], V[k - X + 1]);
+    } else {
+        pred = mean2(V[k - X], V[k - 1]);
+    }
+    const dme_mv v = V[k];
+    dme_cost c = J.pc[tk.ref][k];
+    c.mvcost = (float)(int16_t)norm1(v.x - pred.x, v.y - pred.y);
+    c.total = c.SAD + 0.0f * c.mvcost;  // SetTotal(0.0)
+    J.pc[tk.ref][k] = c;
+}
+
+// ---------------------------------------------------------------------------
 // K7: ModeDecider::SetDC (me_mode_decn.cpp:523-573): block DCs of Y, U, V,
 // then the 3x3 [1 3 1; 3 0 3; 1 3 1]/16 smoothing; row 0 and the first and
 // last columns are copied and the last row is left as it was.
@@ -2221,6 +2438,26 @@ hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, i
 hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dc_smooth, dim3((total + 255) / 256), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks, int total, uint16_t *feat, float *lab,
+                              hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lp_features, dim3((total + 255) / 256), dim3(256), 0, st, jobs, tasks, ntasks, total, feat, lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4, int *stats,
+                            hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lp_refine, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total, out4, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_pel_costs(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lp_pel_costs, dim3((total + 255) / 256), dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 

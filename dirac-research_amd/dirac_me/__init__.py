@@ -190,6 +190,34 @@ class Context:
     def invalidate(self, pic_num=-1):
         check(lib().dme_invalidate_caches(self.h, pic_num))
 
+    # ---- learned proposals (include/dme_learned.h) ----
+    def set_lp_weights(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        check(lib().dme_lp_set_weights(self.h, ptr(p), p.size))
+
+    def motion_estimate_learned(self, pic_num, refs, out=None):
+        a, n = self._refs(refs)
+        check(lib().dme_motion_estimate_learned(self.h, pic_num, a, n, C.byref(out.struct()) if out else None))
+        return out
+
+    def run_batch_learned(self, jobs, outs=None):
+        """jobs: list of (pic_num, refs, params)."""
+        arr = (abi.dme_job * len(jobs))()
+        for i, (pn, refs, prm) in enumerate(jobs):
+            arr[i].pic_num = pn
+            arr[i].nrefs = len(refs)
+            for k, r in enumerate(refs):
+                arr[i].refs[k] = r
+            arr[i].params = prm
+        os_ = None if outs is None else (abi.dme_medata * len(jobs))(*[o.struct() for o in outs])
+        check(lib().dme_run_batch_learned(self.h, arr, len(jobs), os_))
+        return outs
+
+    def lp_stats(self):
+        b, f = C.c_longlong(), C.c_longlong()
+        check(lib().dme_lp_get_stats(self.h, C.byref(b), C.byref(f)))
+        return b.value, f.value
+
 
 def lambda_map_host(vectors1, vectors2, lam, nrefs):
     v1 = np.ascontiguousarray(vectors1, MV_DTYPE)

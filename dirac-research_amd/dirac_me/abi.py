@@ -1,4 +1,5 @@
-"""ctypes mirror of include/dme.h and the loader of libdme.so.
+"""ctypes mirror of include/dme.h and include/dme_learned.h, and the loader
+of libdme.so.
 
 The library is built in-tree (``dirac-research_amd/libdme.so``) by
 ``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
@@ -116,7 +117,28 @@ SIGNATURES = {
     "dme_synth_texture": ([C.c_uint64, _I, _I, _I, _P], _I),
     "dme_synth_warp": ([_P, _I, _I, C.POINTER(dme_synth_motion), _P], _I),
     "dme_synth_chroma": ([_P, _I, _I, C.c_uint64, _P, _P], _I),
+    # include/dme_learned.h
+    "dme_lp_set_weights": ([_P, _P, C.c_size_t], _I),
+    "dme_motion_estimate_learned": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
+    "dme_run_batch_learned": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_lp_get_stats": ([_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
+    "dme_lp_training_pair": ([_P, _I, _P, _P], _I),
+    "dme_lp_prepare": ([_P, _P, _P, _P], _I),
+    "dme_lp_conv": ([_P, _P, _I, _I, _I, _P, _P, _I, _P, _P], _I),
+    "dme_lp_wgrad": ([_P, _P, _P, _I, _I, _I, _P, _P], _I),
+    "dme_lp_relu_backward": ([_P, _P, _P, C.c_size_t, _P], _I),
+    "dme_lp_wta_loss": ([_P, _P, _P, _I, C.c_float, C.c_float, C.c_float, _P, _P], _I),
+    "dme_lp_adam": ([_P, _P, _P, _P, _P, _I, C.c_float, C.c_float, C.c_float, C.c_float, _I], _I),
+    "dme_lp_zero": ([_P, _P, C.c_size_t], _I),
+    "dme_lp_alloc": ([_P, C.c_size_t, C.POINTER(_P)], _I),
+    "dme_lp_free": ([_P, _P], _I),
+    "dme_lp_copy": ([_P, _P, _P, C.c_size_t, _I], _I),
 }
+
+DME_LP_CHANNELS = 32
+DME_LP_LAYER_PARAMS = 9 * 32 * 32 + 32
+DME_LP_PARAMS = 4 * DME_LP_LAYER_PARAMS
+DME_LP_FRAG_ELEMS = 9 * 2 * 64 * 8
 
 _lib = None
 

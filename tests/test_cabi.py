@@ -1,4 +1,4 @@
-"""The C ABI (include/dme.h) without a GPU: the library loads, exports every
+"""The C ABI (include/dme.h, include/dme_learned.h) without a GPU: the library loads, exports every
 declared symbol, and its host-side parts (geometry, lambda map, synthetic
 generator, argument checking) agree with the oracle."""
 import ctypes as C
@@ -16,9 +16,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    src = open(os.path.join(ROOT, "include", "dme.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(dme_[a-z_0-9]+)\s*\(", src)))
+    names = set()
+    for h in ("dme.h", "dme_learned.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(dme_[a-z_0-9]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_symbols_all_exported():
