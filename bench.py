@@ -132,6 +132,43 @@ def cpu_baseline(frames, params, budget_s):
                 "single thread", dt, cpu)}
 
 
+def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
+    """The same batch through learned proposals (dme_run_batch_learned):
+    frames/s, and the Lagrangian cost gap against the plain search (sum of
+    SBCosts, learned / plain - 1) over the distinct pictures of the batch."""
+    import numpy as np
+    import dirac_me
+    from dirac_me import learned
+
+    ctx.set_lp_weights(learned.load_weights())
+    for _ in range(args.warmup):
+        ctx.invalidate(-1)
+        ctx.run_batch_learned(jobs)
+    ctx.synchronize()
+    b0, f0 = ctx.lp_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.invalidate(-1)
+        ctx.run_batch_learned(jobs)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    b1, f1 = ctx.lp_stats()
+    n = len(frames)
+    plain = [dirac_me.MEData(geom) for _ in range(n)]
+    lrn = [dirac_me.MEData(geom) for _ in range(n)]
+    ctx.invalidate(-1)
+    ctx.run_batch(jobs[:n], plain)
+    ctx.invalidate(-1)
+    ctx.run_batch_learned(jobs[:n], lrn)
+    gaps = [float(np.sum(l.sb_costs, dtype=np.float64) / np.sum(p.sb_costs, dtype=np.float64) - 1.0)
+            for p, l in zip(plain, lrn)]
+    fps = B * args.steps / dt
+    return {"value": fps, "unit": "frames/s", "ms_per_step": dt / args.steps * 1e3,
+            "speedup_vs_plain": fps / plain_fps, "mean_sb_cost_gap": float(np.mean(gaps)),
+            "max_sb_cost_gap": float(np.max(gaps)), "fallback_rate": (f1 - f0) / max(1, b1 - b0),
+            "pictures_for_gap": n, "weights": "dirac_me/proposal_weights.npy (2000 training pictures)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +177,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -153,7 +191,7 @@ def main():
     from dirac_me import synth
 
     B = args.batch
-    frames = [synth.config1(s) for s in range(min(B, 8))]
+    frames = [synth.config1(100 + s) for s in range(min(B, 8))]  # seeds outside the proposal training set
     d0 = frames[0]
     params = dirac_me.make_params(*d0["block"], precision=2, depth=4, lam=10.0)
     geom = dirac_me.geometry(params, 1920, 1080)
@@ -217,6 +255,8 @@ def main():
         "roofline": roof, "valu": valu,
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
     }
+    if not args.no_learned:
+        out["learned"] = learned_leg(ctx, jobs, frames, geom, B, args, fps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(frames, params, args.cpu_budget)
     ctx.close()
