@@ -2056,7 +2056,7 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
 // ---------------------------------------------------------------------------
 #define LP_SCALE_PEL 8.0f  // pel per unit of the network's offset outputs
 #define LP_RP 11           // packed region row, dwords (regions up to 40 bytes wide)
-#define LP_RROWS 40        // region rows
+#define LP_RROWS 44        // region rows
 
 __device__ __forceinline__ uint16_t f2bf_rne(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -2131,10 +2131,14 @@ __global__ __launch_bounds__(256) void k_lp_features(const JobDesc *__restrict__
 }
 
 // Exact refinement of the proposals: one wave per (block, reference).
-// Candidates in order: for each of the four hypotheses the +-2 window around
-// it in AddNewVlist order; if the best SAD exceeds 1.2 x the predicted SAD,
-// also the +-4 window around the level-2 guide and the +-2 window around
-// zero.  Keys SAD * 256 + index keep the first minimum.  Writes the level-0
+// Candidates: the union of the +-2 windows around the four hypotheses.  When
+// they fit a 12 x 12 box (the usual case: the hypotheses cluster), one
+// region covering the box is staged and each position of the union is
+// evaluated once, keyed by its raster index in the box; otherwise each
+// window is staged and evaluated on its own, keyed by hypothesis and
+// AddNewVlist order.  If the best SAD exceeds 1.2 x the predicted SAD, the
+// +-4 window around the level-2 guide and the +-2 window around zero join
+// in.  Keys SAD * 256 + index keep the first minimum.  Writes the level-0
 // vector and SAD; stats[job * 2 + {0, 1}] count blocks and fallbacks.
 __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                    int ntasks, int total, const float *__restrict__ out4,
@@ -2176,22 +2180,53 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
     wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic[wv]), 4 * SUB_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
                   (b.xl + 15) & ~15);
     const int RW = b.xl + 4, RH = b.yl + 4;
+    int bx0 = hyp[0].x, bx1 = hyp[0].x, by0 = hyp[0].y, by1 = hyp[0].y;
 #pragma unroll
-    for (int h = 0; h < 4; ++h)
-        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][h]), 4 * LP_RP, ref, W, H, b.xp + hyp[h].x - 2,
-                      b.yp + hyp[h].y - 2, RW, RH, RW);
-    WAVE_SYNC();
-    int key = INT_MAX;
-    for (int i = lane; i < 100; i += WAVE) {
-        const int h = i / 25;
-        const dme_mv w = win_off(i - 25 * h, 2);
-        const int s = packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][h], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
-        key = min(key, s * 256 + i);
+    for (int h = 1; h < 4; ++h) {
+        bx0 = min(bx0, hyp[h].x);
+        bx1 = max(bx1, hyp[h].x);
+        by0 = min(by0, hyp[h].y);
+        by1 = max(by1, hyp[h].y);
     }
-    key = wave_min_i(key);
-    int best_sad = key >> 8;
-    dme_mv best = win_off((key & 255) % 25, 2);
-    {
+    bx0 -= 2;
+    by0 -= 2;
+    const int BW = bx1 + 2 - bx0 + 1, BH = by1 + 2 - by0 + 1;
+    int best_sad;
+    dme_mv best;
+    if (BW <= 12 && BH <= 12) {  // uniform
+        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * LP_RP, ref, W, H, b.xp + bx0, b.yp + by0,
+                      b.xl + BW - 1, b.yl + BH - 1, b.xl + BW - 1);
+        WAVE_SYNC();
+        int key = INT_MAX;
+        for (int i = lane; i < BW * BH; i += WAVE) {
+            const int ox = i % BW, oy = i / BW;
+            bool in = false;
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+                in = in || (abs(bx0 + ox - hyp[h].x) <= 2 && abs(by0 + oy - hyp[h].y) <= 2);
+            if (in)
+                key = min(key, packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][0], LP_RP, ox, oy, b.xl, b.yl) * 256 + i);
+        }
+        key = wave_min_i(key);
+        best_sad = key >> 8;
+        best.x = bx0 + (key & 255) % BW;
+        best.y = by0 + (key & 255) / BW;
+    } else {
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+            wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][h]), 4 * LP_RP, ref, W, H, b.xp + hyp[h].x - 2,
+                          b.yp + hyp[h].y - 2, RW, RH, RW);
+        WAVE_SYNC();
+        int key = INT_MAX;
+        for (int i = lane; i < 100; i += WAVE) {
+            const int h = i / 25;
+            const dme_mv w = win_off(i - 25 * h, 2);
+            const int s = packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][h], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
+            key = min(key, s * 256 + i);
+        }
+        key = wave_min_i(key);
+        best_sad = key >> 8;
+        best = win_off((key & 255) % 25, 2);
         const int h = (key & 255) / 25;
         best.x += hyp[h].x;
         best.y += hyp[h].y;
@@ -2212,7 +2247,7 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
             key2 = min(key2, s * 256 + 100 + i);
         }
         key2 = wave_min_i(key2);
-        if (key2 < key) {
+        if ((key2 >> 8) < best_sad) {
             const int i = (key2 & 255) - 100;
             best_sad = key2 >> 8;
             if (i < 81) {

@@ -197,10 +197,13 @@ __global__ __launch_bounds__(256) void k_lp_adam(float *__restrict__ p, const fl
 // 8r + 2h + {0,1} = offset of hypothesis h in units of LP_SCALE pel from the
 // guide, 16 + r = predicted log(1 + SAD)); labels[p][4r + {0..3}] = target
 // offset in pel (x, y), target log(1 + SAD), valid (0/1).
-//   L = e_best + alpha * sum_h e_h + beta * |c - t|,  e_h = |S o_h - t|_1
-// (winner takes all over the hypotheses).  dout gets dL/dout * scale; the
-// scaled loss is added to *loss.
+//   L = e_best + alpha * sum_h e_h + beta * q(c - t),  e_h = |S o_h - t|_1
+// (winner takes all over the hypotheses); q is the pinball loss of the
+// LP_TAU quantile, so c predicts a high quantile of the log SAD and the
+// fallback search (taken above 1.2 x that) stays rare.  dout gets
+// dL/dout * scale; the scaled loss is added to *loss.
 #define LP_SCALE 8.0f
+#define LP_TAU 0.8f
 __global__ __launch_bounds__(256) void k_lp_wta(const float *__restrict__ out, const float *__restrict__ lab,
                                                 int npos, float alpha, float beta, float scale,
                                                 float *__restrict__ dout, float *__restrict__ loss) {
@@ -228,8 +231,8 @@ __global__ __launch_bounds__(256) void k_lp_wta(const float *__restrict__ out, c
                 d[8 * r + 2 * h + 1] = scale * wgt * LP_SCALE * (sy > 0.0f ? 1.0f : (sy < 0.0f ? -1.0f : 0.0f));
             }
             const float dc = o[16 + r] - t[2];
-            L += beta * fabsf(dc);
-            d[16 + r] = scale * beta * (dc > 0.0f ? 1.0f : (dc < 0.0f ? -1.0f : 0.0f));
+            L += beta * (dc > 0.0f ? (1.0f - LP_TAU) * dc : -LP_TAU * dc);
+            d[16 + r] = scale * beta * (dc > 0.0f ? (1.0f - LP_TAU) : (dc < 0.0f ? -LP_TAU : 0.0f));
         }
     }
     // one atomic per wave

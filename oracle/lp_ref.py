@@ -17,6 +17,7 @@ import numpy as np
 C = 32
 LAYER = 9 * C * C + C
 SCALE = 8.0  # pel per unit of an offset output
+TAU = 0.8  # quantile of the predicted-cost (pinball) term
 
 
 def bf16(x):
@@ -83,9 +84,10 @@ def conv_weight_grad(x, g, dtype=np.float64):
 
 def wta_loss(out, lab, alpha, beta, scale, dtype=np.float32):
     """Winner-takes-all proposal loss over the four hypotheses of each
-    reference (channels 8r + 2h + {0,1}) plus the predicted-cost term
-    (channel 16 + r); labels [..., 8] = per reference (tx, ty, log1p SAD,
-    valid).  Returns (scaled loss, scaled gradient)."""
+    reference (channels 8r + 2h + {0,1}) plus the pinball loss of the TAU
+    quantile for the predicted cost (channel 16 + r); labels [..., 8] = per
+    reference (tx, ty, log1p SAD, valid).  Returns (scaled loss, scaled
+    gradient)."""
     o = out.reshape(-1, C).astype(dtype)
     t = lab.reshape(-1, 8).astype(dtype)
     d = np.zeros_like(o)
@@ -99,14 +101,14 @@ def wta_loss(out, lab, alpha, beta, scale, dtype=np.float32):
         wgt = alpha + (np.arange(4)[None, :] == hb[:, None])
         L += float(np.sum((e[np.arange(len(e)), hb] + alpha * e.sum(axis=1))[v]))
         dc = o[:, 16 + r] - t[:, 4 * r + 2]
-        L += float(np.sum((beta * np.abs(dc))[v]))
+        L += float(np.sum((beta * np.where(dc > 0, (1 - TAU) * dc, -TAU * dc))[v]))
         gx = scale * wgt * SCALE * np.sign(sx)
         gy = scale * wgt * SCALE * np.sign(sy)
         gx[~v] = 0
         gy[~v] = 0
         d[:, 8 * r + 0:8 * r + 8:2] = gx
         d[:, 8 * r + 1:8 * r + 8:2] = gy
-        d[:, 16 + r] = np.where(v, scale * beta * np.sign(dc), 0)
+        d[:, 16 + r] = np.where(v, scale * beta * np.where(dc > 0, 1 - TAU, np.where(dc < 0, -TAU, 0.0)), 0)
     return L * scale, d.reshape(out.shape)
 
 
