@@ -169,6 +169,57 @@ def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
             "pictures_for_gap": n, "weights": "dirac_me/proposal_weights.npy (2000 training pictures)"}
 
 
+def mixed_leg(args):
+    """The mixed batch of 64 pictures (16 CIF, 24 720p, 16 1080p, 8 2160p,
+    1 or 2 references by seed, block parameters per size) in one
+    dme_run_batch call per step: frames/s and luma samples/s."""
+    from concurrent.futures import ThreadPoolExecutor
+    import dirac_me
+    from dirac_me import synth
+
+    mix = synth.config2_mix(0)
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        sets = list(ex.map(lambda m: synth.sized(m[2], m[0], m[1]), mix))
+    ctx = dirac_me.Context(0)
+    jobs = []
+    samples = 0
+    for j, ((size, nrefs, seed), d) in enumerate(zip(mix, sets)):
+        prm = dirac_me.make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+        base = 3 * j
+        nums = [base, base + 2][:nrefs]
+        for n, r in zip(nums, d["refs"]):
+            ctx.upload(n, r)
+        ctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        jobs.append((base + 1, nums, prm))
+        samples += d["cur"].size
+    del sets
+
+    def step():
+        ctx.invalidate(-1)
+        ctx.run_batch(jobs)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.set_profiling(True)
+    step()
+    ctx.synchronize()
+    kt = ctx.kernel_times()
+    ctx.close()
+    return {"metric": "mixed-size ME frames/s (64 pictures per batch)", "value": len(jobs) / dt, "unit": "frames/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+            "config": {"workload": "mixed batch: 16 CIF, 24 1280x720, 16 1920x1080, 8 3840x2160, 1 or 2 refs by seed, "
+                                   "blocks 12/8, 12/8, 16/12, 24/16, quarter pel, lambda 10", "batch": len(jobs)},
+            "luma_samples_per_s": samples / dt,
+            "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +229,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
+                    help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -186,6 +239,10 @@ def main():
               "there is no multi-GPU launcher (pictures would shard by picture with no exchange)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "mixed64":
+        print(json.dumps(mixed_leg(args)))
+        return
 
     import dirac_me
     from dirac_me import synth

@@ -155,7 +155,12 @@ struct dme_ctx {
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
     dme::DevBuf d_jobs, d_tasks, d_planes, d_flag;
-    dme::PinnedBuf pin;  // lambda-stage staging: level-0 vectors in, lambda maps out
+    // Pinned host staging of one call (descriptors, task tables, plane
+    // tasks, lambda-stage vectors and maps): bump-allocated, so no upload has
+    // to wait for an earlier one; reset when a call starts, with the stream
+    // idle.
+    dme::PinnedBuf arena;
+    size_t arena_used = 0;
     // learned proposal: weights (f32 master), BF16 fragments per layer,
     // activations of a batch, per-job counters
     bool lp_ready = false;
@@ -207,6 +212,35 @@ static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
 
 static int set_device(dme_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
+    return DME_OK;
+}
+
+// Staging space for an upload that is queued but not yet done.  Growing the
+// buffer waits for the copies already queued from it.
+static int arena_alloc(dme_ctx *ctx, size_t bytes, void **out) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (ctx->arena_used + bytes > ctx->arena.bytes) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        DME_TRY(ctx->arena.ensure(std::max(2 * ctx->arena.bytes, ctx->arena_used + bytes + ((size_t)1 << 20))));
+        ctx->arena_used = 0;
+    }
+    *out = static_cast<char *>(ctx->arena.p) + ctx->arena_used;
+    ctx->arena_used += bytes;
+    return DME_OK;
+}
+
+// Copies host data to a device buffer in stream order through the arena;
+// the device buffer grows (with the stream synced) when too small.
+static int upload(dme_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
+    if (bytes == 0) return DME_OK;
+    if (dst.n < bytes) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        DME_TRY(dst.ensure(bytes));
+    }
+    void *h = nullptr;
+    DME_TRY(arena_alloc(ctx, bytes, &h));
+    std::memcpy(h, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst.p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
     return DME_OK;
 }
 
@@ -279,10 +313,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         ranges.push_back(std::make_pair(first, off));
     }
     if (all.empty()) return DME_OK;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    DME_TRY(ctx->d_planes.ensure(sizeof(PlaneTask) * all.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->d_planes.p, all.data(), sizeof(PlaneTask) * all.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
+    DME_TRY(upload(ctx, ctx->d_planes, all.data(), sizeof(PlaneTask) * all.size()));
     const PlaneTask *dp = ctx->d_planes.as<PlaneTask>();
     for (int l = 1; l <= maxl; ++l) {
         const auto &rg = ranges[l - 1];
@@ -302,8 +333,6 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         DME_TRY(timed_launch(ctx, KK_BUILD_QP, launches,
                              [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->stream); }));
     }
-    // the task array must outlive the copy
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
     for (Picture *P : ups) P->up_ready = true;
     return DME_OK;
@@ -458,12 +487,7 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
 static int upload_jobs(dme_ctx *ctx, const std::vector<JobState *> &js) {
     std::vector<JobDesc> h(js.size());
     for (size_t i = 0; i < js.size(); ++i) h[i] = js[i]->desc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    DME_TRY(ctx->d_jobs.ensure(sizeof(JobDesc) * h.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->d_jobs.p, h.data(), sizeof(JobDesc) * h.size(), hipMemcpyHostToDevice, ctx->stream));
-    // the host vector dies at return: wait for the copy to leave it
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return DME_OK;
+    return upload(ctx, ctx->d_jobs, h.data(), sizeof(JobDesc) * h.size());
 }
 
 // Task table builder: groups of tasks uploaded in one copy per stage.
@@ -481,13 +505,7 @@ struct TaskTable {
 };
 
 static int upload_tasks(dme_ctx *ctx, const TaskTable &tt) {
-    if (tt.all.empty()) return DME_OK;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    DME_TRY(ctx->d_tasks.ensure(sizeof(Task) * tt.all.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->d_tasks.p, tt.all.data(), sizeof(Task) * tt.all.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return DME_OK;
+    return upload(ctx, ctx->d_tasks, tt.all.data(), sizeof(Task) * tt.all.size());
 }
 
 static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->d_tasks.as<Task>() + r.first; }
@@ -550,8 +568,9 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
         const size_t B = (size_t)js[i]->g.xnb * js[i]->g.ynb;
         off[i + 1] = off[i] + sizeof(dme_mv) * B * js[i]->nrefs + sizeof(float) * (B + B / 4 + B / 16);
     }
-    DME_TRY(ctx->pin.ensure(off.back()));
-    char *base = static_cast<char *>(ctx->pin.p);
+    void *stage = nullptr;
+    DME_TRY(arena_alloc(ctx, off.back(), &stage));
+    char *base = static_cast<char *>(stage);
     for (size_t i = 0; i < js.size(); ++i) {
         const size_t B = (size_t)js[i]->g.xnb * js[i]->g.ynb;
         for (int r = 0; r < js[i]->nrefs; ++r)
@@ -595,8 +614,6 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
                                hipMemcpyHostToDevice, ctx->stream));
         j.stage = 2;
     }
-    // the staging buffer is reused by the next batch: the copies must have left it
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return DME_OK;
 }
 
@@ -728,7 +745,11 @@ static JobState *job_for(dme_ctx *ctx, int pic_num) {
     return slot.get();
 }
 
+// Start of a call that queues work: the stream is idle (every call ends
+// synchronised), so the staging arena can be reused from its start.
 static int begin_timing(dme_ctx *ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->arena_used = 0;
     ctx->ev_used.clear();
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     return DME_OK;
@@ -901,6 +922,8 @@ extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda
     if (it == ctx->jobs.end() || it->second->stage < 1) return set_error(DME_ERR_STATE, "run dme_pixel_search first");
     std::vector<JobState *> js{it->second.get()};
     const float *ov[1] = {lambda_map};
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->arena_used = 0;
     DME_TRY(stage_lambda(ctx, js, ov));
     return export_job(ctx, *js[0], out);
 }
@@ -1177,6 +1200,8 @@ extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_featu
     if (J->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
     std::vector<JobState *> js{J};
     int launches = 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->arena_used = 0;
     DME_TRY(bind_planes(ctx, js, true, launches));
     J->desc.lp_off = 0;
     DME_TRY(upload_jobs(ctx, js));
