@@ -38,6 +38,8 @@ struct JobDesc {
     const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
     const int16_t DME_G *up[2];                       // 2W x 2H up-converted refs
     const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
+    const uint8_t DME_G *pic8, *ref8[2];              // byte planes of picture and refs (k_bytes): sample (0, 0)
+    int p8;                                           // their row pitch in bytes
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
     int lp_off;                                       // first position of the picture in the proposal tensors
     // pel search, per reference and level ([0] aliases the MEData arrays)
@@ -64,6 +66,9 @@ struct JobDesc {
     float DME_G *ucost;                               // level-2 unit costs
     int16_t DME_G *dct[3];                            // block DCs before smoothing
 };
+
+// Replicated samples on every side of a byte plane (k_bytes).
+constexpr int kB8Margin = 32;
 
 // A plane conversion of a batched pyramid / up-conversion launch: `in` of
 // width in_w, `out` of ow x oh samples (the source size for up-conversion),

@@ -31,7 +31,9 @@ namespace dme {
 
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
-hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st);
+hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
+                           int max_blen_y, bool has_guide, hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
                            int max_blen_y, hipStream_t st);
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
@@ -112,6 +114,10 @@ struct Picture {
     int pyr_levels = 0;             // levels built so far
     DevBuf up, qp;  // up-converted plane and its quarter-pel planes
     bool up_ready = false;
+    DevBuf b8;      // byte plane with margins (k_bytes)
+    int bp = 0;     // its row pitch
+    bool b8_ready = false;
+    const uint8_t *b8_origin() const { return b8.as<uint8_t>() + (size_t)kB8Margin * bp + kB8Margin; }
 };
 
 struct JobState {
@@ -145,22 +151,34 @@ struct PinnedBuf {
     }
 };
 
+// A stream with the buffers its queued work reads: descriptors, task
+// tables, plane tasks and the pinned staging arena of the current call.
+struct Lane {
+    hipStream_t stream = nullptr;
+    DevBuf d_jobs, d_tasks, d_planes;
+    PinnedBuf arena;
+    size_t arena_used = 0;
+};
+
 }  // namespace dme
 
 struct dme_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // lane 0's stream
     dme_params params;
     bool params_set = false;
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
-    dme::DevBuf d_jobs, d_tasks, d_planes, d_flag;
-    // Pinned host staging of one call (descriptors, task tables, plane
-    // tasks, lambda-stage vectors and maps): bump-allocated, so no upload has
-    // to wait for an earlier one; reset when a call starts, with the stream
-    // idle.
-    dme::PinnedBuf arena;
-    size_t arena_used = 0;
+    dme::DevBuf d_flag;
+    // Two lanes (stream + buffers); stages queue work on the current lane.
+    // Large batches run as two halves on the two lanes, so that one half's
+    // host work and latency-bound wavefronts overlap the other half's
+    // kernels.  The arena of a lane holds the pinned host staging of one
+    // call (descriptors, task tables, plane tasks, lambda-stage vectors and
+    // maps), bump-allocated so no upload waits for an earlier one; it is
+    // reset when a call starts, with the streams idle.
+    dme::Lane lane[2];
+    dme::Lane *L = &lane[0];
     // learned proposal: weights (f32 master), BF16 fragments per layer,
     // activations of a batch, per-job counters
     bool lp_ready = false;
@@ -168,6 +186,7 @@ struct dme_ctx {
     long long lp_blocks = 0, lp_fallbacks = 0;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // lane 1 after lane 0's planes; lane 0 after lane 1
     dme_stats stats;
     int wave_threads = 1024;
     // per-launch timing while profiling is enabled
@@ -181,12 +200,13 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_COUNT
+    KK_BYTES, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", ""};
+    "k_lp_net", "k_lp_refine", "k_bytes"};
+static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
 template <class F>
@@ -199,11 +219,11 @@ static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
             HIP_TRY(hipEventCreate(&e));
             ctx->ev_pool.push_back(e);
         }
-        HIP_TRY(hipEventRecord(ctx->ev_pool[idx], ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev_pool[idx], ctx->L->stream));
     }
     HIP_TRY(f());
     if (ctx->profiling) {
-        HIP_TRY(hipEventRecord(ctx->ev_pool[idx + 1], ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev_pool[idx + 1], ctx->L->stream));
         ctx->ev_used.push_back(std::make_pair(kind, (int)idx));
     }
     ++launches;
@@ -211,6 +231,7 @@ static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
 }
 
 static int set_device(dme_ctx *ctx) {
+    ctx->L = &ctx->lane[0];  // entry points start on lane 0
     HIP_TRY(hipSetDevice(ctx->device));
     return DME_OK;
 }
@@ -219,13 +240,13 @@ static int set_device(dme_ctx *ctx) {
 // buffer waits for the copies already queued from it.
 static int arena_alloc(dme_ctx *ctx, size_t bytes, void **out) {
     bytes = (bytes + 255) & ~(size_t)255;
-    if (ctx->arena_used + bytes > ctx->arena.bytes) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        DME_TRY(ctx->arena.ensure(std::max(2 * ctx->arena.bytes, ctx->arena_used + bytes + ((size_t)1 << 20))));
-        ctx->arena_used = 0;
+    if (ctx->L->arena_used + bytes > ctx->L->arena.bytes) {
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        DME_TRY(ctx->L->arena.ensure(std::max(2 * ctx->L->arena.bytes, ctx->L->arena_used + bytes + ((size_t)1 << 20))));
+        ctx->L->arena_used = 0;
     }
-    *out = static_cast<char *>(ctx->arena.p) + ctx->arena_used;
-    ctx->arena_used += bytes;
+    *out = static_cast<char *>(ctx->L->arena.p) + ctx->L->arena_used;
+    ctx->L->arena_used += bytes;
     return DME_OK;
 }
 
@@ -234,13 +255,13 @@ static int arena_alloc(dme_ctx *ctx, size_t bytes, void **out) {
 static int upload(dme_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
     if (bytes == 0) return DME_OK;
     if (dst.n < bytes) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
         DME_TRY(dst.ensure(bytes));
     }
     void *h = nullptr;
     DME_TRY(arena_alloc(ctx, bytes, &h));
     std::memcpy(h, src, bytes);
-    HIP_TRY(hipMemcpyAsync(dst.p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dst.p, h, bytes, hipMemcpyHostToDevice, ctx->L->stream));
     return DME_OK;
 }
 
@@ -312,29 +333,57 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         }
         ranges.push_back(std::make_pair(first, off));
     }
+    {
+        // byte planes of every picture of the batch
+        const size_t first = all.size();
+        int off = 0;
+        for (auto &kv : levels) {
+            Picture &P = *kv.first;
+            if (P.b8_ready) continue;
+            P.bp = (P.W + 2 * kB8Margin + 63) & ~63;
+            const size_t dw = (size_t)(P.H + 2 * kB8Margin) * (P.bp / 4);
+            DME_TRY(P.b8.ensure(sizeof(uint32_t) * dw));
+            PlaneTask t;
+            t.in = P.y;
+            t.out = P.b8.as<int16_t>();
+            t.in_w = P.W;
+            t.ow = P.bp;
+            t.oh = P.H;
+            t.offset = off;
+            off += (int)((dw + 255) / 256);
+            all.push_back(t);
+        }
+        ranges.push_back(std::make_pair(first, off));
+    }
     if (all.empty()) return DME_OK;
-    DME_TRY(upload(ctx, ctx->d_planes, all.data(), sizeof(PlaneTask) * all.size()));
-    const PlaneTask *dp = ctx->d_planes.as<PlaneTask>();
+    DME_TRY(upload(ctx, ctx->L->d_planes, all.data(), sizeof(PlaneTask) * all.size()));
+    const PlaneTask *dp = ctx->L->d_planes.as<PlaneTask>();
     for (int l = 1; l <= maxl; ++l) {
         const auto &rg = ranges[l - 1];
         const int n = (int)(ranges[l].first - rg.first);  // the next range starts where this one ends
         if (rg.second == 0) continue;
         DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches,
-                             [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->stream); }));
+                             [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->L->stream); }));
     }
-    const auto &ru = ranges[ranges.size() - 2], &rq = ranges.back();
+    const auto &ru = ranges[ranges.size() - 3], &rq = ranges[ranges.size() - 2], &rb = ranges.back();
     if (ru.second) {
         const int n = (int)(rq.first - ru.first);
         DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
-                             [&] { return launch_upconvert(dp + ru.first, n, ru.second, ctx->stream); }));
+                             [&] { return launch_upconvert(dp + ru.first, n, ru.second, ctx->L->stream); }));
+    }
+    if (rb.second) {
+        const int n = (int)(all.size() - rb.first);
+        DME_TRY(timed_launch(ctx, KK_BYTES, launches,
+                             [&] { return launch_bytes(dp + rb.first, n, rb.second, ctx->L->stream); }));
     }
     if (rq.second) {
-        const int n = (int)(all.size() - rq.first);
+        const int n = (int)(rb.first - rq.first);
         DME_TRY(timed_launch(ctx, KK_BUILD_QP, launches,
-                             [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->stream); }));
+                             [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->L->stream); }));
     }
     for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
     for (Picture *P : ups) P->up_ready = true;
+    for (auto &kv : levels) kv.first->b8_ready = true;
     return DME_OK;
 }
 
@@ -346,7 +395,7 @@ static Picture *find_pic(dme_ctx *ctx, int num) {
 // Allocates a job's MEData and scratch, fills its descriptor.
 static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs) {
     // buffers below may be reallocated: nothing queued may still use them
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     Picture *P = find_pic(ctx, pic_num);
     if (!P) return set_error(DME_ERR_INVALID, "picture " + std::to_string(pic_num) + " is not registered");
     if (nrefs < 1 || nrefs > 2 || !refs) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
@@ -470,10 +519,13 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
         Picture *P = find_pic(ctx, j->pic_num);
         j->desc.pic[0] = P->y;
         for (int l = 1; l <= j->g.depth; ++l) j->desc.pic[l] = P->pyr[l].as<int16_t>();
+        j->desc.pic8 = P->b8_origin();
+        j->desc.p8 = P->bp;
         for (int r = 0; r < j->nrefs; ++r) {
             Picture *R = find_pic(ctx, j->refs[r]);
             j->desc.ref[r][0] = R->y;
             for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
+            j->desc.ref8[r] = R->b8_origin();
             if (need_up) {
                 j->desc.up[r] = R->up.as<int16_t>();
                 j->desc.qp[r] = R->qp.as<uint32_t>();
@@ -487,7 +539,7 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
 static int upload_jobs(dme_ctx *ctx, const std::vector<JobState *> &js) {
     std::vector<JobDesc> h(js.size());
     for (size_t i = 0; i < js.size(); ++i) h[i] = js[i]->desc;
-    return upload(ctx, ctx->d_jobs, h.data(), sizeof(JobDesc) * h.size());
+    return upload(ctx, ctx->L->d_jobs, h.data(), sizeof(JobDesc) * h.size());
 }
 
 // Task table builder: groups of tasks uploaded in one copy per stage.
@@ -505,10 +557,10 @@ struct TaskTable {
 };
 
 static int upload_tasks(dme_ctx *ctx, const TaskTable &tt) {
-    return upload(ctx, ctx->d_tasks, tt.all.data(), sizeof(Task) * tt.all.size());
+    return upload(ctx, ctx->L->d_tasks, tt.all.data(), sizeof(Task) * tt.all.size());
 }
 
-static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->d_tasks.as<Task>() + r.first; }
+static const Task *dtask(dme_ctx *ctx, const TaskTable::Range &r) { return ctx->L->d_tasks.as<Task>() + r.first; }
 
 // ---- stages ---------------------------------------------------------------
 
@@ -539,20 +591,23 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
         for (int r = 0; r < js[i]->nrefs; ++r) wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
     const TaskTable::Range wave = tt.add(wt);
     DME_TRY(upload_tasks(ctx, tt));
-    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     for (int l = maxd; l >= stop; --l) {
         int rows = 1, bx = 1, by = 1;
+        bool guide = false;
         for (auto *j : js)
             if (l <= j->g.depth) {
                 rows = std::max(rows, j->g.lyb[l]);
                 bx = std::max(bx, j->g.blen[2][0]);
                 by = std::max(by, j->g.blen[2][1]);
+                guide = guide || l < j->g.depth;
             }
         DME_TRY(timed_launch(ctx, KK_PEL_BULK, launches, [&] {
-            return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, ctx->stream);
+            return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, bx, by, guide,
+                                   ctx->L->stream);
         }));
         DME_TRY(timed_launch(ctx, KK_PEL_WAVE, launches,
-                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by, ctx->stream); }));
+                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by, ctx->L->stream); }));
     }
     for (auto *j : js) j->stage = 1;
     return DME_OK;
@@ -575,9 +630,9 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
         const size_t B = (size_t)js[i]->g.xnb * js[i]->g.ynb;
         for (int r = 0; r < js[i]->nrefs; ++r)
             HIP_TRY(hipMemcpyAsync(base + off[i] + sizeof(dme_mv) * B * r, js[i]->vec[r].p, sizeof(dme_mv) * B,
-                                   hipMemcpyDeviceToHost, ctx->stream));
+                                   hipMemcpyDeviceToHost, ctx->L->stream));
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     // MEData::SetLambdaMap per picture, pictures spread over host threads
     auto work = [&](size_t i) {
         JobState &j = *js[i];
@@ -607,11 +662,11 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
         JobState &j = *js[i];
         const size_t B = (size_t)j.g.xnb * j.g.ynb;
         const char *lam = base + off[i] + sizeof(dme_mv) * B * j.nrefs;
-        HIP_TRY(hipMemcpyAsync(j.lam.p, lam, sizeof(float) * B, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(j.lam.p, lam, sizeof(float) * B, hipMemcpyHostToDevice, ctx->L->stream));
         HIP_TRY(hipMemcpyAsync(j.lam1.p, lam + sizeof(float) * B, sizeof(float) * B / 4, hipMemcpyHostToDevice,
-                               ctx->stream));
+                               ctx->L->stream));
         HIP_TRY(hipMemcpyAsync(j.lam0.p, lam + sizeof(float) * (B + B / 4), sizeof(float) * B / 16,
-                               hipMemcpyHostToDevice, ctx->stream));
+                               hipMemcpyHostToDevice, ctx->L->stream));
         j.stage = 2;
     }
     return DME_OK;
@@ -640,17 +695,17 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
         }
     const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt);
     DME_TRY(upload_tasks(ctx, tt));
-    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     if (bulk.n) {
         DME_TRY(timed_launch(ctx, KK_SUBPEL_BULK, launches, [&] {
-            return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, bx, by, ctx->stream);
+            return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, bx, by, ctx->L->stream);
         }));
         DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,
-                             [&] { return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows, ctx->stream); }));
+                             [&] { return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows, ctx->L->stream); }));
     }
     if (shift.n) {
         DME_TRY(timed_launch(ctx, KK_SHIFT, launches, [&] {
-            return launch_shift_vectors(dj, dtask(ctx, shift), shift.n, shift.total, 1, ctx->stream);
+            return launch_shift_vectors(dj, dtask(ctx, shift), shift.n, shift.total, 1, ctx->L->stream);
         }));
     }
     for (auto *j : js) j->stage = 3;
@@ -662,12 +717,12 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     for (auto *j : js) {
         const size_t B = (size_t)j->g.xnb * j->g.ynb;
         // zero-initialised MEData arrays (motion.cpp:55-126)
-        HIP_TRY(hipMemsetAsync(j->intra.p, 0, sizeof(float) * B, ctx->stream));
-        HIP_TRY(hipMemsetAsync(j->bi.p, 0, sizeof(dme_cost) * B, ctx->stream));
-        for (int c = 0; c < 3; ++c) HIP_TRY(hipMemsetAsync(j->dc[c].p, 0, sizeof(int16_t) * B, ctx->stream));
+        HIP_TRY(hipMemsetAsync(j->intra.p, 0, sizeof(float) * B, ctx->L->stream));
+        HIP_TRY(hipMemsetAsync(j->bi.p, 0, sizeof(dme_cost) * B, ctx->L->stream));
+        for (int c = 0; c < 3; ++c) HIP_TRY(hipMemsetAsync(j->dc[c].p, 0, sizeof(int16_t) * B, ctx->L->stream));
         if (j->nrefs < 2) {
-            HIP_TRY(hipMemsetAsync(j->vec[1].p, 0, sizeof(dme_mv) * B, ctx->stream));
-            HIP_TRY(hipMemsetAsync(j->pc[1].p, 0, sizeof(dme_cost) * B, ctx->stream));
+            HIP_TRY(hipMemsetAsync(j->vec[1].p, 0, sizeof(dme_mv) * B, ctx->L->stream));
+            HIP_TRY(hipMemsetAsync(j->pc[1].p, 0, sizeof(dme_cost) * B, ctx->L->stream));
         }
     }
     DME_TRY(upload_jobs(ctx, js));
@@ -694,8 +749,8 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     }
     const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rsh = tt.add(sh), rmt = tt.add(mt);
     DME_TRY(upload_tasks(ctx, tt));
-    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
-    hipStream_t st = ctx->stream;
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    hipStream_t st = ctx->L->stream;
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
                          [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
     DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches,
@@ -718,7 +773,7 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
 static int export_job(dme_ctx *ctx, JobState &J, dme_medata *out) {
     if (!out) return DME_OK;
     const size_t B = (size_t)J.g.xnb * J.g.ynb, S = (size_t)J.g.xsb * J.g.ysb;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = ctx->L->stream;
     for (int r = 0; r < J.nrefs; ++r) {
         if (out->vectors[r]) HIP_TRY(hipMemcpyAsync(out->vectors[r], J.vec[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost, st));
         if (out->pred_costs[r])
@@ -745,18 +800,21 @@ static JobState *job_for(dme_ctx *ctx, int pic_num) {
     return slot.get();
 }
 
-// Start of a call that queues work: the stream is idle (every call ends
-// synchronised), so the staging arena can be reused from its start.
+// Start of a call that queues work: the streams are idle (every call ends
+// synchronised), so the staging arenas can be reused from their start.
 static int begin_timing(dme_ctx *ctx) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->arena_used = 0;
+    ctx->L = &ctx->lane[0];
+    for (auto &ln : ctx->lane) {
+        HIP_TRY(hipStreamSynchronize(ln.stream));
+        ln.arena_used = 0;
+    }
     ctx->ev_used.clear();
-    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->L->stream));
     return DME_OK;
 }
 
 static int end_timing(dme_ctx *ctx, int launches) {
-    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->L->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -776,14 +834,60 @@ static int end_timing(dme_ctx *ctx, int launches) {
     return DME_OK;
 }
 
-// The whole live sequence for a batch of jobs.
+// The whole live sequence for a batch of jobs.  A batch of two or more runs
+// as two halves of about equal work on the two lanes, queued so that the
+// host's lambda stage of one half and the latency-bound wavefront kernels of
+// either half overlap the other half's kernels:
+//   lane 0: planes of all jobs, pel(A)            lambda(A) subpel(A) mode(A)
+//   lane 1:            (after planes) pel(B)                lambda(B) subpel(B) mode(B)
+// The planes are built once, before the fork, so no kernel of one lane reads
+// a plane the other lane writes; every other buffer a half uses is its own
+// jobs' or its lane's.  With per-kernel profiling on, the halves run one
+// after the other on lane 0 so that each kernel is timed alone.
 static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
-    DME_TRY(stage_pel(ctx, js, launches));
-    DME_TRY(stage_lambda(ctx, js, lam_over));
-    DME_TRY(stage_subpel(ctx, js, launches));
-    DME_TRY(stage_mode(ctx, js, launches));
+    if (js.size() < 2 || ctx->profiling) {
+        DME_TRY(stage_pel(ctx, js, launches));
+        DME_TRY(stage_lambda(ctx, js, lam_over));
+        DME_TRY(stage_subpel(ctx, js, launches));
+        DME_TRY(stage_mode(ctx, js, launches));
+        DME_TRY(end_timing(ctx, launches));
+        return DME_OK;
+    }
+    // halves: largest job first to the lighter half, order kept within each
+    std::vector<size_t> order(js.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    auto work = [&](size_t i) { return (double)js[i]->g.W * js[i]->g.H * js[i]->nrefs; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+    std::vector<int> side(js.size());
+    double load[2] = {0.0, 0.0};
+    for (size_t i : order) {
+        side[i] = load[1] < load[0] ? 1 : 0;
+        load[side[i]] += work(i);
+    }
+    std::vector<JobState *> half[2];
+    std::vector<const float *> over[2];
+    for (size_t i = 0; i < js.size(); ++i) {
+        half[side[i]].push_back(js[i]);
+        over[side[i]].push_back(lam_over ? lam_over[i] : nullptr);
+    }
+    DME_TRY(bind_planes(ctx, js, true, launches));
+    HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->lane[1].stream, ctx->ev_fork, 0));
+    for (int h = 0; h < 2; ++h) {
+        ctx->L = &ctx->lane[h];
+        DME_TRY(stage_pel(ctx, half[h], launches));
+    }
+    for (int h = 0; h < 2; ++h) {
+        ctx->L = &ctx->lane[h];
+        DME_TRY(stage_lambda(ctx, half[h], over[h].data()));
+        DME_TRY(stage_subpel(ctx, half[h], launches));
+        DME_TRY(stage_mode(ctx, half[h], launches));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev_join, ctx->lane[1].stream));
+    ctx->L = &ctx->lane[0];
+    HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join, 0));
     DME_TRY(end_timing(ctx, launches));
     return DME_OK;
 }
@@ -807,9 +911,13 @@ extern "C" int dme_create(int device, dme_ctx **out) {
     std::unique_ptr<dme_ctx> c(new dme_ctx());
     c->device = device;
     HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &ln : c->lane) HIP_TRY(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+    c->stream = c->lane[0].stream;
+    c->L = &c->lane[0];
     HIP_TRY(hipEventCreate(&c->ev0));
     HIP_TRY(hipEventCreate(&c->ev1));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     std::memset(&c->stats, 0, sizeof(c->stats));
     std::memset(&c->params, 0, sizeof(c->params));
     *out = c.release();
@@ -819,13 +927,16 @@ extern "C" int dme_create(int device, dme_ctx **out) {
 extern "C" int dme_destroy(dme_ctx *ctx) {
     if (!ctx) return DME_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
     ctx->jobs.clear();
     ctx->pics.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (auto &ln : ctx->lane)
+        if (ln.stream) (void)hipStreamDestroy(ln.stream);
     delete ctx;
     return DME_OK;
 }
@@ -874,10 +985,10 @@ static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, co
     // luma must be 8-bit samples minus 128
     DME_TRY(ctx->d_flag.ensure(sizeof(int)));
     int bad = 0;
-    HIP_TRY(hipMemcpyAsync(ctx->d_flag.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_check_range(P->y, n, ctx->d_flag.as<int>(), ctx->stream));
-    HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_flag.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->L->stream));
+    HIP_TRY(launch_check_range(P->y, n, ctx->d_flag.as<int>(), ctx->L->stream));
+    HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     if (bad) return set_error(DME_ERR_INVALID, "luma samples must lie in [-128, 127] (8-bit samples minus 128)");
     ctx->pics[pic_num] = std::move(P);
     return DME_OK;
@@ -897,7 +1008,7 @@ extern "C" int dme_bind_picture_device(dme_ctx *ctx, int pic_num, int width, int
 
 extern "C" int dme_drop_picture(dme_ctx *ctx, int pic_num) {
     CTX_CHECK(ctx);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->pics.erase(pic_num);
     ctx->jobs.erase(pic_num);
     return DME_OK;
@@ -922,8 +1033,8 @@ extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda
     if (it == ctx->jobs.end() || it->second->stage < 1) return set_error(DME_ERR_STATE, "run dme_pixel_search first");
     std::vector<JobState *> js{it->second.get()};
     const float *ov[1] = {lambda_map};
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->arena_used = 0;
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    ctx->L->arena_used = 0;
     DME_TRY(stage_lambda(ctx, js, ov));
     return export_job(ctx, *js[0], out);
 }
@@ -997,24 +1108,25 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
         return set_error(DME_ERR_INVALID, "kind must be 0 or 1");
     }
     if (out_elems < n) return set_error(DME_ERR_INVALID, "output buffer too small");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     HIP_TRY(hipMemcpy(out, src, sizeof(int16_t) * n, hipMemcpyDeviceToHost));
     return DME_OK;
 }
 
 extern "C" int dme_synchronize(dme_ctx *ctx) {
     CTX_CHECK(ctx);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
 }
 
 extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
     CTX_CHECK(ctx);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     for (auto &kv : ctx->pics)
         if (pic_num < 0 || kv.first == pic_num) {
             kv.second->pyr_levels = 0;
             kv.second->up_ready = false;
+            kv.second->b8_ready = false;
         }
     return DME_OK;
 }
@@ -1026,7 +1138,7 @@ extern "C" int dme_set_profiling(dme_ctx *ctx, int enable) {
 }
 
 extern "C" const char *dme_kernel_name(int k) {
-    return (k >= 0 && k < DME_NUM_KERNELS) ? kKernelNames[k] : "";
+    return (k >= 0 && k < DME_NUM_KERNELS && kKernelNames[k]) ? kKernelNames[k] : "";
 }
 
 extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) {
@@ -1068,7 +1180,7 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
     DME_TRY(ctx->lp_out.ensure(sizeof(float) * 32 * (size_t)P));
     DME_TRY(ctx->lp_stats.ensure(sizeof(int) * 2 * js.size()));
     DME_TRY(upload_jobs(ctx, js));
-    HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(int) * 2 * js.size(), ctx->stream));
+    HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(int) * 2 * js.size(), ctx->L->stream));
     TaskTable tt;
     std::vector<Task> fb, rb;
     int offf = 0, offr = 0;
@@ -1083,13 +1195,13 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
     }
     const TaskTable::Range feat = tt.add(fb), refine = tt.add(rb);
     DME_TRY(upload_tasks(ctx, tt));
-    const JobDesc *dj = ctx->d_jobs.as<JobDesc>();
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     uint16_t *act[4];
     for (int a = 0; a < 4; ++a) act[a] = ctx->lp_act[a].as<uint16_t>();
     const uint16_t *frag = ctx->lp_frag.as<uint16_t>();
     const float *w = ctx->lp_w.as<float>();
     DME_TRY(timed_launch(ctx, KK_LP_NET, launches, [&]() -> hipError_t {
-        hipError_t e = launch_lp_features(dj, dtask(ctx, feat), feat.n, feat.total, act[0], nullptr, ctx->stream);
+        hipError_t e = launch_lp_features(dj, dtask(ctx, feat), feat.n, feat.total, act[0], nullptr, ctx->L->stream);
         for (const Group &g : groups) {
             const size_t o = (size_t)g.off * 32;
             const int npos = g.n * g.W * g.H;
@@ -1098,23 +1210,23 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
                 const uint16_t *fr = frag + (size_t)layer * 2 * DME_LP_FRAG_ELEMS;  // forward fragments
                 const float *bias = w + (size_t)layer * DME_LP_LAYER_PARAMS + 9 * 32 * 32;
                 if (layer < 3)
-                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 1, act[layer + 1] + o, nullptr, ctx->stream);
+                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 1, act[layer + 1] + o, nullptr, ctx->L->stream);
                 else
                     e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 0, nullptr, ctx->lp_out.as<float>() + o,
-                                       ctx->stream);
+                                       ctx->L->stream);
             }
         }
         return e;
     }));
     DME_TRY(timed_launch(ctx, KK_LP_REFINE, launches, [&]() -> hipError_t {
         hipError_t e = launch_lp_refine(dj, dtask(ctx, refine), refine.n, refine.total, ctx->lp_out.as<float>(),
-                                        ctx->lp_stats.as<int>(), ctx->stream);
-        if (e == hipSuccess) e = launch_lp_pel_costs(dj, dtask(ctx, refine), refine.n, refine.total, ctx->stream);
+                                        ctx->lp_stats.as<int>(), ctx->L->stream);
+        if (e == hipSuccess) e = launch_lp_pel_costs(dj, dtask(ctx, refine), refine.n, refine.total, ctx->L->stream);
         return e;
     }));
     std::vector<int> st(2 * js.size());
-    HIP_TRY(hipMemcpyAsync(st.data(), ctx->lp_stats.p, sizeof(int) * st.size(), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpyAsync(st.data(), ctx->lp_stats.p, sizeof(int) * st.size(), hipMemcpyDeviceToHost, ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     for (size_t i = 0; i < js.size(); ++i) {
         ctx->lp_blocks += st[2 * i];
         ctx->lp_fallbacks += st[2 * i + 1];
@@ -1142,16 +1254,16 @@ extern "C" int dme_lp_set_weights(dme_ctx *ctx, const float *params, size_t coun
         return set_error(DME_ERR_INVALID, "the network has " + std::to_string(DME_LP_PARAMS) + " parameters");
     for (size_t i = 0; i < count; ++i)
         if (!std::isfinite(params[i])) return set_error(DME_ERR_INVALID, "non-finite network parameter");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     DME_TRY(ctx->lp_w.ensure(sizeof(float) * count));
     DME_TRY(ctx->lp_frag.ensure(sizeof(uint16_t) * 4 * 2 * DME_LP_FRAG_ELEMS));
     HIP_TRY(hipMemcpy(ctx->lp_w.p, params, sizeof(float) * count, hipMemcpyHostToDevice));
     for (int layer = 0; layer < 4; ++layer) {
         uint16_t *f = ctx->lp_frag.as<uint16_t>() + (size_t)layer * 2 * DME_LP_FRAG_ELEMS;
         HIP_TRY(launch_lp_frags(ctx->lp_w.as<float>() + (size_t)layer * DME_LP_LAYER_PARAMS, f, f + DME_LP_FRAG_ELEMS,
-                                ctx->stream));
+                                ctx->L->stream));
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->lp_ready = true;
     return DME_OK;
 }
@@ -1200,8 +1312,8 @@ extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_featu
     if (J->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
     std::vector<JobState *> js{J};
     int launches = 0;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->arena_used = 0;
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    ctx->L->arena_used = 0;
     DME_TRY(bind_planes(ctx, js, true, launches));
     J->desc.lp_off = 0;
     DME_TRY(upload_jobs(ctx, js));
@@ -1209,16 +1321,16 @@ extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_featu
     const int cnt = J->g.xnb * J->g.ynb;
     const TaskTable::Range feat = tt.add(std::vector<Task>{Task{0, 0, cnt, 0}});
     DME_TRY(upload_tasks(ctx, tt));
-    HIP_TRY(launch_lp_features(ctx->d_jobs.as<JobDesc>(), dtask(ctx, feat), feat.n, feat.total, d_features, d_labels,
-                               ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(launch_lp_features(ctx->L->d_jobs.as<JobDesc>(), dtask(ctx, feat), feat.n, feat.total, d_features, d_labels,
+                               ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
 }
 
 extern "C" int dme_lp_prepare(dme_ctx *ctx, const float *d_w, uint16_t *d_frag_fwd, uint16_t *d_frag_bwd) {
     CTX_CHECK(ctx);
     if (!d_w || !d_frag_fwd || !d_frag_bwd) return set_error(DME_ERR_INVALID, "NULL argument");
-    HIP_TRY(launch_lp_frags(d_w, d_frag_fwd, d_frag_bwd, ctx->stream));
+    HIP_TRY(launch_lp_frags(d_w, d_frag_fwd, d_frag_bwd, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1227,7 +1339,7 @@ extern "C" int dme_lp_conv(dme_ctx *ctx, const uint16_t *d_in, int n, int h, int
     CTX_CHECK(ctx);
     if (!d_in || !d_frag || (!d_out && !d_out_f32) || n <= 0 || h <= 0 || w <= 0)
         return set_error(DME_ERR_INVALID, "bad convolution arguments");
-    HIP_TRY(launch_lp_conv(d_in, n * h * w, h, w, d_frag, d_bias, relu, d_out, d_out_f32, ctx->stream));
+    HIP_TRY(launch_lp_conv(d_in, n * h * w, h, w, d_frag, d_bias, relu, d_out, d_out_f32, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1236,7 +1348,7 @@ extern "C" int dme_lp_wgrad(dme_ctx *ctx, const uint16_t *d_in, const uint16_t *
     CTX_CHECK(ctx);
     if (!d_in || !d_g || !d_dw || !d_db || n <= 0 || h <= 0 || w <= 0)
         return set_error(DME_ERR_INVALID, "bad gradient arguments");
-    HIP_TRY(launch_lp_wgrad(d_in, d_g, n * h * w, h, w, d_dw, d_db, ctx->stream));
+    HIP_TRY(launch_lp_wgrad(d_in, d_g, n * h * w, h, w, d_dw, d_db, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1244,7 +1356,7 @@ extern "C" int dme_lp_relu_backward(dme_ctx *ctx, const uint16_t *d_act, const f
                                     uint16_t *d_out) {
     CTX_CHECK(ctx);
     if (!d_g || !d_out) return set_error(DME_ERR_INVALID, "NULL argument");
-    HIP_TRY(launch_lp_relu_bwd(d_act, d_g, count, d_out, ctx->stream));
+    HIP_TRY(launch_lp_relu_bwd(d_act, d_g, count, d_out, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1252,7 +1364,7 @@ extern "C" int dme_lp_wta_loss(dme_ctx *ctx, const float *d_out, const float *d_
                                float beta, float scale, float *d_dout, float *d_loss) {
     CTX_CHECK(ctx);
     if (!d_out || !d_labels || !d_dout || !d_loss || npos <= 0) return set_error(DME_ERR_INVALID, "bad loss arguments");
-    HIP_TRY(launch_lp_wta(d_out, d_labels, npos, alpha, beta, scale, d_dout, d_loss, ctx->stream));
+    HIP_TRY(launch_lp_wta(d_out, d_labels, npos, alpha, beta, scale, d_dout, d_loss, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1261,14 +1373,14 @@ extern "C" int dme_lp_adam(dme_ctx *ctx, float *d_param, const float *d_grad, fl
     CTX_CHECK(ctx);
     if (!d_param || !d_grad || !d_m || !d_v || count <= 0 || t < 1) return set_error(DME_ERR_INVALID, "bad Adam arguments");
     const float bc1 = 1.0f - std::pow(beta1, (float)t), bc2 = 1.0f - std::pow(beta2, (float)t);
-    HIP_TRY(launch_lp_adam(d_param, d_grad, d_m, d_v, count, lr, beta1, beta2, eps, bc1, bc2, ctx->stream));
+    HIP_TRY(launch_lp_adam(d_param, d_grad, d_m, d_v, count, lr, beta1, beta2, eps, bc1, bc2, ctx->L->stream));
     return DME_OK;
 }
 
 extern "C" int dme_lp_zero(dme_ctx *ctx, void *d_ptr, size_t bytes) {
     CTX_CHECK(ctx);
     if (!d_ptr || (bytes & 3)) return set_error(DME_ERR_INVALID, "zeroing needs a pointer and a multiple of 4 bytes");
-    HIP_TRY(launch_lp_zero(d_ptr, bytes, ctx->stream));
+    HIP_TRY(launch_lp_zero(d_ptr, bytes, ctx->L->stream));
     return DME_OK;
 }
 
@@ -1286,7 +1398,7 @@ extern "C" int dme_lp_alloc(dme_ctx *ctx, size_t bytes, void **d_ptr) {
 
 extern "C" int dme_lp_free(dme_ctx *ctx, void *d_ptr) {
     CTX_CHECK(ctx);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     if (d_ptr) HIP_TRY(hipFree(d_ptr));
     return DME_OK;
 }
@@ -1295,7 +1407,7 @@ extern "C" int dme_lp_copy(dme_ctx *ctx, void *dst, const void *src, size_t byte
     CTX_CHECK(ctx);
     if (!dst || !src || kind < 1 || kind > 3) return set_error(DME_ERR_INVALID, "bad copy arguments");
     const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
-    HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
 }

@@ -612,6 +612,140 @@ __device__ __forceinline__ int packed_sad(const uint32_t *__restrict__ pic, int 
 }
 
 // ---------------------------------------------------------------------------
+// Byte planes.  The level-0 planes (the pictures themselves, whose samples
+// are checked to the 8-bit range at registration) are also kept as biased
+// bytes v + 128 with B8_MG replicated samples on every side: rows of bp bytes
+// (a multiple of 64), sample (0, 0) at byte B8_MG * bp + B8_MG of the
+// allocation, every byte of a padded row holding the edge-clamped sample of
+// its column.  A rectangle that stays within the margin is then copied to LDS
+// as whole dwords, with no clamping.
+// ---------------------------------------------------------------------------
+#define B8_MG kB8Margin
+
+// One thread per dword of the padded planes of a launch (PlaneTask: in = the
+// int16 plane, out = the byte allocation, in_w = width, ow = bp, oh = height;
+// tiles of 256 dwords).
+__global__ __launch_bounds__(256) void k_bytes(const PlaneTask *__restrict__ pt, int npt, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    int lo = 0, hi = npt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pt[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneTask p = pt[lo];
+    const int W = p.in_w, H = p.oh, dpr = p.ow >> 2;
+    const int idx = (tile - p.offset) * 256 + (int)threadIdx.x;
+    if (idx >= (H + 2 * B8_MG) * dpr) return;
+    const int ry = idx / dpr, cx = idx - ry * dpr;
+    const int16_t *__restrict__ row = p.in + (size_t)clampi(ry - B8_MG, 0, H - 1) * W;
+    uint32_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d |= (uint32_t)(uint8_t)(row[clampi(cx * 4 + k - B8_MG, 0, W - 1)] + 128) << (8 * k);
+    reinterpret_cast<uint32_t *>(p.out)[idx] = d;
+}
+
+// A rectangle [x0, x0 + rw) x [y0, y0 + rh) of a byte plane to be copied to
+// LDS rows of `pitch` dwords: nd dwords per row, byte i of a row holding
+// column x0 + i, bytes from rw on zero.
+struct B8Rect {
+    const uint8_t *src;  // dword-aligned byte of (x0 & ~3, y0)
+    uint32_t *dst;
+    int pitch, nd, n, sh, rw;
+    float inv;  // 1 / nd
+};
+
+__device__ __forceinline__ B8Rect b8_rect(const uint8_t *base, int bp, uint32_t *dst, int pitch, int x0, int y0, int rw,
+                                          int rh, int padw) {
+    B8Rect r;
+    r.sh = x0 & 3;
+    r.src = base + (ptrdiff_t)y0 * bp + (x0 - r.sh);
+    r.dst = dst;
+    r.pitch = pitch;
+    r.nd = (padw + 3) >> 2;
+    r.n = r.nd * rh;
+    r.rw = rw;
+    r.inv = 1.0f / (float)r.nd;
+    return r;
+}
+
+// True when every byte the dword copy of the rectangle reads lies in the
+// padded plane: columns [-B8_MG, bp - B8_MG), rows [-B8_MG, H + B8_MG).
+__device__ __forceinline__ bool b8_inside(int bp, int H, int x0, int y0, int rh, int padw) {
+    return x0 >= -B8_MG && y0 >= -B8_MG && y0 + rh <= H + B8_MG && x0 + 4 * ((padw + 3) >> 2) + 4 <= bp - B8_MG;
+}
+
+// Element i of a rectangle: row = i / nd, computed as a float product, exact
+// for i < 2^16 and nd <= 64 (the quotient is at least 0.5 / nd from an integer).
+__device__ __forceinline__ void b8_load(const B8Rect &r, int bp, int i0, uint32_t lo[4], uint32_t hi[4]) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = i0 + k * WAVE + lane;
+        lo[k] = hi[k] = 0;
+        if (i < r.n) {
+            const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(r.src + ry * bp) + c;
+            lo[k] = q[0];
+            hi[k] = q[1];
+        }
+    }
+}
+
+__device__ __forceinline__ void b8_store(const B8Rect &r, int i0, const uint32_t lo[4], const uint32_t hi[4]) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = i0 + k * WAVE + lane;
+        if (i < r.n) {
+            const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
+            uint32_t v = __builtin_amdgcn_alignbyte(hi[k], lo[k], r.sh);
+            const int valid = r.rw - 4 * c;
+            if (valid < 4) v &= valid <= 0 ? 0u : ((1u << (8 * valid)) - 1u);
+            r.dst[ry * r.pitch + c] = v;
+        }
+    }
+}
+
+// Copy of a rectangle that leaves the margin: bytes with clamped coordinates.
+__device__ __forceinline__ void b8_copy_clamped(const B8Rect &r, const uint8_t *base, int bp, int W, int H, int x0,
+                                                int y0) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int i = lane; i < r.n; i += WAVE) {
+        const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
+        const uint8_t *row = base + (ptrdiff_t)clampi(y0 + ry, 0, H - 1) * bp;
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (4 * c + k < r.rw) v |= (uint32_t)row[clampi(x0 + 4 * c + k, 0, W - 1)] << (8 * k);
+        r.dst[ry * r.pitch + c] = v;
+    }
+}
+
+// Copies up to three rectangles of byte planes (rect i used if i < nr), all
+// loads of a round issued before any store.
+__device__ __forceinline__ void b8_stage(const B8Rect *rc, const bool *fast, int nr, const uint8_t *const *base, int bp,
+                                         int W, int H, const int *x0, const int *y0) {
+    int nmax = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if (q < nr && fast[q]) nmax = max(nmax, rc[q].n);
+    for (int i0 = 0; i0 < nmax; i0 += 4 * WAVE) {
+        uint32_t lo[3][4], hi[3][4];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (q < nr && fast[q]) b8_load(rc[q], bp, i0, lo[q], hi[q]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (q < nr && fast[q]) b8_store(rc[q], i0, lo[q], hi[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if (q < nr && !fast[q]) b8_copy_clamped(rc[q], base[q], bp, W, H, x0[q], y0[q]);
+}
+
+// ---------------------------------------------------------------------------
 // Pel search, bulk part.  For every block of a level, the SADs of every
 // candidate within R = r + 1 (r = min(level+1, 5), pixel_match.cpp:258-259)
 // of the zero vector and of the guide vector (levels below the coarsest) are
@@ -627,8 +761,6 @@ __device__ __forceinline__ int packed_sad(const uint32_t *__restrict__ pic, int 
 #define PEL_MAXR 6
 #define PEL_REG (PEL_MAXB + 2 * PEL_MAXR)  // 44
 #define PEL_MAXTS ((2 * PEL_MAXR + 1) * (2 * PEL_MAXR + 1))
-#define PEL_PP (PEL_MAXB / 2)        // packed block row: 16 dwords
-#define PEL_RP 25                    // packed region row in dwords: odd, so rows use different LDS banks
 
 __device__ __forceinline__ int pel_radius(int level) { return min(level + 1, 5); }
 // Pel record of a block (PEL_REC ints, blocks in wavefront order):
@@ -652,17 +784,29 @@ __device__ __forceinline__ int wave_min_i(int v) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  \
     } while (0)
 
+// LDS of one wave of k_pel_bulk, in dwords, laid out by the host for the
+// largest block and radius of the launch: the picture block (rows of pp
+// dwords), the two reference regions (rows of rp dwords, rp odd so that
+// rows start in different banks; the key buffer reuses this area once the
+// SADs are done), then the SAD tables.
+struct PelLds {
+    int pp, rp, pic, reg, keys, tab;
+    __device__ int wave() const { return pic + keys + tab; }
+};
+
 // U8: the level's planes hold 8-bit-range samples (level 0: the pictures
-// themselves), staged as bytes for v_sad_u8; otherwise (pyramid levels, whose
-// 12-tap filter can overshoot that range) as 16-bit samples for v_sad_u16.
+// themselves), staged from the byte planes for v_sad_u8; otherwise (pyramid
+// levels, whose 12-tap filter can overshoot that range) as 16-bit samples for
+// v_sad_u16.
 template <bool U8>
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                  int ntasks, int total, int level) {
-    // packed staging: block rows of PEL_PP dwords, region rows of PEL_RP dwords
-    __shared__ uint32_t s_pic[4][PEL_MAXB * PEL_PP];
-    __shared__ uint32_t s_reg[4][2][PEL_REG * PEL_RP];
-    __shared__ int s_tab[4][2 * PEL_MAXTS];
+                                                  int ntasks, int total, int level, PelLds ly) {
+    extern __shared__ int s_dyn[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *__restrict__ s_pic = reinterpret_cast<uint32_t *>(s_dyn) + wv * ly.wave();
+    uint32_t *__restrict__ s_reg0 = s_pic + ly.pic;
+    uint32_t *__restrict__ s_reg1 = s_reg0 + ly.reg;
+    int *__restrict__ s_tab = reinterpret_cast<int *>(s_pic + ly.pic + ly.keys);
     const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;  // each wave works alone
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
@@ -671,8 +815,6 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     const int X = J.lxb[level];
     const int xb = local % X, yb = local / X;
     const int W = J.lw[level], H = J.lh[level];
-    const int16_t *__restrict__ pic = J.pic[level];
-    const int16_t *__restrict__ ref = J.ref[r][level];
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
     const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
     const int n = (2 * rad + 1) * (2 * rad + 1);
@@ -696,41 +838,49 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     if (!b.empty()) {
         const int RW = b.xl + 2 * R, RH = b.yl + 2 * R;
         if (U8) {
-            wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic[wv]), 4 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
-                          (b.xl + 15) & ~15);
-            wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW, RH,
-                          RW);
-            if (has_guide)
-                wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][1]), 4 * PEL_RP, ref, W, H, b.xp + g.x - R,
-                              b.yp + g.y - R, RW, RH, RW);
+            const int bp = J.p8;
+            const uint8_t *const base[3] = {J.pic8, J.ref8[r], J.ref8[r]};
+            const int x0[3] = {b.xp, b.xp - R, b.xp + g.x - R}, y0[3] = {b.yp, b.yp - R, b.yp + g.y - R};
+            const int padw[3] = {(b.xl + 15) & ~15, RW, RW};
+            B8Rect rc[3];
+            bool fast[3];
+            rc[0] = b8_rect(base[0], bp, s_pic, ly.pp, x0[0], y0[0], b.xl, b.yl, padw[0]);
+            rc[1] = b8_rect(base[1], bp, s_reg0, ly.rp, x0[1], y0[1], RW, RH, padw[1]);
+            rc[2] = b8_rect(base[2], bp, s_reg1, ly.rp, x0[2], y0[2], RW, RH, padw[2]);
+            fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl, padw[0]);
+            fast[1] = b8_inside(bp, H, x0[1], y0[1], RH, padw[1]);
+            fast[2] = b8_inside(bp, H, x0[2], y0[2], RH, padw[2]);
+            b8_stage(rc, fast, has_guide ? 3 : 2, base, bp, W, H, x0, y0);
         } else {
-            wave_stage_u16(reinterpret_cast<uint16_t *>(s_pic[wv]), 2 * PEL_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
+            const int16_t *__restrict__ pic = J.pic[level];
+            const int16_t *__restrict__ ref = J.ref[r][level];
+            wave_stage_u16(reinterpret_cast<uint16_t *>(s_pic), 2 * ly.pp, pic, W, H, b.xp, b.yp, b.xl, b.yl,
                            (b.xl + 7) & ~7);
-            wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][0]), 2 * PEL_RP, ref, W, H, b.xp - R, b.yp - R, RW,
-                           RH, RW);
+            wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg0), 2 * ly.rp, ref, W, H, b.xp - R, b.yp - R, RW, RH, RW);
             if (has_guide)
-                wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg[wv][1]), 2 * PEL_RP, ref, W, H, b.xp + g.x - R,
+                wave_stage_u16(reinterpret_cast<uint16_t *>(s_reg1), 2 * ly.rp, ref, W, H, b.xp + g.x - R,
                                b.yp + g.y - R, RW, RH, RW);
         }
         WAVE_SYNC();
         for (int c = lane; c < ntab * TS; c += WAVE) {
             const int t = c >= TS ? 1 : 0, e = c - t * TS;
             const int ox = e % D, oy = e / D;  // region offset of the candidate
-            s_tab[wv][c] = U8 ? packed_sad_u8(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl)
-                              : packed_sad(s_pic[wv], PEL_PP, s_reg[wv][t], PEL_RP, ox, oy, b.xl, b.yl);
+            const uint32_t *reg = t ? s_reg1 : s_reg0;
+            s_tab[c] = U8 ? packed_sad_u8(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
+                          : packed_sad(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
         }
         WAVE_SYNC();
         // first minima (key = SAD * 128 + enumeration index) of the nine
         // windows around each centre: the keys of every (window, candidate)
         // go to LDS (over the dead region staging), then lane t*9 + w scans
         // its window's n keys
-        int *kb = reinterpret_cast<int *>(s_reg[wv][0]);
+        int *kb = reinterpret_cast<int *>(s_reg0);
         const int nk = ntab * 9 * n;
         for (int i = lane; i < nk; i += WAVE) {
             const int tw = i / n, c = i - tw * n;
             const int t = tw >= 9 ? 1 : 0, w = tw - 9 * t;
             const dme_mv o = win_off(c, rad);
-            kb[i] = s_tab[wv][t * TS + (w / 3 - 1 + R + o.y) * D + (w % 3 - 1 + R + o.x)] * 128 + c;
+            kb[i] = s_tab[t * TS + (w / 3 - 1 + R + o.y) * D + (w % 3 - 1 + R + o.x)] * 128 + c;
         }
         WAVE_SYNC();
         if (lane < ntab * 9) {
@@ -2359,13 +2509,39 @@ hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, hipStream_t st) {
+hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    if (level == 0)  // the pictures themselves: samples checked to 8-bit range at registration
-        hipLaunchKernelGGL(k_pel_bulk<true>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total, level);
+    hipLaunchKernelGGL(k_bytes, dim3(total), dim3(256), 0, st, pt, npt, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
+                           int max_blen_y, bool has_guide, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    const bool u8 = level == 0;  // the pictures themselves: samples checked to 8-bit range at registration
+    const int rad = min(level + 1, 5), R = rad + 1, D = 2 * R + 1, n = (2 * rad + 1) * (2 * rad + 1);
+    const int ntab = has_guide ? 2 : 1;
+    // dwords of a packed row: per = samples per dword; the SAD reads 4-dword
+    // chunks of the block row and one dword more of the region row, from
+    // dword offset (2R) / per
+    const int per = u8 ? 4 : 2;
+    const int nch = ((max_blen_x + per - 1) / per + 3) / 4;
+    PelLds ly;
+    ly.pp = 4 * nch;
+    ly.rp = max((max_blen_x + 2 * R + per - 1) / per, (2 * R) / per + 4 * nch + 1) | 1;
+    ly.pic = ly.pp * max_blen_y;
+    ly.reg = ly.rp * (max_blen_y + 2 * R);
+    ly.keys = max(2 * ly.reg, ntab * 9 * n);
+    ly.tab = ntab * D * D;
+    const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pic + ly.keys + ly.tab);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    if (u8)
+        hipLaunchKernelGGL(k_pel_bulk<true>, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total,
+                           level, ly);
     else
-        hipLaunchKernelGGL(k_pel_bulk<false>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total,
-                           level);
+        hipLaunchKernelGGL(k_pel_bulk<false>, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total,
+                           level, ly);
     return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ class dme_job(C.Structure):
     _fields_ = [("pic_num", C.c_int32), ("nrefs", C.c_int32), ("refs", C.c_int32 * 2), ("params", dme_params)]
 
 
-DME_NUM_KERNELS = 16
+DME_NUM_KERNELS = 24
 
 
 class dme_stats(C.Structure):

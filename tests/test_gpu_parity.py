@@ -190,3 +190,18 @@ def test_luma_outside_8bit_range_is_rejected(ctx):
     with pytest.raises(dirac_me.DmeError):
         ctx.upload(502, y)
     ctx.drop(500)
+
+
+def test_large_motion_leaves_byte_margin(ctx):
+    # pans of about 40 pel: the search follows them out of the picture (level-0
+    # vectors beyond 100 pel), so reference regions leave the replicated
+    # margin of the byte planes and are copied with clamped coordinates
+    cur = synth.texture(61, 352, 288, 3)
+    u, v = synth.chroma(cur, 2)
+    refs = [synth.warp(cur, 1.0, (44.0, -38.0), (), 1.0, 5), synth.warp(cur, 1.0, (-41.0, 36.0), (), 1.0, 6)]
+    d = dict(cur=cur, u=u, v=v, refs=refs)
+    p = make_params(12, 12, 8, 8, precision=2, depth=4, lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    far = max(int(np.abs(want.arrays(2)[k][f]).max()) for k in ("vectors1", "vectors2") for f in ("x", "y"))
+    assert far > 4 * 40  # quarter pel
+    _assert_same(want, _gpu(ctx, p, d), 2)
