@@ -47,6 +47,10 @@ __device__ __forceinline__ int xcd_block() {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
+// Index of the calling wave in its workgroup, known to the compiler as
+// wave-uniform, so that what is derived from it stays in scalar registers.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 __device__ __forceinline__ int find_task(const Task *t, int n, int gi) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__
     const int W = p.ow, H = p.oh;
     const int tx = (W + 63) / 64;
     const int x = (local % tx) * 64 + (threadIdx.x & 63);
-    const int y = (local / tx) * 4 + (threadIdx.x >> 6);
+    const int y = (local / tx) * 4 + wave_id();
     const int16_t *__restrict__ in = p.in;
     int16_t *__restrict__ up = p.out;
     if (x >= W || y >= H) return;
@@ -456,6 +460,8 @@ __device__ __forceinline__ void wave_stage_u8(uint8_t *__restrict__ dst, int pit
 
 // SAD of an xl x yl block (packed bytes, rows of pp dwords, zero beyond xl)
 // against packed byte rows of rp dwords starting at byte bo of row row0.
+// MASK false: xl is a multiple of 16, no region byte needs masking.
+template <bool MASK = true>
 __device__ __forceinline__ int packed_sad_u8(const uint32_t *__restrict__ pic, int pp, const uint32_t *__restrict__ reg,
                                              int rp, int bo, int row0, int xl, int yl) {
     const int nd = (xl + 3) >> 2, nch = (nd + 3) >> 2;
@@ -476,7 +482,7 @@ __device__ __forceinline__ int packed_sad_u8(const uint32_t *__restrict__ pic, i
                            r4 = rr[4 * c + 4];
             uint32_t a0 = __builtin_amdgcn_alignbyte(r1, r0, sh), a1 = __builtin_amdgcn_alignbyte(r2, r1, sh);
             uint32_t a2 = __builtin_amdgcn_alignbyte(r3, r2, sh), a3 = __builtin_amdgcn_alignbyte(r4, r3, sh);
-            if (c == nch - 1) {
+            if (MASK && c == nch - 1) {
                 a0 &= m[0];
                 a1 &= m[1];
                 a2 &= m[2];
@@ -574,7 +580,9 @@ __device__ __forceinline__ void wave_stage_u16(uint16_t *__restrict__ dst, int p
 }
 
 // SAD of an xl x yl block (packed rows of pp dwords, zero beyond xl) against
-// the packed region rows of rp dwords starting at sample (ox, oy).
+// the packed region rows of rp dwords starting at sample (ox, oy).  MASK
+// false: xl is a multiple of 8, no region sample needs masking.
+template <bool MASK = true>
 __device__ __forceinline__ int packed_sad(const uint32_t *__restrict__ pic, int pp, const uint32_t *__restrict__ reg,
                                           int rp, int ox, int oy, int xl, int yl) {
     const int nd = (xl + 1) >> 1, nch = (nd + 3) >> 2;
@@ -596,7 +604,7 @@ __device__ __forceinline__ int packed_sad(const uint32_t *__restrict__ pic, int 
                            r4 = rr[4 * c + 4];
             uint32_t a0 = __builtin_amdgcn_alignbyte(r1, r0, sh), a1 = __builtin_amdgcn_alignbyte(r2, r1, sh);
             uint32_t a2 = __builtin_amdgcn_alignbyte(r3, r2, sh), a3 = __builtin_amdgcn_alignbyte(r4, r3, sh);
-            if (c == nch - 1) {
+            if (MASK && c == nch - 1) {
                 a0 &= m[0];
                 a1 &= m[1];
                 a2 &= m[2];
@@ -648,25 +656,34 @@ __global__ __launch_bounds__(256) void k_bytes(const PlaneTask *__restrict__ pt,
 
 // A rectangle [x0, x0 + rw) x [y0, y0 + rh) of a byte plane to be copied to
 // LDS rows of `pitch` dwords: nd dwords per row, byte i of a row holding
-// column x0 + i, bytes from rw on zero.
+// column x0 + i, bytes from rw on zero.  Each lane keeps one dword column of
+// a band of rpp = 64 / nd rows and steps down by rpp rows per pass.
 struct B8Rect {
     const uint8_t *src;  // dword-aligned byte of (x0 & ~3, y0)
     uint32_t *dst;
-    int pitch, nd, n, sh, rw;
-    float inv;  // 1 / nd
+    int pitch, rh, sh, rpp;
+    int row0, col;  // this lane's first row and its dword column
+    uint32_t mask;  // bytes of the lane's dword that lie below rw
+    bool on;        // lane < rpp * nd
 };
 
 __device__ __forceinline__ B8Rect b8_rect(const uint8_t *base, int bp, uint32_t *dst, int pitch, int x0, int y0, int rw,
                                           int rh, int padw) {
+    const int lane = threadIdx.x & (WAVE - 1);
     B8Rect r;
     r.sh = x0 & 3;
     r.src = base + (ptrdiff_t)y0 * bp + (x0 - r.sh);
     r.dst = dst;
     r.pitch = pitch;
-    r.nd = (padw + 3) >> 2;
-    r.n = r.nd * rh;
-    r.rw = rw;
-    r.inv = 1.0f / (float)r.nd;
+    r.rh = rh;
+    const int nd = (padw + 3) >> 2;
+    r.rpp = WAVE / nd;
+    // lane / nd as a float product: exact for lane < 64, nd <= 64
+    r.row0 = (int)(((float)lane + 0.5f) * (1.0f / (float)nd));
+    r.col = lane - r.row0 * nd;
+    r.on = r.row0 < r.rpp;
+    const int valid = rw - 4 * r.col;
+    r.mask = valid >= 4 ? 0xffffffffu : (valid <= 0 ? 0u : ((1u << (8 * valid)) - 1u));
     return r;
 }
 
@@ -676,69 +693,58 @@ __device__ __forceinline__ bool b8_inside(int bp, int H, int x0, int y0, int rh,
     return x0 >= -B8_MG && y0 >= -B8_MG && y0 + rh <= H + B8_MG && x0 + 4 * ((padw + 3) >> 2) + 4 <= bp - B8_MG;
 }
 
-// Element i of a rectangle: row = i / nd, computed as a float product, exact
-// for i < 2^16 and nd <= 64 (the quotient is at least 0.5 / nd from an integer).
-__device__ __forceinline__ void b8_load(const B8Rect &r, int bp, int i0, uint32_t lo[4], uint32_t hi[4]) {
-    const int lane = threadIdx.x & (WAVE - 1);
+// Passes p0 .. p0 + 3 of a rectangle: two aligned dword loads per element...
+__device__ __forceinline__ void b8_load(const B8Rect &r, int bp, int p0, uint32_t lo[4], uint32_t hi[4]) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int i = i0 + k * WAVE + lane;
+        const int row = r.row0 + (p0 + k) * r.rpp;
         lo[k] = hi[k] = 0;
-        if (i < r.n) {
-            const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
-            const uint32_t *q = reinterpret_cast<const uint32_t *>(r.src + ry * bp) + c;
+        if (r.on && row < r.rh) {
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(r.src + row * bp) + r.col;
             lo[k] = q[0];
             hi[k] = q[1];
         }
     }
 }
 
-__device__ __forceinline__ void b8_store(const B8Rect &r, int i0, const uint32_t lo[4], const uint32_t hi[4]) {
-    const int lane = threadIdx.x & (WAVE - 1);
+// ... realigned to the rectangle's first column and masked to its width.
+__device__ __forceinline__ void b8_store(const B8Rect &r, int p0, const uint32_t lo[4], const uint32_t hi[4]) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int i = i0 + k * WAVE + lane;
-        if (i < r.n) {
-            const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
-            uint32_t v = __builtin_amdgcn_alignbyte(hi[k], lo[k], r.sh);
-            const int valid = r.rw - 4 * c;
-            if (valid < 4) v &= valid <= 0 ? 0u : ((1u << (8 * valid)) - 1u);
-            r.dst[ry * r.pitch + c] = v;
-        }
+        const int row = r.row0 + (p0 + k) * r.rpp;
+        if (r.on && row < r.rh) r.dst[row * r.pitch + r.col] = __builtin_amdgcn_alignbyte(hi[k], lo[k], r.sh) & r.mask;
     }
 }
 
 // Copy of a rectangle that leaves the margin: bytes with clamped coordinates.
 __device__ __forceinline__ void b8_copy_clamped(const B8Rect &r, const uint8_t *base, int bp, int W, int H, int x0,
                                                 int y0) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int i = lane; i < r.n; i += WAVE) {
-        const int ry = (int)(((float)i + 0.5f) * r.inv), c = i - ry * r.nd;
-        const uint8_t *row = base + (ptrdiff_t)clampi(y0 + ry, 0, H - 1) * bp;
+    if (!r.on) return;
+    for (int row = r.row0; row < r.rh; row += r.rpp) {
+        const uint8_t *src = base + (ptrdiff_t)clampi(y0 + row, 0, H - 1) * bp;
         uint32_t v = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (4 * c + k < r.rw) v |= (uint32_t)row[clampi(x0 + 4 * c + k, 0, W - 1)] << (8 * k);
-        r.dst[ry * r.pitch + c] = v;
+        for (int k = 0; k < 4; ++k) v |= (uint32_t)src[clampi(x0 + 4 * r.col + k, 0, W - 1)] << (8 * k);
+        r.dst[row * r.pitch + r.col] = v & r.mask;
     }
 }
 
-// Copies up to three rectangles of byte planes (rect i used if i < nr), all
-// loads of a round issued before any store.
+// Copies up to three rectangles of byte planes (rect i used if i < nr), the
+// loads of four passes of every rectangle issued before any store.
 __device__ __forceinline__ void b8_stage(const B8Rect *rc, const bool *fast, int nr, const uint8_t *const *base, int bp,
                                          int W, int H, const int *x0, const int *y0) {
-    int nmax = 0;
+    int np = 0;  // passes of the tallest fast rectangle
 #pragma unroll
     for (int q = 0; q < 3; ++q)
-        if (q < nr && fast[q]) nmax = max(nmax, rc[q].n);
-    for (int i0 = 0; i0 < nmax; i0 += 4 * WAVE) {
+        if (q < nr && fast[q]) np = max(np, (rc[q].rh + rc[q].rpp - 1) / rc[q].rpp);
+    for (int p0 = 0; p0 < np; p0 += 4) {
         uint32_t lo[3][4], hi[3][4];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-            if (q < nr && fast[q]) b8_load(rc[q], bp, i0, lo[q], hi[q]);
+            if (q < nr && fast[q]) b8_load(rc[q], bp, p0, lo[q], hi[q]);
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-            if (q < nr && fast[q]) b8_store(rc[q], i0, lo[q], hi[q]);
+            if (q < nr && fast[q]) b8_store(rc[q], p0, lo[q], hi[q]);
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q)
@@ -784,11 +790,35 @@ __device__ __forceinline__ int wave_min_i(int v) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  \
     } while (0)
 
+// Smallest key SAD * 128 + k over the radius-RAD window centred at tb (a
+// table of row pitch 2 * RAD + 3), k the candidate's index in AddNewVlist
+// order (block_match.cpp:89-126): 0 the centre, 2i-1 and 2i at (+i, 0) and
+// (-i, 0), then for each row j = 1 .. RAD the pairs (x, +j), (x, -j) for x =
+// -RAD .. RAD.  Keys are distinct, so the scan order does not matter.
+template <int RAD>
+__device__ __forceinline__ int pel_window_min(const int *tb) {
+    constexpr int D = 2 * RAD + 3;
+    int v = tb[0] * 128;
+#pragma unroll
+    for (int i = 1; i <= RAD; ++i) {
+        v = min(v, tb[i] * 128 + 2 * i - 1);
+        v = min(v, tb[-i] * 128 + 2 * i);
+    }
+#pragma unroll
+    for (int j = 1; j <= RAD; ++j)
+#pragma unroll
+        for (int x = -RAD; x <= RAD; ++x) {
+            const int k = 2 * RAD + 1 + (j - 1) * 2 * (2 * RAD + 1) + 2 * (x + RAD);
+            v = min(v, tb[j * D + x] * 128 + k);
+            v = min(v, tb[-j * D + x] * 128 + k + 1);
+        }
+    return v;
+}
+
 // LDS of one wave of k_pel_bulk, in dwords, laid out by the host for the
 // largest block and radius of the launch: the picture block (rows of pp
 // dwords), the two reference regions (rows of rp dwords, rp odd so that
-// rows start in different banks; the key buffer reuses this area once the
-// SADs are done), then the SAD tables.
+// rows start in different banks; `keys` dwords in all), then the SAD tables.
 struct PelLds {
     int pp, rp, pic, reg, keys, tab;
     __device__ int wave() const { return pic + keys + tab; }
@@ -802,7 +832,7 @@ template <bool U8>
 __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total, int level, PelLds ly) {
     extern __shared__ int s_dyn[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_id(), lane = threadIdx.x & 63;
     uint32_t *__restrict__ s_pic = reinterpret_cast<uint32_t *>(s_dyn) + wv * ly.wave();
     uint32_t *__restrict__ s_reg0 = s_pic + ly.pic;
     uint32_t *__restrict__ s_reg1 = s_reg0 + ly.reg;
@@ -862,32 +892,31 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
                                b.yp + g.y - R, RW, RH, RW);
         }
         WAVE_SYNC();
+        const bool whole = (b.xl & (U8 ? 15 : 7)) == 0;  // no partial dword chunk
         for (int c = lane; c < ntab * TS; c += WAVE) {
             const int t = c >= TS ? 1 : 0, e = c - t * TS;
-            const int ox = e % D, oy = e / D;  // region offset of the candidate
+            const int oy = e / D, ox = e - oy * D;  // region offset of the candidate
             const uint32_t *reg = t ? s_reg1 : s_reg0;
-            s_tab[c] = U8 ? packed_sad_u8(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
-                          : packed_sad(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
+            if (U8)
+                s_tab[c] = whole ? packed_sad_u8<false>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
+                                 : packed_sad_u8<true>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
+            else
+                s_tab[c] = whole ? packed_sad<false>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
+                                 : packed_sad<true>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
         }
         WAVE_SYNC();
         // first minima (key = SAD * 128 + enumeration index) of the nine
-        // windows around each centre: the keys of every (window, candidate)
-        // go to LDS (over the dead region staging), then lane t*9 + w scans
-        // its window's n keys
-        int *kb = reinterpret_cast<int *>(s_reg0);
-        const int nk = ntab * 9 * n;
-        for (int i = lane; i < nk; i += WAVE) {
-            const int tw = i / n, c = i - tw * n;
-            const int t = tw >= 9 ? 1 : 0, w = tw - 9 * t;
-            const dme_mv o = win_off(c, rad);
-            kb[i] = s_tab[t * TS + (w / 3 - 1 + R + o.y) * D + (w % 3 - 1 + R + o.x)] * 128 + c;
-        }
-        WAVE_SYNC();
+        // windows around each centre: lane t*9 + w scans window w of table t
         if (lane < ntab * 9) {
-            const int *kw = kb + lane * n;
-            int v = kw[0];
-            for (int c = 1; c < n; ++c) v = min(v, kw[c]);
-            mykey = v;
+            const int t = lane >= 9 ? 1 : 0, w = lane - 9 * t;
+            const int *tb = s_tab + t * TS + (w / 3 - 1 + R) * D + (w % 3 - 1 + R);
+            switch (rad) {
+                case 1: mykey = pel_window_min<1>(tb); break;
+                case 2: mykey = pel_window_min<2>(tb); break;
+                case 3: mykey = pel_window_min<3>(tb); break;
+                case 4: mykey = pel_window_min<4>(tb); break;
+                default: mykey = pel_window_min<5>(tb); break;
+            }
         }
         // zero list, then guide list (radius rad, centred windows)
         const int k0 = __shfl(mykey, 4, WAVE), k1 = __shfl(mykey, 13, WAVE);
@@ -1266,22 +1295,34 @@ __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
 // per block, blocks in wavefront order): SAD (dy+rad)*Q + dx+rad, then
 // lambda, pel vector and pel costs at 49..54.
 // ---------------------------------------------------------------------------
-#define SUB_MAXREG (2 * PEL_MAXB + 3)  // 67
 #define SUB_SLOT 56                    // 49 SADs, lambda, pel vector, pel costs
 #define SUB_PP 8                       // packed picture row: 32 bytes
 
 __device__ __forceinline__ int sub_radius(int prec) { return prec == 2 ? 3 : 1; }
 
-// qrows: rows of one phase region; qd: dwords per region row
+// Four qp dwords of consecutive positions (each the four horizontal phases
+// of one position) to the four phase dwords (each one phase of the four
+// positions): a 4 x 4 byte transpose with v_perm.
+__device__ __forceinline__ void qp_transpose(const uint32_t d[4], uint32_t e[4]) {
+    const uint32_t lo01 = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u), hi01 = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+    const uint32_t lo23 = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u), hi23 = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+    e[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    e[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    e[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    e[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// pp: dwords per picture row (4 * chunks); prows: picture rows; qrows: rows
+// of one phase region; qd: dwords per region row
 __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                     int ntasks, int total, int qrows, int qd) {
+                                                     int ntasks, int total, int pp, int prows, int qrows, int qd) {
     extern __shared__ uint32_t s_sub[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_id(), lane = threadIdx.x & 63;
     const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;  // each wave works alone
-    const int per_wave = PEL_MAXB * SUB_PP + 16 * qrows * qd;
-    uint32_t *s_pic = s_sub + wv * per_wave;
-    uint32_t *s_q = s_pic + PEL_MAXB * SUB_PP;  // [fy * 4 + fx][row][qd]
+    const int plane = qrows * qd;  // dwords of one phase region
+    uint32_t *s_pic = s_sub + wv * (pp * prows + 16 * plane);
+    uint32_t *s_q = s_pic + pp * prows;  // [fy * 4 + fx][row][qd]
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, k = gi - tk.offset;
@@ -1297,40 +1338,63 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
         const int cqx = (pel.x << prec) << sq, cqy = (pel.y << prec) << sq, rq = rad << sq;
         const int qx0 = (cqx - rq) >> 2, qy0 = (cqy - rq) >> 2;
         const int RW = b.xl + ((cqx + rq) >> 2) - qx0, RH = b.yl + ((cqy + rq) >> 2) - qy0;
-        wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic), 4 * SUB_PP, J.pic[0], W, H, b.xp, b.yp, b.xl, b.yl,
-                      (b.xl + 15) & ~15);
-        // phase regions: one dword per (phase row fy, position) holds the four fx
+        // picture block from the byte plane; its loads go out with the first
+        // phase-region loads
+        const int ppad = (b.xl + 15) & ~15;
+        const B8Rect pr = b8_rect(J.pic8, J.p8, s_pic, pp, b.xp, b.yp, b.xl, b.yl, ppad);
+        const bool pfast = b8_inside(J.p8, H, b.xp, b.yp, b.yl, ppad);
+        uint32_t plo[4], phi[4];
+        if (pfast) b8_load(pr, J.p8, 0, plo, phi);  // at most 32 rows, 8 or more per pass
+        // phase regions from the quarter-pel planes: element (R, c) holds
+        // positions X0 + 4c .. + 3 of qp row R = ry * 4 + fy (position row
+        // Y0 + ry clamped to [-1, H], phase row fy), four dword loads
+        // transposed into one dword of each of the four phase planes fx
         const uint32_t *__restrict__ qp = J.qp[r];
-        // lanes keep a column; rows run over (fy, ry), RW <= 34
-        const int rpi = WAVE / RW, rx = lane % RW, row0 = lane / RW;
-        const bool on = row0 < rpi;
-        const int Xq = clampi(b.xp + qx0 + rx, -1, W) + 1, NR = 4 * RH;
-        for (int R0 = 0; R0 < NR; R0 += 8 * rpi) {
-            uint32_t v[8];
+        const int X0 = b.xp + qx0, Y0 = b.yp + qy0;
+        const int ncd = (RW + 3) >> 2, rpp = WAVE / ncd;
+        const int lr = (int)(((float)lane + 0.5f) * (1.0f / (float)ncd)), lc = lane - lr * ncd;
+        const bool on = lr < rpp;
+        int xo[4];  // qp column (position + 1) of this lane's four positions, clamped to [-1, W]
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int R = R0 + e * rpi + row0;
-                const int fy = (R >= RH) + (R >= 2 * RH) + (R >= 3 * RH), ry = R - fy * RH;
-                v[e] = 0;
-                if (on && R < NR) v[e] = qp[(size_t)((clampi(b.yp + qy0 + ry, -1, H) + 1) * 4 + fy) * QW + Xq];
+        for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * lc + i, -1, W) + 1;
+        const int NR = 4 * RH, np = (NR + rpp - 1) / rpp;
+        for (int p0 = 0; p0 < np; p0 += 4) {
+            uint32_t d[4][4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int R = lr + (p0 + e) * rpp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) d[e][i] = 0;
+                if (on && R < NR) {
+                    const uint32_t *row = qp + (size_t)((clampi(Y0 + (R >> 2), -1, H) + 1) * 4 + (R & 3)) * QW;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[e][i] = row[xo[i]];
+                }
+            }
+            if (p0 == 0) {
+                if (pfast) b8_store(pr, 0, plo, phi);
+                else b8_copy_clamped(pr, J.pic8, J.p8, W, H, b.xp, b.yp);
             }
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int R = R0 + e * rpi + row0;
-                const int fy = (R >= RH) + (R >= 2 * RH) + (R >= 3 * RH), ry = R - fy * RH;
+            for (int e = 0; e < 4; ++e) {
+                const int R = lr + (p0 + e) * rpp;
                 if (on && R < NR) {
-                    uint8_t *row = reinterpret_cast<uint8_t *>(s_q + (fy * 4) * qrows * qd + ry * qd) + rx;
+                    uint32_t t[4];
+                    qp_transpose(d[e], t);
+                    uint32_t *dst = s_q + (R & 3) * 4 * plane + (R >> 2) * qd + lc;
 #pragma unroll
-                    for (int fx = 0; fx < 4; ++fx) row[fx * qrows * qd * 4] = (uint8_t)(v[e] >> (8 * fx));
+                    for (int fx = 0; fx < 4; ++fx) dst[fx * plane] = t[fx];
                 }
             }
         }
         WAVE_SYNC();
         if (lane < QS) {
-            const int mx = (cqx >> 0) + ((lane % Q - rad) << sq), my = cqy + ((lane / Q - rad) << sq);
+            const int mx = cqx + ((lane % Q - rad) << sq), my = cqy + ((lane / Q - rad) << sq);
             const int fx = mx & 3, fy = my & 3;
-            const uint32_t *plane = s_q + (fy * 4 + fx) * qrows * qd;
-            rec[lane] = packed_sad_u8(s_pic, SUB_PP, plane, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl);
+            const uint32_t *src = s_q + (fy * 4 + fx) * plane;
+            rec[lane] = (b.xl & 15) == 0
+                            ? packed_sad_u8<false>(s_pic, pp, src, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl)
+                            : packed_sad_u8<true>(s_pic, pp, src, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl);
         }
     }
     if (lane == 0) {
@@ -1749,7 +1813,7 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 // parallel: it only reads the block's own refined vectors and costs.
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
-    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + wave_id();
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -1840,7 +1904,7 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
 
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                      int ntasks, int total) {
-    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + wave_id();
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -2040,7 +2104,7 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ jobs) {
     const JobDesc &J = jobs[blockIdx.x];
     const int XS = J.xsb, YS = J.ysb, X2 = J.xnb, X1 = J.xnb / 2;
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
+    const int wave = wave_id(), nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
     const int nr = J.nrefs;
     const dme_mv *__restrict__ V0 = J.vec[0];
     const dme_mv *__restrict__ V1 = J.vec[1];
@@ -2295,7 +2359,7 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
                                                    int *__restrict__ stats) {
     __shared__ uint32_t s_pic[4][PEL_MAXB * SUB_PP];
     __shared__ uint32_t s_reg[4][4][LP_RROWS * LP_RP];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_id(), lane = threadIdx.x & 63;
     const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
@@ -2453,7 +2517,7 @@ __global__ __launch_bounds__(256) void k_lp_pel_costs(const JobDesc *__restrict_
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total) {
-    const int gi = xcd_block() * 4 + (threadIdx.x >> 6);
+    const int gi = xcd_block() * 4 + wave_id();
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
@@ -2532,7 +2596,7 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     ly.rp = max((max_blen_x + 2 * R + per - 1) / per, (2 * R) / per + 4 * nch + 1) | 1;
     ly.pic = ly.pp * max_blen_y;
     ly.reg = ly.rp * (max_blen_y + 2 * R);
-    ly.keys = max(2 * ly.reg, ntab * 9 * n);
+    ly.keys = 2 * ly.reg;
     ly.tab = ntab * D * D;
     const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pic + ly.keys + ly.tab);
     if (lds > 64 * 1024) return hipErrorInvalidValue;
@@ -2572,9 +2636,13 @@ hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, i
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
                               int max_blen_y, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    // a phase region spans the block plus two positions per axis
-    const int qrows = max_blen_y + 2, qd = 4 * (((max_blen_x + 3) / 4 + 3) / 4) + 1;  // reads reach 4 * chunks + 1
-    const size_t lds = sizeof(uint32_t) * 4 * ((size_t)PEL_MAXB * SUB_PP + 16 * (size_t)qrows * qd);
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    // a phase region spans the block plus two positions per axis; the SAD
+    // reads 4-dword chunks of a picture row and one dword more of a region row
+    const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
+    const int pp = 4 * nch, prows = max_blen_y;
+    const int qrows = max_blen_y + 2, qd = 4 * nch + 1;
+    const size_t lds = sizeof(uint32_t) * 4 * ((size_t)pp * prows + 16 * (size_t)qrows * qd);
     if (lds > 159 * 1024) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
@@ -2583,7 +2651,8 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(k_subpel_bulk, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, qrows, qd);
+    hipLaunchKernelGGL(k_subpel_bulk, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, pp, prows,
+                       qrows, qd);
     return hipGetLastError();
 }
 

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_lp_frags(const float *__restrict__ w, u
 __global__ __launch_bounds__(256) void k_lp_conv(const uint16_t *__restrict__ in, int npos, int H, int W,
                                                  const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                  int relu, uint16_t *__restrict__ out, float *__restrict__ out_f32) {
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), l = threadIdx.x & 63;
     const int p0 = wv * 16;
     if (p0 >= npos) return;
     const int p = p0 + (l & 15), kg = l >> 4;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_lp_conv(const uint16_t *__restrict__ in
 // Lane l of the A operand holds channel l%16 of 8 consecutive positions.
 __global__ __launch_bounds__(256) void k_lp_wgrad(const uint16_t *__restrict__ in, const uint16_t *__restrict__ g,
                                                   int npos, int H, int W, float *__restrict__ dw) {
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), l = threadIdx.x & 63;
     const int chunk = wv / 9, tap = wv - chunk * 9;
     const int pc = chunk * 256;
     if (pc >= npos) return;
