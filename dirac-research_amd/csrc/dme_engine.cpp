@@ -57,7 +57,8 @@ hipError_t launch_lp_wta(const float *out, const float *lab, int npos, float alp
                          float *dout, float *loss, hipStream_t st);
 hipError_t launch_lp_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
-hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
+                              int max_x1, int max_y1, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
                                 hipStream_t st);
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
@@ -753,8 +754,16 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     hipStream_t st = ctx->L->stream;
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
                          [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
-    DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches,
-                         [&] { return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, st); }));
+    int mx0 = 1, my0 = 1, mx1 = 1, my1 = 1;
+    for (auto *j : js) {
+        mx0 = std::max(mx0, j->g.blen[0][0]);
+        my0 = std::max(my0, j->g.blen[0][1]);
+        mx1 = std::max(mx1, j->g.blen[1][0]);
+        my1 = std::max(my1, j->g.blen[1][1]);
+    }
+    DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches, [&] {
+        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, st);
+    }));
     DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
                          [&] { return launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, st); }));
     DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,

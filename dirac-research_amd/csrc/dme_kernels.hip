@@ -1899,78 +1899,161 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
 //   m1tab[unit * 24 + r*4 + q]      unit candidate q of reference r
 //   m1tab[unit * 24 + 8 + q1*4 + q2] bi-prediction of candidates q1, q2
 //   m0tab[sb * 32 + r*16 + q]       superblock candidate q (block q of the SB)
-// Duplicate vectors are evaluated once.  One wave per unit.
+// Duplicate vectors are evaluated once.  One wave per unit: the unit's
+// picture samples are staged in LDS from the byte plane; for a candidate all
+// samples share one quarter-pel phase (fx, fy), so a lane gathers byte fx of
+// four consecutive qp dwords of phase row fy with v_perm into one packed
+// dword and adds it with v_sad_u8.  The bi-prediction term
+// |(2p - r0 - r1) >> 1| (me_utils.cpp BiBlockDiff) equals |p - ceil((r0 + r1)
+// / 2)|, a byte SAD against the rounded-up average of the two packed
+// candidates, which level 1 keeps in LDS for the pairs.
 // ---------------------------------------------------------------------------
 
+// Packed rounded-up average of four byte pairs: (a | b) - ((a ^ b) >> 1).
+__device__ __forceinline__ uint32_t avg_up_u8(uint32_t a, uint32_t b) {
+    return (a | b) - (((a ^ b) & 0xfefefefeu) >> 1);
+}
+
+// Per-launch LDS layout of k_mode_tables, in dwords per wave: picture rows of
+// pd dwords (prows of them), then for level 1 eight candidate slots of
+// `slot` dwords.
+struct MtLds {
+    int pd, prows, slot, wave;
+};
+
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                     int ntasks, int total) {
-    const int gi = xcd_block() * 4 + wave_id();
+                                                     int ntasks, int total, MtLds ly) {
+    extern __shared__ uint32_t s_mt[];
+    __shared__ int s_pair[4][16];
+    const int wv = wave_id(), lane = threadIdx.x & 63;
+    const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
-    const int lane = threadIdx.x & 63;
     const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
     const int u = gi - tk.offset;
-    const int X2 = J.xnb, W = J.W, H = J.H, UW = 2 * W, UH = 2 * H, step = J.step, nr = J.nrefs;
+    const int X2 = J.xnb, W = J.W, H = J.H, QW = W + 2, step = J.step, nr = J.nrefs;
     const int XU = level ? J.xnb / 2 : J.xsb;
     const int xu = u % XU, yu = u / XU;
     const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], W, H, xu, yu);
     const int nq = level ? 4 : 16, side = level ? 2 : 4;
-    const int16_t *__restrict__ pic = J.pic[0];
     int *out = level ? J.m1tab + (size_t)u * 24 : J.m0tab + (size_t)u * 32;
     if (b.empty()) {
         if (lane < (level ? 24 : 32)) out[lane] = 0;
         return;
     }
-    // candidate vectors and their first occurrence
-    dme_mv cv[2][16];
-    int first[2][16];
-    for (int r = 0; r < nr; ++r)
-        for (int q = 0; q < nq; ++q) {
-            const int bx = xu * side + (q % side), by = yu * side + (q / side);
-            cv[r][q] = J.vec[r][(size_t)by * X2 + bx];
-            first[r][q] = q;
-            for (int p = 0; p < q; ++p)
-                if (first[r][q] == q && cv[r][p].x == cv[r][q].x && cv[r][p].y == cv[r][q].y) first[r][q] = p;
-        }
-    const int n = b.xl * b.yl;
-    for (int r = 0; r < nr; ++r) {
-        for (int q = 0; q < nq; ++q) {
-            if (first[r][q] != q) continue;
-            int s = 0;
-            for (int i = lane; i < n; i += WAVE) {
-                const int dy = i / b.xl, dx = i - dy * b.xl;
-                s += abs(qp_at(J.qp[r], W, H, up_start(b, cv[r][q], step, UW, UH), dy, dx) -
-                         (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+    uint32_t *s_pic = s_mt + wv * ly.wave;
+    uint32_t *s_slot = s_pic + ly.pd * ly.prows;
+    // lane r*16 + q holds candidate q of reference r and the index of its
+    // first occurrence among the reference's candidates
+    const int cr = lane >> 4, cq = lane & 15;
+    const bool cand = lane < 32 && cr < nr && cq < nq;
+    dme_mv cv;
+    cv.x = cv.y = 0;
+    if (cand) cv = J.vec[cr][(size_t)(yu * side + cq / side) * X2 + xu * side + cq % side];
+    int first = cq;
+    for (int p = nq - 1; p >= 0; --p) {
+        const int px = __shfl(cv.x, (lane & 48) + p, WAVE), py = __shfl(cv.y, (lane & 48) + p, WAVE);
+        if (p < cq && px == cv.x && py == cv.y) first = p;
+    }
+    // picture samples: rows of pd dwords, zero beyond xl
+    {
+        const int ppad = (b.xl + 3) & ~3;
+        const B8Rect pr = b8_rect(J.pic8, J.p8, s_pic, ly.pd, b.xp, b.yp, b.xl, b.yl, ppad);
+        if (b8_inside(J.p8, H, b.xp, b.yp, b.yl, ppad)) {
+            const int np = (b.yl + pr.rpp - 1) / pr.rpp;
+            for (int p0 = 0; p0 < np; p0 += 4) {
+                uint32_t lo[4], hi[4];
+                b8_load(pr, J.p8, p0, lo, hi);
+                b8_store(pr, p0, lo, hi);
             }
-            s = wave_sum(s);
-            for (int p = q; p < nq; ++p)
-                if (first[r][p] == q && lane == 0) out[(level ? 4 : 16) * r + p] = s;
+        } else {
+            b8_copy_clamped(pr, J.pic8, J.p8, W, H, b.xp, b.yp);
         }
     }
-    if (level == 1 && nr > 1) {
-        // bi-prediction SADs for the distinct candidate pairs
-        for (int q1 = 0; q1 < 4; ++q1) {
-            if (first[0][q1] != q1) continue;
-            for (int q2 = 0; q2 < 4; ++q2) {
-                if (first[1][q2] != q2) continue;
-                const UpStart s1 = up_start(b, cv[0][q1], step, UW, UH);
-                const UpStart s2 = up_start(b, cv[1][q2], step, UW, UH);
-                int s = 0;
-                for (int i = lane; i < n; i += WAVE) {
-                    const int dy = i / b.xl, dx = i - dy * b.xl;
-                    const int p = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-                    const int16_t diff = (int16_t)((p << 1) - qp_at(J.qp[0], W, H, s1, dy, dx));
-                    const int16_t temp = (int16_t)(((int)diff - qp_at(J.qp[1], W, H, s2, dy, dx)) >> 1);
-                    s += abs((int)temp);
+    // lane (row0 + k * rpp, col) of the unit in the SAD passes
+    const int nd = (b.xl + 3) >> 2, rpp = WAVE / nd;
+    const int row0 = (int)(((float)lane + 0.5f) * (1.0f / (float)nd)), col = lane - row0 * nd;
+    const bool on = row0 < rpp;
+    const int valid = b.xl - 4 * col;
+    const uint32_t mask = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+    const int npass = (b.yl + rpp - 1) / rpp;
+    WAVE_SYNC();
+    int mysum = 0;
+    unsigned long long todo = __ballot(cand && first == cq);
+    while (todo) {
+        const int L = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const int r = L >> 4, q = L & 15;
+        dme_mv m;
+        m.x = __builtin_amdgcn_readlane(cv.x, L);
+        m.y = __builtin_amdgcn_readlane(cv.y, L);
+        const UpStart us = up_start(b, m, step, 2 * W, 2 * H);
+        const int xq = 2 * us.rsx + us.rx, yq = 2 * us.rsy + us.ry;  // quarter-pel start
+        const int X0 = xq >> 2, Y0 = yq >> 2, fx = xq & 3, fy = yq & 3;
+        const uint32_t sel = (uint32_t)fx | ((uint32_t)(fx + 4) << 8) | 0x0c0c0000u;
+        const uint32_t *__restrict__ qp = J.qp[r];
+        int xo[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * col + i, -1, W) + 1;
+        uint32_t *slot = s_slot + (r * 4 + q) * ly.slot;
+        uint32_t acc = 0;
+        for (int p0 = 0; p0 < npass; p0 += 4) {
+            uint32_t d[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int row = row0 + (p0 + k) * rpp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) d[k][i] = 0;
+                if (on && row < b.yl) {
+                    const uint32_t *qr = qp + (size_t)((clampi(Y0 + row, -1, H) + 1) * 4 + fy) * QW;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[k][i] = qr[xo[i]];
                 }
-                s = wave_sum(s);
-                if (lane == 0)
-                    for (int a = q1; a < 4; ++a)
-                        for (int c = q2; c < 4; ++c)
-                            if (first[0][a] == q1 && first[1][c] == q2) out[8 + a * 4 + c] = s;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int row = row0 + (p0 + k) * rpp;
+                if (on && row < b.yl) {
+                    const uint32_t lo = __builtin_amdgcn_perm(d[k][1], d[k][0], sel);
+                    const uint32_t hi = __builtin_amdgcn_perm(d[k][3], d[k][2], sel);
+                    const uint32_t v = __builtin_amdgcn_perm(hi, lo, 0x05040100u) & mask;
+                    acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], v, acc);
+                    if (level) slot[row * ly.pd + col] = v;
+                }
             }
         }
+        const int sum = wave_sum((int)acc);
+        if (lane == L) mysum = sum;
+    }
+    // every candidate takes the sum of its first occurrence
+    const int fsum = __shfl(mysum, (lane & 48) + first, WAVE);
+    if (cand) out[(level ? 4 : 16) * cr + cq] = fsum;
+    if (level == 1 && nr > 1) {
+        // bi-prediction SADs of the distinct pairs, then of every pair
+        WAVE_SYNC();
+        const unsigned long long d0 = __ballot(cand && cr == 0 && first == cq);
+        const unsigned long long d1 = __ballot(cand && cr == 1 && first == cq) >> 16;
+        for (unsigned long long m0 = d0; m0; m0 &= m0 - 1) {
+            const int q1 = __ffsll((long long)m0) - 1;
+            for (unsigned long long m1 = d1; m1; m1 &= m1 - 1) {
+                const int q2 = __ffsll((long long)m1) - 1;
+                const uint32_t *s0 = s_slot + q1 * ly.slot, *s1 = s_slot + (4 + q2) * ly.slot;
+                uint32_t acc = 0;
+                for (int p = 0; p < npass; ++p) {
+                    const int row = row0 + p * rpp;
+                    if (on && row < b.yl) {
+                        const int o = row * ly.pd + col;
+                        acc = __builtin_amdgcn_sad_u8(s_pic[o], avg_up_u8(s0[o], s1[o]), acc);
+                    }
+                }
+                const int sum = wave_sum((int)acc);
+                if (lane == 0) s_pair[wv][q1 * 4 + q2] = sum;
+            }
+        }
+        WAVE_SYNC();
+        const int f0 = __shfl(first, (lane >> 2) & 3, WAVE), f1 = __shfl(first, 16 + (lane & 3), WAVE);
+        if (lane < 16) out[8 + lane] = s_pair[wv][f0 * 4 + f1];
     }
 }
 
@@ -2698,9 +2781,18 @@ hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, in
     return hipGetLastError();
 }
 
-hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
+                              int max_x1, int max_y1, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mode_tables, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (max_x0 < 1 || max_y0 < 1 || max_x1 < 1 || max_y1 < 1 || max_x0 > 256 || max_x1 > 256) return hipErrorInvalidValue;
+    MtLds ly;
+    ly.pd = (std::max(max_x0, max_x1) + 3) / 4;
+    ly.prows = std::max(max_y0, max_y1);
+    ly.slot = ly.pd * max_y1;  // candidate rows share the picture's pitch
+    ly.wave = ly.pd * ly.prows + 8 * ly.slot;
+    const size_t lds = sizeof(uint32_t) * 4 * (size_t)ly.wave;
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mode_tables, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }
 
