@@ -59,6 +59,8 @@ struct JobDesc {
     int32_t DME_G *split;
     float DME_G *sbc;
     float DME_G *lam;
+    float lambda;                                     // the picture's lambda (SetLambdaMap)
+    int DME_G *lwork;                                 // 2 x blocks: transition maps (k_lambda)
     // mode decision temporaries (levels 1 and 0)
     float DME_G *lam1, *lam0;
     dme_mv DME_G *v1[2], *v0[2];

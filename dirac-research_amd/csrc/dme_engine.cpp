@@ -32,6 +32,7 @@ namespace dme {
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
                            int max_blen_y, bool has_guide, hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
@@ -130,6 +131,8 @@ struct JobState {
     DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3];
     DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1];
     DevBuf ptab[2], qtab[2], m1tab, m0tab;
+    DevBuf lwork;  // transition maps of the device lambda stage
+    int bpos = 0;  // position in the current batch (device lambda flags)
     JobDesc desc;
 };
 
@@ -171,6 +174,7 @@ struct dme_ctx {
     std::map<int, std::unique_ptr<dme::Picture>> pics;
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
     dme::DevBuf d_flag;
+    dme::DevBuf d_lflags;  // per batch position: the device lambda maps need the host (k_lambda)
     // Two lanes (stream + buffers); stages queue work on the current lane.
     // Large batches run as two halves on the two lanes, so that one half's
     // host work and latency-bound wavefronts overlap the other half's
@@ -201,12 +205,12 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_BYTES, KK_COUNT
+    KK_BYTES, KK_LAMBDA, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", "k_bytes"};
+    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -448,6 +452,7 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     DME_TRY(J.lam1.ensure(sizeof(float) * B / 4));
     DME_TRY(J.lam0.ensure(sizeof(float) * B / 16));
     DME_TRY(J.ucost.ensure(sizeof(float) * B));
+    DME_TRY(J.lwork.ensure(sizeof(int) * 2 * B));
 
     JobDesc &d = J.desc;
     std::memset(&d, 0, sizeof(d));
@@ -496,6 +501,8 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     d.lam1 = J.lam1.as<float>();
     d.lam0 = J.lam0.as<float>();
     d.ucost = J.ucost.as<float>();
+    d.lwork = J.lwork.as<int>();
+    d.lambda = p.lambda;
     return DME_OK;
 }
 
@@ -614,9 +621,10 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
     return DME_OK;
 }
 
-// Lambda maps: host computation from the level-0 pel vectors, then upload of
-// the level-2 map and the level-1/0 maps of ModeDecider.
-static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *overrides) {
+// Lambda maps: host computation from the level-0 pel vectors (or a given
+// level-2 map), then upload of the level-2 map and the level-1/0 maps of
+// ModeDecider.
+static int stage_lambda_host(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *overrides) {
     // staging layout per job: vectors of each reference, then the level-2,
     // level-1 and level-0 lambda maps
     std::vector<size_t> off(js.size() + 1, 0);
@@ -673,6 +681,57 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
     return DME_OK;
 }
 
+// Jobs of js (device lambda maps) whose flag is set; waits for the stream.
+static int read_lambda_flags(dme_ctx *ctx, const std::vector<JobState *> &js, std::vector<JobState *> &flagged) {
+    int n = 0;
+    for (auto *j : js) n = std::max(n, j->bpos + 1);
+    std::vector<int> f(n, 0);
+    HIP_TRY(hipMemcpyAsync(f.data(), ctx->d_lflags.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    for (auto *j : js)
+        if (f[j->bpos]) flagged.push_back(j);
+    return DME_OK;
+}
+
+// Lambda maps of a batch: given maps through the host, the others by
+// k_lambda.  A job that k_lambda flags (a threshold too close to an integer
+// to decide in double) needs the host's maps: with check_now the flags are
+// read here and those jobs redone on the host; otherwise the caller reads
+// them after the batch (read_lambda_flags) and redoes the flagged jobs.
+static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *overrides, bool check_now,
+                        int &launches) {
+    std::vector<JobState *> host, dev;
+    std::vector<const float *> host_over;
+    for (size_t i = 0; i < js.size(); ++i) {
+        if (overrides && overrides[i]) {
+            host.push_back(js[i]);
+            host_over.push_back(overrides[i]);
+        }
+    }
+    if (!host.empty()) DME_TRY(stage_lambda_host(ctx, host, host_over.data()));
+    TaskTable tt;
+    std::vector<Task> t;
+    for (size_t i = 0; i < js.size(); ++i)
+        if (!(overrides && overrides[i])) {
+            t.push_back(Task{(int)i, 0, 1, js[i]->bpos});
+            dev.push_back(js[i]);
+        }
+    if (dev.empty()) return DME_OK;
+    const TaskTable::Range rt = tt.add(t);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    DME_TRY(timed_launch(ctx, KK_LAMBDA, launches, [&] {
+        return launch_lambda(dj, dtask(ctx, rt), rt.n, ctx->d_lflags.as<int>(), ctx->L->stream);
+    }));
+    for (auto *j : dev) j->stage = 2;
+    if (!check_now) return DME_OK;
+    std::vector<JobState *> redo;
+    DME_TRY(read_lambda_flags(ctx, dev, redo));
+    ctx->stats.host_lambda_jobs = (int32_t)redo.size();
+    if (!redo.empty()) DME_TRY(stage_lambda_host(ctx, redo, nullptr));
+    return DME_OK;
+}
+
 static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
@@ -714,18 +773,9 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
 }
 
 static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    // k_mode_l2 writes every block's zero-initialised MEData entries
+    // (motion.cpp:55-126) before the later kernels of the stage
     DME_TRY(bind_planes(ctx, js, true, launches));
-    for (auto *j : js) {
-        const size_t B = (size_t)j->g.xnb * j->g.ynb;
-        // zero-initialised MEData arrays (motion.cpp:55-126)
-        HIP_TRY(hipMemsetAsync(j->intra.p, 0, sizeof(float) * B, ctx->L->stream));
-        HIP_TRY(hipMemsetAsync(j->bi.p, 0, sizeof(dme_cost) * B, ctx->L->stream));
-        for (int c = 0; c < 3; ++c) HIP_TRY(hipMemsetAsync(j->dc[c].p, 0, sizeof(int16_t) * B, ctx->L->stream));
-        if (j->nrefs < 2) {
-            HIP_TRY(hipMemsetAsync(j->vec[1].p, 0, sizeof(dme_mv) * B, ctx->L->stream));
-            HIP_TRY(hipMemsetAsync(j->pc[1].p, 0, sizeof(dme_cost) * B, ctx->L->stream));
-        }
-    }
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
     std::vector<Task> l2, dc, sh, mt;
@@ -843,6 +893,32 @@ static int end_timing(dme_ctx *ctx, int launches) {
     return DME_OK;
 }
 
+// Batch positions of the jobs and their device lambda flags (streams idle).
+static int lambda_flag_slots(dme_ctx *ctx, std::vector<JobState *> &js) {
+    for (size_t i = 0; i < js.size(); ++i) js[i]->bpos = (int)i;
+    return ctx->d_lflags.ensure(sizeof(int) * js.size());
+}
+
+// After a batch: the jobs whose device lambda maps were flagged run again
+// with the host's maps (the pel search repeats exactly).
+static int redo_flagged(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
+    std::vector<JobState *> dev, redo;
+    for (size_t i = 0; i < js.size(); ++i)
+        if (!(lam_over && lam_over[i])) dev.push_back(js[i]);
+    ctx->stats.host_lambda_jobs = 0;
+    if (dev.empty()) return DME_OK;
+    DME_TRY(read_lambda_flags(ctx, dev, redo));
+    ctx->stats.host_lambda_jobs = (int32_t)redo.size();
+    if (redo.empty()) return DME_OK;
+    int launches = 0;
+    DME_TRY(stage_pel(ctx, redo, launches));
+    DME_TRY(stage_lambda_host(ctx, redo, nullptr));
+    DME_TRY(stage_subpel(ctx, redo, launches));
+    DME_TRY(stage_mode(ctx, redo, launches));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    return DME_OK;
+}
+
 // The whole live sequence for a batch of jobs.  A batch of two or more runs
 // as two halves of about equal work on the two lanes, queued so that the
 // host's lambda stage of one half and the latency-bound wavefront kernels of
@@ -856,13 +932,14 @@ static int end_timing(dme_ctx *ctx, int launches) {
 static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
+    DME_TRY(lambda_flag_slots(ctx, js));
     if (js.size() < 2 || ctx->profiling) {
         DME_TRY(stage_pel(ctx, js, launches));
-        DME_TRY(stage_lambda(ctx, js, lam_over));
+        DME_TRY(stage_lambda(ctx, js, lam_over, false, launches));
         DME_TRY(stage_subpel(ctx, js, launches));
         DME_TRY(stage_mode(ctx, js, launches));
         DME_TRY(end_timing(ctx, launches));
-        return DME_OK;
+        return redo_flagged(ctx, js, lam_over);
     }
     // halves: largest job first to the lighter half, order kept within each
     std::vector<size_t> order(js.size());
@@ -890,7 +967,7 @@ static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *cons
     }
     for (int h = 0; h < 2; ++h) {
         ctx->L = &ctx->lane[h];
-        DME_TRY(stage_lambda(ctx, half[h], over[h].data()));
+        DME_TRY(stage_lambda(ctx, half[h], over[h].data(), false, launches));
         DME_TRY(stage_subpel(ctx, half[h], launches));
         DME_TRY(stage_mode(ctx, half[h], launches));
     }
@@ -898,7 +975,7 @@ static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *cons
     ctx->L = &ctx->lane[0];
     HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join, 0));
     DME_TRY(end_timing(ctx, launches));
-    return DME_OK;
+    return redo_flagged(ctx, js, lam_over);
 }
 
 }  // namespace dme
@@ -1044,7 +1121,9 @@ extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda
     const float *ov[1] = {lambda_map};
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->L->arena_used = 0;
-    DME_TRY(stage_lambda(ctx, js, ov));
+    DME_TRY(lambda_flag_slots(ctx, js));
+    int launches = 0;
+    DME_TRY(stage_lambda(ctx, js, ov, true, launches));
     return export_job(ctx, *js[0], out);
 }
 
@@ -1247,8 +1326,9 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
 static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
+    DME_TRY(lambda_flag_slots(ctx, js));
     DME_TRY(stage_pel_learned(ctx, js, launches));
-    DME_TRY(stage_lambda(ctx, js, nullptr));
+    DME_TRY(stage_lambda(ctx, js, nullptr, true, launches));
     DME_TRY(stage_subpel(ctx, js, launches));
     DME_TRY(stage_mode(ctx, js, launches));
     DME_TRY(end_timing(ctx, launches));

@@ -1282,6 +1282,126 @@ __device__ __forceinline__ float var_up(dme_mv pred, dme_mv m) {
 }
 
 // ---------------------------------------------------------------------------
+// Lambda maps on the device: MEData::SetLambdaMap(num_refs, lambda)
+// (motion.cpp:128-171) from the pel-search vectors, with FindTransitions'
+// vector part (motion.cpp:253-331; its SAD part is overwritten and has no
+// effect), then SetLambdaMap(level, map, wt) (motion.cpp:173-199) for the
+// level-1 and level-0 maps of ModeDecider.  One workgroup per job.
+//
+// FindTransitions marks a block when its local vector variation val (an
+// integer) reaches thr = mean + 3 sd, computed there in long double and
+// rounded to double.  Here the sums are exact (val integer) and thr is
+// computed in double, within 1e-12 of the exact value, as is the
+// reference's; so `val >= thr` is decided by the integer c = ceil(thr)
+// unless thr lies within 1e-9 of an integer.  Such a job is flagged, and the
+// host computes its maps by the reference's own arithmetic instead.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_lambda(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                 int *__restrict__ flags) {
+    __shared__ unsigned long long s_s1;
+    __shared__ double s_q[32];
+    __shared__ int s_c, s_amb;
+    const Task tk = tasks[blockIdx.x];
+    const JobDesc &J = jobs[tk.job];
+    const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+    const int X = J.xnb, Y = J.ynb, N = X * Y;
+    if (tid == 0) s_amb = 0;
+    for (int r = 0; r < J.nrefs; ++r) {
+        const dme_mv *__restrict__ mv = J.vec[r];
+        int *__restrict__ tr = J.lwork + (size_t)r * N;
+        if (tid == 0) s_s1 = 0;
+        __syncthreads();
+        unsigned long long s1 = 0;
+        for (int k = tid; k < N; k += 1024) {
+            const int j = k / X, i = k - j * X;
+            int v = 0;
+            if (j >= 1 && j < Y - 1 && i >= 1 && i < X - 1) {
+                const dme_mv c = mv[k];
+#pragma unroll
+                for (int q = -1; q <= 1; ++q)
+#pragma unroll
+                    for (int p = -1; p <= 1; ++p) {
+                        const dme_mv n = mv[k + q * X + p];
+                        v = max(v, abs(n.x - c.x) + abs(n.y - c.y));
+                    }
+            }
+            tr[k] = v;
+            s1 += (unsigned long long)v;
+        }
+        for (int m = 32; m >= 1; m >>= 1) s1 += __shfl_xor(s1, m, WAVE);
+        if (lane == 0) atomicAdd(&s_s1, s1);
+        __syncthreads();
+        const double mean = (double)s_s1 / (double)N;
+        double q = 0.0;
+        for (int k = tid; k < N; k += 1024) {
+            const int j = k / X, i = k - j * X;
+            if (j >= 1 && j < Y - 1 && i >= 1 && i < X - 1) {
+                const double d = (double)tr[k] - mean;
+                q += d * d;
+            }
+        }
+        for (int m = 32; m >= 1; m >>= 1) q += __shfl_xor(q, m, WAVE);
+        if (lane == 0) s_q[wv] = q;
+        __syncthreads();
+        if (tid == 0) {
+            double Q = 0.0;
+            for (int w = 0; w < 16; ++w) Q += s_q[w];
+            const double thr = mean + 3.0 * sqrt(Q / (double)N);
+            const double fl = floor(thr), margin = 1e-9 * (1.0 + thr);
+            s_c = (int)ceil(thr);
+            if (thr - fl < margin || fl + 1.0 - thr < margin) s_amb = 1;
+        }
+        __syncthreads();
+        const int c = s_c;
+        for (int k = tid; k < N; k += 1024) tr[k] = tr[k] >= c ? 1 : 0;
+        __syncthreads();
+        // a 4 x 4 group of blocks with a transition is marked whole
+        const int GX = X / 4, GY = Y / 4;
+        for (int g = tid; g < GX * GY; g += 1024) {
+            const int gy = g / GX, gx = g - gy * GX;
+            int *base = tr + (size_t)(4 * gy) * X + 4 * gx;
+            int any = 0;
+#pragma unroll
+            for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx) any |= base[yy * X + xx];
+#pragma unroll
+            for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx) base[yy * X + xx] = any;
+        }
+        __syncthreads();
+    }
+    const float lambda = J.lambda;
+    for (int k = tid; k < N; k += 1024) {
+        const int j = k / X, i = k - j * X;
+        const int t1 = J.lwork[k], t2 = J.nrefs > 1 ? J.lwork[N + k] : 0;
+        float m;
+        if (J.nrefs == 1) m = t1 ? 0.0f : lambda;
+        else if (t1 && t2) m = 0.0f;
+        else if (t1 || t2) m = (float)((double)lambda / 4.0);
+        else m = lambda;
+        if (i < 4 || j < 4) m = (float)((double)m / 5.0);
+        J.lam[k] = m;
+    }
+    __syncthreads();
+    for (int level = 1; level >= 0; --level) {
+        const int f = 1 << (2 - level), Xl = X / f, Yl = Y / f;
+        const float wt = (float)(1.0 / (double)J.lf[level]);
+        float *__restrict__ out = level ? J.lam1 : J.lam0;
+        for (int k = tid; k < Xl * Yl; k += 1024) {
+            const int j = k / Xl, i = k - j * Xl;
+            const float *src = J.lam + (size_t)(f * j) * X + f * i;
+            float m = src[0];
+            for (int qq = 0; qq < f; ++qq)
+                for (int pp = 0; pp < f; ++pp) m = fmaxf(src[qq * X + pp], m);
+            out[k] = m * wt;
+        }
+    }
+    if (tid == 0) flags[tk.offset] = s_amb;
+}
+
+// ---------------------------------------------------------------------------
 // Sub-pel refinement, bulk part.  RefineMatchSubp (block_match.cpp:327-467)
 // only ever evaluates positions within 3 quarter pels (1 half pel at half
 // precision) of the pel vector: stage 1 moves at most one half pel per axis,
@@ -1850,7 +1970,18 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             best_sad = bc.SAD;
         }
         if (lane == 0) J.bi[k] = bc;
+    } else if (lane == 0) {
+        // the second reference's arrays stay zero (motion.cpp:55-126)
+        dme_cost z;
+        z.SAD = z.mvcost = z.total = 0.0f;
+        dme_mv zm;
+        zm.x = zm.y = 0;
+        J.bi[k] = z;
+        J.pc[1][k] = z;
+        J.vec[1][k] = zm;
     }
+    float intra = 0.0f;
+    int dc0 = 0;
     if ((double)best_sad > 4.0 * J.blen[2][0] * J.blen[2][1]) {
         int dc = 0;
         float ic = 0.0f;
@@ -1864,19 +1995,20 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             mode = DME_INTRA;
             min_cost = uc;
         }
-        if (lane == 0) {
-            J.intra[k] = ic;
-            J.dc[0][k] = (int16_t)dc;
-        }
+        intra = ic;
+        dc0 = dc;
     }
     if (lane == 0) {
+        // intra cost and DCs are zero unless set here (motion.cpp:55-126)
+        J.intra[k] = intra;
+        J.dc[0][k] = (int16_t)dc0;
+        J.dc[1][k] = 0;
+        J.dc[2][k] = 0;
         J.mode[k] = mode;
         J.ucost[k] = min_cost;
     }
 }
 
-// Prediction of ModeDecider::DoME from the level-2 vectors
-// (me_mode_decn.cpp:308-321).
 // Prediction of ModeDecider::DoME from the level-2 vectors
 // (me_mode_decn.cpp:308-321).
 __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X2, int xb, int yb) {
@@ -2653,6 +2785,12 @@ hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_upconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lambda, dim3(ntasks), dim3(1024), 0, st, jobs, tasks, flags);
     return hipGetLastError();
 }
 

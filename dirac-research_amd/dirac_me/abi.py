@@ -74,7 +74,8 @@ DME_NUM_KERNELS = 24
 
 class dme_stats(C.Structure):
     _fields_ = [("sad_terms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32),
-                ("reserved", C.c_int32 * 7), ("kernel_ms_by", C.c_double * DME_NUM_KERNELS),
+                ("host_lambda_jobs", C.c_int32), ("reserved", C.c_int32 * 6),
+                ("kernel_ms_by", C.c_double * DME_NUM_KERNELS),
                 ("launches_by", C.c_int32 * DME_NUM_KERNELS)]
 
 

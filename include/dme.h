@@ -239,7 +239,9 @@ typedef struct dme_stats {
     double  sad_terms;        /* algorithmic SAD terms of the last call       */
     double  kernel_ms;        /* stream time of the last call (events)        */
     int32_t launches;         /* kernel launches of the last call             */
-    int32_t reserved[7];
+    int32_t host_lambda_jobs; /* jobs of the last call whose lambda maps the
+                                 host computed (device threshold undecided) */
+    int32_t reserved[6];
     double  kernel_ms_by[DME_NUM_KERNELS];  /* summed time per kernel kind   */
     int32_t launches_by[DME_NUM_KERNELS];   /* launches per kernel kind      */
 } dme_stats;

@@ -45,6 +45,7 @@ def test_cif_p_picture(ctx, seed, prec):
     print("case cif seed", seed, "prec", prec, flush=True)
     want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
     _assert_same(want, _gpu(ctx, p, d), 1)
+    assert ctx.stats().host_lambda_jobs == 0  # lambda maps made on the device
 
 
 @pytest.mark.parametrize("lam,fp", [(10.7, DME_FP_MMX), (10.7, DME_FP_C), (3.3, DME_FP_C)])
@@ -96,6 +97,9 @@ def test_static_picture(ctx):
     want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
     got = _gpu(ctx, p, d)
     _assert_same(want, got, 2)
+    # no vector varies: the transition threshold is exactly 0, too close to an
+    # integer for the device test, so the host made the lambda maps
+    assert ctx.stats().host_lambda_jobs == 1
 
 
 def test_stagewise_entry_points(ctx):
