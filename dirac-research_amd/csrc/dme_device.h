@@ -36,7 +36,6 @@ struct JobDesc {
     // planes
     const int16_t DME_G *pic[DME_MAX_DEPTH + 1];      // [0] = Y of the picture
     const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
-    const int16_t DME_G *up[2];                       // 2W x 2H up-converted refs
     const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
     const uint8_t DME_G *pic8, *ref8[2];              // byte planes of picture and refs (k_bytes): sample (0, 0)
     int p8;                                           // their row pitch in bytes

@@ -114,8 +114,8 @@ struct Picture {
     DevBuf own[3];
     DevBuf pyr[DME_MAX_DEPTH + 1];  // levels 1..
     int pyr_levels = 0;             // levels built so far
-    DevBuf up, qp;  // up-converted plane and its quarter-pel planes
-    bool up_ready = false;
+    DevBuf up, qp;  // up-converted plane (built on request) and the quarter-pel planes
+    bool up_ready = false, qp_ready = false;
     DevBuf b8;      // byte plane with margins (k_bytes)
     int bp = 0;     // its row pitch
     bool b8_ready = false;
@@ -301,39 +301,21 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         ranges.push_back(std::make_pair(first, off));
     }
     {
+        // quarter-pel planes of the references, straight from the pictures
         const size_t first = all.size();
         int off = 0;
         for (Picture *P : ups) {
-            if (P->up_ready) continue;
-            DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
+            if (P->qp_ready) continue;
+            const size_t n = (size_t)(P->W + 2) * 4 * (P->H + 2);  // dwords
+            DME_TRY(P->qp.ensure(sizeof(uint32_t) * n));
             PlaneTask t;
             t.in = P->y;
-            t.out = P->up.as<int16_t>();
+            t.out = P->qp.as<int16_t>();
             t.in_w = P->W;
             t.ow = P->W;
             t.oh = P->H;
             t.offset = off;
-            off += ((P->W + 63) / 64) * ((P->H + 3) / 4);
-            all.push_back(t);
-        }
-        ranges.push_back(std::make_pair(first, off));
-    }
-    {
-        const size_t first = all.size();
-        int off = 0;
-        for (Picture *P : ups) {
-            if (P->up_ready) continue;
-            const size_t n = (size_t)(P->W + 2) * 4 * (P->H + 2);  // dwords
-            DME_TRY(P->qp.ensure(sizeof(uint32_t) * n));
-            const size_t npos = (size_t)(P->W + 2) * (P->H + 2);   // one thread per position
-            PlaneTask t;
-            t.in = P->up.as<int16_t>();
-            t.out = P->qp.as<int16_t>();
-            t.in_w = 2 * P->W;
-            t.ow = P->W;
-            t.oh = P->H;
-            t.offset = off;
-            off += (int)((npos + 255) / 256);
+            off += ((P->W + 2 + 63) / 64) * ((P->H + 2 + 3) / 4);  // 64 x 4 positions per tile
             all.push_back(t);
         }
         ranges.push_back(std::make_pair(first, off));
@@ -370,12 +352,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches,
                              [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->L->stream); }));
     }
-    const auto &ru = ranges[ranges.size() - 3], &rq = ranges[ranges.size() - 2], &rb = ranges.back();
-    if (ru.second) {
-        const int n = (int)(rq.first - ru.first);
-        DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches,
-                             [&] { return launch_upconvert(dp + ru.first, n, ru.second, ctx->L->stream); }));
-    }
+    const auto &rq = ranges[ranges.size() - 2], &rb = ranges.back();
     if (rb.second) {
         const int n = (int)(all.size() - rb.first);
         DME_TRY(timed_launch(ctx, KK_BYTES, launches,
@@ -387,7 +364,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
                              [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->L->stream); }));
     }
     for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
-    for (Picture *P : ups) P->up_ready = true;
+    for (Picture *P : ups) P->qp_ready = true;
     for (auto &kv : levels) kv.first->b8_ready = true;
     return DME_OK;
 }
@@ -534,10 +511,7 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
             j->desc.ref[r][0] = R->y;
             for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
             j->desc.ref8[r] = R->b8_origin();
-            if (need_up) {
-                j->desc.up[r] = R->up.as<int16_t>();
-                j->desc.qp[r] = R->qp.as<uint32_t>();
-            }
+            if (need_up) j->desc.qp[r] = R->qp.as<uint32_t>();
         }
     }
     return DME_OK;
@@ -1189,7 +1163,26 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
         src = P->pyr[level].p;
         n = (size_t)(P->W >> level) * (P->H >> level);
     } else if (kind == 1) {
-        if (!P->up_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
+        // the search reads the quarter-pel planes only; the up-converted
+        // plane itself is built here, on request
+        if (!P->qp_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
+        if (!P->up_ready) {
+            HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+            DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
+            PlaneTask t;
+            t.in = P->y;
+            t.out = P->up.as<int16_t>();
+            t.in_w = P->W;
+            t.ow = P->W;
+            t.oh = P->H;
+            t.offset = 0;
+            DME_TRY(upload(ctx, ctx->L->d_planes, &t, sizeof(t)));
+            HIP_TRY(launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), 1, ((P->W + 63) / 64) * ((P->H + 3) / 4),
+                                     ctx->L->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+            ctx->L->arena_used = 0;
+            P->up_ready = true;
+        }
         src = P->up.p;
         n = 4 * (size_t)P->W * P->H;
     } else {
@@ -1214,6 +1207,7 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
         if (pic_num < 0 || kv.first == pic_num) {
             kv.second->pyr_levels = 0;
             kv.second->up_ready = false;
+            kv.second->qp_ready = false;
             kv.second->b8_ready = false;
         }
     return DME_OK;

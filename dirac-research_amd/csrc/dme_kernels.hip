@@ -388,7 +388,21 @@ __global__ __launch_bounds__(256) void k_check_range(const int16_t *__restrict__
 // (both clamped corners coincide there), so reads clamp X and Y to that range.
 // A half-pel vector m reads the even phases: position 2m in quarter pels.
 // ---------------------------------------------------------------------------
+// Built straight from the picture: a workgroup takes 64 x 4 positions (one
+// thread each); the up-plane samples they use (rows 2Y .. 2Y + 2, columns
+// 2X .. 2X + 2, clamped) are made in LDS first, as K2 makes them: the
+// clamped source tile, its vertical taps, then the 9 x 129 up samples.
+#define QT_X 64
+#define QT_Y 4
+#define QT_SR (2 * QT_Y + 5)   // source rows Y0 - 4 .. Y0 + 8
+#define QT_SC (QT_X + 9)       // source columns X0 - 4 .. X0 + 68
+#define QT_VR (QT_Y + 2)       // vertical-tap rows Y0 - 1 .. Y0 + 4
+#define QT_UR (2 * QT_Y + 1)   // up rows 2Y0 .. 2Y0 + 8
+#define QT_UC (2 * QT_X + 1)   // up columns 2X0 .. 2X0 + 128
 __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ pt, int npt, int total) {
+    __shared__ int16_t s_src[QT_SR][QT_SC];
+    __shared__ int16_t s_v[QT_VR][QT_SC];
+    __shared__ int16_t s_u[QT_UR][QT_UC + 1];
     const int tile = blockIdx.x;
     if (tile >= total) return;
     int lo = 0, hi = npt - 1;
@@ -398,19 +412,62 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
         else hi = mid - 1;
     }
     const PlaneTask p = pt[lo];
-    const int W = p.ow, H = p.oh, UW = 2 * W, UH = 2 * H, QW = W + 2;
-    // one thread per position (X, Y): its 16 phases use up-plane rows and
-    // columns 2Y .. 2Y + 2 and 2X .. 2X + 2
-    const int i = (tile - p.offset) * 256 + threadIdx.x;
-    if (i >= QW * (H + 2)) return;
-    const int yy = i / QW, X = i - yy * QW - 1, Y = yy - 1;
-    const int16_t *__restrict__ up = p.in;
+    const int W = p.ow, H = p.oh, QW = W + 2, QH = H + 2;
+    const int tx = (QW + QT_X - 1) / QT_X, local = tile - p.offset;
+    const int X0 = (local % tx) * QT_X - 1, Y0 = (local / tx) * QT_Y - 1;  // first position of the tile
+    const int16_t *__restrict__ in = p.in;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < QT_SR * QT_SC; i += 256) {
+        const int r = i / QT_SC, c = i - r * QT_SC;
+        s_src[r][c] = in[(size_t)clampi(Y0 - 4 + r, 0, H - 1) * W + clampi(X0 - 4 + c, 0, W - 1)];
+    }
+    __syncthreads();
+    // vertical taps of source row Y0 - 1 + r: rows -3 .. +4 around it
+    for (int i = tid; i < QT_VR * QT_SC; i += 256) {
+        const int r = i / QT_SC, c = i - r * QT_SC;
+        int s8[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s8[t] = s_src[r + t][c];
+        s_v[r][c] = (int16_t)up_tap8(s8);
+    }
+    __syncthreads();
+    // up sample (clamp(2Y0 + a), clamp(2X0 + b)): source sample (y, x) = that / 2
+    for (int i = tid; i < QT_UR * QT_UC; i += 256) {
+        const int a = i / QT_UC, b = i - a * QT_UC;
+        const int cuy = clampi(2 * Y0 + a, 0, 2 * H - 1), cux = clampi(2 * X0 + b, 0, 2 * W - 1);
+        const int y = cuy >> 1, x = cux >> 1;
+        const int sr = y - (Y0 - 4), sc = x - (X0 - 4), vr = y - (Y0 - 1);
+        int v;
+        if (!(cuy & 1)) {
+            if (!(cux & 1)) {
+                v = s_src[sr][sc];
+            } else {
+                int s8[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s8[t] = s_src[sr][sc - 3 + t];
+                v = up_tap8(s8);
+            }
+        } else {
+            if (!(cux & 1)) {
+                v = s_v[vr][sc];
+            } else {
+                int s8[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s8[t] = s_v[vr][sc - 3 + t];
+                v = up_tap8(s8);
+            }
+        }
+        s_u[a][b] = (int16_t)v;
+    }
+    __syncthreads();
+    const int px = tid & (QT_X - 1), py = tid / QT_X;
+    const int X = X0 + px, Y = Y0 + py;
+    if (X >= QW - 1 || Y >= QH - 1) return;  // positions X <= W, Y <= H
     int u[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            u[a][c] = up[(size_t)clampi(2 * Y + a, 0, UH - 1) * UW + clampi(2 * X + c, 0, UW - 1)];
+        for (int c = 0; c < 3; ++c) u[a][c] = s_u[2 * py + a][2 * px + c];
     uint32_t *__restrict__ out = reinterpret_cast<uint32_t *>(p.out);
 #pragma unroll
     for (int fy = 0; fy < 4; ++fy) {
@@ -423,7 +480,7 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
                            (2 - rx) * ry * u[oy + 1][ox] + rx * ry * u[oy + 1][ox + 1] + 2) >> 2;
             d |= (uint32_t)(v + 128) << (8 * fx);
         }
-        out[(size_t)(yy * 4 + fy) * QW + X + 1] = d;
+        out[(size_t)((Y + 1) * 4 + fy) * QW + X + 1] = d;
     }
 }
 
