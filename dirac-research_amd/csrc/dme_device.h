@@ -41,6 +41,7 @@ struct JobDesc {
     int p8;                                           // their row pitch in bytes
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
     int lp_off;                                       // first position of the picture in the proposal tensors
+    int lp_slot;                                      // its pair of refinement counters (k_lp_refine)
     // pel search, per reference and level ([0] aliases the MEData arrays)
     dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];
     dme_cost DME_G *lpc[2][DME_MAX_DEPTH + 1];

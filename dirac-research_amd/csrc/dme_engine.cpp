@@ -43,8 +43,8 @@ hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t 
 hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st);
 hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks, int total, uint16_t *feat, float *lab,
                               hipStream_t st);
-hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4, int *stats,
-                            hipStream_t st);
+hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4,
+                            unsigned long long *stats, int max_blen_x, int max_blen_y, hipStream_t st);
 hipError_t launch_lp_pel_costs(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_lp_frags(const float *w, uint16_t *fwd, uint16_t *bwd, hipStream_t st);
 hipError_t launch_lp_conv(const uint16_t *in, int npos, int H, int W, const uint16_t *frag, const float *bias, int relu,
@@ -187,7 +187,11 @@ struct dme_ctx {
     // learned proposal: weights (f32 master), BF16 fragments per layer,
     // activations of a batch, per-job counters
     bool lp_ready = false;
-    dme::DevBuf lp_w, lp_frag, lp_act[4], lp_out, lp_stats;
+    dme::DevBuf lp_w, lp_frag, lp_act[4], lp_out;
+    // refinement counters, a pair per batch position (k_lp_refine), and the
+    // totals of counters retired when the array grew
+    dme::DevBuf lp_stats;
+    size_t lp_slots = 0;
     long long lp_blocks = 0, lp_fallbacks = 0;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1236,33 +1240,64 @@ namespace dme {
 // Pel stage with learned proposals: the plain search down to level 2, then
 // the network over every picture's level-0 block grid (pictures with equal
 // grids form one NHWC tensor), exact refinement and the pel costs.
-static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
-    if (!ctx->lp_ready) return set_error(DME_ERR_STATE, "dme_lp_set_weights was not called");
-    for (auto *j : js)
-        if (j->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
-    DME_TRY(stage_pel(ctx, js, launches, 2));
-    // group pictures by block grid; offsets in the proposal tensors
+// Positions of a batch in the proposal tensors, from `base`, grouped by
+// block grid so that every group is one run of a convolution launch.
+struct LpGroup {
+    int off, n, W, H;
+};
+static int lp_layout(std::vector<JobState *> &js, int base, std::vector<LpGroup> &groups) {
     std::vector<size_t> order(js.size());
     for (size_t i = 0; i < js.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
         return std::make_pair(js[a]->g.xnb, js[a]->g.ynb) < std::make_pair(js[b]->g.xnb, js[b]->g.ynb);
     });
-    struct Group { int off, n, W, H; };
-    std::vector<Group> groups;
-    int P = 0;
+    groups.clear();
+    int P = base;
     for (size_t o = 0; o < order.size(); ++o) {
         JobState *j = js[order[o]];
         if (groups.empty() || groups.back().W != j->g.xnb || groups.back().H != j->g.ynb)
-            groups.push_back(Group{P, 0, j->g.xnb, j->g.ynb});
+            groups.push_back(LpGroup{P, 0, j->g.xnb, j->g.ynb});
         groups.back().n++;
         j->desc.lp_off = P;
+        j->desc.lp_slot = j->bpos;
         P += j->g.xnb * j->g.ynb;
     }
+    return P;
+}
+
+// Proposal tensors for P positions and counters for nslots jobs (streams
+// idle).  Counters of a smaller array are added to the totals first.
+static int lp_reserve(dme_ctx *ctx, int P, size_t nslots) {
     for (int a = 0; a < 4; ++a) DME_TRY(ctx->lp_act[a].ensure(sizeof(uint16_t) * 32 * (size_t)P));
     DME_TRY(ctx->lp_out.ensure(sizeof(float) * 32 * (size_t)P));
-    DME_TRY(ctx->lp_stats.ensure(sizeof(int) * 2 * js.size()));
+    if (nslots > ctx->lp_slots) {
+        if (ctx->lp_slots) {
+            std::vector<unsigned long long> h(2 * ctx->lp_slots);
+            HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < ctx->lp_slots; ++i) {
+                ctx->lp_blocks += (long long)h[2 * i];
+                ctx->lp_fallbacks += (long long)h[2 * i + 1];
+            }
+        }
+        const size_t n = std::max(nslots, (size_t)64);
+        DME_TRY(ctx->lp_stats.ensure(sizeof(unsigned long long) * 2 * n));
+        HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(unsigned long long) * 2 * n, ctx->L->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        ctx->lp_slots = n;
+    }
+    return DME_OK;
+}
+
+// Learned pel stage of a batch laid out by lp_layout: the plain search of
+// the levels down to 2, the network on the level-0 block grids, then the
+// exact refinement and the pel costs.  Queues work only.
+static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, const std::vector<LpGroup> &groups,
+                             int &launches) {
+    if (!ctx->lp_ready) return set_error(DME_ERR_STATE, "dme_lp_set_weights was not called");
+    for (auto *j : js)
+        if (j->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
+    DME_TRY(stage_pel(ctx, js, launches, 2));
     DME_TRY(upload_jobs(ctx, js));
-    HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(int) * 2 * js.size(), ctx->L->stream));
     TaskTable tt;
     std::vector<Task> fb, rb;
     int offf = 0, offr = 0;
@@ -1284,7 +1319,7 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
     const float *w = ctx->lp_w.as<float>();
     DME_TRY(timed_launch(ctx, KK_LP_NET, launches, [&]() -> hipError_t {
         hipError_t e = launch_lp_features(dj, dtask(ctx, feat), feat.n, feat.total, act[0], nullptr, ctx->L->stream);
-        for (const Group &g : groups) {
+        for (const LpGroup &g : groups) {
             const size_t o = (size_t)g.off * 32;
             const int npos = g.n * g.W * g.H;
             for (int layer = 0; layer < 4 && e == hipSuccess; ++layer) {
@@ -1300,32 +1335,71 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, int &lau
         }
         return e;
     }));
+    int rbx = 1, rby = 1;
+    for (auto *j : js) {
+        rbx = std::max(rbx, j->g.blen[2][0]);
+        rby = std::max(rby, j->g.blen[2][1]);
+    }
     DME_TRY(timed_launch(ctx, KK_LP_REFINE, launches, [&]() -> hipError_t {
         hipError_t e = launch_lp_refine(dj, dtask(ctx, refine), refine.n, refine.total, ctx->lp_out.as<float>(),
-                                        ctx->lp_stats.as<int>(), ctx->L->stream);
+                                        ctx->lp_stats.as<unsigned long long>(), rbx, rby, ctx->L->stream);
         if (e == hipSuccess) e = launch_lp_pel_costs(dj, dtask(ctx, refine), refine.n, refine.total, ctx->L->stream);
         return e;
     }));
-    std::vector<int> st(2 * js.size());
-    HIP_TRY(hipMemcpyAsync(st.data(), ctx->lp_stats.p, sizeof(int) * st.size(), hipMemcpyDeviceToHost, ctx->L->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
-    for (size_t i = 0; i < js.size(); ++i) {
-        ctx->lp_blocks += st[2 * i];
-        ctx->lp_fallbacks += st[2 * i + 1];
-    }
     for (auto *j : js) j->stage = 1;
     return DME_OK;
 }
 
+// The live sequence with learned proposals, in two lanes as run_jobs does.
+// Jobs whose device lambda maps were flagged run again (learned pel stage,
+// host maps).
 static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
     DME_TRY(lambda_flag_slots(ctx, js));
-    DME_TRY(stage_pel_learned(ctx, js, launches));
-    DME_TRY(stage_lambda(ctx, js, nullptr, true, launches));
-    DME_TRY(stage_subpel(ctx, js, launches));
-    DME_TRY(stage_mode(ctx, js, launches));
+    std::vector<JobState *> half[2];
+    if (js.size() < 2 || ctx->profiling) {
+        half[0] = js;
+    } else {
+        for (size_t i = 0; i < js.size(); ++i) half[i & 1].push_back(js[i]);
+    }
+    std::vector<LpGroup> groups[2];
+    int P = 0;
+    for (int h = 0; h < 2; ++h) P = lp_layout(half[h], P, groups[h]);
+    DME_TRY(lp_reserve(ctx, P, js.size()));
+    if (half[1].empty()) {
+        DME_TRY(stage_pel_learned(ctx, js, groups[0], launches));
+        DME_TRY(stage_lambda(ctx, js, nullptr, false, launches));
+        DME_TRY(stage_subpel(ctx, js, launches));
+        DME_TRY(stage_mode(ctx, js, launches));
+    } else {
+        DME_TRY(bind_planes(ctx, js, true, launches));
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->lane[1].stream, ctx->ev_fork, 0));
+        for (int h = 0; h < 2; ++h) {
+            ctx->L = &ctx->lane[h];
+            DME_TRY(stage_pel_learned(ctx, half[h], groups[h], launches));
+            DME_TRY(stage_lambda(ctx, half[h], nullptr, false, launches));
+            DME_TRY(stage_subpel(ctx, half[h], launches));
+            DME_TRY(stage_mode(ctx, half[h], launches));
+        }
+        HIP_TRY(hipEventRecord(ctx->ev_join, ctx->lane[1].stream));
+        ctx->L = &ctx->lane[0];
+        HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join, 0));
+    }
     DME_TRY(end_timing(ctx, launches));
+    std::vector<JobState *> redo;
+    DME_TRY(read_lambda_flags(ctx, js, redo));
+    ctx->stats.host_lambda_jobs = (int32_t)redo.size();
+    if (!redo.empty()) {
+        std::vector<LpGroup> g;
+        lp_layout(redo, 0, g);
+        DME_TRY(stage_pel_learned(ctx, redo, g, launches));
+        DME_TRY(stage_lambda_host(ctx, redo, nullptr));
+        DME_TRY(stage_subpel(ctx, redo, launches));
+        DME_TRY(stage_mode(ctx, redo, launches));
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    }
     return DME_OK;
 }
 
@@ -1380,8 +1454,18 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
 
 extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks) {
     CTX_CHECK(ctx);
-    if (blocks) *blocks = ctx->lp_blocks;
-    if (fallbacks) *fallbacks = ctx->lp_fallbacks;
+    long long b = ctx->lp_blocks, f = ctx->lp_fallbacks;
+    if (ctx->lp_slots) {
+        for (auto &ln : ctx->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
+        std::vector<unsigned long long> h(2 * ctx->lp_slots);
+        HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < ctx->lp_slots; ++i) {
+            b += (long long)h[2 * i];
+            f += (long long)h[2 * i + 1];
+        }
+    }
+    if (blocks) *blocks = b;
+    if (fallbacks) *fallbacks = f;
     return DME_OK;
 }
 

@@ -2541,8 +2541,6 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
 // through the unchanged sub-pel and mode-decision stages.
 // ---------------------------------------------------------------------------
 #define LP_SCALE_PEL 8.0f  // pel per unit of the network's offset outputs
-#define LP_RP 11           // packed region row, dwords (regions up to 40 bytes wide)
-#define LP_RROWS 44        // region rows
 
 __device__ __forceinline__ uint16_t f2bf_rne(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -2616,24 +2614,37 @@ __global__ __launch_bounds__(256) void k_lp_features(const JobDesc *__restrict__
     }
 }
 
-// Exact refinement of the proposals: one wave per (block, reference).
-// Candidates: the union of the +-2 windows around the four hypotheses.  When
-// they fit a 12 x 12 box (the usual case: the hypotheses cluster), one
-// region covering the box is staged and each position of the union is
-// evaluated once, keyed by its raster index in the box; otherwise each
-// window is staged and evaluated on its own, keyed by hypothesis and
-// AddNewVlist order.  If the best SAD exceeds 1.2 x the predicted SAD, the
-// +-4 window around the level-2 guide and the +-2 window around zero join
-// in.  Keys SAD * 256 + index keep the first minimum.  Writes the level-0
-// vector and SAD; stats[job * 2 + {0, 1}] count blocks and fallbacks.
+// Exact refinement of the proposals: one wave per (block, reference).  The
+// four hypotheses are clamped to within LP_BOX - 2 pel of 4 x the level-2
+// vector g; the picture block and the reference box g +- LP_BOX are staged
+// from the byte planes with one round of loads.  Candidates: the union of
+// the +-2 windows around the hypotheses, each position once.  If the best SAD exceeds 1.2 x the predicted SAD, the
+// +-4 window around g and the +-2 window around zero join in (zero's window
+// from a region of its own when it leaves the box) and replace the best only
+// with a smaller SAD.  Keys SAD * 1024 + raster index in the box (625 + k
+// for zero's window outside it) keep the first minimum.  Writes the level-0
+// vector and SAD; stats[lp_slot * 2 + {0, 1}] count blocks and fallbacks.
+#define LP_BOX 12
+#define LP_BD (2 * LP_BOX + 1)
+
+// LDS of one wave of k_lp_refine (dwords): picture rows of pp, the box rows
+// of rp (brows of them), zero's window rows of zp (zrows).
+struct LpLds {
+    int pp, prows, rp, brows, zp, zrows;
+    __device__ int wave() const { return pp * prows + rp * brows + zp * zrows + 100; }
+};
+
 __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                    int ntasks, int total, const float *__restrict__ out4,
-                                                   int *__restrict__ stats) {
-    __shared__ uint32_t s_pic[4][PEL_MAXB * SUB_PP];
-    __shared__ uint32_t s_reg[4][4][LP_RROWS * LP_RP];
+                                                   unsigned long long *__restrict__ stats, LpLds ly) {
+    extern __shared__ uint32_t s_lp[];
     const int wv = wave_id(), lane = threadIdx.x & 63;
     const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;
+    uint32_t *s_pic = s_lp + wv * ly.wave();
+    uint32_t *s_box = s_pic + ly.pp * ly.prows;
+    uint32_t *s_zero = s_box + ly.rp * ly.brows;
+    int *s_list = reinterpret_cast<int *>(s_zero + ly.zp * ly.zrows);  // up to 100 union positions
     const Task tk = tasks[find_task(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
@@ -2647,109 +2658,128 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
         }
         return;
     }
-    const int16_t *__restrict__ pic = J.pic[0];
-    const int16_t *__restrict__ ref = J.ref[r][0];
-    const int W = J.W, H = J.H;
+    const int W = J.W, H = J.H, bp = J.p8;
     dme_mv g;
     float s2;
     lp_guide(J, r, x, y, g, s2);
     g.x *= 4;
     g.y *= 4;
     const float *o = out4 + ((size_t)J.lp_off + k) * 32;
-    dme_mv hyp[4];
+    dme_mv hyp[4];  // offsets from g
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-        hyp[h].x = g.x + (int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f));
-        hyp[h].y = g.y + (int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f));
+        hyp[h].x = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f)), 2 - LP_BOX,
+                          LP_BOX - 2);
+        hyp[h].y = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f)), 2 - LP_BOX,
+                          LP_BOX - 2);
     }
     const float pred_sad = expm1f(fminf(o[16 + r], 20.0f));
-    wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic[wv]), 4 * SUB_PP, pic, W, H, b.xp, b.yp, b.xl, b.yl,
-                  (b.xl + 15) & ~15);
-    const int RW = b.xl + 4, RH = b.yl + 4;
-    int bx0 = hyp[0].x, bx1 = hyp[0].x, by0 = hyp[0].y, by1 = hyp[0].y;
+    // picture block and box in one round of loads
+    {
+        const uint8_t *const base[3] = {J.pic8, J.ref8[r], J.ref8[r]};
+        const int x0[3] = {b.xp, b.xp + g.x - LP_BOX, 0}, y0[3] = {b.yp, b.yp + g.y - LP_BOX, 0};
+        const int ppad = (b.xl + 15) & ~15, RW = b.xl + 2 * LP_BOX, RH = b.yl + 2 * LP_BOX;
+        B8Rect rc[3];
+        bool fast[3];
+        rc[0] = b8_rect(base[0], bp, s_pic, ly.pp, x0[0], y0[0], b.xl, b.yl, ppad);
+        rc[1] = b8_rect(base[1], bp, s_box, ly.rp, x0[1], y0[1], RW, RH, RW);
+        rc[2] = rc[1];
+        fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl, ppad);
+        fast[1] = b8_inside(bp, H, x0[1], y0[1], RH, RW);
+        fast[2] = false;
+        b8_stage(rc, fast, 2, base, bp, W, H, x0, y0);
+    }
+    WAVE_SYNC();
+    const bool whole = (b.xl & 15) == 0;
+    // union of the hypothesis windows: the positions of its bounding box that
+    // lie in a window, compacted into a list (box raster indices), then one
+    // SAD per listed position
+    int ux0 = hyp[0].x, ux1 = hyp[0].x, uy0 = hyp[0].y, uy1 = hyp[0].y;
 #pragma unroll
     for (int h = 1; h < 4; ++h) {
-        bx0 = min(bx0, hyp[h].x);
-        bx1 = max(bx1, hyp[h].x);
-        by0 = min(by0, hyp[h].y);
-        by1 = max(by1, hyp[h].y);
+        ux0 = min(ux0, hyp[h].x);
+        ux1 = max(ux1, hyp[h].x);
+        uy0 = min(uy0, hyp[h].y);
+        uy1 = max(uy1, hyp[h].y);
     }
-    bx0 -= 2;
-    by0 -= 2;
-    const int BW = bx1 + 2 - bx0 + 1, BH = by1 + 2 - by0 + 1;
-    int best_sad;
+    ux0 -= 2;
+    uy0 -= 2;
+    const int UW = ux1 + 3 - ux0, UN = UW * (uy1 + 3 - uy0);
+    int cnt = 0;
+    for (int j0 = 0; j0 < UN; j0 += WAVE) {
+        const int j = j0 + lane, py = uy0 + j / UW, px = ux0 + j % UW;
+        bool in = false;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) in = in || (abs(px - hyp[h].x) <= 2 && abs(py - hyp[h].y) <= 2);
+        in = in && j < UN;
+        const unsigned long long m = __ballot(in);
+        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (in) s_list[at] = (py + LP_BOX) * LP_BD + px + LP_BOX;
+        cnt += __popcll(m);
+    }
+    WAVE_SYNC();
+    int key = INT_MAX;
+    for (int i = lane; i < cnt; i += WAVE) {
+        const int idx = s_list[i], px = idx % LP_BD, py = idx / LP_BD;
+        const int s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl)
+                            : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl);
+        key = min(key, s * 1024 + idx);
+    }
+    key = wave_min_i(key);
+    int best_sad = key >> 10;
     dme_mv best;
-    if (BW <= 12 && BH <= 12) {  // uniform
-        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * LP_RP, ref, W, H, b.xp + bx0, b.yp + by0,
-                      b.xl + BW - 1, b.yl + BH - 1, b.xl + BW - 1);
-        WAVE_SYNC();
-        int key = INT_MAX;
-        for (int i = lane; i < BW * BH; i += WAVE) {
-            const int ox = i % BW, oy = i / BW;
-            bool in = false;
-#pragma unroll
-            for (int h = 0; h < 4; ++h)
-                in = in || (abs(bx0 + ox - hyp[h].x) <= 2 && abs(by0 + oy - hyp[h].y) <= 2);
-            if (in)
-                key = min(key, packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][0], LP_RP, ox, oy, b.xl, b.yl) * 256 + i);
-        }
-        key = wave_min_i(key);
-        best_sad = key >> 8;
-        best.x = bx0 + (key & 255) % BW;
-        best.y = by0 + (key & 255) / BW;
-    } else {
-#pragma unroll
-        for (int h = 0; h < 4; ++h)
-            wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][h]), 4 * LP_RP, ref, W, H, b.xp + hyp[h].x - 2,
-                          b.yp + hyp[h].y - 2, RW, RH, RW);
-        WAVE_SYNC();
-        int key = INT_MAX;
-        for (int i = lane; i < 100; i += WAVE) {
-            const int h = i / 25;
-            const dme_mv w = win_off(i - 25 * h, 2);
-            const int s = packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][h], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
-            key = min(key, s * 256 + i);
-        }
-        key = wave_min_i(key);
-        best_sad = key >> 8;
-        best = win_off((key & 255) % 25, 2);
-        const int h = (key & 255) / 25;
-        best.x += hyp[h].x;
-        best.y += hyp[h].y;
-    }
+    best.x = g.x + (key & 1023) % LP_BD - LP_BOX;
+    best.y = g.y + (key & 1023) / LP_BD - LP_BOX;
     const bool fallback = (float)best_sad > 1.2f * pred_sad;
     if (fallback) {  // uniform
-        WAVE_SYNC();
-        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][0]), 4 * LP_RP, ref, W, H, b.xp + g.x - 4, b.yp + g.y - 4,
-                      b.xl + 8, b.yl + 8, b.xl + 8);
-        wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg[wv][1]), 4 * LP_RP, ref, W, H, b.xp - 2, b.yp - 2, RW, RH, RW);
-        WAVE_SYNC();
+        // zero's window: in the box when |g| <= LP_BOX - 2, else staged apart
+        const bool zin = abs(g.x) <= LP_BOX - 2 && abs(g.y) <= LP_BOX - 2;
+        if (!zin) {
+            const uint8_t *const base[3] = {J.ref8[r], J.ref8[r], J.ref8[r]};
+            const int x0[3] = {b.xp - 2, 0, 0}, y0[3] = {b.yp - 2, 0, 0};
+            B8Rect rc[3];
+            bool fast[3];
+            rc[0] = b8_rect(base[0], bp, s_zero, ly.zp, x0[0], y0[0], b.xl + 4, b.yl + 4, b.xl + 4);
+            rc[1] = rc[2] = rc[0];
+            fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl + 4, b.xl + 4);
+            fast[1] = fast[2] = false;
+            b8_stage(rc, fast, 1, base, bp, W, H, x0, y0);
+            WAVE_SYNC();
+        }
         int key2 = INT_MAX;
         for (int i = lane; i < 106; i += WAVE) {
             const bool gw = i < 81;
             const dme_mv w = gw ? win_off(i, 4) : win_off(i - 81, 2);
-            const int s = gw ? packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][0], LP_RP, w.x + 4, w.y + 4, b.xl, b.yl)
-                             : packed_sad_u8(s_pic[wv], SUB_PP, s_reg[wv][1], LP_RP, w.x + 2, w.y + 2, b.xl, b.yl);
-            key2 = min(key2, s * 256 + 100 + i);
+            int s, idx;
+            if (gw || zin) {
+                const int px = gw ? w.x : w.x - g.x, py = gw ? w.y : w.y - g.y;  // offsets from g
+                s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl)
+                          : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl);
+                idx = (py + LP_BOX) * LP_BD + px + LP_BOX;
+            } else {
+                s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl)
+                          : packed_sad_u8<true>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl);
+                idx = LP_BD * LP_BD + (i - 81);
+            }
+            key2 = min(key2, s * 1024 + idx);
         }
         key2 = wave_min_i(key2);
-        if ((key2 >> 8) < best_sad) {
-            const int i = (key2 & 255) - 100;
-            best_sad = key2 >> 8;
-            if (i < 81) {
-                best = win_off(i, 4);
-                best.x += g.x;
-                best.y += g.y;
+        if ((key2 >> 10) < best_sad) {
+            const int idx = key2 & 1023;
+            best_sad = key2 >> 10;
+            if (idx < LP_BD * LP_BD) {
+                best.x = g.x + idx % LP_BD - LP_BOX;
+                best.y = g.y + idx / LP_BD - LP_BOX;
             } else {
-                best = win_off(i - 81, 2);
+                best = win_off(idx - LP_BD * LP_BD, 2);
             }
         }
     }
     if (lane == 0) {
         V[k] = best;
         C[k].SAD = (float)best_sad;
-        atomicAdd(&stats[2 * tk.job], 1);
-        if (fallback) atomicAdd(&stats[2 * tk.job + 1], 1);
+        atomicAdd(&stats[2 * J.lp_slot], 1ull);
+        if (fallback) atomicAdd(&stats[2 * J.lp_slot + 1], 1ull);
     }
 }
 
@@ -3015,10 +3045,24 @@ hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks
     return hipGetLastError();
 }
 
-hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4, int *stats,
-                            hipStream_t st) {
+hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4,
+                            unsigned long long *stats, int max_blen_x, int max_blen_y, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lp_refine, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total, out4, stats);
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    // the SAD reads 4-dword chunks of a picture row and, from dword offset
+    // (2 * radius) / 4, one dword more of a region row
+    const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
+    LpLds ly;
+    ly.pp = 4 * nch;
+    ly.prows = max_blen_y;
+    ly.rp = max((max_blen_x + 2 * LP_BOX + 3) / 4, (2 * LP_BOX) / 4 + 4 * nch + 1) | 1;
+    ly.brows = max_blen_y + 2 * LP_BOX;
+    ly.zp = max((max_blen_x + 4 + 3) / 4, 1 + 4 * nch + 1) | 1;
+    ly.zrows = max_blen_y + 4;
+    const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 100);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lp_refine, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, out4, stats,
+                       ly);
     return hipGetLastError();
 }
 
