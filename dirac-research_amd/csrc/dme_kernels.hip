@@ -2105,9 +2105,9 @@ __device__ __forceinline__ uint32_t avg_up_u8(uint32_t a, uint32_t b) {
 
 // Per-launch LDS layout of k_mode_tables, in dwords per wave: picture rows of
 // pd dwords (prows of them), then for level 1 eight candidate slots of
-// `slot` dwords.
+// `slot` dwords, rows of spd dwords.
 struct MtLds {
-    int pd, prows, slot, wave;
+    int pd, prows, spd, slot, wave;
 };
 
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
@@ -2208,7 +2208,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
                     const uint32_t hi = __builtin_amdgcn_perm(d[k][3], d[k][2], sel);
                     const uint32_t v = __builtin_amdgcn_perm(hi, lo, 0x05040100u) & mask;
                     acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], v, acc);
-                    if (level) slot[row * ly.pd + col] = v;
+                    if (level) slot[row * ly.spd + col] = v;
                 }
             }
         }
@@ -2232,8 +2232,8 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
                 for (int p = 0; p < npass; ++p) {
                     const int row = row0 + p * rpp;
                     if (on && row < b.yl) {
-                        const int o = row * ly.pd + col;
-                        acc = __builtin_amdgcn_sad_u8(s_pic[o], avg_up_u8(s0[o], s1[o]), acc);
+                        const int o = row * ly.spd + col;
+                        acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], avg_up_u8(s0[o], s1[o]), acc);
                     }
                 }
                 const int sum = wave_sum((int)acc);
@@ -3013,10 +3013,18 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
     MtLds ly;
     ly.pd = (std::max(max_x0, max_x1) + 3) / 4;
     ly.prows = std::max(max_y0, max_y1);
-    ly.slot = ly.pd * max_y1;  // candidate rows share the picture's pitch
+    ly.spd = (max_x1 + 3) / 4;
+    ly.slot = ly.spd * max_y1;
     ly.wave = ly.pd * ly.prows + 8 * ly.slot;
     const size_t lds = sizeof(uint32_t) * 4 * (size_t)ly.wave;
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_mode_tables, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           159 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
     hipLaunchKernelGGL(k_mode_tables, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }

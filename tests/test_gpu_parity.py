@@ -80,6 +80,24 @@ def test_other_sizes(ctx, size, nrefs, seed):
     _assert_same(want, _gpu(ctx, p, d), nrefs)
 
 
+def test_large_blocks_720p(ctx):
+    # 24/16 blocks: 40-sample sub-superblocks and 72-sample superblocks in the
+    # mode tables
+    d = synth.sized(71, "720p", 2)
+    p = make_params(24, 24, 16, 16, precision=2, depth=3, lam=10.0)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 2)
+
+
+def test_2160p_b_picture(ctx):
+    # 3840x2160, 24/16 blocks, 5 levels, pans up to 128 pel
+    d = synth.config4(0)
+    p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    print("case 2160p", flush=True)
+    want, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p, d), 2)
+
+
 def test_overhanging_grid_and_odd_blocks(ctx):
     # 360x240 with 10x14 blocks: rationalised to 12/16 lengths, block grid
     # overhangs on both axes, blocks entirely outside the picture
