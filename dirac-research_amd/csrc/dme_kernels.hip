@@ -1592,13 +1592,19 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
 // whenever every partial sum is representable, which holds when the start is
 // a multiple of the float spacing at the final magnitude; otherwise it is
 // replayed pixel by pixel.
+// With m = start + isad in [2^(e-1), 2^e), the float spacing there is
+// u = 2^(e-24): the test is u <= 1 and start a multiple of u, read off the
+// exponent of m and the lowest set bit of start's significand.
 __device__ __forceinline__ bool chain_is_exact(float start, int isad) {
     const double m = (double)start + (double)isad;
     if (m < 1.0) return start == 0.0f;
-    int e;
-    frexp(m, &e);                        // m in [2^(e-1), 2^e)
-    const double u = ldexp(1.0, e - 24);  // float spacing in that binade
-    return u <= 1.0 && fmod((double)start, u) == 0.0;
+    const int e = (int)((__double_as_longlong(m) >> 52) & 0x7ff) - 1022;
+    if (e > 24) return false;
+    if (start == 0.0f) return true;
+    const unsigned bits = __float_as_uint(start);
+    const int ex = (int)((bits >> 23) & 0xffu);
+    const unsigned mant = (bits & 0x7fffffu) | (ex ? 0x800000u : 0u);
+    return (ex ? ex : 1) - 150 + __builtin_ctz(mant) >= e - 24;  // start = mant * 2^((ex ? ex : 1) - 150)
 }
 
 
@@ -1686,6 +1692,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                                                       int max_rows) {
     extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT] | queue [SUB_QCAP]
     __shared__ int s_cnt[2];
+    __shared__ int s_chain[2];  // per pass parity: some block of the pass may need a float chain
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
     if (tk.ref >= J.nrefs || J.prec == 0) return;
@@ -1702,8 +1709,8 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
     int blen[2] = {J.blen[2][0], J.blen[2][1]}, bsep[2] = {J.bsep[2][0], J.bsep[2][1]};
     int boff[2] = {J.boff[2][0], J.boff[2][1]};
     const bool c_sem = J.fp_sem == DME_FP_C;
-    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
-    int phase = 0;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = s_chain[threadIdx.x] = 0;
+    int phase = 0, gen = 0;
     const int T = (X - 1) + 2 * (Y - 1);
     for (int t = -1; t <= T; ++t) {
         WfCopy<8> cp;
@@ -1780,6 +1787,42 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     }
                 }
             }
+            // Can any candidate this block may test need the per-pixel float
+            // chain?  Only where some candidate's samples leave the picture
+            // (the window's extreme candidates bound that) or with C
+            // semantics, and only when a start cost mvcost * lambda can fall
+            // off the float grid at the total's magnitude: mvcost is an
+            // integer below N = |window|_1 (< 32768), so the start is a
+            // multiple of lambda's lowest set bit, and totals stay below
+            // N * lambda + 255 * xl * yl.  When no block of the pass can, its
+            // chain round trips are skipped.
+            const int cf = gen & 1;
+            if (threadIdx.x == 0) s_chain[cf ^ 1] = 0;  // the previous pass read it before this pass's barrier
+            if (refine && !b.empty()) {
+                const int cqx = pel.x << prec, cqy = pel.y << prec;
+                bool edge = c_sem;
+                for (int i = 1; i <= prec && !edge; ++i) {
+                    const int sh = prec - i, r1 = rad >> sh;
+                    dme_mv lo, hi;
+                    lo.x = (cqx >> sh) - r1;
+                    lo.y = (cqy >> sh) - r1;
+                    hi.x = (cqx >> sh) + r1;
+                    hi.y = (cqy >> sh) + r1;
+                    edge = up_start(b, lo, i, UW, UH).bounds || up_start(b, hi, i, UW, UH).bounds;
+                }
+                bool may = false;
+                if (edge && lambda != 0.0f) {
+                    const int N = abs(cqx) + abs(cqy) + 2 * rad;
+                    const unsigned lb = __float_as_uint(lambda);
+                    const int lex = (int)((lb >> 23) & 0xffu);
+                    const unsigned lm = (lb & 0x7fffffu) | (lex ? 0x800000u : 0u);
+                    const int low = (lex ? lex : 1) - 150 + __builtin_ctz(lm);  // lowest set bit of lambda
+                    const double mmax = (double)N * fabs((double)lambda) + 255.0 * b.xl * b.yl;
+                    const int e = (int)((__double_as_longlong(mmax) >> 52) & 0x7ff) - 1022;  // mmax < 2^e
+                    may = N >= 32768 || lambda < 0.0f || low < e - 24;
+                }
+                if (may) s_chain[cf] = 1;
+            }
             // predictor SADs outside the table
             {
                 SubJob mine[4];
@@ -1799,6 +1842,8 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                 sub_serve<false>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
                 psad = ir[0];
             }
+            const bool chains = s_chain[cf] != 0;  // visible after the round trip's barrier
+            ++gen;
             dme_cost pcost;
             pcost.mvcost = 0.0f;
             pcost.SAD = (float)psad;
@@ -1869,7 +1914,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     }
                     int ir[4] = {0, 0, 0, 0};
                     float fr[4] = {tot[0], tot[1], tot[2], tot[3]};
-                    sub_serve<true>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);
+                    if (chains) sub_serve<true>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);  // uniform
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (on[j] && fr[j] < best.total) {
