@@ -319,7 +319,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
             t.ow = P->W;
             t.oh = P->H;
             t.offset = off;
-            off += ((P->W + 2 + 63) / 64) * ((P->H + 2 + 3) / 4);  // 64 x 4 positions per tile
+            off += ((P->W + 2 + 63) / 64) * ((P->H + 2 + 15) / 16);  // 64 x 16 positions per tile
             all.push_back(t);
         }
         ranges.push_back(std::make_pair(first, off));

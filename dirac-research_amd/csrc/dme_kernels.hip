@@ -388,17 +388,17 @@ __global__ __launch_bounds__(256) void k_check_range(const int16_t *__restrict__
 // (both clamped corners coincide there), so reads clamp X and Y to that range.
 // A half-pel vector m reads the even phases: position 2m in quarter pels.
 // ---------------------------------------------------------------------------
-// Built straight from the picture: a workgroup takes 64 x 4 positions (one
-// thread each); the up-plane samples they use (rows 2Y .. 2Y + 2, columns
+// Built straight from the picture: a workgroup takes 64 x 16 positions (a
+// thread per column, four rows apart); the up-plane samples they use (rows 2Y .. 2Y + 2, columns
 // 2X .. 2X + 2, clamped) are made in LDS first, as K2 makes them: the
 // clamped source tile, its vertical taps, then the 9 x 129 up samples.
 #define QT_X 64
-#define QT_Y 4
-#define QT_SR (2 * QT_Y + 5)   // source rows Y0 - 4 .. Y0 + 8
-#define QT_SC (QT_X + 9)       // source columns X0 - 4 .. X0 + 68
-#define QT_VR (QT_Y + 2)       // vertical-tap rows Y0 - 1 .. Y0 + 4
-#define QT_UR (2 * QT_Y + 1)   // up rows 2Y0 .. 2Y0 + 8
-#define QT_UC (2 * QT_X + 1)   // up columns 2X0 .. 2X0 + 128
+#define QT_Y 16
+#define QT_SR (QT_Y + 9)       // source rows Y0 - 4 .. Y0 + QT_Y + 4
+#define QT_SC (QT_X + 9)       // source columns X0 - 4 .. X0 + QT_X + 4
+#define QT_VR (QT_Y + 2)       // vertical-tap rows Y0 - 1 .. Y0 + QT_Y
+#define QT_UR (2 * QT_Y + 1)   // up rows 2Y0 .. 2Y0 + 2 QT_Y
+#define QT_UC (2 * QT_X + 1)   // up columns 2X0 .. 2X0 + 2 QT_X
 __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ pt, int npt, int total) {
     __shared__ int16_t s_src[QT_SR][QT_SC];
     __shared__ int16_t s_v[QT_VR][QT_SC];
@@ -431,56 +431,56 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
         s_v[r][c] = (int16_t)up_tap8(s8);
     }
     __syncthreads();
-    // up sample (clamp(2Y0 + a), clamp(2X0 + b)): source sample (y, x) = that / 2
-    for (int i = tid; i < QT_UR * QT_UC; i += 256) {
-        const int a = i / QT_UC, b = i - a * QT_UC;
-        const int cuy = clampi(2 * Y0 + a, 0, 2 * H - 1), cux = clampi(2 * X0 + b, 0, 2 * W - 1);
+    // up sample (clamp(2Y0 + a), clamp(2X0 + b)): source sample (y, x) = that / 2.
+    // A thread makes an even and an odd column of one row, so that the
+    // lanes of a wave take the same parity branches (except at the picture's
+    // edges, where clamping changes parities).
+    auto up_val = [&](int cuy, int cux) -> int {
         const int y = cuy >> 1, x = cux >> 1;
         const int sr = y - (Y0 - 4), sc = x - (X0 - 4), vr = y - (Y0 - 1);
-        int v;
+        int s8[8];
         if (!(cuy & 1)) {
-            if (!(cux & 1)) {
-                v = s_src[sr][sc];
-            } else {
-                int s8[8];
+            if (!(cux & 1)) return s_src[sr][sc];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) s8[t] = s_src[sr][sc - 3 + t];
-                v = up_tap8(s8);
-            }
-        } else {
-            if (!(cux & 1)) {
-                v = s_v[vr][sc];
-            } else {
-                int s8[8];
-#pragma unroll
-                for (int t = 0; t < 8; ++t) s8[t] = s_v[vr][sc - 3 + t];
-                v = up_tap8(s8);
-            }
+            for (int t = 0; t < 8; ++t) s8[t] = s_src[sr][sc - 3 + t];
+            return up_tap8(s8);
         }
-        s_u[a][b] = (int16_t)v;
+        if (!(cux & 1)) return s_v[vr][sc];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s8[t] = s_v[vr][sc - 3 + t];
+        return up_tap8(s8);
+    };
+    for (int i = tid; i < QT_UR * (QT_X + 1); i += 256) {
+        const int a = i / (QT_X + 1), j = i - a * (QT_X + 1);
+        const int cuy = clampi(2 * Y0 + a, 0, 2 * H - 1);
+        s_u[a][2 * j] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j, 0, 2 * W - 1));
+        if (2 * j + 1 < QT_UC) s_u[a][2 * j + 1] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j + 1, 0, 2 * W - 1));
     }
     __syncthreads();
-    const int px = tid & (QT_X - 1), py = tid / QT_X;
-    const int X = X0 + px, Y = Y0 + py;
-    if (X >= QW - 1 || Y >= QH - 1) return;  // positions X <= W, Y <= H
-    int u[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[a][c] = s_u[2 * py + a][2 * px + c];
     uint32_t *__restrict__ out = reinterpret_cast<uint32_t *>(p.out);
+    const int px = tid & (QT_X - 1);
+    const int X = X0 + px;
+    for (int py = tid / QT_X; py < QT_Y; py += 256 / QT_X) {
+        const int Y = Y0 + py;
+        if (X >= QW - 1 || Y >= QH - 1) continue;  // positions X <= W, Y <= H
+        int u[3][3];
 #pragma unroll
-    for (int fy = 0; fy < 4; ++fy) {
-        const int oy = fy >> 1, ry = fy & 1;
-        uint32_t d = 0;
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int fx = 0; fx < 4; ++fx) {
-            const int ox = fx >> 1, rx = fx & 1;
-            const int v = ((2 - rx) * (2 - ry) * u[oy][ox] + rx * (2 - ry) * u[oy][ox + 1] +
-                           (2 - rx) * ry * u[oy + 1][ox] + rx * ry * u[oy + 1][ox + 1] + 2) >> 2;
-            d |= (uint32_t)(v + 128) << (8 * fx);
+            for (int c = 0; c < 3; ++c) u[a][c] = s_u[2 * py + a][2 * px + c];
+#pragma unroll
+        for (int fy = 0; fy < 4; ++fy) {
+            const int oy = fy >> 1, ry = fy & 1;
+            uint32_t d = 0;
+#pragma unroll
+            for (int fx = 0; fx < 4; ++fx) {
+                const int ox = fx >> 1, rx = fx & 1;
+                const int v = ((2 - rx) * (2 - ry) * u[oy][ox] + rx * (2 - ry) * u[oy][ox + 1] +
+                               (2 - rx) * ry * u[oy + 1][ox] + rx * ry * u[oy + 1][ox + 1] + 2) >> 2;
+                d |= (uint32_t)(v + 128) << (8 * fx);
+            }
+            out[(size_t)((Y + 1) * 4 + fy) * QW + X + 1] = d;
         }
-        out[(size_t)((Y + 1) * 4 + fy) * QW + X + 1] = d;
     }
 }
 
