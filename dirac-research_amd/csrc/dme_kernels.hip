@@ -47,6 +47,15 @@ __device__ __forceinline__ int xcd_block() {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
+// Table kernels: consecutive blocks (or units) per wave, so that a launch
+// has fewer, longer workgroups and a wave's blocks share reference data
+// (measured per kernel; the pel and sub-pel tables keep one block per wave:
+// a loop there costs registers and occupancy).
+#define PW_MODE_L2 2
+#define PW_MODE_TABLES 4
+#define PW_BLOCK_DC 4
+#define PW_LP_REFINE 2
+
 // Index of the calling wave in its workgroup, known to the compiler as
 // wave-uniform, so that what is derived from it stays in scalar registers.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
@@ -2035,79 +2044,81 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 // parallel: it only reads the block's own refined vectors and costs.
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
-    const int gi = xcd_block() * 4 + wave_id();
-    if (gi >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
-    const JobDesc &J = jobs[tk.job];
-    const int k = gi - tk.offset;
-    const int xb = k % J.xnb, yb = k / J.xnb;
-    const int lane = threadIdx.x & 63;
-    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
-    const float loc_lambda = J.lam[k];
-    const float lf = J.lf[2];
-    const float mode_cost = 0.0f;
-    int mode = DME_REF1_ONLY;
-    dme_cost c1 = J.pc[0][k];
-    c1.total *= lf;
-    float min_cost = c1.total + mode_cost;
-    float best_sad = c1.SAD;
-    if (J.nrefs > 1) {
-        dme_cost c2 = J.pc[1][k];
-        c2.total *= lf;
-        float uc = c2.total + mode_cost;
-        if (uc < min_cost) {
-            mode = DME_REF2_ONLY;
-            min_cost = uc;
-            best_sad = c2.SAD;
+    const int g0 = (xcd_block() * 4 + wave_id()) * PW_MODE_L2, g1 = min(g0 + PW_MODE_L2, total);
+    for (int gi = g0; gi < g1; ++gi) {
+        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
+        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const JobDesc &J = jobs[tk.job];
+        const int k = gi - tk.offset;
+        const int xb = k % J.xnb, yb = k / J.xnb;
+        const int lane = threadIdx.x & 63;
+        const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
+        const float loc_lambda = J.lam[k];
+        const float lf = J.lf[2];
+        const float mode_cost = 0.0f;
+        int mode = DME_REF1_ONLY;
+        dme_cost c1 = J.pc[0][k];
+        c1.total *= lf;
+        float min_cost = c1.total + mode_cost;
+        float best_sad = c1.SAD;
+        if (J.nrefs > 1) {
+            dme_cost c2 = J.pc[1][k];
+            c2.total *= lf;
+            float uc = c2.total + mode_cost;
+            if (uc < min_cost) {
+                mode = DME_REF2_ONLY;
+                min_cost = uc;
+                best_sad = c2.SAD;
+            }
+            dme_cost bc;
+            bc.mvcost = c1.mvcost + c2.mvcost;
+            bc.SAD = b.empty() ? 0.0f : (float)wave_bi_sad(J, b, J.vec[0][k], J.vec[1][k]);
+            bc.total = bc.SAD + loc_lambda * bc.mvcost;
+            bc.total *= lf;
+            uc = bc.total + mode_cost;
+            if (uc < min_cost) {
+                mode = DME_REF1AND2;
+                min_cost = uc;
+                best_sad = bc.SAD;
+            }
+            if (lane == 0) J.bi[k] = bc;
+        } else if (lane == 0) {
+            // the second reference's arrays stay zero (motion.cpp:55-126)
+            dme_cost z;
+            z.SAD = z.mvcost = z.total = 0.0f;
+            dme_mv zm;
+            zm.x = zm.y = 0;
+            J.bi[k] = z;
+            J.pc[1][k] = z;
+            J.vec[1][k] = zm;
         }
-        dme_cost bc;
-        bc.mvcost = c1.mvcost + c2.mvcost;
-        bc.SAD = b.empty() ? 0.0f : (float)wave_bi_sad(J, b, J.vec[0][k], J.vec[1][k]);
-        bc.total = bc.SAD + loc_lambda * bc.mvcost;
-        bc.total *= lf;
-        uc = bc.total + mode_cost;
-        if (uc < min_cost) {
-            mode = DME_REF1AND2;
-            min_cost = uc;
-            best_sad = bc.SAD;
+        float intra = 0.0f;
+        int dc0 = 0;
+        if ((double)best_sad > 4.0 * J.blen[2][0] * J.blen[2][1]) {
+            int dc = 0;
+            float ic = 0.0f;
+            if (!b.empty()) {
+                dc = wave_block_dc(J.pic[0], J.W, b);
+                ic = (float)wave_intra_sad(J.pic[0], J.W, b, dc);
+            }
+            ic *= lf;
+            const float uc = ic + mode_cost;
+            if (uc < min_cost && (double)ic < 0.85 * (double)best_sad) {
+                mode = DME_INTRA;
+                min_cost = uc;
+            }
+            intra = ic;
+            dc0 = dc;
         }
-        if (lane == 0) J.bi[k] = bc;
-    } else if (lane == 0) {
-        // the second reference's arrays stay zero (motion.cpp:55-126)
-        dme_cost z;
-        z.SAD = z.mvcost = z.total = 0.0f;
-        dme_mv zm;
-        zm.x = zm.y = 0;
-        J.bi[k] = z;
-        J.pc[1][k] = z;
-        J.vec[1][k] = zm;
-    }
-    float intra = 0.0f;
-    int dc0 = 0;
-    if ((double)best_sad > 4.0 * J.blen[2][0] * J.blen[2][1]) {
-        int dc = 0;
-        float ic = 0.0f;
-        if (!b.empty()) {
-            dc = wave_block_dc(J.pic[0], J.W, b);
-            ic = (float)wave_intra_sad(J.pic[0], J.W, b, dc);
+        if (lane == 0) {
+            // intra cost and DCs are zero unless set here (motion.cpp:55-126)
+            J.intra[k] = intra;
+            J.dc[0][k] = (int16_t)dc0;
+            J.dc[1][k] = 0;
+            J.dc[2][k] = 0;
+            J.mode[k] = mode;
+            J.ucost[k] = min_cost;
         }
-        ic *= lf;
-        const float uc = ic + mode_cost;
-        if (uc < min_cost && (double)ic < 0.85 * (double)best_sad) {
-            mode = DME_INTRA;
-            min_cost = uc;
-        }
-        intra = ic;
-        dc0 = dc;
-    }
-    if (lane == 0) {
-        // intra cost and DCs are zero unless set here (motion.cpp:55-126)
-        J.intra[k] = intra;
-        J.dc[0][k] = (int16_t)dc0;
-        J.dc[1][k] = 0;
-        J.dc[2][k] = 0;
-        J.mode[k] = mode;
-        J.ucost[k] = min_cost;
     }
 }
 
@@ -2160,134 +2171,136 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
     extern __shared__ uint32_t s_mt[];
     __shared__ int s_pair[4][16];
     const int wv = wave_id(), lane = threadIdx.x & 63;
-    const int gi = xcd_block() * 4 + wv;
-    if (gi >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
-    const JobDesc &J = jobs[tk.job];
-    const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
-    const int u = gi - tk.offset;
-    const int X2 = J.xnb, W = J.W, H = J.H, QW = W + 2, step = J.step, nr = J.nrefs;
-    const int XU = level ? J.xnb / 2 : J.xsb;
-    const int xu = u % XU, yu = u / XU;
-    const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], W, H, xu, yu);
-    const int nq = level ? 4 : 16, side = level ? 2 : 4;
-    int *out = level ? J.m1tab + (size_t)u * 24 : J.m0tab + (size_t)u * 32;
-    if (b.empty()) {
-        if (lane < (level ? 24 : 32)) out[lane] = 0;
-        return;
-    }
-    uint32_t *s_pic = s_mt + wv * ly.wave;
-    uint32_t *s_slot = s_pic + ly.pd * ly.prows;
-    // lane r*16 + q holds candidate q of reference r and the index of its
-    // first occurrence among the reference's candidates
-    const int cr = lane >> 4, cq = lane & 15;
-    const bool cand = lane < 32 && cr < nr && cq < nq;
-    dme_mv cv;
-    cv.x = cv.y = 0;
-    if (cand) cv = J.vec[cr][(size_t)(yu * side + cq / side) * X2 + xu * side + cq % side];
-    int first = cq;
-    for (int p = nq - 1; p >= 0; --p) {
-        const int px = __shfl(cv.x, (lane & 48) + p, WAVE), py = __shfl(cv.y, (lane & 48) + p, WAVE);
-        if (p < cq && px == cv.x && py == cv.y) first = p;
-    }
-    // picture samples: rows of pd dwords, zero beyond xl
-    {
-        const int ppad = (b.xl + 3) & ~3;
-        const B8Rect pr = b8_rect(J.pic8, J.p8, s_pic, ly.pd, b.xp, b.yp, b.xl, b.yl, ppad);
-        if (b8_inside(J.p8, H, b.xp, b.yp, b.yl, ppad)) {
-            const int np = (b.yl + pr.rpp - 1) / pr.rpp;
-            for (int p0 = 0; p0 < np; p0 += 4) {
-                uint32_t lo[4], hi[4];
-                b8_load(pr, J.p8, p0, lo, hi);
-                b8_store(pr, p0, lo, hi);
-            }
-        } else {
-            b8_copy_clamped(pr, J.pic8, J.p8, W, H, b.xp, b.yp);
+    const int g0 = (xcd_block() * 4 + wv) * PW_MODE_TABLES, g1 = min(g0 + PW_MODE_TABLES, total);
+    for (int gi = g0; gi < g1; ++gi) {
+        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
+        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const JobDesc &J = jobs[tk.job];
+        const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
+        const int u = gi - tk.offset;
+        const int X2 = J.xnb, W = J.W, H = J.H, QW = W + 2, step = J.step, nr = J.nrefs;
+        const int XU = level ? J.xnb / 2 : J.xsb;
+        const int xu = u % XU, yu = u / XU;
+        const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], W, H, xu, yu);
+        const int nq = level ? 4 : 16, side = level ? 2 : 4;
+        int *out = level ? J.m1tab + (size_t)u * 24 : J.m0tab + (size_t)u * 32;
+        if (b.empty()) {
+            if (lane < (level ? 24 : 32)) out[lane] = 0;
+            continue;
         }
-    }
-    // lane (row0 + k * rpp, col) of the unit in the SAD passes
-    const int nd = (b.xl + 3) >> 2, rpp = WAVE / nd;
-    const int row0 = (int)(((float)lane + 0.5f) * (1.0f / (float)nd)), col = lane - row0 * nd;
-    const bool on = row0 < rpp;
-    const int valid = b.xl - 4 * col;
-    const uint32_t mask = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
-    const int npass = (b.yl + rpp - 1) / rpp;
-    WAVE_SYNC();
-    int mysum = 0;
-    unsigned long long todo = __ballot(cand && first == cq);
-    while (todo) {
-        const int L = __ffsll((long long)todo) - 1;
-        todo &= todo - 1;
-        const int r = L >> 4, q = L & 15;
-        dme_mv m;
-        m.x = __builtin_amdgcn_readlane(cv.x, L);
-        m.y = __builtin_amdgcn_readlane(cv.y, L);
-        const UpStart us = up_start(b, m, step, 2 * W, 2 * H);
-        const int xq = 2 * us.rsx + us.rx, yq = 2 * us.rsy + us.ry;  // quarter-pel start
-        const int X0 = xq >> 2, Y0 = yq >> 2, fx = xq & 3, fy = yq & 3;
-        const uint32_t sel = (uint32_t)fx | ((uint32_t)(fx + 4) << 8) | 0x0c0c0000u;
-        const uint32_t *__restrict__ qp = J.qp[r];
-        int xo[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * col + i, -1, W) + 1;
-        uint32_t *slot = s_slot + (r * 4 + q) * ly.slot;
-        uint32_t acc = 0;
-        for (int p0 = 0; p0 < npass; p0 += 4) {
-            uint32_t d[4][4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int row = row0 + (p0 + k) * rpp;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) d[k][i] = 0;
-                if (on && row < b.yl) {
-                    const uint32_t *qr = qp + (size_t)((clampi(Y0 + row, -1, H) + 1) * 4 + fy) * QW;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) d[k][i] = qr[xo[i]];
+        uint32_t *s_pic = s_mt + wv * ly.wave;
+        uint32_t *s_slot = s_pic + ly.pd * ly.prows;
+        // lane r*16 + q holds candidate q of reference r and the index of its
+        // first occurrence among the reference's candidates
+        const int cr = lane >> 4, cq = lane & 15;
+        const bool cand = lane < 32 && cr < nr && cq < nq;
+        dme_mv cv;
+        cv.x = cv.y = 0;
+        if (cand) cv = J.vec[cr][(size_t)(yu * side + cq / side) * X2 + xu * side + cq % side];
+        int first = cq;
+        for (int p = nq - 1; p >= 0; --p) {
+            const int px = __shfl(cv.x, (lane & 48) + p, WAVE), py = __shfl(cv.y, (lane & 48) + p, WAVE);
+            if (p < cq && px == cv.x && py == cv.y) first = p;
+        }
+        // picture samples: rows of pd dwords, zero beyond xl
+        {
+            const int ppad = (b.xl + 3) & ~3;
+            const B8Rect pr = b8_rect(J.pic8, J.p8, s_pic, ly.pd, b.xp, b.yp, b.xl, b.yl, ppad);
+            if (b8_inside(J.p8, H, b.xp, b.yp, b.yl, ppad)) {
+                const int np = (b.yl + pr.rpp - 1) / pr.rpp;
+                for (int p0 = 0; p0 < np; p0 += 4) {
+                    uint32_t lo[4], hi[4];
+                    b8_load(pr, J.p8, p0, lo, hi);
+                    b8_store(pr, p0, lo, hi);
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int row = row0 + (p0 + k) * rpp;
-                if (on && row < b.yl) {
-                    const uint32_t lo = __builtin_amdgcn_perm(d[k][1], d[k][0], sel);
-                    const uint32_t hi = __builtin_amdgcn_perm(d[k][3], d[k][2], sel);
-                    const uint32_t v = __builtin_amdgcn_perm(hi, lo, 0x05040100u) & mask;
-                    acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], v, acc);
-                    if (level) slot[row * ly.spd + col] = v;
-                }
+            } else {
+                b8_copy_clamped(pr, J.pic8, J.p8, W, H, b.xp, b.yp);
             }
         }
-        const int sum = wave_sum((int)acc);
-        if (lane == L) mysum = sum;
-    }
-    // every candidate takes the sum of its first occurrence
-    const int fsum = __shfl(mysum, (lane & 48) + first, WAVE);
-    if (cand) out[(level ? 4 : 16) * cr + cq] = fsum;
-    if (level == 1 && nr > 1) {
-        // bi-prediction SADs of the distinct pairs, then of every pair
+        // lane (row0 + k * rpp, col) of the unit in the SAD passes
+        const int nd = (b.xl + 3) >> 2, rpp = WAVE / nd;
+        const int row0 = (int)(((float)lane + 0.5f) * (1.0f / (float)nd)), col = lane - row0 * nd;
+        const bool on = row0 < rpp;
+        const int valid = b.xl - 4 * col;
+        const uint32_t mask = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+        const int npass = (b.yl + rpp - 1) / rpp;
         WAVE_SYNC();
-        const unsigned long long d0 = __ballot(cand && cr == 0 && first == cq);
-        const unsigned long long d1 = __ballot(cand && cr == 1 && first == cq) >> 16;
-        for (unsigned long long m0 = d0; m0; m0 &= m0 - 1) {
-            const int q1 = __ffsll((long long)m0) - 1;
-            for (unsigned long long m1 = d1; m1; m1 &= m1 - 1) {
-                const int q2 = __ffsll((long long)m1) - 1;
-                const uint32_t *s0 = s_slot + q1 * ly.slot, *s1 = s_slot + (4 + q2) * ly.slot;
-                uint32_t acc = 0;
-                for (int p = 0; p < npass; ++p) {
-                    const int row = row0 + p * rpp;
+        int mysum = 0;
+        unsigned long long todo = __ballot(cand && first == cq);
+        while (todo) {
+            const int L = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const int r = L >> 4, q = L & 15;
+            dme_mv m;
+            m.x = __builtin_amdgcn_readlane(cv.x, L);
+            m.y = __builtin_amdgcn_readlane(cv.y, L);
+            const UpStart us = up_start(b, m, step, 2 * W, 2 * H);
+            const int xq = 2 * us.rsx + us.rx, yq = 2 * us.rsy + us.ry;  // quarter-pel start
+            const int X0 = xq >> 2, Y0 = yq >> 2, fx = xq & 3, fy = yq & 3;
+            const uint32_t sel = (uint32_t)fx | ((uint32_t)(fx + 4) << 8) | 0x0c0c0000u;
+            const uint32_t *__restrict__ qp = J.qp[r];
+            int xo[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * col + i, -1, W) + 1;
+            uint32_t *slot = s_slot + (r * 4 + q) * ly.slot;
+            uint32_t acc = 0;
+            for (int p0 = 0; p0 < npass; p0 += 4) {
+                uint32_t d[4][4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int row = row0 + (p0 + k) * rpp;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[k][i] = 0;
                     if (on && row < b.yl) {
-                        const int o = row * ly.spd + col;
-                        acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], avg_up_u8(s0[o], s1[o]), acc);
+                        const uint32_t *qr = qp + (size_t)((clampi(Y0 + row, -1, H) + 1) * 4 + fy) * QW;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) d[k][i] = qr[xo[i]];
                     }
                 }
-                const int sum = wave_sum((int)acc);
-                if (lane == 0) s_pair[wv][q1 * 4 + q2] = sum;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int row = row0 + (p0 + k) * rpp;
+                    if (on && row < b.yl) {
+                        const uint32_t lo = __builtin_amdgcn_perm(d[k][1], d[k][0], sel);
+                        const uint32_t hi = __builtin_amdgcn_perm(d[k][3], d[k][2], sel);
+                        const uint32_t v = __builtin_amdgcn_perm(hi, lo, 0x05040100u) & mask;
+                        acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], v, acc);
+                        if (level) slot[row * ly.spd + col] = v;
+                    }
+                }
             }
+            const int sum = wave_sum((int)acc);
+            if (lane == L) mysum = sum;
         }
-        WAVE_SYNC();
-        const int f0 = __shfl(first, (lane >> 2) & 3, WAVE), f1 = __shfl(first, 16 + (lane & 3), WAVE);
-        if (lane < 16) out[8 + lane] = s_pair[wv][f0 * 4 + f1];
+        // every candidate takes the sum of its first occurrence
+        const int fsum = __shfl(mysum, (lane & 48) + first, WAVE);
+        if (cand) out[(level ? 4 : 16) * cr + cq] = fsum;
+        if (level == 1 && nr > 1) {
+            // bi-prediction SADs of the distinct pairs, then of every pair
+            WAVE_SYNC();
+            const unsigned long long d0 = __ballot(cand && cr == 0 && first == cq);
+            const unsigned long long d1 = __ballot(cand && cr == 1 && first == cq) >> 16;
+            for (unsigned long long m0 = d0; m0; m0 &= m0 - 1) {
+                const int q1 = __ffsll((long long)m0) - 1;
+                for (unsigned long long m1 = d1; m1; m1 &= m1 - 1) {
+                    const int q2 = __ffsll((long long)m1) - 1;
+                    const uint32_t *s0 = s_slot + q1 * ly.slot, *s1 = s_slot + (4 + q2) * ly.slot;
+                    uint32_t acc = 0;
+                    for (int p = 0; p < npass; ++p) {
+                        const int row = row0 + p * rpp;
+                        if (on && row < b.yl) {
+                            const int o = row * ly.spd + col;
+                            acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], avg_up_u8(s0[o], s1[o]), acc);
+                        }
+                    }
+                    const int sum = wave_sum((int)acc);
+                    if (lane == 0) s_pair[wv][q1 * 4 + q2] = sum;
+                }
+            }
+            WAVE_SYNC();
+            const int f0 = __shfl(first, (lane >> 2) & 3, WAVE), f1 = __shfl(first, 16 + (lane & 3), WAVE);
+            if (lane < 16) out[8 + lane] = s_pair[wv][f0 * 4 + f1];
+        }
     }
 }
 
@@ -2684,147 +2697,149 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
                                                    unsigned long long *__restrict__ stats, LpLds ly) {
     extern __shared__ uint32_t s_lp[];
     const int wv = wave_id(), lane = threadIdx.x & 63;
-    const int gi = xcd_block() * 4 + wv;
-    if (gi >= total) return;
-    uint32_t *s_pic = s_lp + wv * ly.wave();
-    uint32_t *s_box = s_pic + ly.pp * ly.prows;
-    uint32_t *s_zero = s_box + ly.rp * ly.brows;
-    int *s_list = reinterpret_cast<int *>(s_zero + ly.zp * ly.zrows);  // up to 100 union positions
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
-    const JobDesc &J = jobs[tk.job];
-    const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
-    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
-    dme_mv *__restrict__ V = J.vec[r];
-    dme_cost *__restrict__ C = J.pc[r];
-    if (b.empty()) {  // out of the picture: no update (pixel_match.cpp:278-281)
-        if (lane == 0) {
-            V[k].x = V[k].y = 0;
-            C[k].SAD = 10000000.0f;
+    const int g0 = (xcd_block() * 4 + wv) * PW_LP_REFINE, g1 = min(g0 + PW_LP_REFINE, total);
+    for (int gi = g0; gi < g1; ++gi) {
+        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
+        uint32_t *s_pic = s_lp + wv * ly.wave();
+        uint32_t *s_box = s_pic + ly.pp * ly.prows;
+        uint32_t *s_zero = s_box + ly.rp * ly.brows;
+        int *s_list = reinterpret_cast<int *>(s_zero + ly.zp * ly.zrows);  // up to 100 union positions
+        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const JobDesc &J = jobs[tk.job];
+        const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
+        const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
+        dme_mv *__restrict__ V = J.vec[r];
+        dme_cost *__restrict__ C = J.pc[r];
+        if (b.empty()) {  // out of the picture: no update (pixel_match.cpp:278-281)
+            if (lane == 0) {
+                V[k].x = V[k].y = 0;
+                C[k].SAD = 10000000.0f;
+            }
+            continue;
         }
-        return;
-    }
-    const int W = J.W, H = J.H, bp = J.p8;
-    dme_mv g;
-    float s2;
-    lp_guide(J, r, x, y, g, s2);
-    g.x *= 4;
-    g.y *= 4;
-    const float *o = out4 + ((size_t)J.lp_off + k) * 32;
-    dme_mv hyp[4];  // offsets from g
+        const int W = J.W, H = J.H, bp = J.p8;
+        dme_mv g;
+        float s2;
+        lp_guide(J, r, x, y, g, s2);
+        g.x *= 4;
+        g.y *= 4;
+        const float *o = out4 + ((size_t)J.lp_off + k) * 32;
+        dme_mv hyp[4];  // offsets from g
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        hyp[h].x = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f)), 2 - LP_BOX,
-                          LP_BOX - 2);
-        hyp[h].y = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f)), 2 - LP_BOX,
-                          LP_BOX - 2);
-    }
-    const float pred_sad = expm1f(fminf(o[16 + r], 20.0f));
-    // picture block and box in one round of loads
-    {
-        const uint8_t *const base[3] = {J.pic8, J.ref8[r], J.ref8[r]};
-        const int x0[3] = {b.xp, b.xp + g.x - LP_BOX, 0}, y0[3] = {b.yp, b.yp + g.y - LP_BOX, 0};
-        const int ppad = (b.xl + 15) & ~15, RW = b.xl + 2 * LP_BOX, RH = b.yl + 2 * LP_BOX;
-        B8Rect rc[3];
-        bool fast[3];
-        rc[0] = b8_rect(base[0], bp, s_pic, ly.pp, x0[0], y0[0], b.xl, b.yl, ppad);
-        rc[1] = b8_rect(base[1], bp, s_box, ly.rp, x0[1], y0[1], RW, RH, RW);
-        rc[2] = rc[1];
-        fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl, ppad);
-        fast[1] = b8_inside(bp, H, x0[1], y0[1], RH, RW);
-        fast[2] = false;
-        b8_stage(rc, fast, 2, base, bp, W, H, x0, y0);
-    }
-    WAVE_SYNC();
-    const bool whole = (b.xl & 15) == 0;
-    // union of the hypothesis windows: the positions of its bounding box that
-    // lie in a window, compacted into a list (box raster indices), then one
-    // SAD per listed position
-    int ux0 = hyp[0].x, ux1 = hyp[0].x, uy0 = hyp[0].y, uy1 = hyp[0].y;
-#pragma unroll
-    for (int h = 1; h < 4; ++h) {
-        ux0 = min(ux0, hyp[h].x);
-        ux1 = max(ux1, hyp[h].x);
-        uy0 = min(uy0, hyp[h].y);
-        uy1 = max(uy1, hyp[h].y);
-    }
-    ux0 -= 2;
-    uy0 -= 2;
-    const int UW = ux1 + 3 - ux0, UN = UW * (uy1 + 3 - uy0);
-    int cnt = 0;
-    for (int j0 = 0; j0 < UN; j0 += WAVE) {
-        const int j = j0 + lane, py = uy0 + j / UW, px = ux0 + j % UW;
-        bool in = false;
-#pragma unroll
-        for (int h = 0; h < 4; ++h) in = in || (abs(px - hyp[h].x) <= 2 && abs(py - hyp[h].y) <= 2);
-        in = in && j < UN;
-        const unsigned long long m = __ballot(in);
-        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        if (in) s_list[at] = (py + LP_BOX) * LP_BD + px + LP_BOX;
-        cnt += __popcll(m);
-    }
-    WAVE_SYNC();
-    int key = INT_MAX;
-    for (int i = lane; i < cnt; i += WAVE) {
-        const int idx = s_list[i], px = idx % LP_BD, py = idx / LP_BD;
-        const int s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl)
-                            : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl);
-        key = min(key, s * 1024 + idx);
-    }
-    key = wave_min_i(key);
-    int best_sad = key >> 10;
-    dme_mv best;
-    best.x = g.x + (key & 1023) % LP_BD - LP_BOX;
-    best.y = g.y + (key & 1023) / LP_BD - LP_BOX;
-    const bool fallback = (float)best_sad > 1.2f * pred_sad;
-    if (fallback) {  // uniform
-        // zero's window: in the box when |g| <= LP_BOX - 2, else staged apart
-        const bool zin = abs(g.x) <= LP_BOX - 2 && abs(g.y) <= LP_BOX - 2;
-        if (!zin) {
-            const uint8_t *const base[3] = {J.ref8[r], J.ref8[r], J.ref8[r]};
-            const int x0[3] = {b.xp - 2, 0, 0}, y0[3] = {b.yp - 2, 0, 0};
+        for (int h = 0; h < 4; ++h) {
+            hyp[h].x = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f)), 2 - LP_BOX,
+                              LP_BOX - 2);
+            hyp[h].y = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f)), 2 - LP_BOX,
+                              LP_BOX - 2);
+        }
+        const float pred_sad = expm1f(fminf(o[16 + r], 20.0f));
+        // picture block and box in one round of loads
+        {
+            const uint8_t *const base[3] = {J.pic8, J.ref8[r], J.ref8[r]};
+            const int x0[3] = {b.xp, b.xp + g.x - LP_BOX, 0}, y0[3] = {b.yp, b.yp + g.y - LP_BOX, 0};
+            const int ppad = (b.xl + 15) & ~15, RW = b.xl + 2 * LP_BOX, RH = b.yl + 2 * LP_BOX;
             B8Rect rc[3];
             bool fast[3];
-            rc[0] = b8_rect(base[0], bp, s_zero, ly.zp, x0[0], y0[0], b.xl + 4, b.yl + 4, b.xl + 4);
-            rc[1] = rc[2] = rc[0];
-            fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl + 4, b.xl + 4);
-            fast[1] = fast[2] = false;
-            b8_stage(rc, fast, 1, base, bp, W, H, x0, y0);
-            WAVE_SYNC();
+            rc[0] = b8_rect(base[0], bp, s_pic, ly.pp, x0[0], y0[0], b.xl, b.yl, ppad);
+            rc[1] = b8_rect(base[1], bp, s_box, ly.rp, x0[1], y0[1], RW, RH, RW);
+            rc[2] = rc[1];
+            fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl, ppad);
+            fast[1] = b8_inside(bp, H, x0[1], y0[1], RH, RW);
+            fast[2] = false;
+            b8_stage(rc, fast, 2, base, bp, W, H, x0, y0);
         }
-        int key2 = INT_MAX;
-        for (int i = lane; i < 106; i += WAVE) {
-            const bool gw = i < 81;
-            const dme_mv w = gw ? win_off(i, 4) : win_off(i - 81, 2);
-            int s, idx;
-            if (gw || zin) {
-                const int px = gw ? w.x : w.x - g.x, py = gw ? w.y : w.y - g.y;  // offsets from g
-                s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl)
-                          : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl);
-                idx = (py + LP_BOX) * LP_BD + px + LP_BOX;
-            } else {
-                s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl)
-                          : packed_sad_u8<true>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl);
-                idx = LP_BD * LP_BD + (i - 81);
+        WAVE_SYNC();
+        const bool whole = (b.xl & 15) == 0;
+        // union of the hypothesis windows: the positions of its bounding box that
+        // lie in a window, compacted into a list (box raster indices), then one
+        // SAD per listed position
+        int ux0 = hyp[0].x, ux1 = hyp[0].x, uy0 = hyp[0].y, uy1 = hyp[0].y;
+#pragma unroll
+        for (int h = 1; h < 4; ++h) {
+            ux0 = min(ux0, hyp[h].x);
+            ux1 = max(ux1, hyp[h].x);
+            uy0 = min(uy0, hyp[h].y);
+            uy1 = max(uy1, hyp[h].y);
+        }
+        ux0 -= 2;
+        uy0 -= 2;
+        const int UW = ux1 + 3 - ux0, UN = UW * (uy1 + 3 - uy0);
+        int cnt = 0;
+        for (int j0 = 0; j0 < UN; j0 += WAVE) {
+            const int j = j0 + lane, py = uy0 + j / UW, px = ux0 + j % UW;
+            bool in = false;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) in = in || (abs(px - hyp[h].x) <= 2 && abs(py - hyp[h].y) <= 2);
+            in = in && j < UN;
+            const unsigned long long m = __ballot(in);
+            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (in) s_list[at] = (py + LP_BOX) * LP_BD + px + LP_BOX;
+            cnt += __popcll(m);
+        }
+        WAVE_SYNC();
+        int key = INT_MAX;
+        for (int i = lane; i < cnt; i += WAVE) {
+            const int idx = s_list[i], px = idx % LP_BD, py = idx / LP_BD;
+            const int s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl)
+                                : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl);
+            key = min(key, s * 1024 + idx);
+        }
+        key = wave_min_i(key);
+        int best_sad = key >> 10;
+        dme_mv best;
+        best.x = g.x + (key & 1023) % LP_BD - LP_BOX;
+        best.y = g.y + (key & 1023) / LP_BD - LP_BOX;
+        const bool fallback = (float)best_sad > 1.2f * pred_sad;
+        if (fallback) {  // uniform
+            // zero's window: in the box when |g| <= LP_BOX - 2, else staged apart
+            const bool zin = abs(g.x) <= LP_BOX - 2 && abs(g.y) <= LP_BOX - 2;
+            if (!zin) {
+                const uint8_t *const base[3] = {J.ref8[r], J.ref8[r], J.ref8[r]};
+                const int x0[3] = {b.xp - 2, 0, 0}, y0[3] = {b.yp - 2, 0, 0};
+                B8Rect rc[3];
+                bool fast[3];
+                rc[0] = b8_rect(base[0], bp, s_zero, ly.zp, x0[0], y0[0], b.xl + 4, b.yl + 4, b.xl + 4);
+                rc[1] = rc[2] = rc[0];
+                fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl + 4, b.xl + 4);
+                fast[1] = fast[2] = false;
+                b8_stage(rc, fast, 1, base, bp, W, H, x0, y0);
+                WAVE_SYNC();
             }
-            key2 = min(key2, s * 1024 + idx);
-        }
-        key2 = wave_min_i(key2);
-        if ((key2 >> 10) < best_sad) {
-            const int idx = key2 & 1023;
-            best_sad = key2 >> 10;
-            if (idx < LP_BD * LP_BD) {
-                best.x = g.x + idx % LP_BD - LP_BOX;
-                best.y = g.y + idx / LP_BD - LP_BOX;
-            } else {
-                best = win_off(idx - LP_BD * LP_BD, 2);
+            int key2 = INT_MAX;
+            for (int i = lane; i < 106; i += WAVE) {
+                const bool gw = i < 81;
+                const dme_mv w = gw ? win_off(i, 4) : win_off(i - 81, 2);
+                int s, idx;
+                if (gw || zin) {
+                    const int px = gw ? w.x : w.x - g.x, py = gw ? w.y : w.y - g.y;  // offsets from g
+                    s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl)
+                              : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl);
+                    idx = (py + LP_BOX) * LP_BD + px + LP_BOX;
+                } else {
+                    s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl)
+                              : packed_sad_u8<true>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl);
+                    idx = LP_BD * LP_BD + (i - 81);
+                }
+                key2 = min(key2, s * 1024 + idx);
+            }
+            key2 = wave_min_i(key2);
+            if ((key2 >> 10) < best_sad) {
+                const int idx = key2 & 1023;
+                best_sad = key2 >> 10;
+                if (idx < LP_BD * LP_BD) {
+                    best.x = g.x + idx % LP_BD - LP_BOX;
+                    best.y = g.y + idx / LP_BD - LP_BOX;
+                } else {
+                    best = win_off(idx - LP_BD * LP_BD, 2);
+                }
             }
         }
-    }
-    if (lane == 0) {
-        V[k] = best;
-        C[k].SAD = (float)best_sad;
-        atomicAdd(&stats[2 * J.lp_slot], 1ull);
-        if (fallback) atomicAdd(&stats[2 * J.lp_slot + 1], 1ull);
+        if (lane == 0) {
+            V[k] = best;
+            C[k].SAD = (float)best_sad;
+            atomicAdd(&stats[2 * J.lp_slot], 1ull);
+            if (fallback) atomicAdd(&stats[2 * J.lp_slot + 1], 1ull);
+        }
     }
 }
 
@@ -2864,22 +2879,24 @@ __global__ __launch_bounds__(256) void k_lp_pel_costs(const JobDesc *__restrict_
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total) {
-    const int gi = xcd_block() * 4 + wave_id();
-    if (gi >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
-    const JobDesc &J = jobs[tk.job];
-    const int comp = tk.ref, k = gi - tk.offset;
-    const int xb = k % J.xnb, yb = k / J.xnb;
-    int dc = 0;
-    if (comp == 0) {
-        const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
-        if (!b.empty()) dc = wave_block_dc(J.pic[0], J.W, b);
-    } else {
-        const int16_t *p = comp == 1 ? J.cu : J.cv;
-        const Blk b = block_limits(J.cblen, J.cbsep, J.cboff, J.CW, J.CH, xb, yb);
-        if (p && !b.empty()) dc = wave_block_dc(p, J.CW, b);
+    const int g0 = (xcd_block() * 4 + wave_id()) * PW_BLOCK_DC, g1 = min(g0 + PW_BLOCK_DC, total);
+    for (int gi = g0; gi < g1; ++gi) {
+        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
+        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const JobDesc &J = jobs[tk.job];
+        const int comp = tk.ref, k = gi - tk.offset;
+        const int xb = k % J.xnb, yb = k / J.xnb;
+        int dc = 0;
+        if (comp == 0) {
+            const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
+            if (!b.empty()) dc = wave_block_dc(J.pic[0], J.W, b);
+        } else {
+            const int16_t *p = comp == 1 ? J.cu : J.cv;
+            const Blk b = block_limits(J.cblen, J.cbsep, J.cboff, J.CW, J.CH, xb, yb);
+            if (p && !b.empty()) dc = wave_block_dc(p, J.CW, b);
+        }
+        if ((threadIdx.x & 63) == 0) J.dct[comp][k] = (int16_t)dc;
     }
-    if ((threadIdx.x & 63) == 0) J.dct[comp][k] = (int16_t)dc;
 }
 
 __global__ __launch_bounds__(256) void k_dc_smooth(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
@@ -3047,7 +3064,7 @@ hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntas
 
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mode_l2, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    hipLaunchKernelGGL(k_mode_l2, dim3((total + 4 * PW_MODE_L2 - 1) / (4 * PW_MODE_L2)), dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 
@@ -3070,7 +3087,7 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(k_mode_tables, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    hipLaunchKernelGGL(k_mode_tables, dim3((total + 4 * PW_MODE_TABLES - 1) / (4 * PW_MODE_TABLES)), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }
 
@@ -3081,7 +3098,7 @@ hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStre
 
 hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_block_dc, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    hipLaunchKernelGGL(k_block_dc, dim3((total + 4 * PW_BLOCK_DC - 1) / (4 * PW_BLOCK_DC)), dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 
@@ -3114,7 +3131,7 @@ hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, 
     ly.zrows = max_blen_y + 4;
     const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 100);
     if (lds > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_lp_refine, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, out4, stats,
+    hipLaunchKernelGGL(k_lp_refine, dim3((total + 4 * PW_LP_REFINE - 1) / (4 * PW_LP_REFINE)), dim3(256), lds, st, jobs, tasks, ntasks, total, out4, stats,
                        ly);
     return hipGetLastError();
 }
