@@ -563,6 +563,27 @@ __device__ __forceinline__ int packed_sad_u8(const uint32_t *__restrict__ pic, i
     return (int)acc;
 }
 
+// packed_sad_u8 for a whole 16 x 16 block with compile-time row pitches:
+// every LDS read takes an immediate offset and the rows need no loop
+// control.
+template <int PP, int RP>
+__device__ __forceinline__ int sad16x16_u8(const uint32_t *__restrict__ pic, const uint32_t *__restrict__ reg, int bo,
+                                           int row0) {
+    const uint32_t *rb = reg + row0 * RP + (bo >> 2);
+    const unsigned sh = bo & 3;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int y = 0; y < 16; ++y) {
+        const uint32_t *pr = pic + y * PP, *rr = rb + y * RP;
+        const uint32_t r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+        acc = __builtin_amdgcn_sad_u8(pr[0], __builtin_amdgcn_alignbyte(r1, r0, sh), acc);
+        acc = __builtin_amdgcn_sad_u8(pr[1], __builtin_amdgcn_alignbyte(r2, r1, sh), acc);
+        acc = __builtin_amdgcn_sad_u8(pr[2], __builtin_amdgcn_alignbyte(r3, r2, sh), acc);
+        acc = __builtin_amdgcn_sad_u8(pr[3], __builtin_amdgcn_alignbyte(r4, r3, sh), acc);
+    }
+    return (int)acc;
+}
+
 // ---------------------------------------------------------------------------
 // Memory-level parallelism.  A wavefront step is a short chain of dependent
 // operations, so its global loads must be issued together, not one per loop
@@ -959,13 +980,16 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
         }
         WAVE_SYNC();
         const bool whole = (b.xl & (U8 ? 15 : 7)) == 0;  // no partial dword chunk
+        // the common level-0 case (16 x 16 blocks, radius 1) with fixed pitches
+        const bool b16 = U8 && b.xl == 16 && b.yl == 16 && ly.pp == 4 && ly.rp == 7;
         for (int c = lane; c < ntab * TS; c += WAVE) {
             const int t = c >= TS ? 1 : 0, e = c - t * TS;
             const int oy = e / D, ox = e - oy * D;  // region offset of the candidate
             const uint32_t *reg = t ? s_reg1 : s_reg0;
             if (U8)
-                s_tab[c] = whole ? packed_sad_u8<false>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
-                                 : packed_sad_u8<true>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
+                s_tab[c] = b16 ? sad16x16_u8<4, 7>(s_pic, reg, ox, oy)
+                               : (whole ? packed_sad_u8<false>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
+                                        : packed_sad_u8<true>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl));
             else
                 s_tab[c] = whole ? packed_sad<false>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl)
                                  : packed_sad<true>(s_pic, ly.pp, reg, ly.rp, ox, oy, b.xl, b.yl);
@@ -1578,9 +1602,11 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
             const int mx = cqx + ((lane % Q - rad) << sq), my = cqy + ((lane / Q - rad) << sq);
             const int fx = mx & 3, fy = my & 3;
             const uint32_t *src = s_q + (fy * 4 + fx) * plane;
-            rec[lane] = (b.xl & 15) == 0
-                            ? packed_sad_u8<false>(s_pic, pp, src, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl)
-                            : packed_sad_u8<true>(s_pic, pp, src, qd, (mx >> 2) - qx0, (my >> 2) - qy0, b.xl, b.yl);
+            const int bo = (mx >> 2) - qx0, r0 = (my >> 2) - qy0;
+            rec[lane] = (b.xl == 16 && b.yl == 16 && pp == 4 && qd == 5)
+                            ? sad16x16_u8<4, 5>(s_pic, src, bo, r0)
+                            : ((b.xl & 15) == 0 ? packed_sad_u8<false>(s_pic, pp, src, qd, bo, r0, b.xl, b.yl)
+                                                : packed_sad_u8<true>(s_pic, pp, src, qd, bo, r0, b.xl, b.yl));
         }
     }
     if (lane == 0) {
