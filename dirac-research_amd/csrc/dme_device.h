@@ -81,6 +81,28 @@ struct PlaneTask {
     int in_w, ow, oh, offset;
 };
 
+// One component of one picture being motion compensated (k_mc / k_mc_pad).
+// Built on the host, read by every workgroup of the component's tiles.
+struct McComp {
+    const int16_t DME_G *up[2];     // up-converted reference planes, 2pw x 2ph
+    int16_t DME_G *out;             // array compensated in place, lx x ly
+    const dme_mv DME_G *vec[2];     // MvData of the picture, xnb x ynb blocks
+    const int DME_G *mode;
+    const int DME_G *split;         // xsb x ysb superblocks
+    const int16_t DME_G *dc;        // this component's DC values
+    const signed char DME_G *hw;    // OBMC weights across: [split][class][hpitch]
+    const signed char DME_G *vw;    // OBMC weights down: [class][yblen]
+    int lx, ly, pw, ph;
+    int xbsep, ybsep, xoff, yoff, xov, yblen, hpitch;
+    int nbx, nby;                   // blocks in use across / down (NUM_USED_BLKS)
+    int xnb, xsb;
+    int shift;                      // vector shift (1 for 4:2:0 chroma)
+    int prec;                       // 0 pel, 1 half, 2 quarter, 3 eighth
+    int direction;                  // 0 add, 1 subtract
+    int mmx;                        // 16-bit rounding of the MMX add path
+    int tiles_x, offset;            // tiles across; first tile in the launch
+};
+
 // One unit of a batched launch: a (job, reference) pair covering `count`
 // consecutive work items starting at `offset` of the launch's index space.
 struct Task {

@@ -31,6 +31,7 @@ namespace dme {
 
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st);
 hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
@@ -116,6 +117,8 @@ struct Picture {
     int pyr_levels = 0;             // levels built so far
     DevBuf up, qp;  // up-converted plane (built on request) and the quarter-pel planes
     bool up_ready = false, qp_ready = false;
+    DevBuf upc[2];  // up-converted chroma planes (motion compensation)
+    bool upc_ready = false;
     DevBuf b8;      // byte plane with margins (k_bytes)
     int bp = 0;     // its row pitch
     bool b8_ready = false;
@@ -175,6 +178,8 @@ struct dme_ctx {
     std::map<int, std::unique_ptr<dme::JobState>> jobs;
     dme::DevBuf d_flag;
     dme::DevBuf d_lflags;  // per batch position: the device lambda maps need the host (k_lambda)
+    dme::DevBuf d_mc;      // motion compensation: descriptors, weights, uploaded MvData
+    dme::DevBuf d_mc_planes;  // host-array compensation: the arrays on the device
     // Two lanes (stream + buffers); stages queue work on the current lane.
     // Large batches run as two halves on the two lanes, so that one half's
     // host work and latency-bound wavefronts overlap the other half's
@@ -209,12 +214,12 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_BYTES, KK_LAMBDA, KK_COUNT
+    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda"};
+    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -1156,6 +1161,344 @@ extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_m
     return DME_OK;
 }
 
+// ---- motion compensation ------------------------------------------------------
+
+namespace dme {
+
+// HWts / VWts of MotionCompensator::CreateBlock (mot_comp.cpp:742-806): the
+// 1-D OBMC window of length len and separation sep.  `full` ramps the leading
+// edge (a unit inside the picture); otherwise the leading edge is flat (a unit
+// at the picture's left or top edge).  CalculateWeights (:707-724) builds the
+// nine 2-D arrays as products of such windows, the trailing-edge variants by
+// mirroring, so the kernels keep the 1-D factors only.
+static void mc_window(int len, int sep, bool full, signed char *w) {
+    const int off = (len - sep) / 2;
+    std::vector<int> h((size_t)len, 8);
+    if (off != 1) {
+        for (int i = 0; i < 2 * off; ++i) {
+            h[i] = 1 + (6 * i + off - 1) / (2 * off - 1);
+            h[i + sep] = 8 - h[i];
+        }
+    } else {
+        h[0] = h[1 + sep] = 3;
+        h[1] = h[sep] = 5;
+    }
+    if (!full)
+        for (int i = 0; i < 2 * off; ++i) h[i] = 8;
+    for (int i = 0; i < len; ++i) w[i] = (signed char)h[i];
+}
+
+// Weights of one component: across, per split mode (a unit of 4, 2 or 1
+// blocks, ReConfig mot_comp.cpp:197-236) and class (left edge, inside, right
+// edge); down, per class (top, inside, bottom).
+struct McTables {
+    int hpitch = 0, yblen = 0;
+    std::vector<signed char> hw, vw;
+};
+
+static void mc_tables(int xblen, int yblen, int xbsep, int ybsep, McTables &t) {
+    const int ov = xblen - xbsep;
+    t.hpitch = 4 * xbsep + ov;
+    t.hw.assign((size_t)9 * t.hpitch, 0);
+    for (int s = 0; s < 3; ++s) {
+        const int k = 4 >> s, len = k * xbsep + ov, sep = k * xbsep;
+        signed char *row = &t.hw[(size_t)s * 3 * t.hpitch];
+        mc_window(len, sep, false, row);
+        mc_window(len, sep, true, row + t.hpitch);
+        for (int i = 0; i < len; ++i) row[2 * t.hpitch + i] = row[len - 1 - i];
+    }
+    t.yblen = yblen;
+    t.vw.assign((size_t)3 * yblen, 0);
+    mc_window(yblen, ybsep, false, &t.vw[0]);
+    mc_window(yblen, ybsep, true, &t.vw[yblen]);
+    for (int i = 0; i < yblen; ++i) t.vw[2 * (size_t)yblen + i] = t.vw[yblen - 1 - i];
+}
+
+// NUM_USED_BLKS (mot_comp.cpp:54)
+static int mc_used_blocks(int w, int sep, int len) { return (w + sep + (len - sep) / 2 - 1) / sep; }
+
+static size_t mc_align(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Validated job: geometry, references and the MvData source.
+struct McJob {
+    const dme_mc_job *j;
+    Geometry g;
+    Picture *ref[2];
+    JobState *est;  // device MvData of the context's estimate, or null (host MvData)
+};
+
+// Queues the compensation of jobs (streams idle, timing started) and waits.
+static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
+    std::vector<McJob> mj((size_t)njobs);
+    for (int i = 0; i < njobs; ++i) {
+        const dme_mc_job &j = jobs[i];
+        McJob &m = mj[i];
+        m.j = &j;
+        if (j.reserved != 0) return set_error(DME_ERR_INVALID, "dme_mc_job.reserved must be zero");
+        if (j.nrefs < 1 || j.nrefs > 2) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
+        if (j.direction != DME_MC_ADD && j.direction != DME_MC_SUBTRACT)
+            return set_error(DME_ERR_INVALID, "direction must be DME_MC_ADD or DME_MC_SUBTRACT");
+        if (j.params.mv_precision < 0 || j.params.mv_precision > 3)
+            return set_error(DME_ERR_INVALID, "mv_precision must be 0..3 for motion compensation");
+        dme_params q = j.params;
+        if (q.mv_precision == 3) q.mv_precision = 2;  // same geometry; eighth pel only changes the sampling
+        DME_TRY(make_geometry(q, j.width, j.height, m.g));
+        if (m.g.xnb != 4 * m.g.xsb || m.g.ynb != 4 * m.g.ysb) return set_error(DME_ERR_INVALID, "block grid");
+        for (int r = 0; r < j.nrefs; ++r) {
+            Picture *P = find_pic(ctx, j.refs[r]);
+            if (!P) return set_error(DME_ERR_INVALID, "unknown reference picture " + std::to_string(j.refs[r]));
+            if (P->W != j.width || P->H != j.height)
+                return set_error(DME_ERR_INVALID, "reference picture size differs from the picture's");
+            if (!P->u || !P->v) return set_error(DME_ERR_INVALID, "reference picture has no chroma planes");
+            m.ref[r] = P;
+        }
+        if (j.nrefs == 1) m.ref[1] = m.ref[0];
+        const int pw[3] = {m.g.W, m.g.CW, m.g.CW}, ph[3] = {m.g.H, m.g.CH, m.g.CH};
+        for (int c = 0; c < 3; ++c)
+            if (!j.planes[c].data || j.planes[c].width < pw[c] || j.planes[c].height < ph[c])
+                return set_error(DME_ERR_INVALID, "plane " + std::to_string(c) + " is NULL or smaller than the picture");
+        m.est = nullptr;
+        if (j.mv) {
+            if (!j.mv->vectors[0] || (j.nrefs == 2 && !j.mv->vectors[1]) || !j.mv->mode || !j.mv->dc[0] ||
+                !j.mv->dc[1] || !j.mv->dc[2] || !j.mv->sb_split)
+                return set_error(DME_ERR_INVALID, "MvData needs vectors, mode, dc[3] and sb_split");
+        } else {
+            auto it = ctx->jobs.find(j.pic_num);
+            if (it == ctx->jobs.end() || it->second->stage < 4)
+                return set_error(DME_ERR_STATE, "no completed estimate for picture " + std::to_string(j.pic_num));
+            JobState *J = it->second.get();
+            if (J->nrefs != j.nrefs || J->g.xnb != m.g.xnb || J->g.ynb != m.g.ynb || J->g.xsb != m.g.xsb)
+                return set_error(DME_ERR_INVALID, "the estimate's references or block grid differ from the job's");
+            m.est = J;
+        }
+    }
+    for (int i = 0; i < njobs; ++i)
+        for (int k = 0; k < i; ++k)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d)
+                    if (jobs[i].planes[c].data == jobs[k].planes[d].data)
+                        return set_error(DME_ERR_INVALID, "two jobs compensate the same array");
+
+    // up-converted reference planes (Picture::UpData, built once and kept)
+    std::vector<PlaneTask> pts;
+    int ptotal = 0;
+    auto add_task = [&](const int16_t *in, int16_t *out, int w, int h) {
+        PlaneTask t;
+        t.in = in;
+        t.out = out;
+        t.in_w = w;
+        t.ow = w;
+        t.oh = h;
+        t.offset = ptotal;
+        ptotal += ((w + 63) / 64) * ((h + 3) / 4);
+        pts.push_back(t);
+    };
+    std::vector<Picture *> luma_up, chroma_up;  // planes to build, each once
+    for (auto &m : mj)
+        for (int r = 0; r < m.j->nrefs; ++r) {
+            Picture *P = m.ref[r];
+            if (!P->up_ready && std::find(luma_up.begin(), luma_up.end(), P) == luma_up.end()) {
+                DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
+                add_task(P->y, P->up.as<int16_t>(), P->W, P->H);
+                luma_up.push_back(P);
+            }
+            if (!P->upc_ready && std::find(chroma_up.begin(), chroma_up.end(), P) == chroma_up.end()) {
+                const int cw = P->W / 2, ch = P->H / 2;
+                for (int c = 0; c < 2; ++c) {
+                    DME_TRY(P->upc[c].ensure(sizeof(int16_t) * 4 * (size_t)cw * ch));
+                    add_task(c ? P->v : P->u, P->upc[c].as<int16_t>(), cw, ch);
+                }
+                chroma_up.push_back(P);
+            }
+        }
+    int launches = 0;
+    if (!pts.empty()) {
+        DME_TRY(upload(ctx, ctx->L->d_planes, pts.data(), sizeof(PlaneTask) * pts.size()));
+        DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches, [&] {
+            return launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), (int)pts.size(), ptotal, ctx->L->stream);
+        }));
+        for (Picture *P : luma_up) P->up_ready = true;
+        for (Picture *P : chroma_up) P->upc_ready = true;
+    }
+
+    // one device blob: component descriptors (main launch, then padding),
+    // weight tables, host MvData
+    std::vector<McTables> tabs((size_t)njobs * 2);
+    size_t ncomp = 3 * (size_t)njobs, npad = 0;
+    for (int i = 0; i < njobs; ++i) {
+        const Geometry &g = mj[i].g;
+        mc_tables(g.blen[2][0], g.blen[2][1], g.bsep[2][0], g.bsep[2][1], tabs[2 * i]);
+        mc_tables(g.cblen[0], g.cblen[1], g.cbsep[0], g.cbsep[1], tabs[2 * i + 1]);
+        if (jobs[i].direction == DME_MC_ADD)
+            for (int c = 0; c < 3; ++c)
+                if (jobs[i].planes[c].width > (c ? g.CW : g.W) || jobs[i].planes[c].height > (c ? g.CH : g.H)) ++npad;
+    }
+    size_t bytes = mc_align(sizeof(McComp) * (ncomp + npad));
+    std::vector<size_t> tab_off(tabs.size()), mv_off((size_t)njobs, 0);
+    for (size_t t = 0; t < tabs.size(); ++t) {
+        tab_off[t] = bytes;
+        bytes += mc_align(tabs[t].hw.size()) + mc_align(tabs[t].vw.size());
+    }
+    for (int i = 0; i < njobs; ++i) {
+        if (mj[i].est) continue;
+        const size_t B = (size_t)mj[i].g.xnb * mj[i].g.ynb, S = (size_t)mj[i].g.xsb * mj[i].g.ysb;
+        mv_off[i] = bytes;
+        bytes += mc_align(sizeof(dme_mv) * B) * 2 + mc_align(sizeof(int32_t) * B) + 3 * mc_align(sizeof(int16_t) * B) +
+                 mc_align(sizeof(int32_t) * S);
+    }
+    if (ctx->d_mc.n < bytes) {
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        DME_TRY(ctx->d_mc.ensure(bytes));
+    }
+    char *dev = ctx->d_mc.as<char>();
+    std::vector<char> blob(bytes, 0);
+    for (size_t t = 0; t < tabs.size(); ++t) {
+        std::memcpy(&blob[tab_off[t]], tabs[t].hw.data(), tabs[t].hw.size());
+        std::memcpy(&blob[tab_off[t] + mc_align(tabs[t].hw.size())], tabs[t].vw.data(), tabs[t].vw.size());
+    }
+    McComp *cs = reinterpret_cast<McComp *>(blob.data());
+    int total = 0, ptot = 0;
+    size_t pi = ncomp;
+    for (int i = 0; i < njobs; ++i) {
+        const dme_mc_job &j = jobs[i];
+        const McJob &m = mj[i];
+        const Geometry &g = m.g;
+        const size_t B = (size_t)g.xnb * g.ynb, S = (size_t)g.xsb * g.ysb;
+        const dme_mv *vec[2];
+        const int *mode, *split;
+        const int16_t *dc[3];
+        if (m.est) {
+            vec[0] = m.est->vec[0].as<dme_mv>();
+            vec[1] = (j.nrefs == 2 ? m.est->vec[1] : m.est->vec[0]).as<dme_mv>();
+            mode = m.est->mode.as<int>();
+            split = m.est->split.as<int>();
+            for (int c = 0; c < 3; ++c) dc[c] = m.est->dc[c].as<int16_t>();
+        } else {
+            size_t o = mv_off[i];
+            const size_t vb = mc_align(sizeof(dme_mv) * B);
+            std::memcpy(&blob[o], j.mv->vectors[0], sizeof(dme_mv) * B);
+            std::memcpy(&blob[o + vb], j.mv->vectors[j.nrefs == 2 ? 1 : 0], sizeof(dme_mv) * B);
+            vec[0] = reinterpret_cast<const dme_mv *>(dev + o);
+            vec[1] = reinterpret_cast<const dme_mv *>(dev + o + vb);
+            o += 2 * vb;
+            std::memcpy(&blob[o], j.mv->mode, sizeof(int32_t) * B);
+            mode = reinterpret_cast<const int *>(dev + o);
+            o += mc_align(sizeof(int32_t) * B);
+            for (int c = 0; c < 3; ++c) {
+                std::memcpy(&blob[o], j.mv->dc[c], sizeof(int16_t) * B);
+                dc[c] = reinterpret_cast<const int16_t *>(dev + o);
+                o += mc_align(sizeof(int16_t) * B);
+            }
+            std::memcpy(&blob[o], j.mv->sb_split, sizeof(int32_t) * S);
+            split = reinterpret_cast<const int *>(dev + o);
+        }
+        for (int c = 0; c < 3; ++c) {
+            McComp &C = cs[3 * i + c];
+            const McTables &T = tabs[2 * i + (c ? 1 : 0)];
+            const size_t to = tab_off[2 * i + (c ? 1 : 0)];
+            const int xblen = c ? g.cblen[0] : g.blen[2][0], yblen = c ? g.cblen[1] : g.blen[2][1];
+            const int xbsep = c ? g.cbsep[0] : g.bsep[2][0], ybsep = c ? g.cbsep[1] : g.bsep[2][1];
+            const Picture *R0 = m.ref[0], *R1 = m.ref[1];
+            C.up[0] = c == 0 ? R0->up.as<int16_t>() : R0->upc[c - 1].as<int16_t>();
+            C.up[1] = c == 0 ? R1->up.as<int16_t>() : R1->upc[c - 1].as<int16_t>();
+            C.out = j.planes[c].data;
+            C.vec[0] = vec[0];
+            C.vec[1] = vec[1];
+            C.mode = mode;
+            C.split = split;
+            C.dc = dc[c];
+            C.hw = reinterpret_cast<const signed char *>(dev + to);
+            C.vw = reinterpret_cast<const signed char *>(dev + to + mc_align(T.hw.size()));
+            C.lx = j.planes[c].width;
+            C.ly = j.planes[c].height;
+            C.pw = c ? g.CW : g.W;
+            C.ph = c ? g.CH : g.H;
+            C.xbsep = xbsep;
+            C.ybsep = ybsep;
+            C.xoff = (xblen - xbsep) / 2;
+            C.yoff = (yblen - ybsep) / 2;
+            C.xov = xblen - xbsep;
+            C.yblen = yblen;
+            C.hpitch = T.hpitch;
+            C.nbx = std::min(mc_used_blocks(C.pw, xbsep, xblen), g.xnb);
+            C.nby = std::min(mc_used_blocks(C.ph, ybsep, yblen), g.ynb);
+            C.xnb = g.xnb;
+            C.xsb = g.xsb;
+            C.shift = c ? 1 : 0;
+            C.prec = j.params.mv_precision;
+            C.direction = j.direction;
+            C.mmx = j.params.fp_semantics == DME_FP_MMX;
+            C.tiles_x = (C.lx + 63) / 64;
+            C.offset = total;
+            total += C.tiles_x * ((C.ly + 3) / 4);
+            if (j.direction == DME_MC_ADD && (C.lx > C.pw || C.ly > C.ph)) {
+                cs[pi] = C;
+                cs[pi].offset = ptot;
+                ptot += C.tiles_x * ((C.ly + 3) / 4);
+                ++pi;
+            }
+        }
+    }
+    DME_TRY(upload(ctx, ctx->d_mc, blob.data(), bytes));
+    const McComp *dcs = ctx->d_mc.as<McComp>();
+    DME_TRY(timed_launch(ctx, KK_MC, launches,
+                         [&] { return launch_mc(dcs, (int)ncomp, total, false, ctx->L->stream); }));
+    if (npad)
+        DME_TRY(timed_launch(ctx, KK_MC_PAD, launches,
+                             [&] { return launch_mc(dcs + ncomp, (int)npad, ptot, true, ctx->L->stream); }));
+    return end_timing(ctx, launches);
+}
+
+}  // namespace dme
+
+extern "C" int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
+    CTX_CHECK(ctx);
+    if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
+    DME_TRY(begin_timing(ctx));
+    return mc_run(ctx, jobs, njobs);
+}
+
+extern "C" int dme_motion_compensate(dme_ctx *ctx, const dme_params *params, int width, int height,
+                                     const dme_medata *mv, const int *refs, int nrefs, int direction,
+                                     const dme_mc_plane planes[3]) {
+    CTX_CHECK(ctx);
+    if (!params || !mv || !refs || !planes) return set_error(DME_ERR_INVALID, "NULL argument");
+    if (nrefs < 1 || nrefs > 2) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
+    size_t n[3], total = 0;
+    for (int c = 0; c < 3; ++c) {
+        if (!planes[c].data || planes[c].width <= 0 || planes[c].height <= 0)
+            return set_error(DME_ERR_INVALID, "plane " + std::to_string(c) + " is NULL or empty");
+        n[c] = (size_t)planes[c].width * planes[c].height;
+        total += mc_align(sizeof(int16_t) * n[c]);
+    }
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(ctx->d_mc_planes.ensure(total));
+    dme_mc_job j;
+    std::memset(&j, 0, sizeof(j));
+    j.pic_num = -1;
+    j.width = width;
+    j.height = height;
+    j.nrefs = nrefs;
+    for (int r = 0; r < nrefs; ++r) j.refs[r] = refs[r];
+    j.direction = direction;
+    j.params = *params;
+    j.mv = mv;
+    char *d = ctx->d_mc_planes.as<char>();
+    for (int c = 0; c < 3; ++c) {
+        j.planes[c].data = reinterpret_cast<int16_t *>(d);
+        j.planes[c].width = planes[c].width;
+        j.planes[c].height = planes[c].height;
+        HIP_TRY(hipMemcpyAsync(d, planes[c].data, sizeof(int16_t) * n[c], hipMemcpyHostToDevice, ctx->L->stream));
+        d += mc_align(sizeof(int16_t) * n[c]);
+    }
+    DME_TRY(mc_run(ctx, &j, 1));
+    for (int c = 0; c < 3; ++c)
+        HIP_TRY(hipMemcpyAsync(planes[c].data, j.planes[c].data, sizeof(int16_t) * n[c], hipMemcpyDeviceToHost,
+                               ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    return DME_OK;
+}
+
 extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int16_t *out, size_t out_elems) {
     CTX_CHECK(ctx);
     Picture *P = find_pic(ctx, pic_num);
@@ -1211,6 +1554,7 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
         if (pic_num < 0 || kv.first == pic_num) {
             kv.second->pyr_levels = 0;
             kv.second->up_ready = false;
+            kv.second->upc_ready = false;
             kv.second->qp_ready = false;
             kv.second->b8_ready = false;
         }

@@ -2957,6 +2957,157 @@ hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Motion compensation: MotionCompensator::CompensatePicture and
+// CompensateComponent (mot_comp.cpp:131-539) as a gather.  The reference walks
+// the blocks of a row, scatters each block's weighted prediction into a strip
+// of rows and flushes the strip; a sample's final sum is the sum over the units
+// whose OBMC windows cover it -- at most two across (the overlap never exceeds
+// a separation) by two down.  So each thread finds those units for its sample
+// and sums their contributions directly: every sample is read and written
+// once, no strip, no atomics.  The reference sums in 16-bit lanes that wrap;
+// wrapping addition is order-independent, so an int sum truncated to 16 bits
+// at the end is exact.  Workgroup = 64 x 4 samples, one row per wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mc_bchk(int v, int max) { return v < 0 ? 0 : (v >= max ? max - 1 : v); }
+
+// BlockPixelPred (mot_comp.cpp:846-1230) for one sample at (x, y): the
+// reads clamp (BChk) to [0, 2 size - 2] of the up-converted plane; the
+// in-bounds branches of the reference give the same values.
+__device__ __forceinline__ int mc_pred(const int16_t DME_G *__restrict__ up, int uw, int tx, int ty, int prec, int x,
+                                       int y, dme_mv mv) {
+    if (prec <= 1) {
+        const int ux = prec == 0 ? 2 * (x + mv.x) : 2 * x + mv.x;
+        const int uy = prec == 0 ? 2 * (y + mv.y) : 2 * y + mv.y;
+        return up[(size_t)mc_bchk(uy, ty) * uw + mc_bchk(ux, tx)];
+    }
+    const int sh = prec - 1, one = 1 << sh;  // quarter: half-pel plane + 1 bit; eighth: + 2 bits
+    const int ux = 2 * x + (mv.x >> sh), uy = 2 * y + (mv.y >> sh);
+    const int fx = mv.x & (one - 1), fy = mv.y & (one - 1);
+    const int x0 = mc_bchk(ux, tx), x1 = mc_bchk(ux + 1, tx);
+    const int16_t DME_G *r0 = up + (size_t)mc_bchk(uy, ty) * uw;
+    const int16_t DME_G *r1 = up + (size_t)mc_bchk(uy + 1, ty) * uw;
+    const int v = (one - fx) * (one - fy) * r0[x0] + fx * (one - fy) * r0[x1] + (one - fx) * fy * r1[x0] +
+                  fx * fy * r1[x1];
+    return (v + (1 << (2 * sh - 1))) >> (2 * sh);
+}
+
+// Split mode as the reference's switch reads it: 0 and 1, anything else 2.
+__device__ __forceinline__ int mc_split(int s) { return (s == 0 || s == 1) ? s : 2; }
+
+__device__ __forceinline__ int mc_comp_of(const McComp *__restrict__ cs, int ncs, int tile) {
+    int lo = 0, hi = ncs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cs[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_mc(const McComp *__restrict__ cs, int ncs, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    const McComp &C = cs[mc_comp_of(cs, ncs, tile)];
+    const int local = tile - C.offset;
+    const int x = (local % C.tiles_x) * 64 + (threadIdx.x & 63);
+    const int y = (local / C.tiles_x) * 4 + wave_id();
+    if (x >= C.lx || y >= C.ly) return;
+    int16_t DME_G *o = C.out + (size_t)y * C.lx + x;
+    if (x >= C.pw || y >= C.ph) {
+        // subtracting zeroes the padding (mot_comp.cpp:449-455, 511-522);
+        // adding replicates the last column and row, in k_mc_pad
+        if (C.direction == DME_MC_SUBTRACT) *o = 0;
+        return;
+    }
+    const int uw = 2 * C.pw, tx = 2 * C.pw - 1, ty = 2 * C.ph - 1;
+    const int r = (y + C.yoff) / C.ybsep;  // the block row whose separation holds y
+    const int c = (x + C.xoff) / C.xbsep;  // the block column whose separation holds x
+    int S = 0;
+    for (int k = 0; k < 2; ++k) {
+        const int rb = r - 1 + k;
+        if (rb < 0 || rb >= C.nby) continue;
+        const int py = rb * C.ybsep - C.yoff, vy = y - py;
+        if (vy >= C.yblen) continue;
+        // weight class down (mot_comp.cpp:339-371): top, inside, bottom
+        const int ycls = py < 0 ? 0 : (rb + 1 < C.nby ? 1 : 2);
+        const int V = C.vw[ycls * C.yblen + vy];
+        const int DME_G *srow = C.split + (size_t)(rb >> 2) * C.xsb;
+        // units across: the one holding block c (if in use), then the one before
+        int ub[2] = {-1, -1}, sp[2] = {0, 0}, pb;
+        if (c < C.nbx) {
+            const int s1 = mc_split(srow[c >> 2]);
+            const int u1 = c & ~((4 >> s1) - 1);
+            ub[1] = u1;
+            sp[1] = s1;
+            pb = u1 - 1;
+        } else {
+            pb = C.nbx - 1;
+        }
+        if (pb >= 0) {
+            const int s0 = mc_split(srow[pb >> 2]);
+            const int k0 = 4 >> s0, u0 = pb & ~(k0 - 1);
+            if (x - (u0 * C.xbsep - C.xoff) < k0 * C.xbsep + C.xov) {
+                ub[0] = u0;
+                sp[0] = s0;
+            }
+        }
+        for (int q = 0; q < 2; ++q) {
+            const int u = ub[q];
+            if (u < 0) continue;
+            const int s = sp[q], ku = 4 >> s;
+            const int px = u * C.xbsep - C.xoff;
+            const int xcls = (px >= 0 && u + ku < C.nbx) ? 1 : (px < 0 ? 0 : 2);
+            const int H = C.hw[(s * 3 + xcls) * C.hpitch + (x - px)];
+            const size_t b = (size_t)rb * C.xnb + u;
+            const int mode = C.mode[b];
+            int val;
+            if (mode == DME_REF1_ONLY || mode == DME_REF2_ONLY || mode == DME_REF1AND2) {
+                dme_mv m1 = C.vec[0][b], m2 = C.vec[1][b];
+                m1.x >>= C.shift;
+                m1.y >>= C.shift;
+                m2.x >>= C.shift;
+                m2.y >>= C.shift;
+                if (mode == DME_REF1_ONLY) val = mc_pred(C.up[0], uw, tx, ty, C.prec, x, y, m1);
+                else if (mode == DME_REF2_ONLY) val = mc_pred(C.up[1], uw, tx, ty, C.prec, x, y, m2);
+                else val = (mc_pred(C.up[0], uw, tx, ty, C.prec, x, y, m1) + mc_pred(C.up[1], uw, tx, ty, C.prec, x, y, m2) + 1) >> 1;
+            } else {
+                val = C.dc[b];  // DCBlock
+            }
+            S += val * V * H;
+        }
+    }
+    const int s16 = (int16_t)S;
+    if (C.direction == DME_MC_SUBTRACT) {
+        *o = (int16_t)(*o - (int16_t)((s16 + 32) >> 6));
+    } else {
+        // the MMX add path rounds in 16-bit lanes over the first (width & ~3)
+        // samples (mot_comp_mmx.cpp:408-424)
+        const int p = (C.mmx && x < (C.pw & ~3)) ? ((int16_t)(s16 + 32)) >> 6 : (s16 + 32) >> 6;
+        *o = (int16_t)(*o + (int16_t)p);
+    }
+}
+
+// Adding: samples beyond the picture take the last column's value, rows below
+// it the last row's (mot_comp.cpp:484-488, 523-537).  Runs after k_mc.
+__global__ __launch_bounds__(256) void k_mc_pad(const McComp *__restrict__ cs, int ncs, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    const McComp &C = cs[mc_comp_of(cs, ncs, tile)];
+    const int local = tile - C.offset;
+    const int x = (local % C.tiles_x) * 64 + (threadIdx.x & 63);
+    const int y = (local / C.tiles_x) * 4 + wave_id();
+    if (x >= C.lx || y >= C.ly || (x < C.pw && y < C.ph)) return;
+    C.out[(size_t)y * C.lx + x] = C.out[(size_t)min(y, C.ph - 1) * C.lx + min(x, C.pw - 1)];
+}
+
+hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (pad) hipLaunchKernelGGL(k_mc_pad, dim3(total), dim3(256), 0, st, cs, ncs, total);
+    else hipLaunchKernelGGL(k_mc, dim3(total), dim3(256), 0, st, cs, ncs, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_upconvert, dim3(total), dim3(256), 0, st, pt, npt, total);

@@ -161,6 +161,41 @@ class Context:
             check(lib().dme_run_batch(self.h, arr, len(jobs), None))
         return outs
 
+    def motion_compensate(self, params, width, height, me, refs, direction, planes):
+        """MotionCompensator::CompensatePicture on host arrays: planes are
+        three C-contiguous int16 arrays (Y, U, V; possibly padded), changed
+        in place.  me: MEData with vectors, mode, dc and sb_split."""
+        for a in planes:
+            if a.dtype != np.int16 or not a.flags.c_contiguous:
+                raise ValueError("planes must be C-contiguous int16 arrays")
+        a, n = self._refs(refs)
+        ps = (abi.dme_mc_plane * 3)(*[abi.dme_mc_plane(p.ctypes.data, p.shape[1], p.shape[0]) for p in planes])
+        ms = me.struct()
+        check(lib().dme_motion_compensate(self.h, C.byref(params), width, height, C.byref(ms), a, n, direction, ps))
+        return planes
+
+    def motion_compensate_batch(self, jobs):
+        """jobs: list of dicts with pic_num, width, height, refs, direction,
+        params, me (MEData or None: the context's estimate for pic_num) and
+        planes: three (device pointer, width, height) triples."""
+        arr = (abi.dme_mc_job * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            arr[i].pic_num = j.get("pic_num", -1)
+            arr[i].width, arr[i].height = j["width"], j["height"]
+            arr[i].nrefs = len(j["refs"])
+            for k, r in enumerate(j["refs"]):
+                arr[i].refs[k] = r
+            arr[i].direction = j["direction"]
+            arr[i].params = j["params"]
+            if j.get("me") is not None:
+                ms = j["me"].struct()
+                keep.append(ms)
+                arr[i].mv = C.pointer(ms)
+            for c, (d, w, h) in enumerate(j["planes"]):
+                arr[i].planes[c] = abi.dme_mc_plane(d, w, h)
+        check(lib().dme_motion_compensate_batch(self.h, arr, len(jobs)))
+
     def get_plane(self, pic_num, kind, level, shape):
         out = np.zeros(shape, np.int16)
         check(lib().dme_get_plane(self.h, pic_num, kind, level, ptr(out), out.size))

@@ -21,6 +21,7 @@ DME_ERR_STATE = -4
 DME_ERR_NO_DEVICE = -5
 
 DME_INTRA, DME_REF1_ONLY, DME_REF2_ONLY, DME_REF1AND2 = 0, 1, 2, 3
+DME_MC_ADD, DME_MC_SUBTRACT = 0, 1
 DME_FP_MMX, DME_FP_C = 0, 1
 DME_MAX_DEPTH = 8
 
@@ -69,6 +70,16 @@ class dme_job(C.Structure):
     _fields_ = [("pic_num", C.c_int32), ("nrefs", C.c_int32), ("refs", C.c_int32 * 2), ("params", dme_params)]
 
 
+class dme_mc_plane(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class dme_mc_job(C.Structure):
+    _fields_ = [("pic_num", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("nrefs", C.c_int32),
+                ("refs", C.c_int32 * 2), ("direction", C.c_int32), ("reserved", C.c_int32), ("params", dme_params),
+                ("mv", C.POINTER(dme_medata)), ("planes", dme_mc_plane * 3)]
+
+
 DME_NUM_KERNELS = 24
 
 
@@ -109,6 +120,9 @@ SIGNATURES = {
     "dme_mode_decide": ([_P, _I, C.POINTER(dme_medata)], _I),
     "dme_motion_estimate": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_run_batch": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_motion_compensate": ([_P, C.POINTER(dme_params), _I, _I, C.POINTER(dme_medata), _P, _I, _I,
+                               C.POINTER(dme_mc_plane)], _I),
+    "dme_motion_compensate_batch": ([_P, C.POINTER(dme_mc_job), _I], _I),
     "dme_get_plane": ([_P, _I, _I, _I, _P, C.c_size_t], _I),
     "dme_synchronize": ([_P], _I),
     "dme_get_stats": ([_P, C.POINTER(dme_stats)], _I),

@@ -69,7 +69,8 @@ typedef struct dme_cost {
  * derived from them there.                                                   */
 typedef struct dme_params {
     int32_t xblen, yblen, xbsep, ybsep;
-    int32_t mv_precision;  /* 0 pel, 1 half, 2 quarter (MVPrecisionType)     */
+    int32_t mv_precision;  /* 0 pel, 1 half, 2 quarter (MVPrecisionType);
+                              3 (eighth) for motion compensation only       */
     int32_t depth;         /* pyramid depth; < 0 selects the reference's own
                               formula floor(min(log2(W/12), log2(H/12)))
                               (pixel_match.cpp:94-95)                          */
@@ -212,6 +213,63 @@ typedef struct dme_job {
 } dme_job;
 int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs,
                   dme_medata *outs);
+
+/* ---- motion compensation ------------------------------------------------ */
+
+/* AddOrSub (common.h:90) */
+#define DME_MC_ADD       0   /* residual + prediction (decoder side)       */
+#define DME_MC_SUBTRACT  1   /* picture - prediction (encoder residual)    */
+
+/* One component array of the picture being compensated: row-major int16,
+ * `width` samples per row and `height` rows, at least the component's own
+ * size (an encoder picture may be padded beyond it).                      */
+typedef struct dme_mc_plane {
+    int16_t *data;
+    int32_t  width;
+    int32_t  height;
+} dme_mc_plane;
+
+/* MotionCompensator::CompensatePicture (mot_comp.cpp:63-99, OBMC per
+ * CompensateComponent/CompensateBlock :238-603): compensates the Y, U and V
+ * arrays of a width x height 4:2:0 picture in place, in `direction`, from
+ * the MvData in `mv` (vectors[0..nrefs), mode, dc[3], sb_split; the layouts
+ * dme_motion_estimate writes) and the registered pictures refs[0..nrefs)
+ * (luma and chroma planes required) as references, with the block
+ * parameters, vector precision (0..3: pel, half, quarter, eighth) and float
+ * semantics (the MMX build's 16-bit rounding when adding) of `params`, and
+ * default reference weights.  Padded samples are handled as the reference
+ * does: zeroed when subtracting, replicated from the last column and row
+ * when adding.  planes[] are host arrays; the call returns when they hold
+ * the result.  The references' up-converted planes (Picture::UpData,
+ * picture.cpp:141-166) are built on first use and kept.                   */
+int dme_motion_compensate(dme_ctx *ctx, const dme_params *params, int width,
+                          int height, const dme_medata *mv, const int *refs,
+                          int nrefs, int direction,
+                          const dme_mc_plane planes[3]);
+
+/* Batched, device-resident form: compensates njobs pictures in one launch,
+ * each as dme_motion_compensate does, with planes[].data in device memory of
+ * the context's device (compensated in place; arrays of different jobs must
+ * not overlap).  mv == NULL takes the MvData the context estimated for
+ * pic_num (dme_motion_estimate / dme_run_batch / the learned forms, which
+ * keep it on the device): motion estimation followed by compensation then
+ * never leaves the GPU.  params.mv_precision may be 3 (eighth pel).  The call
+ * returns when the arrays hold the results.  No counterpart in the reference,
+ * which compensates one picture at a time (PictureCompressor::
+ * MotionCompensate, picture_compress.cpp:322-338).                         */
+typedef struct dme_mc_job {
+    int32_t pic_num;        /* MvData source when mv is NULL                */
+    int32_t width, height;  /* picture size (luma)                           */
+    int32_t nrefs;
+    int32_t refs[2];        /* registered reference pictures                */
+    int32_t direction;      /* DME_MC_ADD or DME_MC_SUBTRACT                */
+    int32_t reserved;       /* must be zero                                  */
+    dme_params params;
+    const dme_medata *mv;   /* host MvData, or NULL (see above)             */
+    dme_mc_plane planes[3]; /* device arrays                                 */
+} dme_mc_job;
+int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs,
+                                int njobs);
 
 /* Diagnostics: copies a cached device plane of picture pic_num to host.
  * kind 0 = pyramid level `level` (1..depth built so far), (W>>level) x

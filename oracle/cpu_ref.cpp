@@ -913,6 +913,238 @@ void export_medata(const MEData &me, dme_medata *out) {
     if (out->lambda_map) for (size_t k = 0; k < B; ++k) out->lambda_map[k] = me.lam[k];
 }
 
+// ---------------------------------------------------------------------------
+// Motion compensation (MotionCompensator, mot_comp.cpp), restated in the
+// reference's own structure: block rows processed in order into a strip of
+// Yblen rows that carries the overlap to the next row (CompensateComponent,
+// :238-539), blocks merged horizontally by the superblock split (:321-400),
+// per block prediction, reference weights, spatial weights and
+// accumulation (CompensateBlock, :541-603).
+// ---------------------------------------------------------------------------
+
+// CreateBlock (mot_comp.cpp:735-804): the OBMC window of a block of
+// xlen x ylen with separations xbsep, ybsep; FullX / FullY false keep the
+// leading edge flat (picture edges).
+Plane mc_create_block(int xlen, int ylen, int xbsep, int ybsep, bool fullx, bool fully) {
+    std::vector<int> hw(xlen), vw(ylen);
+    const int xoff = (xlen - xbsep) / 2, yoff = (ylen - ybsep) / 2;
+    if (xoff != 1) {
+        for (int x = 0; x < 2 * xoff; ++x) {
+            hw[x] = 1 + (6 * x + xoff - 1) / (2 * xoff - 1);
+            hw[x + xbsep] = 8 - hw[x];
+        }
+    } else {
+        hw[0] = hw[1 + xbsep] = 3;
+        hw[1] = hw[xbsep] = 5;
+    }
+    for (int x = 2 * xoff; x < xbsep; ++x) hw[x] = 8;
+    if (yoff != 1) {
+        for (int y = 0; y < 2 * yoff; ++y) {
+            vw[y] = 1 + (6 * y + yoff - 1) / (2 * yoff - 1);
+            vw[y + ybsep] = 8 - vw[y];
+        }
+    } else {
+        vw[0] = vw[1 + ybsep] = 3;
+        vw[1] = vw[ybsep] = 5;
+    }
+    for (int y = 2 * yoff; y < ybsep; ++y) vw[y] = 8;
+    if (!fullx) for (int x = 0; x < 2 * xoff; ++x) hw[x] = 8;
+    if (!fully) for (int y = 0; y < 2 * yoff; ++y) vw[y] = 8;
+    Plane w(xlen, ylen);
+    for (int y = 0; y < ylen; ++y)
+        for (int x = 0; x < xlen; ++x) w.at(y, x) = (int16_t)(vw[y] * hw[x]);
+    return w;
+}
+
+Plane mc_flip_x(const Plane &a) {
+    Plane b(a.w, a.h);
+    for (int y = 0; y < a.h; ++y)
+        for (int x = 0; x < a.w; ++x) b.at(y, x) = a.at(y, a.w - 1 - x);
+    return b;
+}
+
+Plane mc_flip_y(const Plane &a) {
+    Plane b(a.w, a.h);
+    for (int y = 0; y < a.h; ++y)
+        for (int x = 0; x < a.w; ++x) b.at(y, x) = a.at(a.h - 1 - y, x);
+    return b;
+}
+
+// CalculateWeights (mot_comp.cpp:703-733): the nine edge variants.
+void mc_weights(int xlen, int ylen, int xbsep, int ybsep, Plane w[9]) {
+    w[0] = mc_create_block(xlen, ylen, xbsep, ybsep, false, false);
+    w[3] = mc_create_block(xlen, ylen, xbsep, ybsep, false, true);
+    w[1] = mc_create_block(xlen, ylen, xbsep, ybsep, true, false);
+    w[4] = mc_create_block(xlen, ylen, xbsep, ybsep, true, true);
+    w[5] = mc_flip_x(w[3]);
+    w[2] = mc_flip_x(w[0]);
+    w[6] = mc_flip_y(w[0]);
+    w[8] = mc_flip_x(w[6]);
+    w[7] = mc_flip_y(w[1]);
+}
+
+// BlockPixelPred for pixel, half, quarter and eighth precision (mot_comp.cpp:846-1230;
+// MMX mot_comp_mmx.cpp:69-354 gives the same values): the block of
+// ylen x xlen from the up-converted reference, clamped reads via BChk
+// against (2 size - 1).
+void mc_block_pred(Plane &blk, int sx, int sy, int prec, const Plane &up, int pw, int ph, MV mv) {
+    const int tx = (pw << 1) - 1, ty = (ph << 1) - 1;
+    int rx0, ry0, fx = 0, fy = 0;
+    if (prec == 0) {
+        rx0 = (sx + mv.x) << 1;
+        ry0 = (sy + mv.y) << 1;
+    } else if (prec == 1) {
+        rx0 = (sx << 1) + mv.x;
+        ry0 = (sy << 1) + mv.y;
+    } else if (prec == 2) {
+        rx0 = (sx << 1) + (mv.x >> 1);
+        ry0 = (sy << 1) + (mv.y >> 1);
+        fx = mv.x & 1;
+        fy = mv.y & 1;
+    } else {  // eighth pel (:1108-1230): bilinear in quarter steps of the half-pel plane
+        rx0 = (sx << 1) + (mv.x >> 2);
+        ry0 = (sy << 1) + (mv.y >> 2);
+        fx = mv.x & 3;
+        fy = mv.y & 3;
+    }
+    const int one = prec == 3 ? 4 : 2, sh = prec == 3 ? 4 : 2;
+    const int w0 = (one - fx) * (one - fy), w1 = fx * (one - fy), w2 = (one - fx) * fy, w3 = fx * fy;
+    for (int y = 0; y < blk.h; ++y) {
+        const int uy = ry0 + 2 * y;
+        for (int x = 0; x < blk.w; ++x) {
+            const int ux = rx0 + 2 * x;
+            if (prec < 2) {
+                blk.at(y, x) = up.at(bchk(uy, ty), bchk(ux, tx));
+            } else {
+                const int v = w0 * up.at(bchk(uy, ty), bchk(ux, tx)) + w1 * up.at(bchk(uy, ty), bchk(ux + 1, tx)) +
+                              w2 * up.at(bchk(uy + 1, ty), bchk(ux, tx)) + w3 * up.at(bchk(uy + 1, ty), bchk(ux + 1, tx));
+                blk.at(y, x) = (int16_t)((v + (1 << (sh - 1))) >> sh);
+            }
+        }
+    }
+}
+
+struct McArrays {
+    int xnb, ynb, xsb, nrefs;
+    const dme_mv *v[2];
+    const int32_t *mode;
+    const int16_t *dc;
+    const int32_t *split;
+};
+
+// CompensateComponent (mot_comp.cpp:238-539) for one component.
+void mc_component(const OLB &bp, int prec, bool mmx, int pw, int ph, int shift, const McArrays &m,
+                  const Plane *up[2], int direction, int16_t *out, int lx, int ly) {
+    const int bpm = m.xnb / m.xsb, bps = bpm >> 1;  // blocks per superblock / sub-superblock row
+    const int xoff = (bp.xblen - bp.xbsep) / 2, yoff = (bp.yblen - bp.ybsep) / 2;
+    const int mb_xlen = bp.xblen * bpm - (bp.xblen - bp.xbsep) * (bpm - 1);
+    const int mb_xsep = mb_xlen - (bp.xblen - bp.xbsep);
+    const int sb_xlen = bp.xblen * bps - (bp.xblen - bp.xbsep) * (bps - 1);
+    const int sb_xsep = sb_xlen - (bp.xblen - bp.xbsep);
+    Plane wb[9], wm[9], ws[9];
+    mc_weights(bp.xblen, bp.yblen, bp.xbsep, bp.ybsep, wb);
+    mc_weights(mb_xlen, bp.yblen, mb_xsep, bp.ybsep, wm);
+    mc_weights(sb_xlen, bp.yblen, sb_xsep, bp.ybsep, ws);
+    // NUM_USED_BLKS (mot_comp.cpp:54)
+    const int ynb = std::min((ph + bp.ybsep + (bp.yblen - bp.ybsep) / 2 - 1) / bp.ybsep, m.ynb);
+    const int xnb = std::min((pw + bp.xbsep + (bp.xblen - bp.xbsep) / 2 - 1) / bp.xbsep, m.xnb);
+    Plane strip(lx, bp.yblen);  // pic_data: one row of blocks
+    int save_from = bp.ybsep - yoff;
+    const bool overlap = bp.yblen - bp.ybsep > 0;
+    const int x_end = std::min(lx, pw), y_end = std::min(ly, ph);
+    int py = -yoff;
+    for (int yb = 0; yb < ynb; ++yb) {
+        int px = -xoff, inc = 0;
+        for (int xb = 0; xb < xnb; xb += inc) {
+            const int split = m.split[(size_t)(yb / bpm) * m.xsb + xb / bpm];
+            const int bx = split == 0 ? bpm : (split == 1 ? bps : 1);
+            int wi;
+            const int row = py < 0 ? 0 : (yb + 1 < ynb ? 1 : 2);
+            if (px >= 0 && xb + bx < xnb) wi = 3 * row + 1;
+            else if (px < 0) wi = 3 * row;
+            else wi = 3 * row + 2;
+            const Plane &wt = split == 0 ? wm[wi] : (split == 1 ? ws[wi] : wb[wi]);
+            inc = bx;
+            const size_t k = (size_t)yb * m.xnb + xb;
+            const int mode = m.mode[k];
+            MV mv1(m.v[0][k].x >> shift, m.v[0][k].y >> shift);
+            const dme_mv *v2 = m.nrefs > 1 ? m.v[1] : m.v[0];
+            MV mv2(v2[k].x >> shift, v2[k].y >> shift);
+            // CompensateBlock (mot_comp.cpp:541-603)
+            const int sx = std::max(px, 0), sy = std::max(py, 0);
+            const int ex = std::min(px + wt.w, pw), ey = std::min(py + wt.h, ph);
+            if (sx < ex && sy < ey) {
+                Plane v1(ex - sx, ey - sy), v2b(ex - sx, ey - sy);
+                if (mode == DME_REF1_ONLY) mc_block_pred(v1, sx, sy, prec, *up[0], pw, ph, mv1);
+                else if (mode == DME_REF2_ONLY) mc_block_pred(v1, sx, sy, prec, *up[1], pw, ph, mv2);
+                else if (mode == DME_REF1AND2) {
+                    mc_block_pred(v1, sx, sy, prec, *up[0], pw, ph, mv1);
+                    mc_block_pred(v2b, sx, sy, prec, *up[1], pw, ph, mv2);
+                } else {
+                    for (auto &e : v1.d) e = m.dc[k];  // DCBlock (:605-618)
+                }
+                // AdjustBlockByRefWeights, default weights (:620-683)
+                if (mode == DME_REF1AND2)
+                    for (size_t i = 0; i < v1.d.size(); ++i) v1.d[i] = (int16_t)((v1.d[i] + v2b.d[i] + 1) >> 1);
+                // AdjustBlockBySpatialWeights (:685-701) and accumulation
+                for (int y = 0; y < v1.h; ++y)
+                    for (int x = 0; x < v1.w; ++x) {
+                        const int16_t val = (int16_t)(v1.at(y, x) * wt.at(sy - py + y, sx - px + x));
+                        strip.at(y, sx + x) = (int16_t)(strip.at(y, sx + x) + val);
+                    }
+            }
+            px += bp.xbsep * inc;
+        }
+        if (direction == DME_MC_SUBTRACT) {
+            const int s0 = std::max(0, py);
+            int e0 = std::min(py + bp.ybsep, y_end);
+            if (yb == ynb - 1) e0 = std::min(ly, y_end);
+            for (int i = s0, r = 0; i < e0; ++i, ++r) {
+                int16_t *o = out + (size_t)i * lx;
+                for (int j = 0; j < x_end; ++j) o[j] = (int16_t)(o[j] - (int16_t)((strip.at(r, j) + 32) >> 6));
+                for (int j = pw; j < lx; ++j) o[j] = 0;
+            }
+        } else {
+            const int s0 = std::max(0, py);
+            int e0 = std::min(py + bp.ybsep, ly);
+            if (yb == ynb - 1) {
+                e0 += bp.yblen - bp.ybsep;
+                if (e0 > ph) e0 = ph;
+            }
+            for (int i = s0, r = 0; i < e0; ++i, ++r) {
+                int16_t *o = out + (size_t)i * lx;
+                // The MMX build adds the rounding offset in 16-bit lanes over the
+                // first (width & ~3) samples (mot_comp_mmx.cpp:408-424), the C
+                // build in int (mot_comp.cpp:474-482).
+                for (int j = 0; j < pw; ++j) {
+                    const int sum = strip.at(r, j) + 32;
+                    const int16_t p = (int16_t)(mmx && j < (pw & ~3) ? (int16_t)sum >> 6 : sum >> 6);
+                    o[j] = (int16_t)(o[j] + p);
+                }
+                for (int j = pw; j < lx; ++j) o[j] = o[pw - 1];
+            }
+        }
+        py += bp.ybsep;
+        if (overlap) {
+            const int keep = bp.yblen - save_from;
+            for (int r = 0; r < keep; ++r)
+                for (int x = 0; x < lx; ++x) strip.at(r, x) = strip.at(r + save_from, x);
+            for (int r = keep; r < bp.yblen; ++r)
+                for (int x = 0; x < lx; ++x) strip.at(r, x) = 0;
+            save_from = bp.ybsep;
+        } else {
+            std::fill(strip.d.begin(), strip.d.end(), (int16_t)0);
+        }
+    }
+    if (direction == DME_MC_SUBTRACT) {
+        for (int y = ph; y < ly; ++y)
+            for (int x = 0; x < lx; ++x) out[(size_t)y * lx + x] = 0;
+    } else {
+        for (int y = ph; y < ly; ++y)
+            for (int x = 0; x < lx; ++x) out[(size_t)y * lx + x] = out[(size_t)(ph - 1) * lx + x];
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -951,6 +1183,56 @@ extern "C" int oracle_downconvert(const int16_t *in, int in_w, int in_h, int16_t
     downconvert(a, b);
     std::memcpy(out, b.d.data(), sizeof(int16_t) * b.d.size());
     return DME_OK;
+}
+
+// CompensatePicture (mot_comp.cpp:131-186) over the three components.
+static int mc_picture(const Geo &g, int prec, bool mmx, const oracle_picture *refs, int nrefs,
+                      const dme_medata *me, int direction, int16_t *const planes[3], const int widths[3],
+                      const int heights[3]) {
+    if (nrefs < 1 || nrefs > 2 || !refs || !me || !me->vectors[0] || !me->mode || !me->sb_split)
+        return fail("bad arguments");
+    if (direction != DME_MC_SUBTRACT && direction != DME_MC_ADD) return fail("bad direction");
+    for (int c = 0; c < 3; ++c) {
+        const int pw = c ? g.CW : g.W, ph = c ? g.CH : g.H;
+        if (!planes[c] || widths[c] < pw || heights[c] < ph || !me->dc[c]) return fail("bad plane");
+        Plane up[2];
+        const Plane *upp[2];
+        for (int r = 0; r < nrefs; ++r) {
+            const int16_t *src = c == 0 ? refs[r].y : (c == 1 ? refs[r].u : refs[r].v);
+            if (!src) return fail("reference plane missing");
+            upconvert(make_plane(src, pw, ph), up[r]);
+        }
+        upp[0] = &up[0];
+        upp[1] = nrefs > 1 ? &up[1] : &up[0];
+        McArrays m;
+        m.xnb = g.xnb;
+        m.ynb = g.ynb;
+        m.xsb = g.xsb;
+        m.nrefs = nrefs;
+        m.v[0] = me->vectors[0];
+        m.v[1] = nrefs > 1 ? me->vectors[1] : me->vectors[0];
+        m.mode = me->mode;
+        m.dc = me->dc[c];
+        m.split = me->sb_split;
+        mc_component(c ? g.c[2] : g.l[2], prec, mmx, pw, ph, c ? 1 : 0, m, upp, direction, planes[c], widths[c],
+                     heights[c]);
+    }
+    return 0;
+}
+
+extern "C" int oracle_motion_compensate(const dme_params *p, int width, int height, const oracle_picture *refs,
+                                        int nrefs, const dme_medata *me, int direction, int16_t *const planes[3],
+                                        const int widths[3], const int heights[3]) {
+    if (p && p->mv_precision == 3) {  // eighth pel: compensation only, geometry as quarter pel
+        dme_params q = *p;
+        q.mv_precision = 2;
+        Geo g;
+        if (make_geo(&q, width, height, g)) return -1;
+        return mc_picture(g, 3, q.fp_semantics == DME_FP_MMX, refs, nrefs, me, direction, planes, widths, heights);
+    }
+    Geo g;
+    if (make_geo(p, width, height, g)) return -1;
+    return mc_picture(g, p->mv_precision, p->fp_semantics == DME_FP_MMX, refs, nrefs, me, direction, planes, widths, heights);
 }
 
 extern "C" int oracle_upconvert(const int16_t *in, int w, int h, int16_t *out) {

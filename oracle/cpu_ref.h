@@ -64,6 +64,17 @@ int oracle_me(const dme_params *p, const oracle_picture *cur,
               const float *lambda_override, dme_medata *out,
               oracle_capture *cap, double *sad_terms);
 
+/* MotionCompensator::CompensatePicture (mot_comp.cpp:63-600) for one
+ * 4:2:0 picture of width x height: planes[c] (row length widths[c], rows
+ * heights[c], at least the component's size) are compensated in place in
+ * `direction` (DME_MC_SUBTRACT / DME_MC_ADD) from the vectors, modes, DCs
+ * and superblock splits of `me` and the reference pictures' planes, with
+ * the block parameters and precision of p and default reference weights. */
+int oracle_motion_compensate(const dme_params *p, int width, int height,
+                             const oracle_picture *refs, int nrefs,
+                             const dme_medata *me, int direction,
+                             int16_t *const planes[3], const int widths[3],
+                             const int heights[3]);
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus

@@ -190,3 +190,27 @@ def lambda_map(fn_lib, name, v1, v2, lam, nrefs):
                                out.ctypes.data)
     assert rc == 0
     return out
+
+
+def motion_compensate(which, params, refs, me, nrefs, direction, planes):
+    """MotionCompensator::CompensatePicture through the oracle ("oracle") or
+    the reference harness ("mmx" / "c").  refs: list of (y, u, v) planes of
+    the references; me: MEData with vectors, mode, dc and sb_split; planes:
+    three int16 arrays (possibly padded), compensated in place."""
+    H, W = refs[0][0].shape
+    rs = [tuple(np.ascontiguousarray(a, np.int16) for a in r) for r in refs]
+    rp = (oracle_picture * 2)(*[_pic(10 + i, *rs[i]) for i in range(nrefs)])
+    ms = me.struct()
+    ptrs = (C.c_void_p * 3)(*[p.ctypes.data for p in planes])
+    ws = (C.c_int * 3)(*[p.shape[1] for p in planes])
+    hs = (C.c_int * 3)(*[p.shape[0] for p in planes])
+    if which == "oracle":
+        fn = oracle().oracle_motion_compensate
+    else:
+        fn = ref(which).refh_mc
+    fn.argtypes = [C.POINTER(abi.dme_params), C.c_int, C.c_int, C.POINTER(oracle_picture), C.c_int,
+                   C.POINTER(abi.dme_medata), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = fn(C.byref(params), W, H, rp, nrefs, C.byref(ms), direction, ptrs, ws, hs)
+    if rc != 0:
+        raise RuntimeError("motion compensation error %d %s" % (rc, oracle().oracle_last_error().decode()))
+    return planes

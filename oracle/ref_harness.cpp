@@ -20,7 +20,9 @@
 // in the order of PictureCompressor (picture_compress.cpp:76-298).
 
 #include <libdirac_common/common.h>
+#include <libdirac_common/mot_comp.h>
 #include <libdirac_common/motion.h>
+#include <libdirac_common/picture.h>
 #include <libdirac_common/upconvert.h>
 #include <libdirac_encoder/enc_picture.h>
 #include <libdirac_encoder/enc_queue.h>
@@ -328,5 +330,70 @@ extern "C" int refh_lambda_map(int xnb, int ynb, int nrefs, const dme_mv *v1, co
     }
     me.SetLambdaMap(nrefs, lambda);
     dump_arr(me.LambdaMap(), out);
+    return 0;
+}
+
+// MotionCompensator::CompensatePicture (mot_comp.cpp:63-99) on a picture
+// whose component arrays are widths[c] x heights[c] (at least the picture's
+// own size, as a padded encoder picture), with MvData filled from `me` and
+// the given reference pictures.
+extern "C" int refh_mc(const dme_params *p, int W, int H, const oracle_picture *refs, int nrefs,
+                       const dme_medata *me, int direction, int16_t *const planes[3], const int widths[3],
+                       const int heights[3]) {
+    PicturePredParams pp;
+    pp.SetBlockSizes(OLBParams(p->xblen, p->yblen, p->xbsep, p->ybsep), format420);
+    pp.SetMVPrecision(MVPrecisionType(p->mv_precision));
+    pp.SetXNumSB((W + pp.LumaBParams(0).Xbsep() - 1) / pp.LumaBParams(0).Xbsep());
+    pp.SetYNumSB((H + pp.LumaBParams(0).Ybsep() - 1) / pp.LumaBParams(0).Ybsep());
+    pp.SetXNumBlocks(4 * pp.XNumSB());
+    pp.SetYNumBlocks(4 * pp.YNumSB());
+    // The default constructor leaves the picture weights unset; the encoder's
+    // defaults are one weight-precision bit and unit weights.
+    pp.SetPictureWeightsPrecision(1);
+    pp.SetRef1Weight(1);
+    pp.SetRef2Weight(1);
+    MvData mv(pp, nrefs);
+    const int xnb = pp.XNumBlocks(), ynb = pp.YNumBlocks();
+    for (int r = 1; r <= nrefs; ++r) {
+        MvArray &a = mv.Vectors(r);
+        for (int j = 0; j < ynb; ++j)
+            for (int i = 0; i < xnb; ++i) {
+                a[j][i].x = me->vectors[r - 1][(size_t)j * xnb + i].x;
+                a[j][i].y = me->vectors[r - 1][(size_t)j * xnb + i].y;
+            }
+    }
+    for (int j = 0; j < ynb; ++j)
+        for (int i = 0; i < xnb; ++i) {
+            mv.Mode()[j][i] = PredMode(me->mode[(size_t)j * xnb + i]);
+            for (int c = 0; c < 3; ++c) mv.DC(CompSort(c))[j][i] = me->dc[c][(size_t)j * xnb + i];
+        }
+    for (int j = 0; j < pp.YNumSB(); ++j)
+        for (int i = 0; i < pp.XNumSB(); ++i) mv.SBSplit()[j][i] = me->sb_split[(size_t)j * pp.XNumSB() + i];
+
+    std::vector<Picture *> rp;
+    for (int r = 0; r < nrefs; ++r) {
+        PictureParams prp(format420, W, H, 8, 8);
+        prp.SetPictureNum(refs[r].pic_num);
+        prp.SetPicSort(PictureSort::IntraRefPictureSort());
+        Picture *pic = new Picture(prp);
+        fill(pic->Data(Y_COMP), refs[r].y);
+        fill(pic->Data(U_COMP), refs[r].u);
+        fill(pic->Data(V_COMP), refs[r].v);
+        rp.push_back(pic);
+    }
+    PictureParams cp(format420, W, H, 8, 8);
+    cp.SetPictureNum(1000000);
+    cp.SetPicSort(PictureSort::InterRefPictureSort());
+    for (int r = 0; r < nrefs; ++r) cp.Refs().push_back(refs[r].pic_num);
+    Picture cur(cp);
+    for (int c = 0; c < 3; ++c) {
+        PicArray &a = cur.Data(CompSort(c));
+        a.Resize(heights[c], widths[c]);
+        fill(a, planes[c]);
+    }
+    Picture *refsptr[2] = {rp[0], nrefs > 1 ? rp[1] : rp[0]};
+    MotionCompensator::CompensatePicture(pp, direction == DME_MC_ADD ? ADD : SUBTRACT, mv, &cur, refsptr);
+    for (int c = 0; c < 3; ++c) dump(cur.Data(CompSort(c)), planes[c]);
+    for (size_t i = 0; i < rp.size(); ++i) delete rp[i];
     return 0;
 }

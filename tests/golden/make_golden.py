@@ -2,6 +2,7 @@
 own ME code, compiled from its sources by oracle/Makefile (target `ref`).
 
 Run in a container that has the reference tree:  python tests/golden/make_golden.py
+(add --mc-only to rewrite only the motion-compensation fixture, mc.npz).
 
 Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
 seed by the deterministic generator in libdme.so), the parameters, and the
@@ -102,5 +103,44 @@ def main():
         json.dump(index, f, indent=1, sort_keys=True)
 
 
+def main_mc():
+    """Motion-compensation fixtures (mc.npz): for each case in mc_cases.CASES,
+    the MvData from the reference's own motion estimator (or scrambled), and
+    the planes the reference's MotionCompensator::CompensatePicture leaves
+    (the build of the case's float semantics: MMX or plain C)."""
+    import mc_cases as M
+    from dirac_me import MEData, geometry
+
+    out = {}
+    index = {}
+    for name, seed, nrefs, prec, pad, direction, fp, blocks, scr, bigdc, full in M.CASES:
+        d, refs, src = M.pictures(seed, nrefs)
+        blk = blocks or d["block"]
+        p = make_params(*blk, precision=min(prec, 2), depth=d["depth"], lam=10.0, fp=fp)
+        res, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], variant="mmx")
+        me = MEData(geometry(p, d["cur"].shape[1], d["cur"].shape[0]))
+        M.load_mv(me, M.mv_arrays(res, nrefs))
+        if scr:
+            M.scramble(me, seed, nrefs)
+        if bigdc:
+            M.big_dc(me, seed)
+        p.mv_precision = prec
+        planes = M.padded(src, pad)
+        O.motion_compensate("mmx" if fp == DME_FP_MMX else "c", p, refs, me, nrefs, direction, planes)
+        meta = dict(seed=seed, nrefs=nrefs, precision=prec, pad=list(pad), direction=direction, fp=fp,
+                    block=list(blk), inputs_sha256=M.inputs_digest(refs, src),
+                    out_sha256=[M.sha(a) for a in planes])
+        index[name] = meta
+        for k, a in M.mv_arrays(me, nrefs).items():
+            out[name + "/" + k] = a
+        if full:
+            for c, a in enumerate(planes):
+                out[name + "/out%d" % c] = a
+        print("wrote", name)
+    np.savez_compressed(os.path.join(HERE, "mc.npz"), meta=json.dumps(index), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if "--mc-only" not in sys.argv:
+        main()
+    main_mc()
