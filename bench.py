@@ -6,7 +6,9 @@ A step is one pass of the whole live ME sequence (pyramids and up-converted
 references rebuilt, pel search, lambda map, sub-pel refinement, mode
 decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
 one JSON line; the algorithmic byte and SAD-term counts behind the roofline
-figures are in _geometry_terms().
+figures are in _geometry_terms().  The line also carries the learned-proposal
+leg and the motion-compensation leg (residuals of the same batch from its
+estimates, mc_leg()).
 """
 import argparse
 import json
@@ -132,6 +134,97 @@ def cpu_baseline(frames, params, budget_s):
                 "single thread", dt, cpu)}
 
 
+def _mc_bytes(geom, me):
+    """Algorithmic HBM bytes of compensating one picture (k_mc): every sample
+    of the three arrays read and written (int16), the up-converted reference
+    planes (4 int16 per picture sample) read once per reference the sample's
+    blocks predict from (none for DC blocks, two for bi-prediction), and the
+    MvData."""
+    import numpy as np
+    n = np.bincount(me.mode.ravel(), minlength=4)
+    refs_per_sample = float(n[1] + n[2] + 2 * n[3]) / float(me.mode.size)
+    samples = geom.width * geom.height + 2 * geom.chroma_width * geom.chroma_height
+    B = geom.xnum_blocks * geom.ynum_blocks
+    mv = B * (2 * 8 + 4 + 3 * 2) + geom.xnum_sb * geom.ynum_sb * 4
+    return samples * 4 + samples * 8 * refs_per_sample + mv
+
+
+def mc_leg(ctx, jobs, frames, params, geom, B, args):
+    """Motion compensation (subtract: the encoder's residual) of the batch's
+    pictures from their device-side estimates, all pictures in one
+    dme_motion_compensate_batch call per step; the step rebuilds the
+    references' up-converted planes first (the reference builds them once
+    per reference picture, Picture::UpData).  Also the reference's own
+    MotionCompensator on the host, on a bounded sample."""
+    import numpy as np
+    import dirac_me
+    from dirac_me import abi
+    from dirac_me.learned import DeviceArray
+
+    est = [dirac_me.MEData(geom) for _ in frames]
+    ctx.run_batch(jobs[:len(frames)], est)
+    arrays, mjobs = [], []
+    for j, (pn, refs, prm) in enumerate(jobs):
+        d = frames[j % len(frames)]
+        src = (d["cur"], d["u"], d["v"])
+        dp = []
+        for a in src:
+            x = DeviceArray(ctx, a.shape, np.int16)
+            x.upload(a)
+            dp.append(x)
+        arrays.append(dp)
+        mjobs.append(dict(pic_num=pn, width=1920, height=1080, refs=refs, direction=abi.DME_MC_SUBTRACT,
+                          params=prm, me=None, planes=[(x.ptr, a.shape[1], a.shape[0]) for x, a in zip(dp, src)]))
+
+    def step():
+        ctx.invalidate(-1)
+        ctx.motion_compensate_batch(mjobs)
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.set_profiling(True)
+    step()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ms, nl = kt["k_mc"]
+    alg = sum(_mc_bytes(geom, est[j % len(frames)]) for j in range(B))
+    achieved = alg / nl / (ms / nl * 1e-3) / 1e9
+    out = {"value": B / dt, "unit": "pictures/s", "ms_per_step": dt * 1e3, "batch": B,
+           "direction": "subtract", "mv_source": "device-side estimates of the same batch",
+           "roofline": {"kernel": "k_mc", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes_per_launch": alg / nl, "avg_launch_ms": ms / nl},
+           "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
+    for dp in arrays:
+        for x in dp:
+            x.free()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        kind = "reference" if O.ref_available() else "port"
+        n = 0
+        t0 = time.time()
+        while True:
+            d = frames[n % len(frames)]
+            planes = [d["cur"].copy(), d["u"].copy(), d["v"].copy()]
+            O.motion_compensate("mmx" if kind == "reference" else "oracle", params, d["refs_yuv"],
+                                est[n % len(frames)], 2, abi.DME_MC_SUBTRACT, planes)
+            n += 1
+            if time.time() - t0 >= args.cpu_budget / 2 or n >= 100:
+                break
+        cdt = time.time() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "pictures/s", "cores": 1, "kind": kind,
+                               "sample": "%d 1080p B pictures (2 refs), %s, single thread, %.1f s, references' "
+                                         "up-conversion included" % (
+                                             n, "reference MotionCompensator (MMX build, compiled from its sources)"
+                                             if kind == "reference" else "oracle port", cdt)}
+    return out
+
+
 def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
     """The same batch through learned proposals (dme_run_batch_learned):
     frames/s, and the Lagrangian cost gap against the plain search (sum of
@@ -229,6 +322,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -255,11 +349,13 @@ def main():
     ctx = dirac_me.Context(0)
     ctx.set_params(params)
     jobs = []
+    for i, d in enumerate(frames):
+        d["refs_yuv"] = [(r,) + synth.chroma(r, 500 + 2 * i + k) for k, r in enumerate(d["refs"])]
     for j in range(B):
         d = frames[j % len(frames)]
         base = 3 * j
-        ctx.upload(base, d["refs"][0])
-        ctx.upload(base + 2, d["refs"][1])
+        ctx.upload(base, *d["refs_yuv"][0])  # reference chroma: used by motion compensation only
+        ctx.upload(base + 2, *d["refs_yuv"][1])
         ctx.upload(base + 1, d["cur"], d["u"], d["v"])
         jobs.append((base + 1, [base, base + 2], params))
     print("bench: %d pictures uploaded; warmup %d, steps %d" % (B, args.warmup, args.steps), file=sys.stderr, flush=True)
@@ -312,6 +408,8 @@ def main():
         "roofline": roof, "valu": valu,
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
     }
+    if not args.no_mc:
+        out["motion_compensation"] = mc_leg(ctx, jobs, frames, params, geom, B, args)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, jobs, frames, geom, B, args, fps)
     if not args.no_cpu:

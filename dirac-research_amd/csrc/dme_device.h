@@ -81,26 +81,42 @@ struct PlaneTask {
     int in_w, ow, oh, offset;
 };
 
+// k_upconvert tiles: 64 x 16 source samples.
+constexpr int kUpTileW = 64, kUpTileH = 16;
+inline int upconvert_tiles(int w, int h) { return ((w + kUpTileW - 1) / kUpTileW) * ((h + kUpTileH - 1) / kUpTileH); }
+
+// Rows of the column segment one k_mc thread owns; a k_mc tile is 64 columns
+// by 4 segments (k_mc_pad tiles are 64 x 4).
+constexpr int kMcSegRows = 8;
+
 // One component of one picture being motion compensated (k_mc / k_mc_pad).
 // Built on the host, read by every workgroup of the component's tiles.
 struct McComp {
     const int16_t DME_G *up[2];     // up-converted reference planes, 2pw x 2ph
     int16_t DME_G *out;             // array compensated in place, lx x ly
-    const dme_mv DME_G *vec[2];     // MvData of the picture, xnb x ynb blocks
-    const int DME_G *mode;
+    const int4 DME_G *unit;         // per block: {v1.x, v1.y, v2.x, v2.y}, {mode, dc Y, dc U, dc V} (k_mc_pack)
     const int DME_G *split;         // xsb x ysb superblocks
-    const int16_t DME_G *dc;        // this component's DC values
     const signed char DME_G *hw;    // OBMC weights across: [split][class][hpitch]
-    const signed char DME_G *vw;    // OBMC weights down: [class][yblen]
+    const int DME_G *vw;            // OBMC weights down: [class][yblen]
     int lx, ly, pw, ph;
     int xbsep, ybsep, xoff, yoff, xov, yblen, hpitch;
     int nbx, nby;                   // blocks in use across / down (NUM_USED_BLKS)
     int xnb, xsb;
+    int comp;                       // 0 Y, 1 U, 2 V
     int shift;                      // vector shift (1 for 4:2:0 chroma)
     int prec;                       // 0 pel, 1 half, 2 quarter, 3 eighth
     int direction;                  // 0 add, 1 subtract
     int mmx;                        // 16-bit rounding of the MMX add path
     int tiles_x, offset;            // tiles across; first tile in the launch
+};
+
+// MvData of one picture to pack into k_mc's per-block records.
+struct McPackJob {
+    const dme_mv DME_G *vec[2];
+    const int DME_G *mode;
+    const int16_t DME_G *dc[3];
+    int4 DME_G *out;  // 2 records per block
+    int blocks, offset;
 };
 
 // One unit of a batched launch: a (job, reference) pair covering `count`

@@ -32,6 +32,7 @@ namespace dme {
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st);
+hipError_t launch_mc_pack(const McPackJob *jobs, int njobs, int total, hipStream_t st);
 hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
@@ -117,8 +118,8 @@ struct Picture {
     int pyr_levels = 0;             // levels built so far
     DevBuf up, qp;  // up-converted plane (built on request) and the quarter-pel planes
     bool up_ready = false, qp_ready = false;
-    DevBuf upc[2];  // up-converted chroma planes (motion compensation)
-    bool upc_ready = false;
+    DevBuf mcp[3];  // up-converted Y, U, V for motion compensation (k_upconvert)
+    bool mcp_ready = false;
     DevBuf b8;      // byte plane with margins (k_bytes)
     int bp = 0;     // its row pitch
     bool b8_ready = false;
@@ -180,6 +181,7 @@ struct dme_ctx {
     dme::DevBuf d_lflags;  // per batch position: the device lambda maps need the host (k_lambda)
     dme::DevBuf d_mc;      // motion compensation: descriptors, weights, uploaded MvData
     dme::DevBuf d_mc_planes;  // host-array compensation: the arrays on the device
+    dme::DevBuf d_mc_units;   // packed per-block MvData records (k_mc_pack)
     // Two lanes (stream + buffers); stages queue work on the current lane.
     // Large batches run as two halves on the two lanes, so that one half's
     // host work and latency-bound wavefronts overlap the other half's
@@ -214,12 +216,12 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_COUNT
+    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad"};
+    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -1193,7 +1195,8 @@ static void mc_window(int len, int sep, bool full, signed char *w) {
 // edge); down, per class (top, inside, bottom).
 struct McTables {
     int hpitch = 0, yblen = 0;
-    std::vector<signed char> hw, vw;
+    std::vector<signed char> hw;
+    std::vector<int> vw;  // int32: the kernels read it with uniform indices
 };
 
 static void mc_tables(int xblen, int yblen, int xbsep, int ybsep, McTables &t) {
@@ -1208,16 +1211,19 @@ static void mc_tables(int xblen, int yblen, int xbsep, int ybsep, McTables &t) {
         for (int i = 0; i < len; ++i) row[2 * t.hpitch + i] = row[len - 1 - i];
     }
     t.yblen = yblen;
-    t.vw.assign((size_t)3 * yblen, 0);
-    mc_window(yblen, ybsep, false, &t.vw[0]);
-    mc_window(yblen, ybsep, true, &t.vw[yblen]);
-    for (int i = 0; i < yblen; ++i) t.vw[2 * (size_t)yblen + i] = t.vw[yblen - 1 - i];
+    std::vector<signed char> v((size_t)3 * yblen, 0);
+    mc_window(yblen, ybsep, false, &v[0]);
+    mc_window(yblen, ybsep, true, &v[yblen]);
+    for (int i = 0; i < yblen; ++i) v[2 * (size_t)yblen + i] = v[yblen - 1 - i];
+    t.vw.assign(v.begin(), v.end());
 }
 
 // NUM_USED_BLKS (mot_comp.cpp:54)
 static int mc_used_blocks(int w, int sep, int len) { return (w + sep + (len - sep) / 2 - 1) / sep; }
 
 static size_t mc_align(size_t n) { return (n + 255) & ~(size_t)255; }
+
+constexpr int kMcTileRows = 4 * kMcSegRows;  // rows of a k_mc tile; k_mc_pad tiles have 4
 
 // Validated job: geometry, references and the MvData source.
 struct McJob {
@@ -1290,26 +1296,20 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
         t.ow = w;
         t.oh = h;
         t.offset = ptotal;
-        ptotal += ((w + 63) / 64) * ((h + 3) / 4);
+        ptotal += upconvert_tiles(w, h);
         pts.push_back(t);
     };
-    std::vector<Picture *> luma_up, chroma_up;  // planes to build, each once
+    std::vector<Picture *> building;  // references whose planes this call builds, each once
     for (auto &m : mj)
         for (int r = 0; r < m.j->nrefs; ++r) {
             Picture *P = m.ref[r];
-            if (!P->up_ready && std::find(luma_up.begin(), luma_up.end(), P) == luma_up.end()) {
-                DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
-                add_task(P->y, P->up.as<int16_t>(), P->W, P->H);
-                luma_up.push_back(P);
+            if (P->mcp_ready || std::find(building.begin(), building.end(), P) != building.end()) continue;
+            for (int c = 0; c < 3; ++c) {
+                const int w = c ? P->W / 2 : P->W, h = c ? P->H / 2 : P->H;
+                DME_TRY(P->mcp[c].ensure(sizeof(int16_t) * 4 * (size_t)w * h));
+                add_task(c == 0 ? P->y : (c == 1 ? P->u : P->v), P->mcp[c].as<int16_t>(), w, h);
             }
-            if (!P->upc_ready && std::find(chroma_up.begin(), chroma_up.end(), P) == chroma_up.end()) {
-                const int cw = P->W / 2, ch = P->H / 2;
-                for (int c = 0; c < 2; ++c) {
-                    DME_TRY(P->upc[c].ensure(sizeof(int16_t) * 4 * (size_t)cw * ch));
-                    add_task(c ? P->v : P->u, P->upc[c].as<int16_t>(), cw, ch);
-                }
-                chroma_up.push_back(P);
-            }
+            building.push_back(P);
         }
     int launches = 0;
     if (!pts.empty()) {
@@ -1317,8 +1317,7 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
         DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches, [&] {
             return launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), (int)pts.size(), ptotal, ctx->L->stream);
         }));
-        for (Picture *P : luma_up) P->up_ready = true;
-        for (Picture *P : chroma_up) P->upc_ready = true;
+        for (Picture *P : building) P->mcp_ready = true;
     }
 
     // one device blob: component descriptors (main launch, then padding),
@@ -1333,11 +1332,12 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
             for (int c = 0; c < 3; ++c)
                 if (jobs[i].planes[c].width > (c ? g.CW : g.W) || jobs[i].planes[c].height > (c ? g.CH : g.H)) ++npad;
     }
-    size_t bytes = mc_align(sizeof(McComp) * (ncomp + npad));
+    const size_t pack_off = mc_align(sizeof(McComp) * (ncomp + npad));
+    size_t bytes = pack_off + mc_align(sizeof(McPackJob) * njobs);
     std::vector<size_t> tab_off(tabs.size()), mv_off((size_t)njobs, 0);
     for (size_t t = 0; t < tabs.size(); ++t) {
         tab_off[t] = bytes;
-        bytes += mc_align(tabs[t].hw.size()) + mc_align(tabs[t].vw.size());
+        bytes += mc_align(tabs[t].hw.size()) + mc_align(sizeof(int) * tabs[t].vw.size());
     }
     for (int i = 0; i < njobs; ++i) {
         if (mj[i].est) continue;
@@ -1350,11 +1350,19 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
         DME_TRY(ctx->d_mc.ensure(bytes));
     }
+    size_t nunits = 0;
+    for (int i = 0; i < njobs; ++i) nunits += (size_t)mj[i].g.xnb * mj[i].g.ynb;
+    if (ctx->d_mc_units.n < sizeof(int4) * 2 * nunits) {
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        DME_TRY(ctx->d_mc_units.ensure(sizeof(int4) * 2 * nunits));
+    }
     char *dev = ctx->d_mc.as<char>();
     std::vector<char> blob(bytes, 0);
+    McPackJob *packs = reinterpret_cast<McPackJob *>(blob.data() + pack_off);
+    int units_done = 0;
     for (size_t t = 0; t < tabs.size(); ++t) {
         std::memcpy(&blob[tab_off[t]], tabs[t].hw.data(), tabs[t].hw.size());
-        std::memcpy(&blob[tab_off[t] + mc_align(tabs[t].hw.size())], tabs[t].vw.data(), tabs[t].vw.size());
+        std::memcpy(&blob[tab_off[t] + mc_align(tabs[t].hw.size())], tabs[t].vw.data(), sizeof(int) * tabs[t].vw.size());
     }
     McComp *cs = reinterpret_cast<McComp *>(blob.data());
     int total = 0, ptot = 0;
@@ -1392,6 +1400,15 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
             std::memcpy(&blob[o], j.mv->sb_split, sizeof(int32_t) * S);
             split = reinterpret_cast<const int *>(dev + o);
         }
+        McPackJob &pk = packs[i];
+        pk.vec[0] = vec[0];
+        pk.vec[1] = vec[1];
+        pk.mode = mode;
+        for (int c = 0; c < 3; ++c) pk.dc[c] = dc[c];
+        pk.out = ctx->d_mc_units.as<int4>() + 2 * (size_t)units_done;
+        pk.blocks = (int)B;
+        pk.offset = units_done;
+        units_done += (int)B;
         for (int c = 0; c < 3; ++c) {
             McComp &C = cs[3 * i + c];
             const McTables &T = tabs[2 * i + (c ? 1 : 0)];
@@ -1399,16 +1416,14 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
             const int xblen = c ? g.cblen[0] : g.blen[2][0], yblen = c ? g.cblen[1] : g.blen[2][1];
             const int xbsep = c ? g.cbsep[0] : g.bsep[2][0], ybsep = c ? g.cbsep[1] : g.bsep[2][1];
             const Picture *R0 = m.ref[0], *R1 = m.ref[1];
-            C.up[0] = c == 0 ? R0->up.as<int16_t>() : R0->upc[c - 1].as<int16_t>();
-            C.up[1] = c == 0 ? R1->up.as<int16_t>() : R1->upc[c - 1].as<int16_t>();
+            C.up[0] = R0->mcp[c].as<int16_t>();
+            C.up[1] = R1->mcp[c].as<int16_t>();
             C.out = j.planes[c].data;
-            C.vec[0] = vec[0];
-            C.vec[1] = vec[1];
-            C.mode = mode;
+            C.unit = pk.out;
             C.split = split;
-            C.dc = dc[c];
+            C.comp = c;
             C.hw = reinterpret_cast<const signed char *>(dev + to);
-            C.vw = reinterpret_cast<const signed char *>(dev + to + mc_align(T.hw.size()));
+            C.vw = reinterpret_cast<const int *>(dev + to + mc_align(T.hw.size()));
             C.lx = j.planes[c].width;
             C.ly = j.planes[c].height;
             C.pw = c ? g.CW : g.W;
@@ -1430,7 +1445,7 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
             C.mmx = j.params.fp_semantics == DME_FP_MMX;
             C.tiles_x = (C.lx + 63) / 64;
             C.offset = total;
-            total += C.tiles_x * ((C.ly + 3) / 4);
+            total += C.tiles_x * ((C.ly + kMcTileRows - 1) / kMcTileRows);
             if (j.direction == DME_MC_ADD && (C.lx > C.pw || C.ly > C.ph)) {
                 cs[pi] = C;
                 cs[pi].offset = ptot;
@@ -1441,6 +1456,9 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
     }
     DME_TRY(upload(ctx, ctx->d_mc, blob.data(), bytes));
     const McComp *dcs = ctx->d_mc.as<McComp>();
+    DME_TRY(timed_launch(ctx, KK_MC_PACK, launches, [&] {
+        return launch_mc_pack(reinterpret_cast<const McPackJob *>(dev + pack_off), njobs, units_done, ctx->L->stream);
+    }));
     DME_TRY(timed_launch(ctx, KK_MC, launches,
                          [&] { return launch_mc(dcs, (int)ncomp, total, false, ctx->L->stream); }));
     if (npad)
@@ -1524,8 +1542,7 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
             t.oh = P->H;
             t.offset = 0;
             DME_TRY(upload(ctx, ctx->L->d_planes, &t, sizeof(t)));
-            HIP_TRY(launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), 1, ((P->W + 63) / 64) * ((P->H + 3) / 4),
-                                     ctx->L->stream));
+            HIP_TRY(launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), 1, upconvert_tiles(P->W, P->H), ctx->L->stream));
             HIP_TRY(hipStreamSynchronize(ctx->L->stream));
             ctx->L->arena_used = 0;
             P->up_ready = true;
@@ -1554,7 +1571,7 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
         if (pic_num < 0 || kv.first == pic_num) {
             kv.second->pyr_levels = 0;
             kv.second->up_ready = false;
-            kv.second->upc_ready = false;
+            kv.second->mcp_ready = false;
             kv.second->qp_ready = false;
             kv.second->b8_ready = false;
         }

@@ -262,16 +262,31 @@ __global__ __launch_bounds__(256) void k_downconvert(const PlaneTask *__restrict
 
 // ---------------------------------------------------------------------------
 // K2: UpConverter::DoUpConverter (upconvert.cpp:55-200), clip [-128,127].
-// One thread per source sample writes its 2x2 quad of the up plane.  The
-// accumulator is a short in the reference; |sum| <= 16 + 32*255 fits, so int
-// arithmetic gives the same value before the (short) >> 5.
+// The accumulator is a short in the reference; |sum| <= 16 + 32*255 fits, so
+// int arithmetic gives the same value before the (short) >> 5.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int up_tap8(const int s[8]) {
     const int sum = 16 + (s[3] + s[4]) * 21 + (s[2] + s[5]) * -7 + (s[1] + s[6]) * 3 + (s[0] + s[7]) * -1;
     return clampi(sum >> 5, -128, 127);
 }
 
+// A workgroup filters a 64 x 16 tile of source samples: the tile and its
+// apron (4 columns left, 4 right, 3 rows above, 4 below, clamped to the
+// picture) are staged in LDS once, the vertical filter of every staged column
+// goes to LDS, and each thread then writes the 2 x 2 quads of its column for
+// 4 rows, a dword (two horizontally adjacent up samples) per store.
+constexpr int UC_IW = kUpTileW + 8, UC_IH = kUpTileH + 7;  // columns x0 - 4 .. x0 + 67
+
+__device__ __forceinline__ int up_tap8_lds(const int16_t *r) {
+    const int sum = 16 + (r[3] + r[4]) * 21 + (r[2] + r[5]) * -7 + (r[1] + r[6]) * 3 + (r[0] + r[7]) * -1;
+    return clampi(sum >> 5, -128, 127);
+}
+
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16); }
+
 __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__ pt, int npt, int total) {
+    __shared__ int16_t s_in[UC_IH][UC_IW + 1];
+    __shared__ int16_t s_vt[kUpTileH][UC_IW + 1];
     const int tile = blockIdx.x;
     if (tile >= total) return;
     int lo = 0, hi = npt - 1;
@@ -283,30 +298,37 @@ __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__
     const PlaneTask p = pt[lo];
     const int local = tile - p.offset;
     const int W = p.ow, H = p.oh;
-    const int tx = (W + 63) / 64;
-    const int x = (local % tx) * 64 + (threadIdx.x & 63);
-    const int y = (local / tx) * 4 + wave_id();
+    const int tx = (W + kUpTileW - 1) / kUpTileW;
+    const int x0 = (local % tx) * kUpTileW, y0 = (local / tx) * kUpTileH;
     const int16_t *__restrict__ in = p.in;
-    int16_t *__restrict__ up = p.out;
-    if (x >= W || y >= H) return;
-    int rowi[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) rowi[t] = clampi(y - 3 + t, 0, H - 1);
-    int ev[8], vt[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int xx = clampi(x - 3 + t, 0, W - 1);
-        int s[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) s[q] = in[(size_t)rowi[q] * W + xx];
-        ev[t] = s[3];
-        vt[t] = up_tap8(s);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < UC_IH * UC_IW; i += 256) {
+        const int r = i / UC_IW, c = i - r * UC_IW;
+        s_in[r][c] = in[(size_t)clampi(y0 - 3 + r, 0, H - 1) * W + clampi(x0 - 4 + c, 0, W - 1)];
     }
-    const size_t UW = 2 * (size_t)W;
-    up[(2 * y) * UW + 2 * x] = (int16_t)ev[3];
-    up[(2 * y) * UW + 2 * x + 1] = (int16_t)up_tap8(ev);
-    up[(2 * y + 1) * UW + 2 * x] = (int16_t)vt[3];
-    up[(2 * y + 1) * UW + 2 * x + 1] = (int16_t)up_tap8(vt);
+    __syncthreads();
+    for (int i = tid; i < kUpTileH * UC_IW; i += 256) {
+        const int r = i / UC_IW, c = i - r * UC_IW;
+        int16_t col[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) col[t] = s_in[r + t][c];
+        s_vt[r][c] = (int16_t)up_tap8_lds(col);
+    }
+    __syncthreads();
+    const int lane = tid & 63, x = x0 + lane;
+    if (x >= W) return;
+    uint32_t *__restrict__ up = reinterpret_cast<uint32_t *>(p.out);  // two samples per dword, W per up row
+#pragma unroll
+    for (int k = 0; k < kUpTileH / 4; ++k) {
+        const int r = wave_id() * (kUpTileH / 4) + k, y = y0 + r;
+        if (y >= H) break;
+        const int16_t *ri = &s_in[r + 3][lane + 1];  // columns x - 3 .. x + 4
+        const int16_t *rv = &s_vt[r][lane + 1];
+        const int e = ri[3], h = up_tap8_lds(ri), v = rv[3], d = up_tap8_lds(rv);
+        uint32_t *u0 = up + (size_t)(2 * y) * W, *u1 = up + (size_t)(2 * y + 1) * W;
+        u0[x] = pack16(e, h);
+        u1[x] = pack16(v, d);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2967,30 +2989,9 @@ hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream
 // and sums their contributions directly: every sample is read and written
 // once, no strip, no atomics.  The reference sums in 16-bit lanes that wrap;
 // wrapping addition is order-independent, so an int sum truncated to 16 bits
-// at the end is exact.  Workgroup = 64 x 4 samples, one row per wave.
+// at the end is exact.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int mc_bchk(int v, int max) { return v < 0 ? 0 : (v >= max ? max - 1 : v); }
-
-// BlockPixelPred (mot_comp.cpp:846-1230) for one sample at (x, y): the
-// reads clamp (BChk) to [0, 2 size - 2] of the up-converted plane; the
-// in-bounds branches of the reference give the same values.
-__device__ __forceinline__ int mc_pred(const int16_t DME_G *__restrict__ up, int uw, int tx, int ty, int prec, int x,
-                                       int y, dme_mv mv) {
-    if (prec <= 1) {
-        const int ux = prec == 0 ? 2 * (x + mv.x) : 2 * x + mv.x;
-        const int uy = prec == 0 ? 2 * (y + mv.y) : 2 * y + mv.y;
-        return up[(size_t)mc_bchk(uy, ty) * uw + mc_bchk(ux, tx)];
-    }
-    const int sh = prec - 1, one = 1 << sh;  // quarter: half-pel plane + 1 bit; eighth: + 2 bits
-    const int ux = 2 * x + (mv.x >> sh), uy = 2 * y + (mv.y >> sh);
-    const int fx = mv.x & (one - 1), fy = mv.y & (one - 1);
-    const int x0 = mc_bchk(ux, tx), x1 = mc_bchk(ux + 1, tx);
-    const int16_t DME_G *r0 = up + (size_t)mc_bchk(uy, ty) * uw;
-    const int16_t DME_G *r1 = up + (size_t)mc_bchk(uy + 1, ty) * uw;
-    const int v = (one - fx) * (one - fy) * r0[x0] + fx * (one - fy) * r0[x1] + (one - fx) * fy * r1[x0] +
-                  fx * fy * r1[x1];
-    return (v + (1 << (2 * sh - 1))) >> (2 * sh);
-}
 
 // Split mode as the reference's switch reads it: 0 and 1, anything else 2.
 __device__ __forceinline__ int mc_split(int s) { return (s == 0 || s == 1) ? s : 2; }
@@ -3005,47 +3006,134 @@ __device__ __forceinline__ int mc_comp_of(const McComp *__restrict__ cs, int ncs
     return lo;
 }
 
+// One thread owns a column segment of MC_ROWS samples; a workgroup is 64
+// columns by 4 x MC_ROWS rows.  The units covering the thread's column are
+// found once per block row the segment meets (their weights across, modes,
+// vectors and clamped columns do not change down the column), then each row
+// only adds its taps.  Row tests and the weights down depend on the row
+// alone, so they are uniform across the wave.
+constexpr int MC_ROWS = kMcSegRows;
+
+// Where a unit's prediction from one reference reads: the clamped columns j
+// and x1 of the up-converted plane, the row offset, the bilinear fractions.
+struct McTap {
+    int j, x1, dy, fx, fy;
+};
+
+__device__ __forceinline__ McTap mc_tap(int prec, int x, dme_mv mv, int uw) {
+    McTap t;
+    int dx;
+    if (prec == 0) {
+        dx = 2 * mv.x;
+        t.dy = 2 * mv.y;
+        t.fx = t.fy = 0;
+    } else if (prec == 1) {
+        dx = mv.x;
+        t.dy = mv.y;
+        t.fx = t.fy = 0;
+    } else {
+        const int sh = prec - 1;
+        dx = mv.x >> sh;
+        t.dy = mv.y >> sh;
+        t.fx = mv.x & ((1 << sh) - 1);
+        t.fy = mv.y & ((1 << sh) - 1);
+    }
+    // BChk against 2 size - 1 (mot_comp.cpp:1007-1008): columns 0 .. uw - 2
+    t.j = min(max(2 * x + dx, 0), uw - 2);
+    t.x1 = min(max(2 * x + dx + 1, 0), uw - 2);
+    return t;
+}
+
+// BlockPixelPred (mot_comp.cpp:846-1230) for one sample of row y; BIL:
+// quarter or eighth pel (bilinear between four up-converted samples), else
+// one sample.  Rows clamp like columns (BChk), so any y reads inside the plane.
+template <bool BIL>
+__device__ __forceinline__ int mc_eval(const int16_t DME_G *__restrict__ up, int uw, int ty, int sh, int y,
+                                       const McTap &t) {
+    const int r0 = mc_bchk(2 * y + t.dy, ty);
+    const int16_t DME_G *ra = up + r0 * uw;
+    if (!BIL) return ra[t.j];
+    const int r1 = mc_bchk(2 * y + t.dy + 1, ty);
+    const int16_t DME_G *rb = up + r1 * uw;
+    const int one = 1 << sh;
+    const int v = (one - t.fx) * (one - t.fy) * ra[t.j] + t.fx * (one - t.fy) * ra[t.x1] +
+                  (one - t.fx) * t.fy * rb[t.j] + t.fx * t.fy * rb[t.x1];
+    return (v + (1 << (2 * sh - 1))) >> (2 * sh);
+}
+
+// Adds one unit's weighted prediction to the segment's rows; rows outside the
+// unit's window have weight 0 and are skipped (a uniform branch).
+template <bool BIL, bool TWO>
+__device__ __forceinline__ void mc_rows(int (&S)[kMcSegRows], const int16_t DME_G *__restrict__ upa,
+                                        const int16_t DME_G *__restrict__ upb, int uw, int ty, int sh, int y0,
+                                        const int (&W)[kMcSegRows], int H, const McTap &ta, const McTap &tb) {
+#pragma unroll
+    for (int i = 0; i < kMcSegRows; ++i) {
+        const int y = y0 + i;
+        if (W[i] == 0) continue;
+        int val = mc_eval<BIL>(upa, uw, ty, sh, y, ta);
+        if (TWO) val = (val + mc_eval<BIL>(upb, uw, ty, sh, y, tb) + 1) >> 1;
+        S[i] += val * (W[i] * H);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_mc(const McComp *__restrict__ cs, int ncs, int total) {
     const int tile = blockIdx.x;
     if (tile >= total) return;
     const McComp &C = cs[mc_comp_of(cs, ncs, tile)];
     const int local = tile - C.offset;
     const int x = (local % C.tiles_x) * 64 + (threadIdx.x & 63);
-    const int y = (local / C.tiles_x) * 4 + wave_id();
-    if (x >= C.lx || y >= C.ly) return;
-    int16_t DME_G *o = C.out + (size_t)y * C.lx + x;
-    if (x >= C.pw || y >= C.ph) {
+    const int y0 = (local / C.tiles_x) * (4 * MC_ROWS) + wave_id() * MC_ROWS;
+    if (x >= C.lx || y0 >= C.ly) return;
+    int16_t DME_G *col = C.out + x;
+    if (x >= C.pw || y0 >= C.ph) {
         // subtracting zeroes the padding (mot_comp.cpp:449-455, 511-522);
         // adding replicates the last column and row, in k_mc_pad
-        if (C.direction == DME_MC_SUBTRACT) *o = 0;
+        if (C.direction == DME_MC_SUBTRACT)
+#pragma unroll
+            for (int i = 0; i < MC_ROWS; ++i)
+                if (y0 + i < C.ly) col[(size_t)(y0 + i) * C.lx] = 0;
         return;
     }
-    const int uw = 2 * C.pw, tx = 2 * C.pw - 1, ty = 2 * C.ph - 1;
-    const int r = (y + C.yoff) / C.ybsep;  // the block row whose separation holds y
-    const int c = (x + C.xoff) / C.xbsep;  // the block column whose separation holds x
-    int S = 0;
-    for (int k = 0; k < 2; ++k) {
-        const int rb = r - 1 + k;
-        if (rb < 0 || rb >= C.nby) continue;
-        const int py = rb * C.ybsep - C.yoff, vy = y - py;
-        if (vy >= C.yblen) continue;
+    const int uw = 2 * C.pw, ty = 2 * C.ph - 1;
+    const int yend = min(y0 + MC_ROWS, C.ph);  // rows of the picture in this segment
+    const int c = (x + C.xoff) / C.xbsep;      // the block column whose separation holds x
+    int S[MC_ROWS];
+#pragma unroll
+    for (int i = 0; i < MC_ROWS; ++i) S[i] = 0;
+    const int rb0 = max(0, (y0 + C.yoff) / C.ybsep - 1);
+    const int rb1 = min(C.nby - 1, (yend - 1 + C.yoff) / C.ybsep);
+    for (int rb = rb0; rb <= rb1; ++rb) {
+        const int py = rb * C.ybsep - C.yoff;
+        const int ylo = max(py, y0), yhi = min(py + C.yblen, yend);
+        if (ylo >= yhi) continue;
         // weight class down (mot_comp.cpp:339-371): top, inside, bottom
         const int ycls = py < 0 ? 0 : (rb + 1 < C.nby ? 1 : 2);
-        const int V = C.vw[ycls * C.yblen + vy];
+        // the weights down of the segment's rows (0 outside the window)
+        const int DME_G *V = C.vw + ycls * C.yblen;
+        int W[MC_ROWS];
+#pragma unroll
+        for (int i = 0; i < MC_ROWS; ++i) {
+            const int y = y0 + i;
+            W[i] = (y >= ylo && y < yhi) ? V[min(max(y - py, 0), C.yblen - 1)] : 0;
+        }
         const int DME_G *srow = C.split + (size_t)(rb >> 2) * C.xsb;
-        // units across: the one holding block c (if in use), then the one before
+        // units across: the one holding block c (if in use), then the one
+        // before, in this superblock or the previous one (both splits are
+        // loaded at once)
+        const int cb = min(c, C.nbx - 1);
+        const int sa = mc_split(srow[cb >> 2]), sbp = mc_split(srow[max((cb >> 2) - 1, 0)]);
         int ub[2] = {-1, -1}, sp[2] = {0, 0}, pb;
         if (c < C.nbx) {
-            const int s1 = mc_split(srow[c >> 2]);
-            const int u1 = c & ~((4 >> s1) - 1);
+            const int u1 = c & ~((4 >> sa) - 1);
             ub[1] = u1;
-            sp[1] = s1;
+            sp[1] = sa;
             pb = u1 - 1;
         } else {
             pb = C.nbx - 1;
         }
         if (pb >= 0) {
-            const int s0 = mc_split(srow[pb >> 2]);
+            const int s0 = (pb >> 2) == (cb >> 2) ? sa : sbp;
             const int k0 = 4 >> s0, u0 = pb & ~(k0 - 1);
             if (x - (u0 * C.xbsep - C.xoff) < k0 * C.xbsep + C.xov) {
                 ub[0] = u0;
@@ -3060,32 +3148,73 @@ __global__ __launch_bounds__(256) void k_mc(const McComp *__restrict__ cs, int n
             const int xcls = (px >= 0 && u + ku < C.nbx) ? 1 : (px < 0 ? 0 : 2);
             const int H = C.hw[(s * 3 + xcls) * C.hpitch + (x - px)];
             const size_t b = (size_t)rb * C.xnb + u;
-            const int mode = C.mode[b];
-            int val;
+            const int4 rv = C.unit[2 * b], rm = C.unit[2 * b + 1];
+            const int mode = rm.x;
             if (mode == DME_REF1_ONLY || mode == DME_REF2_ONLY || mode == DME_REF1AND2) {
-                dme_mv m1 = C.vec[0][b], m2 = C.vec[1][b];
-                m1.x >>= C.shift;
-                m1.y >>= C.shift;
-                m2.x >>= C.shift;
-                m2.y >>= C.shift;
-                if (mode == DME_REF1_ONLY) val = mc_pred(C.up[0], uw, tx, ty, C.prec, x, y, m1);
-                else if (mode == DME_REF2_ONLY) val = mc_pred(C.up[1], uw, tx, ty, C.prec, x, y, m2);
-                else val = (mc_pred(C.up[0], uw, tx, ty, C.prec, x, y, m1) + mc_pred(C.up[1], uw, tx, ty, C.prec, x, y, m2) + 1) >> 1;
+                dme_mv m1, m2;
+                m1.x = rv.x >> C.shift;
+                m1.y = rv.y >> C.shift;
+                m2.x = rv.z >> C.shift;
+                m2.y = rv.w >> C.shift;
+                const int16_t DME_G *upa = mode == DME_REF2_ONLY ? C.up[1] : C.up[0];
+                const McTap ta = mc_tap(C.prec, x, mode == DME_REF2_ONLY ? m2 : m1, uw);
+                const McTap tb = mc_tap(C.prec, x, m2, uw);
+                const int sh = C.prec - 1;
+                if (C.prec >= 2) {
+                    if (mode == DME_REF1AND2)
+                        mc_rows<true, true>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
+                    else
+                        mc_rows<true, false>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
+                } else {
+                    if (mode == DME_REF1AND2)
+                        mc_rows<false, true>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
+                    else
+                        mc_rows<false, false>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
+                }
             } else {
-                val = C.dc[b];  // DCBlock
+                const int dcv = C.comp == 0 ? rm.y : (C.comp == 1 ? rm.z : rm.w);  // DCBlock
+#pragma unroll
+                for (int i = 0; i < MC_ROWS; ++i) S[i] += dcv * (W[i] * H);
             }
-            S += val * V * H;
         }
     }
-    const int s16 = (int16_t)S;
-    if (C.direction == DME_MC_SUBTRACT) {
-        *o = (int16_t)(*o - (int16_t)((s16 + 32) >> 6));
-    } else {
-        // the MMX add path rounds in 16-bit lanes over the first (width & ~3)
-        // samples (mot_comp_mmx.cpp:408-424)
-        const int p = (C.mmx && x < (C.pw & ~3)) ? ((int16_t)(s16 + 32)) >> 6 : (s16 + 32) >> 6;
-        *o = (int16_t)(*o + (int16_t)p);
+#pragma unroll
+    for (int i = 0; i < MC_ROWS; ++i) {
+        const int y = y0 + i;
+        if (y >= C.ly) break;
+        int16_t DME_G *o = col + (size_t)y * C.lx;
+        if (y >= C.ph) {
+            if (C.direction == DME_MC_SUBTRACT) *o = 0;
+            continue;
+        }
+        const int s16 = (int16_t)S[i];
+        if (C.direction == DME_MC_SUBTRACT) {
+            *o = (int16_t)(*o - (int16_t)((s16 + 32) >> 6));
+        } else {
+            // the MMX add path rounds in 16-bit lanes over the first
+            // (width & ~3) samples (mot_comp_mmx.cpp:408-424)
+            const int p = (C.mmx && x < (C.pw & ~3)) ? ((int16_t)(s16 + 32)) >> 6 : (s16 + 32) >> 6;
+            *o = (int16_t)(*o + (int16_t)p);
+        }
     }
+}
+
+// Packs the MvData k_mc reads per block into two 16-byte records, so that a
+// unit's vectors, mode and DC values arrive with one round trip.
+__global__ __launch_bounds__(256) void k_mc_pack(const McPackJob *__restrict__ jobs, int njobs, int total) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].offset <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const McPackJob &J = jobs[lo];
+    const int b = i - J.offset;
+    const dme_mv a = J.vec[0][b], v = J.vec[1][b];
+    J.out[2 * b] = make_int4(a.x, a.y, v.x, v.y);
+    J.out[2 * b + 1] = make_int4(J.mode[b], J.dc[0][b], J.dc[1][b], J.dc[2][b]);
 }
 
 // Adding: samples beyond the picture take the last column's value, rows below
@@ -3101,7 +3230,14 @@ __global__ __launch_bounds__(256) void k_mc_pad(const McComp *__restrict__ cs, i
     C.out[(size_t)y * C.lx + x] = C.out[(size_t)min(y, C.ph - 1) * C.lx + min(x, C.pw - 1)];
 }
 
+hipError_t launch_mc_pack(const McPackJob *jobs, int njobs, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mc_pack, dim3((total + 255) / 256), dim3(256), 0, st, jobs, njobs, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st) {
+    // total counts 64 x 4 tiles for k_mc_pad and 64 x (4 MC_ROWS) tiles for k_mc
     if (total <= 0) return hipSuccess;
     if (pad) hipLaunchKernelGGL(k_mc_pad, dim3(total), dim3(256), 0, st, cs, ncs, total);
     else hipLaunchKernelGGL(k_mc, dim3(total), dim3(256), 0, st, cs, ncs, total);

@@ -123,3 +123,26 @@ def load_case(name):
     load_mv(me, {k.split("/")[1]: z[k] for k in z.files if k.startswith(name + "/")})
     p.mv_precision = meta["precision"]
     return p, refs, me, padded(src, tuple(meta["pad"])), meta, z
+
+
+def ramp_picture(start, width=352, height=288):
+    """PicturesTest::setupPicture (unit_tests/frames_test.cpp:49-80): every
+    component filled in raster order with start+1, start+2, ... kept to a
+    signed char, as the reference's motion-compensation unit tests use."""
+    out = []
+    for w, h in ((width, height), (width // 2, height // 2), (width // 2, height // 2)):
+        k = np.arange(1, w * h + 1, dtype=np.int64) + start
+        out.append((((k + 128) % 256) - 128).astype(np.int16).reshape(h, w))
+    return tuple(out)
+
+
+def uniform_mv(me, mode, v1, v2, split):
+    """setupMV1Data / setupMV2Data (unit_tests/motion_comp_test.cpp:70-99):
+    one mode and one vector per reference for every block; DC 0."""
+    me.mode[...] = mode
+    me.vectors[0]["x"], me.vectors[0]["y"] = v1
+    me.vectors[1]["x"], me.vectors[1]["y"] = v2
+    for a in me.dc:
+        a[...] = 0
+    me.sb_split[...] = split
+    return me

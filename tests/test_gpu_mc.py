@@ -160,3 +160,31 @@ def test_mc_rejects_bad_arguments(ctx):
         assert np.array_equal(planes[c], fresh[c])
     for n in nums + [510]:
         ctx.drop(n)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_reference_unit_test_ref1_plus_ref2_is_twice_bipred(ctx, prec):
+    """unit_tests/motion_comp_test.cpp:378-460 (testRef1and2) through the GPU,
+    and bit-exact against the oracle on the same calls."""
+    refs = [M.ramp_picture(0), M.ramp_picture(50)]
+    nums = _upload_refs(ctx, refs, 600)
+    p = make_params(12, 12, 8, 8, precision=min(prec, 2), depth=2, lam=10.0)
+    p.mv_precision = prec
+    g = geometry(make_params(12, 12, 8, 8, precision=2, depth=2, lam=10.0), 352, 288)
+    zero = [np.zeros((288, 352), np.int16), np.zeros((144, 176), np.int16), np.zeros((144, 176), np.int16)]
+    calls = [(abi.DME_REF1_ONLY, (5, 5), (5, 5)), (abi.DME_REF2_ONLY, (7, 3), (7, 3))]
+    two, two_o = [a.copy() for a in zero], [a.copy() for a in zero]
+    for mode, v1, v2 in calls:
+        me = M.uniform_mv(MEData(g), mode, v1, v2, 1)
+        ctx.motion_compensate(p, 352, 288, me, nums, ADD, two)
+        O.motion_compensate("oracle", p, refs, me, 2, ADD, two_o)
+    bi, bi_o = [a.copy() for a in zero], [a.copy() for a in zero]
+    me = M.uniform_mv(MEData(g), abi.DME_REF1AND2, (5, 5), (7, 3), 1)
+    for _ in range(2):
+        ctx.motion_compensate(p, 352, 288, me, nums, ADD, bi)
+        O.motion_compensate("oracle", p, refs, me, 2, ADD, bi_o)
+    for c in range(3):
+        assert np.array_equal(two[c], two_o[c]) and np.array_equal(bi[c], bi_o[c]), c
+        assert np.abs(two[c].astype(int) - bi[c]).max() <= 5, c
+    for n in nums:
+        ctx.drop(n)

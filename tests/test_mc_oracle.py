@@ -7,7 +7,12 @@ restatement of MotionCompensator::CompensatePicture, mot_comp.cpp:63-1230).
    references, scrambled vectors/modes/splits, and DC values that drive the
    16-bit OBMC sums through wrap-around (MMX and plain-C builds differ there).
 2. Against the reference build itself, live, where oracle/_ref exists.
-3. Against what the mathematics fixes, independently of the oracle's code:
+3. The reference's own unit test of this path (unit_tests/motion_comp_test.cpp:
+   testRef1and2): adding the ref-1 and the ref-2 predictions one after the
+   other comes within 5 of adding the bi-prediction twice, every precision.
+   (Its other cases compensate an intra picture, which CompensatePicture
+   leaves unchanged, mot_comp.cpp:142: callers compensate inter pictures only.)
+4. Against what the mathematics fixes, independently of the oracle's code:
    the OBMC weights of every block arrangement sum to 64 at every sample, so
    a flat reference predicts itself; zero-fraction vectors copy reference
    samples (clamped at the edges); subtracting then adding the same
@@ -158,3 +163,27 @@ def test_padding_rules():
         a = planes[c]
         assert np.array_equal(a[:h, w:], np.repeat(a[:h, w - 1:w], a.shape[1] - w, axis=1)), c
         assert np.array_equal(a[h:, :], np.repeat(a[h - 1:h, :], a.shape[0] - h, axis=0)), c
+
+
+# ---- the reference's unit test (motion_comp_test.cpp:378-460) ---------------
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+@pytest.mark.parametrize("split", [0, 1, 2])
+def test_reference_unit_test_ref1_plus_ref2_is_twice_bipred(prec, split):
+    refs = [M.ramp_picture(0), M.ramp_picture(50)]
+    p = make_params(12, 12, 8, 8, precision=min(prec, 2), depth=2, lam=10.0)
+    p.mv_precision = prec
+    g = geometry(make_params(12, 12, 8, 8, precision=2, depth=2, lam=10.0), 352, 288)
+    zero = [np.zeros((288, 352), np.int16), np.zeros((144, 176), np.int16), np.zeros((144, 176), np.int16)]
+    two = [a.copy() for a in zero]
+    O.motion_compensate("oracle", p, refs, M.uniform_mv(MEData(g), abi.DME_REF1_ONLY, (5, 5), (5, 5), split), 2, ADD,
+                        two)
+    O.motion_compensate("oracle", p, refs, M.uniform_mv(MEData(g), abi.DME_REF2_ONLY, (7, 3), (7, 3), split), 2, ADD,
+                        two)
+    bi = [a.copy() for a in zero]
+    me = M.uniform_mv(MEData(g), abi.DME_REF1AND2, (5, 5), (7, 3), split)
+    O.motion_compensate("oracle", p, refs, me, 2, ADD, bi)
+    O.motion_compensate("oracle", p, refs, me, 2, ADD, bi)
+    for c in range(3):
+        assert np.abs(two[c].astype(int) - bi[c]).max() <= 5, c
+        assert np.abs(two[c]).max() > 0, c
