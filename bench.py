@@ -71,17 +71,17 @@ def _geometry_terms(geom, nrefs):
 def _profiled_traffic(kernel, batch):
     """HBM bytes per launch of `kernel` measured with rocprofv3 counters on
     this workload, from the summary kept in profiles/ (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "pel_bulk_1080p_2ref_batch64.md")
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    _, rest = line.split(":", 1)
-                    fields = dict(kv.strip().split("=") for kv in rest.split(","))
-                    if fields.get("kernel") == kernel and int(fields.get("batch", -1)) == batch:
-                        return float(fields["bytes"])
-    except (OSError, ValueError, KeyError):
-        return None
+    for name in ("pel_bulk_1080p_2ref_batch64.md", "mc_1080p_2ref_batch64.md"):
+        try:
+            with open(os.path.join(ROOT, "profiles", name)) as f:
+                for line in f:
+                    if line.startswith("traffic_bytes_per_launch:"):
+                        _, rest = line.split(":", 1)
+                        fields = dict(kv.strip().split("=") for kv in rest.split(","))
+                        if fields.get("kernel") == kernel and int(fields.get("batch", -1)) == batch:
+                            return float(fields["bytes"])
+        except (OSError, ValueError, KeyError):
+            continue
     return None
 
 
@@ -135,18 +135,39 @@ def cpu_baseline(frames, params, budget_s):
 
 
 def _mc_bytes(geom, me):
-    """Algorithmic HBM bytes of compensating one picture (k_mc): every sample
-    of the three arrays read and written (int16), the up-converted reference
-    planes (4 int16 per picture sample) read once per reference the sample's
-    blocks predict from (none for DC blocks, two for bi-prediction), and the
-    MvData."""
+    """Algorithmic HBM bytes of compensating one picture (k_mc), counted per
+    unit with no reuse between units, as the reference computes it
+    (CompensateBlock per unit, mot_comp.cpp:541-603): each unit's OBMC
+    window (clipped to the picture) reads 4 up-converted int16 samples per
+    sample from each reference its mode uses (none for DC, two for
+    bi-prediction); every sample of the three arrays is read and written once
+    (int16); plus the MvData."""
     import numpy as np
-    n = np.bincount(me.mode.ravel(), minlength=4)
-    refs_per_sample = float(n[1] + n[2] + 2 * n[3]) / float(me.mode.size)
-    samples = geom.width * geom.height + 2 * geom.chroma_width * geom.chroma_height
-    B = geom.xnum_blocks * geom.ynum_blocks
-    mv = B * (2 * 8 + 4 + 3 * 2) + geom.xnum_sb * geom.ynum_sb * 4
-    return samples * 4 + samples * 8 * refs_per_sample + mv
+    total = 0.0
+    xnb, ynb, xsb = geom.xnum_blocks, geom.ynum_blocks, geom.xnum_sb
+    refs_of = np.array([0, 1, 1, 2, 0, 0, 0, 0])
+    for c in range(2):
+        xbl, ybl = (geom.luma_blen[2] if c == 0 else geom.chroma_blen[2])
+        xbs, ybs = (geom.luma_bsep[2] if c == 0 else geom.chroma_bsep[2])
+        pw, ph = (geom.width, geom.height) if c == 0 else (geom.chroma_width, geom.chroma_height)
+        xoff, yoff, xov = (xbl - xbs) // 2, (ybl - ybs) // 2, xbl - xbs
+        nbx = min((pw + xbs + xoff - 1) // xbs, xnb)
+        nby = min((ph + ybs + yoff - 1) // ybs, ynb)
+        area_refs = 0
+        for rb in range(nby):
+            py = rb * ybs - yoff
+            rows = min(py + ybl, ph) - max(py, 0)
+            u = 0
+            while u < nbx:
+                sp = int(me.sb_split[rb // 4, u // 4])
+                k = 4 >> (sp if sp in (0, 1) else 2)
+                px = u * xbs - xoff
+                cols = min(px + k * xbs + xov, pw) - max(px, 0)
+                area_refs += rows * cols * refs_of[int(me.mode[rb, u]) & 7]
+                u += k
+        total += area_refs * 8 * (1 if c == 0 else 2) + pw * ph * 4 * (1 if c == 0 else 2)
+    B = xnb * ynb
+    return total + B * (2 * 8 + 4 + 3 * 2) + xsb * geom.ynum_sb * 4
 
 
 def mc_leg(ctx, jobs, frames, params, geom, B, args):
@@ -196,7 +217,7 @@ def mc_leg(ctx, jobs, frames, params, geom, B, args):
     out = {"value": B / dt, "unit": "pictures/s", "ms_per_step": dt * 1e3, "batch": B,
            "direction": "subtract", "mv_source": "device-side estimates of the same batch",
            "roofline": {"kernel": "k_mc", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": _profiled_traffic("k_mc", B),
                         "algorithmic_bytes_per_launch": alg / nl, "avg_launch_ms": ms / nl},
            "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
     for dp in arrays:
