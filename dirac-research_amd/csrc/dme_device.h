@@ -42,6 +42,8 @@ struct JobDesc {
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
     int lp_off;                                       // first position of the picture in the proposal tensors
     int lp_slot;                                      // its pair of refinement counters (k_lp_refine)
+    int full;                                         // full search: depth 0, zero window of fxr x fyr (k_pel_full)
+    int fxr[2], fyr[2];                               // its radius per reference, min(tdiff, 3) * range
     // pel search, per reference and level ([0] aliases the MEData arrays)
     dme_mv DME_G *lvec[2][DME_MAX_DEPTH + 1];
     dme_cost DME_G *lpc[2][DME_MAX_DEPTH + 1];

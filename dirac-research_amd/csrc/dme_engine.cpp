@@ -33,6 +33,8 @@ hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st);
 hipError_t launch_mc_pack(const McPackJob *jobs, int njobs, int total, hipStream_t st);
+hipError_t launch_pel_full(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int pp, int rp, int rows,
+                           hipStream_t st);
 hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
@@ -216,12 +218,12 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_COUNT
+    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack"};
+    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -386,6 +388,15 @@ static Picture *find_pic(dme_ctx *ctx, int num) {
 }
 
 // Allocates a job's MEData and scratch, fills its descriptor.
+// k_pel_full's LDS for blocks of xl x yl and a window of radius (xr, yr):
+// minima and the 5 x 5 table, the block, the window with margins of at least 2.
+constexpr size_t kFullLdsMax = 160 * 1024;
+static int full_pitch(int xl, int xr) { return (((xl + 2 * std::max(xr, 2) + 3) / 4) + 5) | 1; }
+static size_t full_lds_bytes(int xl, int yl, int xr, int yr) {
+    const size_t pp = (size_t)((xl + 15) / 16) * 4;
+    return sizeof(int) * (8 + 26) + 4 * (pp * 64 + (size_t)(yl + 2 * std::max(yr, 2)) * full_pitch(xl, xr));
+}
+
 static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs) {
     // buffers below may be reallocated: nothing queued may still use them
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
@@ -402,6 +413,18 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     }
     DME_TRY(make_geometry(p, P->W, P->H, J.g));
     const Geometry &g = J.g;
+    int fxr[2] = {0, 0}, fyr[2] = {0, 0};
+    if (p.full_search) {
+        for (int r = 0; r < 2; ++r) {
+            // m_tdiff and m_big_xr (pixel_match.cpp:79-80, 249-250); one reference: ref2 = ref1
+            const int td = std::abs((r < nrefs ? refs[r] : refs[0]) - pic_num);
+            fxr[r] = std::min(td, 3) * p.x_range_me;
+            fyr[r] = std::min(td, 3) * p.y_range_me;
+            if (full_lds_bytes(g.blen[2][0], g.blen[2][1], fxr[r], fyr[r]) > kFullLdsMax)
+                return set_error(DME_ERR_INVALID, "full-search window too large for one workgroup's LDS "
+                                                  "(reduce x_range_me / y_range_me)");
+        }
+    }
     J.pic_num = pic_num;
     J.nrefs = nrefs;
     J.refs[0] = refs[0];
@@ -447,6 +470,11 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     d.W = g.W; d.H = g.H; d.CW = g.CW; d.CH = g.CH;
     d.xnb = g.xnb; d.ynb = g.ynb; d.xsb = g.xsb; d.ysb = g.ysb; d.depth = g.depth; d.nrefs = nrefs;
     d.prec = p.mv_precision;
+    d.full = p.full_search;
+    for (int r = 0; r < 2; ++r) {
+        d.fxr[r] = fxr[r];
+        d.fyr[r] = fyr[r];
+    }
     d.step = p.mv_precision == 0 ? 1 : p.mv_precision;
     d.fp_sem = p.fp_semantics;
     for (int l = 0; l <= g.depth; ++l) {
@@ -572,13 +600,30 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
         int off = 0;
         for (size_t i = 0; i < js.size(); ++i)
             for (int r = 0; r < js[i]->nrefs; ++r) {
-                if (l > js[i]->g.depth) continue;
+                if (l > js[i]->g.depth || js[i]->p.full_search) continue;
                 const int cnt = js[i]->g.lxb[l] * js[i]->g.lyb[l];
                 t.push_back(Task{(int)i, r, cnt, off});
                 off += cnt;
             }
         bulk[l] = tt.add(t);
     }
+    // full-search jobs: their level-0 tables in k_pel_full
+    std::vector<Task> ft;
+    int foff = 0, fpp = 4, frp = 1, frows = 1;
+    if (stop == 0)
+        for (size_t i = 0; i < js.size(); ++i)
+            for (int r = 0; r < js[i]->nrefs; ++r) {
+                if (!js[i]->p.full_search) continue;
+                const Geometry &g = js[i]->g;
+                const int cnt = g.lxb[0] * g.lyb[0];
+                ft.push_back(Task{(int)i, r, cnt, foff});
+                foff += cnt;
+                const int xr = js[i]->desc.fxr[r], yr = js[i]->desc.fyr[r];
+                fpp = std::max(fpp, ((g.blen[2][0] + 15) / 16) * 4);
+                frp = std::max(frp, full_pitch(g.blen[2][0], xr));
+                frows = std::max(frows, g.blen[2][1] + 2 * std::max(yr, 2));
+            }
+    const TaskTable::Range full = tt.add(ft);
     std::vector<Task> wt;
     for (size_t i = 0; i < js.size(); ++i)
         for (int r = 0; r < js[i]->nrefs; ++r) wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
@@ -599,6 +644,10 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
             return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, bx, by, guide,
                                    ctx->L->stream);
         }));
+        if (l == 0 && full.total > 0)
+            DME_TRY(timed_launch(ctx, KK_PEL_FULL, launches, [&] {
+                return launch_pel_full(dj, dtask(ctx, full), full.n, full.total, fpp, frp, frows, ctx->L->stream);
+            }));
         DME_TRY(timed_launch(ctx, KK_PEL_WAVE, launches,
                              [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by, ctx->L->stream); }));
     }
@@ -1791,6 +1840,8 @@ extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int 
     if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
     JobState *J = job_for(ctx, pic_num);
     DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    if (J->p.full_search)
+        return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
     std::vector<JobState *> js{J};
     DME_TRY(run_jobs_learned(ctx, js));
     return export_job(ctx, *J, out);
@@ -1805,6 +1856,8 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
             if (jobs[k].pic_num == jobs[i].pic_num) return set_error(DME_ERR_INVALID, "a picture appears twice in a batch");
         JobState *J = job_for(ctx, jobs[i].pic_num);
         DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs));
+        if (J->p.full_search)
+            return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
         js.push_back(J);
     }
     DME_TRY(run_jobs_learned(ctx, js));

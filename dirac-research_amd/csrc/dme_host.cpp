@@ -32,8 +32,13 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
         return set_error(DME_ERR_INVALID, "mv_precision must be 0 (pel), 1 (half) or 2 (quarter)");
     if (p.fp_semantics != DME_FP_MMX && p.fp_semantics != DME_FP_C)
         return set_error(DME_ERR_INVALID, "fp_semantics must be DME_FP_MMX or DME_FP_C");
-    for (int i = 0; i < 7; ++i)
+    for (int i = 0; i < 4; ++i)
         if (p.reserved[i]) return set_error(DME_ERR_INVALID, "reserved params must be zero");
+    if (p.full_search != 0 && p.full_search != 1) return set_error(DME_ERR_INVALID, "full_search must be 0 or 1");
+    if (p.full_search && (p.x_range_me < 0 || p.y_range_me < 0 || p.x_range_me > 128 || p.y_range_me > 128))
+        return set_error(DME_ERR_INVALID, "full-search ranges must be in [0, 128]");
+    if (!p.full_search && (p.x_range_me || p.y_range_me))
+        return set_error(DME_ERR_INVALID, "x_range_me / y_range_me apply to full search only");
     if (p.xblen <= 0 || p.yblen <= 0 || p.xbsep <= 0 || p.ybsep <= 0)
         return set_error(DME_ERR_INVALID, "block lengths and separations must be positive");
     if (width < 24 || height < 24 || width > 16384 || height > 16384 || (width & 1) || (height & 1))
@@ -72,12 +77,15 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
     g.xnb = 4 * g.xsb;
     g.ynb = 4 * g.ysb;
 
-    if (p.depth < 0)
+    if (p.full_search)
+        g.depth = 0;  // no hierarchy (pixel_match.cpp:91, 140-155)
+    else if (p.depth < 0)
         g.depth = (int)std::min(std::log(((double)width) / 12.0) / std::log(2.0),
                                 std::log(((double)height) / 12.0) / std::log(2.0));
     else
         g.depth = p.depth;
-    if (g.depth < 1 || g.depth > DME_MAX_DEPTH) return set_error(DME_ERR_INVALID, "pyramid depth must be in [1, 8]");
+    if (!p.full_search && (g.depth < 1 || g.depth > DME_MAX_DEPTH))
+        return set_error(DME_ERR_INVALID, "pyramid depth must be in [1, 8]");
     g.lw[0] = width; g.lh[0] = height; g.lxb[0] = g.xnb; g.lyb[0] = g.ynb;
     int sf = 1, xn = g.xnb, yn = g.ynb;
     for (int i = 1; i <= g.depth; ++i) {

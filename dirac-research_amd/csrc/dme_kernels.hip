@@ -1052,6 +1052,112 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
 }
 
 // ---------------------------------------------------------------------------
+// Full search, table part (PixelMatcher::MatchPic with FullSearch,
+// pixel_match.cpp:91-155, 249-264, 365-370): no pyramid; at level 0 the zero
+// list is the whole window of radius (xr, yr) = min(tdiff, 3) * (x, y range)
+// and the prediction list has radius 1.  One workgroup per (picture,
+// reference, block) stages the block and its reference window once, as
+// biased bytes, and every thread takes candidates of the window; the first
+// strict-< minimum in AddNewVlist order is the smallest (SAD, index) pair.
+// The record is the level-0 record of k_pel_bulk (no guide): the zero-list
+// winner and the keys of the nine radius-1 windows centred within one pel of
+// zero, which k_pel_wave consumes unchanged.
+// ---------------------------------------------------------------------------
+// index of (dx, dy) in AddNewVlist's order over the (2xr+1) x (2yr+1) window
+__device__ __forceinline__ int full_enum(int dx, int dy, int xr) {
+    if (dy == 0) return dx == 0 ? 0 : (dx > 0 ? 2 * dx - 1 : -2 * dx);
+    return 1 + 2 * xr + (abs(dy) - 1) * 2 * (2 * xr + 1) + 2 * (dx + xr) + (dy < 0 ? 1 : 0);
+}
+
+// LDS of k_pel_full, in dwords: per-wave minima, the 5 x 5 SAD table around
+// zero, the block (rows of pp), the window (rows of rp, with margins of
+// rx, ry >= 2 so that the 5 x 5 neighbourhood of zero is staged too).
+struct FullLds {
+    int pp, rp, rows, pic;
+};
+
+__global__ __launch_bounds__(256) void k_pel_full(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                  int ntasks, int total, FullLds ly) {
+    extern __shared__ int s_dyn[];
+    unsigned long long *__restrict__ s_min = reinterpret_cast<unsigned long long *>(s_dyn);  // 4, 8-byte aligned
+    int *__restrict__ s_t25 = s_dyn + 8;
+    uint32_t *__restrict__ s_pic = reinterpret_cast<uint32_t *>(s_t25 + 26);
+    uint32_t *__restrict__ s_reg = s_pic + ly.pic;
+    const int gi = blockIdx.x;
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, local = gi - tk.offset;
+    const int X = J.lxb[0], Y = J.lyb[0];
+    const int xb = local % X, yb = local / X;
+    const int W = J.lw[0], H = J.lh[0];
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
+    int *__restrict__ rec = J.ptab[r] + (size_t)wf_pos(xb, yb, X, Y) * PEL_REC;
+    const int tid = threadIdx.x, wv = wave_id(), lane = tid & 63;
+    if (b.empty()) {
+        if (tid < PEL_REC) rec[tid] = tid == 0 ? 10000000 : (tid >= 8 && tid < 26 ? INT_MAX : 0);
+        return;
+    }
+    const int xr = J.fxr[r], yr = J.fyr[r];
+    const int rx = max(xr, 2), ry = max(yr, 2);
+    const int RW = b.xl + 2 * rx, RH = b.yl + 2 * ry;
+    const int pp = ((b.xl + 15) >> 4) * 4, rp = ly.rp;
+    // stage: wave 0 the block, every wave a quarter of the window's rows
+    if (wv == 0) wave_stage_u8(reinterpret_cast<uint8_t *>(s_pic), 4 * pp, J.pic[0], W, H, b.xp, b.yp, b.xl, b.yl, 4 * pp);
+    {
+        const int q = (RH + 3) >> 2, r0 = wv * q, nr = min(q, RH - r0);
+        if (nr > 0)
+            wave_stage_u8(reinterpret_cast<uint8_t *>(s_reg + r0 * rp), 4 * rp, J.ref[r][0], W, H, b.xp - rx,
+                          b.yp - ry + r0, RW, nr, 4 * rp);
+    }
+    __syncthreads();
+    const bool whole = (b.xl & 15) == 0;
+    auto sad_at = [&](int dx, int dy) {
+        return whole ? packed_sad_u8<false>(s_pic, pp, s_reg, rp, dx + rx, dy + ry, b.xl, b.yl)
+                     : packed_sad_u8<true>(s_pic, pp, s_reg, rp, dx + rx, dy + ry, b.xl, b.yl);
+    };
+    const int WX = 2 * xr + 1, nc = WX * (2 * yr + 1);
+    unsigned long long best = ~0ull;
+    for (int c = tid; c < nc; c += 256) {
+        const int dy = c / WX - yr, dx = c - (dy + yr) * WX - xr;
+        const unsigned long long key = ((unsigned long long)(unsigned)sad_at(dx, dy) << 32) | (unsigned)full_enum(dx, dy, xr);
+        best = key < best ? key : best;
+    }
+    if (tid < 25) s_t25[tid] = sad_at(tid % 5 - 2, tid / 5 - 2);
+    for (int m = 32; m >= 1; m >>= 1) {
+        const unsigned long long o = __shfl_xor(best, m, WAVE);
+        best = o < best ? o : best;
+    }
+    if (lane == 0) s_min[wv] = best;
+    __syncthreads();
+    if (wv != 0) return;
+    best = s_min[0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) best = s_min[w] < best ? s_min[w] : best;
+    // keys of the radius-1 windows centred within one pel of zero
+    int key = INT_MAX;
+    if (lane < 9) key = pel_window_min<1>(s_t25 + (lane / 3 + 1) * 5 + (lane % 3 + 1));
+    if (lane < 18) rec[8 + lane] = key;
+    if (lane == 0) {
+        const int k = (int)(best & 0xffffffffu);
+        int dx, dy;
+        if (k <= 2 * xr) {  // the centre row
+            dy = 0;
+            dx = k == 0 ? 0 : ((k & 1) ? (k + 1) >> 1 : -(k >> 1));
+        } else {
+            const int k2 = k - 1 - 2 * xr, row = k2 / (2 * WX), rem = k2 - row * 2 * WX;
+            dx = (rem >> 1) - xr;
+            dy = (rem & 1) ? -(row + 1) : row + 1;
+        }
+        rec[0] = (int)(best >> 32);
+        rec[1] = dx;
+        rec[2] = dy;
+        rec[3] = 0;
+        rec[4] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pel search, wavefront part (PixelMatcher::MatchPic/DoBlock,
 // pixel_match.cpp:242-380; BlockMatcher::FindBestMatchPel,
 // block_match.cpp:242-275).  One workgroup per (picture, reference), one
@@ -3228,6 +3334,22 @@ __global__ __launch_bounds__(256) void k_mc_pad(const McComp *__restrict__ cs, i
     const int y = (local / C.tiles_x) * 4 + wave_id();
     if (x >= C.lx || y >= C.ly || (x < C.pw && y < C.ph)) return;
     C.out[(size_t)y * C.lx + x] = C.out[(size_t)min(y, C.ph - 1) * C.lx + min(x, C.pw - 1)];
+}
+
+hipError_t launch_pel_full(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int pp, int rp, int rows,
+                           hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    FullLds ly;
+    ly.pp = pp;
+    ly.rp = rp;
+    ly.rows = rows;
+    ly.pic = pp * 64;  // blocks of up to 64 rows
+    const size_t bytes = sizeof(int) * (8 + 26) + sizeof(uint32_t) * ((size_t)ly.pic + (size_t)rows * rp);
+    if (bytes > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute((const void *)k_pel_full, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pel_full, dim3(total), dim3(256), bytes, st, jobs, tasks, ntasks, total, ly);
+    return hipGetLastError();
 }
 
 hipError_t launch_mc_pack(const McPackJob *jobs, int njobs, int total, hipStream_t st) {

@@ -17,7 +17,8 @@ from .abi import (COST_DTYPE, DME_FP_C, DME_FP_MMX, DME_INTRA, DME_REF1_ONLY, DM
                   DME_REF1AND2, MV_DTYPE, DmeError, check, lib, ptr)
 
 
-def make_params(xblen=12, yblen=12, xbsep=8, ybsep=8, precision=2, depth=-1, lam=10.0, fp=DME_FP_MMX):
+def make_params(xblen=12, yblen=12, xbsep=8, ybsep=8, precision=2, depth=-1, lam=10.0, fp=DME_FP_MMX,
+                full_search=False, x_range=32, y_range=32):
     p = abi.dme_params()
     p.xblen, p.yblen, p.xbsep, p.ybsep = xblen, yblen, xbsep, ybsep
     p.mv_precision = precision
@@ -25,6 +26,9 @@ def make_params(xblen=12, yblen=12, xbsep=8, ybsep=8, precision=2, depth=-1, lam
     p.lambda_ = lam
     p.fp_semantics = fp
     p.chroma_format = 0
+    if full_search:
+        p.full_search = 1
+        p.x_range_me, p.y_range_me = x_range, y_range
     return p
 
 

@@ -77,7 +77,16 @@ typedef struct dme_params {
     float   lambda;        /* ME Lagrangian (L1MELambda / L2MELambda)        */
     int32_t fp_semantics;  /* DME_FP_MMX (default) or DME_FP_C               */
     int32_t chroma_format; /* 0 = 4:2:0 (only value supported)               */
-    int32_t reserved[7];   /* must be zero                                   */
+    int32_t full_search;   /* EncoderParams::FullSearch (common.h:1308): no
+                              pyramid (depth is ignored); the zero list is
+                              the whole window of radius min(tdiff, 3) times
+                              the ranges below, tdiff = |reference - picture|
+                              picture numbers (pixel_match.cpp:79-80,
+                              249-264), and the prediction list has radius 1
+                              (pixel_match.cpp:365-370)                      */
+    int32_t x_range_me;    /* XRangeME / YRangeME (common.h:1311-1314), used  */
+    int32_t y_range_me;    /* only with full_search; the encoder default is 32 */
+    int32_t reserved[4];   /* must be zero                                   */
 } dme_params;
 
 /* Derived geometry for a picture of a given size (SetMotionParameters,

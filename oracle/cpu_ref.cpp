@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -135,14 +136,17 @@ int make_geo(const dme_params *p, int W, int H, Geo &g) {
     g.xnb = 4 * g.xsb;
     g.ynb = 4 * g.ysb;
 
-    // depth (pixel_match.cpp:94-95)
-    if (p->depth < 0) {
+    // depth (pixel_match.cpp:94-95); full search has no hierarchy (:91, 140-155)
+    if (p->full_search) {
+        if (p->x_range_me < 0 || p->y_range_me < 0) return fail("full-search ranges must not be negative");
+        g.depth = 0;
+    } else if (p->depth < 0) {
         g.depth = (int)std::min(std::log(((double)W) / 12.0) / std::log(2.0),
                                 std::log(((double)H) / 12.0) / std::log(2.0));
     } else {
         g.depth = p->depth;
     }
-    if (g.depth < 1 || g.depth > DME_MAX_DEPTH) return fail("pyramid depth out of range");
+    if (!p->full_search && (g.depth < 1 || g.depth > DME_MAX_DEPTH)) return fail("pyramid depth out of range");
 
     // pyramid sizes from full resolution (pixel_match.cpp:166-172) and
     // MEData grids (pixel_match.cpp:185-220)
@@ -495,16 +499,20 @@ struct PelLevel {
 };
 
 // MatchPic at one level against one reference (pixel_match.cpp:242-348).
+// full_xr >= 0 selects full search (:249-264, 365-370): the zero list has
+// radius (full_xr, full_yr) and the prediction list radius 1.
 void match_pic_pel(const Plane &pic, const Plane &ref, const OLB &bp, int level,
-                   const PelLevel *guide, PelLevel &out, double &terms) {
+                   const PelLevel *guide, PelLevel &out, double &terms, int full_xr = -1, int full_yr = -1) {
     const int r = std::min(level + 1, 5);
+    const int pr = full_xr >= 0 ? 1 : r;
     const int X = out.xnb, Y = out.ynb;
     out.vec.assign((size_t)X * Y, MV(0, 0));
     out.pc.assign((size_t)X * Y, Cost());
     for (auto &c : out.pc) c.total = 10000000.0f;
 
     std::vector<MV> zero_list, guide_list, pred_list;
-    window(MV(0, 0), r, r, zero_list);
+    if (full_xr >= 0) window(MV(0, 0), full_xr, full_yr, zero_list);
+    else window(MV(0, 0), r, r, zero_list);
 
     auto V = [&](int y, int x) -> MV & { return out.vec[(size_t)y * X + x]; };
 
@@ -519,19 +527,18 @@ void match_pic_pel(const Plane &pic, const Plane &ref, const OLB &bp, int level,
             window(MV(g.x * 2, g.y * 2), r, r, guide_list);
             lists.push_back(&guide_list);
         }
-        window(pred, r, r, pred_list);
+        window(pred, pr, pr, pred_list);
         lists.push_back(&pred_list);
 
         Cost &pc = out.pc[(size_t)y * X + x];
         float best = pc.total;
         MV best_mv = V(y, x);
         if (!b.empty()) {
-            std::vector<MV> seen;
+            std::unordered_set<long long> seen;
             for (const auto *l : lists)
                 for (const MV &c : *l) {
                     // AddVect's skip of duplicates (block_match.cpp:170-193)
-                    if (std::find(seen.begin(), seen.end(), c) != seen.end()) continue;
-                    seen.push_back(c);
+                    if (!seen.insert(((long long)c.x << 32) ^ (unsigned)c.y).second) continue;
                     terms += (double)b.xl * b.yl;
                     const int s = pel_sad(pic, ref, b, c);
                     if ((float)s < best) { best = (float)s; best_mv = c; }
@@ -1297,7 +1304,14 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
             for (int l = D; l >= 0; --l) {
                 lv[l].xnb = g.lxb[l];
                 lv[l].ynb = g.lyb[l];
-                match_pic_pel(pd[l], rd[r][l], g.l[2], l, l < D ? &lv[l + 1] : nullptr, lv[l], terms);
+                int fxr = -1, fyr = -1;
+                if (p->full_search) {
+                    // m_tdiff, m_big_xr (pixel_match.cpp:79-80, 249-250)
+                    const int td = std::abs(refs[r].pic_num - cur->pic_num);
+                    fxr = std::min(td, 3) * p->x_range_me;
+                    fyr = std::min(td, 3) * p->y_range_me;
+                }
+                match_pic_pel(pd[l], rd[r][l], g.l[2], l, l < D ? &lv[l + 1] : nullptr, lv[l], terms, fxr, fyr);
                 if (cap && cap->level_vectors[r][l])
                     for (size_t k = 0; k < lv[l].vec.size(); ++k) {
                         cap->level_vectors[r][l][k].x = lv[l].vec[k].x;

@@ -124,7 +124,7 @@ class Capture:
             self.s.ref_up[r] = self.ref_up[r].ctypes.data
 
 
-def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=()):
+def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=(), pic_nums=None):
     cur = np.ascontiguousarray(cur, np.int16)
     refs = [np.ascontiguousarray(r, np.int16) for r in refs]
     u = None if u is None else np.ascontiguousarray(u, np.int16)
@@ -132,8 +132,9 @@ def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=()):
     H, W = cur.shape
     geom = geometry(params, W, H)
     out = MEData(geom)
-    nums = [0, 2] if len(refs) == 2 else [0]
-    cp = _pic(1, cur, u, v)
+    # picture numbers (they set the temporal distances of full search)
+    cn, nums = pic_nums if pic_nums is not None else (1, [0, 2] if len(refs) == 2 else [0])
+    cp = _pic(cn, cur, u, v)
     rp = (oracle_picture * 2)(*[_pic(nums[i], refs[i]) for i in range(len(refs))])
     cap = Capture(geom, len(refs)) if with_cap else None
     ms = out.struct()
@@ -142,7 +143,8 @@ def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=()):
     return rc, out, cap, geom
 
 
-def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, lambda_override=None):
+def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, lambda_override=None,
+               pic_nums=None):
     terms = C.c_double(0)
     lo = None if lambda_override is None else np.ascontiguousarray(lambda_override, np.float32)
     o = oracle()
@@ -150,14 +152,14 @@ def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False,
     def fn(pp, cp, rp, n, stop_, ms, cap):
         return o.oracle_me(pp, cp, rp, n, stop_, None if lo is None else lo.ctypes.data, ms, cap, C.byref(terms))
 
-    rc, out, cap, geom = _run(fn, params, cur, refs, u, v, stop, capture)
+    rc, out, cap, geom = _run(fn, params, cur, refs, u, v, stop, capture, pic_nums=pic_nums)
     _chk(rc)
     out.sad_terms = terms.value
     return out, cap, geom
 
 
-def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, variant="mmx"):
-    rc, out, cap, geom = _run(ref(variant).refh_me, params, cur, refs, u, v, stop, capture)
+def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, variant="mmx", pic_nums=None):
+    rc, out, cap, geom = _run(ref(variant).refh_me, params, cur, refs, u, v, stop, capture, pic_nums=pic_nums)
     if rc != 0:
         raise RuntimeError("reference harness error %d" % rc)
     return out, cap, geom

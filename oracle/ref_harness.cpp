@@ -222,10 +222,10 @@ extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const ora
                        int stop_after, dme_medata *out, oracle_capture *cap) {
     const int W = cur->width, H = cur->height;
     EncoderParams encp(VIDEO_FORMAT_CUSTOM, INTER_PICTURE, 2, false);
-    encp.SetFullSearch(false);
+    encp.SetFullSearch(p->full_search != 0);
     encp.SetCombinedME(false);
-    encp.SetXRangeME(32);
-    encp.SetYRangeME(32);
+    encp.SetXRangeME(p->full_search ? p->x_range_me : 32);
+    encp.SetYRangeME(p->full_search ? p->y_range_me : 32);
 
     EncQueue q;
     // references first, as intra reference pictures
@@ -264,7 +264,7 @@ extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const ora
     MEData &me = E.GetMEData();
 
     // PixelME (picture_compress.cpp:76-80)
-    if (p->depth < 0) {
+    if (p->depth < 0 || p->full_search) {
         PixelMatcher pm(encp);
         pm.DoSearch(q, cur->pic_num);
     } else {
