@@ -7,8 +7,8 @@ references rebuilt, pel search, lambda map, sub-pel refinement, mode
 decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
 one JSON line; the algorithmic byte and SAD-term counts behind the roofline
 figures are in _geometry_terms().  The line also carries the learned-proposal
-leg and the motion-compensation leg (residuals of the same batch from its
-estimates, mc_leg()).
+leg, the motion-compensation leg (residuals of the same batch from its
+estimates, mc_leg()) and the full-search leg (full_search_leg()).
 """
 import argparse
 import json
@@ -246,6 +246,66 @@ def mc_leg(ctx, jobs, frames, params, geom, B, args):
     return out
 
 
+def full_search_leg(ctx, jobs, frames, args):
+    """Full search (the encoder's FullSearch option, range 32: a 65 x 65
+    window per block and reference at temporal distance 1) over the first
+    16 pictures of the batch, the whole live sequence per step; the
+    reference's own full search on the host beside it."""
+    import dirac_me
+
+    n = min(16, len(jobs))
+    p = dirac_me.make_params(*frames[0]["block"], precision=2, lam=10.0, full_search=True, x_range=32, y_range=32)
+    fj = [(pn, refs, p) for pn, refs, _ in jobs[:n]]
+    for _ in range(max(1, args.warmup)):
+        ctx.invalidate(-1)
+        ctx.run_batch(fj)
+    ctx.synchronize()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.invalidate(-1)
+        ctx.run_batch(fj)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_profiling(True)
+    ctx.invalidate(-1)
+    ctx.run_batch(fj)
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    g = dirac_me.geometry(p, 1920, 1080)
+    xl, yl = g.luma_blen[2]
+    # SAD terms of k_pel_full: every block and reference, the 65 x 65 window
+    # plus the 5 x 5 neighbourhood of zero, block samples clipped to the picture
+    terms = 0
+    for yb in range(g.ynum_blocks):
+        for xb in range(g.xnum_blocks):
+            x0, y0 = xb * g.luma_bsep[2][0] - (xl - g.luma_bsep[2][0]) // 2, yb * g.luma_bsep[2][1] - (yl - g.luma_bsep[2][1]) // 2
+            w = min(x0 + xl, 1920) - max(x0, 0)
+            h = min(y0 + yl, 1080) - max(y0, 0)
+            if w > 0 and h > 0:
+                terms += w * h
+    terms *= (65 * 65 + 25) * 2 * n
+    ms = kt["k_pel_full"][0]
+    out = {"value": n / dt, "unit": "frames/s", "ms_per_step": dt * 1e3, "batch": n, "range": 32,
+           "k_pel_full_ms": ms, "sad_terms_per_s": terms / (ms * 1e-3),
+           "int_ops_frac": 3.0 * terms / (ms * 1e-3) / INT_LANE_OPS_PEAK,
+           "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        kind = "reference" if O.ref_available() else "port"
+        run = O.run_ref if kind == "reference" else O.run_oracle
+        d = frames[0]
+        t0 = time.time()
+        run(p, d["cur"], d["refs"], d["u"], d["v"], pic_nums=(1, [0, 2]))
+        cdt = time.time() - t0
+        out["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "frames/s", "cores": 1, "kind": kind,
+                               "sample": "1 1080p B picture (2 refs, range 32), %s, single thread, %.1f s" % (
+                                   "reference encoder ME with FullSearch (MMX build)" if kind == "reference"
+                                   else "oracle port", cdt)}
+    return out
+
+
 def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
     """The same batch through learned proposals (dme_run_batch_learned):
     frames/s, and the Lagrangian cost gap against the plain search (sum of
@@ -344,6 +404,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-mc", action="store_true")
+    ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -431,6 +492,8 @@ def main():
     }
     if not args.no_mc:
         out["motion_compensation"] = mc_leg(ctx, jobs, frames, params, geom, B, args)
+    if not args.no_full:
+        out["full_search"] = full_search_leg(ctx, jobs, frames, args)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, jobs, frames, geom, B, args, fps)
     if not args.no_cpu:

@@ -1069,6 +1069,50 @@ __device__ __forceinline__ int full_enum(int dx, int dy, int xr) {
     return 1 + 2 * xr + (abs(dy) - 1) * 2 * (2 * xr + 1) + 2 * (dx + xr) + (dy < 0 ? 1 : 0);
 }
 
+// Whole blocks of NW dwords x ROWS rows (the common case): a thread takes
+// groups of four horizontally adjacent candidates, which share each
+// reference row's NW + 2 dword loads (candidate k of a group is the row
+// realigned by k more bytes) and each block row's loads (one LDS broadcast).
+// Returns the smallest (SAD, index) key of the thread's candidates.
+template <int NW, int ROWS>
+__device__ __forceinline__ unsigned long long full_groups(const uint32_t *__restrict__ s_pic,
+                                                          const uint32_t *__restrict__ s_reg, int rp, int rx, int ry,
+                                                          int xr, int yr, int tid) {
+    constexpr int PP = ((NW + 3) / 4) * 4;  // block row pitch in LDS (chunks of 16 bytes)
+    const int WX = 2 * xr + 1, ngx = (WX + 3) >> 2, ng = ngx * (2 * yr + 1);
+    unsigned long long best = ~0ull;
+    for (int gidx = tid; gidx < ng; gidx += 256) {
+        const int gy = gidx / ngx, dy = gy - yr, dx0 = (gidx - gy * ngx) * 4 - xr;
+        const int bo = dx0 + rx;  // byte offset of the group's first candidate in its region row
+        const uint32_t *__restrict__ rb = s_reg + (dy + ry) * rp + (bo >> 2);
+        const unsigned s0 = bo & 3;
+        uint32_t acc[4] = {0, 0, 0, 0};
+#pragma unroll 4
+        for (int y = 0; y < ROWS; ++y) {
+            uint32_t r[NW + 2];
+#pragma unroll
+            for (int j = 0; j < NW + 2; ++j) r[j] = rb[y * rp + j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned t = s0 + k, o = t >> 2, sh = t & 3;
+#pragma unroll
+                for (int j = 0; j < NW; ++j) {
+                    const uint32_t lo = o ? r[j + 1] : r[j], hi = o ? r[j + 2] : r[j + 1];
+                    acc[k] = __builtin_amdgcn_sad_u8(s_pic[y * PP + j], __builtin_amdgcn_alignbyte(hi, lo, sh), acc[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int dx = dx0 + k;
+            if (dx > xr) break;
+            const unsigned long long key = ((unsigned long long)acc[k] << 32) | (unsigned)full_enum(dx, dy, xr);
+            best = key < best ? key : best;
+        }
+    }
+    return best;
+}
+
 // LDS of k_pel_full, in dwords: per-wave minima, the 5 x 5 SAD table around
 // zero, the block (rows of pp), the window (rows of rp, with margins of
 // rx, ry >= 2 so that the 5 x 5 neighbourhood of zero is staged too).
@@ -1118,11 +1162,17 @@ __global__ __launch_bounds__(256) void k_pel_full(const JobDesc *__restrict__ jo
     };
     const int WX = 2 * xr + 1, nc = WX * (2 * yr + 1);
     unsigned long long best = ~0ull;
-    for (int c = tid; c < nc; c += 256) {
-        const int dy = c / WX - yr, dx = c - (dy + yr) * WX - xr;
-        const unsigned long long key = ((unsigned long long)(unsigned)sad_at(dx, dy) << 32) | (unsigned)full_enum(dx, dy, xr);
-        best = key < best ? key : best;
-    }
+    if (b.xl == 16 && b.yl == 16)
+        best = full_groups<4, 16>(s_pic, s_reg, rp, rx, ry, xr, yr, tid);
+    else if (b.xl == 12 && b.yl == 12)
+        best = full_groups<3, 12>(s_pic, s_reg, rp, rx, ry, xr, yr, tid);
+    else
+        for (int c = tid; c < nc; c += 256) {
+            const int dy = c / WX - yr, dx = c - (dy + yr) * WX - xr;
+            const unsigned long long key =
+                ((unsigned long long)(unsigned)sad_at(dx, dy) << 32) | (unsigned)full_enum(dx, dy, xr);
+            best = key < best ? key : best;
+        }
     if (tid < 25) s_t25[tid] = sad_at(tid % 5 - 2, tid / 5 - 2);
     for (int m = 32; m >= 1; m >>= 1) {
         const unsigned long long o = __shfl_xor(best, m, WAVE);
@@ -1240,7 +1290,12 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
                 if (!b.empty()) {
                     const int gx = sl[3], gy = sl[4];
                     int key = -1;
-                    if (has_guide && abs(pred.x - gx) <= 1 && abs(pred.y - gy) <= 1)
+                    if (J.full && abs(pred.x) < J.fxr[r] && abs(pred.y) < J.fyr[r])
+                        // full search: every candidate of the prediction list lies in
+                        // the zero window, whose first minimum is already the best,
+                        // so none is strictly smaller
+                        key = INT_MAX;
+                    else if (has_guide && abs(pred.x - gx) <= 1 && abs(pred.y - gy) <= 1)
                         key = sl[17 + (pred.y - gy + 1) * 3 + (pred.x - gx + 1)];
                     else if (abs(pred.x) <= 1 && abs(pred.y) <= 1)
                         key = sl[8 + (pred.y + 1) * 3 + (pred.x + 1)];

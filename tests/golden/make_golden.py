@@ -2,7 +2,8 @@
 own ME code, compiled from its sources by oracle/Makefile (target `ref`).
 
 Run in a container that has the reference tree:  python tests/golden/make_golden.py
-(add --mc-only to rewrite only the motion-compensation fixture, mc.npz).
+(--mc-only rewrites only the motion-compensation fixture, mc.npz; --full-only
+only the full-search fixture, full_search.npz).
 
 Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
 seed by the deterministic generator in libdme.so), the parameters, and the
@@ -103,6 +104,29 @@ def main():
         json.dump(index, f, indent=1, sort_keys=True)
 
 
+# full search: (name, seed, nrefs, x range, y range, picture numbers (cur, refs), precision)
+FULL_CASES = [
+    ("full_cif_s70_r6x5", 70, 2, 6, 5, (7, [4, 9]), 2),   # tdiff 3 and 2
+    ("full_cif_s71_r32", 71, 1, 32, 32, (1, [0]), 2),      # the encoder default range
+]
+
+
+def main_full():
+    """Full-search fixtures (full_search.npz): the reference's PixelMatcher
+    with FullSearch set, then the rest of the live sequence; full MEData."""
+    out, index = {}, {}
+    for name, seed, nrefs, xr, yr, nums, prec in FULL_CASES:
+        d = synth.sized(seed, "cif", nrefs)
+        p = make_params(*d["block"], precision=prec, lam=10.0, full_search=True, x_range=xr, y_range=yr)
+        res, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], pic_nums=nums)
+        index[name] = dict(seed=seed, nrefs=nrefs, x_range=xr, y_range=yr, pic_nums=[nums[0], list(nums[1])],
+                           precision=prec, inputs_sha256=inputs_digest(d))
+        for k, a in res.arrays(nrefs).items():
+            out[name + "/" + k] = a
+        print("wrote", name)
+    np.savez_compressed(os.path.join(HERE, "full_search.npz"), meta=json.dumps(index), **out)
+
+
 def main_mc():
     """Motion-compensation fixtures (mc.npz): for each case in mc_cases.CASES,
     the MvData from the reference's own motion estimator (or scrambled), and
@@ -141,6 +165,11 @@ def main_mc():
 
 
 if __name__ == "__main__":
-    if "--mc-only" not in sys.argv:
+    if "--full-only" in sys.argv:
+        main_full()
+    elif "--mc-only" in sys.argv:
+        main_mc()
+    else:
         main()
-    main_mc()
+        main_mc()
+        main_full()

@@ -228,3 +228,56 @@ def test_sad_term_count_is_reported():
 def test_mv_dtype_layout():
     assert MV_DTYPE.itemsize == 8
     assert DME_FP_MMX == 0
+
+
+# ---- full search (PixelMatcher with FullSearch, pixel_match.cpp:91-155, 249-264, 365-370) ----
+
+def _full_fixture():
+    z = np.load(os.path.join(GOLD, "full_search.npz"))
+    return json.loads(str(z["meta"])), z
+
+
+@pytest.mark.parametrize("name", ["full_cif_s70_r6x5", "full_cif_s71_r32"])
+def test_oracle_full_search_matches_reference_fixture(name):
+    index, z = _full_fixture()
+    m = index[name]
+    d = synth.sized(m["seed"], "cif", m["nrefs"])
+    digest = _sha(np.concatenate([d["cur"].ravel()] + [r.ravel() for r in d["refs"]] + [d["u"].ravel(), d["v"].ravel()]))
+    assert digest == m["inputs_sha256"], "synthetic generator changed: regenerate full_search.npz"
+    p = make_params(*d["block"], precision=m["precision"], lam=10.0, full_search=True, x_range=m["x_range"],
+                    y_range=m["y_range"])
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], pic_nums=(m["pic_nums"][0], m["pic_nums"][1]))
+    for k, v in got.arrays(m["nrefs"]).items():
+        assert v.tobytes() == z[name + "/" + k].tobytes(), k
+
+
+@needs_ref
+@pytest.mark.parametrize("seed,nrefs,xr,yr,nums,prec,variant", [
+    (72, 2, 5, 4, (7, [4, 9]), 2, "mmx"),
+    (73, 2, 8, 8, (10, [3, 11]), 1, "mmx"),
+    (74, 1, 3, 7, (2, [1]), 0, "c"),
+    (75, 2, 0, 3, (5, [4, 6]), 2, "mmx"),
+])
+def test_oracle_full_search_matches_reference_live(seed, nrefs, xr, yr, nums, prec, variant):
+    d = synth.sized(seed, "cif", nrefs)
+    p = make_params(*d["block"], precision=prec, lam=10.0, fp=DME_FP_C if variant == "c" else DME_FP_MMX,
+                    full_search=True, x_range=xr, y_range=yr)
+    want, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], variant=variant, pic_nums=nums)
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], pic_nums=nums)
+    for k, v in want.arrays(nrefs).items():
+        assert v.tobytes() == got.arrays(nrefs)[k].tobytes(), k
+
+
+@pytest.mark.parametrize("dx,dy", [(21, -13), (-30, 9)])
+def test_full_search_recovers_large_translations(dx, dy):
+    """A picture that is its reference moved by (dx, dy) beyond the
+    hierarchical search's reach but inside the full-search window: every
+    block clear of the picture edges gets exactly that vector, SAD 0."""
+    ref = synth.texture(80, 352, 288, nrects=0)
+    cur = np.roll(ref, (-dy, -dx), axis=(0, 1))
+    p = make_params(12, 12, 8, 8, precision=2, lam=10.0, full_search=True, x_range=32, y_range=32)
+    me, _, g = O.run_oracle(p, cur, [ref], stop=O.STOP_PEL)
+    v, pc = me.vectors[0], me.pred_costs[0]
+    for y in range(4, g.ynum_blocks - 4):
+        for x in range(5, g.xnum_blocks - 5):
+            assert (v[y, x]["x"], v[y, x]["y"]) == (dx, dy) and pc[y, x]["SAD"] == 0, (x, y)
