@@ -524,9 +524,10 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
 
 // Builds what a batch's pictures need, then fills the descriptors' plane
 // pointers.
-static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, int &launches) {
-    std::map<Picture *, int> levels;
-    std::vector<Picture *> ups;
+// The pictures a set of jobs reads: pyramid depth needed per picture, and the
+// references (whose quarter-pel planes the sub-pel and mode stages read).
+static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool need_up,
+                          std::map<Picture *, int> &levels, std::vector<Picture *> &ups) {
     for (auto *j : js) {
         Picture *P = find_pic(ctx, j->pic_num);
         if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
@@ -538,6 +539,33 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
             if (need_up && std::find(ups.begin(), ups.end(), R) == ups.end()) ups.push_back(R);
         }
     }
+    return DME_OK;
+}
+
+// Builds, on the current lane, the planes of the pictures that jobs of both
+// halves read (all they need of them), so that neither lane builds a plane
+// the other reads.
+static int build_shared_planes(dme_ctx *ctx, const std::vector<JobState *> &a, const std::vector<JobState *> &b,
+                               int &launches) {
+    std::map<Picture *, int> la, lb, shared;
+    std::vector<Picture *> ua, ub, sup;
+    DME_TRY(collect_planes(ctx, a, true, la, ua));
+    DME_TRY(collect_planes(ctx, b, true, lb, ub));
+    for (auto &kv : la) {
+        auto it = lb.find(kv.first);
+        if (it != lb.end()) shared[kv.first] = std::max(kv.second, it->second);
+    }
+    if (shared.empty()) return DME_OK;
+    for (const auto *u : {&ua, &ub})
+        for (Picture *P : *u)
+            if (shared.count(P) && std::find(sup.begin(), sup.end(), P) == sup.end()) sup.push_back(P);
+    return build_planes(ctx, shared, sup, launches);
+}
+
+static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, int &launches) {
+    std::map<Picture *, int> levels;
+    std::vector<Picture *> ups;
+    DME_TRY(collect_planes(ctx, js, need_up, levels, ups));
     DME_TRY(build_planes(ctx, levels, ups, launches));
     for (auto *j : js) {
         Picture *P = find_pic(ctx, j->pic_num);
@@ -992,7 +1020,10 @@ static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *cons
         half[side[i]].push_back(js[i]);
         over[side[i]].push_back(lam_over ? lam_over[i] : nullptr);
     }
-    DME_TRY(bind_planes(ctx, js, true, launches));
+    // Planes of pictures both halves use are built on lane 0 before the fork;
+    // every other picture's planes are built by its own half's lane (in
+    // stage_pel), so one lane's plane building overlaps the other's search.
+    DME_TRY(build_shared_planes(ctx, half[0], half[1], launches));
     HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
     HIP_TRY(hipStreamWaitEvent(ctx->lane[1].stream, ctx->ev_fork, 0));
     for (int h = 0; h < 2; ++h) {
