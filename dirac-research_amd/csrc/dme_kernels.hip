@@ -494,23 +494,23 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
     for (int py = tid / QT_X; py < QT_Y; py += 256 / QT_X) {
         const int Y = Y0 + py;
         if (X >= QW - 1 || Y >= QH - 1) continue;  // positions X <= W, Y <= H
-        int u[3][3];
+        // biased samples b = u + 128; the rounded averages commute with the bias
+        uint32_t b[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) u[a][c] = s_u[2 * py + a][2 * px + c];
+            for (int c = 0; c < 3; ++c) b[a][c] = (uint32_t)(s_u[2 * py + a][2 * px + c] + 128);
+        uint32_t *o = out + (size_t)((Y + 1) * 4) * QW + X + 1;
 #pragma unroll
-        for (int fy = 0; fy < 4; ++fy) {
-            const int oy = fy >> 1, ry = fy & 1;
-            uint32_t d = 0;
-#pragma unroll
-            for (int fx = 0; fx < 4; ++fx) {
-                const int ox = fx >> 1, rx = fx & 1;
-                const int v = ((2 - rx) * (2 - ry) * u[oy][ox] + rx * (2 - ry) * u[oy][ox + 1] +
-                               (2 - rx) * ry * u[oy + 1][ox] + rx * ry * u[oy + 1][ox + 1] + 2) >> 2;
-                d |= (uint32_t)(v + 128) << (8 * fx);
-            }
-            out[(size_t)((Y + 1) * 4 + fy) * QW + X + 1] = d;
+        for (int r = 0; r < 2; ++r) {
+            // phase rows fy = 2r (one up row) and fy = 2r + 1 (rows r, r + 1)
+            const uint32_t a0 = b[r][0], a1 = b[r][1], a2 = b[r][2];
+            const uint32_t c0 = b[r + 1][0], c1 = b[r + 1][1], c2 = b[r + 1][2];
+            const uint32_t h01 = (a0 + a1 + 1) >> 1, h12 = (a1 + a2 + 1) >> 1;
+            o[(size_t)(2 * r) * QW] = a0 | (h01 << 8) | (a1 << 16) | (h12 << 24);
+            const uint32_t v0 = (a0 + c0 + 1) >> 1, v1 = (a1 + c1 + 1) >> 1;
+            const uint32_t q01 = (a0 + a1 + c0 + c1 + 2) >> 2, q12 = (a1 + a2 + c1 + c2 + 2) >> 2;
+            o[(size_t)(2 * r + 1) * QW] = v0 | (q01 << 8) | (v1 << 16) | (q12 << 24);
         }
     }
 }
