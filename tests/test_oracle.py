@@ -126,8 +126,8 @@ def test_oracle_c_semantics_match_plain_c_reference(lam):
 
 
 @needs_ref
-def test_reference_drivers_agree_at_reference_depth():
-    # the harness's explicit-depth driver must equal PixelMatcher::DoSearch
+def test_reference_search_paths_agree_at_reference_depth():
+    # the harness's explicit-depth search must equal PixelMatcher::DoSearch
     d = synth.config0(4)
     g = O.geometry(make_params(*d["block"]), 352, 288)
     a, _, _ = O.run_ref(make_params(*d["block"], depth=-1), d["cur"], d["refs"], d["u"], d["v"])
