@@ -481,11 +481,33 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
         for (int t = 0; t < 8; ++t) s8[t] = s_v[vr][sc - 3 + t];
         return up_tap8(s8);
     };
-    for (int i = tid; i < QT_UR * (QT_X + 1); i += 256) {
-        const int a = i / (QT_X + 1), j = i - a * (QT_X + 1);
-        const int cuy = clampi(2 * Y0 + a, 0, 2 * H - 1);
-        s_u[a][2 * j] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j, 0, 2 * W - 1));
-        if (2 * j + 1 < QT_UC) s_u[a][2 * j + 1] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j + 1, 0, 2 * W - 1));
+    const bool inner = 2 * X0 >= 0 && 2 * X0 + QT_UC <= 2 * W && 2 * Y0 >= 0 && 2 * Y0 + QT_UR <= 2 * H;
+    if (inner) {
+        // no up coordinate clamps: thread = (up row, segment of 8 column pairs); the
+        // segment's 15 source values are read once and the 8-tap filter slides along
+        constexpr int SEG = 8, NSEG = (QT_X + 1 + SEG - 1) / SEG;
+        for (int i = tid; i < QT_UR * NSEG; i += 256) {
+            const int a = i / NSEG, j0 = (i - a * NSEG) * SEG;
+            const int16_t *row = (a & 1) ? &s_v[(a >> 1) + 1][0] : &s_src[(a >> 1) + 4][0];  // source row Y0 + a / 2
+            // source column of pair j is X0 + j, stored at column j + 4 of the tile
+            int16_t w[SEG + 7];
+#pragma unroll
+            for (int k = 0; k < SEG + 7; ++k) w[k] = row[min(j0 + 1 + k, QT_SC - 1)];
+#pragma unroll
+            for (int k = 0; k < SEG; ++k) {
+                const int j = j0 + k;
+                if (j > QT_X) break;
+                s_u[a][2 * j] = w[k + 3];
+                if (2 * j + 1 < QT_UC) s_u[a][2 * j + 1] = (int16_t)up_tap8_lds(&w[k]);
+            }
+        }
+    } else {
+        for (int i = tid; i < QT_UR * (QT_X + 1); i += 256) {
+            const int a = i / (QT_X + 1), j = i - a * (QT_X + 1);
+            const int cuy = clampi(2 * Y0 + a, 0, 2 * H - 1);
+            s_u[a][2 * j] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j, 0, 2 * W - 1));
+            if (2 * j + 1 < QT_UC) s_u[a][2 * j + 1] = (int16_t)up_val(cuy, clampi(2 * X0 + 2 * j + 1, 0, 2 * W - 1));
+        }
     }
     __syncthreads();
     uint32_t *__restrict__ out = reinterpret_cast<uint32_t *>(p.out);
