@@ -190,6 +190,32 @@ def test_batch_of_mixed_pictures(ctx):
         _assert_same(w, o, nrefs, "batch")
 
 
+def test_batch_with_pictures_shared_between_jobs(ctx):
+    # a queue of six pictures where pictures are both estimated and referenced,
+    # and references are shared between jobs (so both lanes of the batch read
+    # some of the same planes); each job must equal its own oracle result
+    a, b = synth.sized(61, "cif", 2), synth.sized(62, "cif", 2)
+    frames = [a["refs"][0], a["cur"], a["refs"][1], b["refs"][0], b["cur"], b["refs"][1]]
+    chroma = [synth.chroma(f, 70 + k) for k, f in enumerate(frames)]
+    for n, (f, (u, v)) in enumerate(zip(frames, chroma)):
+        ctx.upload(300 + n, f, u, v)
+    p = make_params(*a["block"], precision=2, depth=a["depth"], lam=10.0)
+    plan = [(1, [0, 2]), (2, [0, 4]), (3, [2, 4]), (4, [3, 5]), (5, [4]), (0, [1, 5])]
+    jobs, outs, wants = [], [], []
+    for cur, refs in plan:
+        jobs.append((300 + cur, [300 + r for r in refs], p))
+        outs.append(dirac_me.MEData(dirac_me.geometry(p, 352, 288)))
+        u, v = chroma[cur]
+        wants.append(O.run_oracle(p, frames[cur], [frames[r] for r in refs], u, v,
+                                  pic_nums=(300 + cur, [300 + r for r in refs]))[0])
+    print("shared-picture batch", flush=True)
+    ctx.run_batch(jobs, outs)
+    for k, ((cur, refs), w, o) in enumerate(zip(plan, wants, outs)):
+        _assert_same(w, o, len(refs), "job %d" % k)
+    for n in range(6):
+        ctx.drop(300 + n)
+
+
 def test_errors_are_reported(ctx):
     p = make_params(12, 12, 8, 8, precision=2, depth=3)
     ctx.set_params(p)
