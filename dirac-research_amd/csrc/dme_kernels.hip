@@ -273,8 +273,9 @@ __device__ __forceinline__ int up_tap8(const int s[8]) {
 // A workgroup filters a 64 x 16 tile of source samples: the tile and its
 // apron (4 columns left, 4 right, 3 rows above, 4 below, clamped to the
 // picture) are staged in LDS once, the vertical filter of every staged column
-// goes to LDS, and each thread then writes the 2 x 2 quads of its column for
-// 4 rows, a dword (two horizontally adjacent up samples) per store.
+// goes to LDS (4 rows per thread, the filter sliding down the column), and
+// each thread then writes the 2 x 2 quads of 4 consecutive columns of one
+// row (the filter sliding along it), 16 bytes per up row.
 constexpr int UC_IW = kUpTileW + 8, UC_IH = kUpTileH + 7;  // columns x0 - 4 .. x0 + 67
 
 __device__ __forceinline__ int up_tap8_lds(const int16_t *r) {
@@ -299,35 +300,53 @@ __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__
     const int local = tile - p.offset;
     const int W = p.ow, H = p.oh;
     const int tx = (W + kUpTileW - 1) / kUpTileW;
-    const int x0 = (local % tx) * kUpTileW, y0 = (local / tx) * kUpTileH;
+    const int x0t = (local % tx) * kUpTileW, y0 = (local / tx) * kUpTileH;
     const int16_t *__restrict__ in = p.in;
     const int tid = threadIdx.x;
     for (int i = tid; i < UC_IH * UC_IW; i += 256) {
         const int r = i / UC_IW, c = i - r * UC_IW;
-        s_in[r][c] = in[(size_t)clampi(y0 - 3 + r, 0, H - 1) * W + clampi(x0 - 4 + c, 0, W - 1)];
+        s_in[r][c] = in[(size_t)clampi(y0 - 3 + r, 0, H - 1) * W + clampi(x0t - 4 + c, 0, W - 1)];
     }
     __syncthreads();
-    for (int i = tid; i < kUpTileH * UC_IW; i += 256) {
-        const int r = i / UC_IW, c = i - r * UC_IW;
-        int16_t col[8];
+    // vertical taps: a thread makes 4 consecutive rows of one staged column
+    for (int i = tid; i < (kUpTileH / 4) * UC_IW; i += 256) {
+        const int g = i / UC_IW, c = i - g * UC_IW, r0 = 4 * g;
+        int16_t col[11];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) col[t] = s_in[r + t][c];
-        s_vt[r][c] = (int16_t)up_tap8_lds(col);
+        for (int t = 0; t < 11; ++t) col[t] = s_in[r0 + t][c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_vt[r0 + k][c] = (int16_t)up_tap8_lds(&col[k]);
     }
     __syncthreads();
-    const int lane = tid & 63, x = x0 + lane;
-    if (x >= W) return;
+    // a thread writes 4 consecutive columns of one source row: the row's 11
+    // staged values (and the vertical taps' 11) are read once
+    const int row = tid >> 4, x0 = x0t + 4 * (tid & 15);
+    const int y = y0 + row;
+    if (y >= H || x0 >= W) return;
     uint32_t *__restrict__ up = reinterpret_cast<uint32_t *>(p.out);  // two samples per dword, W per up row
+    int16_t wi[11], wv[11];
 #pragma unroll
-    for (int k = 0; k < kUpTileH / 4; ++k) {
-        const int r = wave_id() * (kUpTileH / 4) + k, y = y0 + r;
-        if (y >= H) break;
-        const int16_t *ri = &s_in[r + 3][lane + 1];  // columns x - 3 .. x + 4
-        const int16_t *rv = &s_vt[r][lane + 1];
-        const int e = ri[3], h = up_tap8_lds(ri), v = rv[3], d = up_tap8_lds(rv);
-        uint32_t *u0 = up + (size_t)(2 * y) * W, *u1 = up + (size_t)(2 * y + 1) * W;
-        u0[x] = pack16(e, h);
-        u1[x] = pack16(v, d);
+    for (int t = 0; t < 11; ++t) {
+        wi[t] = s_in[row + 3][x0 - x0t + 1 + t];  // columns x0 - 3 .. x0 + 7
+        wv[t] = s_vt[row][x0 - x0t + 1 + t];
+    }
+    uint32_t *u0 = up + (size_t)(2 * y) * W, *u1 = up + (size_t)(2 * y + 1) * W;
+    uint32_t a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = pack16(wi[k + 3], up_tap8_lds(&wi[k]));
+        b[k] = pack16(wv[k + 3], up_tap8_lds(&wv[k]));
+    }
+    if ((W & 3) == 0) {  // rows start 16-byte aligned: one 16-byte store per row
+        *reinterpret_cast<uint4 *>(u0 + x0) = make_uint4(a[0], a[1], a[2], a[3]);
+        *reinterpret_cast<uint4 *>(u1 + x0) = make_uint4(b[0], b[1], b[2], b[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (x0 + k >= W) break;
+            u0[x0 + k] = a[k];
+            u1[x0 + k] = b[k];
+        }
     }
 }
 
