@@ -8,7 +8,8 @@ decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
 one JSON line; the algorithmic byte and SAD-term counts behind the roofline
 figures are in _geometry_terms().  The line also carries the learned-proposal
 leg, the motion-compensation leg (residuals of the same batch from its
-estimates, mc_leg()) and the full-search leg (full_search_leg()).
+estimates, mc_leg()), the full-search leg (full_search_leg()) and the .imt
+diagnostics-text leg (imt_leg()).
 """
 import argparse
 import json
@@ -246,6 +247,51 @@ def mc_leg(ctx, jobs, frames, params, geom, B, args):
     return out
 
 
+def imt_leg(ctx, jobs, frames, geom, B, args):
+    """The .imt diagnostics text of the batch's estimates (host code, one
+    thread): dme_imt_write_picture, then dme_imt_read_picture, over the B
+    1080p two-reference records of one step.  Also the reference's own
+    operator<<(ostream&, MEData&) on a bounded sample."""
+    import dirac_me
+    from dirac_me import imt
+
+    est = [dirac_me.MEData(geom) for _ in frames]
+    ctx.run_batch(jobs[:len(frames)], est)
+    t0 = time.perf_counter()
+    recs = [imt.write_picture(jobs[j][0], geom, jobs[j][1], est[j % len(frames)]) for j in range(B)]
+    tw = time.perf_counter() - t0
+    nbytes = sum(len(r) for r in recs)
+    t0 = time.perf_counter()
+    for r in recs:
+        imt.read_picture(r)
+    tr = time.perf_counter() - t0
+    out = {"write": {"value": B / tw, "unit": "records/s", "MB_per_s": nbytes / tw / 1e6},
+           "read": {"value": B / tr, "unit": "records/s", "MB_per_s": nbytes / tr / 1e6},
+           "bytes_per_record": nbytes / B, "records": B, "threads": 1}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        kind = "reference" if O.ref_available() else "port"
+        n = 0
+        t0 = time.time()
+        while True:
+            me = est[n % len(frames)]
+            if kind == "reference":
+                O.ref_imt_medata_text(me, 2)
+            else:
+                O.imt_record(jobs[n][0], jobs[n][1], geom.luma_bsep[2][1], geom.luma_bsep[2][0], me)
+            n += 1
+            if time.time() - t0 >= args.cpu_budget / 4 or n >= B:
+                break
+        cdt = time.time() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "records/s", "cores": 1, "kind": kind,
+                               "sample": "%d 1080p 2-ref records, %s, %.1f s" % (
+                                   n, "the reference's operator<< on MEData (compiled from its sources), "
+                                   "MEData filled from the arrays included" if kind == "reference"
+                                   else "oracle port (Python)", cdt)}
+    return out
+
+
 def full_search_leg(ctx, jobs, frames, args):
     """Full search (the encoder's FullSearch option, range 32: a 65 x 65
     window per block and reference at temporal distance 1) over the first
@@ -405,6 +451,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--no-full", action="store_true")
+    ap.add_argument("--no-imt", action="store_true")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -494,6 +541,8 @@ def main():
         out["motion_compensation"] = mc_leg(ctx, jobs, frames, params, geom, B, args)
     if not args.no_full:
         out["full_search"] = full_search_leg(ctx, jobs, frames, args)
+    if not args.no_imt:
+        out["imt"] = imt_leg(ctx, jobs, frames, geom, B, args)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, jobs, frames, geom, B, args, fps)
     if not args.no_cpu:

@@ -81,6 +81,18 @@ class dme_mc_job(C.Structure):
                 ("mv", C.POINTER(dme_medata)), ("planes", dme_mc_plane * 3)]
 
 
+class dme_imt_sequence(C.Structure):
+    _fields_ = [("chroma", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("source_sampling", C.c_int32),
+                ("topfieldfirst", C.c_int32), ("frame_rate_num", C.c_int32), ("frame_rate_den", C.c_int32),
+                ("pix_asr_num", C.c_int32), ("pix_asr_den", C.c_int32), ("picture_coding_mode", C.c_int32)]
+
+
+class dme_imt_picture(C.Structure):
+    _fields_ = [("pic_num", C.c_int32), ("intra", C.c_int32), ("num_refs", C.c_int32), ("refs", C.c_int32 * 2),
+                ("ybsep", C.c_int32), ("xbsep", C.c_int32), ("sb_ylen", C.c_int32), ("sb_xlen", C.c_int32),
+                ("mv_ylen", C.c_int32), ("mv_xlen", C.c_int32)]
+
+
 DME_NUM_KERNELS = 24
 
 
@@ -109,6 +121,12 @@ SIGNATURES = {
     "dme_version": ([], _I),
     "dme_geometry_for": ([C.POINTER(dme_params), _I, _I, C.POINTER(dme_geometry)], _I),
     "dme_lambda_map_host": ([_I, _I, _I, _P, _P, C.c_float, _P], _I),
+    "dme_imt_write_sequence": ([C.POINTER(dme_imt_sequence), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)], _I),
+    "dme_imt_write_picture": ([C.POINTER(dme_imt_picture), C.POINTER(dme_medata), C.c_char_p, C.c_size_t,
+                               C.POINTER(C.c_size_t)], _I),
+    "dme_imt_read_sequence": ([C.c_char_p, C.c_size_t, C.POINTER(dme_imt_sequence), C.POINTER(C.c_size_t)], _I),
+    "dme_imt_read_picture": ([C.c_char_p, C.c_size_t, C.POINTER(dme_imt_picture), C.POINTER(dme_medata),
+                              C.POINTER(C.c_size_t)], _I),
     "dme_create": ([_I, C.POINTER(_P)], _I),
     "dme_destroy": ([_P], _I),
     "dme_set_params": ([_P, C.POINTER(dme_params)], _I),

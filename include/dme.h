@@ -146,6 +146,70 @@ int dme_lambda_map_host(int xnum_blocks, int ynum_blocks, int num_refs,
                         const dme_mv *vectors1, const dme_mv *vectors2,
                         float lambda, float *out_lambda_map);
 
+/* ---- .imt diagnostics text (host only) ---------------------------------- *
+ * The encoder's on-disk MEData text: a sequence header, then one record per
+ * coded picture.  Numbers are written as the reference's iostreams write them
+ * (ints in decimal, floats as printf "%.6g" of the value), one array row per
+ * line, each value followed by a space.  Writers return DME_OK and set *len
+ * to the record's byte count; with buf == NULL they only size it, and with
+ * cap < *len they write nothing and fail.  No terminating NUL is written.
+ * Readers parse one header or record at text[0..len), set *consumed to the
+ * offset after it and the whitespace that follows, and fail (naming the
+ * field) on malformed or out-of-range input.                              */
+
+typedef struct dme_imt_sequence {
+    int32_t chroma;               /* ChromaFormat value (format420 = 2)        */
+    int32_t width, height;
+    int32_t source_sampling;
+    int32_t topfieldfirst;        /* 0 or 1                                    */
+    int32_t frame_rate_num, frame_rate_den;
+    int32_t pix_asr_num, pix_asr_den;
+    int32_t picture_coding_mode;  /* 0 frames, 1 fields                        */
+} dme_imt_sequence;
+
+typedef struct dme_imt_picture {
+    int32_t pic_num;
+    int32_t intra;                /* 1: an ">intra" record, no MEData          */
+    int32_t num_refs;             /* 1 or 2 (0 for an intra record)            */
+    int32_t refs[2];
+    int32_t ybsep, xbsep;         /* luma block separation, dme_geometry
+                                     luma_bsep[2][1] and [2][0]                */
+    int32_t sb_ylen, sb_xlen;     /* ynum_sb, xnum_sb                          */
+    int32_t mv_ylen, mv_xlen;     /* ynum_blocks, xnum_blocks                  */
+} dme_imt_picture;
+
+/* Sequence header.  Replaces WriteDiagnosticsHeader (encoder/encmain.cpp:
+ * 165-193).                                                                */
+int dme_imt_write_sequence(const dme_imt_sequence *seq, char *buf, size_t cap,
+                           size_t *len);
+
+/* One picture record from MEData arrays laid out as for the estimator
+ * (lambda_map is not part of the format and is ignored; an intra record
+ * reads no arrays, me may be NULL).  Replaces DiracEncoder::
+ * GetInstrumentationData (libdirac_encoder/dirac_encoder.cpp:338-410) with
+ * WriteDiagnosticsData (encoder/encmain.cpp:195-330); the text after the
+ * record's first line is what operator<<(ostream&, MEData&) writes
+ * (libdirac_common/motion.cpp:452-485).                                    */
+int dme_imt_write_picture(const dme_imt_picture *pic, const dme_medata *me,
+                          char *buf, size_t cap, size_t *len);
+
+/* Sequence header.  Replaces ReadInstrumentationHeader
+ * (util/instrumentation/instrmain.cpp:84-118).                             */
+int dme_imt_read_sequence(const char *text, size_t len, dme_imt_sequence *seq,
+                          size_t *consumed);
+
+/* One picture record.  Replaces ProcessSequence::AddPictureEntry
+ * (util/instrumentation/process_sequence.cpp:119-210) with
+ * operator>>(istream&, MEData&) (libdirac_common/motion.cpp:414-449).
+ * With me == NULL, pic is output only and the arrays are parsed and
+ * dropped: call so to learn the shapes.  With me != NULL, pic's four
+ * lengths on entry are the caller's array shapes and must equal the
+ * record's; NULL array pointers in me skip that array.  pred_costs and
+ * bipred_costs get SAD and mvcost; their total is not in the text and is
+ * left unchanged.                                                          */
+int dme_imt_read_picture(const char *text, size_t len, dme_imt_picture *pic,
+                         dme_medata *me, size_t *consumed);
+
 /* ---- device context ---------------------------------------------------- */
 
 /* Creates a context on HIP device `device`, with its own stream. */

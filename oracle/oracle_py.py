@@ -216,3 +216,108 @@ def motion_compensate(which, params, refs, me, nrefs, direction, planes):
     if rc != 0:
         raise RuntimeError("motion compensation error %d %s" % (rc, oracle().oracle_last_error().decode()))
     return planes
+
+
+# ---- .imt diagnostics text ----------------------------------------------------
+# A restatement of the writer, WriteDiagnosticsHeader / WriteDiagnosticsData
+# (encoder/encmain.cpp:165-330).  Its MEData part is what
+# operator<<(ostream&, MEData&) writes (libdirac_common/motion.cpp:452-485):
+# TwoDArray rows (arrays.h:564-578) with a space after every value,
+# MvCostData as "SAD mvcost" (motion.cpp:381-386), MotionVector as "x y"
+# (motion.h:161-167).  A float goes through libstdc++'s num_put, which prints
+# the value promoted to double with "%.*g" at the default precision 6.
+
+IMT_SEQUENCE_FIELDS = ("chroma", "width", "height", "source_sampling", "topfieldfirst", "frame_rate_num",
+                       "frame_rate_den", "pix_asr_num", "pix_asr_den", "picture_coding_mode")
+
+
+def _imt_rows(a, fmt):
+    return "".join("".join(fmt(v) + " " for v in row) + "\n" for row in a.tolist())
+
+
+def _imt_g(x):
+    return "%.6g" % x
+
+
+def _imt_cost(c):
+    return "%s %s" % (_imt_g(c[0]), _imt_g(c[1]))
+
+
+def imt_sequence(seq):
+    """encmain.cpp:165-193: each field on a line of its own."""
+    return "".join("%d\n" % seq[f] for f in IMT_SEQUENCE_FIELDS).encode()
+
+
+def imt_medata_text(me, nrefs):
+    """motion.cpp:452-485 over a dirac_me.MEData of nrefs references."""
+    s = "\n\n" + _imt_rows(me.sb_split, str)
+    s += "\n" + _imt_rows(me.sb_costs, _imt_g)
+    s += "\n" + _imt_rows(me.mode, str)
+    s += "\n" + _imt_rows(me.intra_costs, _imt_g) + "\n"
+    if nrefs > 1:
+        s += _imt_rows(me.bipred_costs, _imt_cost)
+    for c in range(3):
+        s += "\n" + _imt_rows(me.dc[c], str)
+    for r in range(nrefs):
+        s += "\n" + _imt_rows(me.vectors[r], lambda v: "%d %d" % (v[0], v[1]))
+        s += "\n" + _imt_rows(me.pred_costs[r], _imt_cost) + "\n"
+    return s.encode()
+
+
+def imt_record_head(pic_num, refs, ybsep, xbsep, sb_shape, mv_shape):
+    """encmain.cpp:206-224: a record's lines before its MEData text; refs
+    has one or two picture numbers."""
+    head = "\n[frame:%d]>mo_comp\n\n%d " % (pic_num, len(refs)) + "".join("%d " % r for r in refs)
+    head += "%d %d %d %d %d %d" % ((ybsep, xbsep) + tuple(sb_shape) + tuple(mv_shape))
+    return head.encode()
+
+
+def imt_record(pic_num, refs, ybsep, xbsep, me):
+    """A whole motion-compensated picture's record (encmain.cpp:206-330)."""
+    return imt_record_head(pic_num, refs, ybsep, xbsep, me.sb_split.shape, me.mode.shape) + \
+        imt_medata_text(me, len(refs))
+
+
+def imt_intra(pic_num):
+    """encmain.cpp:206-211."""
+    return ("\n[frame:%d]>intra\n" % pic_num).encode()
+
+
+def ref_imt_medata_text(me, nrefs):
+    """The reference's own operator<<(ostream&, MEData&) on the same arrays
+    (oracle/_ref harness)."""
+    l = ref()
+    fn = l.refh_imt_medata_text
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.dme_medata), C.c_char_p, C.c_size_t,
+                   C.POINTER(C.c_size_t)]
+    by, bx = me.mode.shape
+    sy, sx = me.sb_split.shape
+    ms = me.struct()
+    n = C.c_size_t()
+    assert fn(nrefs, bx, by, sx, sy, C.byref(ms), None, 0, C.byref(n)) == 0
+    buf = C.create_string_buffer(n.value)
+    assert fn(nrefs, bx, by, sx, sy, C.byref(ms), buf, n.value, C.byref(n)) == 0
+    return buf.raw[:n.value]
+
+
+def ref_imt_read(text):
+    """One record parsed by the reference's own reader (ProcessSequence's
+    AddPictureEntry with operator>> for OLBParams and MEData, oracle/_ref
+    harness): (header dict, MEData or None)."""
+    import types
+    l = ref()
+    fn = l.refh_imt_read
+    fn.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32 * 11), C.POINTER(abi.dme_medata)]
+    hdr = (C.c_int32 * 11)()
+    if fn(text, len(text), C.byref(hdr), None) != 0:
+        raise RuntimeError("the reference reader failed")
+    h = list(hdr)
+    info = dict(pic_num=h[0], intra=bool(h[1]), refs=h[3:3 + h[2]], ybsep=h[5], xbsep=h[6], sb_shape=(h[7], h[8]),
+                mv_shape=(h[9], h[10]))
+    if info["intra"]:
+        return info, None
+    me = MEData(types.SimpleNamespace(ynum_blocks=h[9], xnum_blocks=h[10], ynum_sb=h[7], xnum_sb=h[8]))
+    ms = me.struct()
+    if fn(text, len(text), C.byref(hdr), C.byref(ms)) != 0:
+        raise RuntimeError("the reference reader failed")
+    return info, me

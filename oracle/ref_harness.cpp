@@ -34,6 +34,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "cpu_ref.h"
@@ -395,5 +397,101 @@ extern "C" int refh_mc(const dme_params *p, int W, int H, const oracle_picture *
     MotionCompensator::CompensatePicture(pp, direction == DME_MC_ADD ? ADD : SUBTRACT, mv, &cur, refsptr);
     for (int c = 0; c < 3; ++c) dump(cur.Data(CompSort(c)), planes[c]);
     for (size_t i = 0; i < rp.size(); ++i) delete rp[i];
+    return 0;
+}
+
+// ---- .imt text -------------------------------------------------------------
+
+// operator<<(ostream&, MEData&) (motion.cpp:452-485) on a MEData of nrefs
+// references filled from the arrays: the text of a .imt record after its
+// first line.  *len receives the size; the text is copied when cap allows.
+extern "C" int refh_imt_medata_text(int nrefs, int xnb, int ynb, int xsb, int ysb, const dme_medata *in, char *buf,
+                                    size_t cap, size_t *len) {
+    PicturePredParams pp;
+    pp.SetXNumBlocks(xnb);
+    pp.SetYNumBlocks(ynb);
+    pp.SetXNumSB(xsb);
+    pp.SetYNumSB(ysb);
+    MEData me(pp, nrefs);
+    for (int j = 0; j < ynb; ++j)
+        for (int i = 0; i < xnb; ++i) {
+            const size_t k = (size_t)j * xnb + i;
+            for (int r = 1; r <= nrefs; ++r) {
+                me.Vectors(r)[j][i].x = in->vectors[r - 1][k].x;
+                me.Vectors(r)[j][i].y = in->vectors[r - 1][k].y;
+                MvCostData &c = me.PredCosts(r)[j][i];
+                c.SAD = in->pred_costs[r - 1][k].SAD;
+                c.mvcost = in->pred_costs[r - 1][k].mvcost;
+                c.total = in->pred_costs[r - 1][k].total;
+            }
+            me.Mode()[j][i] = PredMode(in->mode[k]);
+            me.IntraCosts()[j][i] = in->intra_costs[k];
+            if (nrefs > 1) {
+                me.BiPredCosts()[j][i].SAD = in->bipred_costs[k].SAD;
+                me.BiPredCosts()[j][i].mvcost = in->bipred_costs[k].mvcost;
+            }
+            for (int c = 0; c < 3; ++c) me.DC(CompSort(c))[j][i] = in->dc[c][k];
+        }
+    for (int j = 0; j < ysb; ++j)
+        for (int i = 0; i < xsb; ++i) {
+            me.SBSplit()[j][i] = in->sb_split[(size_t)j * xsb + i];
+            me.SBCosts()[j][i] = in->sb_costs[(size_t)j * xsb + i];
+        }
+    std::ostringstream os;
+    os << me;
+    const std::string s = os.str();
+    *len = s.size();
+    if (buf && cap >= s.size()) std::memcpy(buf, s.data(), s.size());
+    return 0;
+}
+
+// One .imt record parsed as ProcessSequence reads it: the picture number
+// after the next ':' (process_sequence.cpp:252-253), then AddPictureEntry
+// (:119-206) with the reference's operator>> for OLBParams (common.cpp:
+// 186-201) and MEData (motion.cpp:414-449).  hdr receives pic_num, intra,
+// num_refs, refs[2], ybsep, xbsep, sb_ylen, sb_xlen, mv_ylen, mv_xlen; with
+// out != NULL the arrays are dumped there (pred-cost totals included).
+// Returns 0, or -1 when the stream failed.
+extern "C" int refh_imt_read(const char *text, size_t len, int32_t hdr[11], dme_medata *out) {
+    std::istringstream in(std::string(text, len));
+    int fnum = -1;
+    in.ignore(100000, ':');
+    in >> fnum;
+    in.ignore(10, '>');
+    char sort[10];
+    in >> sort;
+    for (int i = 0; i < 11; ++i) hdr[i] = 0;
+    hdr[0] = fnum;
+    if (std::strcmp(sort, "intra") == 0) {
+        hdr[1] = 1;
+        return in.fail() ? -1 : 0;
+    }
+    int total_refs = 0, ref = -1;
+    in >> total_refs;
+    hdr[2] = total_refs;
+    for (int i = 0; i < total_refs && i < 2; ++i) {
+        in >> ref;
+        hdr[3 + i] = ref;
+    }
+    OLBParams bp;
+    in >> bp;
+    int mb_y = 0, mb_x = 0, mv_y = 0, mv_x = 0;
+    in >> mb_y >> mb_x >> mv_y >> mv_x;
+    hdr[5] = bp.Ybsep();
+    hdr[6] = bp.Xbsep();
+    hdr[7] = mb_y;
+    hdr[8] = mb_x;
+    hdr[9] = mv_y;
+    hdr[10] = mv_x;
+    if (in.fail() || total_refs < 1 || total_refs > 2) return -1;
+    PicturePredParams pp;
+    pp.SetXNumBlocks(mv_x);
+    pp.SetYNumBlocks(mv_y);
+    pp.SetXNumSB(mb_x);
+    pp.SetYNumSB(mb_y);
+    MEData me(pp, total_refs);
+    in >> me;
+    if (in.fail()) return -1;
+    dump_medata(me, total_refs, out);
     return 0;
 }

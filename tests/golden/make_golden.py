@@ -3,7 +3,8 @@ own ME code, compiled from its sources by oracle/Makefile (target `ref`).
 
 Run in a container that has the reference tree:  python tests/golden/make_golden.py
 (--mc-only rewrites only the motion-compensation fixture, mc.npz; --full-only
-only the full-search fixture, full_search.npz).
+only the full-search fixture, full_search.npz; --imt-only only the .imt
+diagnostics fixture, me_cif.imt and imt.json).
 
 Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
 seed by the deterministic generator in libdme.so), the parameters, and the
@@ -22,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "dirac-research_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
 
 import oracle_py as O  # noqa: E402
 from dirac_me import DME_FP_C, DME_FP_MMX, make_params, synth  # noqa: E402
@@ -164,12 +166,40 @@ def main_mc():
     np.savez_compressed(os.path.join(HERE, "mc.npz"), meta=json.dumps(index), **out)
 
 
+def main_imt():
+    """The .imt fixture (me_cif.imt, imt.json): the reference's estimates of
+    the imt_cases pictures written as a diagnostics file, the MEData text of
+    every record by the reference's own operator<<."""
+    import imt_cases as I
+
+    text = O.imt_sequence(I.SEQUENCE) + O.imt_intra(I.INTRA_PIC)
+    index = dict(records={})
+    for case in I.CIF_CASES + [I.HD_CASE]:
+        d, p, nums = I.inputs(case)
+        res, _, g = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], pic_nums=nums)
+        rec = O.imt_record_head(nums[0], nums[1], g.luma_bsep[2][1], g.luma_bsep[2][0], res.sb_split.shape,
+                                res.mode.shape) + O.ref_imt_medata_text(res, len(nums[1]))
+        index["records"][case[0]] = dict(inputs_sha256=I.inputs_digest(d), record_sha256=I.sha(rec),
+                                         record_bytes=len(rec))
+        if case in I.CIF_CASES:
+            text += rec
+        print("wrote", case[0], len(rec), "bytes")
+    index["file_sha256"] = I.sha(text)
+    with open(I.IMT_FILE, "wb") as f:
+        f.write(text)
+    with open(I.INDEX_FILE, "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if "--full-only" in sys.argv:
         main_full()
+    elif "--imt-only" in sys.argv:
+        main_imt()
     elif "--mc-only" in sys.argv:
         main_mc()
     else:
         main()
         main_mc()
         main_full()
+        main_imt()
