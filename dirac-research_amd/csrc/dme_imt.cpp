@@ -33,6 +33,66 @@ using dme::set_error;
 
 // ---- writer ------------------------------------------------------------------
 
+// "%.6g" of a float whose result has the fixed form (decimal exponent X of
+// the value rounded to six digits in [-4, 5]) and whose binary exponent keeps
+// the scaled value below 2^54, computed exactly in integers: v = m * 2^e, and
+// N = m * 10^k * 2^e rounded half to even (printf's rounding of the exact
+// binary value) for the k = 5 - X that puts N in [10^5, 10^6).  Returns the
+// length, or 0 when v is outside that domain (zero, subnormal, |v| >= 10^6 or
+// below about 10^-4, inf, nan), for the general conversion to handle.
+int g6_fixed(float v, char *out) {
+    uint32_t bits;
+    std::memcpy(&bits, &v, 4);
+    const int be = (int)((bits >> 23) & 0xff);
+    if (be == 0 || be == 255) return 0;
+    const uint64_t m = (bits & 0x7fffff) | 0x800000;
+    const int sh = 150 - be;  // v = m / 2^sh
+    if (sh < 1 || sh > 40) return 0;
+    static const uint64_t p10[11] = {1ull, 10ull, 100ull, 1000ull, 10000ull, 100000ull, 1000000ull, 10000000ull,
+                                     100000000ull, 1000000000ull, 10000000000ull};
+    auto scaled = [&](int k) {  // round_half_even(m * 10^k / 2^sh)
+        const uint64_t P = m * p10[k];
+        uint64_t q = P >> sh;
+        const uint64_t r = P & ((1ull << sh) - 1), half = 1ull << (sh - 1);
+        if (r > half || (r == half && (q & 1))) ++q;
+        return q;
+    };
+    // a first k from the binary exponent (log10(2) ~ 0.30103), then adjust
+    int k = 5 - (int)std::floor((23 - sh) * 0.30103);
+    uint64_t N = 0;
+    for (int it = 0; it < 4; ++it) {
+        if (k < 0 || k > 10) return 0;
+        N = scaled(k);
+        if (N < 100000ull)
+            ++k;
+        else if (N >= 1000000ull)
+            --k;
+        else
+            break;
+    }
+    if (N < 100000ull || N >= 1000000ull || k > 9) return 0;  // X < -4 or not settled
+    const int X = 5 - k;                                      // in [-4, 5]
+    char d[6];
+    for (int i = 5; i >= 0; --i, N /= 10) d[i] = (char)('0' + N % 10);
+    int nd = 6;
+    while (nd > 1 && nd > X + 1 && d[nd - 1] == '0') --nd;  // strip fraction zeros
+    int n = 0;
+    if (bits >> 31) out[n++] = '-';
+    if (X >= 0) {
+        for (int i = 0; i <= X; ++i) out[n++] = d[i];
+        if (nd > X + 1) {
+            out[n++] = '.';
+            for (int i = X + 1; i < nd; ++i) out[n++] = d[i];
+        }
+    } else {
+        out[n++] = '0';
+        out[n++] = '.';
+        for (int i = 0; i < -X - 1; ++i) out[n++] = '0';
+        for (int i = 0; i < nd; ++i) out[n++] = d[i];
+    }
+    return n;
+}
+
 struct Out {
     std::string s;
     void put(const char *p) { s += p; }
@@ -44,6 +104,11 @@ struct Out {
     }
     void f(float v) {
         char b[40];
+        const int n = g6_fixed(v, b);
+        if (n) {
+            s.append(b, n);
+            return;
+        }
         auto r = std::to_chars(b, b + sizeof b, (double)v, std::chars_format::general, 6);
         s.append(b, r.ptr);
     }

@@ -18,12 +18,25 @@ SEQUENCE_FIELDS = ("chroma", "width", "height", "source_sampling", "topfieldfirs
                    "frame_rate_den", "pix_asr_num", "pix_asr_den", "picture_coding_mode")
 
 
-def _emit(call):
+def _emit(call, bound=None):
+    """Runs a writer once into a buffer of `bound` bytes (or sizes it first)."""
     n = C.c_size_t()
-    check(call(None, 0, C.byref(n)))
-    buf = C.create_string_buffer(n.value)
-    check(call(buf, n.value, C.byref(n)))
+    if bound is None:
+        check(call(None, 0, C.byref(n)))
+        bound = n.value
+    buf = C.create_string_buffer(bound)
+    check(call(buf, bound, C.byref(n)))
     return buf.raw[:n.value]
+
+
+def record_bound(p):
+    """An upper bound on a record's size: at most 12 characters and a space per
+    number ("%.6g" of a float, an int32), a newline per row, 256 bytes for the
+    first lines and blank lines."""
+    S, B = p.sb_ylen * p.sb_xlen, p.mv_ylen * p.mv_xlen
+    values = 2 * S + B * (2 + (2 if p.num_refs > 1 else 0) + 3 + 4 * p.num_refs)
+    rows = 2 * p.sb_ylen + p.mv_ylen * (2 + (p.num_refs > 1) + 3 + 2 * p.num_refs)
+    return 13 * values + rows + 256
 
 
 def write_sequence(width, height, chroma=2, source_sampling=0, topfieldfirst=1, frame_rate=(24, 1), pix_asr=(1, 1),
@@ -54,7 +67,8 @@ def write_picture(pic_num, geom, refs, me):
     """One motion-compensated picture's record from a dirac_me.MEData."""
     p = picture_info(pic_num, geom, refs)
     m = me.struct()
-    return _emit(lambda b, cap, n: lib().dme_imt_write_picture(C.byref(p), C.byref(m), b, cap, n))
+    bound = record_bound(p) if 1 <= p.num_refs <= 2 else None
+    return _emit(lambda b, cap, n: lib().dme_imt_write_picture(C.byref(p), C.byref(m), b, cap, n), bound)
 
 
 def write_intra(pic_num):

@@ -153,6 +153,9 @@ int dme_lambda_map_host(int xnum_blocks, int ynum_blocks, int num_refs,
  * line, each value followed by a space.  Writers return DME_OK and set *len
  * to the record's byte count; with buf == NULL they only size it, and with
  * cap < *len they write nothing and fail.  No terminating NUL is written.
+ * A record's size is at most 13 bytes per number (a float or int32 takes at
+ * most 12 characters, plus its space), one per row, and 256 for its first
+ * lines and blank lines: a buffer of that bound needs no sizing call.
  * Readers parse one header or record at text[0..len), set *consumed to the
  * offset after it and the whitespace that follows, and fail (naming the
  * field) on malformed or out-of-range input.                              */

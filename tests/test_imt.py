@@ -277,6 +277,44 @@ def test_fixture_reads_and_is_rewritten_from_the_oracle_estimates():
     assert recs[0][1] is None and recs[0][0]["intra"]
 
 
+def test_float_formatting_dense_against_printf():
+    """Floats across the whole range (the writer's integer fast path covers
+    the fixed forms, the general conversion the rest) against Python's
+    correctly rounded "%.6g", which is C's."""
+    rng = np.random.default_rng(5)
+    bits = np.concatenate([rng.integers(0, 2 ** 31, 150000, dtype=np.uint64),          # any positive float
+                           rng.integers(0x38D1B717, 0x49742400, 150000, dtype=np.uint64)])  # 1e-4 .. 1e6
+    v = bits.astype(np.uint32).view(np.float32)
+    v = v[np.isfinite(v)]
+    v = np.concatenate([v, -v[:1000], np.float32([0.0, -0.0, 1e-4, 999999.5, 999999.4, 99999.95, 0.00012345675])])
+    n = len(v)
+    me = _me(1, n, 1, 1)
+    me.intra_costs[0] = v
+    text = imt.write_picture(0, _geom(1, n, 1, 1), [1], me).decode().split("\n")
+    assert text[9] == "0 " * n  # the mode row; a blank line, then the intra costs
+    got = text[11].split(" ")[:-1]
+    want = ["%.6g" % float(x) for x in v]
+    assert got == want
+
+
+def test_record_bound_holds_for_the_widest_values():
+    me = _me(3, 5, 1, 2)
+    for a in (me.sb_costs, me.intra_costs):
+        a[...] = np.float32(-1.23457e-05)
+    for a in [me.bipred_costs] + me.pred_costs:
+        a["SAD"] = a["mvcost"] = np.float32(-0.000123457)
+    me.mode[...] = me.sb_split[...] = -2 ** 31
+    for a in me.dc:
+        a[...] = -32768
+    for v in me.vectors:
+        v["x"] = v["y"] = -2 ** 31
+    for nrefs in (1, 2):
+        p = imt.picture_info(-2 ** 31, _geom(3, 5, 1, 2, ybsep=-2 ** 31, xbsep=-2 ** 31), [-2 ** 31] * nrefs)
+        n = C.c_size_t()
+        assert abi.lib().dme_imt_write_picture(C.byref(p), C.byref(me.struct()), None, 0, C.byref(n)) == 0
+        assert n.value <= imt.record_bound(p)
+
+
 # ---- errors --------------------------------------------------------------------
 
 def _read_raw(text, pic=None, me=None):
