@@ -68,7 +68,8 @@ hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntas
                                 hipStream_t st);
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st);
-hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_width,
+                           hipStream_t st);
 hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 
 #define HIP_TRY(expr)                                                                                   \
@@ -840,8 +841,8 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
-    std::vector<Task> l2, dc, sh, mt;
-    int off = 0, offd = 0, offs = 0, offm = 0;
+    std::vector<Task> l2, dc, dr, sh, mt;
+    int off = 0, offd = 0, offr = 0, offs = 0, offm = 0, max_w = 0;
     for (size_t i = 0; i < js.size(); ++i) {
         const int cnt = js[i]->g.xnb * js[i]->g.ynb;
         l2.push_back(Task{(int)i, 0, cnt, off});
@@ -853,14 +854,17 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
         for (int c = 0; c < 3; ++c) {
             dc.push_back(Task{(int)i, c, cnt, offd});
             offd += cnt;
+            dr.push_back(Task{(int)i, c, js[i]->g.ynb, offr});  // block rows, for k_block_dc
+            offr += js[i]->g.ynb;
         }
+        max_w = std::max(max_w, js[i]->g.W);
         if (js[i]->p.mv_precision == 0)
             for (int r = 0; r < js[i]->nrefs; ++r) {
                 sh.push_back(Task{(int)i, r, cnt, offs});
                 offs += cnt;
             }
     }
-    const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rsh = tt.add(sh), rmt = tt.add(mt);
+    const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rdr = tt.add(dr), rsh = tt.add(sh), rmt = tt.add(mt);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
@@ -879,7 +883,7 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
                          [&] { return launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, st); }));
     DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,
-                         [&] { return launch_block_dc(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
+                         [&] { return launch_block_dc(dj, dtask(ctx, rdr), rdr.n, rdr.total, max_w, st); }));
     DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
                          [&] { return launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
     if (rsh.n) {

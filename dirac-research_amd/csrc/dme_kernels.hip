@@ -53,7 +53,6 @@ __device__ __forceinline__ int xcd_block() {
 // a loop there costs registers and occupancy).
 #define PW_MODE_L2 2
 #define PW_MODE_TABLES 4
-#define PW_BLOCK_DC 4
 #define PW_LP_REFINE 2
 
 // Index of the calling wave in its workgroup, known to the compiler as
@@ -3127,25 +3126,46 @@ __global__ __launch_bounds__(256) void k_lp_pel_costs(const JobDesc *__restrict_
 // then the 3x3 [1 3 1; 3 0 3; 1 3 1]/16 smoothing; row 0 and the first and
 // last columns are copied and the last row is left as it was.
 // ---------------------------------------------------------------------------
+// SetDC's block means (me_mode_decn.cpp:540-550, GetBlockDC :523-538): one
+// workgroup per (picture, component, block row).  Every block of a row covers
+// the same picture rows, so the workgroup first sums each column over those
+// rows (coalesced across threads) into LDS, then each block adds its columns'
+// sums.  Integer sums are order-independent: the same value as the
+// reference's row-by-row loop.  Blocks clipped away entirely get 0, as do
+// chroma blocks of a picture registered without chroma.
 __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                   int ntasks, int total) {
-    const int g0 = (xcd_block() * 4 + wave_id()) * PW_BLOCK_DC, g1 = min(g0 + PW_BLOCK_DC, total);
-    for (int gi = g0; gi < g1; ++gi) {
-        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
-        const Task tk = tasks[find_task(tasks, ntasks, gi)];
-        const JobDesc &J = jobs[tk.job];
-        const int comp = tk.ref, k = gi - tk.offset;
-        const int xb = k % J.xnb, yb = k / J.xnb;
-        int dc = 0;
-        if (comp == 0) {
-            const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
-            if (!b.empty()) dc = wave_block_dc(J.pic[0], J.W, b);
-        } else {
-            const int16_t *p = comp == 1 ? J.cu : J.cv;
-            const Blk b = block_limits(J.cblen, J.cbsep, J.cboff, J.CW, J.CH, xb, yb);
-            if (p && !b.empty()) dc = wave_block_dc(p, J.CW, b);
+    extern __shared__ int s_col[];  // one int per column of the widest plane
+    const int row = blockIdx.x;
+    if (row >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, row)];
+    const JobDesc &J = jobs[tk.job];
+    const int comp = tk.ref, yb = row - tk.offset;
+    const int16_t *__restrict__ p = comp == 0 ? J.pic[0] : (comp == 1 ? J.cu : J.cv);
+    const int *blen = comp == 0 ? J.blen[2] : J.cblen;
+    const int *bsep = comp == 0 ? J.bsep[2] : J.cbsep;
+    const int *boff = comp == 0 ? J.boff[2] : J.cboff;
+    const int W = comp == 0 ? J.W : J.CW, H = comp == 0 ? J.H : J.CH;
+    const Blk rb = block_limits(blen, bsep, boff, W, H, 0, yb);  // the row range of every block in the row
+    const bool any = p && rb.yl > 0;
+    if (any) {
+        for (int c = threadIdx.x; c < W; c += 256) {
+            const int16_t *q = p + (size_t)rb.yp * W + c;
+            int sum = 0;
+            for (int r = 0; r < rb.yl; ++r) sum += q[(size_t)r * W];
+            s_col[c] = sum;
         }
-        if ((threadIdx.x & 63) == 0) J.dct[comp][k] = (int16_t)dc;
+    }
+    __syncthreads();
+    for (int xb = threadIdx.x; xb < J.xnb; xb += 256) {
+        const Blk b = block_limits(blen, bsep, boff, W, H, xb, yb);
+        int dc = 0;
+        if (any && !b.empty()) {
+            int sum = 0;
+            for (int c = 0; c < b.xl; ++c) sum += s_col[b.xp + c];
+            dc = (int16_t)(sum / (b.xl * b.yl));
+        }
+        J.dct[comp][yb * J.xnb + xb] = (int16_t)dc;
     }
 }
 
@@ -3627,9 +3647,11 @@ hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_width,
+                           hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_block_dc, dim3((total + 4 * PW_BLOCK_DC - 1) / (4 * PW_BLOCK_DC)), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    hipLaunchKernelGGL(k_block_dc, dim3(total), dim3(256), (size_t)max_width * sizeof(int), st, jobs, tasks, ntasks,
+                       total);
     return hipGetLastError();
 }
 
