@@ -1832,42 +1832,52 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, const st
     return DME_OK;
 }
 
-// The live sequence with learned proposals, in two lanes as run_jobs does.
+// The live sequence with learned proposals, in parts on the lanes as
+// run_jobs does.
 // Jobs whose device lambda maps were flagged run again (learned pel stage,
 // host maps).
 static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
     DME_TRY(lambda_flag_slots(ctx, js));
-    std::vector<JobState *> half[2];
-    if (js.size() < 2 || ctx->profiling) {
-        half[0] = js;
-    } else {
-        for (size_t i = 0; i < js.size(); ++i) half[i & 1].push_back(js[i]);
-    }
-    std::vector<LpGroup> groups[2];
+    const int np = (js.size() < 2 || ctx->profiling) ? 1 : (int)std::min<size_t>(kParts, js.size());
+    std::vector<std::vector<JobState *>> part(np);
+    for (size_t i = 0; i < js.size(); ++i) part[i % np].push_back(js[i]);
+    std::vector<std::vector<LpGroup>> groups(np);
     int P = 0;
-    for (int h = 0; h < 2; ++h) P = lp_layout(half[h], P, groups[h]);
+    for (int h = 0; h < np; ++h) P = lp_layout(part[h], P, groups[h]);
     DME_TRY(lp_reserve(ctx, P, js.size()));
-    if (half[1].empty()) {
+    if (np == 1) {
         DME_TRY(stage_pel_learned(ctx, js, groups[0], launches));
         DME_TRY(stage_lambda(ctx, js, nullptr, false, launches));
         DME_TRY(stage_subpel(ctx, js, launches));
         DME_TRY(stage_mode(ctx, js, launches));
     } else {
-        DME_TRY(bind_planes(ctx, js, true, launches));
+        // as run_jobs: shared planes before the fork, each part's own planes
+        // after the previous part's
+        DME_TRY(build_shared_planes(ctx, part, launches));
         HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
-        HIP_TRY(hipStreamWaitEvent(ctx->lane[1].stream, ctx->ev_fork, 0));
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 1; h < np; ++h) HIP_TRY(hipStreamWaitEvent(ctx->lane[h].stream, ctx->ev_fork, 0));
+        for (int h = 0; h < np; ++h) {
             ctx->L = &ctx->lane[h];
-            DME_TRY(stage_pel_learned(ctx, half[h], groups[h], launches));
-            DME_TRY(stage_lambda(ctx, half[h], nullptr, false, launches));
-            DME_TRY(stage_subpel(ctx, half[h], launches));
-            DME_TRY(stage_mode(ctx, half[h], launches));
+            if (kStagger) {
+                if (h > 0) HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_planes[h - 1], 0));
+                DME_TRY(bind_planes(ctx, part[h], true, launches));
+                HIP_TRY(hipEventRecord(ctx->ev_planes[h], ctx->L->stream));
+            }
+            DME_TRY(stage_pel_learned(ctx, part[h], groups[h], launches));
         }
-        HIP_TRY(hipEventRecord(ctx->ev_join[1], ctx->lane[1].stream));
+        for (int h = 0; h < np; ++h) {
+            ctx->L = &ctx->lane[h];
+            DME_TRY(stage_lambda(ctx, part[h], nullptr, false, launches));
+            DME_TRY(stage_subpel(ctx, part[h], launches));
+            DME_TRY(stage_mode(ctx, part[h], launches));
+        }
         ctx->L = &ctx->lane[0];
-        HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join[1], 0));
+        for (int h = 1; h < np; ++h) {
+            HIP_TRY(hipEventRecord(ctx->ev_join[h], ctx->lane[h].stream));
+            HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join[h], 0));
+        }
     }
     DME_TRY(end_timing(ctx, launches));
     std::vector<JobState *> redo;
