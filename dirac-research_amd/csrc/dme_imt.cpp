@@ -21,6 +21,7 @@
 #include <charconv>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <system_error>
 #include <vector>
@@ -289,7 +290,7 @@ bool dims_ok(const dme_imt_picture &p) {
 
 }  // namespace
 
-extern "C" int dme_imt_write_sequence(const dme_imt_sequence *seq, char *buf, size_t cap, size_t *len) {
+static int imt_write_sequence(const dme_imt_sequence *seq, char *buf, size_t cap, size_t *len) {
     if (!seq) return set_error(DME_ERR_INVALID, "imt: seq is NULL");
     Out o;
     const int32_t v[10] = {seq->chroma,         seq->width,          seq->height,         seq->source_sampling,
@@ -302,7 +303,7 @@ extern "C" int dme_imt_write_sequence(const dme_imt_sequence *seq, char *buf, si
     return finish(o.s, buf, cap, len);
 }
 
-extern "C" int dme_imt_write_picture(const dme_imt_picture *pic, const dme_medata *me, char *buf, size_t cap,
+static int imt_write_picture(const dme_imt_picture *pic, const dme_medata *me, char *buf, size_t cap,
                                      size_t *len) {
     if (!pic) return set_error(DME_ERR_INVALID, "imt: pic is NULL");
     Out o;
@@ -381,7 +382,7 @@ extern "C" int dme_imt_write_picture(const dme_imt_picture *pic, const dme_medat
     return finish(o.s, buf, cap, len);
 }
 
-extern "C" int dme_imt_read_sequence(const char *text, size_t len, dme_imt_sequence *seq, size_t *consumed) {
+static int imt_read_sequence(const char *text, size_t len, dme_imt_sequence *seq, size_t *consumed) {
     if (!text || !seq || !consumed) return set_error(DME_ERR_INVALID, "imt: NULL argument");
     In in{text, text + len, text, {}};
     dme_imt_sequence s;
@@ -400,7 +401,7 @@ extern "C" int dme_imt_read_sequence(const char *text, size_t len, dme_imt_seque
     return DME_OK;
 }
 
-extern "C" int dme_imt_read_picture(const char *text, size_t len, dme_imt_picture *pic, dme_medata *me,
+static int imt_read_picture(const char *text, size_t len, dme_imt_picture *pic, dme_medata *me,
                                     size_t *consumed) {
     if (!text || !pic || !consumed) return set_error(DME_ERR_INVALID, "imt: NULL argument");
     In in{text, text + len, text, {}};
@@ -436,6 +437,14 @@ extern "C" int dme_imt_read_picture(const char *text, size_t len, dme_imt_pictur
                                               std::to_string(p.sb_xlen) + " / " + std::to_string(p.mv_ylen) + "x" +
                                               std::to_string(p.mv_xlen) + ", the caller's differ");
     in.line();  // operator>>(MEData) first skips the rest of the line
+    {
+        // every number takes a character and a separator: refuse a record
+        // whose arrays cannot fit the text before allocating for them
+        const size_t S = (size_t)p.sb_ylen * p.sb_xlen, B = (size_t)p.mv_ylen * p.mv_xlen;
+        const size_t values = 2 * S + B * (2 + (p.num_refs > 1 ? 2 : 0) + 3 + 4 * (size_t)p.num_refs);
+        if (values && 2 * values - 1 > (size_t)(in.end - in.p))
+            return set_error(DME_ERR_INVALID, "imt: the text is shorter than the record's arrays");
+    }
 
     // Parse everything into scratch first so that a malformed record leaves
     // the caller's arrays untouched.
@@ -493,4 +502,34 @@ extern "C" int dme_imt_read_picture(const char *text, size_t len, dme_imt_pictur
     *pic = p;
     *consumed = (size_t)(in.p - text);
     return DME_OK;
+}
+
+// The C entry points: no C++ exception crosses them.
+template <class F>
+static int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(DME_ERR_NOMEM, "imt: out of host memory");
+    } catch (...) {
+        return set_error(DME_ERR_INVALID, "imt: unexpected error");
+    }
+}
+
+extern "C" int dme_imt_write_sequence(const dme_imt_sequence *seq, char *buf, size_t cap, size_t *len) {
+    return guarded([&] { return imt_write_sequence(seq, buf, cap, len); });
+}
+
+extern "C" int dme_imt_write_picture(const dme_imt_picture *pic, const dme_medata *me, char *buf, size_t cap,
+                                     size_t *len) {
+    return guarded([&] { return imt_write_picture(pic, me, buf, cap, len); });
+}
+
+extern "C" int dme_imt_read_sequence(const char *text, size_t len, dme_imt_sequence *seq, size_t *consumed) {
+    return guarded([&] { return imt_read_sequence(text, len, seq, consumed); });
+}
+
+extern "C" int dme_imt_read_picture(const char *text, size_t len, dme_imt_picture *pic, dme_medata *me,
+                                    size_t *consumed) {
+    return guarded([&] { return imt_read_picture(text, len, pic, me, consumed); });
 }

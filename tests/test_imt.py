@@ -329,6 +329,7 @@ def _read_raw(text, pic=None, me=None):
     b"\n[frame:1]>p_comp\n",                             # unknown sort
     b"\n[frame:x]>intra\n",                              # picture number
     b"\n[frame:1]>mo_comp\n\n3 0 1 2 8 8 1 1 1 1\n",     # three references
+    b"\n[frame:1]>mo_comp\n\n1 0 8 8 4000 4000 16000 16000\n\n0 1 \n",  # arrays longer than the text
 ])
 def test_reader_rejects_malformed_headers(bad):
     with pytest.raises(DmeError):
