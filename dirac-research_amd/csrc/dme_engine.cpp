@@ -1088,7 +1088,7 @@ using namespace dme;
         DME_TRY(set_device(ctx));                                                  \
     } while (0)
 
-extern "C" int dme_create(int device, dme_ctx **out) {
+extern "C" int dme_create(int device, dme_ctx **out) try {
     if (!out) return set_error(DME_ERR_INVALID, "NULL out");
     *out = nullptr;
     int n = 0;
@@ -1111,9 +1111,11 @@ extern "C" int dme_create(int device, dme_ctx **out) {
     std::memset(&c->params, 0, sizeof(c->params));
     *out = c.release();
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_destroy(dme_ctx *ctx) {
+extern "C" int dme_destroy(dme_ctx *ctx) try {
     if (!ctx) return DME_OK;
     (void)hipSetDevice(ctx->device);
     for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
@@ -1131,9 +1133,11 @@ extern "C" int dme_destroy(dme_ctx *ctx) {
         if (ln.stream) (void)hipStreamDestroy(ln.stream);
     delete ctx;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_set_params(dme_ctx *ctx, const dme_params *params) {
+extern "C" int dme_set_params(dme_ctx *ctx, const dme_params *params) try {
     if (!ctx || !params) return set_error(DME_ERR_INVALID, "NULL argument");
     // validate everything but the size-dependent depth on a large picture
     Geometry g;
@@ -1144,6 +1148,8 @@ extern "C" int dme_set_params(dme_ctx *ctx, const dme_params *params) {
     ctx->params = *params;
     ctx->params_set = true;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *y, const int16_t *u,
@@ -1187,26 +1193,32 @@ static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, co
 }
 
 extern "C" int dme_upload_picture(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *y,
-                                  const int16_t *u, const int16_t *v) {
+                                  const int16_t *u, const int16_t *v) try {
     CTX_CHECK(ctx);
     return register_picture(ctx, pic_num, width, height, y, u, v, false);
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_bind_picture_device(dme_ctx *ctx, int pic_num, int width, int height, const int16_t *d_y,
-                                       const int16_t *d_u, const int16_t *d_v) {
+                                       const int16_t *d_u, const int16_t *d_v) try {
     CTX_CHECK(ctx);
     return register_picture(ctx, pic_num, width, height, d_y, d_u, d_v, true);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_drop_picture(dme_ctx *ctx, int pic_num) {
+extern "C" int dme_drop_picture(dme_ctx *ctx, int pic_num) try {
     CTX_CHECK(ctx);
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->pics.erase(pic_num);
     ctx->jobs.erase(pic_num);
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+extern "C" int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) try {
     CTX_CHECK(ctx);
     if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
     JobState *J = job_for(ctx, pic_num);
@@ -1217,9 +1229,11 @@ extern "C" int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int 
     DME_TRY(stage_pel(ctx, js, launches));
     DME_TRY(end_timing(ctx, launches));
     return export_job(ctx, *J, out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda_map, dme_medata *out) {
+extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda_map, dme_medata *out) try {
     CTX_CHECK(ctx);
     auto it = ctx->jobs.find(pic_num);
     if (it == ctx->jobs.end() || it->second->stage < 1) return set_error(DME_ERR_STATE, "run dme_pixel_search first");
@@ -1231,9 +1245,11 @@ extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda
     int launches = 0;
     DME_TRY(stage_lambda(ctx, js, ov, true, launches));
     return export_job(ctx, *js[0], out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out) {
+extern "C" int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out) try {
     CTX_CHECK(ctx);
     auto it = ctx->jobs.find(pic_num);
     if (it == ctx->jobs.end() || it->second->stage < 2) return set_error(DME_ERR_STATE, "set the lambda map first");
@@ -1243,9 +1259,11 @@ extern "C" int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out) {
     DME_TRY(stage_subpel(ctx, js, launches));
     DME_TRY(end_timing(ctx, launches));
     return export_job(ctx, *js[0], out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_mode_decide(dme_ctx *ctx, int pic_num, dme_medata *out) {
+extern "C" int dme_mode_decide(dme_ctx *ctx, int pic_num, dme_medata *out) try {
     CTX_CHECK(ctx);
     auto it = ctx->jobs.find(pic_num);
     if (it == ctx->jobs.end() || it->second->stage < 3) return set_error(DME_ERR_STATE, "run dme_subpel first");
@@ -1255,9 +1273,11 @@ extern "C" int dme_mode_decide(dme_ctx *ctx, int pic_num, dme_medata *out) {
     DME_TRY(stage_mode(ctx, js, launches));
     DME_TRY(end_timing(ctx, launches));
     return export_job(ctx, *js[0], out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+extern "C" int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) try {
     CTX_CHECK(ctx);
     if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
     JobState *J = job_for(ctx, pic_num);
@@ -1265,9 +1285,11 @@ extern "C" int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, i
     std::vector<JobState *> js{J};
     DME_TRY(run_jobs(ctx, js, nullptr));
     return export_job(ctx, *J, out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) {
+extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) try {
     CTX_CHECK(ctx);
     if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
     std::vector<JobState *> js;
@@ -1282,6 +1304,8 @@ extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_m
     if (outs)
         for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 // ---- motion compensation ------------------------------------------------------
@@ -1590,16 +1614,18 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
 
 }  // namespace dme
 
-extern "C" int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
+extern "C" int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) try {
     CTX_CHECK(ctx);
     if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
     DME_TRY(begin_timing(ctx));
     return mc_run(ctx, jobs, njobs);
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_motion_compensate(dme_ctx *ctx, const dme_params *params, int width, int height,
                                      const dme_medata *mv, const int *refs, int nrefs, int direction,
-                                     const dme_mc_plane planes[3]) {
+                                     const dme_mc_plane planes[3]) try {
     CTX_CHECK(ctx);
     if (!params || !mv || !refs || !planes) return set_error(DME_ERR_INVALID, "NULL argument");
     if (nrefs < 1 || nrefs > 2) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
@@ -1636,9 +1662,11 @@ extern "C" int dme_motion_compensate(dme_ctx *ctx, const dme_params *params, int
                                ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int16_t *out, size_t out_elems) {
+extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int16_t *out, size_t out_elems) try {
     CTX_CHECK(ctx);
     Picture *P = find_pic(ctx, pic_num);
     if (!P || !out) return set_error(DME_ERR_INVALID, "unknown picture or NULL output");
@@ -1677,15 +1705,19 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     HIP_TRY(hipMemcpy(out, src, sizeof(int16_t) * n, hipMemcpyDeviceToHost));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_synchronize(dme_ctx *ctx) {
+extern "C" int dme_synchronize(dme_ctx *ctx) try {
     CTX_CHECK(ctx);
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
+extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) try {
     CTX_CHECK(ctx);
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     for (auto &kv : ctx->pics)
@@ -1697,22 +1729,28 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) {
             kv.second->b8_ready = false;
         }
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_set_profiling(dme_ctx *ctx, int enable) {
+extern "C" int dme_set_profiling(dme_ctx *ctx, int enable) try {
     if (!ctx) return set_error(DME_ERR_INVALID, "NULL context");
     ctx->profiling = enable != 0;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" const char *dme_kernel_name(int k) {
     return (k >= 0 && k < DME_NUM_KERNELS && kKernelNames[k]) ? kKernelNames[k] : "";
 }
 
-extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) {
+extern "C" int dme_get_stats(dme_ctx *ctx, dme_stats *out) try {
     if (!ctx || !out) return set_error(DME_ERR_INVALID, "NULL argument");
     *out = ctx->stats;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 // ---- learned proposal --------------------------------------------------------
@@ -1897,7 +1935,7 @@ static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
 
 }  // namespace dme
 
-extern "C" int dme_lp_set_weights(dme_ctx *ctx, const float *params, size_t count) {
+extern "C" int dme_lp_set_weights(dme_ctx *ctx, const float *params, size_t count) try {
     CTX_CHECK(ctx);
     if (!params || count != (size_t)DME_LP_PARAMS)
         return set_error(DME_ERR_INVALID, "the network has " + std::to_string(DME_LP_PARAMS) + " parameters");
@@ -1915,9 +1953,11 @@ extern "C" int dme_lp_set_weights(dme_ctx *ctx, const float *params, size_t coun
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->lp_ready = true;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) {
+extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, dme_medata *out) try {
     CTX_CHECK(ctx);
     if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
     JobState *J = job_for(ctx, pic_num);
@@ -1927,9 +1967,11 @@ extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int 
     std::vector<JobState *> js{J};
     DME_TRY(run_jobs_learned(ctx, js));
     return export_job(ctx, *J, out);
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) {
+extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) try {
     CTX_CHECK(ctx);
     if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
     std::vector<JobState *> js;
@@ -1946,9 +1988,11 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
     if (outs)
         for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks) {
+extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks) try {
     CTX_CHECK(ctx);
     long long b = ctx->lp_blocks, f = ctx->lp_fallbacks;
     if (ctx->lp_slots) {
@@ -1963,9 +2007,11 @@ extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fall
     if (blocks) *blocks = b;
     if (fallbacks) *fallbacks = f;
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_features, float *d_labels) {
+extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_features, float *d_labels) try {
     CTX_CHECK(ctx);
     if (!d_features || !d_labels) return set_error(DME_ERR_INVALID, "NULL output");
     auto it = ctx->jobs.find(pic_num);
@@ -1988,66 +2034,82 @@ extern "C" int dme_lp_training_pair(dme_ctx *ctx, int pic_num, uint16_t *d_featu
                                ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_prepare(dme_ctx *ctx, const float *d_w, uint16_t *d_frag_fwd, uint16_t *d_frag_bwd) {
+extern "C" int dme_lp_prepare(dme_ctx *ctx, const float *d_w, uint16_t *d_frag_fwd, uint16_t *d_frag_bwd) try {
     CTX_CHECK(ctx);
     if (!d_w || !d_frag_fwd || !d_frag_bwd) return set_error(DME_ERR_INVALID, "NULL argument");
     HIP_TRY(launch_lp_frags(d_w, d_frag_fwd, d_frag_bwd, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lp_conv(dme_ctx *ctx, const uint16_t *d_in, int n, int h, int w, const uint16_t *d_frag,
-                           const float *d_bias, int relu, uint16_t *d_out, float *d_out_f32) {
+                           const float *d_bias, int relu, uint16_t *d_out, float *d_out_f32) try {
     CTX_CHECK(ctx);
     if (!d_in || !d_frag || (!d_out && !d_out_f32) || n <= 0 || h <= 0 || w <= 0)
         return set_error(DME_ERR_INVALID, "bad convolution arguments");
     HIP_TRY(launch_lp_conv(d_in, n * h * w, h, w, d_frag, d_bias, relu, d_out, d_out_f32, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lp_wgrad(dme_ctx *ctx, const uint16_t *d_in, const uint16_t *d_g, int n, int h, int w, float *d_dw,
-                            float *d_db) {
+                            float *d_db) try {
     CTX_CHECK(ctx);
     if (!d_in || !d_g || !d_dw || !d_db || n <= 0 || h <= 0 || w <= 0)
         return set_error(DME_ERR_INVALID, "bad gradient arguments");
     HIP_TRY(launch_lp_wgrad(d_in, d_g, n * h * w, h, w, d_dw, d_db, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lp_relu_backward(dme_ctx *ctx, const uint16_t *d_act, const float *d_g, size_t count,
-                                    uint16_t *d_out) {
+                                    uint16_t *d_out) try {
     CTX_CHECK(ctx);
     if (!d_g || !d_out) return set_error(DME_ERR_INVALID, "NULL argument");
     HIP_TRY(launch_lp_relu_bwd(d_act, d_g, count, d_out, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lp_wta_loss(dme_ctx *ctx, const float *d_out, const float *d_labels, int npos, float alpha,
-                               float beta, float scale, float *d_dout, float *d_loss) {
+                               float beta, float scale, float *d_dout, float *d_loss) try {
     CTX_CHECK(ctx);
     if (!d_out || !d_labels || !d_dout || !d_loss || npos <= 0) return set_error(DME_ERR_INVALID, "bad loss arguments");
     HIP_TRY(launch_lp_wta(d_out, d_labels, npos, alpha, beta, scale, d_dout, d_loss, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lp_adam(dme_ctx *ctx, float *d_param, const float *d_grad, float *d_m, float *d_v, int count,
-                           float lr, float beta1, float beta2, float eps, int t) {
+                           float lr, float beta1, float beta2, float eps, int t) try {
     CTX_CHECK(ctx);
     if (!d_param || !d_grad || !d_m || !d_v || count <= 0 || t < 1) return set_error(DME_ERR_INVALID, "bad Adam arguments");
     const float bc1 = 1.0f - std::pow(beta1, (float)t), bc2 = 1.0f - std::pow(beta2, (float)t);
     HIP_TRY(launch_lp_adam(d_param, d_grad, d_m, d_v, count, lr, beta1, beta2, eps, bc1, bc2, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_zero(dme_ctx *ctx, void *d_ptr, size_t bytes) {
+extern "C" int dme_lp_zero(dme_ctx *ctx, void *d_ptr, size_t bytes) try {
     CTX_CHECK(ctx);
     if (!d_ptr || (bytes & 3)) return set_error(DME_ERR_INVALID, "zeroing needs a pointer and a multiple of 4 bytes");
     HIP_TRY(launch_lp_zero(d_ptr, bytes, ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_alloc(dme_ctx *ctx, size_t bytes, void **d_ptr) {
+extern "C" int dme_lp_alloc(dme_ctx *ctx, size_t bytes, void **d_ptr) try {
     CTX_CHECK(ctx);
     if (!d_ptr || bytes == 0) return set_error(DME_ERR_INVALID, "bad allocation arguments");
     *d_ptr = nullptr;
@@ -2057,20 +2119,26 @@ extern "C" int dme_lp_alloc(dme_ctx *ctx, size_t bytes, void **d_ptr) {
         return set_error(DME_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_free(dme_ctx *ctx, void *d_ptr) {
+extern "C" int dme_lp_free(dme_ctx *ctx, void *d_ptr) try {
     CTX_CHECK(ctx);
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     if (d_ptr) HIP_TRY(hipFree(d_ptr));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
-extern "C" int dme_lp_copy(dme_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) {
+extern "C" int dme_lp_copy(dme_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) try {
     CTX_CHECK(ctx);
     if (!dst || !src || kind < 1 || kind > 3) return set_error(DME_ERR_INVALID, "bad copy arguments");
     const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,28 @@ static thread_local std::string t_last_error;
 int set_error(int code, const std::string &msg) {
     t_last_error = msg;
     return code;
+}
+
+int exception_status() noexcept {
+    auto note = [](const char *a, const char *b) {
+        try {
+            t_last_error = a;
+            t_last_error += b;
+        } catch (...) {  // no memory for the message: keep the status
+        }
+    };
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        note("host allocation failed", "");
+        return DME_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        note("unexpected error: ", e.what());
+        return DME_ERR_INVALID;
+    } catch (...) {
+        note("unexpected error", "");
+        return DME_ERR_INVALID;
+    }
 }
 
 // PicturePredParams::SetBlockSizes (common.cpp:207-320) for 4:2:0, then
@@ -182,7 +206,7 @@ extern "C" const char *dme_last_error(void) { return dme::t_last_error.c_str(); 
 
 extern "C" int dme_version(void) { return 100; }
 
-extern "C" int dme_geometry_for(const dme_params *params, int width, int height, dme_geometry *out) {
+extern "C" int dme_geometry_for(const dme_params *params, int width, int height, dme_geometry *out) try {
     if (!params || !out) return dme::set_error(DME_ERR_INVALID, "NULL argument");
     dme::Geometry g;
     int rc = dme::make_geometry(*params, width, height, g);
@@ -212,13 +236,17 @@ extern "C" int dme_geometry_for(const dme_params *params, int width, int height,
     for (int l = 0; l < 3; ++l)
         for (int a = 0; a < 2; ++a) { out->chroma_blen[l][a] = cbl[l][a]; out->chroma_bsep[l][a] = cbs[l][a]; }
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_lambda_map_host(int xnum_blocks, int ynum_blocks, int num_refs, const dme_mv *vectors1,
-                                   const dme_mv *vectors2, float lambda, float *out_lambda_map) {
+                                   const dme_mv *vectors2, float lambda, float *out_lambda_map) try {
     if (xnum_blocks < 3 || ynum_blocks < 3 || !vectors1 || !out_lambda_map || num_refs < 1 || num_refs > 2 ||
         (num_refs == 2 && !vectors2))
         return dme::set_error(DME_ERR_INVALID, "bad lambda-map arguments");
     dme::lambda_map_l2(xnum_blocks, ynum_blocks, num_refs, vectors1, vectors2, lambda, out_lambda_map);
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }

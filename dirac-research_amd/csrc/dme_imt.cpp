@@ -509,10 +509,8 @@ template <class F>
 static int guarded(F f) {
     try {
         return f();
-    } catch (const std::bad_alloc &) {
-        return set_error(DME_ERR_NOMEM, "imt: out of host memory");
     } catch (...) {
-        return set_error(DME_ERR_INVALID, "imt: unexpected error");
+        return dme::exception_status();
     }
 }
 

@@ -13,6 +13,11 @@ namespace dme {
 // Records msg as the calling thread's last error and returns code.
 int set_error(int code, const std::string &msg);
 
+// Inside a catch handler: records the exception being handled as the last
+// error and returns its status (DME_ERR_NOMEM for an allocation failure), so
+// that no C++ exception leaves a C entry point.
+int exception_status() noexcept;
+
 // Geometry with everything the kernels need (host-computed).
 struct Geometry {
     int W, H, CW, CH;

@@ -43,7 +43,7 @@ inline int clamp255(double v) {
 
 }  // namespace
 
-extern "C" int dme_synth_texture(uint64_t seed, int width, int height, int nrects, int16_t *out_y) {
+extern "C" int dme_synth_texture(uint64_t seed, int width, int height, int nrects, int16_t *out_y) try {
     if (width <= 0 || height <= 0 || !out_y || nrects < 0) return dme::set_error(DME_ERR_INVALID, "bad synth_texture args");
     Rng rng(seed);
     const int W = width, H = height;
@@ -87,10 +87,12 @@ extern "C" int dme_synth_texture(uint64_t seed, int width, int height, int nrect
     }
     for (size_t i = 0; i < a.size(); ++i) out_y[i] = (int16_t)(clamp255(a[i]) - 128);
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_synth_warp(const int16_t *cur, int width, int height, const dme_synth_motion *m,
-                              int16_t *out_ref) {
+                              int16_t *out_ref) try {
     if (!cur || !out_ref || !m || width <= 0 || height <= 0 || (m->nobjects > 0 && !m->objects))
         return dme::set_error(DME_ERR_INVALID, "bad synth_warp args");
     const int W = width, H = height;
@@ -124,10 +126,12 @@ extern "C" int dme_synth_warp(const int16_t *cur, int width, int height, const d
             out_ref[(size_t)y * W + x] = (int16_t)(clamp255(v + n) - 128);
         }
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
 
 extern "C" int dme_synth_chroma(const int16_t *y, int width, int height, uint64_t seed, int16_t *out_u,
-                                int16_t *out_v) {
+                                int16_t *out_v) try {
     if (!y || !out_u || !out_v || width < 2 || height < 2) return dme::set_error(DME_ERR_INVALID, "bad synth_chroma args");
     Rng rng(seed);
     const int CW = width >> 1, CH = height >> 1;
@@ -141,4 +145,6 @@ extern "C" int dme_synth_chroma(const int16_t *y, int width, int height, uint64_
             out_v[(size_t)j * CW + i] = (int16_t)(clamp255(128.0 - 0.4 * (m - 128.0) + ov) - 128);
         }
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
 }
