@@ -212,6 +212,7 @@ struct dme_ctx {
     // totals of counters retired when the array grew
     dme::DevBuf lp_stats;
     size_t lp_slots = 0;
+    bool lp_dirty = false;  // a learned batch counted since the last fold
     long long lp_blocks = 0, lp_fallbacks = 0;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -410,7 +411,8 @@ static size_t full_lds_bytes(int xl, int yl, int xr, int yr) {
     return sizeof(int) * (8 + 26) + 4 * (pp * 64 + (size_t)(yl + 2 * std::max(yr, 2)) * full_pitch(xl, xr));
 }
 
-static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs) {
+static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs,
+                       bool learned = false) {
     // buffers below may be reallocated: nothing queued may still use them
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     Picture *P = find_pic(ctx, pic_num);
@@ -426,6 +428,10 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     }
     DME_TRY(make_geometry(p, P->W, P->H, J.g));
     const Geometry &g = J.g;
+    {
+        const std::string lim = launch_limits(g, p.full_search != 0, learned);
+        if (!lim.empty()) return set_error(DME_ERR_INVALID, "unsupported picture or block size: " + lim);
+    }
     int fxr[2] = {0, 0}, fyr[2] = {0, 0};
     if (p.full_search) {
         for (int r = 0; r < 2; ++r) {
@@ -801,6 +807,8 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
         }
     if (dev.empty()) return DME_OK;
     const TaskTable::Range rt = tt.add(t);
+    // this call's descriptors: the lane may hold another call's
+    DME_TRY(upload_jobs(ctx, js));
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     DME_TRY(timed_launch(ctx, KK_LAMBDA, launches, [&] {
@@ -1015,7 +1023,7 @@ static int redo_flagged(dme_ctx *ctx, std::vector<JobState *> &js, const float *
 // of one lane reads a plane another lane writes; every other buffer a part
 // uses is its own jobs' or its lane's.  With per-kernel profiling on, the
 // parts run one after the other on lane 0 so that each kernel is timed alone.
-static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
+static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
     int launches = 0;
     DME_TRY(begin_timing(ctx));
     DME_TRY(lambda_flag_slots(ctx, js));
@@ -1076,6 +1084,20 @@ static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *cons
     }
     DME_TRY(end_timing(ctx, launches));
     return redo_flagged(ctx, js, lam_over);
+}
+
+// Waits for every lane: after a call that failed part-way, work queued on
+// the other lanes may still read pictures and job buffers that a later call
+// frees or reuses.
+static void sync_lanes(dme_ctx *ctx) {
+    for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
+    ctx->L = &ctx->lane[0];
+}
+
+static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
+    const int rc = run_jobs_parts(ctx, js, lam_over);
+    if (rc != DME_OK) sync_lanes(ctx);
+    return rc;
 }
 
 }  // namespace dme
@@ -1808,6 +1830,23 @@ static int lp_reserve(dme_ctx *ctx, int P, size_t nslots) {
     return DME_OK;
 }
 
+// Adds the refinement counters of the last learned batch to the context's
+// totals and zeroes them, so that the slots of a batch hold its own counts
+// (streams idle: every call ends with them idle).
+static int lp_fold_stats(dme_ctx *ctx) {
+    if (!ctx->lp_slots || !ctx->lp_dirty) return DME_OK;
+    std::vector<unsigned long long> h(2 * ctx->lp_slots);
+    HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ctx->lp_slots; ++i) {
+        ctx->lp_blocks += (long long)h[2 * i];
+        ctx->lp_fallbacks += (long long)h[2 * i + 1];
+    }
+    HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), ctx->L->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    ctx->lp_dirty = false;
+    return DME_OK;
+}
+
 // Learned pel stage of a batch laid out by lp_layout: the plain search of
 // the levels down to 2, the network on the level-0 block grids, then the
 // exact refinement and the pel costs.  Queues work only.
@@ -1860,6 +1899,7 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, const st
         rbx = std::max(rbx, j->g.blen[2][0]);
         rby = std::max(rby, j->g.blen[2][1]);
     }
+    ctx->lp_dirty = true;
     DME_TRY(timed_launch(ctx, KK_LP_REFINE, launches, [&]() -> hipError_t {
         hipError_t e = launch_lp_refine(dj, dtask(ctx, refine), refine.n, refine.total, ctx->lp_out.as<float>(),
                                         ctx->lp_stats.as<unsigned long long>(), rbx, rby, ctx->L->stream);
@@ -1874,8 +1914,9 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, const st
 // run_jobs does.
 // Jobs whose device lambda maps were flagged run again (learned pel stage,
 // host maps).
-static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
+static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
     int launches = 0;
+    DME_TRY(lp_fold_stats(ctx));
     DME_TRY(begin_timing(ctx));
     DME_TRY(lambda_flag_slots(ctx, js));
     const int np = (js.size() < 2 || ctx->profiling) ? 1 : (int)std::min<size_t>(kParts, js.size());
@@ -1924,6 +1965,11 @@ static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
     if (!redo.empty()) {
         std::vector<LpGroup> g;
         lp_layout(redo, 0, g);
+        // the slots hold this batch's counts only (lp_fold_stats): the
+        // redone jobs count anew
+        for (auto *j : redo)
+            HIP_TRY(launch_lp_zero(ctx->lp_stats.as<unsigned long long>() + 2 * j->bpos, 2 * sizeof(unsigned long long),
+                                   ctx->L->stream));
         DME_TRY(stage_pel_learned(ctx, redo, g, launches));
         DME_TRY(stage_lambda_host(ctx, redo, nullptr));
         DME_TRY(stage_subpel(ctx, redo, launches));
@@ -1931,6 +1977,12 @@ static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     }
     return DME_OK;
+}
+
+static int run_jobs_learned(dme_ctx *ctx, std::vector<JobState *> &js) {
+    const int rc = run_jobs_learned_parts(ctx, js);
+    if (rc != DME_OK) sync_lanes(ctx);
+    return rc;
 }
 
 }  // namespace dme
@@ -1961,7 +2013,9 @@ extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int 
     CTX_CHECK(ctx);
     if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
     JobState *J = job_for(ctx, pic_num);
-    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    if (ctx->params.full_search)
+        return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
+    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs, true));
     if (J->p.full_search)
         return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
     std::vector<JobState *> js{J};
@@ -1979,7 +2033,9 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
         for (int k = 0; k < i; ++k)
             if (jobs[k].pic_num == jobs[i].pic_num) return set_error(DME_ERR_INVALID, "a picture appears twice in a batch");
         JobState *J = job_for(ctx, jobs[i].pic_num);
-        DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs));
+        if (jobs[i].params.full_search)
+            return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
+        DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs, true));
         if (J->p.full_search)
             return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
         js.push_back(J);

@@ -56,7 +56,10 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
         return set_error(DME_ERR_INVALID, "mv_precision must be 0 (pel), 1 (half) or 2 (quarter)");
     if (p.fp_semantics != DME_FP_MMX && p.fp_semantics != DME_FP_C)
         return set_error(DME_ERR_INVALID, "fp_semantics must be DME_FP_MMX or DME_FP_C");
-    for (int i = 0; i < 4; ++i)
+    if (p.combined_me)
+        return set_error(DME_ERR_INVALID, "combined-component ME (EncoderParams::CombinedME) is not supported: "
+                                          "estimate on luma (combined_me = 0)");
+    for (int i = 0; i < 3; ++i)
         if (p.reserved[i]) return set_error(DME_ERR_INVALID, "reserved params must be zero");
     if (p.full_search != 0 && p.full_search != 1) return set_error(DME_ERR_INVALID, "full_search must be 0 or 1");
     if (p.full_search && (p.x_range_me < 0 || p.y_range_me < 0 || p.x_range_me > 128 || p.y_range_me > 128))

@@ -28,8 +28,10 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <string>
 
 #include "dme_device.h"
+#include "dme_internal.h"
 
 namespace dme {
 
@@ -3500,27 +3502,35 @@ hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
-                           int max_blen_y, bool has_guide, hipStream_t st) {
-    if (total <= 0) return hipSuccess;
-    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
-    const bool u8 = level == 0;  // the pictures themselves: samples checked to 8-bit range at registration
-    const int rad = min(level + 1, 5), R = rad + 1, D = 2 * R + 1, n = (2 * rad + 1) * (2 * rad + 1);
+// Dynamic LDS of k_pel_bulk (four waves) for blocks up to bx x by at
+// `level`, with its layout.
+static size_t pel_bulk_lds(int level, int max_blen_x, int max_blen_y, bool has_guide, PelLds &ly) {
+    const bool u8 = level == 0;
+    const int rad = min(level + 1, 5), R = rad + 1, D = 2 * R + 1;
     const int ntab = has_guide ? 2 : 1;
     // dwords of a packed row: per = samples per dword; the SAD reads 4-dword
     // chunks of the block row and one dword more of the region row, from
     // dword offset (2R) / per
     const int per = u8 ? 4 : 2;
     const int nch = ((max_blen_x + per - 1) / per + 3) / 4;
-    PelLds ly;
     ly.pp = 4 * nch;
     ly.rp = max((max_blen_x + 2 * R + per - 1) / per, (2 * R) / per + 4 * nch + 1) | 1;
     ly.pic = ly.pp * max_blen_y;
     ly.reg = ly.rp * (max_blen_y + 2 * R);
     ly.keys = 2 * ly.reg;
     ly.tab = ntab * D * D;
-    const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pic + ly.keys + ly.tab);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    return sizeof(uint32_t) * 4 * (size_t)(ly.pic + ly.keys + ly.tab);
+}
+constexpr size_t kPelBulkLdsMax = 64 * 1024;
+
+hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
+                           int max_blen_y, bool has_guide, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    const bool u8 = level == 0;  // the pictures themselves: samples checked to 8-bit range at registration
+    PelLds ly;
+    const size_t lds = pel_bulk_lds(level, max_blen_x, max_blen_y, has_guide, ly);
+    if (lds > kPelBulkLdsMax) return hipErrorInvalidValue;
     if (u8)
         hipLaunchKernelGGL(k_pel_bulk<true>, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total,
                            level, ly);
@@ -3530,15 +3540,25 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     return hipGetLastError();
 }
 
+// LDS of k_pel_wave: a fixed part for max_rows block rows and a part per
+// queued miss (at least one must fit).
+static void pel_wave_lds(int level, int max_rows, int max_blen_x, int max_blen_y, int &stage_elems, size_t &fixed,
+                         size_t &per_miss) {
+    const int rad = min(level + 1, 5), n = (2 * rad + 1) * (2 * rad + 1);
+    stage_elems = (max_blen_x * max_blen_y + (max_blen_x + 2 * rad) * (max_blen_y + 2 * rad) + 1) & ~1;
+    fixed = sizeof(int) * (6 * (size_t)max_rows + 2 * (size_t)max_rows * PEL_REC);
+    per_miss = sizeof(PelMiss) + sizeof(int) * n + sizeof(int16_t) * stage_elems;
+}
+constexpr size_t kPelWaveLdsMax = 160 * 1024 - 1024;  // static LDS of the kernel stays below 1 KiB
+
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
                            int max_blen_y, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
     const int threads = 1024;
-    const int rad = min(level + 1, 5), n = (2 * rad + 1) * (2 * rad + 1);
-    const int stage_elems = (max_blen_x * max_blen_y + (max_blen_x + 2 * rad) * (max_blen_y + 2 * rad) + 1) & ~1;
-    const size_t fixed = sizeof(int) * (6 * (size_t)max_rows + 2 * (size_t)max_rows * PEL_REC);
-    const size_t per_miss = sizeof(PelMiss) + sizeof(int) * n + sizeof(int16_t) * stage_elems;
-    const size_t budget = 160 * 1024 - 1024;  // static LDS of the kernel stays below 1 KiB
+    int stage_elems = 0;
+    size_t fixed = 0, per_miss = 0;
+    pel_wave_lds(level, max_rows, max_blen_x, max_blen_y, stage_elems, fixed, per_miss);
+    const size_t budget = kPelWaveLdsMax;
     if (fixed + per_miss > budget) return hipErrorInvalidValue;
     const int miss_cap = (int)min((size_t)256, (budget - fixed) / per_miss);
     const size_t lds = fixed + per_miss * miss_cap;
@@ -3554,17 +3574,31 @@ hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, i
     return hipGetLastError();
 }
 
+constexpr size_t kTableLdsMax = 159 * 1024;  // table and sub-pel wavefront kernels (their attribute)
+
+// LDS of k_subpel_bulk (four waves) for blocks up to bx x by: a phase region
+// spans the block plus two positions per axis; the SAD reads 4-dword chunks
+// of a picture row and one dword more of a region row.
+static size_t subpel_bulk_lds(int max_blen_x, int max_blen_y, int &pp, int &prows, int &qrows, int &qd) {
+    const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
+    pp = 4 * nch;
+    prows = max_blen_y;
+    qrows = max_blen_y + 2;
+    qd = 4 * nch + 1;
+    return sizeof(uint32_t) * 4 * ((size_t)pp * prows + 16 * (size_t)qrows * qd);
+}
+
+static size_t subpel_wave_lds(int max_rows) {
+    return sizeof(int) * (8 + 2 * SUB_SLOT) * (size_t)max_rows + sizeof(SubJob) * SUB_QCAP;
+}
+
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
                               int max_blen_y, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
-    // a phase region spans the block plus two positions per axis; the SAD
-    // reads 4-dword chunks of a picture row and one dword more of a region row
-    const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
-    const int pp = 4 * nch, prows = max_blen_y;
-    const int qrows = max_blen_y + 2, qd = 4 * nch + 1;
-    const size_t lds = sizeof(uint32_t) * 4 * ((size_t)pp * prows + 16 * (size_t)qrows * qd);
-    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    int pp, prows, qrows, qd;
+    const size_t lds = subpel_bulk_lds(max_blen_x, max_blen_y, pp, prows, qrows, qd);
+    if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)k_subpel_bulk, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3593,8 +3627,8 @@ hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t 
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
     const int threads = 512;
-    const size_t lds = sizeof(int) * (8 + 2 * SUB_SLOT) * (size_t)max_rows + sizeof(SubJob) * SUB_QCAP;
-    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    const size_t lds = subpel_wave_lds(max_rows);
+    if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)k_subpel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3619,18 +3653,24 @@ hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, in
     return hipGetLastError();
 }
 
-hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
-                              int max_x1, int max_y1, hipStream_t st) {
-    if (total <= 0) return hipSuccess;
-    if (max_x0 < 1 || max_y0 < 1 || max_x1 < 1 || max_y1 < 1 || max_x0 > 256 || max_x1 > 256) return hipErrorInvalidValue;
-    MtLds ly;
+// LDS of k_mode_tables (four waves) for level-0 units up to x0 x y0 and
+// level-1 units up to x1 x y1.
+static size_t mode_tables_lds(int max_x0, int max_y0, int max_x1, int max_y1, MtLds &ly) {
     ly.pd = (std::max(max_x0, max_x1) + 3) / 4;
     ly.prows = std::max(max_y0, max_y1);
     ly.spd = (max_x1 + 3) / 4;
     ly.slot = ly.spd * max_y1;
     ly.wave = ly.pd * ly.prows + 8 * ly.slot;
-    const size_t lds = sizeof(uint32_t) * 4 * (size_t)ly.wave;
-    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    return sizeof(uint32_t) * 4 * (size_t)ly.wave;
+}
+
+hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
+                              int max_x1, int max_y1, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (max_x0 < 1 || max_y0 < 1 || max_x1 < 1 || max_y1 < 1 || max_x0 > 256 || max_x1 > 256) return hipErrorInvalidValue;
+    MtLds ly;
+    const size_t lds = mode_tables_lds(max_x0, max_y0, max_x1, max_y1, ly);
+    if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mode_tables, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3668,22 +3708,60 @@ hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks
     return hipGetLastError();
 }
 
-hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4,
-                            unsigned long long *stats, int max_blen_x, int max_blen_y, hipStream_t st) {
-    if (total <= 0) return hipSuccess;
-    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
-    // the SAD reads 4-dword chunks of a picture row and, from dword offset
-    // (2 * radius) / 4, one dword more of a region row
+// LDS of k_lp_refine (four waves): the SAD reads 4-dword chunks of a
+// picture row and, from dword offset (2 * radius) / 4, one dword more of a
+// region row.
+static size_t lp_refine_lds(int max_blen_x, int max_blen_y, LpLds &ly) {
     const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
-    LpLds ly;
     ly.pp = 4 * nch;
     ly.prows = max_blen_y;
     ly.rp = max((max_blen_x + 2 * LP_BOX + 3) / 4, (2 * LP_BOX) / 4 + 4 * nch + 1) | 1;
     ly.brows = max_blen_y + 2 * LP_BOX;
     ly.zp = max((max_blen_x + 4 + 3) / 4, 1 + 4 * nch + 1) | 1;
     ly.zrows = max_blen_y + 4;
-    const size_t lds = sizeof(uint32_t) * 4 * (size_t)(ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 100);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    return sizeof(uint32_t) * 4 * (size_t)(ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 100);
+}
+constexpr size_t kLpRefineLdsMax = 64 * 1024;
+
+// The limits of one job's launches, checked on the host before anything is
+// queued: "" if every kernel of the job's stages fits, else the limit.
+std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
+    const int bx = g.blen[2][0], by = g.blen[2][1];
+    if (bx > PEL_MAXB || by > PEL_MAXB)
+        return "block lengths above " + std::to_string(PEL_MAXB) + " after rationalising";
+    PelLds pl;
+    for (int l = full_search ? 0 : g.depth; l >= 0; --l) {
+        if (pel_bulk_lds(l, bx, by, l < g.depth, pl) > kPelBulkLdsMax)
+            return "pel tables of level " + std::to_string(l) + " exceed 64 KiB of LDS";
+        int se = 0;
+        size_t fixed = 0, per_miss = 0;
+        pel_wave_lds(l, g.lyb[l], bx, by, se, fixed, per_miss);
+        if (fixed + per_miss > kPelWaveLdsMax)
+            return "pel wavefront of level " + std::to_string(l) + ": " + std::to_string(g.lyb[l]) +
+                   " block rows exceed one workgroup's LDS";
+        if (learned) break;  // levels 1 and 0: the learned refinement
+    }
+    int pp, prows, qrows, qd;
+    if (subpel_bulk_lds(bx, by, pp, prows, qrows, qd) > kTableLdsMax) return "sub-pel tables exceed the LDS";
+    if (subpel_wave_lds(g.ynb) > kTableLdsMax)
+        return "sub-pel wavefront: " + std::to_string(g.ynb) + " block rows exceed one workgroup's LDS";
+    MtLds ml;
+    if (g.blen[0][0] > 256 || g.blen[1][0] > 256 ||
+        mode_tables_lds(g.blen[0][0], g.blen[0][1], g.blen[1][0], g.blen[1][1], ml) > kTableLdsMax)
+        return "mode-decision units exceed the LDS of the unit tables";
+    LpLds ll;
+    if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 64 KiB of LDS";
+    if ((size_t)g.W * sizeof(int) > 64 * 1024) return "picture width exceeds the block-DC row buffer";
+    return std::string();
+}
+
+hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, int total, const float *out4,
+                            unsigned long long *stats, int max_blen_x, int max_blen_y, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    LpLds ly;
+    const size_t lds = lp_refine_lds(max_blen_x, max_blen_y, ly);
+    if (lds > kLpRefineLdsMax) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_lp_refine, dim3((total + 4 * PW_LP_REFINE - 1) / (4 * PW_LP_REFINE)), dim3(256), lds, st, jobs, tasks, ntasks, total, out4, stats,
                        ly);
     return hipGetLastError();

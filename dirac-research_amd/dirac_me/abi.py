@@ -42,7 +42,8 @@ class dme_params(C.Structure):
         ("xblen", C.c_int32), ("yblen", C.c_int32), ("xbsep", C.c_int32), ("ybsep", C.c_int32),
         ("mv_precision", C.c_int32), ("depth", C.c_int32), ("lambda_", C.c_float),
         ("fp_semantics", C.c_int32), ("chroma_format", C.c_int32), ("full_search", C.c_int32),
-        ("x_range_me", C.c_int32), ("y_range_me", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("x_range_me", C.c_int32), ("y_range_me", C.c_int32), ("combined_me", C.c_int32),
+                ("reserved", C.c_int32 * 3),
     ]
 
 

@@ -86,7 +86,14 @@ typedef struct dme_params {
                               (pixel_match.cpp:365-370)                      */
     int32_t x_range_me;    /* XRangeME / YRangeME (common.h:1311-1314), used  */
     int32_t y_range_me;    /* only with full_search; the encoder default is 32 */
-    int32_t reserved[4];   /* must be zero                                   */
+    int32_t combined_me;   /* EncoderParams::CombinedME (common.h:1317): ME on
+                              the combined luma+chroma picture
+                              (EncPicture::DataForME/UpDataForME,
+                              enc_picture.cpp:114-128).  Not supported: 1
+                              gives DME_ERR_INVALID, so a caller with
+                              CombinedME() set cannot get luma-only results
+                              silently.  0 = ME on luma, the encoder default */
+    int32_t reserved[3];   /* must be zero                                   */
 } dme_params;
 
 /* Derived geometry for a picture of a given size (SetMotionParameters,

@@ -57,6 +57,19 @@ def test_geometry_rejects_bad_params():
         dirac_me.geometry(make_params(), 351, 288)
 
 
+def test_combined_component_me_is_refused():
+    # EncoderParams::CombinedME (common.h:1317) estimates on the combined
+    # picture (enc_picture.cpp:114-128); refused, not estimated on luma
+    p = make_params()
+    p.combined_me = 1
+    with pytest.raises(dirac_me.DmeError) as e:
+        dirac_me.geometry(p, 352, 288)
+    assert e.value.code == abi.DME_ERR_INVALID
+    assert "combined" in str(e.value)
+    p.combined_me = 0
+    dirac_me.geometry(p, 352, 288)
+
+
 @pytest.mark.parametrize("nrefs", [1, 2])
 @pytest.mark.parametrize("lam", [10.0, 10.7, 3.3])
 def test_host_lambda_map_matches_oracle(nrefs, lam):
