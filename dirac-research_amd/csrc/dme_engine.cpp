@@ -85,18 +85,34 @@ hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, 
         if (rc_ != DME_OK) return rc_; \
     } while (0)
 
-// Device buffer that only grows.
+// Device buffer that only grows.  A borrowed buffer points into memory it
+// does not own (the no-allocation scratch): it is never freed and never
+// grows, so a request beyond it fails instead of allocating.
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;
     DevBuf() {}
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
+    ~DevBuf() { release(); }
+    void release() {
+        if (p && !borrowed) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        borrowed = false;
+    }
+    void borrow(void *q, size_t bytes) {
+        release();
+        p = q;
+        n = bytes;
+        borrowed = true;
     }
     int ensure(size_t bytes) {
         if (bytes <= n) return DME_OK;
+        if (borrowed)
+            return set_error(DME_ERR_INVALID, "scratch layout: " + std::to_string(bytes) + " bytes requested of a " +
+                                                  std::to_string(n) + "-byte buffer");
         if (p) {
             (void)hipFree(p);
             p = nullptr;
@@ -143,12 +159,16 @@ struct JobState {
     JobDesc desc;
 };
 
-// Pinned host staging that grows on demand (freed with the context).
+// Pinned host staging that grows on demand (freed with the context).  A
+// fixed one (the no-allocation lane's, allocated when the context is
+// created) never grows.
 struct PinnedBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
     int ensure(size_t n) {
         if (n <= bytes) return DME_OK;
+        if (fixed) return set_error(DME_ERR_INVALID, "no-allocation staging of " + std::to_string(bytes) + " bytes is too small");
         if (p) (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
@@ -182,6 +202,13 @@ constexpr int kLanes = 4;
 constexpr int kParts = 4;
 constexpr bool kStagger = true;
 
+// Pinned host staging of the no-allocation lane: descriptors, task tables
+// and, for a picture whose device lambda threshold is undecided, its pel
+// vectors and host-made lambda maps (about 1.3 MB at 60,000 blocks).
+constexpr size_t kNoallocStagingBytes = (size_t)4 << 20;
+
+int noalloc_scratch(int device, void **out);  // dme_noalloc.hip
+
 }  // namespace dme
 
 struct dme_ctx {
@@ -204,6 +231,17 @@ struct dme_ctx {
     // reset when a call starts, with the streams idle.
     dme::Lane lane[dme::kLanes];
     dme::Lane *L = &lane[0];
+    dme::Lane *L0 = &lane[0];            // the lane a call starts on
+    dme::DevBuf *lflags = &d_lflags;     // the device lambda flags of the current call
+    // No-allocation calls (dme_motion_estimate_noalloc): their pictures, job
+    // and lane, whose device buffers are borrowed from this device's scratch
+    // array for the call, and the lane's pinned staging, allocated here.
+    void *scratch = nullptr;             // the device's scratch array (hipGetSymbolAddress)
+    dme::Picture na_pic[3];
+    int na_npics = 0;                    // > 0 while a no-allocation call runs
+    dme::JobState na_job;
+    dme::Lane na_lane;
+    dme::DevBuf na_lflags, na_flag;
     // learned proposal: weights (f32 master), BF16 fragments per layer,
     // activations of a batch, per-job counters
     bool lp_ready = false;
@@ -263,7 +301,7 @@ static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
 }
 
 static int set_device(dme_ctx *ctx) {
-    ctx->L = &ctx->lane[0];  // entry points start on lane 0
+    ctx->L = ctx->L0;  // entry points start on lane 0
     HIP_TRY(hipSetDevice(ctx->device));
     return DME_OK;
 }
@@ -397,6 +435,11 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
 }
 
 static Picture *find_pic(dme_ctx *ctx, int num) {
+    if (ctx->na_npics) {
+        for (int i = 0; i < ctx->na_npics; ++i)
+            if (ctx->na_pic[i].num == num) return &ctx->na_pic[i];
+        return nullptr;
+    }
     auto it = ctx->pics.find(num);
     return it == ctx->pics.end() ? nullptr : it->second.get();
 }
@@ -775,7 +818,7 @@ static int read_lambda_flags(dme_ctx *ctx, const std::vector<JobState *> &js, st
     int n = 0;
     for (auto *j : js) n = std::max(n, j->bpos + 1);
     std::vector<int> f(n, 0);
-    HIP_TRY(hipMemcpyAsync(f.data(), ctx->d_lflags.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->L->stream));
+    HIP_TRY(hipMemcpyAsync(f.data(), ctx->lflags->p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     for (auto *j : js)
         if (f[j->bpos]) flagged.push_back(j);
@@ -812,7 +855,7 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     DME_TRY(timed_launch(ctx, KK_LAMBDA, launches, [&] {
-        return launch_lambda(dj, dtask(ctx, rt), rt.n, ctx->d_lflags.as<int>(), ctx->L->stream);
+        return launch_lambda(dj, dtask(ctx, rt), rt.n, ctx->lflags->as<int>(), ctx->L->stream);
     }));
     for (auto *j : dev) j->stage = 2;
     if (!check_now) return DME_OK;
@@ -923,25 +966,26 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     return DME_OK;
 }
 
-static int export_job(dme_ctx *ctx, JobState &J, dme_medata *out) {
+// Copies a job's MEData to out's non-NULL arrays: host arrays (kind
+// hipMemcpyDeviceToHost), or arrays in either memory (hipMemcpyDefault).
+static int export_job(dme_ctx *ctx, JobState &J, dme_medata *out, hipMemcpyKind kind = hipMemcpyDeviceToHost) {
     if (!out) return DME_OK;
     const size_t B = (size_t)J.g.xnb * J.g.ynb, S = (size_t)J.g.xsb * J.g.ysb;
     hipStream_t st = ctx->L->stream;
     for (int r = 0; r < J.nrefs; ++r) {
-        if (out->vectors[r]) HIP_TRY(hipMemcpyAsync(out->vectors[r], J.vec[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost, st));
-        if (out->pred_costs[r])
-            HIP_TRY(hipMemcpyAsync(out->pred_costs[r], J.pc[r].p, sizeof(dme_cost) * B, hipMemcpyDeviceToHost, st));
+        if (out->vectors[r]) HIP_TRY(hipMemcpyAsync(out->vectors[r], J.vec[r].p, sizeof(dme_mv) * B, kind, st));
+        if (out->pred_costs[r]) HIP_TRY(hipMemcpyAsync(out->pred_costs[r], J.pc[r].p, sizeof(dme_cost) * B, kind, st));
     }
     if (J.stage >= 2 && out->lambda_map)
-        HIP_TRY(hipMemcpyAsync(out->lambda_map, J.lam.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(out->lambda_map, J.lam.p, sizeof(float) * B, kind, st));
     if (J.stage >= 4) {
-        if (out->mode) HIP_TRY(hipMemcpyAsync(out->mode, J.mode.p, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
-        if (out->intra_costs) HIP_TRY(hipMemcpyAsync(out->intra_costs, J.intra.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
-        if (out->bipred_costs) HIP_TRY(hipMemcpyAsync(out->bipred_costs, J.bi.p, sizeof(dme_cost) * B, hipMemcpyDeviceToHost, st));
+        if (out->mode) HIP_TRY(hipMemcpyAsync(out->mode, J.mode.p, sizeof(int32_t) * B, kind, st));
+        if (out->intra_costs) HIP_TRY(hipMemcpyAsync(out->intra_costs, J.intra.p, sizeof(float) * B, kind, st));
+        if (out->bipred_costs) HIP_TRY(hipMemcpyAsync(out->bipred_costs, J.bi.p, sizeof(dme_cost) * B, kind, st));
         for (int c = 0; c < 3; ++c)
-            if (out->dc[c]) HIP_TRY(hipMemcpyAsync(out->dc[c], J.dc[c].p, sizeof(int16_t) * B, hipMemcpyDeviceToHost, st));
-        if (out->sb_split) HIP_TRY(hipMemcpyAsync(out->sb_split, J.split.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-        if (out->sb_costs) HIP_TRY(hipMemcpyAsync(out->sb_costs, J.sbc.p, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+            if (out->dc[c]) HIP_TRY(hipMemcpyAsync(out->dc[c], J.dc[c].p, sizeof(int16_t) * B, kind, st));
+        if (out->sb_split) HIP_TRY(hipMemcpyAsync(out->sb_split, J.split.p, sizeof(int32_t) * S, kind, st));
+        if (out->sb_costs) HIP_TRY(hipMemcpyAsync(out->sb_costs, J.sbc.p, sizeof(float) * S, kind, st));
     }
     HIP_TRY(hipStreamSynchronize(st));
     return DME_OK;
@@ -956,11 +1000,12 @@ static JobState *job_for(dme_ctx *ctx, int pic_num) {
 // Start of a call that queues work: the streams are idle (every call ends
 // synchronised), so the staging arenas can be reused from their start.
 static int begin_timing(dme_ctx *ctx) {
-    ctx->L = &ctx->lane[0];
+    ctx->L = ctx->L0;
     for (auto &ln : ctx->lane) {
         HIP_TRY(hipStreamSynchronize(ln.stream));
         ln.arena_used = 0;
     }
+    ctx->L0->arena_used = 0;
     ctx->ev_used.clear();
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->L->stream));
     return DME_OK;
@@ -990,7 +1035,7 @@ static int end_timing(dme_ctx *ctx, int launches) {
 // Batch positions of the jobs and their device lambda flags (streams idle).
 static int lambda_flag_slots(dme_ctx *ctx, std::vector<JobState *> &js) {
     for (size_t i = 0; i < js.size(); ++i) js[i]->bpos = (int)i;
-    return ctx->d_lflags.ensure(sizeof(int) * js.size());
+    return ctx->lflags->ensure(sizeof(int) * js.size());
 }
 
 // After a batch: the jobs whose device lambda maps were flagged run again
@@ -1091,7 +1136,7 @@ static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float
 // frees or reuses.
 static void sync_lanes(dme_ctx *ctx) {
     for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
-    ctx->L = &ctx->lane[0];
+    ctx->L = ctx->L0;
 }
 
 static int run_jobs(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
@@ -1131,6 +1176,12 @@ extern "C" int dme_create(int device, dme_ctx **out) try {
     }
     std::memset(&c->stats, 0, sizeof(c->stats));
     std::memset(&c->params, 0, sizeof(c->params));
+    // the no-allocation entry point: this device's scratch array, and its
+    // lane (lane 0's stream) with pinned staging of a fixed size
+    DME_TRY(noalloc_scratch(device, &c->scratch));
+    c->na_lane.stream = c->lane[0].stream;
+    DME_TRY(c->na_lane.arena.ensure(kNoallocStagingBytes));
+    c->na_lane.arena.fixed = true;
     *out = c.release();
     return DME_OK;
 } catch (...) {
@@ -1326,6 +1377,155 @@ extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_m
     if (outs)
         for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
     return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+// ---- no-allocation entry point -----------------------------------------------
+
+namespace dme {
+
+// Bump allocation in the scratch array.  Past the end it only counts, so
+// that the error can state the size the picture needs.
+struct ScratchBump {
+    char *base;
+    size_t cap, used = 0;
+    bool ok = true;
+    void give(DevBuf &b, size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + bytes;
+        if (used > cap) {
+            ok = false;
+            return;
+        }
+        b.borrow(base + off, bytes);
+    }
+};
+
+// Lends every device buffer that one picture's estimation against nrefs
+// references can request (prepare_job, build_planes, the stage uploads) out
+// of the scratch array, each with the size that is requested of it.
+static void noalloc_layout(dme_ctx *ctx, ScratchBump &sb, const Geometry &g, int nrefs) {
+    const int W = g.W, H = g.H;
+    for (int i = 0; i <= nrefs; ++i) {  // the picture, then its references
+        Picture &P = ctx->na_pic[i];
+        for (int l = 1; l <= g.depth; ++l) sb.give(P.pyr[l], sizeof(int16_t) * (size_t)(W >> l) * (H >> l));
+        const int bp = (W + 2 * kB8Margin + 63) & ~63;
+        sb.give(P.b8, sizeof(uint32_t) * (size_t)(H + 2 * kB8Margin) * (bp / 4));
+        if (i > 0) sb.give(P.qp, sizeof(uint32_t) * (size_t)(W + 2) * 4 * (H + 2));
+    }
+    JobState &J = ctx->na_job;
+    const size_t B = (size_t)g.xnb * g.ynb, S = (size_t)g.xsb * g.ysb;
+    size_t maxt = 0;
+    for (int l = 0; l <= g.depth; ++l) maxt = std::max(maxt, (size_t)g.lxb[l] * g.lyb[l] * 28);
+    for (int r = 0; r < 2; ++r) {
+        sb.give(J.vec[r], sizeof(dme_mv) * B);
+        sb.give(J.pc[r], sizeof(dme_cost) * B);
+        for (int l = 1; l <= g.depth; ++l) {
+            const size_t n = (size_t)g.lxb[l] * g.lyb[l];
+            sb.give(J.lvec[r][l], sizeof(dme_mv) * n);
+            sb.give(J.lpc[r][l], sizeof(dme_cost) * n);
+        }
+        sb.give(J.ptab[r], sizeof(int) * maxt);
+        sb.give(J.qtab[r], sizeof(int) * 56 * B);
+    }
+    sb.give(J.m1tab, sizeof(int) * 24 * (B / 4));
+    sb.give(J.m0tab, sizeof(int) * 32 * (B / 16));
+    sb.give(J.mode, sizeof(int32_t) * B);
+    sb.give(J.intra, sizeof(float) * B);
+    sb.give(J.bi, sizeof(dme_cost) * B);
+    for (int c = 0; c < 3; ++c) {
+        sb.give(J.dc[c], sizeof(int16_t) * B);
+        sb.give(J.dct[c], sizeof(int16_t) * B);
+    }
+    sb.give(J.split, sizeof(int32_t) * S);
+    sb.give(J.sbc, sizeof(float) * S);
+    sb.give(J.lam, sizeof(float) * B);
+    sb.give(J.lam1, sizeof(float) * B / 4);
+    sb.give(J.lam0, sizeof(float) * B / 16);
+    sb.give(J.ucost, sizeof(float) * B);
+    sb.give(J.lwork, sizeof(int) * 2 * B);
+    // the lane's descriptor, task and plane-task buffers (one job: at most
+    // 22 tasks and 29 plane tasks per upload)
+    sb.give(ctx->na_lane.d_jobs, sizeof(JobDesc));
+    sb.give(ctx->na_lane.d_tasks, sizeof(Task) * 64);
+    sb.give(ctx->na_lane.d_planes, sizeof(PlaneTask) * 64);
+    sb.give(ctx->na_lflags, sizeof(int));
+    sb.give(ctx->na_flag, sizeof(int));
+}
+
+// Restores the context's own lane, flags and pictures when a no-allocation
+// call ends.
+struct NoallocScope {
+    dme_ctx *ctx;
+    explicit NoallocScope(dme_ctx *c) : ctx(c) {}
+    ~NoallocScope() {
+        ctx->na_npics = 0;
+        ctx->L0 = &ctx->lane[0];
+        ctx->L = &ctx->lane[0];
+        ctx->lflags = &ctx->d_lflags;
+    }
+};
+
+}  // namespace dme
+
+extern "C" int dme_motion_estimate_noalloc(dme_ctx *ctx, const dme_params *params, int width, int height,
+                                           int pic_num, const int16_t *d_y, const int16_t *d_u, const int16_t *d_v,
+                                           int nrefs, const int *ref_nums, const int16_t *const *d_ref_y,
+                                           dme_medata *out) try {
+    CTX_CHECK(ctx);
+    if (!params || !d_y || !d_u || !d_v || !ref_nums || !d_ref_y)
+        return set_error(DME_ERR_INVALID, "NULL argument (the picture needs luma and chroma planes)");
+    if (nrefs < 1 || nrefs > 2) return set_error(DME_ERR_INVALID, "nrefs must be 1 or 2");
+    for (int r = 0; r < nrefs; ++r)
+        if (!d_ref_y[r]) return set_error(DME_ERR_INVALID, "NULL reference plane");
+    if (!ctx->scratch) return set_error(DME_ERR_STATE, "no scratch array on this device");
+    Geometry g;
+    DME_TRY(make_geometry(*params, width, height, g));
+    {
+        const std::string lim = launch_limits(g, params->full_search != 0, false);
+        if (!lim.empty()) return set_error(DME_ERR_INVALID, "unsupported picture or block size: " + lim);
+    }
+    // every work buffer from the scratch array; nothing is allocated
+    ScratchBump sb{static_cast<char *>(ctx->scratch), (size_t)DME_NOALLOC_SCRATCH_BYTES};
+    HIP_TRY(hipStreamSynchronize(ctx->lane[0].stream));  // the previous call's work reads the same array
+    noalloc_layout(ctx, sb, g, nrefs);
+    if (!sb.ok)
+        return set_error(DME_ERR_INVALID, "the picture needs " + std::to_string(sb.used) + " bytes of scratch, more than " +
+                                              std::to_string((size_t)DME_NOALLOC_SCRATCH_BYTES));
+    for (int i = 0; i <= nrefs; ++i) {
+        Picture &P = ctx->na_pic[i];
+        P.num = i == 0 ? pic_num : ref_nums[i - 1];
+        P.W = width;
+        P.H = height;
+        P.y = i == 0 ? d_y : d_ref_y[i - 1];
+        P.u = i == 0 ? d_u : nullptr;
+        P.v = i == 0 ? d_v : nullptr;
+        P.pyr_levels = 0;
+        P.up_ready = P.qp_ready = P.mcp_ready = P.b8_ready = false;
+        P.bp = 0;
+    }
+    NoallocScope scope(ctx);
+    ctx->na_npics = 1 + nrefs;
+    ctx->L0 = ctx->L = &ctx->na_lane;
+    ctx->lflags = &ctx->na_lflags;
+    // luma must be 8-bit samples minus 128 (the flag cleared by a fill kernel)
+    {
+        hipStream_t st = ctx->L->stream;
+        int *flag = ctx->na_flag.as<int>();
+        HIP_TRY(launch_lp_zero(flag, sizeof(int), st));
+        for (int i = 0; i <= nrefs; ++i)
+            HIP_TRY(launch_check_range(ctx->na_pic[i].y, (size_t)width * height, flag, st));
+        int bad = 0;
+        HIP_TRY(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (bad) return set_error(DME_ERR_INVALID, "luma samples must lie in [-128, 127] (8-bit samples minus 128)");
+    }
+    JobState &J = ctx->na_job;
+    DME_TRY(prepare_job(ctx, J, *params, pic_num, ref_nums, nrefs));
+    std::vector<JobState *> js{&J};
+    DME_TRY(run_jobs(ctx, js, nullptr));
+    return export_job(ctx, J, out, hipMemcpyDefault);
 } catch (...) {
     return dme::exception_status();
 }

@@ -165,6 +165,22 @@ class Context:
             check(lib().dme_run_batch(self.h, arr, len(jobs), None))
         return outs
 
+    def motion_estimate_noalloc(self, params, width, height, pic_num, d_planes, ref_nums, d_refs, out=None):
+        """dme_motion_estimate_noalloc: d_planes = device addresses of the
+        picture's Y, U, V; d_refs = device addresses of the references' Y;
+        out = an MEData (host arrays) or a dme_medata struct (any memory)."""
+        nums = (C.c_int * 2)(*(list(ref_nums) + [0] * (2 - len(ref_nums))))
+        refs = (C.c_void_p * 2)(*(list(d_refs) + [None] * (2 - len(d_refs))))
+        if isinstance(out, MEData):
+            ms = out.struct()
+        else:
+            ms = out
+        check(lib().dme_motion_estimate_noalloc(self.h, C.byref(params), width, height, pic_num,
+                                                C.c_void_p(d_planes[0]), C.c_void_p(d_planes[1]),
+                                                C.c_void_p(d_planes[2]), len(ref_nums), nums, refs,
+                                                C.byref(ms) if ms is not None else None))
+        return out
+
     def motion_compensate(self, params, width, height, me, refs, direction, planes):
         """MotionCompensator::CompensatePicture on host arrays: planes are
         three C-contiguous int16 arrays (Y, U, V; possibly padded), changed

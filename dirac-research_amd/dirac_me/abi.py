@@ -140,6 +140,8 @@ SIGNATURES = {
     "dme_mode_decide": ([_P, _I, C.POINTER(dme_medata)], _I),
     "dme_motion_estimate": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_run_batch": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_motion_estimate_noalloc": ([_P, C.POINTER(dme_params), _I, _I, _I, _P, _P, _P, _I, _P, _P,
+                                     C.POINTER(dme_medata)], _I),
     "dme_motion_compensate": ([_P, C.POINTER(dme_params), _I, _I, C.POINTER(dme_medata), _P, _I, _I,
                                C.POINTER(dme_mc_plane)], _I),
     "dme_motion_compensate_batch": ([_P, C.POINTER(dme_mc_job), _I], _I),

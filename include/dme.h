@@ -297,6 +297,33 @@ typedef struct dme_job {
 int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs,
                   dme_medata *outs);
 
+/* ---- no-allocation motion estimation ----------------------------------- *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation.  dme_motion_estimate for one picture whose planes are
+ * given in device memory of the context's device (luma and chroma of the
+ * picture, luma of its nrefs references, row-major at stride = width); the
+ * pictures need not be registered.  Every intermediate buffer (pyramids, byte
+ * and quarter-pel planes, search records, the MEData under estimation) lives
+ * in the library's static device scratch array, DME_NOALLOC_SCRATCH_BYTES per
+ * GPU, held from library load until the process ends.  The call allocates no
+ * device or pinned host memory (the context reserved its small pinned
+ * staging at dme_create).  A picture whose buffers exceed the scratch gives
+ * DME_ERR_INVALID and names the size it needs (1920x1080 with 2 references
+ * and 16/12 blocks needs about 90 MB).  pic_num and ref_nums are picture
+ * numbers (full search's temporal distance).  out's non-NULL arrays may be
+ * host or device memory; the call returns when they hold the results.
+ * Calls on one GPU share its scratch array, which keeps the last call's
+ * contents: calls that overlap on one GPU (two contexts, two threads) race. */
+#define DME_NOALLOC_SCRATCH_BYTES (128u << 20)
+int dme_motion_estimate_noalloc(dme_ctx *ctx, const dme_params *params,
+                                int width, int height, int pic_num,
+                                const int16_t *d_y, const int16_t *d_u,
+                                const int16_t *d_v, int nrefs,
+                                const int *ref_nums,
+                                const int16_t *const *d_ref_y,
+                                dme_medata *out);
+
 /* ---- motion compensation ------------------------------------------------ */
 
 /* AddOrSub (common.h:90) */
