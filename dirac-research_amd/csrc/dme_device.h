@@ -65,14 +65,23 @@ struct JobDesc {
     int DME_G *lwork;                                 // 2 x blocks: transition maps (k_lambda)
     // mode decision temporaries (levels 1 and 0)
     float DME_G *lam1, *lam0;
-    dme_mv DME_G *v1[2], *v0[2];
-    dme_cost DME_G *c1[2], *c0[2];
     float DME_G *ucost;                               // level-2 unit costs
+    // the fast form of the learned path (k_fast_*): per reference the
+    // level-0 integer vectors, the pre-merge sub-pel vectors and their SADs;
+    // per superblock the SADs of the chosen units (FSB_INTS each)
+    dme_mv DME_G *fv0[2], *fv2[2];
+    int DME_G *fs2[2];
+    int DME_G *fsb;
     int16_t DME_G *dct[3];                            // block DCs before smoothing
 };
 
 // Replicated samples on every side of a byte plane (k_bytes).
 constexpr int kB8Margin = 32;
+
+// Per superblock of the fast form: for each of up to four units of its
+// split level, the integer SADs of its two vectors and of their
+// bi-prediction (k_fast_mode), read by the exact-cost pass (k_fast_eval).
+constexpr int kFsbInts = 12;
 
 // A plane conversion of a batched pyramid / up-conversion launch: `in` of
 // width in_w, `out` of ow x oh samples (the source size for up-conversion),

@@ -70,6 +70,12 @@ hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, in
 hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st);
 hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_width,
                            hipStream_t st);
+hipError_t launch_fast_subpel(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                              int max_blen_y, hipStream_t st);
+hipError_t launch_fast_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_fast_mode(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x, int max_y,
+                            hipStream_t st);
+hipError_t launch_fast_eval(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 
 #define HIP_TRY(expr)                                                                                   \
@@ -155,6 +161,7 @@ struct JobState {
     DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1];
     DevBuf ptab[2], qtab[2], m1tab, m0tab;
     DevBuf lwork;  // transition maps of the device lambda stage
+    DevBuf fv0[2], fv2[2], fs2[2], fsb;  // the learned path's fast form (learned jobs only)
     int bpos = 0;  // position in the current batch (device lambda flags)
     JobDesc desc;
 };
@@ -251,6 +258,7 @@ struct dme_ctx {
     dme::DevBuf lp_stats;
     size_t lp_slots = 0;
     bool lp_dirty = false;  // a learned batch counted since the last fold
+    int lp_form = DME_LP_FORM_FAST;
     long long lp_blocks = 0, lp_fallbacks = 0;
     std::vector<dme::Task> h_tasks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -270,12 +278,14 @@ namespace dme {
 enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
-    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_COUNT
+    KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_FAST_SUBPEL, KK_FAST_L2, KK_FAST_MODE,
+    KK_FAST_EVAL, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
-    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full"};
+    "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full",
+    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -526,6 +536,14 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     DME_TRY(J.lam0.ensure(sizeof(float) * B / 16));
     DME_TRY(J.ucost.ensure(sizeof(float) * B));
     DME_TRY(J.lwork.ensure(sizeof(int) * 2 * B));
+    if (learned) {
+        for (int r = 0; r < 2; ++r) {
+            DME_TRY(J.fv0[r].ensure(sizeof(dme_mv) * B));
+            DME_TRY(J.fv2[r].ensure(sizeof(dme_mv) * B));
+            DME_TRY(J.fs2[r].ensure(sizeof(int) * B));
+        }
+        DME_TRY(J.fsb.ensure(sizeof(int) * kFsbInts * S));
+    }
 
     JobDesc &d = J.desc;
     std::memset(&d, 0, sizeof(d));
@@ -580,6 +598,12 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     d.lam0 = J.lam0.as<float>();
     d.ucost = J.ucost.as<float>();
     d.lwork = J.lwork.as<int>();
+    for (int r = 0; r < 2; ++r) {
+        d.fv0[r] = J.fv0[r].as<dme_mv>();
+        d.fv2[r] = J.fv2[r].as<dme_mv>();
+        d.fs2[r] = J.fs2[r].as<int>();
+    }
+    d.fsb = J.fsb.as<int>();
     d.lambda = p.lambda;
     return DME_OK;
 }
@@ -2110,6 +2134,155 @@ static int stage_pel_learned(dme_ctx *ctx, std::vector<JobState *> &js, const st
     return DME_OK;
 }
 
+// The learned pel stage of the fast form: the plain search of the coarse
+// levels down to 2 (the network's inputs are that search's level-2 results),
+// the network on the level-0 block grids and the exact refinement of its
+// proposals.  The level-0 vectors keep their SAD; their mvcost and total are
+// not needed (the sub-pel stage recomputes both).
+static int stage_pel_fast(dme_ctx *ctx, std::vector<JobState *> &js, const std::vector<LpGroup> &groups,
+                          int &launches) {
+    if (!ctx->lp_ready) return set_error(DME_ERR_STATE, "dme_lp_set_weights was not called");
+    int bx = 1, by = 1;
+    for (auto *j : js) {
+        if (j->g.depth < 2) return set_error(DME_ERR_INVALID, "learned proposals need a pyramid depth of at least 2");
+        bx = std::max(bx, j->g.blen[2][0]);
+        by = std::max(by, j->g.blen[2][1]);
+    }
+    DME_TRY(stage_pel(ctx, js, launches, 2));
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<Task> fb, rb;
+    int offf = 0, offr = 0;
+    for (size_t i = 0; i < js.size(); ++i) {
+        const int cnt = js[i]->g.xnb * js[i]->g.ynb;
+        fb.push_back(Task{(int)i, 0, cnt, offf});
+        offf += cnt;
+        for (int r = 0; r < js[i]->nrefs; ++r) {
+            rb.push_back(Task{(int)i, r, cnt, offr});
+            offr += cnt;
+        }
+    }
+    const TaskTable::Range feat = tt.add(fb), refine = tt.add(rb);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    hipStream_t st = ctx->L->stream;
+    uint16_t *act[4];
+    for (int a = 0; a < 4; ++a) act[a] = ctx->lp_act[a].as<uint16_t>();
+    const uint16_t *frag = ctx->lp_frag.as<uint16_t>();
+    const float *w = ctx->lp_w.as<float>();
+    DME_TRY(timed_launch(ctx, KK_LP_NET, launches, [&]() -> hipError_t {
+        hipError_t e = launch_lp_features(dj, dtask(ctx, feat), feat.n, feat.total, act[0], nullptr, st);
+        for (const LpGroup &g : groups) {
+            const size_t o = (size_t)g.off * 32;
+            const int npos = g.n * g.W * g.H;
+            for (int layer = 0; layer < 4 && e == hipSuccess; ++layer) {
+                const uint16_t *in = act[layer] + o;
+                const uint16_t *fr = frag + (size_t)layer * 2 * DME_LP_FRAG_ELEMS;
+                const float *bias = w + (size_t)layer * DME_LP_LAYER_PARAMS + 9 * 32 * 32;
+                if (layer < 3)
+                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 1, act[layer + 1] + o, nullptr, st);
+                else
+                    e = launch_lp_conv(in, npos, g.H, g.W, fr, bias, 0, nullptr, ctx->lp_out.as<float>() + o, st);
+            }
+        }
+        return e;
+    }));
+    ctx->lp_dirty = true;
+    DME_TRY(timed_launch(ctx, KK_LP_REFINE, launches, [&] {
+        return launch_lp_refine(dj, dtask(ctx, refine), refine.n, refine.total, ctx->lp_out.as<float>(),
+                                ctx->lp_stats.as<unsigned long long>(), bx, by, st);
+    }));
+    for (auto *j : js) j->stage = 1;
+    return DME_OK;
+}
+
+// Sub-pel stage of the fast form: every block refined at once with the
+// predictor of the integer vectors (k_fast_subpel), then the level-2 costs of
+// the final vectors with the reference's predictor (k_fast_l2).
+static int stage_subpel_fast(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    DME_TRY(bind_planes(ctx, js, true, launches));
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<Task> bt;
+    int off = 0, bx = 1, by = 1;
+    for (size_t i = 0; i < js.size(); ++i)
+        for (int r = 0; r < js[i]->nrefs; ++r) {
+            const int cnt = js[i]->g.xnb * js[i]->g.ynb;
+            bt.push_back(Task{(int)i, r, cnt, off});
+            off += cnt;
+            bx = std::max(bx, js[i]->g.blen[2][0]);
+            by = std::max(by, js[i]->g.blen[2][1]);
+        }
+    const TaskTable::Range blk = tt.add(bt);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    hipStream_t st = ctx->L->stream;
+    DME_TRY(timed_launch(ctx, KK_FAST_SUBPEL, launches,
+                         [&] { return launch_fast_subpel(dj, dtask(ctx, blk), blk.n, blk.total, bx, by, st); }));
+    DME_TRY(timed_launch(ctx, KK_FAST_L2, launches,
+                         [&] { return launch_fast_l2(dj, dtask(ctx, blk), blk.n, blk.total, st); }));
+    for (auto *j : js) j->stage = 3;
+    return DME_OK;
+}
+
+// Mode decision of the fast form: the level-2 unit decisions (k_mode_l2, as
+// the plain path), levels 1 and 0 of every superblock at once (k_fast_mode),
+// the reference's costs of the chosen split (k_fast_eval), SetDC.
+static int stage_mode_fast(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    DME_TRY(bind_planes(ctx, js, true, launches));
+    DME_TRY(upload_jobs(ctx, js));
+    TaskTable tt;
+    std::vector<Task> l2, sb, dc, dr;
+    int off = 0, offs = 0, offd = 0, offr = 0, max_w = 0, ux = 1, uy = 1;
+    for (size_t i = 0; i < js.size(); ++i) {
+        const Geometry &g = js[i]->g;
+        const int cnt = g.xnb * g.ynb;
+        l2.push_back(Task{(int)i, 0, cnt, off});
+        off += cnt;
+        sb.push_back(Task{(int)i, 0, g.xsb * g.ysb, offs});
+        offs += g.xsb * g.ysb;
+        for (int c = 0; c < 3; ++c) {
+            dc.push_back(Task{(int)i, c, cnt, offd});
+            offd += cnt;
+            dr.push_back(Task{(int)i, c, g.ynb, offr});
+            offr += g.ynb;
+        }
+        max_w = std::max(max_w, g.W);
+        ux = std::max(ux, std::max(g.blen[0][0], g.blen[1][0]));
+        uy = std::max(uy, std::max(g.blen[0][1], g.blen[1][1]));
+    }
+    const TaskTable::Range rl2 = tt.add(l2), rsb = tt.add(sb), rdc = tt.add(dc), rdr = tt.add(dr);
+    DME_TRY(upload_tasks(ctx, tt));
+    const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    hipStream_t st = ctx->L->stream;
+    DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_FAST_MODE, launches,
+                         [&] { return launch_fast_mode(dj, dtask(ctx, rsb), rsb.n, rsb.total, ux, uy, st); }));
+    DME_TRY(timed_launch(ctx, KK_FAST_EVAL, launches,
+                         [&] { return launch_fast_eval(dj, dtask(ctx, rsb), rsb.n, rsb.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,
+                         [&] { return launch_block_dc(dj, dtask(ctx, rdr), rdr.n, rdr.total, max_w, st); }));
+    DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
+                         [&] { return launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
+    for (auto *j : js) j->stage = 4;
+    return DME_OK;
+}
+
+// The stages of a part of a learned batch, in the form the context selects.
+static int learned_pel(dme_ctx *ctx, std::vector<JobState *> &js, const std::vector<LpGroup> &g, int &launches) {
+    return ctx->lp_form == DME_LP_FORM_FAST ? stage_pel_fast(ctx, js, g, launches)
+                                            : stage_pel_learned(ctx, js, g, launches);
+}
+static int learned_rest(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
+    if (ctx->lp_form == DME_LP_FORM_FAST) {
+        DME_TRY(stage_subpel_fast(ctx, js, launches));
+        return stage_mode_fast(ctx, js, launches);
+    }
+    DME_TRY(stage_subpel(ctx, js, launches));
+    return stage_mode(ctx, js, launches);
+}
+
 // The live sequence with learned proposals, in parts on the lanes as
 // run_jobs does.
 // Jobs whose device lambda maps were flagged run again (learned pel stage,
@@ -2127,10 +2300,9 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
     for (int h = 0; h < np; ++h) P = lp_layout(part[h], P, groups[h]);
     DME_TRY(lp_reserve(ctx, P, js.size()));
     if (np == 1) {
-        DME_TRY(stage_pel_learned(ctx, js, groups[0], launches));
+        DME_TRY(learned_pel(ctx, js, groups[0], launches));
         DME_TRY(stage_lambda(ctx, js, nullptr, false, launches));
-        DME_TRY(stage_subpel(ctx, js, launches));
-        DME_TRY(stage_mode(ctx, js, launches));
+        DME_TRY(learned_rest(ctx, js, launches));
     } else {
         // as run_jobs: shared planes before the fork, each part's own planes
         // after the previous part's
@@ -2144,13 +2316,12 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
                 DME_TRY(bind_planes(ctx, part[h], true, launches));
                 HIP_TRY(hipEventRecord(ctx->ev_planes[h], ctx->L->stream));
             }
-            DME_TRY(stage_pel_learned(ctx, part[h], groups[h], launches));
+            DME_TRY(learned_pel(ctx, part[h], groups[h], launches));
         }
         for (int h = 0; h < np; ++h) {
             ctx->L = &ctx->lane[h];
             DME_TRY(stage_lambda(ctx, part[h], nullptr, false, launches));
-            DME_TRY(stage_subpel(ctx, part[h], launches));
-            DME_TRY(stage_mode(ctx, part[h], launches));
+            DME_TRY(learned_rest(ctx, part[h], launches));
         }
         ctx->L = &ctx->lane[0];
         for (int h = 1; h < np; ++h) {
@@ -2170,12 +2341,22 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
         for (auto *j : redo)
             HIP_TRY(launch_lp_zero(ctx->lp_stats.as<unsigned long long>() + 2 * j->bpos, 2 * sizeof(unsigned long long),
                                    ctx->L->stream));
-        DME_TRY(stage_pel_learned(ctx, redo, g, launches));
+        DME_TRY(learned_pel(ctx, redo, g, launches));
         DME_TRY(stage_lambda_host(ctx, redo, nullptr));
-        DME_TRY(stage_subpel(ctx, redo, launches));
-        DME_TRY(stage_mode(ctx, redo, launches));
+        DME_TRY(learned_rest(ctx, redo, launches));
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     }
+    return DME_OK;
+}
+
+// The fast form decides sub-pel vectors, so it needs half or quarter pel,
+// and costs them with the MMX build's float semantics.
+static int learned_params_ok(dme_ctx *ctx, const dme_params &p) {
+    if (ctx->lp_form != DME_LP_FORM_FAST) return DME_OK;
+    if (p.mv_precision < 1)
+        return set_error(DME_ERR_INVALID, "the fast learned form refines to half or quarter pel (mv_precision 1 or 2)");
+    if (p.fp_semantics != DME_FP_MMX)
+        return set_error(DME_ERR_INVALID, "the fast learned form uses the MMX build's float semantics");
     return DME_OK;
 }
 
@@ -2215,6 +2396,7 @@ extern "C" int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int 
     JobState *J = job_for(ctx, pic_num);
     if (ctx->params.full_search)
         return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
+    DME_TRY(learned_params_ok(ctx, ctx->params));
     DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs, true));
     if (J->p.full_search)
         return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
@@ -2235,6 +2417,7 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
         JobState *J = job_for(ctx, jobs[i].pic_num);
         if (jobs[i].params.full_search)
             return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
+        DME_TRY(learned_params_ok(ctx, jobs[i].params));
         DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs, true));
         if (J->p.full_search)
             return set_error(DME_ERR_INVALID, "learned proposals replace the hierarchical search; use full search without them");
@@ -2243,6 +2426,33 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
     DME_TRY(run_jobs_learned(ctx, js));
     if (outs)
         for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_lp_set_form(dme_ctx *ctx, int form) try {
+    CTX_CHECK(ctx);
+    if (form != DME_LP_FORM_FAST && form != DME_LP_FORM_EXACT_STAGES)
+        return set_error(DME_ERR_INVALID, "form must be DME_LP_FORM_FAST or DME_LP_FORM_EXACT_STAGES");
+    ctx->lp_form = form;
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_lp_get_vectors(dme_ctx *ctx, int pic_num, int which, dme_mv *out1, dme_mv *out2) try {
+    CTX_CHECK(ctx);
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 4 || !it->second->fv0[0].p)
+        return set_error(DME_ERR_STATE, "no fast learned estimate of picture " + std::to_string(pic_num));
+    if (which != 0 && which != 1) return set_error(DME_ERR_INVALID, "which must be 0 or 1");
+    JobState &J = *it->second;
+    const size_t B = (size_t)J.g.xnb * J.g.ynb;
+    dme_mv *out[2] = {out1, out2};
+    for (int r = 0; r < J.nrefs; ++r)
+        if (out[r])
+            HIP_TRY(hipMemcpy(out[r], which ? J.fv2[r].p : J.fv0[r].p, sizeof(dme_mv) * B, hipMemcpyDeviceToHost));
     return DME_OK;
 } catch (...) {
     return dme::exception_status();

@@ -29,6 +29,7 @@
 #include <climits>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "dme_device.h"
 #include "dme_internal.h"
@@ -61,6 +62,20 @@ __device__ __forceinline__ int xcd_block() {
 // wave-uniform, so that what is derived from it stays in scalar registers.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// The task of work item gi for a wave whose lanes all hold the same gi:
+// the number of tasks starting at or before gi, counted by a ballot over
+// one coalesced load per 64 tasks (one memory round trip instead of a
+// binary search's chain of dependent loads).  Tasks are sorted by offset.
+__device__ __forceinline__ int find_task_wave(const Task *t, int n, int gi) {
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        cnt += __popcll(__ballot(j < n && t[j].offset <= gi));
+    }
+    return cnt - 1;
+}
+
 __device__ __forceinline__ int find_task(const Task *t, int n, int gi) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -79,9 +94,18 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
+// Sum over the 64 lanes of the wave (all active), returned to every lane:
+// an inclusive scan within each row of 16 lanes by DPP row shifts (1, 2, 4,
+// 8; a lane whose source is outside its row adds 0), the rows joined by the
+// row broadcasts of lanes 15 and 31, the total read from lane 63.
 __device__ __forceinline__ int wave_sum(int v) {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 to rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 to rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // BlockDiffParams::SetBlockLimits (me_utils.cpp:58-78)
@@ -3197,6 +3221,743 @@ __global__ __launch_bounds__(256) void k_dc_smooth(const JobDesc *__restrict__ j
 // Launch wrappers (host side, same translation unit as the kernels)
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// The fast form of the learned path (no counterpart in the reference).  The
+// learned stage gives each block its level-0 integer vector; then every
+// decision is made without waiting for its neighbours' decisions of the same
+// pass, and the reference's Lagrangian of the final vectors is evaluated with
+// the reference's own predictors:
+//   k_fast_subpel  RefineMatchSubp (block_match.cpp:327-467) with the
+//                  predictor of the integer vectors (GetPred of me_subpel.cpp
+//                  on them), costs float(SAD) + start in one rounding
+//   k_fast_l2      the level-2 costs SubpelRefine leaves, with GetPred of the
+//                  final sub-pel vectors
+//   k_mode_l2      (shared with the plain path) level-2 unit decisions
+//   k_fast_mode    levels 1 and 0 of every superblock with DoME's predictor
+//                  taken from the pre-merge level-2 vectors; the split
+//   k_fast_eval    superblocks split into units: the unit costs with the
+//                  predictor DoModeDecn's raster pass would form (the final
+//                  vectors of earlier superblocks, the pre-merge ones inside),
+//                  DoUnitDecn and the sum
+// The oracle restates the same steps (oracle_fast_me).
+// ---------------------------------------------------------------------------
+
+// Packed biased bytes of samples x .. x + 3 of row y of a byte plane (origin
+// dword-aligned, margins around the picture).
+__device__ __forceinline__ uint32_t b8_dword(const uint8_t *__restrict__ origin, int pitch, int x, int y) {
+    const uintptr_t a = (uintptr_t)(origin + (ptrdiff_t)y * pitch + x);
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
+}
+
+// Lane layout of a wave over a unit of xl x yl samples: dword column col
+// (four samples) of rows row0, row0 + rpp, ...; mask keeps the samples of a
+// partial last column.
+struct UnitLanes {
+    int nd, rpp, row0, col, npass;
+    uint32_t mask;
+    bool on;
+};
+
+__device__ __forceinline__ UnitLanes unit_lanes(const Blk &b) {
+    UnitLanes L;
+    const int lane = threadIdx.x & (WAVE - 1);
+    L.nd = (b.xl + 3) >> 2;
+    L.rpp = WAVE / L.nd;
+    L.row0 = (int)(((float)lane + 0.5f) * (1.0f / (float)L.nd));
+    L.col = lane - L.row0 * L.nd;
+    L.on = L.row0 < L.rpp;
+    const int valid = b.xl - 4 * L.col;
+    L.mask = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+    L.npass = (b.yl + L.rpp - 1) / L.rpp;
+    return L;
+}
+
+// Row pass p of lane layout L: the row, or -1 off the unit.
+__device__ __forceinline__ int unit_row(const UnitLanes &L, int p, int yl) {
+    const int row = L.row0 + p * L.rpp;
+    return L.on && row < yl ? row : -1;
+}
+
+// Packed quarter-pel samples of candidate c (quarter-pel vector) for lane
+// layout L at unit row `row`: byte fx of four consecutive dwords of phase
+// row fy (qp layout of k_build_qp), clamped reads.
+struct QpCand {
+    const uint32_t *qp;
+    int X0, Y0;
+    uint32_t sel;
+    int fy;
+};
+
+__device__ __forceinline__ QpCand qp_cand(const uint32_t *__restrict__ qp, const Blk &b, dme_mv c) {
+    QpCand q;
+    const int xq = 4 * b.xp + c.x, yq = 4 * b.yp + c.y;
+    q.qp = qp;
+    q.X0 = xq >> 2;
+    q.Y0 = yq >> 2;
+    const int fx = xq & 3;
+    q.fy = yq & 3;
+    q.sel = (uint32_t)fx | ((uint32_t)(fx + 4) << 8) | 0x0c0c0000u;
+    return q;
+}
+
+__device__ __forceinline__ uint32_t qp_pack(const QpCand &q, int W, int H, int col, int row) {
+    const uint32_t QW = (uint32_t)W + 2u;
+    // 32-bit offsets from the plane (a plane is far below 4 G dwords)
+    const uint32_t ro = (uint32_t)((clampi(q.Y0 + row, -1, H) + 1) * 4 + q.fy) * QW;
+    uint32_t d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d[i] = q.qp[ro + (uint32_t)(clampi(q.X0 + 4 * col + i, -1, W) + 1)];
+    const uint32_t lo = __builtin_amdgcn_perm(d[1], d[0], q.sel);
+    const uint32_t hi = __builtin_amdgcn_perm(d[3], d[2], q.sel);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// Up to four wave-uniform candidates and their SADs, in registers.
+struct Cand4 {
+    dme_mv c[4];
+};
+struct Sad4 {
+    int s[4];
+};
+
+// Integer SADs of the first n (<= N) quarter-pel candidates of unit b
+// against the quarter-pel planes qp; the unit's packed picture dwords are
+// s_pic[row * pd + col].  Wave-uniform results.
+template <int N>
+__device__ __forceinline__ Sad4 qp_sads(const JobDesc &J, const uint32_t *__restrict__ qp, const Blk &b,
+                                        const UnitLanes &L, const uint32_t *__restrict__ s_pic, int pd, const Cand4 &c,
+                                        int n) {
+    QpCand q[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) q[i] = qp_cand(qp, b, c.c[i < n ? i : 0]);
+    uint32_t acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = 0u;
+    for (int p = 0; p < L.npass; ++p) {
+        const int row = unit_row(L, p, b.yl);
+        if (row < 0) continue;
+        const uint32_t pv = s_pic[row * pd + L.col];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < n) acc[i] = __builtin_amdgcn_sad_u8(pv, qp_pack(q[i], J.W, J.H, L.col, row) & L.mask, acc[i]);
+    }
+    Sad4 out;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out.s[i] = (i < N && i < n) ? wave_sum((int)acc[i < N ? i : 0]) : 0;
+    return out;
+}
+
+// Bi-prediction SAD of unit b (BiBlock*Pel::Diff, me_utils.cpp:1172-1543):
+// |((2p - r1) - r2) >> 1| = |p - ceil((r1 + r2) / 2)|, a byte SAD against the
+// rounded-up average of the two candidates' biased samples.
+__device__ __forceinline__ int qp_bi_sad(const JobDesc &J, const Blk &b, const UnitLanes &L,
+                                      const uint32_t *__restrict__ s_pic, int pd, dme_mv c1, dme_mv c2) {
+    // the vectors are in 1/2^step pel, the planes in quarter pel
+    const int qs = 2 - J.step;
+    c1.x <<= qs;
+    c1.y <<= qs;
+    c2.x <<= qs;
+    c2.y <<= qs;
+    const QpCand q1 = qp_cand(J.qp[0], b, c1), q2 = qp_cand(J.qp[1], b, c2);
+    uint32_t acc = 0;
+    for (int p = 0; p < L.npass; ++p) {
+        const int row = unit_row(L, p, b.yl);
+        if (row < 0) continue;
+        const uint32_t a = qp_pack(q1, J.W, J.H, L.col, row), d = qp_pack(q2, J.W, J.H, L.col, row);
+        acc = __builtin_amdgcn_sad_u8(s_pic[row * pd + L.col], avg_up_u8(a, d) & L.mask, acc);
+    }
+    return wave_sum((int)acc);
+}
+
+// Stages unit b's packed picture bytes (masked past xl) into s_pic rows of pd
+// dwords.
+__device__ __forceinline__ void stage_unit_pic(const JobDesc &J, const Blk &b, const UnitLanes &L, uint32_t *s_pic,
+                                               int pd) {
+    for (int p = 0; p < L.npass; ++p) {
+        const int row = unit_row(L, p, b.yl);
+        if (row >= 0) s_pic[row * pd + L.col] = b8_dword(J.pic8, J.p8, b.xp + 4 * L.col, b.yp + row) & L.mask;
+    }
+    WAVE_SYNC();
+}
+
+// GetPred (me_subpel.cpp:145-176): MvMedian of the available neighbours
+// left, top-left, top, top-right of field V (scaled by `sh`).  Which are
+// available depends only on the position: none or the left one in row 0,
+// top and top-right in column 0, left, top-left and top in the last column,
+// all four elsewhere.
+__device__ __forceinline__ dme_mv subpel_pred(const dme_mv *__restrict__ V, int X, int Y, int x, int y, int sh) {
+    auto at = [&](int xx, int yy) {
+        dme_mv v = V[(size_t)yy * X + xx];
+        v.x <<= sh;
+        v.y <<= sh;
+        return v;
+    };
+    dme_mv p;
+    p.x = p.y = 0;
+    if (y == 0) {
+        if (x > 0) p = at(x - 1, 0);
+    } else if (x == 0) {
+        p = X > 1 ? mean2(at(0, y - 1), at(1, y - 1)) : at(0, y - 1);
+    } else if (x == X - 1) {
+        p = med3(at(x - 1, y), at(x - 1, y - 1), at(x, y - 1));
+    } else {
+        const dme_mv a = at(x - 1, y), b = at(x - 1, y - 1), c = at(x, y - 1), d = at(x + 1, y - 1);
+        p.x = (a.x + b.x + c.x + d.x - max(max(a.x, b.x), max(c.x, d.x)) - min(min(a.x, b.x), min(c.x, d.x)) + 1) >> 1;
+        p.y = (a.y + b.y + c.y + d.y - max(max(a.y, b.y), max(c.y, d.y)) - min(min(a.y, b.y), min(c.y, d.y)) + 1) >> 1;
+    }
+    return p;
+}
+
+// Sub-pel refinement of every block at once: RefineMatchSubp's staged
+// search (block_match.cpp:327-467) with the predictor of the integer
+// vectors.  One wave per (block, reference).  Every candidate of the staged
+// search lies within 3 quarter pels of 4 x the integer vector v, so the wave
+// first stages the quarter-pel samples of positions v - 1 .. v + len of all
+// 16 phases, transposed into 16 byte planes (phase (fx, fy) at plane
+// 4 fy + fx, rows of FS_PITCH bytes); a candidate's four samples of a lane are
+// then one unaligned dword of one plane.  The predictor, a multiple of four
+// quarter pels away, is read from the staged planes when it equals 4 v and
+// gathered from the quarter-pel planes otherwise.  Writes the integer
+// vectors (fv0), the sub-pel vectors (fv2) and the SAD each sub-pel vector's
+// cost is measured by (fs2): the pel search's SAD while the vector is the
+// scaled integer vector (RefineMatchSubp keeps that cost), else its
+// up-sampled SAD.
+struct FsLds {
+    int rows;   // staged rows per phase plane (block rows + 2)
+    int pitch;  // bytes per staged row (a multiple of 4)
+    int pd;     // dwords per picture row
+    int wave;   // dwords per wave: 16 planes, then the picture block
+};
+
+// Sum over the wave of two non-negative values below 2^16 in one reduction.
+__device__ __forceinline__ void wave_sum2(int &a, int &b, bool packed) {
+    if (packed) {
+        const int s = wave_sum(a | (b << 16));
+        a = s & 0xffff;
+        b = (int)((unsigned)s >> 16);
+    } else {
+        a = wave_sum(a);
+        b = wave_sum(b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fast_subpel(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                     int ntasks, int total, FsLds ly) {
+    extern __shared__ uint32_t s_fs[];
+    const int wv = wave_id(), lane = threadIdx.x & (WAVE - 1);
+    const int gi = xcd_block() * 4 + wv;
+    if (gi >= total) return;
+    uint32_t *s_ph = s_fs + wv * ly.wave;
+    uint32_t *s_pic = s_ph + 16 * ly.rows * (ly.pitch >> 2);
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, k = gi - tk.offset, X = J.xnb, Y = J.ynb, xb = k % X, yb = k / X;
+    const int prec = J.prec, W = J.W, H = J.H;
+    const dme_mv *__restrict__ V0 = J.vec[r];
+    const dme_mv pel = V0[k];
+    const dme_mv pred = subpel_pred(V0, X, Y, xb, yb, prec);
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
+    dme_mv out = pred;
+    int out_sad = 0;
+    dme_mv pel_s;
+    pel_s.x = pel.x << prec;
+    pel_s.y = pel.y << prec;
+    const float pel_sad = J.pc[r][k].SAD;
+    if (!b.empty()) {
+        const float lambda = J.lam[k];
+        const float lim = (float)(2 * b.xl * b.yl);
+        float best_total = pel_sad + lambda * var_up(pred, pel_s);
+        if (pel_sad < lim) {
+            out = pel_s;
+            out_sad = (int)pel_sad;
+        } else {
+            const UnitLanes L = unit_lanes(b);
+            const uint32_t *__restrict__ qp = J.qp[r];
+            // stage: qp rows (Y + 1) * 4 + fy for Y = yp + v.y - 1 .. yp + v.y + yl,
+            // four positions per lane-item, transposed into the phase planes
+            {
+                // a lane keeps one group of four positions, gx, across staged rows r0, r0 + rpp, ...
+                const int QW = W + 2, Y0 = b.yp + pel.y - 1, X0 = b.xp + pel.x - 1;
+                const int ng = ly.pitch >> 2, rpp = WAVE / ng, r0 = lane / ng, gx = lane - r0 * ng;
+                int xo[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * gx + i, -1, W) + 1;
+                // loads of up to eight passes in flight before any is used
+                const int nrows = 4 * ly.rows, first = r0 < rpp ? r0 : nrows;
+                for (int rb = first; rb < nrows; rb += 8 * rpp) {
+                    uint32_t d[8][4];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int rr = rb + u * rpp;
+                        if (rr < nrows) {
+                            const int ry = rr >> 2, fy = rr & 3;
+                            const uint32_t ro = (uint32_t)((clampi(Y0 + ry, -1, H) + 1) * 4 + fy) * (uint32_t)QW;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) d[u][i] = qp[ro + (uint32_t)xo[i]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int rr = rb + u * rpp;
+                        if (rr >= nrows) break;
+                        const int ry = rr >> 2, fy = rr & 3;
+                        const uint32_t lo01 = __builtin_amdgcn_perm(d[u][1], d[u][0], 0x05010400u);
+                        const uint32_t hi01 = __builtin_amdgcn_perm(d[u][1], d[u][0], 0x07030602u);
+                        const uint32_t lo23 = __builtin_amdgcn_perm(d[u][3], d[u][2], 0x05010400u);
+                        const uint32_t hi23 = __builtin_amdgcn_perm(d[u][3], d[u][2], 0x07030602u);
+                        uint32_t *o = s_ph + (fy * 4 * ly.rows + ry) * ng + gx;
+                        const int ps = ly.rows * ng;  // dwords per plane
+                        o[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+                        o[ps] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+                        o[2 * ps] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+                        o[3 * ps] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+                    }
+                }
+            }
+            // per pass (at most four: blocks up to 32 rows, eight or more rows a pass)
+            // a lane's picture dword and the byte offset of its row and column in a plane
+            uint32_t pv[4];
+            int loff[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const int row = p < L.npass ? unit_row(L, p, b.yl) : -1;
+                pv[p] = row >= 0 ? b8_dword(J.pic8, J.p8, b.xp + 4 * L.col, b.yp + row) & L.mask : 0u;
+                loff[p] = row >= 0 ? (row + 1) * ly.pitch + 4 * L.col + 1 : -1;
+                if (row >= 0) s_pic[row * ly.pd + L.col] = pv[p];  // for the gathered predictor
+            }
+            WAVE_SYNC();
+            const bool packed = b.xl * b.yl * 255 < 65536;
+            const int plane = ly.rows * ly.pitch;
+            // partial SAD of a lane for quarter-pel offset (dx, dy) from 4 v; the
+            // plane offset and the byte alignment are the same for every lane
+            auto part = [&](int dx, int dy) -> int {
+                const int ix = dx >> 2, fx = dx & 3, iy = dy >> 2, fy = dy & 3;
+                const int base = (fy * 4 + fx) * plane + iy * ly.pitch + ix;
+                const uint32_t al = (uint32_t)((1 + ix) & 3);
+                const uint8_t *pl = reinterpret_cast<const uint8_t *>(s_ph);
+                uint32_t acc = 0;
+#pragma unroll
+                for (int pp = 0; pp < 4; ++pp) {
+                    if (pp >= L.npass) break;
+                    if (loff[pp] < 0) continue;
+                    const int off = loff[pp] + base;
+                    const uint32_t *q = reinterpret_cast<const uint32_t *>(pl + (off & ~3));
+                    const uint32_t v = __builtin_amdgcn_alignbyte(q[1], q[0], al);
+                    acc = __builtin_amdgcn_sad_u8(pv[pp], v & L.mask, acc);
+                }
+                return (int)acc;
+            };
+            int ps;
+            if (pred.x << (2 - prec) == 4 * pel.x && pred.y << (2 - prec) == 4 * pel.y) {
+                int z = 0;
+                ps = part(0, 0);
+                wave_sum2(ps, z, packed);
+            } else {
+                Cand4 c;
+                c.c[0].x = pred.x << (2 - prec);
+                c.c[0].y = pred.y << (2 - prec);
+                ps = qp_sads<1>(J, qp, b, L, s_pic, ly.pd, c, 1).s[0];
+            }
+            const float pred_total = (float)ps;
+            if (pred_total < lim) {
+                out = pred;
+                out_sad = ps;
+            } else {
+                dme_mv best = pel;  // in 1/2^i pel at stage i
+                int best_sad = -1;  // still the pel vector
+                bool bailed = false;
+                for (int i = 1; i <= prec; ++i) {
+                    best.x <<= 1;
+                    best.y <<= 1;
+                    dme_mv tb = best;
+                    const int sh = prec - i, q = 2 - i;
+                    // offsets from 4 v in quarter pel of candidate c (1/2^i pel)
+                    auto qoff = [&](dme_mv c, int &dx, int &dy) {
+                        dx = (c.x << q) - 4 * pel.x;
+                        dy = (c.y << q) - 4 * pel.y;
+                    };
+                    auto consider = [&](dme_mv c, int sad) {
+                        dme_mv cp;
+                        cp.x = c.x << sh;
+                        cp.y = c.y << sh;
+                        const float t = (float)sad + var_up(pred, cp) * lambda;
+                        if (t < best_total) {
+                            best_total = t;
+                            best_sad = sad;
+                            tb = c;
+                        }
+                    };
+                    dme_mv c4[4];
+                    c4[0].x = best.x - 1; c4[0].y = best.y;
+                    c4[1].x = best.x + 1; c4[1].y = best.y;
+                    c4[2].x = best.x;     c4[2].y = best.y - 1;
+                    c4[3].x = best.x;     c4[3].y = best.y + 1;
+                    int s4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        int dx, dy;
+                        qoff(c4[j], dx, dy);
+                        s4[j] = part(dx, dy);
+                    }
+                    wave_sum2(s4[0], s4[1], packed);
+                    wave_sum2(s4[2], s4[3], packed);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) consider(c4[j], s4[j]);
+                    if (tb.x != best.x || tb.y != best.y) {
+                        const bool movx = tb.x != best.x;  // else y moved
+                        dme_mv c2[2];
+                        c2[0].x = movx ? tb.x : tb.x - 1; c2[0].y = movx ? tb.y - 1 : tb.y;
+                        c2[1].x = movx ? tb.x : tb.x + 1; c2[1].y = movx ? tb.y + 1 : tb.y;
+                        int s2[2];
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) {
+                            int dx, dy;
+                            qoff(c2[j], dx, dy);
+                            s2[j] = part(dx, dy);
+                        }
+                        wave_sum2(s2[0], s2[1], packed);
+                        consider(c2[0], s2[0]);
+                        consider(c2[1], s2[1]);
+                    }
+                    best = tb;
+                    if ((double)best_total > 1.1 * (double)pred_total) {
+                        bailed = true;
+                        break;
+                    }
+                }
+                if (bailed) {
+                    out = pred;
+                    out_sad = ps;
+                } else {
+                    out = best;
+                    out_sad = best_sad < 0 ? (int)pel_sad : best_sad;
+                }
+            }
+        }
+        // a vector equal to the scaled integer vector is costed by the pel SAD
+        if (out.x == pel_s.x && out.y == pel_s.y) out_sad = (int)pel_sad;
+    }
+    if (lane == 0) {
+        J.fv0[r][k] = pel;
+        J.fv2[r][k] = out;
+        J.fs2[r][k] = out_sad;
+    }
+}
+
+// The level-2 costs of the final sub-pel vectors (the PredCosts SubpelRefine
+// leaves): SAD, mvcost = GetVarUp(GetPred of the final field, mv), total =
+// SetTotal(lambda); {0, 0, 0} for a block with no samples.  One thread per
+// (block, reference); the vectors go to the MEData.
+__global__ __launch_bounds__(256) void k_fast_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                 int ntasks, int total) {
+    const int gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= total) return;
+    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, k = gi - tk.offset, X = J.xnb, Y = J.ynb, xb = k % X, yb = k / X;
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, xb, yb);
+    const dme_mv *__restrict__ V2 = J.fv2[r];
+    const dme_mv v = V2[k];
+    dme_cost c;
+    if (b.empty()) {
+        c.SAD = c.mvcost = c.total = 0.0f;
+    } else {
+        c.SAD = (float)J.fs2[r][k];
+        c.mvcost = var_up(subpel_pred(V2, X, Y, xb, yb, 0), v);
+        c.total = c.SAD + J.lam[k] * c.mvcost;
+    }
+    J.vec[r][k] = v;
+    J.pc[r][k] = c;
+}
+
+// One unit of DoME at level 1 or 0 for one reference
+// (me_mode_decn.cpp:279-364, FindBestMatchSubp block_match.cpp:277-325):
+// candidates g[0..4) deduplicated in order, the predictor pred, costs
+// float(SAD) + start in one rounding, strict < from {0, 0, 1e8} and (0, 0)
+// (the zero-initialised level-1/0 MEData).  Returns the best vector and its
+// cost and integer SAD.
+struct UnitBest {
+    dme_mv v;
+    dme_cost c;
+    int sad;
+};
+
+__device__ __forceinline__ UnitBest unit_best(const JobDesc &J, const uint32_t *__restrict__ qp, const Blk &b,
+                                              const UnitLanes &L, const uint32_t *__restrict__ s_pic, int pd,
+                                              const Cand4 &g, dme_mv pred, float lambda) {
+    UnitBest u;
+    u.v.x = u.v.y = 0;
+    u.c.SAD = u.c.mvcost = 0.0f;
+    u.c.total = 100000000.0f;
+    u.sad = 0;
+    if (b.empty()) return u;
+    // AddNewVlist's deduplication: the distinct candidates in order (unrolled
+    // selects, so that they stay in registers)
+    Cand4 cc = g;
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < i; ++j) dup = dup || (g.c[j].x == g.c[i].x && g.c[j].y == g.c[i].y);
+        if (!dup) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t == n) cc.c[t] = g.c[i];
+            ++n;
+        }
+    }
+    Cand4 qc;
+    const int qs = 2 - J.step;  // mode-decision vectors are in 1/2^step pel
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        qc.c[i].x = cc.c[i].x << qs;
+        qc.c[i].y = cc.c[i].y << qs;
+    }
+    const Sad4 sads = qp_sads<4>(J, qp, b, L, s_pic, pd, qc, n);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= n) break;
+        const float mvcost = var_up(pred, cc.c[i]);
+        const float start = mvcost * lambda;
+        const float t = (float)sads.s[i] + start;
+        if (t < u.c.total) {
+            u.v = cc.c[i];
+            u.c.total = t;
+            u.c.mvcost = mvcost;
+            u.c.SAD = t - start;
+            u.sad = sads.s[i];
+        }
+    }
+    return u;
+}
+
+// DoUnitDecn below level 2 (me_mode_decn.cpp:368-462): REF1, REF2 by strict
+// <, then the bi-prediction; no intra.  Returns the unit cost and its mode.
+__device__ __forceinline__ float unit_decn(int nrefs, float lf, float lambda, const dme_cost &c1, const dme_cost &c2,
+                                           int bi_sad, int &mode) {
+    mode = DME_REF1_ONLY;
+    float best = c1.total * lf;
+    if (nrefs > 1) {
+        const float t2 = c2.total * lf;
+        if (t2 < best) {
+            mode = DME_REF2_ONLY;
+            best = t2;
+        }
+        dme_cost bc;
+        bc.mvcost = c1.mvcost + c2.mvcost;
+        bc.SAD = (float)bi_sad;
+        bc.total = bc.SAD + lambda * bc.mvcost;
+        bc.total *= lf;
+        if (bc.total < best) {
+            mode = DME_REF1AND2;
+            best = bc.total;
+        }
+    }
+    return best;
+}
+
+// Levels 1 and 0 of every superblock at once (DoSBDecn and DoLevelDecn,
+// me_mode_decn.cpp:179-276) with DoME's predictor from the pre-merge level-2
+// vectors (fv2).  One wave per superblock; the level-2 unit costs come from
+// k_mode_l2.  Writes the split, the superblock cost, the merged vectors and
+// modes, and the chosen units' SADs (fsb) for k_fast_eval.
+// Level-1 results of a superblock, per wave in LDS.
+struct FastUnits {
+    dme_mv v[4][2];
+    dme_cost c[4][2];
+    int sad[4][2], bi[4], mode[4];
+};
+
+__global__ __launch_bounds__(256) void k_fast_mode(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                   int ntasks, int total, int pd, int prows) {
+    extern __shared__ uint32_t s_fm[];
+    __shared__ FastUnits s_units[4];
+    const int wv = wave_id(), lane = threadIdx.x & (WAVE - 1);
+    const int gi = xcd_block() * 4 + wv;
+    if (gi >= total) return;
+    uint32_t *s_pic = s_fm + wv * pd * prows;
+    FastUnits &U = s_units[wv];
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int s = gi - tk.offset, XS = J.xsb, xs = s % XS, ys = s / XS;
+    const int X = J.xnb, nr = J.nrefs;
+    // level 2: the sum of the blocks' unit costs in raster order
+    float sbc = 0.0f;
+    for (int j = 4 * ys; j < 4 * ys + 4; ++j)
+        for (int i = 4 * xs; i < 4 * xs + 4; ++i) sbc += J.ucost[(size_t)j * X + i];
+    int split = 2;
+    // level 1
+    float c1 = 0.0f;
+    for (int q = 0; q < 4; ++q) {
+        const int i = 2 * xs + (q & 1), j = 2 * ys + (q >> 1), xb = i << 1, yb = j << 1;
+        const Blk b = block_limits(J.blen[1], J.bsep[1], J.boff[1], J.W, J.H, i, j);
+        const float lambda = J.lam1[(size_t)j * (X >> 1) + i];
+        UnitLanes L = {};
+        if (!b.empty()) {
+            L = unit_lanes(b);
+            WAVE_SYNC();
+            stage_unit_pic(J, b, L, s_pic, pd);
+        }
+        UnitBest ub[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) {
+                ub[r] = ub[0];
+                continue;
+            }
+            const dme_mv *__restrict__ V2 = J.fv2[r];
+            Cand4 g;
+            g.c[0] = V2[(size_t)yb * X + xb];
+            g.c[1] = V2[(size_t)yb * X + xb + 1];
+            g.c[2] = V2[(size_t)(yb + 1) * X + xb];
+            g.c[3] = V2[(size_t)(yb + 1) * X + xb + 1];
+            ub[r] = unit_best(J, J.qp[r], b, L, s_pic, pd, g, mode_pred(V2, X, xb, yb), lambda);
+        }
+        const int bs = (nr > 1 && !b.empty()) ? qp_bi_sad(J, b, L, s_pic, pd, ub[0].v, ub[1].v) : 0;
+        int mode;
+        c1 += unit_decn(nr, J.lf[1], lambda, ub[0].c, ub[1].c, bs, mode);
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                U.v[q][r] = ub[r].v;
+                U.c[q][r] = ub[r].c;
+                U.sad[q][r] = ub[r].sad;
+            }
+            U.bi[q] = bs;
+            U.mode[q] = mode;
+        }
+    }
+    WAVE_SYNC();
+    if (c1 <= sbc) {
+        sbc = c1;
+        split = 1;
+    }
+    // level 0: the guides are the level-1 results
+    UnitBest u0[2];
+    int m0, bs0;
+    float c0;
+    {
+        const int xb = 4 * xs, yb = 4 * ys;
+        const Blk b = block_limits(J.blen[0], J.bsep[0], J.boff[0], J.W, J.H, xs, ys);
+        const float lambda = J.lam0[(size_t)ys * (X >> 2) + xs];
+        UnitLanes L = {};
+        if (!b.empty()) {
+            L = unit_lanes(b);
+            WAVE_SYNC();
+            stage_unit_pic(J, b, L, s_pic, pd);
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) {
+                u0[r] = u0[0];
+                continue;
+            }
+            Cand4 g;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g.c[q] = U.v[q][r];
+            u0[r] = unit_best(J, J.qp[r], b, L, s_pic, pd, g, mode_pred(J.fv2[r], X, xb, yb), lambda);
+        }
+        bs0 = (nr > 1 && !b.empty()) ? qp_bi_sad(J, b, L, s_pic, pd, u0[0].v, u0[1].v) : 0;
+        c0 = unit_decn(nr, J.lf[0], lambda, u0[0].c, u0[1].c, bs0, m0);
+    }
+    if (c0 <= sbc) {
+        sbc = c0;
+        split = 0;
+    }
+    // merges into the level-2 arrays (DoLevelDecn's copy; the level-1/0 DC
+    // is zero) and the chosen units' SADs
+    if (split < 2) {
+        for (int t = lane; t < 16; t += WAVE) {
+            const int bi = 4 * xs + (t & 3), bj = 4 * ys + (t >> 2);
+            const size_t d = (size_t)bj * X + bi;
+            const int q = ((t >> 3) << 1) | ((t & 3) >> 1);
+            J.mode[d] = split == 0 ? m0 : U.mode[q];
+            J.dc[0][d] = 0;
+            J.vec[0][d] = split == 0 ? u0[0].v : U.v[q][0];
+            if (nr > 1) J.vec[1][d] = split == 0 ? u0[1].v : U.v[q][1];
+        }
+        if (lane < 4 && (split == 1 || lane == 0)) {
+            int *f = J.fsb + (size_t)s * kFsbInts + 3 * lane;
+            f[0] = split == 0 ? u0[0].sad : U.sad[lane][0];
+            f[1] = nr < 2 ? 0 : (split == 0 ? u0[1].sad : U.sad[lane][1]);
+            f[2] = split == 0 ? bs0 : U.bi[lane];
+        }
+    }
+    if (lane == 0) {
+        J.split[s] = split;
+        J.sbc[s] = sbc;
+    }
+}
+
+// The exact cost of every superblock split into units: each unit's costs
+// with the predictor DoME forms (me_mode_decn.cpp:308-321) from the level-2
+// vectors as DoModeDecn's raster pass sees them -- the final vectors of
+// earlier superblocks, the pre-merge ones inside this one -- the reference's
+// float semantics (one rounding, or the per-sample chain where its
+// bounds-checked path adds per sample), DoUnitDecn and the sum in raster
+// order.  One wave per superblock.
+__global__ __launch_bounds__(256) void k_fast_eval(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                   int ntasks, int total) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int gi = xcd_block() * 4 + wave_id();
+    if (gi >= total) return;
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int s = gi - tk.offset, XS = J.xsb, xs = s % XS, ys = s / XS;
+    const int lev = J.split[s];
+    if (lev == 2) return;
+    const int X = J.xnb, nr = J.nrefs, UW = 2 * J.W, UH = 2 * J.H;
+    const float lf = J.lf[lev];
+    auto field = [&](int r, int u, int v) -> dme_mv {
+        const int sx = u >> 2, sy = v >> 2;
+        const bool earlier = sy < ys || (sy == ys && sx < xs);
+        return earlier ? J.vec[r][(size_t)v * X + u] : J.fv2[r][(size_t)v * X + u];
+    };
+    float sbc = 0.0f;
+    const int n = 1 << lev;  // units across
+    for (int q = 0; q < n * n; ++q) {
+        const int i = (xs << lev) + (q % n), j = (ys << lev) + (q / n);
+        const int xb = i << (2 - lev), yb = j << (2 - lev);
+        const Blk b = block_limits(J.blen[lev], J.bsep[lev], J.boff[lev], J.W, J.H, i, j);
+        const float lambda = lev ? J.lam1[(size_t)j * (X >> 1) + i] : J.lam0[(size_t)j * (X >> 2) + i];
+        const int *f = J.fsb + (size_t)s * kFsbInts + 3 * q;
+        dme_cost c[2];
+        c[1].SAD = c[1].mvcost = 0.0f;
+        c[1].total = 100000000.0f;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) break;
+            dme_mv pred;
+            if (xb > 0 && yb > 0) pred = med3(field(r, xb - 1, yb), field(r, xb - 1, yb - 1), field(r, xb, yb - 1));
+            else if (xb == 0 && yb > 0) pred = mean2(field(r, xb, yb - 1), field(r, xb + 1, yb - 1));
+            else if (xb > 0 && yb == 0) pred = mean2(field(r, xb - 1, yb), field(r, xb - 1, yb + 1));
+            else { pred.x = 0; pred.y = 0; }
+            const dme_mv v = J.vec[r][(size_t)yb * X + xb];
+            c[r].SAD = c[r].mvcost = 0.0f;
+            c[r].total = 100000000.0f;
+            if (!b.empty()) {
+                const float mvcost = var_up(pred, v);
+                const float start = mvcost * lambda;
+                const UpStart us = up_start(b, v, J.step, UW, UH);
+                const float t = us.bounds ? wave_chain(start, J.pic[0], J.W, J.qp[r], UW, UH, b, us)
+                                          : (float)f[r] + start;
+                if (t < c[r].total) {
+                    c[r].total = t;
+                    c[r].mvcost = mvcost;
+                    c[r].SAD = t - start;
+                }
+            }
+        }
+        int mode;
+        sbc += unit_decn(nr, lf, lambda, c[0], c[nr > 1 ? 1 : 0], f[2], mode);
+        const int ub = 1 << (2 - lev);  // the unit's blocks across
+        for (int t = lane; t < ub * ub; t += WAVE) J.mode[(size_t)(yb + t / ub) * X + xb + t % ub] = mode;
+    }
+    if (lane == 0) J.sbc[s] = sbc;
+}
+
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_downconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
@@ -3723,6 +4484,14 @@ static size_t lp_refine_lds(int max_blen_x, int max_blen_y, LpLds &ly) {
 }
 constexpr size_t kLpRefineLdsMax = 64 * 1024;
 
+// LDS of k_fast_mode (four waves): the packed picture rows of the largest
+// unit of the launch (level-1 and level-0 lengths).
+static size_t fast_mode_lds(int max_x, int max_y, int &pd) {
+    pd = (max_x + 3) / 4;
+    return sizeof(uint32_t) * 4 * (size_t)pd * max_y;
+}
+constexpr size_t kFastModeLdsMax = 60 * 1024;  // beside its static per-wave unit results
+
 // The limits of one job's launches, checked on the host before anything is
 // queued: "" if every kernel of the job's stages fits, else the limit.
 std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
@@ -3751,6 +4520,10 @@ std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
         return "mode-decision units exceed the LDS of the unit tables";
     LpLds ll;
     if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 64 KiB of LDS";
+    int fpd;
+    if (learned && fast_mode_lds(std::max(g.blen[0][0], g.blen[1][0]), std::max(g.blen[0][1], g.blen[1][1]), fpd) >
+                       kFastModeLdsMax)
+        return "the learned form's superblock units exceed its LDS";
     if ((size_t)g.W * sizeof(int) > 64 * 1024) return "picture width exceeds the block-DC row buffer";
     return std::string();
 }
@@ -3770,6 +4543,54 @@ hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, 
 hipError_t launch_lp_pel_costs(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_lp_pel_costs, dim3((total + 255) / 256), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+
+// ---- the fast form of the learned path ----------------------------------------
+
+// LDS of k_fast_subpel (four waves) for blocks up to bx x by: 16 phase
+// planes of (by + 2) rows of the positions v - 1 .. v + bx (padded to whole
+// dwords, plus one dword for the unaligned reads), then the picture block.
+static size_t fast_subpel_lds(int max_blen_x, int max_blen_y, FsLds &ly) {
+    ly.rows = max_blen_y + 2;
+    ly.pitch = 4 * ((max_blen_x + 2 + 3) / 4 + 1);
+    ly.pd = (max_blen_x + 3) / 4;
+    ly.wave = 16 * ly.rows * (ly.pitch / 4) + ly.pd * max_blen_y;
+    return sizeof(uint32_t) * 4 * (size_t)ly.wave;
+}
+constexpr size_t kFastSubpelLdsMax = 64 * 1024;
+
+hipError_t launch_fast_subpel(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                              int max_blen_y, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    FsLds ly;
+    const size_t lds = fast_subpel_lds(max_blen_x, max_blen_y, ly);
+    if (lds > kFastSubpelLdsMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fast_subpel, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fast_l2, dim3((total + 255) / 256), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_mode(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x, int max_y,
+                            hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    int pd;
+    const size_t lds = fast_mode_lds(max_x, max_y, pd);
+    if (max_x < 1 || max_y < 1 || lds > kFastModeLdsMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fast_mode, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, pd, max_y);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_eval(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fast_eval, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 

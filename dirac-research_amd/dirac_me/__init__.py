@@ -268,6 +268,17 @@ class Context:
         check(lib().dme_run_batch_learned(self.h, arr, len(jobs), os_))
         return outs
 
+    def set_lp_form(self, form):
+        check(lib().dme_lp_set_form(self.h, form))
+
+    def lp_vectors(self, pic_num, which, geom, nrefs):
+        """Level-0 integer vectors (which 0) or pre-merge sub-pel vectors
+        (which 1) of a fast-form learned estimate, one array per reference."""
+        out = [np.zeros((geom.ynum_blocks, geom.xnum_blocks), MV_DTYPE) for _ in range(nrefs)]
+        p = [o.ctypes.data for o in out] + [None] * (2 - nrefs)
+        check(lib().dme_lp_get_vectors(self.h, pic_num, which, C.c_void_p(p[0]), C.c_void_p(p[1])))
+        return out
+
     def lp_stats(self):
         b, f = C.c_longlong(), C.c_longlong()
         check(lib().dme_lp_get_stats(self.h, C.byref(b), C.byref(f)))

@@ -94,7 +94,7 @@ class dme_imt_picture(C.Structure):
                 ("mv_ylen", C.c_int32), ("mv_xlen", C.c_int32)]
 
 
-DME_NUM_KERNELS = 24
+DME_NUM_KERNELS = 32
 
 
 class dme_stats(C.Structure):
@@ -158,6 +158,8 @@ SIGNATURES = {
     "dme_lp_set_weights": ([_P, _P, C.c_size_t], _I),
     "dme_motion_estimate_learned": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_run_batch_learned": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_lp_set_form": ([_P, _I], _I),
+    "dme_lp_get_vectors": ([_P, _I, _I, _P, _P], _I),
     "dme_lp_get_stats": ([_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
     "dme_lp_training_pair": ([_P, _I, _P, _P], _I),
     "dme_lp_prepare": ([_P, _P, _P, _P], _I),
@@ -176,6 +178,8 @@ DME_LP_CHANNELS = 32
 DME_LP_LAYER_PARAMS = 9 * 32 * 32 + 32
 DME_LP_PARAMS = 4 * DME_LP_LAYER_PARAMS
 DME_LP_FRAG_ELEMS = 9 * 2 * 64 * 8
+DME_LP_FORM_FAST = 0
+DME_LP_FORM_EXACT_STAGES = 1
 
 _lib = None
 

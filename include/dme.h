@@ -402,7 +402,7 @@ int dme_invalidate_caches(dme_ctx *ctx, int pic_num);
 /* Counters of the last completed pipeline call on this context.  Per-kernel
  * times are recorded only while profiling is enabled (events around every
  * launch on the context's stream).                                          */
-#define DME_NUM_KERNELS 24
+#define DME_NUM_KERNELS 32
 typedef struct dme_stats {
     double  sad_terms;        /* algorithmic SAD terms of the last call       */
     double  kernel_ms;        /* stream time of the last call (events)        */

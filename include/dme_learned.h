@@ -10,7 +10,8 @@
  * Exact integer SADs over +-2 pel windows around the candidates choose the
  * level-0 pel vector; where that SAD exceeds 1.2 x the predicted one, a wider
  * exact search (+-4 around the level-2 guide, +-2 around zero) joins in.
- * Sub-pel refinement and mode decision then run unchanged.
+ * Sub-pel refinement and mode decision follow in one of two forms
+ * (dme_lp_set_form).
  *
  * Tensors passed to the layer entry points are device pointers, NHWC over
  * the block grid with 32 channels; activations and gradients fed to MFMA are
@@ -48,6 +49,27 @@ int dme_motion_estimate_learned(dme_ctx *ctx, int pic_num, const int *refs, int 
 
 /* Batched form of dme_motion_estimate_learned (as dme_run_batch). */
 int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs);
+
+/* The form of the learned entry points.
+ *   DME_LP_FORM_FAST (the default): the network's level-0 vectors, then
+ *     sub-pel refinement and mode decision with every block and superblock
+ *     decided at once (the predictors come from the integer vectors and from
+ *     the pre-merge sub-pel vectors), then the reference's Lagrangian of the
+ *     final vectors with the reference's own predictors (SubpelRefine's
+ *     GetPred, me_subpel.cpp:145-176; ModeDecider::DoME's predictor in
+ *     DoModeDecn's raster order, me_mode_decn.cpp:308-321): the PredCosts,
+ *     SBCosts and modes it writes are those the reference assigns to these
+ *     vectors.  Half or quarter pel, MMX float semantics.
+ *   DME_LP_FORM_EXACT_STAGES: the network's level-0 vectors, then the plain
+ *     path's sub-pel and mode-decision stages unchanged. */
+#define DME_LP_FORM_FAST          0
+#define DME_LP_FORM_EXACT_STAGES  1
+int dme_lp_set_form(dme_ctx *ctx, int form);
+
+/* After a fast-form estimate of pic_num: its level-0 integer vectors (which
+ * 0) or its sub-pel vectors before mode decision merged them (which 1), one
+ * array of ynum_blocks * xnum_blocks per reference (NULL skips). */
+int dme_lp_get_vectors(dme_ctx *ctx, int pic_num, int which, dme_mv *out_ref1, dme_mv *out_ref2);
 
 /* Blocks refined and blocks that took the fallback search since the
  * context was created. */

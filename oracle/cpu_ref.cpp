@@ -347,6 +347,10 @@ inline int up_value(const Plane &up, const UpSample &s, int dy, int dx) {
 // the candidate's total beats best.total.  MMX semantics: in-bounds total =
 // float(int SAD) + start, one rounding (me_utils_mmx.cpp:270-278); the
 // bounds path (and the plain C build everywhere) adds per pixel in float.
+// fp_sem FP_ONE_ROUNDING: float(int SAD) + start for every block (the fast
+// form's own searches, which no reference path constrains).
+constexpr int FP_ONE_ROUNDING = 2;
+
 bool up_diff_update(const Plane &pic, const Plane &up, const Blk &b, const MV &mv, int step,
                     float mvcost, float lambda, int fp_sem, Cost &best, MV &best_mv,
                     double &terms) {
@@ -355,7 +359,7 @@ bool up_diff_update(const Plane &pic, const Plane &up, const Blk &b, const MV &m
     const float start_val = mvcost * lambda;
     terms += (double)b.xl * b.yl;
     float sum;
-    if (!s.bounds && fp_sem == DME_FP_MMX) {
+    if ((!s.bounds && fp_sem == DME_FP_MMX) || fp_sem == FP_ONE_ROUNDING) {
         int isad = 0;
         for (int dy = 0; dy < b.yl; ++dy)
             for (int dx = 0; dx < b.xl; ++dx)
@@ -648,10 +652,14 @@ void level_lambda(const MEData &base, int level, float wt, int xn, int yn, std::
 // (block_match.cpp:327-467)
 // ---------------------------------------------------------------------------
 
+// pred_from non-NULL (the fast form): the predictor of every block is taken
+// from that field (its integer vectors scaled to 1/2^prec pel) instead of the
+// vectors refined so far, so that every block is independent.
 void subpel_ref(const Plane &pic, const Plane &up, const Geo &g, int prec, int fp_sem, MEData &me,
-                int r, double &terms) {
+                int r, double &terms, const std::vector<MV> *pred_from = nullptr) {
     const int X = me.xnb, Y = me.ynb;
     std::vector<MV> &V = me.vec[r];
+    const std::vector<MV> &PV = pred_from ? *pred_from : V;
     std::vector<Cost> &C = me.pc[r];
     const OLB &bp = g.l[2];
     // neighbour order of m_nshift (me_subpel.cpp:46-63)
@@ -662,7 +670,7 @@ void subpel_ref(const Plane &pic, const Plane &up, const Geo &g, int prec, int f
             int n = 0;
             for (int k = 0; k < 4; ++k) {
                 const int xx = x + nsx[k], yy = y + nsy[k];
-                if (xx >= 0 && yy >= 0 && xx < X && yy < Y) nb[n++] = V[me.idx(yy, xx)];
+                if (xx >= 0 && yy >= 0 && xx < X && yy < Y) nb[n++] = PV[me.idx(yy, xx)];
             }
             const MV pred = mv_median_list(nb, n);
             const float lambda = me.lam[me.idx(y, x)];
@@ -737,6 +745,9 @@ struct ModeCtx {
     int nrefs, step, fp_sem;
     MEData *lv[3];  // level 0, 1 (temporary) and 2 (the picture's)
     double *terms;
+    // the fast form: level-1/0 predictors from this (pre-merge) field instead
+    // of the level-2 vectors merged so far
+    const std::vector<MV> *pred_v2[2] = {nullptr, nullptr};
 };
 
 // DoME (me_mode_decn.cpp:279-364) with FindBestMatchSubp
@@ -758,7 +769,7 @@ void mode_do_me_ref(ModeCtx &m, int xpos, int ypos, int level, int r) {
             if (!dup) cands[nc++] = c;
         }
     MV pred;
-    const std::vector<MV> &V2 = b2.vec[r];
+    const std::vector<MV> &V2 = m.pred_v2[r] ? *m.pred_v2[r] : b2.vec[r];
     if (xb > 0 && yb > 0)
         pred = mv_median3(V2[b2.idx(yb, xb - 1)], V2[b2.idx(yb - 1, xb - 1)], V2[b2.idx(yb - 1, xb)]);
     else if (xb == 0 && yb > 0)
@@ -1152,6 +1163,136 @@ void mc_component(const OLB &bp, int prec, bool mmx, int pw, int ph, int shift, 
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The fast form of the learned path (no counterpart in the reference; the
+// build's own algorithm, restated here so that its GPU kernels are pinned
+// bit-exactly given the same level-0 integer vectors).  Sub-pel refinement
+// and mode decision run with predictors that do not depend on decisions
+// made in the same pass (the integer vectors for the sub-pel search, the
+// pre-merge level-2 vectors for the level-1/0 units), so every block and
+// superblock is independent.  Then the reference's Lagrangian is evaluated
+// on the final vectors with the reference's own predictors:
+//  - level-2 block costs (the PredCosts SubpelRefine leaves,
+//    block_match.cpp:327-467): SAD of the final vector, mvcost =
+//    GetVarUp(GetPred of the final field, mv) (me_subpel.cpp:145-176), total
+//    = SAD + lambda mvcost (SetTotal, motion.h:201); a block with no samples
+//    costs {0, 0, 0} (the reference's result for it, whatever its vector);
+//  - a superblock split into level-1 or level-0 units: each unit's costs
+//    with the predictor DoME forms (me_mode_decn.cpp:308-321) from the
+//    level-2 vectors as the raster pass of DoModeDecn sees them: the final
+//    (merged) vectors of earlier superblocks, the pre-merge ones inside the
+//    superblock; then DoUnitDecn (me_mode_decn.cpp:368-462) and the sum.
+// ---------------------------------------------------------------------------
+
+// GetPred (me_subpel.cpp:145-176) at block (x, y) of field V.
+MV subpel_pred(const std::vector<MV> &V, int X, int Y, int x, int y) {
+    const int nsx[4] = {-1, -1, 0, 1}, nsy[4] = {0, -1, -1, -1};
+    MV nb[4];
+    int n = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int xx = x + nsx[k], yy = y + nsy[k];
+        if (xx >= 0 && yy >= 0 && xx < X && yy < Y) nb[n++] = V[(size_t)yy * X + xx];
+    }
+    return mv_median_list(nb, n);
+}
+
+float getvarup(const MV &pred, const MV &m) {
+    const int a = norm1(MV(m.x - pred.x, m.y - pred.y));
+    return (float)(int16_t)std::min(a, norm1(m));
+}
+
+// Exact level-2 costs of the final sub-pel vectors of reference r.  A final
+// vector equal to the scaled integer vector keeps the pel search's SAD
+// (RefineMatchSubp starts from it and no candidate replaced it), whose reads
+// clamp in the picture rather than in the up-converted plane; v0/pel_sad are
+// the integer vectors and their SADs.
+void fast_l2_costs(const Plane &pic, const Plane &up, const Geo &g, int prec, MEData &me, int r,
+                   const std::vector<MV> &v0, const std::vector<float> &pel_sad, double &terms) {
+    const int X = me.xnb, Y = me.ynb;
+    for (int y = 0; y < Y; ++y)
+        for (int x = 0; x < X; ++x) {
+            const size_t k = me.idx(y, x);
+            const Blk b = block_limits(g.l[2], pic.w, pic.h, x, y);
+            Cost c;
+            if (b.empty()) {
+                c.SAD = c.mvcost = c.total = 0.f;
+            } else {
+                const MV mv = me.vec[r][k];
+                const bool at_pel = mv == MV(v0[k].x << prec, v0[k].y << prec);
+                c.SAD = at_pel ? pel_sad[k] : up_diff_plain(pic, up, b, mv, prec, terms);
+                c.mvcost = getvarup(subpel_pred(me.vec[r], X, Y, x, y), mv);
+                c.total = c.SAD + me.lam[k] * c.mvcost;
+            }
+            me.pc[r][k] = c;
+        }
+}
+
+// The exact cost of superblock (xs, ys) at its split level s (0 or 1), from
+// the unit vectors of level s, and the unit modes that go with it.
+void fast_eval_sb(ModeCtx &m, const std::vector<MV> *V2, int xs, int ys) {
+    MEData &b2 = *m.lv[2];
+    const size_t si = (size_t)ys * b2.xsb + xs;
+    const int s = b2.split[si];
+    if (s == 2) return;
+    const MEData &lv = *m.lv[s];
+    const float lf = m.g->level_factor[s];
+    // the level-2 vector field as DoModeDecn's raster pass sees it at (xs, ys)
+    auto field = [&](int r, int u, int v) -> MV {
+        const int sx = u >> 2, sy = v >> 2;
+        const bool earlier = sy < ys || (sy == ys && sx < xs);
+        return earlier ? b2.vec[r][b2.idx(v, u)] : V2[r][b2.idx(v, u)];
+    };
+    float sb_cost = 0.0f;
+    for (int j = ys << s; j < (ys + 1) << s; ++j)
+        for (int i = xs << s; i < (xs + 1) << s; ++i) {
+            const size_t k = lv.idx(j, i);
+            const int xb = i << (2 - s), yb = j << (2 - s);
+            const float lambda = lv.lam[k];
+            const Blk b = block_limits(m.g->l[s], m.pic->w, m.pic->h, i, j);
+            // the unit's vectors: a merge copied them to all its blocks
+            MV uv[2];
+            for (int r = 0; r < m.nrefs; ++r) uv[r] = b2.vec[r][b2.idx(yb, xb)];
+            Cost c[2];
+            for (int r = 0; r < m.nrefs; ++r) {
+                MV pred;
+                if (xb > 0 && yb > 0)
+                    pred = mv_median3(field(r, xb - 1, yb), field(r, xb - 1, yb - 1), field(r, xb, yb - 1));
+                else if (xb == 0 && yb > 0)
+                    pred = mv_mean(field(r, xb, yb - 1), field(r, xb + 1, yb - 1));
+                else if (xb > 0 && yb == 0)
+                    pred = mv_mean(field(r, xb - 1, yb), field(r, xb - 1, yb + 1));
+                else
+                    pred = MV(0, 0);
+                const MV v = uv[r];
+                c[r].SAD = c[r].mvcost = 0.f;
+                c[r].total = 100000000.0f;
+                MV dummy = v;
+                // (a unit with no samples keeps {0, 0, 1e8}: FindBestMatchSubp's
+                // start, from zero-initialised level-1/0 MEData)
+                up_diff_update(*m.pic, *m.up[r], b, v, m.step, getvarup(pred, v), lambda, DME_FP_MMX, c[r], dummy,
+                               *m.terms);
+            }
+            // DoUnitDecn at level s (no intra below level 2)
+            int mode = DME_REF1_ONLY;
+            float best = c[0].total * lf;
+            if (m.nrefs > 1) {
+                const float t2 = c[1].total * lf;
+                if (t2 < best) { mode = DME_REF2_ONLY; best = t2; }
+                Cost bc;
+                bc.mvcost = c[0].mvcost + c[1].mvcost;
+                bc.SAD = bi_diff(*m.pic, *m.up[0], *m.up[1], b, uv[0], uv[1], m.step, *m.terms);
+                bc.total = bc.SAD + lambda * bc.mvcost;
+                bc.total *= lf;
+                if (bc.total < best) { mode = DME_REF1AND2; best = bc.total; }
+            }
+            sb_cost += best;
+            for (int v = yb; v < (j + 1) << (2 - s); ++v)
+                for (int u = xb; u < (i + 1) << (2 - s); ++u) b2.mode[b2.idx(v, u)] = mode;
+        }
+    b2.sbc[si] = sb_cost;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1405,5 +1546,173 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
 
     export_medata(me, out);
     if (sad_terms) *sad_terms = terms;
+    return DME_OK;
+}
+
+// The fast form of the learned path from level-0 integer vectors v0[r]
+// (B each; blocks with no samples must hold (0, 0)): pel SADs, lambda map
+// (MEData::SetLambdaMap), the independent sub-pel refinement and mode
+// decision, the exact-cost pass and SetDC.  v2_out[r] (may be NULL) receive
+// the pre-merge sub-pel vectors.  Quarter- or half-pel precision, MMX
+// float semantics.
+extern "C" int oracle_fast_me(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
+                              const dme_mv *const *v0, dme_medata *out, dme_mv *const *v2_out, double *sad_terms) {
+    if (!cur || !cur->y || !refs || !v0 || nrefs < 1 || nrefs > 2) return fail("bad arguments");
+    if (p->mv_precision < 1 || p->fp_semantics != DME_FP_MMX || p->full_search)
+        return fail("the fast form runs half or quarter pel, MMX semantics, hierarchical search");
+    Geo g;
+    int rc = make_geo(p, cur->width, cur->height, g);
+    if (rc) return rc;
+    const int W = g.W, H = g.H, prec = p->mv_precision;
+    double terms = 0.0;
+    Plane pic = make_plane(cur->y, W, H);
+    Plane ref[2], up[2];
+    for (int r = 0; r < nrefs; ++r) {
+        if (!refs[r].y || refs[r].width != W || refs[r].height != H) return fail("reference size differs");
+        ref[r] = make_plane(refs[r].y, W, H);
+        upconvert(ref[r], up[r]);
+    }
+    MEData me;
+    me.init(g.xnb, g.ynb, g.xsb, g.ysb, nrefs);
+    const size_t B = (size_t)g.xnb * g.ynb;
+    for (int r = 0; r < nrefs; ++r)
+        for (int y = 0; y < g.ynb; ++y)
+            for (int x = 0; x < g.xnb; ++x) {
+                const size_t k = me.idx(y, x);
+                const Blk b = block_limits(g.l[2], W, H, x, y);
+                me.vec[r][k] = MV(v0[r][k].x, v0[r][k].y);
+                me.pc[r][k].SAD = b.empty() ? 10000000.0f : (float)pel_sad(pic, ref[r], b, me.vec[r][k]);
+                if (!b.empty()) terms += (double)b.xl * b.yl;
+            }
+    lambda_map(g.xnb, g.ynb, nrefs, me.vec[0].data(), nrefs > 1 ? me.vec[1].data() : nullptr, p->lambda,
+               me.lam.data());
+    std::vector<MV> V0[2];
+    std::vector<float> PS[2];
+    for (int r = 0; r < nrefs; ++r) {
+        V0[r] = me.vec[r];
+        PS[r].resize(B);
+        for (size_t k = 0; k < B; ++k) PS[r][k] = me.pc[r][k].SAD;
+    }
+    // independent sub-pel refinement: predictors from the integer vectors
+    for (int r = 0; r < nrefs; ++r) {
+        std::vector<MV> pf(B);
+        for (size_t k = 0; k < B; ++k) pf[k] = MV(me.vec[r][k].x << prec, me.vec[r][k].y << prec);
+        subpel_ref(pic, up[r], g, prec, FP_ONE_ROUNDING, me, r, terms, &pf);
+    }
+    std::vector<MV> V2[2];
+    for (int r = 0; r < nrefs; ++r) {
+        V2[r] = me.vec[r];
+        if (v2_out && v2_out[r])
+            for (size_t k = 0; k < B; ++k) { v2_out[r][k].x = V2[r][k].x; v2_out[r][k].y = V2[r][k].y; }
+        fast_l2_costs(pic, up[r], g, prec, me, r, V0[r], PS[r], terms);
+    }
+    MEData l0, l1;
+    l0.init(g.xnb / 4, g.ynb / 4, g.xsb, g.ysb, nrefs);
+    l1.init(g.xnb / 2, g.ynb / 2, g.xsb, g.ysb, nrefs);
+    level_lambda(me, 0, (float)(1.0 / g.level_factor[0]), l0.xnb, l0.ynb, l0.lam);
+    level_lambda(me, 1, (float)(1.0 / g.level_factor[1]), l1.xnb, l1.ynb, l1.lam);
+    ModeCtx m;
+    m.g = &g; m.pic = &pic; m.up[0] = &up[0]; m.up[1] = nrefs > 1 ? &up[1] : &up[0];
+    m.nrefs = nrefs; m.step = prec; m.fp_sem = FP_ONE_ROUNDING;
+    m.lv[0] = &l0; m.lv[1] = &l1; m.lv[2] = &me; m.terms = &terms;
+    m.pred_v2[0] = &V2[0];
+    m.pred_v2[1] = nrefs > 1 ? &V2[1] : nullptr;
+    for (int ys = 0; ys < g.ysb; ++ys)
+        for (int xs = 0; xs < g.xsb; ++xs) {
+            mode_level(m, xs, ys, 2);
+            mode_level(m, xs, ys, 1);
+            mode_level(m, xs, ys, 0);
+        }
+    for (int ys = 0; ys < g.ysb; ++ys)
+        for (int xs = 0; xs < g.xsb; ++xs) fast_eval_sb(m, V2, xs, ys);
+    set_dc(pic, g.l[2], me, 0);
+    if (cur->u && cur->v) {
+        Plane pu = make_plane(cur->u, g.CW, g.CH), pv = make_plane(cur->v, g.CW, g.CH);
+        set_dc(pu, g.c[2], me, 1);
+        set_dc(pv, g.c[2], me, 2);
+    }
+    export_medata(me, out);
+    if (sad_terms) *sad_terms = terms;
+    return DME_OK;
+}
+
+// The exact-cost pass alone, on any final decisions: v2[r] the pre-merge
+// sub-pel vectors, final_vec[r] / split / lambda_map the final MEData.
+// Writes the level-2 costs (pc[r], B each) and the superblock costs (sbc, S)
+// that the reference's Lagrangian gives those decisions; for the plain
+// search's own decisions they are its PredCosts and SBCosts.
+extern "C" int oracle_exact_costs(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs,
+                                  int nrefs, const dme_mv *const *v0, const dme_mv *const *v2,
+                                  const dme_mv *const *final_vec,
+                                  const int32_t *split, const float *lambda_map, dme_cost *const *pc, float *sbc,
+                                  int32_t *mode) {
+    Geo g;
+    int rc = make_geo(p, cur->width, cur->height, g);
+    if (rc) return rc;
+    const int W = g.W, H = g.H, prec = p->mv_precision == 0 ? 1 : p->mv_precision;
+    double terms = 0.0;
+    Plane pic = make_plane(cur->y, W, H);
+    Plane up[2];
+    for (int r = 0; r < nrefs; ++r) upconvert(make_plane(refs[r].y, W, H), up[r]);
+    MEData me;
+    me.init(g.xnb, g.ynb, g.xsb, g.ysb, nrefs);
+    const size_t B = (size_t)g.xnb * g.ynb, S = (size_t)g.xsb * g.ysb;
+    std::memcpy(me.lam.data(), lambda_map, sizeof(float) * B);
+    std::vector<MV> V2[2], V0[2];
+    std::vector<float> PS[2];
+    for (int r = 0; r < nrefs; ++r) {
+        const Plane rf = make_plane(refs[r].y, W, H);
+        me.vec[r].resize(B);
+        V2[r].resize(B);
+        V0[r].resize(B);
+        PS[r].resize(B);
+        for (size_t k = 0; k < B; ++k) {
+            V2[r][k] = MV(v2[r][k].x, v2[r][k].y);
+            V0[r][k] = MV(v0[r][k].x, v0[r][k].y);
+            me.vec[r][k] = V2[r][k];
+            const Blk b = block_limits(g.l[2], W, H, (int)(k % g.xnb), (int)(k / g.xnb));
+            PS[r][k] = b.empty() ? 10000000.0f : (float)pel_sad(pic, rf, b, V0[r][k]);
+        }
+        fast_l2_costs(pic, up[r], g, prec, me, r, V0[r], PS[r], terms);
+        for (size_t k = 0; k < B; ++k) {
+            pc[r][k].SAD = me.pc[r][k].SAD;
+            pc[r][k].mvcost = me.pc[r][k].mvcost;
+            pc[r][k].total = me.pc[r][k].total;
+            me.vec[r][k] = MV(final_vec[r][k].x, final_vec[r][k].y);
+        }
+    }
+    MEData l0, l1;
+    l0.init(g.xnb / 4, g.ynb / 4, g.xsb, g.ysb, nrefs);
+    l1.init(g.xnb / 2, g.ynb / 2, g.xsb, g.ysb, nrefs);
+    level_lambda(me, 0, (float)(1.0 / g.level_factor[0]), l0.xnb, l0.ynb, l0.lam);
+    level_lambda(me, 1, (float)(1.0 / g.level_factor[1]), l1.xnb, l1.ynb, l1.lam);
+    ModeCtx m;
+    m.g = &g; m.pic = &pic; m.up[0] = &up[0]; m.up[1] = nrefs > 1 ? &up[1] : &up[0];
+    m.nrefs = nrefs; m.step = prec; m.fp_sem = DME_FP_MMX;
+    m.lv[0] = &l0; m.lv[1] = &l1; m.lv[2] = &me; m.terms = &terms;
+    // level-2 superblock costs: DoLevelDecn(2) on the exact block costs
+    for (int ys = 0; ys < g.ysb; ++ys)
+        for (int xs = 0; xs < g.xsb; ++xs) {
+            const size_t si = (size_t)ys * g.xsb + xs;
+            std::vector<MV> keep[2];
+            // DoUnitDecn(2) reads the level-2 vectors for the bi-prediction: the pre-merge ones
+            for (int r = 0; r < nrefs; ++r)
+                for (int v = 4 * ys; v < 4 * ys + 4; ++v)
+                    for (int u = 4 * xs; u < 4 * xs + 4; ++u) {
+                        keep[r].push_back(me.vec[r][me.idx(v, u)]);
+                        me.vec[r][me.idx(v, u)] = V2[r][me.idx(v, u)];
+                    }
+            mode_level(m, xs, ys, 2);
+            for (int r = 0; r < nrefs; ++r) {
+                size_t q = 0;
+                for (int v = 4 * ys; v < 4 * ys + 4; ++v)
+                    for (int u = 4 * xs; u < 4 * xs + 4; ++u) me.vec[r][me.idx(v, u)] = keep[r][q++];
+            }
+            me.split[si] = split[si];
+        }
+    for (int ys = 0; ys < g.ysb; ++ys)
+        for (int xs = 0; xs < g.xsb; ++xs) fast_eval_sb(m, V2, xs, ys);
+    std::memcpy(sbc, me.sbc.data(), sizeof(float) * S);
+    if (mode) for (size_t k = 0; k < B; ++k) mode[k] = me.mode[k];
     return DME_OK;
 }

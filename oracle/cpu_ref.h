@@ -75,6 +75,16 @@ int oracle_motion_compensate(const dme_params *p, int width, int height,
                              const dme_medata *me, int direction,
                              int16_t *const planes[3], const int widths[3],
                              const int heights[3]);
+/* The fast form of the learned path (the build's own algorithm): from the
+ * level-0 integer vectors v0[r] (blocks with no samples hold (0, 0)), the
+ * lambda map, sub-pel refinement and mode decision with predictors that do
+ * not depend on decisions of the same pass, then the reference Lagrangian
+ * of the final vectors with the reference's predictors, and SetDC.
+ * v2_out[r] (may be NULL) receive the pre-merge sub-pel vectors.          */
+int oracle_fast_me(const dme_params *p, const oracle_picture *cur,
+                   const oracle_picture *refs, int nrefs,
+                   const dme_mv *const *v0, dme_medata *out,
+                   dme_mv *const *v2_out, double *sad_terms);
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus

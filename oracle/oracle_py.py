@@ -158,6 +158,70 @@ def run_oracle(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False,
     return out, cap, geom
 
 
+def run_oracle_fast(params, cur, refs, v0, u=None, v=None):
+    """The fast form of the learned path from level-0 integer vectors v0
+    (one MV_DTYPE array per reference): returns (MEData, pre-merge sub-pel
+    vectors per reference)."""
+    o = oracle()
+    if not hasattr(o, "_fast_bound"):
+        o.oracle_fast_me.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture), C.POINTER(oracle_picture),
+                                     C.c_int, C.c_void_p, C.POINTER(abi.dme_medata), C.c_void_p,
+                                     C.POINTER(C.c_double)]
+        o._fast_bound = True
+    cur = np.ascontiguousarray(cur, np.int16)
+    refs = [np.ascontiguousarray(r, np.int16) for r in refs]
+    u = None if u is None else np.ascontiguousarray(u, np.int16)
+    v = None if v is None else np.ascontiguousarray(v, np.int16)
+    H, W = cur.shape
+    geom = geometry(params, W, H)
+    out = MEData(geom)
+    nrefs = len(refs)
+    v0 = [np.ascontiguousarray(a, abi.MV_DTYPE) for a in v0]
+    v2 = [np.zeros_like(v0[0]) for _ in range(nrefs)]
+    cp = _pic(1, cur, u, v)
+    nums = [0, 2] if nrefs == 2 else [0]
+    rp = (oracle_picture * 2)(*[_pic(nums[i], refs[i]) for i in range(nrefs)])
+    pv0 = (C.c_void_p * 2)(*([a.ctypes.data for a in v0] + [None] * (2 - nrefs)))
+    pv2 = (C.c_void_p * 2)(*([a.ctypes.data for a in v2] + [None] * (2 - nrefs)))
+    terms = C.c_double(0)
+    ms = out.struct()
+    _chk(o.oracle_fast_me(C.byref(params), C.byref(cp), rp, nrefs, pv0, C.byref(ms), pv2, C.byref(terms)))
+    out.sad_terms = terms.value
+    return out, v2
+
+
+def exact_costs(params, cur, refs, v0, v2, final):
+    """The reference Lagrangian of final decisions (oracle_exact_costs):
+    v0 = integer (pel-search) vectors and v2 = pre-merge sub-pel vectors per
+    reference, final = the final MEData
+    (vectors, sb_split, lambda_map).  Returns (level-2 costs per reference,
+    superblock costs, modes)."""
+    o = oracle()
+    if not hasattr(o, "_exact_bound"):
+        o.oracle_exact_costs.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture),
+                                         C.POINTER(oracle_picture), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        o._exact_bound = True
+    cur = np.ascontiguousarray(cur, np.int16)
+    refs = [np.ascontiguousarray(r, np.int16) for r in refs]
+    nrefs = len(refs)
+    v2 = [np.ascontiguousarray(a, abi.MV_DTYPE) for a in v2]
+    v0 = [np.ascontiguousarray(a, abi.MV_DTYPE) for a in v0]
+    fv = [np.ascontiguousarray(final.vectors[r], abi.MV_DTYPE) for r in range(nrefs)]
+    split = np.ascontiguousarray(final.sb_split, np.int32)
+    lam = np.ascontiguousarray(final.lambda_map, np.float32)
+    pc = [np.zeros(v2[0].shape, abi.COST_DTYPE) for _ in range(nrefs)]
+    sbc = np.zeros(split.shape, np.float32)
+    mode = np.zeros(v2[0].shape, np.int32)
+    cp = _pic(1, cur)
+    nums = [0, 2] if nrefs == 2 else [0]
+    rp = (oracle_picture * 2)(*[_pic(nums[i], refs[i]) for i in range(nrefs)])
+    arr = lambda xs: (C.c_void_p * 2)(*([a.ctypes.data for a in xs] + [None] * (2 - len(xs))))
+    _chk(o.oracle_exact_costs(C.byref(params), C.byref(cp), rp, nrefs, arr(v0), arr(v2), arr(fv), split.ctypes.data,
+                              lam.ctypes.data, arr(pc), sbc.ctypes.data, mode.ctypes.data))
+    return pc, sbc, mode
+
+
 def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, variant="mmx", pic_nums=None):
     rc, out, cap, geom = _run(ref(variant).refh_me, params, cur, refs, u, v, stop, capture, pic_nums=pic_nums)
     if rc != 0:

@@ -1,0 +1,103 @@
+"""GPU parity of the fast learned form (DME_LP_FORM_FAST).
+
+The network's level-0 integer vectors come from BF16 arithmetic on the GPU;
+everything after them is integer and float work that the oracle restates
+(oracle_fast_me).  Given the GPU's own level-0 vectors, every MEData array
+of the fast form must equal the oracle's bit for bit, and so must the
+pre-merge sub-pel vectors.  The cost gap against the plain search is
+asserted on held-out pictures."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import dirac_me
+from dirac_me import abi, learned, make_params, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _assert_same(want, got, nrefs, tag=""):
+    for k, v in want.arrays(nrefs).items():
+        g = got.arrays(nrefs)[k]
+        if v.tobytes() != g.tobytes():
+            n = int((v.view(np.uint8) != g.view(np.uint8)).sum())
+            raise AssertionError("%s %s differs (%d bytes)" % (tag, k, n))
+
+
+@pytest.fixture(scope="module")
+def lctx():
+    c = dirac_me.Context(0)
+    c.set_lp_weights(learned.load_weights())
+    yield c
+    c.close()
+
+
+def _fast(ctx, d, p, base):
+    nums = [base, base + 2][: len(d["refs"])]
+    for n, r in zip(nums, d["refs"]):
+        ctx.upload(n, r)
+    ctx.upload(base + 1, d["cur"], d["u"], d["v"])
+    H, W = d["cur"].shape
+    g = dirac_me.geometry(p, W, H)
+    got = dirac_me.MEData(g)
+    ctx.run_batch_learned([(base + 1, nums, p)], [got])
+    v0 = ctx.lp_vectors(base + 1, 0, g, len(nums))
+    v2 = ctx.lp_vectors(base + 1, 1, g, len(nums))
+    return got, v0, v2, nums
+
+
+@pytest.mark.parametrize("case", ["cif2", "cif1", "1080p", "720p_2416"])
+def test_fast_form_matches_oracle(lctx, case):
+    if case == "cif2":
+        d, p = synth.sized(23, "cif", 2), None
+    elif case == "cif1":
+        d, p = synth.config0(3), None
+    elif case == "1080p":
+        d, p = synth.config1(104), None
+    else:
+        d = synth.sized(71, "720p", 2)
+        p = make_params(24, 24, 16, 16, precision=2, depth=3, lam=10.0)
+    if p is None:
+        p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    print("fast form case", case, flush=True)
+    got, v0, v2, nums = _fast(lctx, d, p, 5000)
+    want, wv2 = O.run_oracle_fast(p, d["cur"], d["refs"], v0, d["u"], d["v"])
+    for r in range(len(nums)):
+        assert wv2[r].tobytes() == v2[r].tobytes(), "pre-merge sub-pel vectors of reference %d" % (r + 1)
+    _assert_same(want, got, len(nums), case)
+    for n in [5001] + nums:
+        lctx.drop(n)
+
+
+def test_fast_form_half_pel(lctx):
+    d = synth.sized(24, "cif", 2)
+    p = make_params(*d["block"], precision=1, depth=d["depth"], lam=10.0)
+    got, v0, v2, nums = _fast(lctx, d, p, 5100)
+    want, _ = O.run_oracle_fast(p, d["cur"], d["refs"], v0, d["u"], d["v"])
+    _assert_same(want, got, 2, "half pel")
+
+
+def test_fast_form_gap_on_held_out_pictures(lctx):
+    # mean SBCosts gap against the plain search <= 1% over 8 held-out 1080p
+    # pictures (the network was trained on seeds below 100000)
+    frames = [synth.config1(100 + s) for s in range(8)]
+    p = make_params(*frames[0]["block"], precision=2, depth=4, lam=10.0)
+    g = dirac_me.geometry(p, 1920, 1080)
+    jobs = []
+    for j, d in enumerate(frames):
+        base = 5200 + 3 * j
+        lctx.upload(base, d["refs"][0])
+        lctx.upload(base + 2, d["refs"][1])
+        lctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        jobs.append((base + 1, [base, base + 2], p))
+    plain = [dirac_me.MEData(g) for _ in jobs]
+    fast = [dirac_me.MEData(g) for _ in jobs]
+    lctx.run_batch(jobs, plain)
+    lctx.run_batch_learned(jobs, fast)
+    gaps = [float(np.sum(f.sb_costs, dtype=np.float64) / np.sum(q.sb_costs, dtype=np.float64) - 1.0)
+            for q, f in zip(plain, fast)]
+    print("gaps", ["%.4f" % x for x in gaps], flush=True)
+    assert np.mean(gaps) <= 0.01
+    for pn, nums, _ in jobs:
+        for n in [pn] + nums:
+            lctx.drop(n)
