@@ -16,6 +16,12 @@
 #else
 #define DME_G
 #endif
+// helpers shared by host and device code
+#if defined(__HIPCC__)
+#define DME_HD __host__ __device__
+#else
+#define DME_HD
+#endif
 
 namespace dme {
 
@@ -38,6 +44,8 @@ struct JobDesc {
     const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
     const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
     const uint8_t DME_G *pic8, *ref8[2];              // byte planes of picture and refs (k_bytes): sample (0, 0)
+    const uint8_t DME_G *up8[2];                      // byte up-planes of the refs (fast form): up sample (0, 0)
+    int u8p;                                          // their row pitch in bytes
     int p8;                                           // their row pitch in bytes
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
     int lp_off;                                       // first position of the picture in the proposal tensors
@@ -77,6 +85,16 @@ struct JobDesc {
 
 // Replicated samples on every side of a byte plane (k_bytes).
 constexpr int kB8Margin = 32;
+
+// Byte up-planes of the fast form (k_build_qp<true>): the 2W x 2H
+// up-converted reference as biased bytes with a replicated margin of
+// kUp8Margin samples; made by tiles of 64 x 16 positions (128 x 32 up
+// samples), so the plane is whole tiles across and down.
+constexpr int kUp8Margin = 64;
+DME_HD inline int up8_tiles_x(int W) { return (W + kUp8Margin + 63) / 64; }
+DME_HD inline int up8_tiles_y(int H) { return (H + kUp8Margin + 15) / 16; }
+DME_HD inline int up8_pitch(int W) { return 128 * up8_tiles_x(W); }
+DME_HD inline int up8_rows(int H) { return 32 * up8_tiles_y(H); }
 
 // Per superblock of the fast form: for each of up to four units of its
 // split level, the integer SADs of its two vectors and of their

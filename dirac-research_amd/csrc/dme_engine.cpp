@@ -43,7 +43,7 @@ hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, i
                            int max_blen_y, hipStream_t st);
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
                               int max_blen_y, hipStream_t st);
-hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, bool up8, hipStream_t st);
 hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st);
 hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks, int total, uint16_t *feat, float *lab,
                               hipStream_t st);
@@ -66,7 +66,7 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
                               int max_x1, int max_y1, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
                                 hipStream_t st);
-hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, hipStream_t st);
 hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st);
 hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_width,
                            hipStream_t st);
@@ -148,6 +148,9 @@ struct Picture {
     DevBuf b8;      // byte plane with margins (k_bytes)
     int bp = 0;     // its row pitch
     bool b8_ready = false;
+    DevBuf u8;      // byte up-plane with margins (the learned path's fast form)
+    bool u8_ready = false;
+    const uint8_t *u8_origin() const { return u8.as<uint8_t>() + (size_t)kUp8Margin * up8_pitch(W) + kUp8Margin; }
     const uint8_t *b8_origin() const { return b8.as<uint8_t>() + (size_t)kB8Margin * bp + kB8Margin; }
 };
 
@@ -163,6 +166,7 @@ struct JobState {
     DevBuf lwork;  // transition maps of the device lambda stage
     DevBuf fv0[2], fv2[2], fs2[2], fsb;  // the learned path's fast form (learned jobs only)
     int bpos = 0;  // position in the current batch (device lambda flags)
+    bool fast = false;  // the learned path's fast form: byte up-planes instead of quarter-pel planes
     JobDesc desc;
 };
 
@@ -279,13 +283,13 @@ enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
     KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_FAST_SUBPEL, KK_FAST_L2, KK_FAST_MODE,
-    KK_FAST_EVAL, KK_COUNT
+    KK_FAST_EVAL, KK_BUILD_UP8, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
     "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full",
-    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval"};
+    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval", "k_build_up8"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -350,7 +354,7 @@ static int upload(dme_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
 // resolution) and up-converted planes (EncPicture::UpOrigData,
 // enc_picture.cpp:131-156) of a set of pictures, one launch per level.
 static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, const std::vector<Picture *> &ups,
-                        int &launches) {
+                        const std::vector<Picture *> &ups8, int &launches) {
     int maxl = 0;
     for (auto &kv : levels) maxl = std::max(maxl, kv.second);
     std::vector<PlaneTask> all;
@@ -396,6 +400,25 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         ranges.push_back(std::make_pair(first, off));
     }
     {
+        // byte up-planes of the references of fast-form jobs
+        const size_t first = all.size();
+        int off = 0;
+        for (Picture *P : ups8) {
+            if (P->u8_ready) continue;
+            DME_TRY(P->u8.ensure((size_t)up8_pitch(P->W) * up8_rows(P->H)));
+            PlaneTask t;
+            t.in = P->y;
+            t.out = P->u8.as<int16_t>();
+            t.in_w = P->W;
+            t.ow = P->W;
+            t.oh = P->H;
+            t.offset = off;
+            off += up8_tiles_x(P->W) * up8_tiles_y(P->H);
+            all.push_back(t);
+        }
+        ranges.push_back(std::make_pair(first, off));
+    }
+    {
         // byte planes of every picture of the batch
         const size_t first = all.size();
         int off = 0;
@@ -427,19 +450,25 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         DME_TRY(timed_launch(ctx, KK_DOWNCONVERT, launches,
                              [&] { return launch_downconvert(dp + rg.first, n, rg.second, ctx->L->stream); }));
     }
-    const auto &rq = ranges[ranges.size() - 2], &rb = ranges.back();
+    const auto &rq = ranges[ranges.size() - 3], &ru = ranges[ranges.size() - 2], &rb = ranges.back();
     if (rb.second) {
         const int n = (int)(all.size() - rb.first);
         DME_TRY(timed_launch(ctx, KK_BYTES, launches,
                              [&] { return launch_bytes(dp + rb.first, n, rb.second, ctx->L->stream); }));
     }
     if (rq.second) {
-        const int n = (int)(rb.first - rq.first);
+        const int n = (int)(ru.first - rq.first);
         DME_TRY(timed_launch(ctx, KK_BUILD_QP, launches,
-                             [&] { return launch_build_qp(dp + rq.first, n, rq.second, ctx->L->stream); }));
+                             [&] { return launch_build_qp(dp + rq.first, n, rq.second, false, ctx->L->stream); }));
+    }
+    if (ru.second) {
+        const int n = (int)(rb.first - ru.first);
+        DME_TRY(timed_launch(ctx, KK_BUILD_UP8, launches,
+                             [&] { return launch_build_qp(dp + ru.first, n, ru.second, true, ctx->L->stream); }));
     }
     for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
     for (Picture *P : ups) P->qp_ready = true;
+    for (Picture *P : ups8) P->u8_ready = true;
     for (auto &kv : levels) kv.first->b8_ready = true;
     return DME_OK;
 }
@@ -503,6 +532,7 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     J.refs[1] = nrefs > 1 ? refs[1] : refs[0];
     J.p = p;
     J.stage = 0;
+    J.fast = learned && ctx->lp_form == DME_LP_FORM_FAST;
     const size_t B = (size_t)g.xnb * g.ynb, S = (size_t)g.xsb * g.ysb;
     for (int r = 0; r < 2; ++r) {
         DME_TRY(J.vec[r].ensure(sizeof(dme_mv) * B));
@@ -613,7 +643,8 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
 // The pictures a set of jobs reads: pyramid depth needed per picture, and the
 // references (whose quarter-pel planes the sub-pel and mode stages read).
 static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool need_up,
-                          std::map<Picture *, int> &levels, std::vector<Picture *> &ups) {
+                          std::map<Picture *, int> &levels, std::vector<Picture *> &ups,
+                          std::vector<Picture *> &ups8) {
     for (auto *j : js) {
         Picture *P = find_pic(ctx, j->pic_num);
         if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
@@ -622,7 +653,8 @@ static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool 
             Picture *R = find_pic(ctx, j->refs[r]);
             if (!R) return set_error(DME_ERR_STATE, "reference was dropped");
             levels[R] = std::max(levels[R], j->g.depth);
-            if (need_up && std::find(ups.begin(), ups.end(), R) == ups.end()) ups.push_back(R);
+            std::vector<Picture *> &u = j->fast ? ups8 : ups;
+            if (need_up && std::find(u.begin(), u.end(), R) == u.end()) u.push_back(R);
         }
     }
     return DME_OK;
@@ -633,10 +665,10 @@ static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool 
 // another lane reads.
 static int build_shared_planes(dme_ctx *ctx, const std::vector<std::vector<JobState *>> &parts, int &launches) {
     std::vector<std::map<Picture *, int>> lv(parts.size());
-    std::vector<std::vector<Picture *>> up(parts.size());
+    std::vector<std::vector<Picture *>> up(parts.size()), up8(parts.size());
     std::map<Picture *, int> seen, shared;
     for (size_t h = 0; h < parts.size(); ++h) {
-        DME_TRY(collect_planes(ctx, parts[h], true, lv[h], up[h]));
+        DME_TRY(collect_planes(ctx, parts[h], true, lv[h], up[h], up8[h]));
         for (auto &kv : lv[h]) {
             auto it = seen.find(kv.first);
             if (it == seen.end()) {
@@ -649,18 +681,21 @@ static int build_shared_planes(dme_ctx *ctx, const std::vector<std::vector<JobSt
     }
     if (shared.empty()) return DME_OK;
     for (auto &kv : shared) kv.second = seen[kv.first];
-    std::vector<Picture *> sup;
+    std::vector<Picture *> sup, sup8;
     for (const auto &u : up)
         for (Picture *P : u)
             if (shared.count(P) && std::find(sup.begin(), sup.end(), P) == sup.end()) sup.push_back(P);
-    return build_planes(ctx, shared, sup, launches);
+    for (const auto &u : up8)
+        for (Picture *P : u)
+            if (shared.count(P) && std::find(sup8.begin(), sup8.end(), P) == sup8.end()) sup8.push_back(P);
+    return build_planes(ctx, shared, sup, sup8, launches);
 }
 
 static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, int &launches) {
     std::map<Picture *, int> levels;
-    std::vector<Picture *> ups;
-    DME_TRY(collect_planes(ctx, js, need_up, levels, ups));
-    DME_TRY(build_planes(ctx, levels, ups, launches));
+    std::vector<Picture *> ups, ups8;
+    DME_TRY(collect_planes(ctx, js, need_up, levels, ups, ups8));
+    DME_TRY(build_planes(ctx, levels, ups, ups8, launches));
     for (auto *j : js) {
         Picture *P = find_pic(ctx, j->pic_num);
         j->desc.pic[0] = P->y;
@@ -672,7 +707,11 @@ static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, 
             j->desc.ref[r][0] = R->y;
             for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
             j->desc.ref8[r] = R->b8_origin();
-            if (need_up) j->desc.qp[r] = R->qp.as<uint32_t>();
+            if (need_up && !j->fast) j->desc.qp[r] = R->qp.as<uint32_t>();
+            if (need_up && j->fast) {
+                j->desc.up8[r] = R->u8_origin();
+                j->desc.u8p = up8_pitch(R->W);
+            }
         }
     }
     return DME_OK;
@@ -964,7 +1003,7 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
-                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, false, st); }));
     int mx0 = 1, my0 = 1, mx1 = 1, my1 = 1;
     for (auto *j : js) {
         mx0 = std::max(mx0, j->g.blen[0][0]);
@@ -1526,7 +1565,7 @@ extern "C" int dme_motion_estimate_noalloc(dme_ctx *ctx, const dme_params *param
         P.u = i == 0 ? d_u : nullptr;
         P.v = i == 0 ? d_v : nullptr;
         P.pyr_levels = 0;
-        P.up_ready = P.qp_ready = P.mcp_ready = P.b8_ready = false;
+        P.up_ready = P.qp_ready = P.mcp_ready = P.b8_ready = P.u8_ready = false;
         P.bp = 0;
     }
     NoallocScope scope(ctx);
@@ -1923,9 +1962,9 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
         src = P->pyr[level].p;
         n = (size_t)(P->W >> level) * (P->H >> level);
     } else if (kind == 1) {
-        // the search reads the quarter-pel planes only; the up-converted
-        // plane itself is built here, on request
-        if (!P->qp_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
+        // the search reads the quarter-pel or byte up-planes only; the
+        // up-converted plane itself is built here, on request
+        if (!P->qp_ready && !P->u8_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
         if (!P->up_ready) {
             HIP_TRY(hipStreamSynchronize(ctx->L->stream));
             DME_TRY(P->up.ensure(sizeof(int16_t) * 4 * (size_t)P->W * P->H));
@@ -1944,8 +1983,17 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
         }
         src = P->up.p;
         n = 4 * (size_t)P->W * P->H;
+    } else if (kind == 2) {
+        if (!P->u8_ready) return set_error(DME_ERR_STATE, "byte up-plane not built");
+        n = (size_t)up8_pitch(P->W) * up8_rows(P->H);
+        if (out_elems < n) return set_error(DME_ERR_INVALID, "output buffer too small");
+        std::vector<uint8_t> b(n);
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        HIP_TRY(hipMemcpy(b.data(), P->u8.p, n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) out[i] = (int16_t)((int)b[i] - 128);
+        return DME_OK;
     } else {
-        return set_error(DME_ERR_INVALID, "kind must be 0 or 1");
+        return set_error(DME_ERR_INVALID, "kind must be 0, 1 or 2");
     }
     if (out_elems < n) return set_error(DME_ERR_INVALID, "output buffer too small");
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
@@ -1973,6 +2021,7 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) try {
             kv.second->mcp_ready = false;
             kv.second->qp_ready = false;
             kv.second->b8_ready = false;
+            kv.second->u8_ready = false;
         }
     return DME_OK;
 } catch (...) {
@@ -2256,7 +2305,7 @@ static int stage_mode_fast(dme_ctx *ctx, std::vector<JobState *> &js, int &launc
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
-                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, st); }));
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, true, st); }));
     DME_TRY(timed_launch(ctx, KK_FAST_MODE, launches,
                          [&] { return launch_fast_mode(dj, dtask(ctx, rsb), rsb.n, rsb.total, ux, uy, st); }));
     DME_TRY(timed_launch(ctx, KK_FAST_EVAL, launches,

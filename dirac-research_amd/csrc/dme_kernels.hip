@@ -474,6 +474,11 @@ __global__ __launch_bounds__(256) void k_check_range(const int16_t *__restrict__
 #define QT_VR (QT_Y + 2)       // vertical-tap rows Y0 - 1 .. Y0 + QT_Y
 #define QT_UR (2 * QT_Y + 1)   // up rows 2Y0 .. 2Y0 + 2 QT_Y
 #define QT_UC (2 * QT_X + 1)   // up columns 2X0 .. 2X0 + 2 QT_X
+// UP8: the same tiles make the byte up-plane instead (up8_* below): biased
+// up samples v + 128 of up rows and columns -64 .. 2 H + 63 / 2 W + 63 (a
+// replicated margin of 64: the reference's clamped reads for those), rows of
+// up8_pitch(W) bytes.
+template <bool UP8>
 __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ pt, int npt, int total) {
     __shared__ int16_t s_src[QT_SR][QT_SC];
     __shared__ int16_t s_v[QT_VR][QT_SC];
@@ -488,13 +493,20 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
     }
     const PlaneTask p = pt[lo];
     const int W = p.ow, H = p.oh, QW = W + 2, QH = H + 2;
-    const int tx = (QW + QT_X - 1) / QT_X, local = tile - p.offset;
-    const int X0 = (local % tx) * QT_X - 1, Y0 = (local / tx) * QT_Y - 1;  // first position of the tile
+    const int tx = UP8 ? up8_tiles_x(W) : (QW + QT_X - 1) / QT_X, local = tile - p.offset;
+    // first position of the tile
+    const int X0 = (local % tx) * QT_X - (UP8 ? kUp8Margin / 2 : 1);
+    const int Y0 = (local / tx) * QT_Y - (UP8 ? kUp8Margin / 2 : 1);
+    // origin of the source tile: the clamped coordinates of every position of
+    // the tile lie in SX0 .. SX0 + QT_X, SY0 .. SY0 + QT_Y (for UP8 tiles in
+    // the margin the tile itself can lie outside the picture)
+    const int SX0 = UP8 ? clampi(X0, 0, max(W - QT_X, 0)) : X0;
+    const int SY0 = UP8 ? clampi(Y0, 0, max(H - QT_Y, 0)) : Y0;
     const int16_t *__restrict__ in = p.in;
     const int tid = threadIdx.x;
     for (int i = tid; i < QT_SR * QT_SC; i += 256) {
         const int r = i / QT_SC, c = i - r * QT_SC;
-        s_src[r][c] = in[(size_t)clampi(Y0 - 4 + r, 0, H - 1) * W + clampi(X0 - 4 + c, 0, W - 1)];
+        s_src[r][c] = in[(size_t)clampi(SY0 - 4 + r, 0, H - 1) * W + clampi(SX0 - 4 + c, 0, W - 1)];
     }
     __syncthreads();
     // vertical taps of source row Y0 - 1 + r: rows -3 .. +4 around it
@@ -512,7 +524,7 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
     // edges, where clamping changes parities).
     auto up_val = [&](int cuy, int cux) -> int {
         const int y = cuy >> 1, x = cux >> 1;
-        const int sr = y - (Y0 - 4), sc = x - (X0 - 4), vr = y - (Y0 - 1);
+        const int sr = y - (SY0 - 4), sc = x - (SX0 - 4), vr = y - (SY0 - 1);
         int s8[8];
         if (!(cuy & 1)) {
             if (!(cux & 1)) return s_src[sr][sc];
@@ -525,6 +537,7 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
         for (int t = 0; t < 8; ++t) s8[t] = s_v[vr][sc - 3 + t];
         return up_tap8(s8);
     };
+    // (an inner tile has SX0 == X0 and SY0 == Y0)
     const bool inner = 2 * X0 >= 0 && 2 * X0 + QT_UC <= 2 * W && 2 * Y0 >= 0 && 2 * Y0 + QT_UR <= 2 * H;
     if (inner) {
         // no up coordinate clamps: thread = (up row, segment of 8 column pairs); the
@@ -555,6 +568,18 @@ __global__ __launch_bounds__(256) void k_build_qp(const PlaneTask *__restrict__ 
     }
     __syncthreads();
     uint32_t *__restrict__ out = reinterpret_cast<uint32_t *>(p.out);
+    if (UP8) {
+        // up rows 2 Y0 .. 2 Y0 + 31, columns 2 X0 .. 2 X0 + 127: dword-aligned in the plane
+        const int pitch = up8_pitch(W);
+        uint32_t *o = out + ((size_t)(2 * Y0 + kUp8Margin) * pitch + 2 * X0 + kUp8Margin) / 4;
+        for (int i = tid; i < 2 * QT_Y * (2 * QT_X / 4); i += 256) {
+            const int a = i / (2 * QT_X / 4), c = 4 * (i - a * (2 * QT_X / 4));
+            o[(size_t)a * (pitch / 4) + c / 4] = (uint32_t)(s_u[a][c] + 128) | ((uint32_t)(s_u[a][c + 1] + 128) << 8) |
+                                                  ((uint32_t)(s_u[a][c + 2] + 128) << 16) |
+                                                  ((uint32_t)(s_u[a][c + 3] + 128) << 24);
+        }
+        return;
+    }
     const int px = tid & (QT_X - 1);
     const int X = X0 + px;
     for (int py = tid / QT_X; py < QT_Y; py += 256 / QT_X) {
@@ -2255,6 +2280,10 @@ __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict
 // Mode decision helpers
 // ---------------------------------------------------------------------------
 
+// The bi-prediction SAD of a block from the fast form's byte up-planes
+// (defined with them below).
+__device__ int up8_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2);
+
 // BiBlock{Half,Quarter}Pel::Diff (me_utils.cpp:1172-1543): sum of
 // |((2 pic - r1) - r2) >> 1| with short intermediates.
 template <int NB = 8>
@@ -2316,7 +2345,9 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 }
 
 // DoUnitDecn at level 2 (me_mode_decn.cpp:368-462), for every block in
-// parallel: it only reads the block's own refined vectors and costs.
+// parallel: it only reads the block's own refined vectors and costs.  UP8:
+// the references are byte up-planes (the learned path's fast form).
+template <bool UP8>
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
     const int g0 = (xcd_block() * 4 + wave_id()) * PW_MODE_L2, g1 = min(g0 + PW_MODE_L2, total);
@@ -2347,7 +2378,9 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             }
             dme_cost bc;
             bc.mvcost = c1.mvcost + c2.mvcost;
-            bc.SAD = b.empty() ? 0.0f : (float)wave_bi_sad(J, b, J.vec[0][k], J.vec[1][k]);
+            bc.SAD = b.empty() ? 0.0f
+                               : (float)(UP8 ? up8_bi_sad(J, b, J.vec[0][k], J.vec[1][k])
+                                             : wave_bi_sad(J, b, J.vec[0][k], J.vec[1][k]));
             bc.total = bc.SAD + loc_lambda * bc.mvcost;
             bc.total *= lf;
             uc = bc.total + mode_cost;
@@ -3250,6 +3283,64 @@ __device__ __forceinline__ uint32_t b8_dword(const uint8_t *__restrict__ origin,
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
 }
 
+// Byte up-planes (the fast form's references, k_build_qp<true>).  A plane
+// holds the reference's up-converted samples (UpConverter, upconvert.cpp:
+// 55-200) as biased bytes, its margin replicating the edge samples, so that
+// a read inside the plane is the reference's clamped read (BChk,
+// common.h:1574-1579).  A quarter-pel sample (BlockDiffQuarterPel /
+// BiBlockQuarterPel, me_utils.cpp:573-867, 1288-1543) at up position
+// (ux + rx / 2, uy + ry / 2) is a, (a + b + 1) >> 1, (a + c + 1) >> 1 or
+// (a + b + c + d + 2) >> 2 of the up samples a (ux, uy), b (ux + 1, uy),
+// c (ux, uy + 1) and d (ux + 1, uy + 1); the bias of 128 commutes with all
+// four.
+
+// Clamped read of up sample (ux, uy) of reference r (biased).
+__device__ __forceinline__ int up8_at(const JobDesc &J, int r, int ux, int uy) {
+    return J.up8[r][(ptrdiff_t)clampi(uy, 0, 2 * J.H - 1) * J.u8p + clampi(ux, 0, 2 * J.W - 1)];
+}
+
+// Quarter-pel sample (biased) at quarter-pel position (qx, qy), clamped reads.
+__device__ __forceinline__ int up8_sample(const JobDesc &J, int r, int qx, int qy) {
+    const int ux = qx >> 1, uy = qy >> 1, rx = qx & 1, ry = qy & 1;
+    const int a = up8_at(J, r, ux, uy), b = up8_at(J, r, ux + 1, uy);
+    const int c = up8_at(J, r, ux, uy + 1), d = up8_at(J, r, ux + 1, uy + 1);
+    return ((2 - rx) * (2 - ry) * a + rx * (2 - ry) * b + (2 - rx) * ry * c + rx * ry * d + 2) >> 2;
+}
+
+// Rounded averages of packed bytes: (a + b + 1) >> 1 (v_lerp_u8) and
+// (a + b + c + d + 2) >> 2 (in 16-bit lanes).
+__device__ __forceinline__ uint32_t avg2_u8(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, 0x01010101u); }
+__device__ __forceinline__ uint32_t avg4_u8(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    const uint32_t m = 0x00ff00ffu;
+    const uint32_t lo = (a & m) + (b & m) + (c & m) + (d & m) + 0x00020002u;
+    const uint32_t hi = ((a >> 8) & m) + ((b >> 8) & m) + ((c >> 8) & m) + ((d >> 8) & m) + 0x00020002u;
+    return ((lo >> 2) & m) | (((hi >> 2) & m) << 8);
+}
+
+// The even and the odd bytes of the 8 bytes from byte offset o of the dwords
+// at q (reads the three dwords from dword o / 4).  Offsets, not integer
+// casts of pointers, so that LDS reads stay LDS reads.
+__device__ __forceinline__ void bytes8(const uint32_t *__restrict__ q, uint32_t o, uint32_t &even, uint32_t &odd) {
+    const uint32_t *__restrict__ p = q + (o >> 2);
+    const uint32_t sh = o & 3u;
+    const uint32_t d0 = __builtin_amdgcn_alignbyte(p[1], p[0], sh), d1 = __builtin_amdgcn_alignbyte(p[2], p[1], sh);
+    even = __builtin_amdgcn_perm(d1, d0, 0x06040200u);
+    odd = __builtin_amdgcn_perm(d1, d0, 0x07050301u);
+}
+
+// Packed quarter-pel samples of four consecutive pixels of one phase (rx,
+// ry) whose up samples start at byte offset o of q (the next up row `pitch`
+// bytes further).
+__device__ __forceinline__ uint32_t quad_u8(const uint32_t *__restrict__ q, uint32_t o, uint32_t pitch, int rx,
+                                            int ry) {
+    uint32_t a, b;
+    bytes8(q, o, a, b);
+    if (!ry) return rx ? avg2_u8(a, b) : a;
+    uint32_t c, d;
+    bytes8(q, o + pitch, c, d);
+    return rx ? avg4_u8(a, b, c, d) : avg2_u8(a, c);
+}
+
 // Lane layout of a wave over a unit of xl x yl samples: dword column col
 // (four samples) of rows row0, row0 + rpp, ...; mask keeps the samples of a
 // partial last column.
@@ -3279,38 +3370,40 @@ __device__ __forceinline__ int unit_row(const UnitLanes &L, int p, int yl) {
     return L.on && row < yl ? row : -1;
 }
 
-// Packed quarter-pel samples of candidate c (quarter-pel vector) for lane
-// layout L at unit row `row`: byte fx of four consecutive dwords of phase
-// row fy (qp layout of k_build_qp), clamped reads.
-struct QpCand {
-    const uint32_t *qp;
-    int X0, Y0;
-    uint32_t sel;
-    int fy;
+// A quarter-pel candidate m of unit b against reference r's byte up-plane:
+// lane (col, row)'s four pixels start at up column 2 (xp + 4 col) + (m.x >> 1)
+// of up row 2 (yp + row) + (m.y >> 1).  `fast` when every read of the unit
+// lies inside the plane (its margin replicates the clamped samples);
+// otherwise the samples are read one by one with clamped coordinates.
+struct Up8Cand {
+    const uint32_t *plane;  // the plane's first dword (up row and column -kUp8Margin)
+    uint32_t o0;            // byte offset of the unit's first up sample
+    int rx, ry, qx0, qy0, r;
+    bool fast;
 };
 
-__device__ __forceinline__ QpCand qp_cand(const uint32_t *__restrict__ qp, const Blk &b, dme_mv c) {
-    QpCand q;
-    const int xq = 4 * b.xp + c.x, yq = 4 * b.yp + c.y;
-    q.qp = qp;
-    q.X0 = xq >> 2;
-    q.Y0 = yq >> 2;
-    const int fx = xq & 3;
-    q.fy = yq & 3;
-    q.sel = (uint32_t)fx | ((uint32_t)(fx + 4) << 8) | 0x0c0c0000u;
-    return q;
+__device__ __forceinline__ Up8Cand up8_cand(const JobDesc &J, int r, const Blk &b, int nd, dme_mv m) {
+    Up8Cand c;
+    c.r = r;
+    c.rx = m.x & 1;
+    c.ry = m.y & 1;
+    const int ux0 = 2 * b.xp + (m.x >> 1), uy0 = 2 * b.yp + (m.y >> 1);
+    c.plane = reinterpret_cast<const uint32_t *>(J.up8[r] - ((ptrdiff_t)kUp8Margin * J.u8p + kUp8Margin));
+    c.fast = ux0 >= -kUp8Margin && ux0 + 8 * nd + 12 <= J.u8p - kUp8Margin && uy0 >= -kUp8Margin &&
+             uy0 + 2 * b.yl + 2 <= up8_rows(J.H) - kUp8Margin;
+    c.o0 = c.fast ? (uint32_t)((uy0 + kUp8Margin) * J.u8p + ux0 + kUp8Margin) : 0u;
+    c.qx0 = 4 * b.xp + m.x;
+    c.qy0 = 4 * b.yp + m.y;
+    return c;
 }
 
-__device__ __forceinline__ uint32_t qp_pack(const QpCand &q, int W, int H, int col, int row) {
-    const uint32_t QW = (uint32_t)W + 2u;
-    // 32-bit offsets from the plane (a plane is far below 4 G dwords)
-    const uint32_t ro = (uint32_t)((clampi(q.Y0 + row, -1, H) + 1) * 4 + q.fy) * QW;
-    uint32_t d[4];
+__device__ __forceinline__ uint32_t up8_pack(const JobDesc &J, const Up8Cand &c, int col, int row) {
+    if (c.fast) return quad_u8(c.plane, c.o0 + (uint32_t)(2 * row * J.u8p + 8 * col), (uint32_t)J.u8p, c.rx, c.ry);
+    uint32_t v = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) d[i] = q.qp[ro + (uint32_t)(clampi(q.X0 + 4 * col + i, -1, W) + 1)];
-    const uint32_t lo = __builtin_amdgcn_perm(d[1], d[0], q.sel);
-    const uint32_t hi = __builtin_amdgcn_perm(d[3], d[2], q.sel);
-    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+    for (int i = 0; i < 4; ++i)
+        v |= (uint32_t)up8_sample(J, c.r, c.qx0 + 4 * (4 * col + i), c.qy0 + 4 * row) << (8 * i);
+    return v;
 }
 
 // Up to four wave-uniform candidates and their SADs, in registers.
@@ -3322,15 +3415,14 @@ struct Sad4 {
 };
 
 // Integer SADs of the first n (<= N) quarter-pel candidates of unit b
-// against the quarter-pel planes qp; the unit's packed picture dwords are
+// against reference r; the unit's packed picture dwords are
 // s_pic[row * pd + col].  Wave-uniform results.
 template <int N>
-__device__ __forceinline__ Sad4 qp_sads(const JobDesc &J, const uint32_t *__restrict__ qp, const Blk &b,
-                                        const UnitLanes &L, const uint32_t *__restrict__ s_pic, int pd, const Cand4 &c,
-                                        int n) {
-    QpCand q[N];
+__device__ __forceinline__ Sad4 up8_sads(const JobDesc &J, int r, const Blk &b, const UnitLanes &L,
+                                         const uint32_t *__restrict__ s_pic, int pd, const Cand4 &c, int n) {
+    Up8Cand q[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) q[i] = qp_cand(qp, b, c.c[i < n ? i : 0]);
+    for (int i = 0; i < N; ++i) q[i] = up8_cand(J, r, b, L.nd, c.c[i < n ? i : 0]);
     uint32_t acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = 0u;
@@ -3340,7 +3432,7 @@ __device__ __forceinline__ Sad4 qp_sads(const JobDesc &J, const uint32_t *__rest
         const uint32_t pv = s_pic[row * pd + L.col];
 #pragma unroll
         for (int i = 0; i < N; ++i)
-            if (i < n) acc[i] = __builtin_amdgcn_sad_u8(pv, qp_pack(q[i], J.W, J.H, L.col, row) & L.mask, acc[i]);
+            if (i < n) acc[i] = __builtin_amdgcn_sad_u8(pv, up8_pack(J, q[i], L.col, row) & L.mask, acc[i]);
     }
     Sad4 out;
 #pragma unroll
@@ -3350,24 +3442,99 @@ __device__ __forceinline__ Sad4 qp_sads(const JobDesc &J, const uint32_t *__rest
 
 // Bi-prediction SAD of unit b (BiBlock*Pel::Diff, me_utils.cpp:1172-1543):
 // |((2p - r1) - r2) >> 1| = |p - ceil((r1 + r2) / 2)|, a byte SAD against the
-// rounded-up average of the two candidates' biased samples.
-__device__ __forceinline__ int qp_bi_sad(const JobDesc &J, const Blk &b, const UnitLanes &L,
-                                      const uint32_t *__restrict__ s_pic, int pd, dme_mv c1, dme_mv c2) {
-    // the vectors are in 1/2^step pel, the planes in quarter pel
+// rounded-up average of the two candidates' biased samples.  Vectors in
+// 1/2^step pel.
+__device__ __forceinline__ int up8_bi_sad_s(const JobDesc &J, const Blk &b, const UnitLanes &L,
+                                            const uint32_t *__restrict__ s_pic, int pd, dme_mv c1, dme_mv c2) {
     const int qs = 2 - J.step;
     c1.x <<= qs;
     c1.y <<= qs;
     c2.x <<= qs;
     c2.y <<= qs;
-    const QpCand q1 = qp_cand(J.qp[0], b, c1), q2 = qp_cand(J.qp[1], b, c2);
+    const Up8Cand q1 = up8_cand(J, 0, b, L.nd, c1), q2 = up8_cand(J, 1, b, L.nd, c2);
     uint32_t acc = 0;
     for (int p = 0; p < L.npass; ++p) {
         const int row = unit_row(L, p, b.yl);
         if (row < 0) continue;
-        const uint32_t a = qp_pack(q1, J.W, J.H, L.col, row), d = qp_pack(q2, J.W, J.H, L.col, row);
+        const uint32_t a = up8_pack(J, q1, L.col, row), d = up8_pack(J, q2, L.col, row);
         acc = __builtin_amdgcn_sad_u8(s_pic[row * pd + L.col], avg_up_u8(a, d) & L.mask, acc);
     }
     return wave_sum((int)acc);
+}
+
+// The same for a level-2 block without a staged picture (k_mode_l2<true>).
+__device__ int up8_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
+    const UnitLanes L = unit_lanes(b);
+    const int qs = 2 - J.step;
+    m1.x <<= qs;
+    m1.y <<= qs;
+    m2.x <<= qs;
+    m2.y <<= qs;
+    const Up8Cand q1 = up8_cand(J, 0, b, L.nd, m1), q2 = up8_cand(J, 1, b, L.nd, m2);
+    uint32_t acc = 0;
+    for (int p = 0; p < L.npass; ++p) {
+        const int row = unit_row(L, p, b.yl);
+        if (row < 0) continue;
+        const uint32_t pv = b8_dword(J.pic8, J.p8, b.xp + 4 * L.col, b.yp + row) & L.mask;
+        const uint32_t a = up8_pack(J, q1, L.col, row), d = up8_pack(J, q2, L.col, row);
+        acc = __builtin_amdgcn_sad_u8(pv, avg_up_u8(a, d) & L.mask, acc);
+    }
+    return wave_sum((int)acc);
+}
+
+// The reference's per-sample float chain of a bounds-checked up-sampled Diff
+// (as wave_chain) from reference r's byte up-plane.
+__device__ __noinline__ float up8_chain(float start, const JobDesc *__restrict__ jp, int r, Blk b, UpStart st) {
+    const JobDesc &J = *jp;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int n = b.xl * b.yl;
+    const int16_t *__restrict__ pic = J.pic[0];
+    float s = start;
+    for (int b0 = 0; b0 < n; b0 += 4 * WAVE) {
+        int d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            const int ii = i < n ? i : n - 1;
+            const int dy = ii / b.xl, dx = ii - dy * b.xl;
+            const int q = up8_sample(J, r, 2 * st.rsx + st.rx + 4 * dx, 2 * st.rsy + st.ry + 4 * dy) - 128;
+            const int v = abs(q - (int)pic[(size_t)(b.yp + dy) * J.W + b.xp + dx]);
+            d[k] = i < n ? v : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (b0 + k * WAVE >= n) break;
+            int x = d[k];
+#pragma unroll
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const int y = __shfl_up(x, off, WAVE);
+                if (lane >= off) x += y;
+            }
+            const int total = __builtin_amdgcn_readlane(x, WAVE - 1);
+            int consumed = 0, first = 0;
+            while (first < WAVE) {
+                if (s >= 16777216.0f) {
+                    for (int j = first; j < WAVE; ++j)
+                        s += (float)(__builtin_amdgcn_readlane(x, j) - (j ? __builtin_amdgcn_readlane(x, j - 1) : 0));
+                    break;
+                }
+                const double D = (double)binade_limit(s) - (double)s;
+                const unsigned long long m = __ballot(lane >= first && (double)(x - consumed) >= D);
+                if (!m) {
+                    s += (float)(total - consumed);
+                    break;
+                }
+                const int j = __ffsll((long long)m) - 1;
+                const int before = j ? __builtin_amdgcn_readlane(x, j - 1) : 0;
+                const int at = __builtin_amdgcn_readlane(x, j);
+                s += (float)(before - consumed);
+                s += (float)(at - before);
+                consumed = at;
+                first = j + 1;
+            }
+        }
+    }
+    return s;
 }
 
 // Stages unit b's packed picture bytes (masked past xl) into s_pic rows of pd
@@ -3413,21 +3580,20 @@ __device__ __forceinline__ dme_mv subpel_pred(const dme_mv *__restrict__ V, int 
 // search (block_match.cpp:327-467) with the predictor of the integer
 // vectors.  One wave per (block, reference).  Every candidate of the staged
 // search lies within 3 quarter pels of 4 x the integer vector v, so the wave
-// first stages the quarter-pel samples of positions v - 1 .. v + len of all
-// 16 phases, transposed into 16 byte planes (phase (fx, fy) at plane
-// 4 fy + fx, rows of FS_PITCH bytes); a candidate's four samples of a lane are
-// then one unaligned dword of one plane.  The predictor, a multiple of four
-// quarter pels away, is read from the staged planes when it equals 4 v and
-// gathered from the quarter-pel planes otherwise.  Writes the integer
-// vectors (fv0), the sub-pel vectors (fv2) and the SAD each sub-pel vector's
-// cost is measured by (fs2): the pel search's SAD while the vector is the
-// scaled integer vector (RefineMatchSubp keeps that cost), else its
-// up-sampled SAD.
+// first copies the byte up-plane samples those candidates read -- up rows
+// 2 (yp + v.y) - 2 .. 2 (yp + v.y + yl) and columns from the dword at or
+// before 2 (xp + v.x) - 2 -- into LDS, rows of `pitch` dwords; a candidate's
+// four samples of a lane are then one quad_u8 of the staged rows.  The
+// predictor, a whole number of pels away, is staged when it equals 4 v and
+// read from the plane otherwise.  Writes the integer vectors (fv0), the
+// sub-pel vectors (fv2) and the SAD each sub-pel vector's cost is measured
+// by (fs2): the pel search's SAD while the vector is the scaled integer
+// vector (RefineMatchSubp keeps that cost), else its up-sampled SAD.
 struct FsLds {
-    int rows;   // staged rows per phase plane (block rows + 2)
-    int pitch;  // bytes per staged row (a multiple of 4)
+    int rows;   // staged up rows (2 by + 3)
+    int pitch;  // dwords per staged row (2 ceil(bx / 4) + 3, odd)
     int pd;     // dwords per picture row
-    int wave;   // dwords per wave: 16 planes, then the picture block
+    int wave;   // dwords per wave: the staged rows, then the picture block
 };
 
 // Sum over the wave of two non-negative values below 2^16 in one reduction.
@@ -3448,8 +3614,8 @@ __global__ __launch_bounds__(256) void k_fast_subpel(const JobDesc *__restrict__
     const int wv = wave_id(), lane = threadIdx.x & (WAVE - 1);
     const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;
-    uint32_t *s_ph = s_fs + wv * ly.wave;
-    uint32_t *s_pic = s_ph + 16 * ly.rows * (ly.pitch >> 2);
+    uint32_t *s_up = s_fs + wv * ly.wave;
+    uint32_t *s_pic = s_up + ly.rows * ly.pitch;
     const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, k = gi - tk.offset, X = J.xnb, Y = J.ynb, xb = k % X, yb = k / X;
@@ -3473,77 +3639,66 @@ __global__ __launch_bounds__(256) void k_fast_subpel(const JobDesc *__restrict__
             out_sad = (int)pel_sad;
         } else {
             const UnitLanes L = unit_lanes(b);
-            const uint32_t *__restrict__ qp = J.qp[r];
-            // stage: qp rows (Y + 1) * 4 + fy for Y = yp + v.y - 1 .. yp + v.y + yl,
-            // four positions per lane-item, transposed into the phase planes
+            // stage: up rows uy0 .. uy0 + nrow - 1, dwords from up column cx
+            const int ux0 = 2 * (b.xp + pel.x) - 2, uy0 = 2 * (b.yp + pel.y) - 2;
+            const int cx = ux0 & ~3, a0 = ux0 - cx, nrow = 2 * b.yl + 3, nd = ly.pitch;
             {
-                // a lane keeps one group of four positions, gx, across staged rows r0, r0 + rpp, ...
-                const int QW = W + 2, Y0 = b.yp + pel.y - 1, X0 = b.xp + pel.x - 1;
-                const int ng = ly.pitch >> 2, rpp = WAVE / ng, r0 = lane / ng, gx = lane - r0 * ng;
-                int xo[4];
+                const int rpp = WAVE / nd, r0 = lane / nd, c = lane - r0 * nd;
+                const bool inside = cx >= -kUp8Margin && cx + 4 * nd <= J.u8p - kUp8Margin && uy0 >= -kUp8Margin &&
+                                    uy0 + nrow <= up8_rows(H) - kUp8Margin;
+                if (inside) {
+                    // plane rows are whole dwords and the origin is dword-aligned
+                    const uint32_t *__restrict__ src =
+                        reinterpret_cast<const uint32_t *>(J.up8[r] + (ptrdiff_t)uy0 * J.u8p + cx) + c;
+                    const int dp = J.u8p >> 2;
+                    // loads of up to eight row passes in flight before any is stored
+                    for (int rb = r0 < rpp ? r0 : nrow; rb < nrow; rb += 8 * rpp) {
+                        uint32_t d[8];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * gx + i, -1, W) + 1;
-                // loads of up to eight passes in flight before any is used
-                const int nrows = 4 * ly.rows, first = r0 < rpp ? r0 : nrows;
-                for (int rb = first; rb < nrows; rb += 8 * rpp) {
-                    uint32_t d[8][4];
+                        for (int u = 0; u < 8; ++u) {
+                            const int rr = rb + u * rpp;
+                            d[u] = rr < nrow ? src[(ptrdiff_t)rr * dp] : 0u;
+                        }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int rr = rb + u * rpp;
-                        if (rr < nrows) {
-                            const int ry = rr >> 2, fy = rr & 3;
-                            const uint32_t ro = (uint32_t)((clampi(Y0 + ry, -1, H) + 1) * 4 + fy) * (uint32_t)QW;
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) d[u][i] = qp[ro + (uint32_t)xo[i]];
+                        for (int u = 0; u < 8; ++u) {
+                            const int rr = rb + u * rpp;
+                            if (rr < nrow) s_up[rr * nd + c] = d[u];
                         }
                     }
+                } else {
+                    // near the plane's edges: sample by sample, clamped
+                    for (int rr = r0 < rpp ? r0 : nrow; rr < nrow; rr += rpp) {
+                        uint32_t v = 0;
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int rr = rb + u * rpp;
-                        if (rr >= nrows) break;
-                        const int ry = rr >> 2, fy = rr & 3;
-                        const uint32_t lo01 = __builtin_amdgcn_perm(d[u][1], d[u][0], 0x05010400u);
-                        const uint32_t hi01 = __builtin_amdgcn_perm(d[u][1], d[u][0], 0x07030602u);
-                        const uint32_t lo23 = __builtin_amdgcn_perm(d[u][3], d[u][2], 0x05010400u);
-                        const uint32_t hi23 = __builtin_amdgcn_perm(d[u][3], d[u][2], 0x07030602u);
-                        uint32_t *o = s_ph + (fy * 4 * ly.rows + ry) * ng + gx;
-                        const int ps = ly.rows * ng;  // dwords per plane
-                        o[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-                        o[ps] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-                        o[2 * ps] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-                        o[3 * ps] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+                        for (int i = 0; i < 4; ++i) v |= (uint32_t)up8_at(J, r, cx + 4 * c + i, uy0 + rr) << (8 * i);
+                        s_up[rr * nd + c] = v;
                     }
                 }
             }
             // per pass (at most four: blocks up to 32 rows, eight or more rows a pass)
-            // a lane's picture dword and the byte offset of its row and column in a plane
+            // a lane's picture dword and the byte offset of its samples for offset 0
             uint32_t pv[4];
             int loff[4];
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 const int row = p < L.npass ? unit_row(L, p, b.yl) : -1;
                 pv[p] = row >= 0 ? b8_dword(J.pic8, J.p8, b.xp + 4 * L.col, b.yp + row) & L.mask : 0u;
-                loff[p] = row >= 0 ? (row + 1) * ly.pitch + 4 * L.col + 1 : -1;
-                if (row >= 0) s_pic[row * ly.pd + L.col] = pv[p];  // for the gathered predictor
+                loff[p] = row >= 0 ? (2 * row + 2) * 4 * nd + a0 + 8 * L.col + 2 : -1;
+                if (row >= 0) s_pic[row * ly.pd + L.col] = pv[p];  // for the predictor read from the plane
             }
             WAVE_SYNC();
             const bool packed = b.xl * b.yl * 255 < 65536;
-            const int plane = ly.rows * ly.pitch;
-            // partial SAD of a lane for quarter-pel offset (dx, dy) from 4 v; the
-            // plane offset and the byte alignment are the same for every lane
+            // partial SAD of a lane for quarter-pel offset (dx, dy) from 4 v: up
+            // offset (dx >> 1, dy >> 1), phase (dx & 1, dy & 1), the same for every lane
             auto part = [&](int dx, int dy) -> int {
-                const int ix = dx >> 2, fx = dx & 3, iy = dy >> 2, fy = dy & 3;
-                const int base = (fy * 4 + fx) * plane + iy * ly.pitch + ix;
-                const uint32_t al = (uint32_t)((1 + ix) & 3);
-                const uint8_t *pl = reinterpret_cast<const uint8_t *>(s_ph);
+                const int base = (dy >> 1) * 4 * nd + (dx >> 1), rx = dx & 1, ry = dy & 1;
                 uint32_t acc = 0;
 #pragma unroll
                 for (int pp = 0; pp < 4; ++pp) {
                     if (pp >= L.npass) break;
                     if (loff[pp] < 0) continue;
-                    const int off = loff[pp] + base;
-                    const uint32_t *q = reinterpret_cast<const uint32_t *>(pl + (off & ~3));
-                    const uint32_t v = __builtin_amdgcn_alignbyte(q[1], q[0], al);
+                    // loff + base >= 0: the staged rows start 2 up rows and columns before offset 0
+                    const uint32_t v = quad_u8(s_up, (uint32_t)(loff[pp] + base), 4 * nd, rx, ry);
                     acc = __builtin_amdgcn_sad_u8(pv[pp], v & L.mask, acc);
                 }
                 return (int)acc;
@@ -3557,7 +3712,7 @@ __global__ __launch_bounds__(256) void k_fast_subpel(const JobDesc *__restrict__
                 Cand4 c;
                 c.c[0].x = pred.x << (2 - prec);
                 c.c[0].y = pred.y << (2 - prec);
-                ps = qp_sads<1>(J, qp, b, L, s_pic, ly.pd, c, 1).s[0];
+                ps = up8_sads<1>(J, r, b, L, s_pic, ly.pd, c, 1).s[0];
             }
             const float pred_total = (float)ps;
             if (pred_total < lim) {
@@ -3683,7 +3838,7 @@ struct UnitBest {
     int sad;
 };
 
-__device__ __forceinline__ UnitBest unit_best(const JobDesc &J, const uint32_t *__restrict__ qp, const Blk &b,
+__device__ __forceinline__ UnitBest unit_best(const JobDesc &J, int r, const Blk &b,
                                               const UnitLanes &L, const uint32_t *__restrict__ s_pic, int pd,
                                               const Cand4 &g, dme_mv pred, float lambda) {
     UnitBest u;
@@ -3715,7 +3870,7 @@ __device__ __forceinline__ UnitBest unit_best(const JobDesc &J, const uint32_t *
         qc.c[i].x = cc.c[i].x << qs;
         qc.c[i].y = cc.c[i].y << qs;
     }
-    const Sad4 sads = qp_sads<4>(J, qp, b, L, s_pic, pd, qc, n);
+    const Sad4 sads = up8_sads<4>(J, r, b, L, s_pic, pd, qc, n);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (i >= n) break;
@@ -3813,9 +3968,9 @@ __global__ __launch_bounds__(256) void k_fast_mode(const JobDesc *__restrict__ j
             g.c[1] = V2[(size_t)yb * X + xb + 1];
             g.c[2] = V2[(size_t)(yb + 1) * X + xb];
             g.c[3] = V2[(size_t)(yb + 1) * X + xb + 1];
-            ub[r] = unit_best(J, J.qp[r], b, L, s_pic, pd, g, mode_pred(V2, X, xb, yb), lambda);
+            ub[r] = unit_best(J, r, b, L, s_pic, pd, g, mode_pred(V2, X, xb, yb), lambda);
         }
-        const int bs = (nr > 1 && !b.empty()) ? qp_bi_sad(J, b, L, s_pic, pd, ub[0].v, ub[1].v) : 0;
+        const int bs = (nr > 1 && !b.empty()) ? up8_bi_sad_s(J, b, L, s_pic, pd, ub[0].v, ub[1].v) : 0;
         int mode;
         c1 += unit_decn(nr, J.lf[1], lambda, ub[0].c, ub[1].c, bs, mode);
         if (lane == 0) {
@@ -3857,9 +4012,9 @@ __global__ __launch_bounds__(256) void k_fast_mode(const JobDesc *__restrict__ j
             Cand4 g;
 #pragma unroll
             for (int q = 0; q < 4; ++q) g.c[q] = U.v[q][r];
-            u0[r] = unit_best(J, J.qp[r], b, L, s_pic, pd, g, mode_pred(J.fv2[r], X, xb, yb), lambda);
+            u0[r] = unit_best(J, r, b, L, s_pic, pd, g, mode_pred(J.fv2[r], X, xb, yb), lambda);
         }
-        bs0 = (nr > 1 && !b.empty()) ? qp_bi_sad(J, b, L, s_pic, pd, u0[0].v, u0[1].v) : 0;
+        bs0 = (nr > 1 && !b.empty()) ? up8_bi_sad_s(J, b, L, s_pic, pd, u0[0].v, u0[1].v) : 0;
         c0 = unit_decn(nr, J.lf[0], lambda, u0[0].c, u0[1].c, bs0, m0);
     }
     if (c0 <= sbc) {
@@ -3869,20 +4024,27 @@ __global__ __launch_bounds__(256) void k_fast_mode(const JobDesc *__restrict__ j
     // merges into the level-2 arrays (DoLevelDecn's copy; the level-1/0 DC
     // is zero) and the chosen units' SADs
     if (split < 2) {
+        // (values read from LDS first, then selected: no pointer selects
+        // between LDS and registers)
+        const dme_mv z0 = u0[0].v, z1 = u0[1].v;
+        const int zs0 = u0[0].sad, zs1 = u0[1].sad;
         for (int t = lane; t < 16; t += WAVE) {
             const int bi = 4 * xs + (t & 3), bj = 4 * ys + (t >> 2);
             const size_t d = (size_t)bj * X + bi;
             const int q = ((t >> 3) << 1) | ((t & 3) >> 1);
-            J.mode[d] = split == 0 ? m0 : U.mode[q];
+            const dme_mv l0 = U.v[q][0], l1 = U.v[q][1];
+            const int lm = U.mode[q];
+            J.mode[d] = split == 0 ? m0 : lm;
             J.dc[0][d] = 0;
-            J.vec[0][d] = split == 0 ? u0[0].v : U.v[q][0];
-            if (nr > 1) J.vec[1][d] = split == 0 ? u0[1].v : U.v[q][1];
+            J.vec[0][d] = split == 0 ? z0 : l0;
+            if (nr > 1) J.vec[1][d] = split == 0 ? z1 : l1;
         }
         if (lane < 4 && (split == 1 || lane == 0)) {
             int *f = J.fsb + (size_t)s * kFsbInts + 3 * lane;
-            f[0] = split == 0 ? u0[0].sad : U.sad[lane][0];
-            f[1] = nr < 2 ? 0 : (split == 0 ? u0[1].sad : U.sad[lane][1]);
-            f[2] = split == 0 ? bs0 : U.bi[lane];
+            const int ls0 = U.sad[lane][0], ls1 = U.sad[lane][1], lb = U.bi[lane];
+            f[0] = split == 0 ? zs0 : ls0;
+            f[1] = nr < 2 ? 0 : (split == 0 ? zs1 : ls1);
+            f[2] = split == 0 ? bs0 : lb;
         }
     }
     if (lane == 0) {
@@ -3941,7 +4103,7 @@ __global__ __launch_bounds__(256) void k_fast_eval(const JobDesc *__restrict__ j
                 const float mvcost = var_up(pred, v);
                 const float start = mvcost * lambda;
                 const UpStart us = up_start(b, v, J.step, UW, UH);
-                const float t = us.bounds ? wave_chain(start, J.pic[0], J.W, J.qp[r], UW, UH, b, us)
+                const float t = us.bounds ? up8_chain(start, &J, r, b, us)
                                           : (float)f[r] + start;
                 if (t < c[r].total) {
                     c[r].total = t;
@@ -4379,9 +4541,10 @@ hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, hipStream_t st) {
+hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, bool up8, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_build_qp, dim3(total), dim3(256), 0, st, pt, npt, total);
+    if (up8) hipLaunchKernelGGL(k_build_qp<true>, dim3(total), dim3(256), 0, st, pt, npt, total);
+    else hipLaunchKernelGGL(k_build_qp<false>, dim3(total), dim3(256), 0, st, pt, npt, total);
     return hipGetLastError();
 }
 
@@ -4408,9 +4571,11 @@ hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntas
     return hipGetLastError();
 }
 
-hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mode_l2, dim3((total + 4 * PW_MODE_L2 - 1) / (4 * PW_MODE_L2)), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    const dim3 grid((total + 4 * PW_MODE_L2 - 1) / (4 * PW_MODE_L2));
+    if (up8) hipLaunchKernelGGL(k_mode_l2<true>, grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    else hipLaunchKernelGGL(k_mode_l2<false>, grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 
@@ -4492,6 +4657,19 @@ static size_t fast_mode_lds(int max_x, int max_y, int &pd) {
 }
 constexpr size_t kFastModeLdsMax = 60 * 1024;  // beside its static per-wave unit results
 
+// LDS of k_fast_subpel (four waves) for blocks up to bx x by: 2 by + 3
+// staged up rows of 2 ceil(bx / 4) + 3 dwords (the up samples of the block's
+// candidates from the dword before them, and the two dwords quad_u8 reads
+// past a lane's first sample), then the picture block.
+static size_t fast_subpel_lds(int max_blen_x, int max_blen_y, FsLds &ly) {
+    ly.rows = 2 * max_blen_y + 3;
+    ly.pitch = 2 * ((max_blen_x + 3) / 4) + 3;
+    ly.pd = (max_blen_x + 3) / 4;
+    ly.wave = ly.rows * ly.pitch + ly.pd * max_blen_y;
+    return sizeof(uint32_t) * 4 * (size_t)ly.wave;
+}
+constexpr size_t kFastSubpelLdsMax = 64 * 1024;
+
 // The limits of one job's launches, checked on the host before anything is
 // queued: "" if every kernel of the job's stages fits, else the limit.
 std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
@@ -4524,6 +4702,9 @@ std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
     if (learned && fast_mode_lds(std::max(g.blen[0][0], g.blen[1][0]), std::max(g.blen[0][1], g.blen[1][1]), fpd) >
                        kFastModeLdsMax)
         return "the learned form's superblock units exceed its LDS";
+    FsLds fl;
+    if (learned && fast_subpel_lds(bx, by, fl) > kFastSubpelLdsMax)
+        return "the learned form's sub-pel staging exceeds 64 KiB of LDS";
     if ((size_t)g.W * sizeof(int) > 64 * 1024) return "picture width exceeds the block-DC row buffer";
     return std::string();
 }
@@ -4549,17 +4730,6 @@ hipError_t launch_lp_pel_costs(const JobDesc *jobs, const Task *tasks, int ntask
 
 // ---- the fast form of the learned path ----------------------------------------
 
-// LDS of k_fast_subpel (four waves) for blocks up to bx x by: 16 phase
-// planes of (by + 2) rows of the positions v - 1 .. v + bx (padded to whole
-// dwords, plus one dword for the unaligned reads), then the picture block.
-static size_t fast_subpel_lds(int max_blen_x, int max_blen_y, FsLds &ly) {
-    ly.rows = max_blen_y + 2;
-    ly.pitch = 4 * ((max_blen_x + 2 + 3) / 4 + 1);
-    ly.pd = (max_blen_x + 3) / 4;
-    ly.wave = 16 * ly.rows * (ly.pitch / 4) + ly.pd * max_blen_y;
-    return sizeof(uint32_t) * 4 * (size_t)ly.wave;
-}
-constexpr size_t kFastSubpelLdsMax = 64 * 1024;
 
 hipError_t launch_fast_subpel(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
                               int max_blen_y, hipStream_t st) {

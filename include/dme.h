@@ -387,6 +387,11 @@ int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs,
  * pixel_match.cpp:159-183, which the reference keeps private);
  * kind 1 = the 2W x 2H up-converted plane (EncPicture::UpOrigData,
  * enc_picture.cpp:131-156), built once the picture served as a reference.
+ * kind 2 = the byte up-plane the learned path's fast form reads, built once
+ * the picture served it as a reference: the up-converted samples of up rows
+ * -64 .. R - 65 and columns -64 .. P - 65 (R x P samples, R = 32 *
+ * ceil((H + 64) / 16), P = 128 * ceil((W + 64) / 64)), each the sample at
+ * the clamped coordinates (the reference's bounds-checked reads).
  * out_elems must be at least the plane's sample count.                    */
 int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level,
                   int16_t *out, size_t out_elems);

@@ -101,3 +101,24 @@ def test_fast_form_gap_on_held_out_pictures(lctx):
     for pn, nums, _ in jobs:
         for n in [pn] + nums:
             lctx.drop(n)
+
+
+@pytest.mark.parametrize("W,H", [(352, 288), (392, 248)])
+def test_byte_up_plane_is_the_clamped_up_plane(lctx, W, H):
+    # the fast form's byte up-plane, margins included, holds the up-converted
+    # sample at the clamped coordinates; 392 x 248 has margin tiles that lie
+    # wholly outside the picture
+    d = synth.picture_set(31, W, H, 1, 4.0, 2, 8.0)
+    p = make_params(12, 12, 8, 8, precision=2, depth=3, lam=10.0)
+    lctx.upload(5300, d["refs"][0])
+    lctx.upload(5301, d["cur"], d["u"], d["v"])
+    g = dirac_me.geometry(p, W, H)
+    lctx.run_batch_learned([(5301, [5300], p)], [dirac_me.MEData(g)])
+    up = lctx.get_plane(5300, 1, 0, (2 * H, 2 * W))
+    R, P = 32 * (-(-(H + 64) // 16)), 128 * (-(-(W + 64) // 64))
+    got = lctx.get_plane(5300, 2, 0, (R, P))
+    iy = np.clip(np.arange(R) - 64, 0, 2 * H - 1)
+    ix = np.clip(np.arange(P) - 64, 0, 2 * W - 1)
+    assert np.array_equal(got, up[iy][:, ix])
+    for n in (5300, 5301):
+        lctx.drop(n)
