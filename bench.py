@@ -5,8 +5,8 @@ lambda 10), batched, on one MI355X.
 A step is one pass of the whole live ME sequence (pyramids and up-converted
 references rebuilt, pel search, lambda map, sub-pel refinement, mode
 decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
-one JSON line; the algorithmic byte and SAD-term counts behind the roofline
-figures are in _geometry_terms().  The line also carries the learned-proposal
+one JSON line; the compulsory bytes and algorithmic SAD terms behind the
+roofline figures come from _pel_bulk_bytes() and _sad_terms().  The line also carries the learned-proposal
 leg, the motion-compensation leg (residuals of the same batch from its
 estimates, mc_leg()), the full-search leg (full_search_leg()) and the .imt
 diagnostics-text leg (imt_leg()).
@@ -23,50 +23,31 @@ sys.path.insert(0, os.path.join(ROOT, "dirac-research_amd"))
 
 METRIC = "1080p 2-ref ME frames/s: plain GPU vs learned-proposal at ≤1% mean cost gap"
 HBM_PEAK_GBS = 8000.0
-INT_LANE_OPS_PEAK = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes per clock x 2.4 GHz
+SAD_TERMS_PEAK = 256 * 64 * 2.4e9 * 4  # v_sad_u8: 4 absolute differences per lane-op
 
 
-def _geometry_terms(geom, nrefs):
-    """Algorithmic HBM bytes and SAD terms per picture for the main kernels.
-
-    Bytes count what a kernel reads and writes per work unit with no reuse
-    between units (int16 samples, int32 table entries); terms count the
-    absolute differences of the SADs it evaluates.
-    """
-    xl, yl = geom.luma_blen[2][0], geom.luma_blen[2][1]
-    B = geom.xnum_blocks * geom.ynum_blocks
-    out = {}
-    # pel tables: picture block + one or two reference regions of radius
-    # R = r + 1 read, a 28-int record written, per block, level and reference
-    b_bytes = t_terms = w_bytes = 0.0
+def _pel_bulk_bytes(geom, nrefs):
+    """Compulsory HBM bytes of k_pel_bulk per picture, summed over the levels
+    (one launch per level): every sample of the level's picture and of its
+    references read once (level 0: the byte planes, 1 byte a sample; the
+    pyramid levels: int16), each block's guide vector read (8 bytes) and its
+    28-int pel record written, per reference."""
+    total = 0.0
     for l in range(geom.depth + 1):
+        n = geom.level_width[l] * geom.level_height[l]
         nb = geom.level_xblocks[l] * geom.level_yblocks[l]
-        r = min(l + 1, 5)
-        R = r + 1
-        TS = (2 * R + 1) ** 2
-        ntab = 2 if l < geom.depth else 1
-        b_bytes += nb * nrefs * (2 * xl * yl + ntab * 2 * (xl + 2 * R) * (yl + 2 * R) + 8 + 28 * 4)
-        t_terms += nb * nrefs * ntab * TS * xl * yl
-        w_bytes += nb * nrefs * (28 * 4 + 8 + 12)
-    out["k_pel_bulk"] = (b_bytes, t_terms)
-    out["k_pel_wave"] = (w_bytes, 0.0)
-    # sub-pel tables: picture block + 16 quarter-pel phase regions of
-    # (xl + 2) x (yl + 2) bytes read, a 56-int record written
-    out["k_subpel_bulk"] = (B * nrefs * (2 * xl * yl + 16 * (xl + 2) * (yl + 2) + 8 + 12 + 4 + 56 * 4),
-                            B * nrefs * 49 * xl * yl)
-    out["k_subpel_wave"] = (B * nrefs * (56 * 4 + 8 + 12), 0.0)
-    # mode decision: level-1 and level-0 unit tables, superblock wavefront
-    l1 = geom.luma_blen[1][0] * geom.luma_blen[1][1]
-    l0 = geom.luma_blen[0][0] * geom.luma_blen[0][1]
-    nu1, nsb = B // 4, B // 16
-    out["k_mode_tables"] = (nu1 * (2 * l1 + nrefs * 4 * 4 * l1 + 24 * 4) + nsb * (2 * l0 + nrefs * 16 * 4 * l0 + 32 * 4),
-                            nu1 * (nrefs * 4 + 16) * l1 + nsb * nrefs * 16 * l0)
-    out["k_mode_wave"] = (nsb * (16 * 4 + 4 * 24 * 4 + 32 * 4 + 2 * l0 * 9 + 16 * 20), nsb * l0)
-    out["k_upconvert"] = (nrefs * geom.width * geom.height * 2 * 5, 0.0)
-    out["k_build_qp"] = (nrefs * (geom.width * geom.height * 4 * 2 + (geom.width + 2) * (geom.height + 2) * 16), 0.0)
-    out["k_downconvert"] = (sum((1 + nrefs) * (geom.level_width[l] * geom.level_height[l] * 2 * 5)
-                                for l in range(1, geom.depth + 1)), 0.0)
-    return out
+        total += (1 + nrefs) * n * (1 if l == 0 else 2) + nrefs * nb * (28 * 4 + (8 if l < geom.depth else 0))
+    return total
+
+
+def _sad_terms():
+    """Per-stage algorithmic SAD terms (N_sad) per picture of the benchmark
+    pictures, counted by the oracle when the fixture was made
+    (tests/golden/nsad_1080p.json, tests/golden/make_nsad.py): mean over
+    the 8 pictures of the batch."""
+    with open(os.path.join(ROOT, "tests", "golden", "nsad_1080p.json")) as f:
+        rows = json.load(f)["frames"]
+    return {k: sum(r[k] for r in rows) / len(rows) for k in ("pel", "subpel", "mode")}
 
 
 def _profiled_traffic(kernel, batch):
@@ -103,36 +84,70 @@ def _copy_calibration(mb):
     hip.hipFree(b)
 
 
-def cpu_baseline(frames, params, budget_s):
-    """Reference encoder ME (oracle/_ref, single thread) or the oracle, on a
-    bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O
-
-    kind = "reference" if O.ref_available() else "port"
-    run = O.run_ref if kind == "reference" else O.run_oracle
-    n = 0
-    t0 = time.time()
-    while True:
-        d = frames[n % len(frames)]
-        run(params, d["cur"], d["refs"], d["u"], d["v"])
-        n += 1
-        if time.time() - t0 >= budget_s or n >= 200:
-            break
-    dt = time.time() - t0
+def _cpu_model():
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": n / dt, "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": "%d 1080p B pictures (2 refs) of the same synthetic workload, %s %s, %.1f s on %s" % (
-                n, "reference encoder ME (MMX build, compiled from its sources)" if kind == "reference" else "oracle port",
-                "single thread", dt, cpu)}
+    return cpu
+
+
+def _cpu_threads():
+    """Host threads for the N-core CPU baseline: the cores this process may
+    use, at most 16 (the host share of one GPU on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(frames, params, budget_s):
+    """Reference encoder ME (oracle/_ref, compiled from the reference's
+    sources) or the oracle, on a bounded sample of the same workload: on one
+    thread, and on N threads with one picture per thread (the reference is
+    single-threaded per picture; pictures are independent).  The top-level
+    figure is the N-thread one."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    kind = "reference" if O.ref_available() else "port"
+    run = O.run_ref if kind == "reference" else O.run_oracle
+    what = "reference encoder ME (MMX build, compiled from its sources)" if kind == "reference" else "oracle port"
+
+    def one(i):
+        d = frames[i % len(frames)]
+        run(params, d["cur"], d["refs"], d["u"], d["v"])
+
+    n = 0
+    t0 = time.time()
+    while True:
+        one(n)
+        n += 1
+        if time.time() - t0 >= budget_s / 2 or n >= 200:
+            break
+    dt1 = time.time() - t0
+    T = _cpu_threads()
+    m = 0
+    t0 = time.time()
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        while True:
+            list(ex.map(one, range(m, m + T)))
+            m += T
+            if time.time() - t0 >= budget_s / 2 or m >= 50 * T:
+                break
+    dtn = time.time() - t0
+    cpu = _cpu_model()
+    return {"value": m / dtn, "unit": "frames/s", "cores": T, "kind": kind,
+            "sample": "%d 1080p B pictures (2 refs) of the same synthetic workload, %s, %d threads (one picture "
+                      "per thread), %.1f s on %s" % (m, what, T, dtn, cpu),
+            "single_core": {"value": n / dt1, "unit": "frames/s", "cores": 1,
+                            "sample": "%d pictures, one thread, %.1f s" % (n, dt1)}}
 
 
 def _mc_bytes(geom, me):
@@ -169,6 +184,31 @@ def _mc_bytes(geom, me):
         total += area_refs * 8 * (1 if c == 0 else 2) + pw * ph * 4 * (1 if c == 0 else 2)
     B = xnb * ynb
     return total + B * (2 * 8 + 4 + 3 * 2) + xsb * geom.ynum_sb * 4
+
+
+def export_leg(ctx, jobs, geom, B, args, step_s):
+    """The same step with every picture's MEData copied to host memory (the
+    arrays a drop-in caller reads; about 1 MB per 1080p picture), so that
+    the cost the headline leaves on the device is visible beside it."""
+    import dirac_me
+
+    outs = [dirac_me.MEData(geom) for _ in range(B)]
+
+    def step():
+        ctx.invalidate(-1)
+        ctx.run_batch(jobs, outs)
+
+    step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    nbytes = sum(a.nbytes for a in outs[0].arrays(2).values())
+    return {"value": B / dt, "unit": "frames/s", "ms_per_step": dt * 1e3,
+            "export_ms_per_step": (dt - step_s) * 1e3, "medata_bytes_per_picture": nbytes,
+            "note": "PCIe-inclusive: MEData of every picture copied to pageable host arrays each step"}
 
 
 def mc_leg(ctx, jobs, frames, params, geom, B, args):
@@ -334,7 +374,7 @@ def full_search_leg(ctx, jobs, frames, args):
     ms = kt["k_pel_full"][0]
     out = {"value": n / dt, "unit": "frames/s", "ms_per_step": dt * 1e3, "batch": n, "range": 32,
            "k_pel_full_ms": ms, "sad_terms_per_s": terms / (ms * 1e-3),
-           "int_ops_frac": 3.0 * terms / (ms * 1e-3) / INT_LANE_OPS_PEAK,
+           "sad_frac": terms / (ms * 1e-3) / SAD_TERMS_PEAK,
            "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -383,7 +423,14 @@ def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
     gaps = [float(np.sum(l.sb_costs, dtype=np.float64) / np.sum(p.sb_costs, dtype=np.float64) - 1.0)
             for p, l in zip(plain, lrn)]
     fps = B * args.steps / dt
-    return {"value": fps, "unit": "frames/s", "ms_per_step": dt / args.steps * 1e3,
+    ctx.set_profiling(True)
+    ctx.invalidate(-1)
+    ctx.run_batch_learned(jobs)
+    ctx.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    return {"value": fps, "unit": "frames/s", "ms_per_step": dt / args.steps * 1e3, "form": "fast",
+            "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
             "speedup_vs_plain": fps / plain_fps, "mean_sb_cost_gap": float(np.mean(gaps)),
             "max_sb_cost_gap": float(np.max(gaps)), "fallback_rate": (f1 - f0) / max(1, b1 - b0),
             "pictures_for_gap": n, "weights": "dirac_me/proposal_weights.npy (2000 training pictures)"}
@@ -452,6 +499,7 @@ def main():
     ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-imt", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -510,22 +558,34 @@ def main():
     ctx.synchronize()
     kt = ctx.kernel_times()
     ctx.set_profiling(False)
+    # the dominant kernel: compulsory HBM bytes per launch against its mean
+    # launch time (HBM basis), and the pel stage's algorithmic SAD terms
+    # against the v_sad_u8 peak (the integer-VALU basis that bounds it)
     dom = max(kt, key=lambda k: kt[k][0])
-    terms = _geometry_terms(geom, 2)
-    alg_bytes, alg_terms = terms.get(dom, (0.0, 0.0))
     ms, nl = kt[dom]
     per_launch_ms = ms / nl
-    achieved = alg_bytes * B / nl / (per_launch_ms * 1e-3) / 1e9
-    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": _profiled_traffic(dom, B),
-            "algorithmic_bytes_per_launch": alg_bytes * B / nl, "avg_launch_ms": per_launch_ms,
-            "launches_per_step": nl}
-    valu = {}
-    for k in ("k_pel_bulk", "k_subpel_bulk", "k_mode_tables"):
-        if k in kt:
-            ops = 3.0 * terms[k][1] * B
-            valu[k] = {"sad_terms_per_s": terms[k][1] * B / (kt[k][0] * 1e-3),
-                       "int_ops_frac": ops / (kt[k][0] * 1e-3) / INT_LANE_OPS_PEAK}
+    alg_bytes = _pel_bulk_bytes(geom, 2) * B if dom == "k_pel_bulk" else None
+    roof = {"kernel": dom, "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "avg_launch_ms": per_launch_ms,
+            "launches_per_step": nl, "traffic": _profiled_traffic(dom, B)}
+    if alg_bytes is not None:
+        achieved = alg_bytes / nl / (per_launch_ms * 1e-3) / 1e9
+        roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS, algorithmic_bytes_per_launch=alg_bytes / nl)
+    nsad = _sad_terms()
+    stages = {"pel": ("k_pel_bulk", "k_pel_wave"), "subpel": ("k_subpel_bulk", "k_subpel_wave"),
+              "mode": ("k_mode_l2", "k_mode_tables", "k_mode_wave")}
+    sad = {}
+    for st, ks in stages.items():
+        t = sum(kt[k][0] for k in ks if k in kt) * 1e-3
+        if t > 0:
+            a = nsad[st] * B / t
+            sad[st] = {"kernels": list(ks), "sad_terms_per_step": nsad[st] * B, "achieved": a,
+                       "frac": a / SAD_TERMS_PEAK}
+    a = nsad["pel"] * B / (kt["k_pel_bulk"][0] * 1e-3)
+    roof_sad = {"kernel": "k_pel_bulk", "bound": "valu", "achieved": a, "peak": SAD_TERMS_PEAK,
+                "unit": "SAD terms/s", "frac": a / SAD_TERMS_PEAK,
+                "terms_per_launch": nsad["pel"] * B / kt["k_pel_bulk"][1],
+                "note": "the pel stage's algorithmic SAD terms (N_sad, counted by the oracle) over k_pel_bulk's time; "
+                        "peak = 256 CUs x 64 lanes x 2.4 GHz x 4 terms per v_sad_u8"}
     out = {
         "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -534,9 +594,10 @@ def main():
                                "pyramid, quarter pel, lambda 10; batch of %d pictures per step" % B,
                    "batch": B, "width": 1920, "height": 1080, "refs": 2, "depth": 4, "precision": "quarter"},
         "block_decisions_per_s": fps * geom.xnum_blocks * geom.ynum_blocks * 2,
-        "roofline": roof, "valu": valu,
+        "roofline": roof, "roofline_sad": roof_sad, "sad_terms_by_stage": sad,
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
     }
+    out["with_export"] = export_leg(ctx, jobs, geom, B, args, dt / args.steps)
     if not args.no_mc:
         out["motion_compensation"] = mc_leg(ctx, jobs, frames, params, geom, B, args)
     if not args.no_full:
@@ -548,6 +609,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(frames, params, args.cpu_budget)
     ctx.close()
+    if not args.no_mixed:
+        out["mixed64"] = mixed_leg(args)
     if args.copy_calibration_mb:
         _copy_calibration(args.copy_calibration_mb)
     print(json.dumps(out))

@@ -69,6 +69,7 @@ struct JobDesc {
     int32_t DME_G *split;
     float DME_G *sbc;
     float DME_G *lam;
+    int DME_G *icount;                                // INTRA blocks of the final modes (k_intra_count)
     float lambda;                                     // the picture's lambda (SetLambdaMap)
     int DME_G *lwork;                                 // 2 x blocks: transition maps (k_lambda)
     // mode decision temporaries (levels 1 and 0)
@@ -152,6 +153,13 @@ struct McPackJob {
 // consecutive work items starting at `offset` of the launch's index space.
 struct Task {
     int job, ref, count, offset;
+};
+
+// One array of a batch's MEData export: `bytes` (a multiple of 2) from src
+// to byte `off` (a multiple of 16) of the packed export buffer.
+struct PackSeg {
+    const void *src;
+    unsigned long long off, bytes;
 };
 
 }  // namespace dme

@@ -77,6 +77,8 @@ hipError_t launch_fast_mode(const JobDesc *jobs, const Task *tasks, int ntasks, 
                             hipStream_t st);
 hipError_t launch_fast_eval(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
 hipError_t launch_dc_smooth(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_pack(const PackSeg *segs, int nseg, uint8_t *dst, hipStream_t st);
+hipError_t launch_intra_count(const JobDesc *jobs, int njobs, hipStream_t st);
 
 #define HIP_TRY(expr)                                                                                   \
     do {                                                                                                \
@@ -160,7 +162,7 @@ struct JobState {
     dme_params p;
     Geometry g;
     int stage = 0;  // 1 pel, 2 lambda, 3 subpel, 4 mode
-    DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3];
+    DevBuf vec[2], pc[2], mode, intra, bi, dc[3], split, sbc, lam, lam1, lam0, ucost, dct[3], icount;
     DevBuf lvec[2][DME_MAX_DEPTH + 1], lpc[2][DME_MAX_DEPTH + 1];
     DevBuf ptab[2], qtab[2], m1tab, m0tab;
     DevBuf lwork;  // transition maps of the device lambda stage
@@ -234,6 +236,10 @@ struct dme_ctx {
     dme::DevBuf d_mc;      // motion compensation: descriptors, weights, uploaded MvData
     dme::DevBuf d_mc_planes;  // host-array compensation: the arrays on the device
     dme::DevBuf d_mc_units;   // packed per-block MvData records (k_mc_pack)
+    // batch export: the packed MEData arrays on the device and in pinned
+    // host memory, and the segment table (k_pack)
+    dme::DevBuf x_dev, x_segs;
+    dme::PinnedBuf x_host;
     // Lanes (stream + buffers); stages queue work on the current lane.
     // Large batches run as parts on the lanes, so that one part's host work
     // and latency-bound wavefronts overlap the other parts' kernels.  The arena of a lane holds the pinned host staging of one
@@ -283,13 +289,13 @@ enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
     KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_FAST_SUBPEL, KK_FAST_L2, KK_FAST_MODE,
-    KK_FAST_EVAL, KK_BUILD_UP8, KK_COUNT
+    KK_FAST_EVAL, KK_BUILD_UP8, KK_INTRA, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
     "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full",
-    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval", "k_build_up8"};
+    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval", "k_build_up8", "k_intra_count"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -553,6 +559,7 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     DME_TRY(J.m1tab.ensure(sizeof(int) * 24 * (B / 4)));
     DME_TRY(J.m0tab.ensure(sizeof(int) * 32 * (B / 16)));
     DME_TRY(J.mode.ensure(sizeof(int32_t) * B));
+    DME_TRY(J.icount.ensure(sizeof(int)));
     DME_TRY(J.intra.ensure(sizeof(float) * B));
     DME_TRY(J.bi.ensure(sizeof(dme_cost) * B));
     for (int c = 0; c < 3; ++c) {
@@ -624,6 +631,7 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     d.split = J.split.as<int32_t>();
     d.sbc = J.sbc.as<float>();
     d.lam = J.lam.as<float>();
+    d.icount = J.icount.as<int>();
     d.lam1 = J.lam1.as<float>();
     d.lam0 = J.lam0.as<float>();
     d.ucost = J.ucost.as<float>();
@@ -1020,6 +1028,7 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
                          [&] { return launch_block_dc(dj, dtask(ctx, rdr), rdr.n, rdr.total, max_w, st); }));
     DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
                          [&] { return launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_INTRA, launches, [&] { return launch_intra_count(dj, (int)js.size(), st); }));
     if (rsh.n) {
         DME_TRY(timed_launch(ctx, KK_SHIFT, launches, [&] {
             return launch_shift_vectors(dj, dtask(ctx, rsh), rsh.n, rsh.total, 0, st);
@@ -1051,6 +1060,86 @@ static int export_job(dme_ctx *ctx, JobState &J, dme_medata *out, hipMemcpyKind 
         if (out->sb_costs) HIP_TRY(hipMemcpyAsync(out->sb_costs, J.sbc.p, sizeof(float) * S, kind, st));
     }
     HIP_TRY(hipStreamSynchronize(st));
+    if (J.stage >= 4 && out->intra_block_ratio) {
+        int cnt = 0;
+        HIP_TRY(hipMemcpy(&cnt, J.icount.p, sizeof(int), hipMemcpyDeviceToHost));
+        const float ratio = static_cast<float>(static_cast<double>(cnt) / static_cast<double>(B));
+        HIP_TRY(hipMemcpy(out->intra_block_ratio, &ratio, sizeof(float), hipMemcpyDefault));
+    }
+    return DME_OK;
+}
+
+// The MEData of a batch to host memory: k_pack gathers every requested
+// array into one device buffer, one copy moves it to pinned host memory, and
+// host threads scatter it into the caller's arrays (one DMA instead of a
+// dozen pageable copies per picture).
+static int export_batch(dme_ctx *ctx, const std::vector<JobState *> &js, dme_medata *outs) {
+    std::vector<PackSeg> segs;
+    std::vector<void *> dsts;
+    unsigned long long off = 0;
+    auto add = [&](void *dst, const DevBuf &src, size_t n) {
+        if (!dst || !n) return;
+        segs.push_back(PackSeg{src.p, off, (unsigned long long)n});
+        dsts.push_back(dst);
+        off += (n + 15) & ~(size_t)15;
+    };
+    for (size_t i = 0; i < js.size(); ++i) {
+        JobState &J = *js[i];
+        dme_medata *out = &outs[i];
+        const size_t B = (size_t)J.g.xnb * J.g.ynb, S = (size_t)J.g.xsb * J.g.ysb;
+        for (int r = 0; r < J.nrefs; ++r) {
+            add(out->vectors[r], J.vec[r], sizeof(dme_mv) * B);
+            add(out->pred_costs[r], J.pc[r], sizeof(dme_cost) * B);
+        }
+        if (J.stage >= 2) add(out->lambda_map, J.lam, sizeof(float) * B);
+        if (J.stage >= 4) {
+            add(out->mode, J.mode, sizeof(int32_t) * B);
+            add(out->intra_costs, J.intra, sizeof(float) * B);
+            add(out->bipred_costs, J.bi, sizeof(dme_cost) * B);
+            for (int c = 0; c < 3; ++c) add(out->dc[c], J.dc[c], sizeof(int16_t) * B);
+            add(out->sb_split, J.split, sizeof(int32_t) * S);
+            add(out->sb_costs, J.sbc, sizeof(float) * S);
+        }
+    }
+    // the INTRA counts, one int per picture, after the arrays
+    std::vector<int> cnt(js.size(), 0);
+    std::vector<unsigned long long> coff(js.size(), ~0ull);
+    for (size_t i = 0; i < js.size(); ++i) {
+        if (js[i]->stage < 4 || !outs[i].intra_block_ratio) continue;
+        coff[i] = off;
+        segs.push_back(PackSeg{js[i]->icount.p, off, (unsigned long long)sizeof(int)});
+        dsts.push_back(&cnt[i]);
+        off += 16;
+    }
+    if (segs.empty()) return DME_OK;
+    hipStream_t st = ctx->L->stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    DME_TRY(ctx->x_dev.ensure(off));
+    DME_TRY(ctx->x_host.ensure(off));
+    DME_TRY(upload(ctx, ctx->x_segs, segs.data(), sizeof(PackSeg) * segs.size()));
+    HIP_TRY(launch_pack(ctx->x_segs.as<PackSeg>(), (int)segs.size(), ctx->x_dev.as<uint8_t>(), st));
+    HIP_TRY(hipMemcpyAsync(ctx->x_host.p, ctx->x_dev.p, off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint8_t *h = static_cast<const uint8_t *>(ctx->x_host.p);
+    auto scatter = [&](size_t a, size_t b) {
+        for (size_t k = a; k < b; ++k) std::memcpy(dsts[k], h + segs[k].off, segs[k].bytes);
+    };
+    const size_t nt = std::min<size_t>(8, std::max<size_t>(1, off >> 23));  // a thread per 8 MiB, at most 8
+    if (nt == 1) {
+        scatter(0, segs.size());
+    } else {
+        std::vector<std::thread> th;
+        const size_t per = (segs.size() + nt - 1) / nt;
+        for (size_t t = 0; t < nt; ++t) {
+            const size_t a = t * per, b = std::min(segs.size(), a + per);
+            if (a < b) th.emplace_back(scatter, a, b);
+        }
+        for (auto &x : th) x.join();
+    }
+    for (size_t i = 0; i < js.size(); ++i)
+        if (coff[i] != ~0ull)
+            *outs[i].intra_block_ratio = static_cast<float>(static_cast<double>(cnt[i]) /
+                                                            static_cast<double>((size_t)js[i]->g.xnb * js[i]->g.ynb));
     return DME_OK;
 }
 
@@ -1437,8 +1526,7 @@ extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_m
         js.push_back(J);
     }
     DME_TRY(run_jobs(ctx, js, nullptr));
-    if (outs)
-        for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
+    if (outs) DME_TRY(export_batch(ctx, js, outs));
     return DME_OK;
 } catch (...) {
     return dme::exception_status();
@@ -1508,6 +1596,7 @@ static void noalloc_layout(dme_ctx *ctx, ScratchBump &sb, const Geometry &g, int
     sb.give(J.lam0, sizeof(float) * B / 16);
     sb.give(J.ucost, sizeof(float) * B);
     sb.give(J.lwork, sizeof(int) * 2 * B);
+    sb.give(J.icount, sizeof(int));
     // the lane's descriptor, task and plane-task buffers (one job: at most
     // 22 tasks and 29 plane tasks per upload)
     sb.give(ctx->na_lane.d_jobs, sizeof(JobDesc));
@@ -2314,6 +2403,7 @@ static int stage_mode_fast(dme_ctx *ctx, std::vector<JobState *> &js, int &launc
                          [&] { return launch_block_dc(dj, dtask(ctx, rdr), rdr.n, rdr.total, max_w, st); }));
     DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
                          [&] { return launch_dc_smooth(dj, dtask(ctx, rdc), rdc.n, rdc.total, st); }));
+    DME_TRY(timed_launch(ctx, KK_INTRA, launches, [&] { return launch_intra_count(dj, (int)js.size(), st); }));
     for (auto *j : js) j->stage = 4;
     return DME_OK;
 }
@@ -2474,7 +2564,7 @@ extern "C" int dme_run_batch_learned(dme_ctx *ctx, const dme_job *jobs, int njob
     }
     DME_TRY(run_jobs_learned(ctx, js));
     if (outs)
-        for (int i = 0; i < njobs; ++i) DME_TRY(export_job(ctx, *js[i], &outs[i]));
+        DME_TRY(export_batch(ctx, js, outs));
     return DME_OK;
 } catch (...) {
     return dme::exception_status();

@@ -1028,37 +1028,37 @@ struct PelLds {
 // themselves), staged from the byte planes for v_sad_u8; otherwise (pyramid
 // levels, whose 12-tap filter can overshoot that range) as 16-bit samples for
 // v_sad_u16.
+// The guide of block `local` of a level below the coarsest: twice the vector
+// of the block above it (clamped to the coarser level's grid).
+__device__ __forceinline__ dme_mv pel_guide(const JobDesc &J, int r, int level, int local) {
+    const int X = J.lxb[level], xb = local % X, yb = local / X;
+    const int GX = J.lxb[level + 1], GY = J.lyb[level + 1];
+    dme_mv g = J.lvec[r][level + 1][(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
+    g.x *= 2;
+    g.y *= 2;
+    return g;
+}
+
+// Blocks of k_pel_bulk per wave (1: no loop).  With more, a wave looks its
+// task up once and loads the guides of all its blocks together.
+#ifndef PW_PEL_BULK
+#define PW_PEL_BULK 1
+#endif
+
 template <bool U8>
-__global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                  int ntasks, int total, int level, PelLds ly) {
-    extern __shared__ int s_dyn[];
-    const int wv = wave_id(), lane = threadIdx.x & 63;
-    uint32_t *__restrict__ s_pic = reinterpret_cast<uint32_t *>(s_dyn) + wv * ly.wave();
-    uint32_t *__restrict__ s_reg0 = s_pic + ly.pic;
-    uint32_t *__restrict__ s_reg1 = s_reg0 + ly.reg;
-    int *__restrict__ s_tab = reinterpret_cast<int *>(s_pic + ly.pic + ly.keys);
-    const int gi = xcd_block() * 4 + wv;
-    if (gi >= total) return;  // each wave works alone
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
-    const JobDesc &J = jobs[tk.job];
-    const int r = tk.ref, local = gi - tk.offset;
+__device__ __forceinline__ void pel_bulk_block(const JobDesc &J, int r, int local, int level, dme_mv g,
+                                               const PelLds &ly, uint32_t *__restrict__ s_pic,
+                                               uint32_t *__restrict__ s_reg0, uint32_t *__restrict__ s_reg1,
+                                               int *__restrict__ s_tab) {
+    const int lane = threadIdx.x & 63;
     const int X = J.lxb[level];
     const int xb = local % X, yb = local / X;
     const int W = J.lw[level], H = J.lh[level];
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
     const int rad = pel_radius(level), R = rad + 1, D = 2 * R + 1, TS = D * D;
-    const int n = (2 * rad + 1) * (2 * rad + 1);
     const bool has_guide = level < J.depth;
     const int ntab = has_guide ? 2 : 1;
-    dme_mv g;
-    g.x = 0;
-    g.y = 0;
-    if (has_guide) {
-        const int GX = J.lxb[level + 1], GY = J.lyb[level + 1];
-        const dme_mv gv = J.lvec[r][level + 1][(size_t)clampi(yb >> 1, 0, GY - 1) * GX + clampi(xb >> 1, 0, GX - 1)];
-        g.x = gv.x * 2;
-        g.y = gv.y * 2;
-    }
+    if (!has_guide) g.x = g.y = 0;
     int *__restrict__ rec = J.ptab[r] + (size_t)wf_pos(xb, yb, X, J.lyb[level]) * PEL_REC;
     int best_sad = 10000000;
     dme_mv best_mv;
@@ -1142,6 +1142,56 @@ __global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jo
     }
 }
 
+template <bool U8>
+__global__ __launch_bounds__(256) void k_pel_bulk(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                  int ntasks, int total, int level, PelLds ly) {
+    extern __shared__ int s_dyn[];
+    const int wv = wave_id(), lane = threadIdx.x & 63;
+    uint32_t *__restrict__ s_pic = reinterpret_cast<uint32_t *>(s_dyn) + wv * ly.wave();
+    uint32_t *__restrict__ s_reg0 = s_pic + ly.pic;
+    uint32_t *__restrict__ s_reg1 = s_reg0 + ly.reg;
+    int *__restrict__ s_tab = reinterpret_cast<int *>(s_pic + ly.pic + ly.keys);
+#if PW_PEL_BULK > 1
+    const int g0 = (xcd_block() * 4 + wv) * PW_PEL_BULK;
+    if (g0 >= total) return;  // each wave works alone
+    const int g1 = min(g0 + PW_PEL_BULK, total);
+    int ti = find_task_wave(tasks, ntasks, g0);
+    // lane k < g1 - g0: the guide of block g0 + k (its own task lookup)
+    int gx = 0, gy = 0;
+    if (lane < g1 - g0) {
+        const int gi = g0 + lane;
+        int t = ti;
+        while (t + 1 < ntasks && tasks[t + 1].offset <= gi) ++t;
+        const Task tk = tasks[t];
+        const JobDesc &J = jobs[tk.job];
+        if (level < J.depth) {
+            const dme_mv g = pel_guide(J, tk.ref, level, gi - tk.offset);
+            gx = g.x;
+            gy = g.y;
+        }
+    }
+#pragma unroll 1
+    for (int gi = g0; gi < g1; ++gi) {
+        while (ti + 1 < ntasks && tasks[ti + 1].offset <= gi) ++ti;
+        const Task tk = tasks[ti];
+        dme_mv g;
+        g.x = __builtin_amdgcn_readlane(gx, gi - g0);
+        g.y = __builtin_amdgcn_readlane(gy, gi - g0);
+        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
+        pel_bulk_block<U8>(jobs[tk.job], tk.ref, gi - tk.offset, level, g, ly, s_pic, s_reg0, s_reg1, s_tab);
+    }
+#else
+    const int gi = xcd_block() * 4 + wv;
+    if (gi >= total) return;  // each wave works alone
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    dme_mv g;
+    g.x = g.y = 0;
+    if (level < J.depth) g = pel_guide(J, tk.ref, level, gi - tk.offset);
+    pel_bulk_block<U8>(J, tk.ref, gi - tk.offset, level, g, ly, s_pic, s_reg0, s_reg1, s_tab);
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Full search, table part (PixelMatcher::MatchPic with FullSearch,
 // pixel_match.cpp:91-155, 249-264, 365-370): no pyramid; at level 0 the zero
@@ -1220,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_pel_full(const JobDesc *__restrict__ jo
     uint32_t *__restrict__ s_reg = s_pic + ly.pic;
     const int gi = blockIdx.x;
     if (gi >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, local = gi - tk.offset;
     const int X = J.lxb[0], Y = J.lyb[0];
@@ -1807,7 +1857,7 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
     const int plane = qrows * qd;  // dwords of one phase region
     uint32_t *s_pic = s_sub + wv * (pp * prows + 16 * plane);
     uint32_t *s_q = s_pic + pp * prows;  // [fy * 4 + fx][row][qd]
-    const Task tk = tasks[find_task(tasks, ntasks, gi)];
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, k = gi - tk.offset;
     const int X = J.xnb, W = J.W, H = J.H, QW = W + 2;
@@ -2353,7 +2403,7 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
     const int g0 = (xcd_block() * 4 + wave_id()) * PW_MODE_L2, g1 = min(g0 + PW_MODE_L2, total);
     for (int gi = g0; gi < g1; ++gi) {
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
-        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
         const int k = gi - tk.offset;
         const int xb = k % J.xnb, yb = k / J.xnb;
@@ -2482,7 +2532,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
     const int g0 = (xcd_block() * 4 + wv) * PW_MODE_TABLES, g1 = min(g0 + PW_MODE_TABLES, total);
     for (int gi = g0; gi < g1; ++gi) {
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
-        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
         const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
         const int u = gi - tk.offset;
@@ -3012,7 +3062,7 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
         uint32_t *s_box = s_pic + ly.pp * ly.prows;
         uint32_t *s_zero = s_box + ly.rp * ly.brows;
         int *s_list = reinterpret_cast<int *>(s_zero + ly.zp * ly.zrows);  // up to 100 union positions
-        const Task tk = tasks[find_task(tasks, ntasks, gi)];
+        const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
         const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
         const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
@@ -3197,7 +3247,7 @@ __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jo
     extern __shared__ int s_col[];  // one int per column of the widest plane
     const int row = blockIdx.x;
     if (row >= total) return;
-    const Task tk = tasks[find_task(tasks, ntasks, row)];
+    const Task tk = tasks[find_task_wave(tasks, ntasks, row)];
     const JobDesc &J = jobs[tk.job];
     const int comp = tk.ref, yb = row - tk.offset;
     const int16_t *__restrict__ p = comp == 0 ? J.pic[0] : (comp == 1 ? J.cu : J.cv);
@@ -4120,6 +4170,49 @@ __global__ __launch_bounds__(256) void k_fast_eval(const JobDesc *__restrict__ j
     if (lane == 0) J.sbc[s] = sbc;
 }
 
+// The MEData arrays of a batch packed into one buffer, one workgroup per
+// array: 16-byte copies (device allocations are 256-aligned, offsets 16),
+// then the 2-byte tail.
+__global__ __launch_bounds__(256) void k_pack(const PackSeg *__restrict__ segs, uint8_t *__restrict__ dst) {
+    const PackSeg sg = segs[blockIdx.x];
+    const uint4 *__restrict__ s = reinterpret_cast<const uint4 *>(sg.src);
+    uint4 *__restrict__ d = reinterpret_cast<uint4 *>(dst + sg.off);
+    const unsigned long long n16 = sg.bytes / 16;
+    for (unsigned long long i = threadIdx.x; i < n16; i += 256) d[i] = s[i];
+    const unsigned long long t = (sg.bytes - 16 * n16) / 2;
+    if (threadIdx.x < t)
+        reinterpret_cast<uint16_t *>(dst + sg.off + 16 * n16)[threadIdx.x] =
+            reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(sg.src) + 16 * n16)[threadIdx.x];
+}
+
+// PictureCompressor::IntraModeAnalyse (picture_compress.cpp:300-320): the
+// INTRA blocks of each job's final Mode array, one workgroup per job.
+__global__ __launch_bounds__(256) void k_intra_count(const JobDesc *__restrict__ jobs) {
+    __shared__ int s_n;
+    const JobDesc &J = jobs[blockIdx.x];
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int n = J.xnb * J.ynb;
+    int c = 0;
+    for (int k = threadIdx.x; k < n; k += 256) c += J.mode[k] == DME_INTRA ? 1 : 0;
+    c = wave_sum(c);
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_n, c);
+    __syncthreads();
+    if (threadIdx.x == 0) *J.icount = s_n;
+}
+
+hipError_t launch_intra_count(const JobDesc *jobs, int njobs, hipStream_t st) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_intra_count, dim3(njobs), dim3(256), 0, st, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const PackSeg *segs, int nseg, uint8_t *dst, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3(nseg), dim3(256), 0, st, segs, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_downconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
@@ -4454,12 +4547,9 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     PelLds ly;
     const size_t lds = pel_bulk_lds(level, max_blen_x, max_blen_y, has_guide, ly);
     if (lds > kPelBulkLdsMax) return hipErrorInvalidValue;
-    if (u8)
-        hipLaunchKernelGGL(k_pel_bulk<true>, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total,
-                           level, ly);
-    else
-        hipLaunchKernelGGL(k_pel_bulk<false>, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total,
-                           level, ly);
+    const dim3 grid((total + 4 * PW_PEL_BULK - 1) / (4 * PW_PEL_BULK));
+    if (u8) hipLaunchKernelGGL(k_pel_bulk<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
+    else hipLaunchKernelGGL(k_pel_bulk<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
     return hipGetLastError();
 }
 

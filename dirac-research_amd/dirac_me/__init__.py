@@ -56,6 +56,7 @@ class MEData:
         self.sb_split = np.zeros((YS, XS), np.int32)
         self.sb_costs = np.zeros((YS, XS), np.float32)
         self.lambda_map = np.zeros((Y, X), np.float32)
+        self.intra_block_ratio = np.zeros(1, np.float32)  # IntraModeAnalyse (picture_compress.cpp:300-320)
 
     def struct(self):
         m = abi.dme_medata()
@@ -70,6 +71,7 @@ class MEData:
         m.sb_split = self.sb_split.ctypes.data
         m.sb_costs = self.sb_costs.ctypes.data
         m.lambda_map = self.lambda_map.ctypes.data
+        m.intra_block_ratio = self.intra_block_ratio.ctypes.data
         return m
 
     def arrays(self, nrefs=2):

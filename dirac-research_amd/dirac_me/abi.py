@@ -65,6 +65,7 @@ class dme_medata(C.Structure):
         ("vectors", C.c_void_p * 2), ("pred_costs", C.c_void_p * 2), ("mode", C.c_void_p),
         ("intra_costs", C.c_void_p), ("bipred_costs", C.c_void_p), ("dc", C.c_void_p * 3),
         ("sb_split", C.c_void_p), ("sb_costs", C.c_void_p), ("lambda_map", C.c_void_p),
+        ("intra_block_ratio", C.c_void_p),
     ]
 
 

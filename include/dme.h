@@ -126,6 +126,12 @@ typedef struct dme_medata {
     int32_t  *sb_split;        /* S  SBSplit                                  */
     float    *sb_costs;        /* S  SBCosts                                  */
     float    *lambda_map;      /* B  LambdaMap                                */
+    float    *intra_block_ratio; /* 1  MEData::IntraBlockRatio (a float,
+                                  * motion.h:356-359): INTRA blocks over all
+                                  * blocks of the final Mode array, divided
+                                  * in double (PictureCompressor::
+                                  * IntraModeAnalyse, picture_compress.cpp:
+                                  * 300-320)                                 */
 } dme_medata;
 
 typedef struct dme_ctx dme_ctx;

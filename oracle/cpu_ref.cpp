@@ -929,6 +929,14 @@ void export_medata(const MEData &me, dme_medata *out) {
     if (out->sb_split) for (size_t k = 0; k < S; ++k) out->sb_split[k] = me.split[k];
     if (out->sb_costs) for (size_t k = 0; k < S; ++k) out->sb_costs[k] = me.sbc[k];
     if (out->lambda_map) for (size_t k = 0; k < B; ++k) out->lambda_map[k] = me.lam[k];
+    // PictureCompressor::IntraModeAnalyse (picture_compress.cpp:300-320):
+    // INTRA blocks over all blocks, divided in double, kept as a float
+    // (MEData::SetIntraBlockRatio, motion.h:359)
+    if (out->intra_block_ratio) {
+        size_t c = 0;
+        for (size_t k = 0; k < B; ++k) c += me.mode[k] == DME_INTRA;
+        *out->intra_block_ratio = static_cast<float>(static_cast<double>(c) / static_cast<double>(B));
+    }
 }
 
 // ---------------------------------------------------------------------------

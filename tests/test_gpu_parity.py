@@ -120,6 +120,38 @@ def test_static_picture(ctx):
     assert ctx.stats().host_lambda_jobs == 1
 
 
+def test_intra_block_ratio(ctx):
+    # IntraModeAnalyse on the device (k_intra_count): references unrelated to
+    # the picture give many INTRA blocks; the ratio equals the oracle's and
+    # the count of the device's own Mode array, for the single-picture call
+    # and for a batch (the packed export)
+    cases = [(synth.config0(5), synth.config0(6)), (synth.config0(7), synth.config0(8))]
+    p = make_params(12, 12, 8, 8, precision=2, depth=3, lam=10.0)
+    g = dirac_me.geometry(p, 352, 288)
+    wants = []
+    for d, e in cases:
+        dd = dict(cur=d["cur"], u=d["u"], v=d["v"], refs=[e["cur"]])
+        want, _, _ = O.run_oracle(p, dd["cur"], dd["refs"], dd["u"], dd["v"])
+        got = _gpu(ctx, p, dd)
+        _assert_same(want, got, 1)
+        assert (got.mode == 0).sum() > 100
+        assert got.intra_block_ratio[0] == want.intra_block_ratio[0] == np.float32((got.mode == 0).sum() / got.mode.size)
+        wants.append(want)
+    jobs, outs = [], []
+    for j, (d, e) in enumerate(cases):
+        ctx.upload(6000 + 2 * j, e["cur"])
+        ctx.upload(6001 + 2 * j, d["cur"], d["u"], d["v"])
+        jobs.append((6001 + 2 * j, [6000 + 2 * j], p))
+        outs.append(dirac_me.MEData(g))
+    ctx.run_batch(jobs, outs)
+    for w, o in zip(wants, outs):
+        _assert_same(w, o, 1)
+        assert o.intra_block_ratio[0] == w.intra_block_ratio[0]
+    for j in range(2):
+        ctx.drop(6000 + 2 * j)
+        ctx.drop(6001 + 2 * j)
+
+
 def test_stagewise_entry_points(ctx):
     d = synth.config0(7)
     p = make_params(*d["block"], precision=2, depth=3, lam=10.0)

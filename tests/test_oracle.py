@@ -225,6 +225,70 @@ def test_sad_term_count_is_reported():
     assert 1e6 < out.sad_terms < 1e9
 
 
+def _pel_terms_static(g):
+    """N_sad of the pel search when the references equal the picture: every
+    list but the zero list of radius min(level + 1, 5) deduplicates away, so
+    the count is the sum over levels and blocks of (2r + 1)^2 Xl Yl
+    (SetBlockLimits, me_utils.cpp:58-78, clipped to the level)."""
+    xbl, ybl = g.luma_blen[2]
+    xbs, ybs = g.luma_bsep[2]
+    total = 0
+    for l in range(g.depth + 1):
+        W, H = g.level_width[l], g.level_height[l]
+        r = min(l + 1, 5)
+        for yb in range(g.level_yblocks[l]):
+            y0 = yb * ybs - (ybl - ybs) // 2
+            yl = min(y0 + ybl, H) - max(y0, 0)
+            for xb in range(g.level_xblocks[l]):
+                x0 = xb * xbs - (xbl - xbs) // 2
+                xl = min(x0 + xbl, W) - max(x0, 0)
+                if xl > 0 and yl > 0:
+                    total += (2 * r + 1) ** 2 * xl * yl
+    return total
+
+
+def test_pel_sad_term_count_closed_form():
+    # a static picture pins the counting rule of N_sad (the algorithmic work
+    # of the pel search) independently of the oracle's search code
+    from dirac_me import geometry
+    d = synth.config0(2)
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    out, _, _ = O.run_oracle(p, d["cur"], [d["cur"].copy()], d["u"], d["v"], stop=O.STOP_PEL)
+    assert out.sad_terms == _pel_terms_static(geometry(p, 352, 288))
+
+
+def test_sad_term_fixture_of_the_benchmark_pictures():
+    # tests/golden/nsad_1080p.json (make_nsad.py) holds the per-stage counts
+    # bench.py prices its SAD-term roofline with; recount one picture, and
+    # every picture stays below the analytic upper bounds of the stages
+    # (every list full, no deduplication)
+    z = json.load(open(os.path.join(GOLD, "nsad_1080p.json")))
+    row = z["frames"][0]
+    d = synth.config1(row["seed"])
+    p = make_params(*d["block"], precision=2, depth=4, lam=10.0)
+    pel = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], stop=O.STOP_PEL)[0].sad_terms
+    sub = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], stop=O.STOP_SUBPEL)[0].sad_terms
+    assert (pel, sub - pel) == (row["pel"], row["subpel"])
+    for r in z["frames"]:
+        assert 0 < r["pel"] < 458.1e6 and 0 < r["subpel"] < 98.0e6 and 0 < r["mode"] < 65.3e6
+
+
+@pytest.mark.parametrize("seeds", [(5, 6), (7, 8)])
+def test_intra_block_ratio_is_intra_modes_over_blocks(seeds):
+    # IntraModeAnalyse (picture_compress.cpp:300-320): INTRA blocks of the
+    # final Mode array over all blocks, divided in double and kept as the
+    # float of MEData::SetIntraBlockRatio.  References unrelated to the
+    # picture give a third of the blocks INTRA; the count is taken from the
+    # reference build's own Mode array where it exists
+    d, e = synth.config0(seeds[0]), synth.config0(seeds[1])
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    out, _, _ = O.run_oracle(p, d["cur"], [e["cur"]], d["u"], d["v"])
+    modes = O.run_ref(p, d["cur"], [e["cur"]], d["u"], d["v"])[0].mode if O.ref_available() else out.mode
+    n = int((modes == 0).sum())
+    assert n > 100
+    assert out.intra_block_ratio[0] == np.float32(n / modes.size)
+
+
 def test_mv_dtype_layout():
     assert MV_DTYPE.itemsize == 8
     assert DME_FP_MMX == 0
