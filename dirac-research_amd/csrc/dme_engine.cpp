@@ -63,7 +63,7 @@ hipError_t launch_lp_wta(const float *out, const float *lab, int npos, float alp
 hipError_t launch_lp_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
-                              int max_x1, int max_y1, hipStream_t st);
+                              int max_x1, int max_y1, bool eighth, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
                                 hipStream_t st);
 hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, hipStream_t st);
@@ -1013,14 +1013,16 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
                          [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, false, st); }));
     int mx0 = 1, my0 = 1, mx1 = 1, my1 = 1;
+    bool eighth = false;
     for (auto *j : js) {
+        eighth = eighth || j->p.mv_precision == 3;
         mx0 = std::max(mx0, j->g.blen[0][0]);
         my0 = std::max(my0, j->g.blen[0][1]);
         mx1 = std::max(mx1, j->g.blen[1][0]);
         my1 = std::max(my1, j->g.blen[1][1]);
     }
     DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches, [&] {
-        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, st);
+        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, eighth, st);
     }));
     DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
                          [&] { return launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, st); }));
@@ -2492,7 +2494,7 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
 // and costs them with the MMX build's float semantics.
 static int learned_params_ok(dme_ctx *ctx, const dme_params &p) {
     if (ctx->lp_form != DME_LP_FORM_FAST) return DME_OK;
-    if (p.mv_precision < 1)
+    if (p.mv_precision < 1 || p.mv_precision > 2)
         return set_error(DME_ERR_INVALID, "the fast learned form refines to half or quarter pel (mv_precision 1 or 2)");
     if (p.fp_semantics != DME_FP_MMX)
         return set_error(DME_ERR_INVALID, "the fast learned form uses the MMX build's float semantics");

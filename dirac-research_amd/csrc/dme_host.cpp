@@ -52,8 +52,8 @@ int exception_status() noexcept {
 // MakeMEDataHierarchy (pixel_match.cpp:185-220).
 int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
     if (p.chroma_format != 0) return set_error(DME_ERR_INVALID, "only 4:2:0 chroma is supported");
-    if (p.mv_precision < 0 || p.mv_precision > 2)
-        return set_error(DME_ERR_INVALID, "mv_precision must be 0 (pel), 1 (half) or 2 (quarter)");
+    if (p.mv_precision < 0 || p.mv_precision > 3)
+        return set_error(DME_ERR_INVALID, "mv_precision must be 0 (pel), 1 (half), 2 (quarter) or 3 (eighth)");
     if (p.fp_semantics != DME_FP_MMX && p.fp_semantics != DME_FP_C)
         return set_error(DME_ERR_INVALID, "fp_semantics must be DME_FP_MMX or DME_FP_C");
     if (p.combined_me)

@@ -1551,29 +1551,56 @@ __global__ __launch_bounds__(1024) void k_pel_wave(const JobDesc *__restrict__ j
 // Up-sampled SADs (BlockDiffHalfPel / QuarterPel, me_utils.cpp:410-867;
 // MMX simple_block_diff_up_mmx_4, me_utils_mmx.cpp:201-533).
 // step 1: mv in half pel, sample up[2y+mv.y][2x+mv.x]; step 2: quarter pel,
-// rounded average of 1, 2 or 4 half-pel samples.  Clamped reads equal both
-// reference paths' values.
+// rounded average of 1, 2 or 4 half-pel samples; step 3: eighth pel
+// (BlockDiffEighthPel, me_utils.cpp:890-1170), start mv >> 2 in the up
+// plane, weights (4-rx)(4-ry), rx(4-ry), (4-rx)ry, rx ry of the remainders,
+// +8 >> 4.  Clamped reads equal both reference paths' values.
 // ---------------------------------------------------------------------------
 struct UpStart {
     int rsx, rsy, rx, ry;
     bool bounds;
+    bool e;  // eighth pel: rx, ry in 0..3
 };
 
 __device__ __forceinline__ UpStart up_start(const Blk &b, dme_mv mv, int step, int UW, int UH) {
     UpStart s;
+    s.e = step == 3;
     if (step == 1) {
         s.rsx = (b.xp << 1) + mv.x; s.rsy = (b.yp << 1) + mv.y; s.rx = 0; s.ry = 0;
-    } else {
+    } else if (step == 2) {
         s.rsx = (b.xp << 1) + (mv.x >> 1); s.rsy = (b.yp << 1) + (mv.y >> 1);
         s.rx = mv.x & 1; s.ry = mv.y & 1;
+    } else {
+        s.rsx = (b.xp << 1) + (mv.x >> 2); s.rsy = (b.yp << 1) + (mv.y >> 2);
+        s.rx = mv.x & 3; s.ry = mv.y & 3;
     }
     s.bounds = s.rsx < 0 || s.rsx + (b.xl << 1) >= UW || s.rsy < 0 || s.rsy + (b.yl << 1) >= UH;
     return s;
 }
 
+// Up-plane sample (ux, uy), clamped, from the quarter-pel planes: their even
+// phases hold the up samples themselves (position (X, Y) phase (2 fx', 2 fy')
+// is up sample (2X + fx', 2Y + fy')).  Biased (v + 128).
+__device__ __forceinline__ int qp_raw(const uint32_t *__restrict__ qp, int W, int H, int ux, int uy) {
+    ux = clampi(ux, 0, 2 * W - 1);
+    uy = clampi(uy, 0, 2 * H - 1);
+    const uint32_t d = qp[(size_t)(((uy >> 1) + 1) * 4 + 2 * (uy & 1)) * (W + 2) + (ux >> 1) + 1];
+    return (int)((d >> (16 * (ux & 1))) & 0xffu);
+}
+
+// Eighth-pel sample (biased) at up position (ux + rx / 4, uy + ry / 4); the
+// bias commutes with the weights, which sum to 16.
+__device__ __forceinline__ int qp_eighth(const uint32_t *__restrict__ qp, int W, int H, int ux, int uy, int rx, int ry) {
+    const int a = qp_raw(qp, W, H, ux, uy), b = qp_raw(qp, W, H, ux + 1, uy);
+    const int c = qp_raw(qp, W, H, ux, uy + 1), d = qp_raw(qp, W, H, ux + 1, uy + 1);
+    return ((4 - rx) * (4 - ry) * a + rx * (4 - ry) * b + (4 - rx) * ry * c + rx * ry * d + 8) >> 4;
+}
+
 // Up-plane sample of pixel (dy, dx) for start s, read from the quarter-pel
-// planes: quarter-pel position (2 rsx + rx + 4 dx, 2 rsy + ry + 4 dy).
+// planes: quarter-pel position (2 rsx + rx + 4 dx, 2 rsy + ry + 4 dy), or
+// the eighth-pel sample at up position (rsx + 2 dx, rsy + 2 dy).
 __device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int H, const UpStart &s, int dy, int dx) {
+    if (s.e) return qp_eighth(qp, W, H, s.rsx + 2 * dx, s.rsy + 2 * dy, s.rx, s.ry) - 128;
     const int Xq = 2 * s.rsx + s.rx + 4 * dx, Yq = 2 * s.rsy + s.ry + 4 * dy;
     const int X = clampi(Xq >> 2, -1, W), Y = clampi(Yq >> 2, -1, H);
     const uint32_t d = qp[(size_t)((Y + 1) * 4 + (Yq & 3)) * (W + 2) + X + 1];
@@ -1832,7 +1859,12 @@ __global__ __launch_bounds__(1024) void k_lambda(const JobDesc *__restrict__ job
 #define SUB_SLOT 56                    // 49 SADs, lambda, pel vector, pel costs
 #define SUB_PP 8                       // packed picture row: 32 bytes
 
-__device__ __forceinline__ int sub_radius(int prec) { return prec == 2 ? 3 : 1; }
+// The sub-pel table: candidates within rad units of 1/2^tq pel around the
+// scaled integer vector, tq = min(prec, 2) (eighth-pel refinement tabulates
+// its half- and quarter-pel stages; its eighth-pel stage is computed on
+// demand).
+__device__ __forceinline__ int sub_radius(int prec) { return prec >= 2 ? 3 : 1; }
+__device__ __forceinline__ int sub_unit(int prec) { return prec < 2 ? prec : 2; }
 
 // Four qp dwords of consecutive positions (each the four horizontal phases
 // of one position) to the four phase dwords (each one phase of the four
@@ -1868,8 +1900,8 @@ __global__ __launch_bounds__(256) void k_subpel_bulk(const JobDesc *__restrict__
     int *__restrict__ rec = J.qtab[r] + (size_t)wf_pos(xb, yb, X, J.ynb) * SUB_SLOT;
     if (!b.empty() && prec > 0) {
         // candidate range in quarter pels and the integer offsets it spans
-        const int sq = 2 - prec;  // quarter pels per candidate unit
-        const int cqx = (pel.x << prec) << sq, cqy = (pel.y << prec) << sq, rq = rad << sq;
+        const int tq = sub_unit(prec), sq = 2 - tq;  // quarter pels per candidate unit
+        const int cqx = (pel.x << tq) << sq, cqy = (pel.y << tq) << sq, rq = rad << sq;
         const int qx0 = (cqx - rq) >> 2, qy0 = (cqy - rq) >> 2;
         const int RW = b.xl + ((cqx + rq) >> 2) - qx0, RH = b.yl + ((cqy + rq) >> 2) - qy0;
         // picture block from the byte plane; its loads go out with the first
@@ -2140,8 +2172,9 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                 } else {
                     refine = true;
                     if (!b.empty()) {
+                        // an eighth-pel predictor is always computed on demand
                         const int dx = pred.x - pel_up.x, dy = pred.y - pel_up.y;
-                        if (abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
+                        if (prec <= 2 && abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
                         else miss = true;
                     }
                 }
@@ -2180,7 +2213,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     const int e = (int)((__double_as_longlong(mmax) >> 52) & 0x7ff) - 1022;  // mmax < 2^e
                     may = N >= 32768 || lambda < 0.0f || low < e - 24;
                 }
-                if (may) s_chain[cf] = 1;
+                if (may || prec == 3) s_chain[cf] = 1;  // the eighth-pel stage always adds per pixel
             }
             // predictor SADs outside the table
             {
@@ -2218,7 +2251,8 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
             dme_mv bmv = pel;
             bool live = refine, bailed = false;
             const bool tests = !b.empty();
-            const int cx = pel.x << prec, cy = pel.y << prec;
+            const int tq = sub_unit(prec);  // the table's unit: 1/2^tq pel
+            const int cx = pel.x << tq, cy = pel.y << tq;
             for (int i = 1; i <= prec; ++i) {
                 if (live) {
                     bmv.x <<= 1;
@@ -2248,18 +2282,22 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     float mvc[4], st[4], tot[4];
                     bool need[4];
                     SubJob mine[4];
+                    // stages in half and quarter pel read the table; the
+                    // eighth-pel stage (BlockDiffEighthPel, per-pixel float
+                    // adds in both builds) is a chain round trip
+                    const bool tab = i <= tq;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         dme_mv mf;
                         mf.x = m[j].x << sh;
                         mf.y = m[j].y << sh;
-                        const int ti = (mf.y - cy + rad) * Q + (mf.x - cx + rad);
-                        sd[j] = on[j] ? mytab[ti] : 0;
+                        const int ti = tab ? ((m[j].y << (tq - i)) - cy + rad) * Q + ((m[j].x << (tq - i)) - cx + rad) : 0;
+                        sd[j] = on[j] && tab ? mytab[ti] : 0;
                         mvc[j] = var_up(pred, mf);
                         st[j] = mvc[j] * lambda;
                         tot[j] = (float)sd[j] + st[j];
                         const UpStart us = up_start(b, m[j], i, UW, UH);
-                        need[j] = on[j] && (us.bounds || c_sem) && !chain_is_exact(st[j], sd[j]);
+                        need[j] = on[j] && (!tab || ((us.bounds || c_sem) && !chain_is_exact(st[j], sd[j])));
                         mine[j].xp = b.xp;
                         mine[j].yp = b.yp;
                         mine[j].xl = b.xl;
@@ -2334,14 +2372,26 @@ __global__ __launch_bounds__(256) void k_shift_vectors(const JobDesc *__restrict
 // (defined with them below).
 __device__ int up8_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2);
 
-// BiBlock{Half,Quarter}Pel::Diff (me_utils.cpp:1172-1543): sum of
-// |((2 pic - r1) - r2) >> 1| with short intermediates.
+// The second reference's start of BiBlockEighthPel: its own, unless its
+// window takes the bounds-checked path, which reads it from the first
+// reference's start with its own remainders (me_utils.cpp:1764-1768).
+__device__ __forceinline__ UpStart bi_second(const UpStart &s1, UpStart s2) {
+    if (s2.e && s2.bounds) {
+        s2.rsx = s1.rsx;
+        s2.rsy = s1.rsy;
+    }
+    return s2;
+}
+
+// BiBlock{Half,Quarter,Eighth}Pel::Diff (me_utils.cpp:1172-1785): sum of
+// |((2 pic - r1) - r2) >> 1| with short intermediates (the eighth-pel
+// form's float adds of integers are exact).
 template <int NB = 8>
 __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int UW = 2 * J.W, UH = 2 * J.H;
     const UpStart s1 = up_start(b, m1, J.step, UW, UH);
-    const UpStart s2 = up_start(b, m2, J.step, UW, UH);
+    const UpStart s2 = bi_second(s1, up_start(b, m2, J.step, UW, UH));
     const int16_t *__restrict__ pic = J.pic[0];
     const uint32_t *__restrict__ q1 = J.qp[0];
     const uint32_t *__restrict__ q2 = J.qp[1];
@@ -2524,6 +2574,9 @@ struct MtLds {
     int pd, prows, spd, slot, wave;
 };
 
+// E8: some job of the launch estimates at eighth pel (those gathers need
+// registers the others do not).
+template <bool E8>
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                      int ntasks, int total, MtLds ly) {
     extern __shared__ uint32_t s_mt[];
@@ -2602,7 +2655,22 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
             for (int i = 0; i < 4; ++i) xo[i] = clampi(X0 + 4 * col + i, -1, W) + 1;
             uint32_t *slot = s_slot + (r * 4 + q) * ly.slot;
             uint32_t acc = 0;
-            for (int p0 = 0; p0 < npass; p0 += 4) {
+            // eighth pel: each sample from four up samples (no single phase)
+            const bool e8 = E8 && us.e;
+            for (int p = 0; e8 && p < npass; ++p) {
+                const int row = row0 + p * rpp;
+                if (on && row < b.yl) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        v |= (uint32_t)qp_eighth(qp, W, H, us.rsx + 2 * (4 * col + i), us.rsy + 2 * row, us.rx, us.ry)
+                             << (8 * i);
+                    v &= mask;
+                    acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], v, acc);
+                    if (level) slot[row * ly.spd + col] = v;
+                }
+            }
+            for (int p0 = 0; !e8 && p0 < npass; p0 += 4) {
                 uint32_t d[4][4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -2643,12 +2711,32 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
                 for (unsigned long long m1 = d1; m1; m1 &= m1 - 1) {
                     const int q2 = __ffsll((long long)m1) - 1;
                     const uint32_t *s0 = s_slot + q1 * ly.slot, *s1 = s_slot + (4 + q2) * ly.slot;
+                    // BiBlockEighthPel reads a second reference whose window
+                    // leaves the plane from the first one's start
+                    dme_mv v1, v2m;
+                    v1.x = __builtin_amdgcn_readlane(cv.x, q1);
+                    v1.y = __builtin_amdgcn_readlane(cv.y, q1);
+                    v2m.x = __builtin_amdgcn_readlane(cv.x, 16 + q2);
+                    v2m.y = __builtin_amdgcn_readlane(cv.y, 16 + q2);
+                    const UpStart u1 = up_start(b, v1, step, 2 * W, 2 * H);
+                    const UpStart u2 = up_start(b, v2m, step, 2 * W, 2 * H);
+                    const bool moved = E8 && u2.e && u2.bounds;
                     uint32_t acc = 0;
                     for (int p = 0; p < npass; ++p) {
                         const int row = row0 + p * rpp;
                         if (on && row < b.yl) {
                             const int o = row * ly.spd + col;
-                            acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], avg_up_u8(s0[o], s1[o]), acc);
+                            uint32_t v2 = s1[o];
+                            if (moved) {
+                                v2 = 0;
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+                                    v2 |= (uint32_t)qp_eighth(J.qp[1], W, H, u1.rsx + 2 * (4 * col + i), u1.rsy + 2 * row,
+                                                              u2.rx, u2.ry)
+                                          << (8 * i);
+                                v2 &= mask;
+                            }
+                            acc = __builtin_amdgcn_sad_u8(s_pic[row * ly.pd + col], avg_up_u8(s0[o], v2), acc);
                         }
                     }
                     const int sum = wave_sum((int)acc);
@@ -2685,7 +2773,7 @@ __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pr
         st[q] = mvc[q] * lambda;
         tot[q] = (float)gs[q] + st[q];
         const UpStart us = up_start(b, g[q], J.step, UW, UH);
-        need[q] = cand[q] && (us.bounds || J.fp_sem == DME_FP_C) && !chain_is_exact(st[q], gs[q]);
+        need[q] = cand[q] && (us.bounds || us.e || J.fp_sem == DME_FP_C) && !chain_is_exact(st[q], gs[q]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -4681,7 +4769,7 @@ static size_t mode_tables_lds(int max_x0, int max_y0, int max_x1, int max_y1, Mt
 }
 
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
-                              int max_x1, int max_y1, hipStream_t st) {
+                              int max_x1, int max_y1, bool eighth, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     if (max_x0 < 1 || max_y0 < 1 || max_x1 < 1 || max_y1 < 1 || max_x0 > 256 || max_x1 > 256) return hipErrorInvalidValue;
     MtLds ly;
@@ -4689,12 +4777,15 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
     if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mode_tables, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           159 * 1024);
-        if (e != hipSuccess) return e;
+        for (const void *f : {(const void *)k_mode_tables<false>, (const void *)k_mode_tables<true>}) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+            if (e != hipSuccess) return e;
+        }
         attr = true;
     }
-    hipLaunchKernelGGL(k_mode_tables, dim3((total + 4 * PW_MODE_TABLES - 1) / (4 * PW_MODE_TABLES)), dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    const dim3 grid((total + 4 * PW_MODE_TABLES - 1) / (4 * PW_MODE_TABLES));
+    if (eighth) hipLaunchKernelGGL(k_mode_tables<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    else hipLaunchKernelGGL(k_mode_tables<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }
 

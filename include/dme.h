@@ -69,8 +69,9 @@ typedef struct dme_cost {
  * derived from them there.                                                   */
 typedef struct dme_params {
     int32_t xblen, yblen, xbsep, ybsep;
-    int32_t mv_precision;  /* 0 pel, 1 half, 2 quarter (MVPrecisionType);
-                              3 (eighth) for motion compensation only       */
+    int32_t mv_precision;  /* 0 pel, 1 half, 2 quarter, 3 eighth
+                              (MVPrecisionType); the learned path's fast
+                              form takes 1 or 2                              */
     int32_t depth;         /* pyramid depth; < 0 selects the reference's own
                               formula floor(min(log2(W/12), log2(H/12)))
                               (pixel_match.cpp:94-95)                          */

@@ -99,8 +99,8 @@ int make_geo(const dme_params *p, int W, int H, Geo &g) {
     if (p->chroma_format != 0) return fail("only 4:2:0 is supported");
     if (p->xbsep <= 0 || p->ybsep <= 0 || p->xblen <= 0 || p->yblen <= 0)
         return fail("block lengths and separations must be positive");
-    if (p->mv_precision < 0 || p->mv_precision > 2)
-        return fail("mv_precision must be 0, 1 or 2");
+    if (p->mv_precision < 0 || p->mv_precision > 3)
+        return fail("mv_precision must be 0, 1, 2 or 3");
     if (W < 24 || H < 24) return fail("picture too small");
     g.W = W; g.H = H; g.CW = W >> 1; g.CH = H >> 1;
 
@@ -308,24 +308,32 @@ int pel_sad(const Plane &pic, const Plane &ref, const Blk &b, const MV &mv) {
 // ---------------------------------------------------------------------------
 // K4: up-sampled block SADs (me_utils.cpp:410-1170, MMX
 // simple_block_diff_up_mmx_4, me_utils_mmx.cpp:201-533).
-// prec_step 1 = BlockDiffHalfPel (mv in half pel), 2 = BlockDiffQuarterPel.
-// Sample value at quarter pel uses the rounded average of 1/2/4 half-pel
-// samples; the bounds path's weighted form (2-rx)(2-ry)... +2 >> 2 equals it.
+// prec_step 1 = BlockDiffHalfPel (mv in half pel), 2 = BlockDiffQuarterPel,
+// 3 = BlockDiffEighthPel.  Sample value at quarter pel uses the rounded
+// average of 1/2/4 half-pel samples; the bounds path's weighted form
+// (2-rx)(2-ry)... +2 >> 2 equals it.  At eighth pel the start is mv >> 2 in
+// the up plane and the weights (4-rx)(4-ry), rx(4-ry), (4-rx)ry, rx ry of
+// the remainders rx, ry in 0..3, +8 >> 4 (me_utils.cpp:890-1011).
 // ---------------------------------------------------------------------------
 
 struct UpSample {
     int rsx, rsy;   // ref start in the up plane
-    int rx, ry;     // quarter remainders (0 for half pel)
+    int rx, ry;     // quarter (0..1) or eighth (0..3) remainders (0 for half pel)
+    bool eighth;    // eighth-pel weights
     bool bounds;    // true when the reference takes the bounds-checked path
 };
 
 UpSample up_start(const Blk &b, const MV &mv, int step, int upw, int uph) {
     UpSample s;
+    s.eighth = step == 3;
     if (step == 1) {
         s.rsx = (b.xp << 1) + mv.x; s.rsy = (b.yp << 1) + mv.y; s.rx = 0; s.ry = 0;
-    } else {
+    } else if (step == 2) {
         s.rsx = (b.xp << 1) + (mv.x >> 1); s.rsy = (b.yp << 1) + (mv.y >> 1);
         s.rx = mv.x & 1; s.ry = mv.y & 1;
+    } else {
+        s.rsx = (b.xp << 1) + (mv.x >> 2); s.rsy = (b.yp << 1) + (mv.y >> 2);
+        s.rx = mv.x & 3; s.ry = mv.y & 3;
     }
     const int stopx = s.rsx + (b.xl << 1), stopy = s.rsy + (b.yl << 1);
     s.bounds = s.rsx < 0 || stopx >= upw || s.rsy < 0 || stopy >= uph;
@@ -336,10 +344,11 @@ inline int up_value(const Plane &up, const UpSample &s, int dy, int dx) {
     const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
     const int by = bchk(uy, up.h), bx = bchk(ux, up.w);
     const int by1 = bchk(uy + 1, up.h), bx1 = bchk(ux + 1, up.w);
-    const int w0 = (2 - s.rx) * (2 - s.ry), w1 = s.rx * (2 - s.ry);
-    const int w2 = (2 - s.rx) * s.ry, w3 = s.rx * s.ry;
-    return (w0 * up.at(by, bx) + w1 * up.at(by, bx1) + w2 * up.at(by1, bx) +
-            w3 * up.at(by1, bx1) + 2) >> 2;
+    const int one = s.eighth ? 4 : 2;
+    const int w0 = (one - s.rx) * (one - s.ry), w1 = s.rx * (one - s.ry);
+    const int w2 = (one - s.rx) * s.ry, w3 = s.rx * s.ry;
+    const int v = w0 * up.at(by, bx) + w1 * up.at(by, bx1) + w2 * up.at(by1, bx) + w3 * up.at(by1, bx1);
+    return s.eighth ? (v + 8) >> 4 : (v + 2) >> 2;
 }
 
 // Bail-out Diff with a start cost (BlockDiffHalfPel/QuarterPel::Diff with
@@ -348,7 +357,9 @@ inline int up_value(const Plane &up, const UpSample &s, int dy, int dx) {
 // float(int SAD) + start, one rounding (me_utils_mmx.cpp:270-278); the
 // bounds path (and the plain C build everywhere) adds per pixel in float.
 // fp_sem FP_ONE_ROUNDING: float(int SAD) + start for every block (the fast
-// form's own searches, which no reference path constrains).
+// form's own searches, which no reference path constrains).  Eighth pel has
+// no MMX form: it adds per pixel in float in both builds (me_utils.cpp:
+// 1013-1170).
 constexpr int FP_ONE_ROUNDING = 2;
 
 bool up_diff_update(const Plane &pic, const Plane &up, const Blk &b, const MV &mv, int step,
@@ -359,7 +370,7 @@ bool up_diff_update(const Plane &pic, const Plane &up, const Blk &b, const MV &m
     const float start_val = mvcost * lambda;
     terms += (double)b.xl * b.yl;
     float sum;
-    if ((!s.bounds && fp_sem == DME_FP_MMX) || fp_sem == FP_ONE_ROUNDING) {
+    if (((!s.bounds && fp_sem == DME_FP_MMX) || fp_sem == FP_ONE_ROUNDING) && !s.eighth) {
         int isad = 0;
         for (int dy = 0; dy < b.yl; ++dy)
             for (int dx = 0; dx < b.xl; ++dx)
@@ -395,12 +406,20 @@ float up_diff_plain(const Plane &pic, const Plane &up, const Blk &b, const MV &m
 
 // K5: BiBlockHalfPel / BiBlockQuarterPel::Diff (me_utils.cpp:1172-1543; MMX
 // me_utils_mmx.cpp:540-1048): sum |((2*pic - r1) - r2) >> 1| in short
-// intermediates, returned as float of the integer sum.
+// intermediates, returned as float of the integer sum.  BiBlockEighthPel
+// (me_utils.cpp:1545-1785) is the same sum of eighth-pel samples (its float
+// adds of integers are exact), except that when the second reference takes
+// the bounds-checked path it is read from the first reference's start with
+// its own weights (the loop starts at ref_start1, :1764-1768).
 float bi_diff(const Plane &pic, const Plane &up1, const Plane &up2, const Blk &b, const MV &m1,
               const MV &m2, int step, double &terms) {
     if (b.empty()) return 0.f;
     const UpSample s1 = up_start(b, m1, step, up1.w, up1.h);
-    const UpSample s2 = up_start(b, m2, step, up2.w, up2.h);
+    UpSample s2 = up_start(b, m2, step, up2.w, up2.h);
+    if (step == 3 && s2.bounds) {
+        s2.rsx = s1.rsx;
+        s2.rsy = s1.rsy;
+    }
     terms += 2.0 * b.xl * b.yl;
     int sum = 0;
     for (int dy = 0; dy < b.yl; ++dy)
@@ -1566,7 +1585,7 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
 extern "C" int oracle_fast_me(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
                               const dme_mv *const *v0, dme_medata *out, dme_mv *const *v2_out, double *sad_terms) {
     if (!cur || !cur->y || !refs || !v0 || nrefs < 1 || nrefs > 2) return fail("bad arguments");
-    if (p->mv_precision < 1 || p->fp_semantics != DME_FP_MMX || p->full_search)
+    if (p->mv_precision < 1 || p->mv_precision > 2 || p->fp_semantics != DME_FP_MMX || p->full_search)
         return fail("the fast form runs half or quarter pel, MMX semantics, hierarchical search");
     Geo g;
     int rc = make_geo(p, cur->width, cur->height, g);

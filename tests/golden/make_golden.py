@@ -4,7 +4,8 @@ own ME code, compiled from its sources by oracle/Makefile (target `ref`).
 Run in a container that has the reference tree:  python tests/golden/make_golden.py
 (--mc-only rewrites only the motion-compensation fixture, mc.npz; --full-only
 only the full-search fixture, full_search.npz; --imt-only only the .imt
-diagnostics fixture, me_cif.imt and imt.json).
+diagnostics fixture, me_cif.imt and imt.json; --eighth-only only the
+eighth-pel fixture, eighth.npz).
 
 Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
 seed by the deterministic generator in libdme.so), the parameters, and the
@@ -129,6 +130,38 @@ def main_full():
     np.savez_compressed(os.path.join(HERE, "full_search.npz"), meta=json.dumps(index), **out)
 
 
+# eighth-pel estimation: (name, seed, size, nrefs, depth, lambda, fp, variant);
+# two-reference cases exercise BiBlockEighthPel's bounds-checked read of the
+# second reference (me_utils.cpp:1764-1768)
+EIGHTH_CASES = [
+    ("e_cif1_s0", 0, "cif", 1, 3, 10.0, DME_FP_MMX, "mmx"),
+    ("e_cif2_s3", 3, "cif", 2, 3, 10.0, DME_FP_MMX, "mmx"),
+    ("e_cif2_s7_l10.7", 7, "cif", 2, 3, 10.7, DME_FP_MMX, "mmx"),
+    ("e_cif2_s3_c", 3, "cif", 2, 3, 10.0, DME_FP_C, "c"),
+    ("e_720p2_s5", 5, "720p", 2, 4, 10.0, DME_FP_MMX, "mmx"),
+]
+
+
+def main_eighth():
+    """Eighth-pel fixtures (eighth.npz): the reference's live sequence at
+    MV_PRECISION_EIGHTH_PIXEL (BlockDiffEighthPel, BiBlockEighthPel); full
+    MEData of the CIF cases, digests plus mode and superblock arrays of the
+    720p case."""
+    out, index = {}, {}
+    for name, seed, size, nrefs, depth, lam, fp, variant in EIGHTH_CASES:
+        d = synth.sized(seed, size, nrefs)
+        p = make_params(*d["block"], precision=3, depth=depth, lam=lam, fp=fp)
+        res, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], variant=variant)
+        arrs = res.arrays(nrefs)
+        index[name] = dict(seed=seed, size=size, nrefs=nrefs, depth=depth, lam=lam, fp=fp, variant=variant,
+                           inputs_sha256=inputs_digest(d), sha256={k: sha(v) for k, v in arrs.items()})
+        for k, a in arrs.items():
+            if size == "cif" or k in ("mode", "sb_split", "sb_costs"):
+                out[name + "/" + k] = a
+        print("wrote", name)
+    np.savez_compressed(os.path.join(HERE, "eighth.npz"), meta=json.dumps(index), **out)
+
+
 def main_mc():
     """Motion-compensation fixtures (mc.npz): for each case in mc_cases.CASES,
     the MvData from the reference's own motion estimator (or scrambled), and
@@ -198,8 +231,11 @@ if __name__ == "__main__":
         main_imt()
     elif "--mc-only" in sys.argv:
         main_mc()
+    elif "--eighth-only" in sys.argv:
+        main_eighth()
     else:
         main()
         main_mc()
         main_full()
         main_imt()
+        main_eighth()

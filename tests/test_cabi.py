@@ -49,8 +49,9 @@ def test_geometry_matches_oracle(W, H, blk, depth):
 
 def test_geometry_rejects_bad_params():
     with pytest.raises(dirac_me.DmeError) as e:
-        dirac_me.geometry(make_params(precision=3), 352, 288)
+        dirac_me.geometry(make_params(precision=4), 352, 288)
     assert e.value.code == abi.DME_ERR_INVALID
+    dirac_me.geometry(make_params(precision=3), 352, 288)  # eighth pel
     with pytest.raises(dirac_me.DmeError):
         dirac_me.geometry(make_params(depth=7), 352, 288)  # levels below 12 samples
     with pytest.raises(dirac_me.DmeError):

@@ -2,12 +2,18 @@
 on every MEData array, at the workload sizes and at the edges the reference
 handles (overhanging block grids, pictures that do not move, one and two
 references, every MV precision, both float semantics)."""
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
 import oracle_py as O
 import dirac_me
 from dirac_me import DME_FP_C, DME_FP_MMX, make_params, synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -150,6 +156,52 @@ def test_intra_block_ratio(ctx):
     for j in range(2):
         ctx.drop(6000 + 2 * j)
         ctx.drop(6001 + 2 * j)
+
+
+EIGHTH = ["e_cif1_s0", "e_cif2_s3", "e_cif2_s7_l10.7", "e_cif2_s3_c", "e_720p2_s5"]
+
+
+@pytest.mark.parametrize("name", EIGHTH)
+def test_eighth_pel_matches_reference_fixture(ctx, name):
+    # eighth-pel estimation (BlockDiffEighthPel: per-pixel float adds in both
+    # builds; BiBlockEighthPel with its bounds-checked read of the second
+    # reference from the first one's start) against the reference's own
+    # output (tests/golden/eighth.npz)
+    z = np.load(os.path.join(GOLD, "eighth.npz"))
+    meta = json.loads(str(z["meta"]))[name]
+    d = synth.sized(meta["seed"], meta["size"], meta["nrefs"])
+    p = make_params(*d["block"], precision=3, depth=meta["depth"], lam=meta["lam"], fp=meta["fp"])
+    print("eighth-pel case", name, flush=True)
+    got = _gpu(ctx, p, d)
+    for k, v in got.arrays(meta["nrefs"]).items():
+        assert hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest() == meta["sha256"][k], k
+
+
+def test_eighth_pel_1080p_and_mixed_batch(ctx):
+    # a 1080p B picture at eighth pel against the oracle, and a batch that
+    # mixes eighth- and quarter-pel pictures (one table launch serves both)
+    d = synth.config1(106)
+    p3 = make_params(*d["block"], precision=3, depth=4, lam=10.0)
+    want, _, _ = O.run_oracle(p3, d["cur"], d["refs"], d["u"], d["v"])
+    _assert_same(want, _gpu(ctx, p3, d), 2, "1080p eighth")
+    cases = [(synth.sized(21, "cif", 2), 3), (synth.sized(22, "cif", 2), 2), (synth.sized(23, "cif", 1), 3)]
+    jobs, outs, wants = [], [], []
+    for j, (e, prec) in enumerate(cases):
+        pe = make_params(*e["block"], precision=prec, depth=e["depth"], lam=10.0)
+        base = 7000 + 3 * j
+        nums = [base, base + 2][: len(e["refs"])]
+        for n, r in zip(nums, e["refs"]):
+            ctx.upload(n, r)
+        ctx.upload(base + 1, e["cur"], e["u"], e["v"])
+        jobs.append((base + 1, nums, pe))
+        outs.append(dirac_me.MEData(dirac_me.geometry(pe, 352, 288)))
+        wants.append(O.run_oracle(pe, e["cur"], e["refs"], e["u"], e["v"])[0])
+    ctx.run_batch(jobs, outs)
+    for j, (w, o) in enumerate(zip(wants, outs)):
+        _assert_same(w, o, len(cases[j][0]["refs"]), "batch job %d" % j)
+    for pn, nums, _ in jobs:
+        for n in [pn] + nums:
+            ctx.drop(n)
 
 
 def test_stagewise_entry_points(ctx):

@@ -289,6 +289,44 @@ def test_intra_block_ratio_is_intra_modes_over_blocks(seeds):
     assert out.intra_block_ratio[0] == np.float32(n / modes.size)
 
 
+def _eighth(name):
+    z = np.load(os.path.join(GOLD, "eighth.npz"))
+    meta = json.loads(str(z["meta"]))[name]
+    d = synth.sized(meta["seed"], meta["size"], meta["nrefs"])
+    digest = _sha(np.concatenate([d["cur"].ravel()] + [r.ravel() for r in d["refs"]] + [d["u"].ravel(), d["v"].ravel()]))
+    assert digest == meta["inputs_sha256"], "synthetic generator changed: regenerate the fixtures"
+    p = make_params(*d["block"], precision=3, depth=meta["depth"], lam=meta["lam"], fp=meta["fp"])
+    return meta, z, d, p
+
+
+EIGHTH = ["e_cif1_s0", "e_cif2_s3", "e_cif2_s7_l10.7", "e_cif2_s3_c", "e_720p2_s5"]
+
+
+@pytest.mark.parametrize("name", EIGHTH)
+def test_oracle_eighth_pel_matches_reference_fixture(name):
+    # BlockDiffEighthPel (per-pixel float adds in both builds) and
+    # BiBlockEighthPel with its bounds-checked read of the second reference
+    # from the first reference's start (me_utils.cpp:890-1170, 1545-1785)
+    meta, z, d, p = _eighth(name)
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    for k, v in got.arrays(meta["nrefs"]).items():
+        assert _sha(v) == meta["sha256"][k], k
+        if name + "/" + k in z:
+            assert np.array_equal(v.view(np.uint8), z[name + "/" + k].view(np.uint8)), k
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="the reference build (oracle/_ref) is absent")
+@pytest.mark.parametrize("seed,size,nrefs,fp", [(11, "cif", 2, DME_FP_MMX), (12, "cif", 2, DME_FP_C),
+                                                (13, "cif", 1, DME_FP_MMX), (9, "720p", 2, DME_FP_C)])
+def test_oracle_eighth_pel_matches_reference_live(seed, size, nrefs, fp):
+    d = synth.sized(seed, size, nrefs)
+    p = make_params(*d["block"], precision=3, depth=d["depth"], lam=10.0, fp=fp)
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
+    want, _, _ = O.run_ref(p, d["cur"], d["refs"], d["u"], d["v"], variant="mmx" if fp == DME_FP_MMX else "c")
+    for k, v in want.arrays(nrefs).items():
+        assert v.tobytes() == got.arrays(nrefs)[k].tobytes(), k
+
+
 def test_mv_dtype_layout():
     assert MV_DTYPE.itemsize == 8
     assert DME_FP_MMX == 0
