@@ -1476,6 +1476,38 @@ extern "C" int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda
     return dme::exception_status();
 }
 
+extern "C" int dme_load_medata(dme_ctx *ctx, int pic_num, const int *refs, int nrefs, const dme_medata *in,
+                               int stage) try {
+    CTX_CHECK(ctx);
+    if (!ctx->params_set) return set_error(DME_ERR_STATE, "dme_set_params was not called");
+    if (!in || stage < 1 || stage > 3) return set_error(DME_ERR_INVALID, "NULL MEData or stage outside 1..3");
+    for (int r = 0; r < nrefs && r < 2; ++r)
+        if (!in->vectors[r] || !in->pred_costs[r])
+            return set_error(DME_ERR_INVALID, "the vectors and PredCosts of every reference are needed");
+    if (stage >= 2 && !in->lambda_map) return set_error(DME_ERR_INVALID, "stages 2 and 3 need the lambda map");
+    JobState *J = job_for(ctx, pic_num);
+    DME_TRY(prepare_job(ctx, *J, ctx->params, pic_num, refs, nrefs));
+    const size_t B = (size_t)J->g.xnb * J->g.ynb;
+    for (int r = 0; r < J->nrefs; ++r) {
+        HIP_TRY(hipMemcpy(J->vec[r].p, in->vectors[r], sizeof(dme_mv) * B, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(J->pc[r].p, in->pred_costs[r], sizeof(dme_cost) * B, hipMemcpyHostToDevice));
+    }
+    J->stage = 1;
+    if (stage >= 2) {
+        std::vector<JobState *> js{J};
+        const float *ov[1] = {in->lambda_map};
+        ctx->L->arena_used = 0;
+        DME_TRY(lambda_flag_slots(ctx, js));
+        int launches = 0;
+        DME_TRY(stage_lambda(ctx, js, ov, true, launches));
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+    }
+    J->stage = stage;
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
 extern "C" int dme_subpel(dme_ctx *ctx, int pic_num, dme_medata *out) try {
     CTX_CHECK(ctx);
     auto it = ctx->jobs.find(pic_num);

@@ -138,6 +138,12 @@ class Context:
         check(lib().dme_set_lambda_map(self.h, pic_num, ptr(lm), C.byref(out.struct()) if out else None))
         return out
 
+    def load_medata(self, pic_num, refs, me, stage):
+        """Hands the MEData state of `stage` (1 after the pel search, 2 with
+        the lambda map, 3 after sub-pel refinement) back to the device."""
+        r = (C.c_int * len(refs))(*refs)
+        check(lib().dme_load_medata(self.h, pic_num, r, len(refs), C.byref(me.struct()), stage))
+
     def subpel(self, pic_num, out=None):
         check(lib().dme_subpel(self.h, pic_num, C.byref(out.struct()) if out else None))
         return out

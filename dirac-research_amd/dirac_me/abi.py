@@ -138,6 +138,7 @@ SIGNATURES = {
     "dme_pixel_search": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_set_lambda_map": ([_P, _I, _P, C.POINTER(dme_medata)], _I),
     "dme_subpel": ([_P, _I, C.POINTER(dme_medata)], _I),
+    "dme_load_medata": ([_P, _I, _P, _I, C.POINTER(dme_medata), _I], _I),
     "dme_mode_decide": ([_P, _I, C.POINTER(dme_medata)], _I),
     "dme_motion_estimate": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_run_batch": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),

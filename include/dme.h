@@ -272,6 +272,19 @@ int dme_pixel_search(dme_ctx *ctx, int pic_num, const int *refs, int nrefs,
 int dme_set_lambda_map(dme_ctx *ctx, int pic_num, const float *lambda_map,
                        dme_medata *out);
 
+/* The MEData state the next stage reads, from the caller: the reference's
+ * classes read it from the MEData the caller owns (PictureCompressor edits
+ * it between stages: SetLambdaMap, and the pixel-precision doubling of
+ * picture_compress.cpp:240-259), so a stage-wise caller can hand it back.
+ * Prepares pic_num's estimation against refs under the context's current
+ * parameters (dme_set_params), then loads, from `in`, the vectors and
+ * PredCosts of every reference (stage 1: after PixelMatcher::DoSearch),
+ * and the lambda map (stage 2: after MEData::SetLambdaMap), or the same
+ * after sub-pel refinement (stage 3: ModeDecider::DoModeDecn follows).
+ * Vectors are in the units of the current mv_precision (pel at stages 1-2). */
+int dme_load_medata(dme_ctx *ctx, int pic_num, const int *refs, int nrefs,
+                    const dme_medata *in, int stage);
+
 /* SubpelRefine::DoSubpel(EncQueue&, pic_num) (me_subpel.h:77,
  * me_subpel.cpp:65-99), or the pixel-precision vector doubling of
  * PictureCompressor::SubPixelME (picture_compress.cpp:240-259).            */

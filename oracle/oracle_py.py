@@ -20,6 +20,9 @@ from dirac_me import MEData  # noqa: E402
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libdirac_ref.so")
 REF_C_SO = os.path.join(HERE, "_ref", "libdirac_ref_c.so")
+# the reference harness with the drop-in classes (dirac-research_amd/shim) in
+# place of its ME classes; links libdme.so, so it runs where a GPU is
+REF_DROPIN_SO = os.path.join(HERE, "_ref", "libdirac_dropin.so")
 
 STOP_PEL, STOP_LAMBDA, STOP_SUBPEL, STOP_MODE = 1, 2, 3, 4
 
@@ -227,6 +230,33 @@ def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, va
     if rc != 0:
         raise RuntimeError("reference harness error %d" % rc)
     return out, cap, geom
+
+
+def dropin_available():
+    return os.path.exists(REF_DROPIN_SO)
+
+
+def run_dropin(params, cur, refs, u=None, v=None, pic_nums=None):
+    """The reference encoder's ME sequence (PictureCompressor's steps) with
+    the drop-in PixelMatcher, SubpelRefine and ModeDecider: the MEData the
+    reference's own objects hold afterwards."""
+    if "dropin" not in _ref:
+        l = _load(REF_DROPIN_SO)
+        if l is None:
+            raise RuntimeError("oracle/_ref/libdirac_dropin.so missing (built where the reference tree is)")
+        l.refh_me_dropin.argtypes = [C.POINTER(abi.dme_params), C.POINTER(oracle_picture), C.POINTER(oracle_picture),
+                                     C.c_int, C.POINTER(abi.dme_medata), C.c_char_p, C.c_int]
+        _ref["dropin"] = l
+    l = _ref["dropin"]
+    err = C.create_string_buffer(1024)
+
+    def fn(pp, cp, rp, n, stop_, ms, cap):
+        return l.refh_me_dropin(pp, cp, rp, n, ms, err, len(err))
+
+    rc, out, _, geom = _run(fn, params, cur, refs, u, v, STOP_MODE, False, pic_nums=pic_nums)
+    if rc != 0:
+        raise RuntimeError("drop-in run failed: " + err.value.decode(errors="replace"))
+    return out
 
 
 def downconvert(fn_lib, arr, ow, oh, name):

@@ -33,12 +33,17 @@
 #include <libdirac_motionest/pixel_match.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <exception>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "cpu_ref.h"
+#ifdef DME_DROPIN
+#include "../dirac-research_amd/shim/dme_dropin.h"
+#endif
 
 using namespace dirac;
 
@@ -220,50 +225,64 @@ extern "C" int refh_upconvert(const int16_t *in, int w, int h, int16_t *out) {
     return 0;
 }
 
+// The encoder state refh_me and refh_me_dropin start from: EncoderParams,
+// the pictures in an EncQueue and the current picture's MEData, as
+// SequenceCompressor sets them up.
+struct MeSetup {
+    EncoderParams encp;
+    EncQueue q;
+    MEData *me_;
+    MeSetup(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs)
+        : encp(VIDEO_FORMAT_CUSTOM, INTER_PICTURE, 2, false), me_(0) {
+        const int W = cur->width, H = cur->height;
+        encp.SetFullSearch(p->full_search != 0);
+        encp.SetCombinedME(p->combined_me != 0);
+        encp.SetXRangeME(p->full_search ? p->x_range_me : 32);
+        encp.SetYRangeME(p->full_search ? p->y_range_me : 32);
+
+        // references first, as intra reference pictures
+        for (int r = 0; r < nrefs; ++r) {
+            PictureParams pp(format420, W, H, 8, 8);
+            pp.SetPictureNum(refs[r].pic_num);
+            pp.SetPicSort(PictureSort::IntraRefPictureSort());
+            q.PushPicture(pp);
+            EncPicture &e = q.GetPicture(refs[r].pic_num);
+            fill(e.Data(Y_COMP), refs[r].y);
+            if (refs[r].u) fill(e.Data(U_COMP), refs[r].u);
+            if (refs[r].v) fill(e.Data(V_COMP), refs[r].v);
+            e.SetOrigData();
+        }
+        PictureParams cp(format420, W, H, 8, 8);
+        cp.SetPictureNum(cur->pic_num);
+        cp.SetPicSort(PictureSort::InterRefPictureSort());
+        for (int r = 0; r < nrefs; ++r) cp.Refs().push_back(refs[r].pic_num);
+        q.PushPicture(cp);
+        EncPicture &E = q.GetPicture(cur->pic_num);
+        fill(E.Data(Y_COMP), cur->y);
+        if (cur->u) fill(E.Data(U_COMP), cur->u);
+        if (cur->v) fill(E.Data(V_COMP), cur->v);
+        E.SetOrigData();
+
+        // Block parameters and MEData as SequenceCompressor sets them
+        // (seq_compress.cpp:107-133, 266)
+        PicturePredParams pp;
+        pp.SetBlockSizes(OLBParams(p->xblen, p->yblen, p->xbsep, p->ybsep), format420);
+        pp.SetMVPrecision(MVPrecisionType(p->mv_precision));
+        pp.SetXNumSB((W + pp.LumaBParams(0).Xbsep() - 1) / pp.LumaBParams(0).Xbsep());
+        pp.SetYNumSB((H + pp.LumaBParams(0).Ybsep() - 1) / pp.LumaBParams(0).Ybsep());
+        pp.SetXNumBlocks(4 * pp.XNumSB());
+        pp.SetYNumBlocks(4 * pp.YNumSB());
+        E.InitMEData(pp, nrefs);
+        me_ = &E.GetMEData();
+    }
+};
+
 extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
                        int stop_after, dme_medata *out, oracle_capture *cap) {
-    const int W = cur->width, H = cur->height;
-    EncoderParams encp(VIDEO_FORMAT_CUSTOM, INTER_PICTURE, 2, false);
-    encp.SetFullSearch(p->full_search != 0);
-    encp.SetCombinedME(false);
-    encp.SetXRangeME(p->full_search ? p->x_range_me : 32);
-    encp.SetYRangeME(p->full_search ? p->y_range_me : 32);
-
-    EncQueue q;
-    // references first, as intra reference pictures
-    for (int r = 0; r < nrefs; ++r) {
-        PictureParams pp(format420, W, H, 8, 8);
-        pp.SetPictureNum(refs[r].pic_num);
-        pp.SetPicSort(PictureSort::IntraRefPictureSort());
-        q.PushPicture(pp);
-        EncPicture &e = q.GetPicture(refs[r].pic_num);
-        fill(e.Data(Y_COMP), refs[r].y);
-        if (refs[r].u) fill(e.Data(U_COMP), refs[r].u);
-        if (refs[r].v) fill(e.Data(V_COMP), refs[r].v);
-        e.SetOrigData();
-    }
-    PictureParams cp(format420, W, H, 8, 8);
-    cp.SetPictureNum(cur->pic_num);
-    cp.SetPicSort(PictureSort::InterRefPictureSort());
-    for (int r = 0; r < nrefs; ++r) cp.Refs().push_back(refs[r].pic_num);
-    q.PushPicture(cp);
-    EncPicture &E = q.GetPicture(cur->pic_num);
-    fill(E.Data(Y_COMP), cur->y);
-    if (cur->u) fill(E.Data(U_COMP), cur->u);
-    if (cur->v) fill(E.Data(V_COMP), cur->v);
-    E.SetOrigData();
-
-    // Block parameters and MEData as SequenceCompressor sets them
-    // (seq_compress.cpp:107-133, 266)
-    PicturePredParams pp;
-    pp.SetBlockSizes(OLBParams(p->xblen, p->yblen, p->xbsep, p->ybsep), format420);
-    pp.SetMVPrecision(MVPrecisionType(p->mv_precision));
-    pp.SetXNumSB((W + pp.LumaBParams(0).Xbsep() - 1) / pp.LumaBParams(0).Xbsep());
-    pp.SetYNumSB((H + pp.LumaBParams(0).Ybsep() - 1) / pp.LumaBParams(0).Ybsep());
-    pp.SetXNumBlocks(4 * pp.XNumSB());
-    pp.SetYNumBlocks(4 * pp.YNumSB());
-    E.InitMEData(pp, nrefs);
-    MEData &me = E.GetMEData();
+    MeSetup S(p, cur, refs, nrefs);
+    EncoderParams &encp = S.encp;
+    EncQueue &q = S.q;
+    MEData &me = *S.me_;
 
     // PixelME (picture_compress.cpp:76-80)
     if (p->depth < 0 || p->full_search) {
@@ -311,6 +330,57 @@ extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const ora
             if (cap->ref_up[r]) dump(q.GetPicture(refs[r].pic_num).UpDataForME(false), cap->ref_up[r]);
     return 0;
 }
+
+#ifdef DME_DROPIN
+// The same sequence with the drop-in classes of dirac-research_amd/shim in
+// place of PixelMatcher, SubpelRefine and ModeDecider (the GPU path through
+// the C ABI), the caller's own steps between them unchanged: SetLambdaMap
+// and the pixel-precision doubling and halving of PictureCompressor
+// (picture_compress.cpp:76-80, 208-298).  The reference formula's depth.
+extern "C" int refh_me_dropin(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
+                              dme_medata *out, char *err, int err_len) {
+    try {
+        MeSetup S(p, cur, refs, nrefs);
+        EncQueue &q = S.q;
+        MEData &me = *S.me_;
+        const int pic = cur->pic_num;
+        {
+            dme_dropin::PixelMatcher pm(S.encp);
+            pm.DoSearch(q, pic);
+        }
+        me.SetLambdaMap(nrefs, p->lambda);
+        const MVPrecisionType orig = me.GetPicPredParams().MVPrecision();
+        if (orig != MV_PRECISION_PIXEL) {
+            dme_dropin::SubpelRefine sr(S.encp);
+            sr.DoSubpel(q, pic);
+        } else {
+            for (int r = 1; r <= nrefs; ++r) {
+                MvArray &a = me.Vectors(r);
+                for (int j = 0; j < a.LengthY(); ++j)
+                    for (int i = 0; i < a.LengthX(); ++i) a[j][i] = a[j][i] << 1;
+            }
+            me.GetPicPredParams().SetMVPrecision(MV_PRECISION_HALF_PIXEL);
+        }
+        {
+            dme_dropin::ModeDecider md(S.encp);
+            md.DoModeDecn(q, pic);
+        }
+        if (orig == MV_PRECISION_PIXEL) {
+            for (int r = 1; r <= nrefs; ++r) {
+                MvArray &a = me.Vectors(r);
+                for (int j = 0; j < a.LengthY(); ++j)
+                    for (int i = 0; i < a.LengthX(); ++i) a[j][i] = a[j][i] >> 1;
+            }
+            me.GetPicPredParams().SetMVPrecision(MV_PRECISION_PIXEL);
+        }
+        dump_medata(me, nrefs, out);
+        return 0;
+    } catch (const std::exception &e) {
+        std::snprintf(err, (size_t)err_len, "%s", e.what());
+        return -1;
+    }
+}
+#endif
 
 extern "C" int refh_lambda_map(int xnb, int ynb, int nrefs, const dme_mv *v1, const dme_mv *v2, float lambda,
                                float *out) {
