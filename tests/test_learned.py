@@ -131,6 +131,59 @@ def test_gpu_conv_forward_and_data_gradient_match_oracle(gctx):
 
 
 @pytest.mark.gpu
+def test_gpu_weight_and_bias_gradient_match_oracle(gctx):
+    # dme_lp_wgrad alone on BF16 inputs: every product is exact in f32, so
+    # each element of dw and db agrees with the float64 sum to the f32
+    # accumulation error of its own terms (a mis-indexed tap or channel is
+    # off by whole terms)
+    from dirac_me import learned as L
+    from dirac_me.abi import check, lib
+
+    rng = np.random.default_rng(11)
+    n, h, w = 2, 6, 7
+    x = R.bf16(rng.standard_normal((n, h, w, 32)).astype(np.float32))
+    g = R.bf16(rng.standard_normal((n, h, w, 32)).astype(np.float32))
+    dx, dg = _dev(gctx, _bits(x).ravel(), np.uint16), _dev(gctx, _bits(g).ravel(), np.uint16)
+    dw, db = L.DeviceArray(gctx, (9, 32, 32), np.float32), L.DeviceArray(gctx, (32,), np.float32)
+    dw.zero()
+    db.zero()
+    check(lib().dme_lp_wgrad(gctx.h, dx.at(), dg.at(), n, h, w, dw.at(), db.at()))
+    want_w, want_b = R.conv_weight_grad(x, g)
+    bound_w, bound_b = R.conv_weight_grad(np.abs(x), np.abs(g))
+    assert np.all(np.abs(dw.download() - want_w) <= 1e-5 * bound_w + 1e-7)
+    assert np.all(np.abs(db.download() - want_b) <= 1e-5 * bound_b + 1e-7)
+
+
+@pytest.mark.gpu
+def test_gpu_relu_backward_and_wta_loss_match_oracle(gctx):
+    from dirac_me import learned as L
+    from dirac_me.abi import check, lib
+
+    rng = np.random.default_rng(12)
+    # ReLU backward: bf16(g) where the activation is positive, else 0, exactly
+    count = 4 * 5 * 6 * 32
+    act = R.bf16(rng.standard_normal(count).astype(np.float32))
+    g = rng.standard_normal(count).astype(np.float32)
+    da, dg = _dev(gctx, _bits(act), np.uint16), _dev(gctx, g, np.float32)
+    out = L.DeviceArray(gctx, (count,), np.uint16)
+    check(lib().dme_lp_relu_backward(gctx.h, da.at(), dg.at(), count, out.at()))
+    assert np.array_equal(out.download(), _bits(np.where(act > 0, g, 0.0).astype(np.float32)))
+    # winner-takes-all loss and its gradient on random outputs (no ties)
+    _, _, lab = _small_problem(13, n=2, h=5, w=6)
+    npos = lab.size // 8
+    o = rng.standard_normal((npos, 32)).astype(np.float32)
+    do, dl = _dev(gctx, o.ravel(), np.float32), _dev(gctx, lab.astype(np.float32).ravel(), np.float32)
+    dd, loss = L.DeviceArray(gctx, (npos, 32), np.float32), L.DeviceArray(gctx, (1,), np.float32)
+    loss.zero()
+    scale = 1.0 / npos
+    check(lib().dme_lp_wta_loss(gctx.h, do.at(), dl.at(), npos, C.c_float(0.1), C.c_float(0.5), C.c_float(scale),
+                                dd.at(), loss.at()))
+    want_l, want_d = R.wta_loss(o, lab.reshape(npos, 8), 0.1, 0.5, scale, dtype=np.float64)
+    assert np.allclose(dd.download(), want_d, rtol=1e-6, atol=1e-9)
+    assert abs(float(loss.download()[0]) - want_l) <= 1e-5 * abs(want_l)
+
+
+@pytest.mark.gpu
 def test_gpu_training_step_matches_oracle(gctx):
     from dirac_me import learned as L
 
