@@ -8,8 +8,9 @@ decision, DC) over one batch of synthetic pictures resident in HBM.  Prints
 one JSON line; the compulsory bytes and algorithmic SAD terms behind the
 roofline figures come from _pel_bulk_bytes() and _sad_terms().  The line also carries the learned-proposal
 leg, the motion-compensation leg (residuals of the same batch from its
-estimates, mc_leg()), the full-search leg (full_search_leg()) and the .imt
-diagnostics-text leg (imt_leg()).
+estimates, mc_leg()), the full-search leg (full_search_leg()), the .imt
+diagnostics-text leg (imt_leg()), the mixed-size batch (mixed_leg()) and the
+8 x 2160p large-motion batch, plain against learned (large_motion_leg()).
 """
 import argparse
 import json
@@ -487,6 +488,68 @@ def mixed_leg(args):
             "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
 
 
+def large_motion_leg(args):
+    """Large-motion stress: 8 pictures of 3840 x 2160 per batch (2 refs, pan
+    up to +-128, 16 objects up to +-96, noise 3, 24/16 blocks, 5 levels,
+    quarter pel), plain batch against the learned batch: frames/s of both,
+    the speedup and the SBCosts gap over the 8 pictures."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    import dirac_me
+    from dirac_me import learned, synth
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        sets = list(ex.map(synth.config4, range(700, 708)))
+    ctx = dirac_me.Context(0)
+    ctx.set_lp_weights(learned.load_weights())
+    prm = dirac_me.make_params(*sets[0]["block"], precision=2, depth=5, lam=10.0)
+    geom = dirac_me.geometry(prm, 3840, 2160)
+    jobs = []
+    for j, d in enumerate(sets):
+        base = 3 * j
+        ctx.upload(base, d["refs"][0])
+        ctx.upload(base + 2, d["refs"][1])
+        ctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        jobs.append((base + 1, [base, base + 2], prm))
+    del sets
+    out = {}
+    for form, run in (("plain", ctx.run_batch), ("learned", ctx.run_batch_learned)):
+        for _ in range(args.warmup):
+            ctx.invalidate(-1)
+            run(jobs)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.invalidate(-1)
+            run(jobs)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ctx.set_profiling(True)
+        ctx.invalidate(-1)
+        run(jobs)
+        ctx.synchronize()
+        kt = ctx.kernel_times()
+        ctx.set_profiling(False)
+        out[form] = {"value": len(jobs) / dt, "unit": "frames/s", "ms_per_step": dt * 1e3,
+                     "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
+    res = {f: [dirac_me.MEData(geom) for _ in jobs] for f in ("plain", "learned")}
+    ctx.invalidate(-1)
+    ctx.run_batch(jobs, res["plain"])
+    b0, f0 = ctx.lp_stats()
+    ctx.invalidate(-1)
+    ctx.run_batch_learned(jobs, res["learned"])
+    b1, f1 = ctx.lp_stats()
+    ctx.close()
+    gaps = [float(np.sum(l.sb_costs, dtype=np.float64) / np.sum(p.sb_costs, dtype=np.float64) - 1.0)
+            for p, l in zip(res["plain"], res["learned"])]
+    return {"config": {"workload": "8 x 3840x2160 B pictures per batch, 2 refs, pan up to +-128, 16 objects up to "
+                                   "+-96, noise 3, blocks 24/16, 5 levels, quarter pel, lambda 10", "batch": len(jobs)},
+            "plain": out["plain"], "learned": out["learned"],
+            "speedup_vs_plain": out["learned"]["value"] / out["plain"]["value"],
+            "mean_sb_cost_gap": float(np.mean(gaps)), "max_sb_cost_gap": float(np.max(gaps)),
+            "fallback_rate": (f1 - f0) / max(1, b1 - b0)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,6 +563,7 @@ def main():
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-imt", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 8 x 2160p large-motion leg")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -611,6 +675,8 @@ def main():
     ctx.close()
     if not args.no_mixed:
         out["mixed64"] = mixed_leg(args)
+    if not args.no_large:
+        out["large_motion"] = large_motion_leg(args)
     if args.copy_calibration_mb:
         _copy_calibration(args.copy_calibration_mb)
     print(json.dumps(out))

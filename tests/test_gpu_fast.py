@@ -46,7 +46,7 @@ def _fast(ctx, d, p, base):
     return got, v0, v2, nums
 
 
-@pytest.mark.parametrize("case", ["cif2", "cif1", "1080p", "720p_2416"])
+@pytest.mark.parametrize("case", ["cif2", "cif1", "1080p", "720p_2416", "2160p_large_motion"])
 def test_fast_form_matches_oracle(lctx, case):
     if case == "cif2":
         d, p = synth.sized(23, "cif", 2), None
@@ -54,6 +54,8 @@ def test_fast_form_matches_oracle(lctx, case):
         d, p = synth.config0(3), None
     elif case == "1080p":
         d, p = synth.config1(104), None
+    elif case == "2160p_large_motion":
+        d, p = synth.config4(105), None
     else:
         d = synth.sized(71, "720p", 2)
         p = make_params(24, 24, 16, 16, precision=2, depth=3, lam=10.0)
@@ -97,6 +99,40 @@ def test_fast_form_gap_on_held_out_pictures(lctx):
     gaps = [float(np.sum(f.sb_costs, dtype=np.float64) / np.sum(q.sb_costs, dtype=np.float64) - 1.0)
             for q, f in zip(plain, fast)]
     print("gaps", ["%.4f" % x for x in gaps], flush=True)
+    assert np.mean(gaps) <= 0.01
+    for pn, nums, _ in jobs:
+        for n in [pn] + nums:
+            lctx.drop(n)
+
+
+def test_large_motion_batch_of_2160p_pictures(lctx):
+    # 8 pictures of 3840 x 2160 (pan up to +-128, 16 objects up to +-96,
+    # noise 3, 24/16 blocks, 5 levels) in one learned batch: the batch equals
+    # the pictures run one at a time, and the mean SBCosts gap against the
+    # plain batch is <= 1%
+    frames = [synth.config4(300 + s) for s in range(8)]
+    p = make_params(*frames[0]["block"], precision=2, depth=5, lam=10.0)
+    g = dirac_me.geometry(p, 3840, 2160)
+    jobs = []
+    for j, d in enumerate(frames):
+        base = 5400 + 3 * j
+        lctx.upload(base, d["refs"][0])
+        lctx.upload(base + 2, d["refs"][1])
+        lctx.upload(base + 1, d["cur"], d["u"], d["v"])
+        jobs.append((base + 1, [base, base + 2], p))
+    del frames
+    plain = [dirac_me.MEData(g) for _ in jobs]
+    fast = [dirac_me.MEData(g) for _ in jobs]
+    lctx.run_batch(jobs, plain)
+    b0, f0 = lctx.lp_stats()
+    lctx.run_batch_learned(jobs, fast)
+    b1, f1 = lctx.lp_stats()
+    one = dirac_me.MEData(g)
+    lctx.run_batch_learned(jobs[5:6], [one])
+    _assert_same(fast[5], one, 2, "batch vs single")
+    gaps = [float(np.sum(f.sb_costs, dtype=np.float64) / np.sum(q.sb_costs, dtype=np.float64) - 1.0)
+            for q, f in zip(plain, fast)]
+    print("gaps", ["%.4f" % x for x in gaps], "fallback rate %.4f" % ((f1 - f0) / max(1, b1 - b0)), flush=True)
     assert np.mean(gaps) <= 0.01
     for pn, nums, _ in jobs:
         for n in [pn] + nums:
