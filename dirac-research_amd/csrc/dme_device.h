@@ -149,6 +149,11 @@ struct McPackJob {
     int blocks, offset;
 };
 
+// Kinds of job a launch of the sub-pel and mode-decision kernels covers
+// (a bit mask): their eighth-pel forms are separate instantiations, so that
+// the others keep their register budget.
+constexpr int DME_KIND_BELOW_EIGHTH = 1, DME_KIND_EIGHTH = 2;
+
 // One unit of a batched launch: a (job, reference) pair covering `count`
 // consecutive work items starting at `offset` of the launch's index space.
 struct Task {

@@ -15,10 +15,14 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <set>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -40,7 +44,7 @@ hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
                            int max_blen_y, bool has_guide, hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
-                           int max_blen_y, hipStream_t st);
+                           int max_blen_y, int threads, hipStream_t st);
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
                               int max_blen_y, hipStream_t st);
 hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, bool up8, hipStream_t st);
@@ -61,13 +65,15 @@ hipError_t launch_lp_adam(float *p, const float *g, float *m, float *v, int coun
 hipError_t launch_lp_wta(const float *out, const float *lab, int npos, float alpha, float beta, float scale,
                          float *dout, float *loss, hipStream_t st);
 hipError_t launch_lp_zero(void *p, size_t bytes, hipStream_t st);
-hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st);
+hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, int threads, int kinds,
+                              hipStream_t st);
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
-                              int max_x1, int max_y1, bool eighth, hipStream_t st);
+                              int max_x1, int max_y1, int kinds, hipStream_t st);
 hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int left,
                                 hipStream_t st);
-hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, hipStream_t st);
-hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st);
+hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, int kinds,
+                          hipStream_t st);
+hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, int kinds, hipStream_t st);
 hipError_t launch_block_dc(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_width,
                            hipStream_t st);
 hipError_t launch_fast_subpel(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
@@ -92,6 +98,11 @@ hipError_t launch_intra_count(const JobDesc *jobs, int njobs, hipStream_t st);
         int rc_ = (expr);          \
         if (rc_ != DME_OK) return rc_; \
     } while (0)
+
+// True on a thread while it captures a batch step into a HIP graph: no
+// buffer may grow then (an allocation or a stream synchronisation cannot be
+// captured), so growth fails instead.
+thread_local bool t_capturing = false;
 
 // Device buffer that only grows.  A borrowed buffer points into memory it
 // does not own (the no-allocation scratch): it is never freed and never
@@ -118,6 +129,7 @@ struct DevBuf {
     }
     int ensure(size_t bytes) {
         if (bytes <= n) return DME_OK;
+        if (t_capturing) return set_error(DME_ERR_STATE, "a device buffer would grow during graph capture");
         if (borrowed)
             return set_error(DME_ERR_INVALID, "scratch layout: " + std::to_string(bytes) + " bytes requested of a " +
                                                   std::to_string(n) + "-byte buffer");
@@ -135,10 +147,16 @@ struct DevBuf {
     T *as() const {
         return static_cast<T *>(p);
     }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(borrowed, o.borrowed);
+    }
 };
 
 struct Picture {
     int num = 0, W = 0, H = 0;
+    uint64_t serial = 0;  // distinct for every registration (batch graphs check it)
     const int16_t *y = nullptr, *u = nullptr, *v = nullptr;  // device planes (owned or bound)
     DevBuf own[3];
     DevBuf pyr[DME_MAX_DEPTH + 1];  // levels 1..
@@ -181,6 +199,7 @@ struct PinnedBuf {
     bool fixed = false;
     int ensure(size_t n) {
         if (n <= bytes) return DME_OK;
+        if (t_capturing) return set_error(DME_ERR_STATE, "pinned staging would grow during graph capture");
         if (fixed) return set_error(DME_ERR_INVALID, "no-allocation staging of " + std::to_string(bytes) + " bytes is too small");
         if (p) (void)hipHostFree(p);
         p = nullptr;
@@ -193,6 +212,11 @@ struct PinnedBuf {
     ~PinnedBuf() {
         if (p) (void)hipHostFree(p);
     }
+    void swap(PinnedBuf &o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(fixed, o.fixed);
+    }
 };
 
 // A stream with the buffers its queued work reads: descriptors, task
@@ -204,16 +228,27 @@ struct Lane {
     size_t arena_used = 0;
 };
 
-// Streams of a context, and how many parts a batch is split into: one part
-// per stream, as many streams as the device's default hardware queues (4;
-// more streams share queues and serialise).  Each part's planes are built
-// after the previous part's, so the parts run out of step: the latency-bound
-// wavefront kernels of one part overlap the wide kernels of another instead
-// of running at the same time as theirs (one step of 64 1080p pictures:
-// 22.9 ms with two parts in step, 21.0 ms with four staggered parts).
+// Streams of a context: as many as the device's default hardware queues (4;
+// more streams share queues and serialise).  A batch runs as up to that many
+// parts, one per stream (dme_tuning.parts).  With stagger, each part's planes
+// are built after the previous part's, so the parts run out of step: the
+// latency-bound wavefront kernels of one part overlap the wide kernels of
+// another instead of running at the same time as theirs (one step of 64
+// 1080p pictures: 22.9 ms with two parts in step, 21.0 ms with four
+// staggered parts).
 constexpr int kLanes = 4;
-constexpr int kParts = 4;
-constexpr bool kStagger = true;
+
+// The built-in launch setting.
+inline dme_tuning default_tuning() {
+    dme_tuning t;
+    t.parts = 4;
+    t.stagger = 1;
+    t.pel_wave_threads = 1024;
+    t.subpel_wave_threads = 512;
+    t.mode_wave_threads = 1024;
+    t.step_ms = 0.0f;
+    return t;
+}
 
 // Pinned host staging of the no-allocation lane: descriptors, task tables
 // and, for a picture whose device lambda threshold is undecided, its pel
@@ -276,11 +311,37 @@ struct dme_ctx {
     hipEvent_t ev_join[dme::kLanes] = {};      // lane 0 after each other lane
     hipEvent_t ev_planes[dme::kLanes] = {};    // a lane's planes are built (staggered starts)
     dme_stats stats;
-    int wave_threads = 1024;
+    // launch settings: the default, the table by batch signature, the one
+    // of the call that runs (fixed while dme_autotune_batch times settings)
+    dme_tuning tune = dme::default_tuning();
+    dme_tuning act = dme::default_tuning();
+    bool act_fixed = false;
+    std::map<std::string, dme_tuning> tune_table;
+    std::set<dme_batch_graph *> graphs;  // live batch graphs (destroyed with the context)
+    uint64_t next_serial = 1;            // Picture::serial
     // per-launch timing while profiling is enabled
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_used;  // (kernel kind, pool index of the start event)
+};
+
+// A captured batch step (dme_batch_graph_create): its jobs' state and the
+// lane buffers and staging the captured copies and kernels read, all owned
+// here so that no later call of the context can move or overwrite them.
+struct dme_batch_graph {
+    dme_ctx *ctx = nullptr;
+    std::vector<std::unique_ptr<dme::JobState>> jobs;
+    std::vector<std::pair<int, uint64_t>> pics;  // picture numbers and serials at capture
+    dme::Lane lanes[dme::kLanes];                // buffers only; the streams are the context's
+    dme::DevBuf lflags;
+    dme_tuning tuning;
+    int launches = 0;  // kernel launches of one replay
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~dme_batch_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 namespace dme {
@@ -322,6 +383,7 @@ static int timed_launch(dme_ctx *ctx, int kind, int &launches, F f) {
 
 static int set_device(dme_ctx *ctx) {
     ctx->L = ctx->L0;  // entry points start on lane 0
+    if (!ctx->act_fixed) ctx->act = ctx->tune;
     HIP_TRY(hipSetDevice(ctx->device));
     return DME_OK;
 }
@@ -331,6 +393,7 @@ static int set_device(dme_ctx *ctx) {
 static int arena_alloc(dme_ctx *ctx, size_t bytes, void **out) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (ctx->L->arena_used + bytes > ctx->L->arena.bytes) {
+        if (t_capturing) return set_error(DME_ERR_STATE, "the staging arena would grow during graph capture");
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
         DME_TRY(ctx->L->arena.ensure(std::max(2 * ctx->L->arena.bytes, ctx->L->arena_used + bytes + ((size_t)1 << 20))));
         ctx->L->arena_used = 0;
@@ -345,6 +408,7 @@ static int arena_alloc(dme_ctx *ctx, size_t bytes, void **out) {
 static int upload(dme_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
     if (bytes == 0) return DME_OK;
     if (dst.n < bytes) {
+        if (t_capturing) return set_error(DME_ERR_STATE, "a device buffer would grow during graph capture");
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
         DME_TRY(dst.ensure(bytes));
     }
@@ -818,7 +882,10 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
                 return launch_pel_full(dj, dtask(ctx, full), full.n, full.total, fpp, frp, frows, ctx->L->stream);
             }));
         DME_TRY(timed_launch(ctx, KK_PEL_WAVE, launches,
-                             [&] { return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by, ctx->L->stream); }));
+                             [&] {
+                                 return launch_pel_wave(dj, dtask(ctx, wave), wave.n, l, rows, bx, by,
+                                                        ctx->act.pel_wave_threads, ctx->L->stream);
+                             }));
     }
     for (auto *j : js) j->stage = 1;
     return DME_OK;
@@ -937,6 +1004,13 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
     return DME_OK;
 }
 
+// Which kinds of job (DME_KIND_*) a set of jobs holds.
+static int job_kinds(const std::vector<JobState *> &js) {
+    int k = 0;
+    for (auto *j : js) k |= j->p.mv_precision == 3 ? DME_KIND_EIGHTH : DME_KIND_BELOW_EIGHTH;
+    return k;
+}
+
 static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) {
     DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
@@ -961,12 +1035,16 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
     const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
+    const int kinds = job_kinds(js);
     if (bulk.n) {
         DME_TRY(timed_launch(ctx, KK_SUBPEL_BULK, launches, [&] {
             return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, bx, by, ctx->L->stream);
         }));
         DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,
-                             [&] { return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows, ctx->L->stream); }));
+                             [&] {
+                                 return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows,
+                                                           ctx->act.subpel_wave_threads, kinds, ctx->L->stream);
+                             }));
     }
     if (shift.n) {
         DME_TRY(timed_launch(ctx, KK_SHIFT, launches, [&] {
@@ -1010,22 +1088,21 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
+    const int kinds = job_kinds(js);
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
-                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, false, st); }));
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, false, kinds, st); }));
     int mx0 = 1, my0 = 1, mx1 = 1, my1 = 1;
-    bool eighth = false;
     for (auto *j : js) {
-        eighth = eighth || j->p.mv_precision == 3;
         mx0 = std::max(mx0, j->g.blen[0][0]);
         my0 = std::max(my0, j->g.blen[0][1]);
         mx1 = std::max(mx1, j->g.blen[1][0]);
         my1 = std::max(my1, j->g.blen[1][1]);
     }
     DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches, [&] {
-        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, eighth, st);
+        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, kinds, st);
     }));
     DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
-                         [&] { return launch_mode_wave(dj, (int)js.size(), ctx->wave_threads, st); }));
+                         [&] { return launch_mode_wave(dj, (int)js.size(), ctx->act.mode_wave_threads, kinds, st); }));
     DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,
                          [&] { return launch_block_dc(dj, dtask(ctx, rdr), rdr.n, rdr.total, max_w, st); }));
     DME_TRY(timed_launch(ctx, KK_DC_SMOOTH, launches,
@@ -1212,30 +1289,58 @@ static int redo_flagged(dme_ctx *ctx, std::vector<JobState *> &js, const float *
     return DME_OK;
 }
 
-// The whole live sequence for a batch of jobs.  A batch of two or more runs
-// as up to kParts parts of about equal work, one per lane, each part's start
-// after the previous part's planes:
+// Signature of a batch for the tuning table: its jobs' sizes, reference
+// counts and parameters as a sorted multiset ("count*key;..."); jobs that
+// differ only in picture numbers or lambda share a key.
+static std::string batch_signature(const std::vector<JobState *> &js) {
+    std::map<std::string, int> keys;
+    for (auto *j : js) {
+        const dme_params &p = j->p;
+        char b[192];
+        std::snprintf(b, sizeof(b), "%dx%d,r%d,b%dx%d/%dx%d,p%d,d%d,f%d,s%d/%d/%d,c%d", j->g.W, j->g.H, j->nrefs,
+                      p.xblen, p.yblen, p.xbsep, p.ybsep, p.mv_precision, j->g.depth, p.fp_semantics, p.full_search,
+                      p.full_search ? p.x_range_me : 0, p.full_search ? p.y_range_me : 0, p.combined_me);
+        ++keys[b];
+    }
+    std::string sig;
+    for (auto &kv : keys) {
+        if (!sig.empty()) sig += ';';
+        sig += std::to_string(kv.second) + '*' + kv.first;
+    }
+    return sig;
+}
+
+static bool tuning_ok(const dme_tuning &t) {
+    return t.parts >= 1 && t.parts <= kLanes && (t.stagger == 0 || t.stagger == 1) &&
+           (t.pel_wave_threads == 256 || t.pel_wave_threads == 512 || t.pel_wave_threads == 1024) &&
+           (t.subpel_wave_threads == 256 || t.subpel_wave_threads == 512) &&
+           (t.mode_wave_threads == 256 || t.mode_wave_threads == 512 || t.mode_wave_threads == 1024);
+}
+
+// The setting a batch runs with: its table entry, else the default.
+static void choose_tuning(dme_ctx *ctx, const std::vector<JobState *> &js) {
+    if (ctx->act_fixed) return;
+    ctx->act = ctx->tune;
+    if (ctx->tune_table.empty()) return;
+    auto it = ctx->tune_table.find(batch_signature(js));
+    if (it != ctx->tune_table.end()) ctx->act = it->second;
+}
+
+// Queues the whole live sequence of a batch (streams idle, flag slots set)
+// as up to act.parts parts of about equal work, one per lane:
 //   lane 0: shared planes, planes(A) pel(A)  lambda(A) subpel(A) mode(A)
 //   lane 1:                   planes(B) pel(B)  lambda(B) subpel(B) mode(B)
 //   lane 2:                             planes(C) pel(C) ...
+// (staggered starts; without stagger every part starts at the fork).
 // Planes that two parts read are built once, before the fork, so no kernel
 // of one lane reads a plane another lane writes; every other buffer a part
-// uses is its own jobs' or its lane's.  With per-kernel profiling on, the
-// parts run one after the other on lane 0 so that each kernel is timed alone.
-static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
-    int launches = 0;
-    DME_TRY(begin_timing(ctx));
-    DME_TRY(lambda_flag_slots(ctx, js));
-    if (js.size() < 2 || ctx->profiling) {
-        DME_TRY(stage_pel(ctx, js, launches));
-        DME_TRY(stage_lambda(ctx, js, lam_over, false, launches));
-        DME_TRY(stage_subpel(ctx, js, launches));
-        DME_TRY(stage_mode(ctx, js, launches));
-        DME_TRY(end_timing(ctx, launches));
-        return redo_flagged(ctx, js, lam_over);
-    }
+// uses is its own jobs' or its lane's.  Ends with lane 0 waiting for the
+// others.  With per-kernel profiling on, the batch runs as one part on
+// lane 0 so that each kernel is timed alone.  Only stream operations: a
+// graph capture records it as it stands.
+static int queue_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over, int &launches) {
+    const int np = ctx->profiling ? 1 : (int)std::min<size_t>((size_t)ctx->act.parts, js.size());
     // parts: largest job first to the lightest part, order kept within each
-    const int np = (int)std::min<size_t>(kParts, js.size());
     std::vector<size_t> order(js.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     auto work = [&](size_t i) { return (double)js[i]->g.W * js[i]->g.H * js[i]->nrefs; };
@@ -1255,15 +1360,18 @@ static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float
         part[side[i]].push_back(js[i]);
         over[side[i]].push_back(lam_over ? lam_over[i] : nullptr);
     }
-    // Planes of pictures two parts use are built on lane 0 before the fork;
-    // every other picture's planes are built by its own part's lane, so one
-    // lane's plane building overlaps another's search.
-    DME_TRY(build_shared_planes(ctx, part, launches));
-    HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
-    for (int h = 1; h < np; ++h) HIP_TRY(hipStreamWaitEvent(ctx->lane[h].stream, ctx->ev_fork, 0));
+    ctx->L = &ctx->lane[0];
+    if (np > 1) {
+        // Planes of pictures two parts use are built on lane 0 before the
+        // fork; every other picture's planes are built by its own part's
+        // lane, so one lane's plane building overlaps another's search.
+        DME_TRY(build_shared_planes(ctx, part, launches));
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->lane[0].stream));
+        for (int h = 1; h < np; ++h) HIP_TRY(hipStreamWaitEvent(ctx->lane[h].stream, ctx->ev_fork, 0));
+    }
     for (int h = 0; h < np; ++h) {
         ctx->L = &ctx->lane[h];
-        if (kStagger) {
+        if (np > 1 && ctx->act.stagger) {
             if (h > 0) HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_planes[h - 1], 0));
             DME_TRY(bind_planes(ctx, part[h], true, launches));
             HIP_TRY(hipEventRecord(ctx->ev_planes[h], ctx->L->stream));
@@ -1281,6 +1389,17 @@ static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float
         HIP_TRY(hipEventRecord(ctx->ev_join[h], ctx->lane[h].stream));
         HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_join[h], 0));
     }
+    return DME_OK;
+}
+
+// The whole live sequence for a batch of jobs, then the host's redo of jobs
+// whose device lambda maps were flagged.
+static int run_jobs_parts(dme_ctx *ctx, std::vector<JobState *> &js, const float *const *lam_over) {
+    int launches = 0;
+    choose_tuning(ctx, js);
+    DME_TRY(begin_timing(ctx));
+    DME_TRY(lambda_flag_slots(ctx, js));
+    DME_TRY(queue_parts(ctx, js, lam_over, launches));
     DME_TRY(end_timing(ctx, launches));
     return redo_flagged(ctx, js, lam_over);
 }
@@ -1346,6 +1465,8 @@ extern "C" int dme_destroy(dme_ctx *ctx) try {
     if (!ctx) return DME_OK;
     (void)hipSetDevice(ctx->device);
     for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
+    for (auto *g : ctx->graphs) delete g;
+    ctx->graphs.clear();
     ctx->jobs.clear();
     ctx->pics.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1415,6 +1536,7 @@ static int register_picture(dme_ctx *ctx, int pic_num, int width, int height, co
     HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     if (bad) return set_error(DME_ERR_INVALID, "luma samples must lie in [-128, 127] (8-bit samples minus 128)");
+    P->serial = ctx->next_serial++;
     ctx->pics[pic_num] = std::move(P);
     return DME_OK;
 }
@@ -1548,19 +1670,345 @@ extern "C" int dme_motion_estimate(dme_ctx *ctx, int pic_num, const int *refs, i
     return dme::exception_status();
 }
 
+namespace dme {
+static int prepare_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, std::vector<JobState *> &js,
+                         std::vector<std::unique_ptr<JobState>> *own);
+}
+
 extern "C" int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_medata *outs) try {
     CTX_CHECK(ctx);
-    if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
     std::vector<JobState *> js;
+    DME_TRY(prepare_batch(ctx, jobs, njobs, js, nullptr));
+    DME_TRY(run_jobs(ctx, js, nullptr));
+    if (outs) DME_TRY(export_batch(ctx, js, outs));
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+// ---- launch tuning and captured batch steps -----------------------------------
+
+namespace dme {
+
+// The jobs of a batch entry point, validated and prepared in js (the
+// context's job state when own is NULL, else new state kept in own).
+static int prepare_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, std::vector<JobState *> &js,
+                         std::vector<std::unique_ptr<JobState>> *own) {
+    if (!jobs || njobs <= 0) return set_error(DME_ERR_INVALID, "empty batch");
     for (int i = 0; i < njobs; ++i) {
         for (int k = 0; k < i; ++k)
             if (jobs[k].pic_num == jobs[i].pic_num) return set_error(DME_ERR_INVALID, "a picture appears twice in a batch");
-        JobState *J = job_for(ctx, jobs[i].pic_num);
+        JobState *J = nullptr;
+        if (own) {
+            own->emplace_back(new JobState());
+            J = own->back().get();
+        } else {
+            J = job_for(ctx, jobs[i].pic_num);
+        }
         DME_TRY(prepare_job(ctx, *J, jobs[i].params, jobs[i].pic_num, jobs[i].refs, jobs[i].nrefs));
         js.push_back(J);
     }
-    DME_TRY(run_jobs(ctx, js, nullptr));
+    return DME_OK;
+}
+
+// Marks the planes of the pictures a batch reads as not built.
+static void invalidate_batch_planes(dme_ctx *ctx, const std::vector<JobState *> &js) {
+    for (auto *j : js) {
+        int nums[3] = {j->pic_num, j->refs[0], j->refs[1]};
+        for (int k = 0; k <= j->nrefs; ++k) {
+            Picture *P = find_pic(ctx, nums[k]);
+            if (!P) continue;
+            P->pyr_levels = 0;
+            P->qp_ready = P->b8_ready = P->u8_ready = P->up_ready = false;
+        }
+    }
+}
+
+// Exchanges the context's lane buffers and lambda flags with a graph's.
+static void swap_graph_buffers(dme_ctx *ctx, dme_batch_graph *g) {
+    for (int h = 0; h < kLanes; ++h) {
+        Lane &a = ctx->lane[h], &b = g->lanes[h];
+        a.d_jobs.swap(b.d_jobs);
+        a.d_tasks.swap(b.d_tasks);
+        a.d_planes.swap(b.d_planes);
+        a.arena.swap(b.arena);
+        std::swap(a.arena_used, b.arena_used);
+    }
+    ctx->d_lflags.swap(g->lflags);
+}
+
+// With the graph's buffers in the context: one eager step that sizes every
+// buffer and builds the planes, then the capture of the same step from
+// unbuilt planes.
+static int warm_and_capture(dme_ctx *ctx, dme_batch_graph *g, std::vector<JobState *> &js) {
+    int launches = 0;
+    for (auto &ln : ctx->lane) ln.arena_used = 0;
+    DME_TRY(lambda_flag_slots(ctx, js));
+    invalidate_batch_planes(ctx, js);
+    DME_TRY(queue_parts(ctx, js, nullptr, launches));
+    for (auto &ln : ctx->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
+    for (auto &ln : ctx->lane) ln.arena_used = 0;
+    invalidate_batch_planes(ctx, js);
+    launches = 0;
+    hipStream_t s0 = ctx->lane[0].stream;
+    HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    t_capturing = true;
+    const int rc = queue_parts(ctx, js, nullptr, launches);
+    t_capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s0, &graph);
+    if (rc != DME_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(DME_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    g->graph = graph;
+    HIP_TRY(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+    g->launches = launches;
+    // the host's plane flags say built again: the warm-up built them from
+    // the same contents, and every replay rebuilds them
+    return DME_OK;
+}
+
+// Wall-clock time of one batch step under the current setting (act).
+static int time_step(dme_ctx *ctx, std::vector<JobState *> &js, double &ms) {
+    invalidate_batch_planes(ctx, js);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = run_jobs(ctx, js, nullptr);
+    const auto t1 = std::chrono::steady_clock::now();
+    DME_TRY(rc);
+    ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return DME_OK;
+}
+
+static std::string tuning_key(const dme_tuning &t) {
+    return std::to_string(t.parts) + "," + std::to_string(t.stagger) + "," + std::to_string(t.pel_wave_threads) + "," +
+           std::to_string(t.subpel_wave_threads) + "," + std::to_string(t.mode_wave_threads);
+}
+
+}  // namespace dme
+
+extern "C" int dme_set_tuning(dme_ctx *ctx, const dme_tuning *t) try {
+    CTX_CHECK(ctx);
+    const dme_tuning v = t ? *t : default_tuning();
+    if (!tuning_ok(v))
+        return set_error(DME_ERR_INVALID, "tuning: parts 1..4, stagger 0/1, pel wave threads 256/512/1024, "
+                                          "sub-pel 256/512, mode 256/512/1024");
+    ctx->tune = v;
+    ctx->tune.step_ms = 0.0f;
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_get_tuning(dme_ctx *ctx, dme_tuning *out) try {
+    if (!ctx || !out) return set_error(DME_ERR_INVALID, "NULL argument");
+    *out = ctx->tune;
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_autotune_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, int reps, dme_tuning *out) try {
+    CTX_CHECK(ctx);
+    if (reps < 1 || reps > 100) return set_error(DME_ERR_INVALID, "reps must be in [1, 100]");
+    if (ctx->profiling) return set_error(DME_ERR_STATE, "autotuning times whole steps: disable profiling first");
+    std::vector<JobState *> js;
+    DME_TRY(prepare_batch(ctx, jobs, njobs, js, nullptr));
+    std::map<std::string, double> seen;
+    // best of reps timed steps after one warm-up step; the setting stays
+    // fixed for those steps
+    auto measure = [&](const dme_tuning &t, double &best) -> int {
+        const std::string k = tuning_key(t);
+        auto it = seen.find(k);
+        if (it != seen.end()) {
+            best = it->second;
+            return DME_OK;
+        }
+        ctx->act = t;
+        ctx->act_fixed = true;
+        double ms = 0.0;
+        int rc = time_step(ctx, js, ms);
+        best = 1e30;
+        for (int i = 0; i < reps && rc == DME_OK; ++i) {
+            rc = time_step(ctx, js, ms);
+            best = std::min(best, ms);
+        }
+        ctx->act_fixed = false;
+        DME_TRY(rc);
+        seen[k] = best;
+        return DME_OK;
+    };
+    dme_tuning best = default_tuning();
+    double best_ms = 0.0;
+    DME_TRY(measure(best, best_ms));
+    auto consider = [&](const dme_tuning &t) -> int {
+        double ms = 0.0;
+        DME_TRY(measure(t, ms));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+        return DME_OK;
+    };
+    // parts and stagger, then each wavefront kernel's threads
+    for (int parts = 1; parts <= kLanes && parts <= njobs; ++parts)
+        for (int st = 0; st <= (parts > 1 ? 1 : 0); ++st) {
+            dme_tuning t = best;
+            t.parts = parts;
+            t.stagger = st;
+            DME_TRY(consider(t));
+        }
+    for (int th : {256, 512, 1024}) {
+        dme_tuning t = best;
+        t.pel_wave_threads = th;
+        DME_TRY(consider(t));
+    }
+    for (int th : {256, 512}) {
+        dme_tuning t = best;
+        t.subpel_wave_threads = th;
+        DME_TRY(consider(t));
+    }
+    for (int th : {256, 512, 1024}) {
+        dme_tuning t = best;
+        t.mode_wave_threads = th;
+        DME_TRY(consider(t));
+    }
+    best.step_ms = (float)best_ms;
+    ctx->tune_table[batch_signature(js)] = best;
+    if (out) *out = best;
+    return DME_OK;
+} catch (...) {
+    if (ctx) ctx->act_fixed = false;
+    return dme::exception_status();
+}
+
+extern "C" int dme_tuning_save(dme_ctx *ctx, const char *path) try {
+    if (!ctx || !path) return set_error(DME_ERR_INVALID, "NULL argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) return set_error(DME_ERR_INVALID, std::string("cannot write ") + path);
+    std::fprintf(f, "# dme tuning table 1: signature parts stagger pel_wave subpel_wave mode_wave step_ms\n");
+    for (auto &kv : ctx->tune_table) {
+        const dme_tuning &t = kv.second;
+        std::fprintf(f, "%s %d %d %d %d %d %.4f\n", kv.first.c_str(), t.parts, t.stagger, t.pel_wave_threads,
+                     t.subpel_wave_threads, t.mode_wave_threads, (double)t.step_ms);
+    }
+    const bool ok = std::ferror(f) == 0;
+    if (std::fclose(f) != 0 || !ok) return set_error(DME_ERR_INVALID, std::string("cannot write ") + path);
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_tuning_load(dme_ctx *ctx, const char *path) try {
+    if (!ctx || !path) return set_error(DME_ERR_INVALID, "NULL argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_error(DME_ERR_INVALID, std::string("cannot read ") + path);
+    std::map<std::string, dme_tuning> add;
+    char line[4096];
+    int ln = 0;
+    bool bad = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++ln;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        std::istringstream is(line);
+        std::string sig;
+        dme_tuning t;
+        if (!(is >> sig >> t.parts >> t.stagger >> t.pel_wave_threads >> t.subpel_wave_threads >> t.mode_wave_threads >>
+              t.step_ms) ||
+            !tuning_ok(t)) {
+            bad = true;
+            break;
+        }
+        add[sig] = t;
+    }
+    std::fclose(f);
+    if (bad) return set_error(DME_ERR_INVALID, std::string(path) + ": bad tuning line " + std::to_string(ln));
+    for (auto &kv : add) ctx->tune_table[kv.first] = kv.second;
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_batch_graph_create(dme_ctx *ctx, const dme_job *jobs, int njobs, dme_batch_graph **out) try {
+    CTX_CHECK(ctx);
+    if (!out) return set_error(DME_ERR_INVALID, "NULL out");
+    *out = nullptr;
+    if (ctx->profiling) return set_error(DME_ERR_STATE, "a captured step has no per-kernel events: disable profiling");
+    std::unique_ptr<dme_batch_graph> g(new dme_batch_graph());
+    g->ctx = ctx;
+    std::vector<JobState *> js;
+    DME_TRY(prepare_batch(ctx, jobs, njobs, js, &g->jobs));
+    std::set<int> nums;
+    for (auto *j : js) {
+        nums.insert(j->pic_num);
+        for (int r = 0; r < j->nrefs; ++r) nums.insert(j->refs[r]);
+    }
+    for (int n : nums) g->pics.push_back(std::make_pair(n, find_pic(ctx, n)->serial));
+    choose_tuning(ctx, js);
+    g->tuning = ctx->act;
+    for (auto &ln : ctx->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
+    swap_graph_buffers(ctx, g.get());
+    const int rc = warm_and_capture(ctx, g.get(), js);
+    if (rc != DME_OK)
+        for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
+    swap_graph_buffers(ctx, g.get());
+    ctx->L = ctx->L0;
+    DME_TRY(rc);
+    ctx->graphs.insert(g.get());
+    *out = g.release();
+    return DME_OK;
+} catch (...) {
+    dme::t_capturing = false;
+    return dme::exception_status();
+}
+
+extern "C" int dme_batch_graph_launch(dme_batch_graph *g, dme_medata *outs) try {
+    if (!g) return set_error(DME_ERR_INVALID, "NULL graph");
+    dme_ctx *ctx = g->ctx;
+    DME_TRY(set_device(ctx));
+    for (auto &pn : g->pics) {
+        const Picture *P = find_pic(ctx, pn.first);
+        if (!P || P->serial != pn.second)
+            return set_error(DME_ERR_STATE, "picture " + std::to_string(pn.first) +
+                                                " was dropped or registered again since the graph was captured");
+    }
+    std::vector<JobState *> js;
+    for (auto &j : g->jobs) js.push_back(j.get());
+    hipStream_t s0 = ctx->lane[0].stream;
+    for (auto &ln : ctx->lane) ln.arena_used = 0;  // streams idle: every call ends synchronised
+    ctx->ev_used.clear();
+    HIP_TRY(hipEventRecord(ctx->ev0, s0));
+    HIP_TRY(hipGraphLaunch(g->exec, s0));
+    DME_TRY(end_timing(ctx, g->launches));
+    // the replay built the pictures' planes
+    for (auto *j : js) {
+        int pn[3] = {j->pic_num, j->refs[0], j->refs[1]};
+        for (int k = 0; k <= j->nrefs; ++k) {
+            Picture *P = find_pic(ctx, pn[k]);
+            P->pyr_levels = std::max(P->pyr_levels, j->g.depth);
+            P->b8_ready = true;
+            if (k > 0) P->qp_ready = true;
+        }
+    }
+    DevBuf *saved = ctx->lflags;
+    ctx->lflags = &g->lflags;
+    const int rc = redo_flagged(ctx, js, nullptr);
+    ctx->lflags = saved;
+    DME_TRY(rc);
     if (outs) DME_TRY(export_batch(ctx, js, outs));
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
+extern "C" int dme_batch_graph_destroy(dme_batch_graph *g) try {
+    if (!g) return DME_OK;
+    dme_ctx *ctx = g->ctx;
+    (void)hipSetDevice(ctx->device);
+    for (auto &ln : ctx->lane) (void)hipStreamSynchronize(ln.stream);
+    ctx->graphs.erase(g);
+    delete g;
     return DME_OK;
 } catch (...) {
     return dme::exception_status();
@@ -2428,7 +2876,7 @@ static int stage_mode_fast(dme_ctx *ctx, std::vector<JobState *> &js, int &launc
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
     DME_TRY(timed_launch(ctx, KK_MODE_L2, launches,
-                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, true, st); }));
+                         [&] { return launch_mode_l2(dj, dtask(ctx, rl2), rl2.n, rl2.total, true, DME_KIND_BELOW_EIGHTH, st); }));
     DME_TRY(timed_launch(ctx, KK_FAST_MODE, launches,
                          [&] { return launch_fast_mode(dj, dtask(ctx, rsb), rsb.n, rsb.total, ux, uy, st); }));
     DME_TRY(timed_launch(ctx, KK_FAST_EVAL, launches,
@@ -2465,7 +2913,7 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
     DME_TRY(lp_fold_stats(ctx));
     DME_TRY(begin_timing(ctx));
     DME_TRY(lambda_flag_slots(ctx, js));
-    const int np = (js.size() < 2 || ctx->profiling) ? 1 : (int)std::min<size_t>(kParts, js.size());
+    const int np = (js.size() < 2 || ctx->profiling) ? 1 : (int)std::min<size_t>((size_t)ctx->tune.parts, js.size());
     std::vector<std::vector<JobState *>> part(np);
     for (size_t i = 0; i < js.size(); ++i) part[i % np].push_back(js[i]);
     std::vector<std::vector<LpGroup>> groups(np);
@@ -2484,7 +2932,7 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
         for (int h = 1; h < np; ++h) HIP_TRY(hipStreamWaitEvent(ctx->lane[h].stream, ctx->ev_fork, 0));
         for (int h = 0; h < np; ++h) {
             ctx->L = &ctx->lane[h];
-            if (kStagger) {
+            if (ctx->tune.stagger) {
                 if (h > 0) HIP_TRY(hipStreamWaitEvent(ctx->L->stream, ctx->ev_planes[h - 1], 0));
                 DME_TRY(bind_planes(ctx, part[h], true, launches));
                 HIP_TRY(hipEventRecord(ctx->ev_planes[h], ctx->L->stream));

@@ -1562,12 +1562,14 @@ struct UpStart {
     bool e;  // eighth pel: rx, ry in 0..3
 };
 
+// E8 false: step is 1 or 2 (the forms below eighth pel compile without it).
+template <bool E8 = true>
 __device__ __forceinline__ UpStart up_start(const Blk &b, dme_mv mv, int step, int UW, int UH) {
     UpStart s;
-    s.e = step == 3;
+    s.e = E8 && step == 3;
     if (step == 1) {
         s.rsx = (b.xp << 1) + mv.x; s.rsy = (b.yp << 1) + mv.y; s.rx = 0; s.ry = 0;
-    } else if (step == 2) {
+    } else if (!E8 || step == 2) {
         s.rsx = (b.xp << 1) + (mv.x >> 1); s.rsy = (b.yp << 1) + (mv.y >> 1);
         s.rx = mv.x & 1; s.ry = mv.y & 1;
     } else {
@@ -1599,8 +1601,9 @@ __device__ __forceinline__ int qp_eighth(const uint32_t *__restrict__ qp, int W,
 // Up-plane sample of pixel (dy, dx) for start s, read from the quarter-pel
 // planes: quarter-pel position (2 rsx + rx + 4 dx, 2 rsy + ry + 4 dy), or
 // the eighth-pel sample at up position (rsx + 2 dx, rsy + 2 dy).
+template <bool E8>
 __device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int H, const UpStart &s, int dy, int dx) {
-    if (s.e) return qp_eighth(qp, W, H, s.rsx + 2 * dx, s.rsy + 2 * dy, s.rx, s.ry) - 128;
+    if (E8 && s.e) return qp_eighth(qp, W, H, s.rsx + 2 * dx, s.rsy + 2 * dy, s.rx, s.ry) - 128;
     const int Xq = 2 * s.rsx + s.rx + 4 * dx, Yq = 2 * s.rsy + s.ry + 4 * dy;
     const int X = clampi(Xq >> 2, -1, W), Y = clampi(Yq >> 2, -1, H);
     const uint32_t d = qp[(size_t)((Y + 1) * 4 + (Yq & 3)) * (W + 2) + X + 1];
@@ -1610,11 +1613,12 @@ __device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int
 // Integer SADs of up to 4 candidates against the up-converted reference;
 // wave-uniform.  (candidate, pixel) items are spread over the lanes, 8 per
 // lane at a time with all their loads issued together.
+template <bool E8>
 __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const uint32_t *__restrict__ qp, int UW,
                                           int UH, const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
     const int lane = threadIdx.x & (WAVE - 1);
     UpStart st[4];
-    for (int c = 0; c < 4; ++c) st[c] = up_start(b, cands[c < nc ? c : 0], step, UW, UH);
+    for (int c = 0; c < 4; ++c) st[c] = up_start<E8>(b, cands[c < nc ? c : 0], step, UW, UH);
     const int np = b.xl * b.yl, n = nc * np;
     int acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
     for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
@@ -1633,7 +1637,7 @@ __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W
                     if (c == e) s0 = st[e];
                 cid[k] = c;
                 pv[k] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-                rv[k] = qp_at(qp, UW >> 1, UH >> 1, s0, dy, dx);
+                rv[k] = qp_at<E8>(qp, UW >> 1, UH >> 1, s0, dy, dx);
             }
         }
 #pragma unroll
@@ -1665,6 +1669,7 @@ __device__ __forceinline__ float binade_limit(float s) {
     return __uint_as_float((e + 1) << 23);
 }
 
+template <bool E8>
 __device__ __noinline__ float wave_chain(float start, const int16_t *__restrict__ pic, int W,
                                         const uint32_t *__restrict__ qp, int UW, int UH, const Blk &b, const UpStart &st) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1677,7 +1682,7 @@ __device__ __noinline__ float wave_chain(float start, const int16_t *__restrict_
             const int i = b0 + k * WAVE + lane;
             const int ii = i < n ? i : n - 1;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
-            const int v = abs(qp_at(qp, UW >> 1, UH >> 1, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+            const int v = abs(qp_at<E8>(qp, UW >> 1, UH >> 1, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
             d[k] = i < n ? v : 0;
         }
 #pragma unroll
@@ -2027,7 +2032,7 @@ struct SubJob {
 // barrier of its round trip and reset (when it was used) before the next
 // round trip's first barrier, which every thread passes before the round trip
 // after that touches the counter again; an empty round trip costs one barrier.
-template <bool CHAIN>
+template <bool CHAIN, bool E8>
 __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, const JobDesc &J,
                                           const uint32_t *__restrict__ qp, const SubJob (&mine)[4],
                                           const bool (&want)[4], int (&ires)[4], float (&fres)[4]) {
@@ -2058,12 +2063,12 @@ __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, 
             m.x = q.mx;
             m.y = q.my;
             if (CHAIN) {
-                const UpStart us = up_start(b, m, q.step, UW, UH);
-                const float v = wave_chain(q.start, J.pic[0], J.W, qp, UW, UH, b, us);
+                const UpStart us = up_start<E8>(b, m, q.step, UW, UH);
+                const float v = wave_chain<E8>(q.start, J.pic[0], J.W, qp, UW, UH, b, us);
                 if (lane == 0) queue[e].fres = v;
             } else {
                 int v = 0;
-                wave_up_sads(J.pic[0], J.W, qp, UW, UH, b, q.step, &m, 1, &v);
+                wave_up_sads<E8>(J.pic[0], J.W, qp, UW, UH, b, q.step, &m, 1, &v);
                 if (lane == 0) queue[e].ires = v;
             }
         }
@@ -2079,6 +2084,8 @@ __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, 
     if (tid == 0) *cnt = 0;
 }
 
+// E8: the eighth-pel jobs of the launch (the others exit), else the rest.
+template <bool E8>
 __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                       int max_rows) {
     extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT] | queue [SUB_QCAP]
@@ -2086,7 +2093,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
     __shared__ int s_chain[2];  // per pass parity: some block of the pass may need a float chain
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
-    if (tk.ref >= J.nrefs || J.prec == 0) return;
+    if (tk.ref >= J.nrefs || J.prec == 0 || (J.prec == 3) != E8) return;  // uniform per workgroup
     const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1;
     const int X = J.xnb, Y = J.ynb, W = J.W, H = J.H;
     const int UW = 2 * W, UH = 2 * H;
@@ -2174,7 +2181,8 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     if (!b.empty()) {
                         // an eighth-pel predictor is always computed on demand
                         const int dx = pred.x - pel_up.x, dy = pred.y - pel_up.y;
-                        if (prec <= 2 && abs(dx) <= rad && abs(dy) <= rad) psad = mytab[(dy + rad) * Q + dx + rad];
+                        if ((!E8 || prec <= 2) && abs(dx) <= rad && abs(dy) <= rad)
+                            psad = mytab[(dy + rad) * Q + dx + rad];
                         else miss = true;
                     }
                 }
@@ -2200,7 +2208,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     lo.y = (cqy >> sh) - r1;
                     hi.x = (cqx >> sh) + r1;
                     hi.y = (cqy >> sh) + r1;
-                    edge = up_start(b, lo, i, UW, UH).bounds || up_start(b, hi, i, UW, UH).bounds;
+                    edge = up_start<E8>(b, lo, i, UW, UH).bounds || up_start<E8>(b, hi, i, UW, UH).bounds;
                 }
                 bool may = false;
                 if (edge && lambda != 0.0f) {
@@ -2213,7 +2221,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     const int e = (int)((__double_as_longlong(mmax) >> 52) & 0x7ff) - 1022;  // mmax < 2^e
                     may = N >= 32768 || lambda < 0.0f || low < e - 24;
                 }
-                if (may || prec == 3) s_chain[cf] = 1;  // the eighth-pel stage always adds per pixel
+                if (may || (E8 && prec == 3)) s_chain[cf] = 1;  // the eighth-pel stage always adds per pixel
             }
             // predictor SADs outside the table
             {
@@ -2231,7 +2239,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                 mine[0].ires = 0;
                 mine[0].start = mine[0].fres = 0.0f;
                 mine[1] = mine[2] = mine[3] = mine[0];
-                sub_serve<false>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
+                sub_serve<false, E8>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
                 psad = ir[0];
             }
             const bool chains = s_chain[cf] != 0;  // visible after the round trip's barrier
@@ -2251,7 +2259,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
             dme_mv bmv = pel;
             bool live = refine, bailed = false;
             const bool tests = !b.empty();
-            const int tq = sub_unit(prec);  // the table's unit: 1/2^tq pel
+            const int tq = E8 ? sub_unit(prec) : prec;  // the table's unit: 1/2^tq pel
             const int cx = pel.x << tq, cy = pel.y << tq;
             for (int i = 1; i <= prec; ++i) {
                 if (live) {
@@ -2285,7 +2293,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     // stages in half and quarter pel read the table; the
                     // eighth-pel stage (BlockDiffEighthPel, per-pixel float
                     // adds in both builds) is a chain round trip
-                    const bool tab = i <= tq;
+                    const bool tab = !E8 || i <= tq;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         dme_mv mf;
@@ -2296,7 +2304,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                         mvc[j] = var_up(pred, mf);
                         st[j] = mvc[j] * lambda;
                         tot[j] = (float)sd[j] + st[j];
-                        const UpStart us = up_start(b, m[j], i, UW, UH);
+                        const UpStart us = up_start<E8>(b, m[j], i, UW, UH);
                         need[j] = on[j] && (!tab || ((us.bounds || c_sem) && !chain_is_exact(st[j], sd[j])));
                         mine[j].xp = b.xp;
                         mine[j].yp = b.yp;
@@ -2311,7 +2319,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     }
                     int ir[4] = {0, 0, 0, 0};
                     float fr[4] = {tot[0], tot[1], tot[2], tot[3]};
-                    if (chains) sub_serve<true>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);  // uniform
+                    if (chains) sub_serve<true, E8>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);  // uniform
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (on[j] && fr[j] < best.total) {
@@ -2386,12 +2394,12 @@ __device__ __forceinline__ UpStart bi_second(const UpStart &s1, UpStart s2) {
 // BiBlock{Half,Quarter,Eighth}Pel::Diff (me_utils.cpp:1172-1785): sum of
 // |((2 pic - r1) - r2) >> 1| with short intermediates (the eighth-pel
 // form's float adds of integers are exact).
-template <int NB = 8>
+template <bool E8, int NB = 8>
 __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int UW = 2 * J.W, UH = 2 * J.H;
-    const UpStart s1 = up_start(b, m1, J.step, UW, UH);
-    const UpStart s2 = bi_second(s1, up_start(b, m2, J.step, UW, UH));
+    const UpStart s1 = up_start<E8>(b, m1, J.step, UW, UH);
+    const UpStart s2 = E8 ? bi_second(s1, up_start<E8>(b, m2, J.step, UW, UH)) : up_start<E8>(b, m2, J.step, UW, UH);
     const int16_t *__restrict__ pic = J.pic[0];
     const uint32_t *__restrict__ q1 = J.qp[0];
     const uint32_t *__restrict__ q2 = J.qp[1];
@@ -2407,8 +2415,8 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
             const int ii = ok[k] ? i : 0;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
             p[k] = pic[(unsigned)((b.yp + dy) * J.W + b.xp + dx)];
-            r1[k] = qp_at(q1, J.W, J.H, s1, dy, dx);
-            r2[k] = qp_at(q2, J.W, J.H, s2, dy, dx);
+            r1[k] = qp_at<E8>(q1, J.W, J.H, s1, dy, dx);
+            r2[k] = qp_at<E8>(q2, J.W, J.H, s2, dy, dx);
         }
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -2446,8 +2454,9 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 
 // DoUnitDecn at level 2 (me_mode_decn.cpp:368-462), for every block in
 // parallel: it only reads the block's own refined vectors and costs.  UP8:
-// the references are byte up-planes (the learned path's fast form).
-template <bool UP8>
+// the references are byte up-planes (the learned path's fast form).  E8:
+// the eighth-pel jobs of the launch (the others are skipped), else the rest.
+template <bool UP8, bool E8>
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
     const int g0 = (xcd_block() * 4 + wave_id()) * PW_MODE_L2, g1 = min(g0 + PW_MODE_L2, total);
@@ -2455,6 +2464,7 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
         const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
+        if ((J.step == 3) != E8) continue;  // uniform per wave
         const int k = gi - tk.offset;
         const int xb = k % J.xnb, yb = k / J.xnb;
         const int lane = threadIdx.x & 63;
@@ -2480,7 +2490,7 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             bc.mvcost = c1.mvcost + c2.mvcost;
             bc.SAD = b.empty() ? 0.0f
                                : (float)(UP8 ? up8_bi_sad(J, b, J.vec[0][k], J.vec[1][k])
-                                             : wave_bi_sad(J, b, J.vec[0][k], J.vec[1][k]));
+                                             : wave_bi_sad<E8>(J, b, J.vec[0][k], J.vec[1][k]));
             bc.total = bc.SAD + loc_lambda * bc.mvcost;
             bc.total *= lf;
             uc = bc.total + mode_cost;
@@ -2574,8 +2584,8 @@ struct MtLds {
     int pd, prows, spd, slot, wave;
 };
 
-// E8: some job of the launch estimates at eighth pel (those gathers need
-// registers the others do not).
+// E8: the eighth-pel jobs of the launch (their gathers need registers the
+// others do not; the other jobs are skipped), else the rest.
 template <bool E8>
 __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                      int ntasks, int total, MtLds ly) {
@@ -2587,6 +2597,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
         const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
+        if ((J.step == 3) != E8) continue;  // uniform per wave: the other launch's job
         const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
         const int u = gi - tk.offset;
         const int X2 = J.xnb, W = J.W, H = J.H, QW = W + 2, step = J.step, nr = J.nrefs;
@@ -2757,6 +2768,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
 // strict-< comparison anyway).  Totals that need the per-pixel float chain
 // are computed by the whole wave, so every lane of the wave calls this.
 // Returns the index of the chosen guide (-1 when nothing was chosen).
+template <bool E8>
 __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pred, const dme_mv g[4],
                        const int gs[4], float lambda, dme_mv &out_mv, dme_cost &out_cost) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -2772,7 +2784,7 @@ __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pr
         mvc[q] = var_up(pred, g[q]);
         st[q] = mvc[q] * lambda;
         tot[q] = (float)gs[q] + st[q];
-        const UpStart us = up_start(b, g[q], J.step, UW, UH);
+        const UpStart us = up_start<E8>(b, g[q], J.step, UW, UH);
         need[q] = cand[q] && (us.bounds || us.e || J.fp_sem == DME_FP_C) && !chain_is_exact(st[q], gs[q]);
     }
 #pragma unroll
@@ -2791,8 +2803,8 @@ __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pr
             m.y = __builtin_amdgcn_readlane(g[q].y, src);
             const float start = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(st[q]), src));
             const int rs = __builtin_amdgcn_readlane(r, src);
-            const UpStart us = up_start(bs, m, J.step, UW, UH);
-            const float v = wave_chain(start, J.pic[0], J.W, rs ? J.qp[1] : J.qp[0], UW, UH, bs, us);
+            const UpStart us = up_start<E8>(bs, m, J.step, UW, UH);
+            const float v = wave_chain<E8>(start, J.pic[0], J.W, rs ? J.qp[1] : J.qp[0], UW, UH, bs, us);
             if (lane == src) tot[q] = v;
         }
     }
@@ -2877,8 +2889,11 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 #ifndef MODE_NB
 #define MODE_NB 4
 #endif
+// E8: the eighth-pel jobs of the launch (the others exit), else the rest.
+template <bool E8>
 __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ jobs) {
     const JobDesc &J = jobs[blockIdx.x];
+    if ((J.step == 3) != E8) return;  // uniform per workgroup
     const int XS = J.xsb, YS = J.ysb, X2 = J.xnb, X1 = J.xnb / 2;
     const int wave = wave_id(), nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
     const int nr = J.nrefs;
@@ -2942,7 +2957,7 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
             const Blk b1 = block_limits(J.blen[1], J.bsep[1], J.boff[1], J.W, J.H, xu, yu);
             dme_mv lv;
             dme_cost lc;
-            const int lq = unit_me(J, on1, b1, r1, pred1, g, gs, lam1, lv, lc);
+            const int lq = unit_me<E8>(J, on1, b1, r1, pred1, g, gs, lam1, lv, lc);
             // unit decisions, lane u for unit u
             const int sa = (2 * lane) & 63, sb = (2 * lane + 1) & 63;
             dme_cost cu[2] = {shfl_cost(lc, sa), shfl_cost(lc, sb)};
@@ -3001,17 +3016,17 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
                     m.x = __builtin_amdgcn_readlane(g0[q].x, src);
                     m.y = __builtin_amdgcn_readlane(g0[q].y, src);
                     int sd = 0;
-                    wave_up_sads(J.pic[0], J.W, src ? J.qp[1] : J.qp[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
+                    wave_up_sads<E8>(J.pic[0], J.W, src ? J.qp[1] : J.qp[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
                     if (lane == src) gs0[q] = sd;
                 }
             }
             dme_mv mv0;
             dme_cost mc0;
-            unit_me(J, lane < nr, b0, r0, pred0, g0, gs0, lam0, mv0, mc0);
+            unit_me<E8>(J, lane < nr, b0, r0, pred0, g0, gs0, lam0, mv0, mc0);
             const dme_mv va0 = shfl_mv(mv0, 0), vb0 = shfl_mv(mv0, 1);
             dme_cost c0[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
             int bis0 = 0;
-            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<MODE_NB>(J, b0, va0, vb0);
+            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<E8, MODE_NB>(J, b0, va0, vb0);
             int m0;
             const float sb0 = unit_decn(J, 0, lam0, c0, bis0, m0);
             if (sb0 <= sbc) {
@@ -4568,8 +4583,13 @@ hipError_t launch_pel_full(const JobDesc *jobs, const Task *tasks, int ntasks, i
     ly.pic = pp * 64;  // blocks of up to 64 rows
     const size_t bytes = sizeof(int) * (8 + 26) + sizeof(uint32_t) * ((size_t)ly.pic + (size_t)rows * rp);
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute((const void *)k_pel_full, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
+    static size_t attr = 0;  // the largest size set so far
+    if (bytes > attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pel_full, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bytes);
+        if (e != hipSuccess) return e;
+        attr = bytes;
+    }
     hipLaunchKernelGGL(k_pel_full, dim3(total), dim3(256), bytes, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }
@@ -4653,9 +4673,9 @@ static void pel_wave_lds(int level, int max_rows, int max_blen_x, int max_blen_y
 constexpr size_t kPelWaveLdsMax = 160 * 1024 - 1024;  // static LDS of the kernel stays below 1 KiB
 
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
-                           int max_blen_y, hipStream_t st) {
+                           int max_blen_y, int threads, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
-    const int threads = 1024;
+    if (threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;  // the kernel's launch bound
     int stage_elems = 0;
     size_t fixed = 0, per_miss = 0;
     pel_wave_lds(level, max_rows, max_blen_x, max_blen_y, stage_elems, fixed, per_miss);
@@ -4726,19 +4746,24 @@ hipError_t launch_build_qp(const PlaneTask *pt, int npt, int total, bool up8, hi
     return hipGetLastError();
 }
 
-hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, hipStream_t st) {
+hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int max_rows, int threads, int kinds,
+                              hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
-    const int threads = 512;
+    if (threads < 64 || threads > 512 || threads % 64) return hipErrorInvalidValue;  // the kernel's launch bound
     const size_t lds = subpel_wave_lds(max_rows);
     if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_subpel_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           159 * 1024);
-        if (e != hipSuccess) return e;
+        for (const void *f : {(const void *)k_subpel_wave<false>, (const void *)k_subpel_wave<true>}) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+            if (e != hipSuccess) return e;
+        }
         attr = true;
     }
-    hipLaunchKernelGGL(k_subpel_wave, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+    if (kinds & DME_KIND_BELOW_EIGHTH)
+        hipLaunchKernelGGL(k_subpel_wave<false>, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+    if (kinds & DME_KIND_EIGHTH)
+        hipLaunchKernelGGL(k_subpel_wave<true>, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
     return hipGetLastError();
 }
 
@@ -4749,11 +4774,18 @@ hipError_t launch_shift_vectors(const JobDesc *jobs, const Task *tasks, int ntas
     return hipGetLastError();
 }
 
-hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, hipStream_t st) {
+hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool up8, int kinds,
+                          hipStream_t st) {
     if (total <= 0) return hipSuccess;
     const dim3 grid((total + 4 * PW_MODE_L2 - 1) / (4 * PW_MODE_L2));
-    if (up8) hipLaunchKernelGGL(k_mode_l2<true>, grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
-    else hipLaunchKernelGGL(k_mode_l2<false>, grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (up8) {  // the fast form: quarter or half pel only
+        hipLaunchKernelGGL((k_mode_l2<true, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+        return hipGetLastError();
+    }
+    if (kinds & DME_KIND_BELOW_EIGHTH)
+        hipLaunchKernelGGL((k_mode_l2<false, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (kinds & DME_KIND_EIGHTH)
+        hipLaunchKernelGGL((k_mode_l2<false, true>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 
@@ -4769,7 +4801,7 @@ static size_t mode_tables_lds(int max_x0, int max_y0, int max_x1, int max_y1, Mt
 }
 
 hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_x0, int max_y0,
-                              int max_x1, int max_y1, bool eighth, hipStream_t st) {
+                              int max_x1, int max_y1, int kinds, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     if (max_x0 < 1 || max_y0 < 1 || max_x1 < 1 || max_y1 < 1 || max_x0 > 256 || max_x1 > 256) return hipErrorInvalidValue;
     MtLds ly;
@@ -4784,13 +4816,17 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
         attr = true;
     }
     const dim3 grid((total + 4 * PW_MODE_TABLES - 1) / (4 * PW_MODE_TABLES));
-    if (eighth) hipLaunchKernelGGL(k_mode_tables<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
-    else hipLaunchKernelGGL(k_mode_tables<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    if (kinds & DME_KIND_BELOW_EIGHTH)
+        hipLaunchKernelGGL(k_mode_tables<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
+    if (kinds & DME_KIND_EIGHTH)
+        hipLaunchKernelGGL(k_mode_tables<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, ly);
     return hipGetLastError();
 }
 
-hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, hipStream_t st) {
-    hipLaunchKernelGGL(k_mode_wave, dim3(njobs), dim3(threads), 0, st, jobs);
+hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, int kinds, hipStream_t st) {
+    if (threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;  // the kernel's launch bound
+    if (kinds & DME_KIND_BELOW_EIGHTH) hipLaunchKernelGGL(k_mode_wave<false>, dim3(njobs), dim3(threads), 0, st, jobs);
+    if (kinds & DME_KIND_EIGHTH) hipLaunchKernelGGL(k_mode_wave<true>, dim3(njobs), dim3(threads), 0, st, jobs);
     return hipGetLastError();
 }
 

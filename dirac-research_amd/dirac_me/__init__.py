@@ -9,6 +9,8 @@ surface: ``pixel_search`` (PixelMatcher::DoSearch, pixel_match.h:81),
 live sequence of PictureCompressor, picture_compress.cpp:76-298).
 """
 import ctypes as C
+import os
+import weakref
 
 import numpy as np
 
@@ -95,9 +97,12 @@ class Context:
         self.h = h
         self.params = None
         self._keep = {}
+        self._graphs = weakref.WeakSet()
 
     def close(self):
         if self.h:
+            for g in list(self._graphs):
+                g.close()
             lib().dme_destroy(self.h)
             self.h = None
 
@@ -157,8 +162,8 @@ class Context:
         check(lib().dme_motion_estimate(self.h, pic_num, a, n, C.byref(out.struct()) if out else None))
         return out
 
-    def run_batch(self, jobs, outs=None):
-        """jobs: list of (pic_num, refs, params)."""
+    @staticmethod
+    def _jobs(jobs):
         arr = (abi.dme_job * len(jobs))()
         for i, (pn, refs, prm) in enumerate(jobs):
             arr[i].pic_num = pn
@@ -166,12 +171,45 @@ class Context:
             for k, r in enumerate(refs):
                 arr[i].refs[k] = r
             arr[i].params = prm
+        return arr
+
+    def run_batch(self, jobs, outs=None):
+        """jobs: list of (pic_num, refs, params)."""
+        arr = self._jobs(jobs)
         if outs is not None:
             os_ = (abi.dme_medata * len(jobs))(*[o.struct() for o in outs])
             check(lib().dme_run_batch(self.h, arr, len(jobs), os_))
         else:
             check(lib().dme_run_batch(self.h, arr, len(jobs), None))
         return outs
+
+    # ---- launch tuning and captured batch steps ----
+    def set_tuning(self, parts=4, stagger=1, pel_wave_threads=1024, subpel_wave_threads=512, mode_wave_threads=1024):
+        t = abi.dme_tuning(parts, stagger, pel_wave_threads, subpel_wave_threads, mode_wave_threads, 0.0)
+        check(lib().dme_set_tuning(self.h, C.byref(t)))
+
+    def reset_tuning(self):
+        check(lib().dme_set_tuning(self.h, None))
+
+    def get_tuning(self):
+        t = abi.dme_tuning()
+        check(lib().dme_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def autotune_batch(self, jobs, reps=2):
+        t = abi.dme_tuning()
+        check(lib().dme_autotune_batch(self.h, self._jobs(jobs), len(jobs), reps, C.byref(t)))
+        return t
+
+    def save_tuning(self, path):
+        check(lib().dme_tuning_save(self.h, os.fsencode(path)))
+
+    def load_tuning(self, path):
+        check(lib().dme_tuning_load(self.h, os.fsencode(path)))
+
+    def batch_graph(self, jobs):
+        """dme_batch_graph_create: a BatchGraph of these jobs."""
+        return BatchGraph(self, jobs)
 
     def motion_estimate_noalloc(self, params, width, height, pic_num, d_planes, ref_nums, d_refs, out=None):
         """dme_motion_estimate_noalloc: d_planes = device addresses of the
@@ -291,6 +329,41 @@ class Context:
         b, f = C.c_longlong(), C.c_longlong()
         check(lib().dme_lp_get_stats(self.h, C.byref(b), C.byref(f)))
         return b.value, f.value
+
+
+class BatchGraph:
+    """A captured batch step (dme_batch_graph_*)."""
+
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        self.n = len(jobs)
+        h = C.c_void_p()
+        check(lib().dme_batch_graph_create(ctx.h, Context._jobs(jobs), len(jobs), C.byref(h)))
+        self.h = h
+        ctx._graphs.add(self)
+
+    def launch(self, outs=None):
+        if outs is not None:
+            if len(outs) != self.n:
+                raise ValueError("one MEData per job")
+            os_ = (abi.dme_medata * self.n)(*[o.struct() for o in outs])
+            check(lib().dme_batch_graph_launch(self.h, os_))
+        else:
+            check(lib().dme_batch_graph_launch(self.h, None))
+        return outs
+
+    def close(self):
+        # dme_destroy destroys the graphs still alive: only destroy while
+        # the context is open
+        if self.h and self.ctx.h:
+            check(lib().dme_batch_graph_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def lambda_map_host(vectors1, vectors2, lam, nrefs):

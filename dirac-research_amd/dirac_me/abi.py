@@ -98,6 +98,11 @@ class dme_imt_picture(C.Structure):
 DME_NUM_KERNELS = 32
 
 
+class dme_tuning(C.Structure):
+    _fields_ = [("parts", C.c_int32), ("stagger", C.c_int32), ("pel_wave_threads", C.c_int32),
+                ("subpel_wave_threads", C.c_int32), ("mode_wave_threads", C.c_int32), ("step_ms", C.c_float)]
+
+
 class dme_stats(C.Structure):
     _fields_ = [("sad_terms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32),
                 ("host_lambda_jobs", C.c_int32), ("reserved", C.c_int32 * 6),
@@ -142,6 +147,14 @@ SIGNATURES = {
     "dme_mode_decide": ([_P, _I, C.POINTER(dme_medata)], _I),
     "dme_motion_estimate": ([_P, _I, _P, _I, C.POINTER(dme_medata)], _I),
     "dme_run_batch": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
+    "dme_set_tuning": ([_P, C.POINTER(dme_tuning)], _I),
+    "dme_get_tuning": ([_P, C.POINTER(dme_tuning)], _I),
+    "dme_autotune_batch": ([_P, C.POINTER(dme_job), _I, _I, C.POINTER(dme_tuning)], _I),
+    "dme_tuning_save": ([_P, C.c_char_p], _I),
+    "dme_tuning_load": ([_P, C.c_char_p], _I),
+    "dme_batch_graph_create": ([_P, C.POINTER(dme_job), _I, C.POINTER(C.c_void_p)], _I),
+    "dme_batch_graph_launch": ([_P, C.POINTER(dme_medata)], _I),
+    "dme_batch_graph_destroy": ([_P], _I),
     "dme_motion_estimate_noalloc": ([_P, C.POINTER(dme_params), _I, _I, _I, _P, _P, _P, _I, _P, _P,
                                      C.POINTER(dme_medata)], _I),
     "dme_motion_compensate": ([_P, C.POINTER(dme_params), _I, _I, C.POINTER(dme_medata), _P, _I, _I,

@@ -317,6 +317,61 @@ typedef struct dme_job {
 int dme_run_batch(dme_ctx *ctx, const dme_job *jobs, int njobs,
                   dme_medata *outs);
 
+/* ---- launch tuning and the captured batch step ------------------------- *
+ * No counterpart in the reference.  A batch runs as `parts` parts of about
+ * equal work, one per stream, each part's planes built after the previous
+ * part's when `stagger` is set; the three anti-diagonal wavefront kernels run
+ * one workgroup per (picture, reference) or picture with the given thread
+ * counts.  No setting changes any result.  A batch whose signature (the
+ * multiset of its jobs' sizes, reference counts and parameters) is in the
+ * context's tuning table runs with the table's setting; any other with the
+ * context's default setting (dme_set_tuning).                               */
+typedef struct dme_tuning {
+    int32_t parts;               /* 1..4                                      */
+    int32_t stagger;             /* 0 or 1                                    */
+    int32_t pel_wave_threads;    /* k_pel_wave: 256, 512 or 1024              */
+    int32_t subpel_wave_threads; /* k_subpel_wave: 256 or 512                 */
+    int32_t mode_wave_threads;   /* k_mode_wave: 256, 512 or 1024             */
+    float   step_ms;             /* dme_autotune_batch: the setting's best
+                                    measured step (0 elsewhere)               */
+} dme_tuning;
+/* Default setting of batches not in the tuning table (NULL: the built-in
+ * default 4 parts, staggered, 1024 / 512 / 1024 threads).                   */
+int dme_set_tuning(dme_ctx *ctx, const dme_tuning *t);
+int dme_get_tuning(dme_ctx *ctx, dme_tuning *out);
+/* Times the batch under candidate settings (reps timed steps each, after one
+ * warm-up step; coordinate search: parts and stagger, then each wavefront
+ * kernel's threads), stores the fastest in the tuning table under the
+ * batch's signature and returns it in *out (may be NULL).  The pictures'
+ * planes are rebuilt in every step, as dme_invalidate_caches leaves them.    */
+int dme_autotune_batch(dme_ctx *ctx, const dme_job *jobs, int njobs, int reps,
+                       dme_tuning *out);
+/* The tuning table as text, one line per signature; load adds to (and
+ * replaces entries of) the context's table.                                 */
+int dme_tuning_save(dme_ctx *ctx, const char *path);
+int dme_tuning_load(dme_ctx *ctx, const char *path);
+
+/* A batch step captured once as a HIP graph and replayed.  create runs the
+ * batch once (sizing every buffer), then captures the whole live sequence of
+ * dme_run_batch for these jobs -- the pictures' pyramids, byte and
+ * quarter-pel planes built from their current luma, pel search, lambda maps,
+ * sub-pel refinement, mode decision, DC, the INTRA counts -- across the
+ * streams of its tuning setting.  The graph owns the MEData buffers of its
+ * jobs (not shared with the context's other entry points) and its own
+ * staging.  launch replays it, then does on the host what a graph cannot:
+ * jobs whose device lambda threshold was undecided are redone with the
+ * host's maps, and outs (may be NULL; else njobs entries) receive the
+ * MEData.  Replays read the pictures' current contents: the planes of a
+ * picture bound with dme_bind_picture_device may be rewritten between
+ * replays.  Re-registering or dropping a picture of the graph makes launch
+ * fail with DME_ERR_STATE; the graph must then be created again.  Results
+ * equal dme_run_batch's bit for bit.                                         */
+typedef struct dme_batch_graph dme_batch_graph;
+int dme_batch_graph_create(dme_ctx *ctx, const dme_job *jobs, int njobs,
+                           dme_batch_graph **out);
+int dme_batch_graph_launch(dme_batch_graph *g, dme_medata *outs);
+int dme_batch_graph_destroy(dme_batch_graph *g);
+
 /* ---- no-allocation motion estimation ----------------------------------- *
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
