@@ -42,12 +42,17 @@ struct JobDesc {
     // planes
     const int16_t DME_G *pic[DME_MAX_DEPTH + 1];      // [0] = Y of the picture
     const int16_t DME_G *ref[2][DME_MAX_DEPTH + 1];   // reference pyramids
-    const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp)
+    const uint32_t DME_G *qp[2];                      // quarter-pel planes of the refs (k_build_qp); with comb,
+                                                      // the refs' int16 up planes (2W x 2H) instead
     const uint8_t DME_G *pic8, *ref8[2];              // byte planes of picture and refs (k_bytes): sample (0, 0)
     const uint8_t DME_G *up8[2];                      // byte up-planes of the refs (fast form): up sample (0, 0)
     int u8p;                                          // their row pitch in bytes
     int p8;                                           // their row pitch in bytes
     const int16_t DME_G *cu, *cv;                     // chroma of the picture
+    const int16_t DME_G *py;                          // luma of the picture (SetDC reads it; pic[0] may be
+                                                      // the combined-component picture)
+    int comb;                                         // combined-component ME: pic and ref are the combined
+                                                      // pictures' pyramids and qp[r] their int16 up planes
     int lp_off;                                       // first position of the picture in the proposal tensors
     int lp_slot;                                      // its pair of refinement counters (k_lp_refine)
     int full;                                         // full search: depth 0, zero window of fxr x fyr (k_pel_full)
@@ -111,6 +116,15 @@ struct PlaneTask {
     int in_w, ow, oh, offset;
 };
 
+// The combined-component picture of a 4:2:0 picture (k_combine): W x H luma,
+// W/2 x H/2 chroma, covering 256-sample tiles [offset, offset + tiles).
+struct CombTask {
+    const int16_t DME_G *y, *u, *v;
+    int16_t DME_G *out;
+    int W, H, offset;
+};
+inline int combine_tiles(int W, int H) { return (int)(((size_t)W * H + 255) / 256); }
+
 // k_upconvert tiles: 64 x 16 source samples.
 constexpr int kUpTileW = 64, kUpTileH = 16;
 inline int upconvert_tiles(int w, int h) { return ((w + kUpTileW - 1) / kUpTileW) * ((h + kUpTileH - 1) / kUpTileH); }
@@ -153,6 +167,8 @@ struct McPackJob {
 // (a bit mask): their eighth-pel forms are separate instantiations, so that
 // the others keep their register budget.
 constexpr int DME_KIND_BELOW_EIGHTH = 1, DME_KIND_EIGHTH = 2;
+// and the combined-component jobs (int16 up planes) of each
+constexpr int DME_KIND_COMB_BELOW_EIGHTH = 4, DME_KIND_COMB_EIGHTH = 8;
 
 // One unit of a batched launch: a (job, reference) pair covering `count`
 // consecutive work items starting at `offset` of the launch's index space.

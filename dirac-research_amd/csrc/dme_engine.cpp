@@ -42,7 +42,11 @@ hipError_t launch_pel_full(const JobDesc *jobs, const Task *tasks, int ntasks, i
 hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int *flags, hipStream_t st);
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
-                           int max_blen_y, bool has_guide, hipStream_t st);
+                           int max_blen_y, bool has_guide, bool u8, hipStream_t st);
+hipError_t launch_combine(const CombTask *ct, int nct, int total, hipStream_t st);
+hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_mode_tables16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool eighth,
+                                hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
                            int max_blen_y, int threads, hipStream_t st);
 hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
@@ -170,6 +174,14 @@ struct Picture {
     bool b8_ready = false;
     DevBuf u8;      // byte up-plane with margins (the learned path's fast form)
     bool u8_ready = false;
+    // combined-component ME (EncPicture::CombinedData, enc_picture.cpp:237-
+    // 254): the combined picture as a picture of its own (its y in own[0],
+    // its pyramid and int16 up plane built as a luma picture's are), made
+    // from this picture's Y, U, V when a job needs it
+    std::unique_ptr<Picture> cpic;
+    bool comb_ready = false;
+    bool is_comb = false;  // this is such a combined picture (no byte planes)
+    int chroma_ok = -1;    // chroma within [-128, 127]: -1 not yet checked
     const uint8_t *u8_origin() const { return u8.as<uint8_t>() + (size_t)kUp8Margin * up8_pitch(W) + kUp8Margin; }
     const uint8_t *b8_origin() const { return b8.as<uint8_t>() + (size_t)kB8Margin * bp + kB8Margin; }
 };
@@ -223,7 +235,7 @@ struct PinnedBuf {
 // tables, plane tasks and the pinned staging arena of the current call.
 struct Lane {
     hipStream_t stream = nullptr;
-    DevBuf d_jobs, d_tasks, d_planes;
+    DevBuf d_jobs, d_tasks, d_planes, d_planes16, d_comb;
     PinnedBuf arena;
     size_t arena_used = 0;
 };
@@ -350,13 +362,14 @@ enum KernelKind {
     KK_DOWNCONVERT, KK_UPCONVERT, KK_PEL_BULK, KK_PEL_WAVE, KK_SUBPEL_BULK, KK_SUBPEL_WAVE, KK_SHIFT,
     KK_MODE_L2, KK_MODE_TABLES, KK_MODE_WAVE, KK_BLOCK_DC, KK_DC_SMOOTH, KK_BUILD_QP, KK_LP_NET, KK_LP_REFINE,
     KK_BYTES, KK_LAMBDA, KK_MC, KK_MC_PAD, KK_MC_PACK, KK_PEL_FULL, KK_FAST_SUBPEL, KK_FAST_L2, KK_FAST_MODE,
-    KK_FAST_EVAL, KK_BUILD_UP8, KK_INTRA, KK_COUNT
+    KK_FAST_EVAL, KK_BUILD_UP8, KK_INTRA, KK_COMBINE, KK_SUBPEL_T16, KK_MODE_T16, KK_COUNT
 };
 static const char *const kKernelNames[DME_NUM_KERNELS] = {
     "k_downconvert", "k_upconvert", "k_pel_bulk", "k_pel_wave", "k_subpel_bulk", "k_subpel_wave",
     "k_shift_vectors", "k_mode_l2", "k_mode_tables", "k_mode_wave", "k_block_dc", "k_dc_smooth", "k_build_qp",
     "k_lp_net", "k_lp_refine", "k_bytes", "k_lambda", "k_mc", "k_mc_pad", "k_mc_pack", "k_pel_full",
-    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval", "k_build_up8", "k_intra_count"};
+    "k_fast_subpel", "k_fast_l2", "k_fast_mode", "k_fast_eval", "k_build_up8", "k_intra_count", "k_combine",
+    "k_subpel_table16", "k_mode_tables16"};
 static_assert(KK_COUNT <= DME_NUM_KERNELS, "kernel kinds");
 
 // Runs a launch; with profiling on, brackets it with a pair of events.
@@ -419,12 +432,77 @@ static int upload(dme_ctx *ctx, DevBuf &dst, const void *src, size_t bytes) {
     return DME_OK;
 }
 
-// Builds the missing pyramid levels (PixelMatcher::MakePicHierarchy,
+// What a set of jobs needs built: pyramid depth per picture (luma pictures
+// and combined pictures alike), the references' quarter-pel planes, byte
+// up-planes (fast form) and int16 up planes (combined pictures), and the
+// pictures whose combined picture must be made first.
+struct PlaneNeeds {
+    std::map<Picture *, int> levels;
+    std::vector<Picture *> ups, ups8, ups16, combs;
+};
+
+static void add_once(std::vector<Picture *> &v, Picture *P) {
+    if (std::find(v.begin(), v.end(), P) == v.end()) v.push_back(P);
+}
+
+// Builds the missing combined pictures (EncPicture::Combine,
+// enc_picture.cpp:285-367), pyramid levels (PixelMatcher::MakePicHierarchy,
 // pixel_match.cpp:159-183: each level from the previous, sizes from full
-// resolution) and up-converted planes (EncPicture::UpOrigData,
-// enc_picture.cpp:131-156) of a set of pictures, one launch per level.
-static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, const std::vector<Picture *> &ups,
-                        const std::vector<Picture *> &ups8, int &launches) {
+// resolution) and up-converted planes (EncPicture::UpOrigData /
+// UpCombinedData, enc_picture.cpp:131-156, 256-280) of a set of pictures,
+// one launch per level.
+static int build_planes(dme_ctx *ctx, const PlaneNeeds &nd, int &launches) {
+    const std::map<Picture *, int> &levels = nd.levels;
+    const std::vector<Picture *> &ups = nd.ups, &ups8 = nd.ups8;
+    {
+        std::vector<CombTask> ct;
+        int off = 0;
+        for (Picture *P : nd.combs) {
+            if (P->comb_ready) continue;
+            CombTask t;
+            t.y = P->y;
+            t.u = P->u;
+            t.v = P->v;
+            t.out = P->cpic->own[0].as<int16_t>();
+            t.W = P->W;
+            t.H = P->H;
+            t.offset = off;
+            off += combine_tiles(P->W, P->H);
+            ct.push_back(t);
+        }
+        if (!ct.empty()) {
+            DME_TRY(upload(ctx, ctx->L->d_comb, ct.data(), sizeof(CombTask) * ct.size()));
+            DME_TRY(timed_launch(ctx, KK_COMBINE, launches, [&] {
+                return launch_combine(ctx->L->d_comb.as<CombTask>(), (int)ct.size(), off, ctx->L->stream);
+            }));
+            for (Picture *P : nd.combs) P->comb_ready = true;
+        }
+    }
+    {
+        // int16 up planes of the combined references (k_upconvert; their
+        // quarter-pel samples are formed from it on the fly)
+        std::vector<PlaneTask> ut;
+        int off = 0;
+        for (Picture *P : nd.ups16) {
+            if (P->up_ready) continue;
+            PlaneTask t;
+            t.in = P->y;
+            t.out = P->up.as<int16_t>();
+            t.in_w = P->W;
+            t.ow = P->W;
+            t.oh = P->H;
+            t.offset = off;
+            off += upconvert_tiles(P->W, P->H);
+            ut.push_back(t);
+        }
+        if (!ut.empty()) {
+            DME_TRY(upload(ctx, ctx->L->d_planes16, ut.data(), sizeof(PlaneTask) * ut.size()));
+            DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches, [&] {
+                return launch_upconvert(ctx->L->d_planes16.as<PlaneTask>(), (int)ut.size(), off, ctx->L->stream);
+            }));
+            for (Picture *P : nd.ups16) P->up_ready = true;
+        }
+    }
     int maxl = 0;
     for (auto &kv : levels) maxl = std::max(maxl, kv.second);
     std::vector<PlaneTask> all;
@@ -494,7 +572,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         int off = 0;
         for (auto &kv : levels) {
             Picture &P = *kv.first;
-            if (P.b8_ready) continue;
+            if (P.b8_ready || P.is_comb) continue;  // combined pictures are searched on 16-bit samples
             P.bp = (P.W + 2 * kB8Margin + 63) & ~63;
             const size_t dw = (size_t)(P.H + 2 * kB8Margin) * (P.bp / 4);
             DME_TRY(P.b8.ensure(sizeof(uint32_t) * dw));
@@ -536,6 +614,7 @@ static int build_planes(dme_ctx *ctx, const std::map<Picture *, int> &levels, co
         DME_TRY(timed_launch(ctx, KK_BUILD_UP8, launches,
                              [&] { return launch_build_qp(dp + ru.first, n, ru.second, true, ctx->L->stream); }));
     }
+
     for (auto &kv : levels) kv.first->pyr_levels = std::max(kv.first->pyr_levels, kv.second);
     for (Picture *P : ups) P->qp_ready = true;
     for (Picture *P : ups8) P->u8_ready = true;
@@ -563,6 +642,61 @@ static size_t full_lds_bytes(int xl, int yl, int xr, int yr) {
     return sizeof(int) * (8 + 26) + 4 * (pp * 64 + (size_t)(yl + 2 * std::max(yr, 2)) * full_pitch(xl, xr));
 }
 
+// Chroma of a picture within 8-bit range (checked once per registration):
+// EncPicture::Combine's float squares and sums are then exact.
+static int check_chroma(dme_ctx *ctx, Picture *P) {
+    if (P->chroma_ok < 0) {
+        const size_t cn = (size_t)(P->W / 2) * (P->H / 2);
+        DME_TRY(ctx->d_flag.ensure(sizeof(int)));
+        int bad = 0;
+        HIP_TRY(hipMemcpyAsync(ctx->d_flag.p, &bad, sizeof(int), hipMemcpyHostToDevice, ctx->L->stream));
+        HIP_TRY(launch_check_range(P->u, cn, ctx->d_flag.as<int>(), ctx->L->stream));
+        HIP_TRY(launch_check_range(P->v, cn, ctx->d_flag.as<int>(), ctx->L->stream));
+        HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->L->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->L->stream));
+        P->chroma_ok = bad ? 0 : 1;
+    }
+    if (!P->chroma_ok)
+        return set_error(DME_ERR_INVALID, "combined-component ME: chroma of picture " + std::to_string(P->num) +
+                                              " must lie in [-128, 127] (8-bit samples minus 128)");
+    return DME_OK;
+}
+
+// A combined-component job (EncPicture::DataForME(true)): its pictures need
+// chroma; their combined pictures are made (buffers allocated here, so that
+// no later stage allocates).  The learned forms, full search and the
+// no-allocation entry point search luma only.
+static int prepare_combined(dme_ctx *ctx, Picture *P, Picture *const *R, int nrefs, const dme_params &p,
+                            bool learned) {
+    if (learned)
+        return set_error(DME_ERR_INVALID, "the learned forms estimate on luma: combined_me needs the plain search");
+    if (p.full_search)
+        return set_error(DME_ERR_INVALID, "combined-component ME with full search is not supported");
+    if (ctx->na_npics)
+        return set_error(DME_ERR_INVALID, "the no-allocation entry point estimates on luma (combined_me = 0)");
+    Picture *all[3] = {P, R[0], nrefs > 1 ? R[1] : nullptr};
+    for (Picture *Q : all) {
+        if (!Q) continue;
+        if (!Q->u || !Q->v)
+            return set_error(DME_ERR_INVALID, "combined-component ME needs the chroma of picture " +
+                                                  std::to_string(Q->num));
+        DME_TRY(check_chroma(ctx, Q));
+        if (!Q->cpic) {
+            std::unique_ptr<Picture> C(new Picture());
+            C->num = Q->num;
+            C->W = Q->W;
+            C->H = Q->H;
+            C->is_comb = true;
+            DME_TRY(C->own[0].ensure(sizeof(int16_t) * (size_t)Q->W * Q->H));
+            C->y = C->own[0].as<int16_t>();
+            Q->cpic = std::move(C);
+            Q->comb_ready = false;
+        }
+    }
+    for (int r = 0; r < nrefs; ++r) DME_TRY(R[r]->cpic->up.ensure(sizeof(int16_t) * 4 * (size_t)R[r]->W * R[r]->H));
+    return DME_OK;
+}
+
 static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_num, const int *refs, int nrefs,
                        bool learned = false) {
     // buffers below may be reallocated: nothing queued may still use them
@@ -581,9 +715,10 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     DME_TRY(make_geometry(p, P->W, P->H, J.g));
     const Geometry &g = J.g;
     {
-        const std::string lim = launch_limits(g, p.full_search != 0, learned);
+        const std::string lim = launch_limits(g, p.full_search != 0, learned, p.combined_me != 0);
         if (!lim.empty()) return set_error(DME_ERR_INVALID, "unsupported picture or block size: " + lim);
     }
+    if (p.combined_me) DME_TRY(prepare_combined(ctx, P, R, nrefs, p, learned));
     int fxr[2] = {0, 0}, fyr[2] = {0, 0};
     if (p.full_search) {
         for (int r = 0; r < 2; ++r) {
@@ -668,6 +803,8 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     for (int a = 0; a < 2; ++a) { d.cblen[a] = g.cblen[a]; d.cbsep[a] = g.cbsep[a]; d.cboff[a] = g.cboff[a]; }
     for (int l = 0; l < 3; ++l) d.lf[l] = g.level_factor[l];
     d.pic[0] = P->y;
+    d.py = P->y;
+    d.comb = p.combined_me != 0;
     d.cu = P->u;
     d.cv = P->v;
     for (int r = 0; r < 2; ++r) {
@@ -710,23 +847,29 @@ static int prepare_job(dme_ctx *ctx, JobState &J, const dme_params &p, int pic_n
     return DME_OK;
 }
 
-// Builds what a batch's pictures need, then fills the descriptors' plane
-// pointers.
+// The pictures a job's search reads: the picture and its references, or
+// for combined-component ME their combined pictures.
+static Picture *me_pic(Picture *P, const JobState *j) { return j->p.combined_me ? P->cpic.get() : P; }
+
 // The pictures a set of jobs reads: pyramid depth needed per picture, and the
 // references (whose quarter-pel planes the sub-pel and mode stages read).
-static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool need_up,
-                          std::map<Picture *, int> &levels, std::vector<Picture *> &ups,
-                          std::vector<Picture *> &ups8) {
+static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool need_up, PlaneNeeds &nd) {
     for (auto *j : js) {
-        Picture *P = find_pic(ctx, j->pic_num);
-        if (!P) return set_error(DME_ERR_STATE, "picture was dropped");
-        levels[P] = std::max(levels[P], j->g.depth);
+        Picture *P0 = find_pic(ctx, j->pic_num);
+        if (!P0) return set_error(DME_ERR_STATE, "picture was dropped");
+        const bool comb = j->p.combined_me != 0;
+        if (comb && !P0->cpic) return set_error(DME_ERR_STATE, "combined picture missing");
+        if (comb) add_once(nd.combs, P0);
+        Picture *P = me_pic(P0, j);
+        nd.levels[P] = std::max(nd.levels[P], j->g.depth);
         for (int r = 0; r < j->nrefs; ++r) {
-            Picture *R = find_pic(ctx, j->refs[r]);
-            if (!R) return set_error(DME_ERR_STATE, "reference was dropped");
-            levels[R] = std::max(levels[R], j->g.depth);
-            std::vector<Picture *> &u = j->fast ? ups8 : ups;
-            if (need_up && std::find(u.begin(), u.end(), R) == u.end()) u.push_back(R);
+            Picture *R0 = find_pic(ctx, j->refs[r]);
+            if (!R0) return set_error(DME_ERR_STATE, "reference was dropped");
+            if (comb && !R0->cpic) return set_error(DME_ERR_STATE, "combined picture missing");
+            if (comb) add_once(nd.combs, R0);
+            Picture *R = me_pic(R0, j);
+            nd.levels[R] = std::max(nd.levels[R], j->g.depth);
+            if (need_up) add_once(comb ? nd.ups16 : (j->fast ? nd.ups8 : nd.ups), R);
         }
     }
     return DME_OK;
@@ -736,12 +879,12 @@ static int collect_planes(dme_ctx *ctx, const std::vector<JobState *> &js, bool 
 // or more parts read (all they need of them), so that no lane builds a plane
 // another lane reads.
 static int build_shared_planes(dme_ctx *ctx, const std::vector<std::vector<JobState *>> &parts, int &launches) {
-    std::vector<std::map<Picture *, int>> lv(parts.size());
-    std::vector<std::vector<Picture *>> up(parts.size()), up8(parts.size());
+    std::vector<PlaneNeeds> nd(parts.size());
     std::map<Picture *, int> seen, shared;
+    std::map<Picture *, int> comb_parts;  // pictures whose combined picture a part needs: parts counted
     for (size_t h = 0; h < parts.size(); ++h) {
-        DME_TRY(collect_planes(ctx, parts[h], true, lv[h], up[h], up8[h]));
-        for (auto &kv : lv[h]) {
+        DME_TRY(collect_planes(ctx, parts[h], true, nd[h]));
+        for (auto &kv : nd[h].levels) {
             auto it = seen.find(kv.first);
             if (it == seen.end()) {
                 seen[kv.first] = kv.second;
@@ -750,36 +893,45 @@ static int build_shared_planes(dme_ctx *ctx, const std::vector<std::vector<JobSt
                 shared[kv.first] = it->second;
             }
         }
+        for (Picture *P : nd[h].combs) ++comb_parts[P];
     }
-    if (shared.empty()) return DME_OK;
-    for (auto &kv : shared) kv.second = seen[kv.first];
-    std::vector<Picture *> sup, sup8;
-    for (const auto &u : up)
-        for (Picture *P : u)
-            if (shared.count(P) && std::find(sup.begin(), sup.end(), P) == sup.end()) sup.push_back(P);
-    for (const auto &u : up8)
-        for (Picture *P : u)
-            if (shared.count(P) && std::find(sup8.begin(), sup8.end(), P) == sup8.end()) sup8.push_back(P);
-    return build_planes(ctx, shared, sup, sup8, launches);
+    PlaneNeeds sn;
+    for (auto &kv : comb_parts)
+        if (kv.second > 1 || shared.count(kv.first->cpic.get())) sn.combs.push_back(kv.first);
+    if (shared.empty() && sn.combs.empty()) return DME_OK;
+    for (auto &kv : shared) sn.levels[kv.first] = seen[kv.first];
+    for (const auto &n : nd) {
+        for (Picture *P : n.ups)
+            if (shared.count(P)) add_once(sn.ups, P);
+        for (Picture *P : n.ups8)
+            if (shared.count(P)) add_once(sn.ups8, P);
+        for (Picture *P : n.ups16)
+            if (shared.count(P)) add_once(sn.ups16, P);
+    }
+    return build_planes(ctx, sn, launches);
 }
 
+// Builds what a batch's pictures need, then fills the descriptors' plane
+// pointers.
 static int bind_planes(dme_ctx *ctx, std::vector<JobState *> &js, bool need_up, int &launches) {
-    std::map<Picture *, int> levels;
-    std::vector<Picture *> ups, ups8;
-    DME_TRY(collect_planes(ctx, js, need_up, levels, ups, ups8));
-    DME_TRY(build_planes(ctx, levels, ups, ups8, launches));
+    PlaneNeeds nd;
+    DME_TRY(collect_planes(ctx, js, need_up, nd));
+    DME_TRY(build_planes(ctx, nd, launches));
     for (auto *j : js) {
-        Picture *P = find_pic(ctx, j->pic_num);
+        Picture *P0 = find_pic(ctx, j->pic_num), *P = me_pic(P0, j);
+        const bool comb = j->p.combined_me != 0;
         j->desc.pic[0] = P->y;
         for (int l = 1; l <= j->g.depth; ++l) j->desc.pic[l] = P->pyr[l].as<int16_t>();
-        j->desc.pic8 = P->b8_origin();
-        j->desc.p8 = P->bp;
+        j->desc.py = P0->y;
+        j->desc.pic8 = comb ? nullptr : P->b8_origin();
+        j->desc.p8 = comb ? 0 : P->bp;
         for (int r = 0; r < j->nrefs; ++r) {
-            Picture *R = find_pic(ctx, j->refs[r]);
+            Picture *R = me_pic(find_pic(ctx, j->refs[r]), j);
             j->desc.ref[r][0] = R->y;
             for (int l = 1; l <= j->g.depth; ++l) j->desc.ref[r][l] = R->pyr[l].as<int16_t>();
-            j->desc.ref8[r] = R->b8_origin();
-            if (need_up && !j->fast) j->desc.qp[r] = R->qp.as<uint32_t>();
+            j->desc.ref8[r] = comb ? nullptr : R->b8_origin();
+            if (need_up && comb) j->desc.qp[r] = R->up.as<uint32_t>();
+            if (need_up && !comb && !j->fast) j->desc.qp[r] = R->qp.as<uint32_t>();
             if (need_up && j->fast) {
                 j->desc.up8[r] = R->u8_origin();
                 j->desc.u8p = up8_pitch(R->W);
@@ -827,18 +979,27 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
     for (auto *j : js) maxd = std::max(maxd, j->g.depth);
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
+    // per level: the jobs searched from 16-bit samples, and at level 0 the
+    // luma jobs searched from the byte planes
     std::vector<TaskTable::Range> bulk(maxd + 1);
+    TaskTable::Range bulk0_u8 = tt.add(std::vector<Task>());
     for (int l = maxd; l >= stop; --l) {
-        std::vector<Task> t;
-        int off = 0;
+        std::vector<Task> t, t8;
+        int off = 0, off8 = 0;
         for (size_t i = 0; i < js.size(); ++i)
             for (int r = 0; r < js[i]->nrefs; ++r) {
                 if (l > js[i]->g.depth || js[i]->p.full_search) continue;
                 const int cnt = js[i]->g.lxb[l] * js[i]->g.lyb[l];
-                t.push_back(Task{(int)i, r, cnt, off});
-                off += cnt;
+                if (l == 0 && !js[i]->p.combined_me) {
+                    t8.push_back(Task{(int)i, r, cnt, off8});
+                    off8 += cnt;
+                } else {
+                    t.push_back(Task{(int)i, r, cnt, off});
+                    off += cnt;
+                }
             }
         bulk[l] = tt.add(t);
+        if (l == 0) bulk0_u8 = tt.add(t8);
     }
     // full-search jobs: their level-0 tables in k_pel_full
     std::vector<Task> ft;
@@ -874,7 +1035,12 @@ static int stage_pel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches, i
                 guide = guide || l < j->g.depth;
             }
         DME_TRY(timed_launch(ctx, KK_PEL_BULK, launches, [&] {
-            return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, bx, by, guide,
+            if (l == 0) {
+                const hipError_t e = launch_pel_bulk(dj, dtask(ctx, bulk0_u8), bulk0_u8.n, bulk0_u8.total, 0, bx, by,
+                                                     guide, true, ctx->L->stream);
+                if (e != hipSuccess) return e;
+            }
+            return launch_pel_bulk(dj, dtask(ctx, bulk[l]), bulk[l].n, bulk[l].total, l, bx, by, guide, false,
                                    ctx->L->stream);
         }));
         if (l == 0 && full.total > 0)
@@ -1007,7 +1173,11 @@ static int stage_lambda(dme_ctx *ctx, std::vector<JobState *> &js, const float *
 // Which kinds of job (DME_KIND_*) a set of jobs holds.
 static int job_kinds(const std::vector<JobState *> &js) {
     int k = 0;
-    for (auto *j : js) k |= j->p.mv_precision == 3 ? DME_KIND_EIGHTH : DME_KIND_BELOW_EIGHTH;
+    for (auto *j : js) {
+        const bool e = j->p.mv_precision == 3;
+        if (j->p.combined_me) k |= e ? DME_KIND_COMB_EIGHTH : DME_KIND_COMB_BELOW_EIGHTH;
+        else k |= e ? DME_KIND_EIGHTH : DME_KIND_BELOW_EIGHTH;
+    }
     return k;
 }
 
@@ -1015,8 +1185,8 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
     DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
-    std::vector<Task> wt, sh, bt;
-    int off = 0, offb = 0, rows = 1, bx = 1, by = 1;
+    std::vector<Task> wt, sh, bt, b16;
+    int off = 0, offb = 0, off16 = 0, rows = 1, bx = 1, by = 1;
     for (size_t i = 0; i < js.size(); ++i)
         for (int r = 0; r < js[i]->nrefs; ++r) {
             const int cnt = js[i]->g.xnb * js[i]->g.ynb;
@@ -1025,21 +1195,31 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
                 off += cnt;
             } else {
                 wt.push_back(Task{(int)i, r, 1, (int)wt.size()});
-                bt.push_back(Task{(int)i, r, cnt, offb});
-                offb += cnt;
+                if (js[i]->p.combined_me) {
+                    b16.push_back(Task{(int)i, r, cnt, off16});
+                    off16 += cnt;
+                } else {
+                    bt.push_back(Task{(int)i, r, cnt, offb});
+                    offb += cnt;
+                }
                 rows = std::max(rows, js[i]->g.ynb);
                 bx = std::max(bx, js[i]->g.blen[2][0]);
                 by = std::max(by, js[i]->g.blen[2][1]);
             }
         }
-    const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt);
+    const TaskTable::Range wave = tt.add(wt), shift = tt.add(sh), bulk = tt.add(bt), bulk16 = tt.add(b16);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     const int kinds = job_kinds(js);
-    if (bulk.n) {
+    if (bulk.n)
         DME_TRY(timed_launch(ctx, KK_SUBPEL_BULK, launches, [&] {
             return launch_subpel_bulk(dj, dtask(ctx, bulk), bulk.n, bulk.total, bx, by, ctx->L->stream);
         }));
+    if (bulk16.n)
+        DME_TRY(timed_launch(ctx, KK_SUBPEL_T16, launches, [&] {
+            return launch_subpel_table16(dj, dtask(ctx, bulk16), bulk16.n, bulk16.total, ctx->L->stream);
+        }));
+    if (wave.n) {
         DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,
                              [&] {
                                  return launch_subpel_wave(dj, dtask(ctx, wave), wave.n, rows,
@@ -1061,16 +1241,20 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
     DME_TRY(bind_planes(ctx, js, true, launches));
     DME_TRY(upload_jobs(ctx, js));
     TaskTable tt;
-    std::vector<Task> l2, dc, dr, sh, mt;
-    int off = 0, offd = 0, offr = 0, offs = 0, offm = 0, max_w = 0;
+    std::vector<Task> l2, dc, dr, sh, mt, mt16, mt16e;
+    int off = 0, offd = 0, offr = 0, offs = 0, offm = 0, offm16 = 0, offm16e = 0, max_w = 0;
     for (size_t i = 0; i < js.size(); ++i) {
         const int cnt = js[i]->g.xnb * js[i]->g.ynb;
         l2.push_back(Task{(int)i, 0, cnt, off});
         off += cnt;
-        mt.push_back(Task{(int)i, 1, cnt / 4, offm});
-        offm += cnt / 4;
-        mt.push_back(Task{(int)i, 0, cnt / 16, offm});
-        offm += cnt / 16;
+        // unit tables: luma jobs in k_mode_tables, combined-component jobs in
+        // k_mode_tables16 (eighth-pel ones apart)
+        std::vector<Task> &m = js[i]->p.combined_me ? (js[i]->p.mv_precision == 3 ? mt16e : mt16) : mt;
+        int &om = js[i]->p.combined_me ? (js[i]->p.mv_precision == 3 ? offm16e : offm16) : offm;
+        m.push_back(Task{(int)i, 1, cnt / 4, om});
+        om += cnt / 4;
+        m.push_back(Task{(int)i, 0, cnt / 16, om});
+        om += cnt / 16;
         for (int c = 0; c < 3; ++c) {
             dc.push_back(Task{(int)i, c, cnt, offd});
             offd += cnt;
@@ -1085,6 +1269,7 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
             }
     }
     const TaskTable::Range rl2 = tt.add(l2), rdc = tt.add(dc), rdr = tt.add(dr), rsh = tt.add(sh), rmt = tt.add(mt);
+    const TaskTable::Range rmt16 = tt.add(mt16), rmt16e = tt.add(mt16e);
     DME_TRY(upload_tasks(ctx, tt));
     const JobDesc *dj = ctx->L->d_jobs.as<JobDesc>();
     hipStream_t st = ctx->L->stream;
@@ -1098,9 +1283,16 @@ static int stage_mode(dme_ctx *ctx, std::vector<JobState *> &js, int &launches) 
         mx1 = std::max(mx1, j->g.blen[1][0]);
         my1 = std::max(my1, j->g.blen[1][1]);
     }
-    DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches, [&] {
-        return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, kinds, st);
-    }));
+    if (rmt.n)
+        DME_TRY(timed_launch(ctx, KK_MODE_TABLES, launches, [&] {
+            return launch_mode_tables(dj, dtask(ctx, rmt), rmt.n, rmt.total, mx0, my0, mx1, my1, kinds, st);
+        }));
+    if (rmt16.n || rmt16e.n)
+        DME_TRY(timed_launch(ctx, KK_MODE_T16, launches, [&] {
+            const hipError_t e = launch_mode_tables16(dj, dtask(ctx, rmt16), rmt16.n, rmt16.total, false, st);
+            if (e != hipSuccess) return e;
+            return launch_mode_tables16(dj, dtask(ctx, rmt16e), rmt16e.n, rmt16e.total, true, st);
+        }));
     DME_TRY(timed_launch(ctx, KK_MODE_WAVE, launches,
                          [&] { return launch_mode_wave(dj, (int)js.size(), ctx->act.mode_wave_threads, kinds, st); }));
     DME_TRY(timed_launch(ctx, KK_BLOCK_DC, launches,
@@ -1719,7 +1911,11 @@ static void invalidate_batch_planes(dme_ctx *ctx, const std::vector<JobState *> 
             Picture *P = find_pic(ctx, nums[k]);
             if (!P) continue;
             P->pyr_levels = 0;
-            P->qp_ready = P->b8_ready = P->u8_ready = P->up_ready = false;
+            P->qp_ready = P->b8_ready = P->u8_ready = P->up_ready = P->comb_ready = false;
+            if (P->cpic) {
+                P->cpic->pyr_levels = 0;
+                P->cpic->up_ready = false;
+            }
         }
     }
 }
@@ -1731,6 +1927,8 @@ static void swap_graph_buffers(dme_ctx *ctx, dme_batch_graph *g) {
         a.d_jobs.swap(b.d_jobs);
         a.d_tasks.swap(b.d_tasks);
         a.d_planes.swap(b.d_planes);
+        a.d_planes16.swap(b.d_planes16);
+        a.d_comb.swap(b.d_comb);
         a.arena.swap(b.arena);
         std::swap(a.arena_used, b.arena_used);
     }
@@ -1985,10 +2183,12 @@ extern "C" int dme_batch_graph_launch(dme_batch_graph *g, dme_medata *outs) try 
     for (auto *j : js) {
         int pn[3] = {j->pic_num, j->refs[0], j->refs[1]};
         for (int k = 0; k <= j->nrefs; ++k) {
-            Picture *P = find_pic(ctx, pn[k]);
+            Picture *P0 = find_pic(ctx, pn[k]);
+            if (j->p.combined_me) P0->comb_ready = true;
+            Picture *P = me_pic(P0, j);
             P->pyr_levels = std::max(P->pyr_levels, j->g.depth);
-            P->b8_ready = true;
-            if (k > 0) P->qp_ready = true;
+            if (!j->p.combined_me) P->b8_ready = true;
+            if (k > 0) (j->p.combined_me ? P->up_ready : P->qp_ready) = true;
         }
     }
     DevBuf *saved = ctx->lflags;
@@ -2114,10 +2314,12 @@ extern "C" int dme_motion_estimate_noalloc(dme_ctx *ctx, const dme_params *param
     for (int r = 0; r < nrefs; ++r)
         if (!d_ref_y[r]) return set_error(DME_ERR_INVALID, "NULL reference plane");
     if (!ctx->scratch) return set_error(DME_ERR_STATE, "no scratch array on this device");
+    if (params->combined_me)
+        return set_error(DME_ERR_INVALID, "the no-allocation entry point estimates on luma (combined_me = 0)");
     Geometry g;
     DME_TRY(make_geometry(*params, width, height, g));
     {
-        const std::string lim = launch_limits(g, params->full_search != 0, false);
+        const std::string lim = launch_limits(g, params->full_search != 0, false, false);
         if (!lim.empty()) return set_error(DME_ERR_INVALID, "unsupported picture or block size: " + lim);
     }
     // every work buffer from the scratch array; nothing is allocated
@@ -2563,8 +2765,24 @@ extern "C" int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level, int
         HIP_TRY(hipMemcpy(b.data(), P->u8.p, n, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) out[i] = (int16_t)((int)b[i] - 128);
         return DME_OK;
+    } else if (kind >= 3 && kind <= 5) {
+        // the combined-component picture, its pyramid levels, its up plane
+        const Picture *C = P->cpic.get();
+        if (!C || !P->comb_ready) return set_error(DME_ERR_STATE, "combined picture not built");
+        if (kind == 3) {
+            src = C->y;
+            n = (size_t)P->W * P->H;
+        } else if (kind == 4) {
+            if (level < 1 || level > C->pyr_levels) return set_error(DME_ERR_STATE, "pyramid level not built");
+            src = C->pyr[level].p;
+            n = (size_t)(P->W >> level) * (P->H >> level);
+        } else {
+            if (!C->up_ready) return set_error(DME_ERR_STATE, "up-converted plane not built");
+            src = C->up.p;
+            n = 4 * (size_t)P->W * P->H;
+        }
     } else {
-        return set_error(DME_ERR_INVALID, "kind must be 0, 1 or 2");
+        return set_error(DME_ERR_INVALID, "kind must be 0 .. 5");
     }
     if (out_elems < n) return set_error(DME_ERR_INVALID, "output buffer too small");
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
@@ -2593,6 +2811,11 @@ extern "C" int dme_invalidate_caches(dme_ctx *ctx, int pic_num) try {
             kv.second->qp_ready = false;
             kv.second->b8_ready = false;
             kv.second->u8_ready = false;
+            kv.second->comb_ready = false;
+            if (kv.second->cpic) {
+                kv.second->cpic->pyr_levels = 0;
+                kv.second->cpic->up_ready = false;
+            }
         }
     return DME_OK;
 } catch (...) {

@@ -56,9 +56,7 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g) {
         return set_error(DME_ERR_INVALID, "mv_precision must be 0 (pel), 1 (half), 2 (quarter) or 3 (eighth)");
     if (p.fp_semantics != DME_FP_MMX && p.fp_semantics != DME_FP_C)
         return set_error(DME_ERR_INVALID, "fp_semantics must be DME_FP_MMX or DME_FP_C");
-    if (p.combined_me)
-        return set_error(DME_ERR_INVALID, "combined-component ME (EncoderParams::CombinedME) is not supported: "
-                                          "estimate on luma (combined_me = 0)");
+    if (p.combined_me != 0 && p.combined_me != 1) return set_error(DME_ERR_INVALID, "combined_me must be 0 or 1");
     for (int i = 0; i < 3; ++i)
         if (p.reserved[i]) return set_error(DME_ERR_INVALID, "reserved params must be zero");
     if (p.full_search != 0 && p.full_search != 1) return set_error(DME_ERR_INVALID, "full_search must be 0 or 1");

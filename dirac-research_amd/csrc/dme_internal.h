@@ -34,7 +34,7 @@ int make_geometry(const dme_params &p, int width, int height, Geometry &g);
 
 // The launch limits (LDS of the table and wavefront kernels) of one job's
 // stages: "" if they hold, else the limit that does not (dme_kernels.hip).
-std::string launch_limits(const Geometry &g, bool full_search, bool learned);
+std::string launch_limits(const Geometry &g, bool full_search, bool learned, bool combined);
 
 // Host lambda maps: level 2 from the pel-search vectors (MEData::
 // SetLambdaMap, motion.cpp:128-171), levels 1 and 0 by max-pooling with the

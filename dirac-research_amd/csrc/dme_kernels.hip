@@ -452,6 +452,39 @@ __global__ __launch_bounds__(256) void k_check_range(const int16_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Combined-component pictures (EncPicture::Combine, enc_picture.cpp:285-367,
+// 4:2:0): c = ValueType(sqrt(u^2 + v^2 + (y + 128)^2) - 128), truncated
+// toward zero, for each luma sample with the chroma sample of its 2 x 2
+// group.  With 8-bit samples x = u^2 + v^2 + (y + 128)^2 is an integer below
+// 2^17, exact in the reference's float squares and sums, and its correctly
+// rounded float square root never rounds onto or off an integer (an integer
+// n <= 313 is more than 2^-15 away from sqrt(n^2 +- 1)).  So with
+// s = floor(sqrt(x)): c = s - 128 when x is a square or s >= 128, else
+// s - 127 (the negative non-integer rounds toward zero).  Integer work only.
+// One thread per sample; CombTask.offset counts 256-sample tiles.
+__global__ __launch_bounds__(256) void k_combine(const CombTask *__restrict__ ct, int nct, int total) {
+    const int tile = blockIdx.x;
+    if (tile >= total) return;
+    int lo = 0, hi = nct - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ct[mid].offset <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const CombTask c = ct[lo];
+    const size_t i = (size_t)(tile - c.offset) * 256 + threadIdx.x;
+    if (i >= (size_t)c.W * c.H) return;
+    const int y = (int)(i / c.W), x = (int)(i - (size_t)y * c.W);
+    const size_t k = (size_t)(y >> 1) * (c.W >> 1) + (x >> 1);
+    const int yv = c.y[i] + 128, uv = c.u[k], vv = c.v[k];
+    const int q = uv * uv + vv * vv + yv * yv;
+    int r = (int)sqrtf((float)q);
+    while (r * r > q) --r;
+    while ((r + 1) * (r + 1) <= q) ++r;
+    c.out[i] = (int16_t)(r * r == q || r >= 128 ? r - 128 : r - 127);
+}
+
+// ---------------------------------------------------------------------------
 // Quarter-pel planes.  Every value a sub-pel Diff can read from the
 // up-converted reference (BlockDiffHalfPel / QuarterPel, me_utils.cpp:410-867)
 // is the sample at a quarter-pel position (4X + fx, 4Y + fy): the rounded
@@ -1601,8 +1634,28 @@ __device__ __forceinline__ int qp_eighth(const uint32_t *__restrict__ qp, int W,
 // Up-plane sample of pixel (dy, dx) for start s, read from the quarter-pel
 // planes: quarter-pel position (2 rsx + rx + 4 dx, 2 rsy + ry + 4 dy), or
 // the eighth-pel sample at up position (rsx + 2 dx, rsy + 2 dy).
+// The same sample of a combined-component job, from the int16 up plane
+// (2W x 2H) itself: the weights of the four clamped up samples around it
+// ((2 - rx)(2 - ry) ... + 2 >> 2; eighth pel (4 - rx)(4 - ry) ... + 8 >> 4),
+// as BlockDiff{Half,Quarter,Eighth}Pel form it (me_utils.cpp:410-1170).
 template <bool E8>
+__device__ __forceinline__ int up16_at(const int16_t *__restrict__ up, int W, int H, const UpStart &s, int dy, int dx) {
+    const int UW = 2 * W, UH = 2 * H;
+    const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
+    const int y0 = clampi(uy, 0, UH - 1), y1 = clampi(uy + 1, 0, UH - 1);
+    const int x0 = clampi(ux, 0, UW - 1), x1 = clampi(ux + 1, 0, UW - 1);
+    const bool e = E8 && s.e;
+    const int one = e ? 4 : 2;
+    const int w0 = (one - s.rx) * (one - s.ry), w1 = s.rx * (one - s.ry), w2 = (one - s.rx) * s.ry, w3 = s.rx * s.ry;
+    const int v = w0 * up[(size_t)y0 * UW + x0] + w1 * up[(size_t)y0 * UW + x1] + w2 * up[(size_t)y1 * UW + x0] +
+                  w3 * up[(size_t)y1 * UW + x1];
+    return e ? (v + 8) >> 4 : (v + 2) >> 2;
+}
+
+// COMB: qp is a combined-component job's int16 up plane (up16_at).
+template <bool E8, bool COMB = false>
 __device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int H, const UpStart &s, int dy, int dx) {
+    if (COMB) return up16_at<E8>(reinterpret_cast<const int16_t *>(qp), W, H, s, dy, dx);
     if (E8 && s.e) return qp_eighth(qp, W, H, s.rsx + 2 * dx, s.rsy + 2 * dy, s.rx, s.ry) - 128;
     const int Xq = 2 * s.rsx + s.rx + 4 * dx, Yq = 2 * s.rsy + s.ry + 4 * dy;
     const int X = clampi(Xq >> 2, -1, W), Y = clampi(Yq >> 2, -1, H);
@@ -1613,7 +1666,7 @@ __device__ __forceinline__ int qp_at(const uint32_t *__restrict__ qp, int W, int
 // Integer SADs of up to 4 candidates against the up-converted reference;
 // wave-uniform.  (candidate, pixel) items are spread over the lanes, 8 per
 // lane at a time with all their loads issued together.
-template <bool E8>
+template <bool E8, bool COMB = false>
 __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W, const uint32_t *__restrict__ qp, int UW,
                                           int UH, const Blk &b, int step, const dme_mv *cands, int nc, int *out) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1637,7 +1690,7 @@ __device__ __noinline__ void wave_up_sads(const int16_t *__restrict__ pic, int W
                     if (c == e) s0 = st[e];
                 cid[k] = c;
                 pv[k] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
-                rv[k] = qp_at<E8>(qp, UW >> 1, UH >> 1, s0, dy, dx);
+                rv[k] = qp_at<E8, COMB>(qp, UW >> 1, UH >> 1, s0, dy, dx);
             }
         }
 #pragma unroll
@@ -1669,7 +1722,7 @@ __device__ __forceinline__ float binade_limit(float s) {
     return __uint_as_float((e + 1) << 23);
 }
 
-template <bool E8>
+template <bool E8, bool COMB = false>
 __device__ __noinline__ float wave_chain(float start, const int16_t *__restrict__ pic, int W,
                                         const uint32_t *__restrict__ qp, int UW, int UH, const Blk &b, const UpStart &st) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1682,7 +1735,8 @@ __device__ __noinline__ float wave_chain(float start, const int16_t *__restrict_
             const int i = b0 + k * WAVE + lane;
             const int ii = i < n ? i : n - 1;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
-            const int v = abs(qp_at<E8>(qp, UW >> 1, UH >> 1, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
+            const int v =
+                abs(qp_at<E8, COMB>(qp, UW >> 1, UH >> 1, st, dy, dx) - (int)pic[(size_t)(b.yp + dy) * W + b.xp + dx]);
             d[k] = i < n ? v : 0;
         }
 #pragma unroll
@@ -2032,7 +2086,7 @@ struct SubJob {
 // barrier of its round trip and reset (when it was used) before the next
 // round trip's first barrier, which every thread passes before the round trip
 // after that touches the counter again; an empty round trip costs one barrier.
-template <bool CHAIN, bool E8>
+template <bool CHAIN, bool E8, bool COMB>
 __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, const JobDesc &J,
                                           const uint32_t *__restrict__ qp, const SubJob (&mine)[4],
                                           const bool (&want)[4], int (&ires)[4], float (&fres)[4]) {
@@ -2064,11 +2118,11 @@ __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, 
             m.y = q.my;
             if (CHAIN) {
                 const UpStart us = up_start<E8>(b, m, q.step, UW, UH);
-                const float v = wave_chain<E8>(q.start, J.pic[0], J.W, qp, UW, UH, b, us);
+                const float v = wave_chain<E8, COMB>(q.start, J.pic[0], J.W, qp, UW, UH, b, us);
                 if (lane == 0) queue[e].fres = v;
             } else {
                 int v = 0;
-                wave_up_sads<E8>(J.pic[0], J.W, qp, UW, UH, b, q.step, &m, 1, &v);
+                wave_up_sads<E8, COMB>(J.pic[0], J.W, qp, UW, UH, b, q.step, &m, 1, &v);
                 if (lane == 0) queue[e].ires = v;
             }
         }
@@ -2084,8 +2138,9 @@ __device__ __forceinline__ void sub_serve(int *cnts, int &phase, SubJob *queue, 
     if (tid == 0) *cnt = 0;
 }
 
-// E8: the eighth-pel jobs of the launch (the others exit), else the rest.
-template <bool E8>
+// E8: the eighth-pel jobs of the launch (the others exit), else the rest;
+// COMB likewise for the combined-component jobs.
+template <bool E8, bool COMB>
 __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                       int max_rows) {
     extern __shared__ int s_dyn[];  // ring [max_rows][4] mv | slots [2][max_rows][SUB_SLOT] | queue [SUB_QCAP]
@@ -2093,7 +2148,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
     __shared__ int s_chain[2];  // per pass parity: some block of the pass may need a float chain
     const Task tk = tasks[blockIdx.x];
     const JobDesc &J = jobs[tk.job];
-    if (tk.ref >= J.nrefs || J.prec == 0 || (J.prec == 3) != E8) return;  // uniform per workgroup
+    if (tk.ref >= J.nrefs || J.prec == 0 || (J.prec == 3) != E8 || (J.comb != 0) != COMB) return;  // uniform
     const int r = tk.ref, prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1;
     const int X = J.xnb, Y = J.ynb, W = J.W, H = J.H;
     const int UW = 2 * W, UH = 2 * H;
@@ -2239,7 +2294,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                 mine[0].ires = 0;
                 mine[0].start = mine[0].fres = 0.0f;
                 mine[1] = mine[2] = mine[3] = mine[0];
-                sub_serve<false, E8>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
+                sub_serve<false, E8, COMB>(s_cnt, phase, queue, J, qp, mine, want, ir, fr);
                 psad = ir[0];
             }
             const bool chains = s_chain[cf] != 0;  // visible after the round trip's barrier
@@ -2319,7 +2374,7 @@ __global__ __launch_bounds__(512) void k_subpel_wave(const JobDesc *__restrict__
                     }
                     int ir[4] = {0, 0, 0, 0};
                     float fr[4] = {tot[0], tot[1], tot[2], tot[3]};
-                    if (chains) sub_serve<true, E8>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);  // uniform
+                    if (chains) sub_serve<true, E8, COMB>(s_cnt, phase, queue, J, qp, mine, need, ir, fr);  // uniform
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (on[j] && fr[j] < best.total) {
@@ -2394,7 +2449,7 @@ __device__ __forceinline__ UpStart bi_second(const UpStart &s1, UpStart s2) {
 // BiBlock{Half,Quarter,Eighth}Pel::Diff (me_utils.cpp:1172-1785): sum of
 // |((2 pic - r1) - r2) >> 1| with short intermediates (the eighth-pel
 // form's float adds of integers are exact).
-template <bool E8, int NB = 8>
+template <bool E8, bool COMB, int NB = 8>
 __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int UW = 2 * J.W, UH = 2 * J.H;
@@ -2415,8 +2470,8 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
             const int ii = ok[k] ? i : 0;
             const int dy = ii / b.xl, dx = ii - dy * b.xl;
             p[k] = pic[(unsigned)((b.yp + dy) * J.W + b.xp + dx)];
-            r1[k] = qp_at<E8>(q1, J.W, J.H, s1, dy, dx);
-            r2[k] = qp_at<E8>(q2, J.W, J.H, s2, dy, dx);
+            r1[k] = qp_at<E8, COMB>(q1, J.W, J.H, s1, dy, dx);
+            r2[k] = qp_at<E8, COMB>(q2, J.W, J.H, s2, dy, dx);
         }
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -2455,8 +2510,9 @@ __device__ int wave_intra_sad(const int16_t *__restrict__ p, int W, const Blk &b
 // DoUnitDecn at level 2 (me_mode_decn.cpp:368-462), for every block in
 // parallel: it only reads the block's own refined vectors and costs.  UP8:
 // the references are byte up-planes (the learned path's fast form).  E8:
-// the eighth-pel jobs of the launch (the others are skipped), else the rest.
-template <bool UP8, bool E8>
+// the eighth-pel jobs of the launch (the others are skipped), else the rest;
+// COMB likewise for the combined-component jobs.
+template <bool UP8, bool E8, bool COMB>
 __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                  int ntasks, int total) {
     const int g0 = (xcd_block() * 4 + wave_id()) * PW_MODE_L2, g1 = min(g0 + PW_MODE_L2, total);
@@ -2464,7 +2520,7 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
         const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
-        if ((J.step == 3) != E8) continue;  // uniform per wave
+        if ((J.step == 3) != E8 || (J.comb != 0) != COMB) continue;  // uniform per wave
         const int k = gi - tk.offset;
         const int xb = k % J.xnb, yb = k / J.xnb;
         const int lane = threadIdx.x & 63;
@@ -2490,7 +2546,7 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             bc.mvcost = c1.mvcost + c2.mvcost;
             bc.SAD = b.empty() ? 0.0f
                                : (float)(UP8 ? up8_bi_sad(J, b, J.vec[0][k], J.vec[1][k])
-                                             : wave_bi_sad<E8>(J, b, J.vec[0][k], J.vec[1][k]));
+                                             : wave_bi_sad<E8, COMB>(J, b, J.vec[0][k], J.vec[1][k]));
             bc.total = bc.SAD + loc_lambda * bc.mvcost;
             bc.total *= lf;
             uc = bc.total + mode_cost;
@@ -2597,7 +2653,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
         WAVE_SYNC();  // the previous block no longer reads the wave's LDS
         const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
         const JobDesc &J = jobs[tk.job];
-        if ((J.step == 3) != E8) continue;  // uniform per wave: the other launch's job
+        if ((J.step == 3) != E8 || J.comb) continue;  // uniform per wave: another launch's job
         const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
         const int u = gi - tk.offset;
         const int X2 = J.xnb, W = J.W, H = J.H, QW = W + 2, step = J.step, nr = J.nrefs;
@@ -2761,6 +2817,103 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tables of the combined-component jobs, whose samples need 9 bits: the same
+// records as k_subpel_bulk and k_mode_tables write, with every SAD summed
+// over the picture's int16 samples and the int16 up plane (up16_at), one
+// wave per block (sub-pel) or unit (mode decision).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_subpel_table16(const JobDesc *__restrict__ jobs,
+                                                        const Task *__restrict__ tasks, int ntasks, int total) {
+    const int lane = threadIdx.x & 63;
+    const int gi = xcd_block() * 4 + wave_id();
+    if (gi >= total) return;  // each wave works alone
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int r = tk.ref, k = gi - tk.offset;
+    const int X = J.xnb, W = J.W, H = J.H;
+    const int xb = k % X, yb = k / X;
+    const int prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
+    const dme_mv pel = J.vec[r][k];
+    int *__restrict__ rec = J.qtab[r] + (size_t)wf_pos(xb, yb, X, J.ynb) * SUB_SLOT;
+    if (!b.empty() && prec > 0) {
+        // candidate c: (pel << tq) + (c % Q - rad, c / Q - rad) in units of
+        // 1/2^tq pel, as k_subpel_bulk tabulates it
+        const int tq = sub_unit(prec);
+        for (int c0 = 0; c0 < QS; c0 += 4) {
+            dme_mv m[4];
+            const int nc = min(4, QS - c0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int c = c0 + (i < nc ? i : 0);
+                m[i].x = (pel.x << tq) + c % Q - rad;
+                m[i].y = (pel.y << tq) + c / Q - rad;
+            }
+            int sd[4] = {0, 0, 0, 0};
+            wave_up_sads<false, true>(J.pic[0], W, J.qp[r], 2 * W, 2 * H, b, tq, m, nc, sd);  // tq <= 2
+            if (lane < nc) rec[c0 + lane] = lane == 0 ? sd[0] : (lane == 1 ? sd[1] : (lane == 2 ? sd[2] : sd[3]));
+        }
+    }
+    if (lane == 0) {
+        rec[49] = __float_as_int(J.lam[k]);
+        rec[50] = pel.x;
+        rec[51] = pel.y;
+        const dme_cost pc = J.pc[r][k];
+        rec[52] = __float_as_int(pc.SAD);
+        rec[53] = __float_as_int(pc.mvcost);
+        rec[54] = __float_as_int(pc.total);
+    }
+}
+
+template <bool E8>
+__global__ __launch_bounds__(256) void k_mode_tables16(const JobDesc *__restrict__ jobs,
+                                                       const Task *__restrict__ tasks, int ntasks, int total) {
+    const int lane = threadIdx.x & 63;
+    const int gi = xcd_block() * 4 + wave_id();
+    if (gi >= total) return;  // each wave works alone
+    const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const JobDesc &J = jobs[tk.job];
+    const int level = tk.ref;  // 1: sub-superblock table, 0: superblock table
+    const int u = gi - tk.offset;
+    const int X2 = J.xnb, W = J.W, H = J.H, step = J.step, nr = J.nrefs;
+    const int XU = level ? J.xnb / 2 : J.xsb;
+    const int xu = u % XU, yu = u / XU;
+    const Blk b = block_limits(J.blen[level], J.bsep[level], J.boff[level], W, H, xu, yu);
+    const int nq = level ? 4 : 16, side = level ? 2 : 4;
+    int *out = level ? J.m1tab + (size_t)u * 24 : J.m0tab + (size_t)u * 32;
+    if (b.empty()) {
+        if (lane < (level ? 24 : 32)) out[lane] = 0;
+        return;
+    }
+    // candidate q of reference r: the vector of block q of the unit
+    // (candidates that no lane reads stay as they are, as k_mode_tables
+    // leaves them)
+    for (int r = 0; r < nr; ++r)
+        for (int q0 = 0; q0 < nq; q0 += 4) {
+            dme_mv m[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int q = q0 + i;
+                m[i] = J.vec[r][(size_t)(yu * side + q / side) * X2 + xu * side + q % side];
+            }
+            int sd[4] = {0, 0, 0, 0};
+            wave_up_sads<E8, true>(J.pic[0], W, J.qp[r], 2 * W, 2 * H, b, step, m, 4, sd);
+            if (lane < 4) out[nq * r + q0 + lane] = lane == 0 ? sd[0] : (lane == 1 ? sd[1] : (lane == 2 ? sd[2] : sd[3]));
+        }
+    if (level == 1 && nr > 1) {
+        // bi-prediction SADs of every pair (candidate q1 of reference 1,
+        // candidate q2 of reference 2)
+        for (int pr = 0; pr < 16; ++pr) {
+            const int q1 = pr >> 2, q2 = pr & 3;
+            const dme_mv v1 = J.vec[0][(size_t)(yu * 2 + q1 / 2) * X2 + xu * 2 + q1 % 2];
+            const dme_mv v2 = J.vec[1][(size_t)(yu * 2 + q2 / 2) * X2 + xu * 2 + q2 % 2];
+            const int s = wave_bi_sad<E8, true>(J, b, v1, v2);
+            if (lane == 0) out[8 + pr] = s;
+        }
+    }
+}
+
 // ModeDecider::DoME for one unit and reference (me_mode_decn.cpp:279-364;
 // BlockMatcher::FindBestMatchSubp, block_match.cpp:277-325) from tabulated
 // SADs, one (unit, reference) per lane.  g are the four guide vectors with
@@ -2768,7 +2921,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
 // strict-< comparison anyway).  Totals that need the per-pixel float chain
 // are computed by the whole wave, so every lane of the wave calls this.
 // Returns the index of the chosen guide (-1 when nothing was chosen).
-template <bool E8>
+template <bool E8, bool COMB>
 __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pred, const dme_mv g[4],
                        const int gs[4], float lambda, dme_mv &out_mv, dme_cost &out_cost) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -2804,7 +2957,7 @@ __device__ int unit_me(const JobDesc &J, bool on, const Blk &b, int r, dme_mv pr
             const float start = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(st[q]), src));
             const int rs = __builtin_amdgcn_readlane(r, src);
             const UpStart us = up_start<E8>(bs, m, J.step, UW, UH);
-            const float v = wave_chain<E8>(start, J.pic[0], J.W, rs ? J.qp[1] : J.qp[0], UW, UH, bs, us);
+            const float v = wave_chain<E8, COMB>(start, J.pic[0], J.W, rs ? J.qp[1] : J.qp[0], UW, UH, bs, us);
             if (lane == src) tot[q] = v;
         }
     }
@@ -2889,11 +3042,12 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 #ifndef MODE_NB
 #define MODE_NB 4
 #endif
-// E8: the eighth-pel jobs of the launch (the others exit), else the rest.
-template <bool E8>
+// E8: the eighth-pel jobs of the launch (the others exit), else the rest;
+// COMB likewise for the combined-component jobs.
+template <bool E8, bool COMB>
 __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ jobs) {
     const JobDesc &J = jobs[blockIdx.x];
-    if ((J.step == 3) != E8) return;  // uniform per workgroup
+    if ((J.step == 3) != E8 || (J.comb != 0) != COMB) return;  // uniform per workgroup
     const int XS = J.xsb, YS = J.ysb, X2 = J.xnb, X1 = J.xnb / 2;
     const int wave = wave_id(), nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
     const int nr = J.nrefs;
@@ -2957,7 +3111,7 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
             const Blk b1 = block_limits(J.blen[1], J.bsep[1], J.boff[1], J.W, J.H, xu, yu);
             dme_mv lv;
             dme_cost lc;
-            const int lq = unit_me<E8>(J, on1, b1, r1, pred1, g, gs, lam1, lv, lc);
+            const int lq = unit_me<E8, COMB>(J, on1, b1, r1, pred1, g, gs, lam1, lv, lc);
             // unit decisions, lane u for unit u
             const int sa = (2 * lane) & 63, sb = (2 * lane + 1) & 63;
             dme_cost cu[2] = {shfl_cost(lc, sa), shfl_cost(lc, sb)};
@@ -3016,17 +3170,18 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
                     m.x = __builtin_amdgcn_readlane(g0[q].x, src);
                     m.y = __builtin_amdgcn_readlane(g0[q].y, src);
                     int sd = 0;
-                    wave_up_sads<E8>(J.pic[0], J.W, src ? J.qp[1] : J.qp[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1, &sd);
+                    wave_up_sads<E8, COMB>(J.pic[0], J.W, src ? J.qp[1] : J.qp[0], 2 * J.W, 2 * J.H, b0, J.step, &m, 1,
+                                           &sd);
                     if (lane == src) gs0[q] = sd;
                 }
             }
             dme_mv mv0;
             dme_cost mc0;
-            unit_me<E8>(J, lane < nr, b0, r0, pred0, g0, gs0, lam0, mv0, mc0);
+            unit_me<E8, COMB>(J, lane < nr, b0, r0, pred0, g0, gs0, lam0, mv0, mc0);
             const dme_mv va0 = shfl_mv(mv0, 0), vb0 = shfl_mv(mv0, 1);
             dme_cost c0[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
             int bis0 = 0;
-            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<E8, MODE_NB>(J, b0, va0, vb0);
+            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<E8, COMB, MODE_NB>(J, b0, va0, vb0);
             int m0;
             const float sb0 = unit_decn(J, 0, lam0, c0, bis0, m0);
             if (sb0 <= sbc) {
@@ -3353,7 +3508,7 @@ __global__ __launch_bounds__(256) void k_block_dc(const JobDesc *__restrict__ jo
     const Task tk = tasks[find_task_wave(tasks, ntasks, row)];
     const JobDesc &J = jobs[tk.job];
     const int comp = tk.ref, yb = row - tk.offset;
-    const int16_t *__restrict__ p = comp == 0 ? J.pic[0] : (comp == 1 ? J.cu : J.cv);
+    const int16_t *__restrict__ p = comp == 0 ? J.py : (comp == 1 ? J.cu : J.cv);  // SetDC reads luma
     const int *blen = comp == 0 ? J.blen[2] : J.cblen;
     const int *bsep = comp == 0 ? J.bsep[2] : J.cbsep;
     const int *boff = comp == 0 ? J.boff[2] : J.cboff;
@@ -4628,8 +4783,7 @@ hipError_t launch_bytes(const PlaneTask *pt, int npt, int total, hipStream_t st)
 
 // Dynamic LDS of k_pel_bulk (four waves) for blocks up to bx x by at
 // `level`, with its layout.
-static size_t pel_bulk_lds(int level, int max_blen_x, int max_blen_y, bool has_guide, PelLds &ly) {
-    const bool u8 = level == 0;
+static size_t pel_bulk_lds(int level, int max_blen_x, int max_blen_y, bool has_guide, bool u8, PelLds &ly) {
     const int rad = min(level + 1, 5), R = rad + 1, D = 2 * R + 1;
     const int ntab = has_guide ? 2 : 1;
     // dwords of a packed row: per = samples per dword; the SAD reads 4-dword
@@ -4647,13 +4801,16 @@ static size_t pel_bulk_lds(int level, int max_blen_x, int max_blen_y, bool has_g
 }
 constexpr size_t kPelBulkLdsMax = 64 * 1024;
 
+// u8: level 0 of luma jobs (the pictures themselves, whose samples were
+// checked to 8-bit range at registration), from the byte planes; else 16-bit
+// samples (the pyramid levels, and every level of combined-component jobs).
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
-                           int max_blen_y, bool has_guide, hipStream_t st) {
+                           int max_blen_y, bool has_guide, bool u8, hipStream_t st) {
     if (total <= 0) return hipSuccess;
     if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
-    const bool u8 = level == 0;  // the pictures themselves: samples checked to 8-bit range at registration
+    if (u8 && level != 0) return hipErrorInvalidValue;
     PelLds ly;
-    const size_t lds = pel_bulk_lds(level, max_blen_x, max_blen_y, has_guide, ly);
+    const size_t lds = pel_bulk_lds(level, max_blen_x, max_blen_y, has_guide, u8, ly);
     if (lds > kPelBulkLdsMax) return hipErrorInvalidValue;
     const dim3 grid((total + 4 * PW_PEL_BULK - 1) / (4 * PW_PEL_BULK));
     if (u8) hipLaunchKernelGGL(k_pel_bulk<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
@@ -4732,6 +4889,27 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
     return hipGetLastError();
 }
 
+hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_subpel_table16, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_mode_tables16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool eighth,
+                                hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (eighth) hipLaunchKernelGGL(k_mode_tables16<true>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks,
+                                   total);
+    else hipLaunchKernelGGL(k_mode_tables16<false>, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const CombTask *ct, int nct, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_combine, dim3(total), dim3(256), 0, st, ct, nct, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_check_range(const int16_t *y, size_t n, int *flag, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const int blocks = (int)std::min<size_t>((n + 1023) / 1024, 4096);
@@ -4754,16 +4932,21 @@ hipError_t launch_subpel_wave(const JobDesc *jobs, const Task *tasks, int ntasks
     if (lds > kTableLdsMax) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
-        for (const void *f : {(const void *)k_subpel_wave<false>, (const void *)k_subpel_wave<true>}) {
+        for (const void *f : {(const void *)k_subpel_wave<false, false>, (const void *)k_subpel_wave<true, false>,
+                              (const void *)k_subpel_wave<false, true>, (const void *)k_subpel_wave<true, true>}) {
             hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
             if (e != hipSuccess) return e;
         }
         attr = true;
     }
     if (kinds & DME_KIND_BELOW_EIGHTH)
-        hipLaunchKernelGGL(k_subpel_wave<false>, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+        hipLaunchKernelGGL((k_subpel_wave<false, false>), dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
     if (kinds & DME_KIND_EIGHTH)
-        hipLaunchKernelGGL(k_subpel_wave<true>, dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+        hipLaunchKernelGGL((k_subpel_wave<true, false>), dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+    if (kinds & DME_KIND_COMB_BELOW_EIGHTH)
+        hipLaunchKernelGGL((k_subpel_wave<false, true>), dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
+    if (kinds & DME_KIND_COMB_EIGHTH)
+        hipLaunchKernelGGL((k_subpel_wave<true, true>), dim3(ntasks), dim3(threads), lds, st, jobs, tasks, max_rows);
     return hipGetLastError();
 }
 
@@ -4778,14 +4961,18 @@ hipError_t launch_mode_l2(const JobDesc *jobs, const Task *tasks, int ntasks, in
                           hipStream_t st) {
     if (total <= 0) return hipSuccess;
     const dim3 grid((total + 4 * PW_MODE_L2 - 1) / (4 * PW_MODE_L2));
-    if (up8) {  // the fast form: quarter or half pel only
-        hipLaunchKernelGGL((k_mode_l2<true, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (up8) {  // the fast form: quarter or half pel only, on luma
+        hipLaunchKernelGGL((k_mode_l2<true, false, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
         return hipGetLastError();
     }
     if (kinds & DME_KIND_BELOW_EIGHTH)
-        hipLaunchKernelGGL((k_mode_l2<false, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+        hipLaunchKernelGGL((k_mode_l2<false, false, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
     if (kinds & DME_KIND_EIGHTH)
-        hipLaunchKernelGGL((k_mode_l2<false, true>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+        hipLaunchKernelGGL((k_mode_l2<false, true, false>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (kinds & DME_KIND_COMB_BELOW_EIGHTH)
+        hipLaunchKernelGGL((k_mode_l2<false, false, true>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (kinds & DME_KIND_COMB_EIGHTH)
+        hipLaunchKernelGGL((k_mode_l2<false, true, true>), grid, dim3(256), 0, st, jobs, tasks, ntasks, total);
     return hipGetLastError();
 }
 
@@ -4825,8 +5012,13 @@ hipError_t launch_mode_tables(const JobDesc *jobs, const Task *tasks, int ntasks
 
 hipError_t launch_mode_wave(const JobDesc *jobs, int njobs, int threads, int kinds, hipStream_t st) {
     if (threads < 64 || threads > 1024 || threads % 64) return hipErrorInvalidValue;  // the kernel's launch bound
-    if (kinds & DME_KIND_BELOW_EIGHTH) hipLaunchKernelGGL(k_mode_wave<false>, dim3(njobs), dim3(threads), 0, st, jobs);
-    if (kinds & DME_KIND_EIGHTH) hipLaunchKernelGGL(k_mode_wave<true>, dim3(njobs), dim3(threads), 0, st, jobs);
+    if (kinds & DME_KIND_BELOW_EIGHTH)
+        hipLaunchKernelGGL((k_mode_wave<false, false>), dim3(njobs), dim3(threads), 0, st, jobs);
+    if (kinds & DME_KIND_EIGHTH) hipLaunchKernelGGL((k_mode_wave<true, false>), dim3(njobs), dim3(threads), 0, st, jobs);
+    if (kinds & DME_KIND_COMB_BELOW_EIGHTH)
+        hipLaunchKernelGGL((k_mode_wave<false, true>), dim3(njobs), dim3(threads), 0, st, jobs);
+    if (kinds & DME_KIND_COMB_EIGHTH)
+        hipLaunchKernelGGL((k_mode_wave<true, true>), dim3(njobs), dim3(threads), 0, st, jobs);
     return hipGetLastError();
 }
 
@@ -4889,13 +5081,13 @@ constexpr size_t kFastSubpelLdsMax = 64 * 1024;
 
 // The limits of one job's launches, checked on the host before anything is
 // queued: "" if every kernel of the job's stages fits, else the limit.
-std::string launch_limits(const Geometry &g, bool full_search, bool learned) {
+std::string launch_limits(const Geometry &g, bool full_search, bool learned, bool combined) {
     const int bx = g.blen[2][0], by = g.blen[2][1];
     if (bx > PEL_MAXB || by > PEL_MAXB)
         return "block lengths above " + std::to_string(PEL_MAXB) + " after rationalising";
     PelLds pl;
     for (int l = full_search ? 0 : g.depth; l >= 0; --l) {
-        if (pel_bulk_lds(l, bx, by, l < g.depth, pl) > kPelBulkLdsMax)
+        if (pel_bulk_lds(l, bx, by, l < g.depth, l == 0 && !combined, pl) > kPelBulkLdsMax)
             return "pel tables of level " + std::to_string(l) + " exceed 64 KiB of LDS";
         int se = 0;
         size_t fixed = 0, per_miss = 0;

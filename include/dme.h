@@ -87,13 +87,16 @@ typedef struct dme_params {
                               (pixel_match.cpp:365-370)                      */
     int32_t x_range_me;    /* XRangeME / YRangeME (common.h:1311-1314), used  */
     int32_t y_range_me;    /* only with full_search; the encoder default is 32 */
-    int32_t combined_me;   /* EncoderParams::CombinedME (common.h:1317): ME on
-                              the combined luma+chroma picture
-                              (EncPicture::DataForME/UpDataForME,
-                              enc_picture.cpp:114-128).  Not supported: 1
-                              gives DME_ERR_INVALID, so a caller with
-                              CombinedME() set cannot get luma-only results
-                              silently.  0 = ME on luma, the encoder default */
+    int32_t combined_me;   /* EncoderParams::CombinedME (common.h:1317): 1 =
+                              ME on the combined-component picture of every
+                              picture (EncPicture::DataForME/UpDataForME,
+                              enc_picture.cpp:114-128; Combine, :285-367),
+                              whose samples need 9 bits; the picture and its
+                              references must have chroma in [-128, 127];
+                              SetDC still reads luma.  Not with full search,
+                              the learned forms or the no-allocation entry
+                              point (DME_ERR_INVALID).  0 = ME on luma, the
+                              encoder default                               */
     int32_t reserved[3];   /* must be zero                                   */
 } dme_params;
 
@@ -467,6 +470,10 @@ int dme_motion_compensate_batch(dme_ctx *ctx, const dme_mc_job *jobs,
  * -64 .. R - 65 and columns -64 .. P - 65 (R x P samples, R = 32 *
  * ceil((H + 64) / 16), P = 128 * ceil((W + 64) / 64)), each the sample at
  * the clamped coordinates (the reference's bounds-checked reads).
+ * kinds 3, 4, 5 = the combined-component picture (W x H; EncPicture::
+ * CombinedData, enc_picture.cpp:237-254), its pyramid level `level` and its
+ * 2W x 2H up-converted plane (UpCombinedData, :256-280), built once a
+ * combined-component job read the picture.
  * out_elems must be at least the plane's sample count.                    */
 int dme_get_plane(dme_ctx *ctx, int pic_num, int kind, int level,
                   int16_t *out, size_t out_elems);

@@ -922,6 +922,30 @@ Plane make_plane(const int16_t *p, int w, int h) {
     return q;
 }
 
+// EncPicture::Combine, 4:2:0 branch (enc_picture.cpp:285-367; CombinedData,
+// :237-254): each sample of a 2 x 2 luma group with the group's chroma,
+// ValueType(sqrt(u^2 + v^2 + (y + 128)^2) - 128), the squares and sums in
+// float (exact for 8-bit samples), sqrt in float, the subtraction in double,
+// truncated toward zero.
+Plane combine(const int16_t *y, const int16_t *u, const int16_t *v, int W, int H) {
+    Plane c(W, H);
+    const int CW = W >> 1;
+    for (int j = 0; j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            const int k = (j >> 1) * CW + (i >> 1);
+            float val = float(u[k]);
+            val *= val;
+            float valc = val;
+            val = float(v[k]);
+            val *= val;
+            valc += val;
+            float valy = float(y[(size_t)j * W + i]) + 128.0f;
+            valy *= valy;
+            c.at(j, i) = (int16_t)(std::sqrt(valc + valy) - 128.0);
+        }
+    return c;
+}
+
 void export_medata(const MEData &me, dme_medata *out) {
     if (!out) return;
     const size_t B = (size_t)me.xnb * me.ynb, S = (size_t)me.xsb * me.ysb;
@@ -1410,6 +1434,13 @@ extern "C" int oracle_motion_compensate(const dme_params *p, int width, int heig
     return mc_picture(g, p->mv_precision, p->fp_semantics == DME_FP_MMX, refs, nrefs, me, direction, planes, widths, heights);
 }
 
+extern "C" int oracle_combine(const int16_t *y, const int16_t *u, const int16_t *v, int w, int h, int16_t *out) {
+    if (!y || !u || !v || !out || w < 2 || h < 2 || (w & 1) || (h & 1)) return fail("bad arguments");
+    const Plane c = combine(y, u, v, w, h);
+    std::memcpy(out, c.d.data(), sizeof(int16_t) * c.d.size());
+    return DME_OK;
+}
+
 extern "C" int oracle_upconvert(const int16_t *in, int w, int h, int16_t *out) {
     if (!in || !out) return fail("NULL plane");
     if (w < 8 || h < 8) return fail("upconvert needs at least 8x8");
@@ -1446,9 +1477,19 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
     double terms = 0.0;
     const int W = g.W, H = g.H;
 
-    Plane pic = make_plane(cur->y, W, H);
+    // the picture ME works on: luma, or the combined-component picture
+    // (EncPicture::DataForME, enc_picture.cpp:114-120); SetDC reads luma
+    const Plane luma = make_plane(cur->y, W, H);
+    Plane pic = luma;
     Plane ref[2];
     for (int r = 0; r < nrefs; ++r) ref[r] = make_plane(refs[r].y, W, H);
+    if (p->combined_me) {
+        if (!cur->u || !cur->v) return fail("combined-component ME needs the picture's chroma");
+        for (int r = 0; r < nrefs; ++r)
+            if (!refs[r].u || !refs[r].v) return fail("combined-component ME needs the references' chroma");
+        pic = combine(cur->y, cur->u, cur->v, W, H);
+        for (int r = 0; r < nrefs; ++r) ref[r] = combine(refs[r].y, refs[r].u, refs[r].v, W, H);
+    }
 
     MEData me;
     me.init(g.xnb, g.ynb, g.xsb, g.ysb, nrefs);
@@ -1559,7 +1600,7 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
                 if (me.sbc[(size_t)ys * g.xsb + xs] <= old_best) mode_level(m, xs, ys, 0);
             }
         // SetDC (me_mode_decn.cpp:567-573) on the original Y/U/V
-        set_dc(pic, g.l[2], me, 0);
+        set_dc(luma, g.l[2], me, 0);
         if (cur->u && cur->v) {
             Plane pu = make_plane(cur->u, g.CW, g.CH), pv = make_plane(cur->v, g.CW, g.CH);
             set_dc(pu, g.c[2], me, 1);
@@ -1585,8 +1626,9 @@ extern "C" int oracle_me(const dme_params *p, const oracle_picture *cur, const o
 extern "C" int oracle_fast_me(const dme_params *p, const oracle_picture *cur, const oracle_picture *refs, int nrefs,
                               const dme_mv *const *v0, dme_medata *out, dme_mv *const *v2_out, double *sad_terms) {
     if (!cur || !cur->y || !refs || !v0 || nrefs < 1 || nrefs > 2) return fail("bad arguments");
-    if (p->mv_precision < 1 || p->mv_precision > 2 || p->fp_semantics != DME_FP_MMX || p->full_search)
-        return fail("the fast form runs half or quarter pel, MMX semantics, hierarchical search");
+    if (p->mv_precision < 1 || p->mv_precision > 2 || p->fp_semantics != DME_FP_MMX || p->full_search ||
+        p->combined_me)
+        return fail("the fast form runs half or quarter pel, MMX semantics, hierarchical search on luma");
     Geo g;
     int rc = make_geo(p, cur->width, cur->height, g);
     if (rc) return rc;

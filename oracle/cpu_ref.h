@@ -50,6 +50,10 @@ int oracle_downconvert(const int16_t *in, int in_w, int in_h,
 /* UpConverter::DoUpConverter(-128, 127) on a w x h plane into 2w x 2h. */
 int oracle_upconvert(const int16_t *in, int w, int h, int16_t *out);
 
+/* EncPicture::Combine (enc_picture.cpp:285-367), 4:2:0: the combined-
+ * component picture of w x h luma and w/2 x h/2 chroma planes.              */
+int oracle_combine(const int16_t *y, const int16_t *u, const int16_t *v, int w, int h, int16_t *out);
+
 /* MEData::SetLambdaMap(num_refs, lambda) (host long double statistics). */
 int oracle_lambda_map(int xnb, int ynb, int num_refs, const dme_mv *v1,
                       const dme_mv *v2, float lambda, float *out);

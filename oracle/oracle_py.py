@@ -127,9 +127,17 @@ class Capture:
             self.s.ref_up[r] = self.ref_up[r].ctypes.data
 
 
+def _planes(r):
+    """A reference given as its luma array or as (y, u, v)."""
+    a = lambda x: None if x is None else np.ascontiguousarray(x, np.int16)
+    if isinstance(r, (tuple, list)):
+        return tuple(a(x) for x in r)
+    return (a(r), None, None)
+
+
 def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=(), pic_nums=None):
     cur = np.ascontiguousarray(cur, np.int16)
-    refs = [np.ascontiguousarray(r, np.int16) for r in refs]
+    refs = [_planes(r) for r in refs]
     u = None if u is None else np.ascontiguousarray(u, np.int16)
     v = None if v is None else np.ascontiguousarray(v, np.int16)
     H, W = cur.shape
@@ -138,7 +146,7 @@ def _run(fn, params, cur, refs, u, v, stop, with_cap, extra=(), pic_nums=None):
     # picture numbers (they set the temporal distances of full search)
     cn, nums = pic_nums if pic_nums is not None else (1, [0, 2] if len(refs) == 2 else [0])
     cp = _pic(cn, cur, u, v)
-    rp = (oracle_picture * 2)(*[_pic(nums[i], refs[i]) for i in range(len(refs))])
+    rp = (oracle_picture * 2)(*[_pic(nums[i], *refs[i]) for i in range(len(refs))])
     cap = Capture(geom, len(refs)) if with_cap else None
     ms = out.struct()
     rc = fn(C.byref(params), C.byref(cp), rp, len(refs), stop, *extra, C.byref(ms),
@@ -223,6 +231,24 @@ def exact_costs(params, cur, refs, v0, v2, final):
     _chk(o.oracle_exact_costs(C.byref(params), C.byref(cp), rp, nrefs, arr(v0), arr(v2), arr(fv), split.ctypes.data,
                               lam.ctypes.data, arr(pc), sbc.ctypes.data, mode.ctypes.data))
     return pc, sbc, mode
+
+
+def combine(y, u, v, which="oracle", variant="mmx"):
+    """The combined-component picture (EncPicture::Combine) of a 4:2:0
+    picture, by the oracle or by the reference build."""
+    y, u, v = (np.ascontiguousarray(a, np.int16) for a in (y, u, v))
+    H, W = y.shape
+    out = np.empty_like(y)
+    if which == "oracle":
+        o = oracle()
+        o.oracle_combine.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p]
+        _chk(o.oracle_combine(y.ctypes.data, u.ctypes.data, v.ctypes.data, W, H, out.ctypes.data))
+    else:
+        l = ref(variant)
+        l.refh_combine.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p]
+        if l.refh_combine(y.ctypes.data, u.ctypes.data, v.ctypes.data, W, H, out.ctypes.data) != 0:
+            raise RuntimeError("reference harness error")
+    return out
 
 
 def run_ref(params, cur, refs, u=None, v=None, stop=STOP_MODE, capture=False, variant="mmx", pic_nums=None):

@@ -144,15 +144,15 @@ void match_pic(const PicArray &pic, const PicArray &ref, MEData &me, const MvDat
 }
 
 // PixelMatcher::DoSearch with an explicit depth (pixel_match.cpp:59-157).
-void pel_search_depth(EncQueue &q, int pic_num, int depth, oracle_capture *cap) {
+void pel_search_depth(EncQueue &q, int pic_num, int depth, bool combined, oracle_capture *cap) {
     EncPicture &P = q.GetPicture(pic_num);
     MEData &me = P.GetMEData();
     const PicturePredParams &pp = me.GetPicPredParams();
-    const PicArray &pic = P.DataForME(false);
+    const PicArray &pic = P.DataForME(combined);
     const std::vector<int> &refs = P.GetPparams().Refs();
     const int ref1 = refs[0], ref2 = refs.size() > 1 ? refs[1] : refs[0];
-    const PicArray &r1 = q.GetPicture(ref1).DataForME(false);
-    const PicArray &r2 = q.GetPicture(ref2).DataForME(false);
+    const PicArray &r1 = q.GetPicture(ref1).DataForME(combined);
+    const PicArray &r2 = q.GetPicture(ref2).DataForME(combined);
 
     std::vector<PicArray *> pd(depth + 1), d1(depth + 1), d2(depth + 1);
     DownConverter dcon;
@@ -213,6 +213,22 @@ extern "C" int refh_downconvert(const int16_t *in, int in_w, int in_h, int16_t *
     DownConverter d;
     d.DoDownConvert(a, b);
     dump(b, out);
+    return 0;
+}
+
+// EncPicture::CombinedData (enc_picture.cpp:237-254) of one 4:2:0 picture.
+extern "C" int refh_combine(const int16_t *y, const int16_t *u, const int16_t *v, int w, int h, int16_t *out) {
+    EncQueue q;
+    PictureParams pp(format420, w, h, 8, 8);
+    pp.SetPictureNum(0);
+    pp.SetPicSort(PictureSort::IntraRefPictureSort());
+    q.PushPicture(pp);
+    EncPicture &e = q.GetPicture(0);
+    fill(e.Data(Y_COMP), y);
+    fill(e.Data(U_COMP), u);
+    fill(e.Data(V_COMP), v);
+    e.SetOrigData();
+    dump(e.DataForME(true), out);
     return 0;
 }
 
@@ -289,7 +305,7 @@ extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const ora
         PixelMatcher pm(encp);
         pm.DoSearch(q, cur->pic_num);
     } else {
-        pel_search_depth(q, cur->pic_num, p->depth, cap);
+        pel_search_depth(q, cur->pic_num, p->depth, p->combined_me != 0, cap);
     }
     if (stop_after > ORACLE_STOP_PEL) {
         // SubPixelME (picture_compress.cpp:208-261)
@@ -327,7 +343,7 @@ extern "C" int refh_me(const dme_params *p, const oracle_picture *cur, const ora
     dump_medata(me, nrefs, out);
     if (cap)
         for (int r = 0; r < nrefs; ++r)
-            if (cap->ref_up[r]) dump(q.GetPicture(refs[r].pic_num).UpDataForME(false), cap->ref_up[r]);
+            if (cap->ref_up[r]) dump(q.GetPicture(refs[r].pic_num).UpDataForME(p->combined_me != 0), cap->ref_up[r]);
     return 0;
 }
 

@@ -5,7 +5,8 @@ Run in a container that has the reference tree:  python tests/golden/make_golden
 (--mc-only rewrites only the motion-compensation fixture, mc.npz; --full-only
 only the full-search fixture, full_search.npz; --imt-only only the .imt
 diagnostics fixture, me_cif.imt and imt.json; --eighth-only only the
-eighth-pel fixture, eighth.npz).
+eighth-pel fixture, eighth.npz; --combined-only only the combined-component
+fixture, combined.npz).
 
 Each fixture holds the inputs' SHA-256 (inputs are regenerated from their
 seed by the deterministic generator in libdme.so), the parameters, and the
@@ -162,6 +163,36 @@ def main_eighth():
     np.savez_compressed(os.path.join(HERE, "eighth.npz"), meta=json.dumps(index), **out)
 
 
+def main_combined():
+    """Combined-component fixtures (combined.npz, cases in combined_cases.py):
+    the reference's live sequence with EncoderParams::CombinedME set, which
+    makes every stage read EncPicture::CombinedData / UpCombinedData; full
+    MEData of the CIF cases, digests plus mode and superblock arrays of the
+    720p case; and the combined pictures of the CIF cases' current
+    pictures (EncPicture::CombinedData itself)."""
+    import combined_cases as CC
+
+    out, index = {}, {}
+    for name, seed, size, nrefs, prec, depth, lam, fp, variant, bright in CC.CASES:
+        d = CC.pictures(seed, size, nrefs, bright)
+        p = make_params(*d["block"], precision=prec, depth=depth, lam=lam, fp=fp)
+        p.combined_me = 1
+        res, _, _ = O.run_ref(p, d["cur"], d["refs_yuv"], d["u"], d["v"], variant=variant)
+        arrs = res.arrays(nrefs)
+        comb = O.combine(d["cur"], d["u"], d["v"], "ref", variant)
+        index[name] = dict(seed=seed, size=size, nrefs=nrefs, precision=prec, depth=depth, lam=lam, fp=fp,
+                           variant=variant, bright=bright, inputs_sha256=CC.inputs_digest(d),
+                           sha256={k: sha(v) for k, v in arrs.items()}, combined_sha256=sha(comb),
+                           above_127=float((comb > 127).mean()))
+        for k, a in arrs.items():
+            if size == "cif" or k in ("mode", "sb_split", "sb_costs"):
+                out[name + "/" + k] = a
+        if size == "cif":
+            out[name + "/combined"] = comb
+        print("wrote", name, "combined samples above 127: %.3f" % index[name]["above_127"])
+    np.savez_compressed(os.path.join(HERE, "combined.npz"), meta=json.dumps(index), **out)
+
+
 def main_mc():
     """Motion-compensation fixtures (mc.npz): for each case in mc_cases.CASES,
     the MvData from the reference's own motion estimator (or scrambled), and
@@ -233,9 +264,12 @@ if __name__ == "__main__":
         main_mc()
     elif "--eighth-only" in sys.argv:
         main_eighth()
+    elif "--combined-only" in sys.argv:
+        main_combined()
     else:
         main()
         main_mc()
         main_full()
         main_imt()
         main_eighth()
+        main_combined()

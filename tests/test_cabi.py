@@ -58,17 +58,17 @@ def test_geometry_rejects_bad_params():
         dirac_me.geometry(make_params(), 351, 288)
 
 
-def test_combined_component_me_is_refused():
-    # EncoderParams::CombinedME (common.h:1317) estimates on the combined
-    # picture (enc_picture.cpp:114-128); refused, not estimated on luma
+def test_combined_component_me_flag_is_validated():
+    # EncoderParams::CombinedME (common.h:1317) is a bool: 0 or 1
     p = make_params()
-    p.combined_me = 1
+    p.combined_me = 2
     with pytest.raises(dirac_me.DmeError) as e:
         dirac_me.geometry(p, 352, 288)
     assert e.value.code == abi.DME_ERR_INVALID
     assert "combined" in str(e.value)
-    p.combined_me = 0
-    dirac_me.geometry(p, 352, 288)
+    for v in (0, 1):
+        p.combined_me = v
+        dirac_me.geometry(p, 352, 288)
 
 
 @pytest.mark.parametrize("nrefs", [1, 2])

@@ -38,12 +38,33 @@ def test_dropin_classes_reproduce_the_reference(case, prec):
     _same(want, got, len(d["refs"]), "%s prec %d" % (case, prec))
 
 
+def test_dropin_classes_with_combined_component_me():
+    # EncoderParams::CombinedME set: the reference's stages read its
+    # EncPicture::CombinedData; the drop-in classes upload Y, U, V and the
+    # library forms the same combined pictures
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import combined_cases as CC
+
+    d = CC.pictures(34, "cif", 2, bright=True)
+    p = make_params(*d["block"], precision=2, depth=-1, lam=10.0)
+    p.combined_me = 1
+    want, _, _ = O.run_ref(p, d["cur"], d["refs_yuv"], d["u"], d["v"])
+    got = O.run_dropin(p, d["cur"], d["refs_yuv"], d["u"], d["v"])
+    _same(want, got, 2, "combined")
+
+
 def test_dropin_reports_refused_parameters():
-    # combined-component ME is refused by the library; the drop-in class
-    # surfaces the library's message as an exception the harness reports
+    # combined-component ME with full search is refused by the library; the
+    # drop-in class surfaces the library's message as an exception the
+    # harness reports
     d = synth.sized(33, "cif", 1)
     p = make_params(*d["block"], precision=2, depth=-1, lam=10.0)
     p.combined_me = 1
+    p.full_search = 1
+    p.x_range_me = p.y_range_me = 8
+    refs = [(r,) + synth.chroma(r, 3) for r in d["refs"]]
     with pytest.raises(RuntimeError) as e:
-        O.run_dropin(p, d["cur"], d["refs"], d["u"], d["v"])
-    assert "combined" in str(e.value)
+        O.run_dropin(p, d["cur"], refs, d["u"], d["v"])
+    assert "full search" in str(e.value)

@@ -11,6 +11,7 @@
 import hashlib
 import json
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -19,6 +20,8 @@ import oracle_py as O
 from dirac_me import DME_FP_C, DME_FP_MMX, MV_DTYPE, make_params, synth
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLD)
+import combined_cases as CC  # noqa: E402
 
 
 def _sha(a):
@@ -383,3 +386,93 @@ def test_full_search_recovers_large_translations(dx, dy):
     for y in range(4, g.ynum_blocks - 4):
         for x in range(5, g.xnum_blocks - 5):
             assert (v[y, x]["x"], v[y, x]["y"]) == (dx, dy) and pc[y, x]["SAD"] == 0, (x, y)
+
+
+# ---- combined-component ME (EncoderParams::CombinedME, enc_picture.cpp:114-128, 237-367) ----
+
+def _combined(name):
+    z = np.load(os.path.join(GOLD, "combined.npz"))
+    meta = json.loads(str(z["meta"]))[name]
+    d = CC.pictures(meta["seed"], meta["size"], meta["nrefs"], meta["bright"])
+    assert CC.inputs_digest(d) == meta["inputs_sha256"], "synthetic generator changed: regenerate the fixtures"
+    p = make_params(*d["block"], precision=meta["precision"], depth=meta["depth"], lam=meta["lam"], fp=meta["fp"])
+    p.combined_me = 1
+    return meta, z, d, p
+
+
+COMBINED = ["cc_cif2_q", "cc_cif1_h", "cc_cif2_p", "cc_cif2_e", "cc_cif2_q_c", "cc_cif2_dref", "cc_cif2_q_bright",
+            "cc_cif2_e_bright", "cc_720p2_q"]
+
+
+def test_combined_picture_closed_forms():
+    # ValueType(sqrt(u^2 + v^2 + (y + 128)^2) - 128), truncated toward zero:
+    # a Pythagorean triple is exact, a negative non-integer rounds up, the
+    # brightest, most saturated sample is 184, a grey picture is itself
+    y = np.full((4, 4), -128, np.int16)
+    u = np.full((2, 2), 3, np.int16)
+    v = np.full((2, 2), 4, np.int16)
+    assert np.all(O.combine(y, u, v) == 5 - 128)
+    y[:] = -127
+    u[:] = 1
+    v[:] = 0
+    assert np.all(O.combine(y, u, v) == -126)  # sqrt(2) - 128 = -126.59
+    y[:] = 127
+    u[:] = -128
+    v[:] = -128
+    assert np.all(O.combine(y, u, v) == 184)  # sqrt(97793) - 128 = 184.7
+    g = np.arange(-128, 128, dtype=np.int16).reshape(16, 16)
+    z = np.zeros((8, 8), np.int16)
+    assert np.array_equal(O.combine(g, z, z), g)
+    # the 2 x 2 luma group shares one chroma sample (4:2:0 branch)
+    rng = np.random.default_rng(5)
+    y = rng.integers(-128, 128, (6, 8)).astype(np.int16)
+    u = rng.integers(-128, 128, (3, 4)).astype(np.int16)
+    v = rng.integers(-128, 128, (3, 4)).astype(np.int16)
+    c = O.combine(y, u, v)
+    for j in range(6):
+        for i in range(8):
+            q = int(u[j // 2, i // 2]) ** 2 + int(v[j // 2, i // 2]) ** 2 + (int(y[j, i]) + 128) ** 2
+            assert c[j, i] == int(np.trunc(np.sqrt(q) - 128.0))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="the reference build (oracle/_ref) is absent")
+def test_combined_picture_matches_reference_live():
+    rng = np.random.default_rng(6)
+    for W, H in ((64, 48), (352, 288)):
+        y = rng.integers(-128, 128, (H, W)).astype(np.int16)
+        u = rng.integers(-128, 128, (H // 2, W // 2)).astype(np.int16)
+        v = rng.integers(-128, 128, (H // 2, W // 2)).astype(np.int16)
+        assert np.array_equal(O.combine(y, u, v), O.combine(y, u, v, "ref"))
+
+
+@pytest.mark.parametrize("name", COMBINED)
+def test_oracle_combined_me_matches_reference_fixture(name):
+    meta, z, d, p = _combined(name)
+    if meta["size"] == "cif":
+        assert np.array_equal(O.combine(d["cur"], d["u"], d["v"]), z[name + "/combined"])
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs_yuv"], d["u"], d["v"])
+    for k, v in got.arrays(meta["nrefs"]).items():
+        assert CC.sha(v) == meta["sha256"][k], k
+        if name + "/" + k in z:
+            assert np.array_equal(v.view(np.uint8), z[name + "/" + k].view(np.uint8)), k
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="the reference build (oracle/_ref) is absent")
+@pytest.mark.parametrize("seed,nrefs,prec,fp,bright", [(51, 2, 2, DME_FP_MMX, True), (52, 1, 3, DME_FP_C, True),
+                                                       (53, 2, 1, DME_FP_C, False)])
+def test_oracle_combined_me_matches_reference_live(seed, nrefs, prec, fp, bright):
+    d = CC.pictures(seed, "cif", nrefs, bright)
+    p = make_params(*d["block"], precision=prec, depth=d["depth"], lam=10.0, fp=fp)
+    p.combined_me = 1
+    got, _, _ = O.run_oracle(p, d["cur"], d["refs_yuv"], d["u"], d["v"])
+    want, _, _ = O.run_ref(p, d["cur"], d["refs_yuv"], d["u"], d["v"], variant="mmx" if fp == DME_FP_MMX else "c")
+    for k, v in want.arrays(nrefs).items():
+        assert v.tobytes() == got.arrays(nrefs)[k].tobytes(), k
+
+
+def test_combined_me_needs_reference_chroma():
+    d = synth.sized(54, "cif", 1)
+    p = make_params(*d["block"], precision=2, depth=3, lam=10.0)
+    p.combined_me = 1
+    with pytest.raises(RuntimeError, match="chroma"):
+        O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"])
