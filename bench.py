@@ -441,6 +441,39 @@ def learned_leg(ctx, jobs, frames, geom, B, args, plain_fps):
             "pictures_for_gap": n, "weights": "dirac_me/proposal_weights.npy (2000 training pictures)"}
 
 
+def combined_leg(ctx, jobs, args):
+    """The same batch with combined-component ME (EncoderParams::CombinedME:
+    every stage on the combined luma+chroma pictures, 16-bit samples):
+    frames/s of the eager step."""
+    import copy
+
+    cjobs = []
+    for pn, refs, prm in jobs:
+        q = copy.copy(prm)
+        q.combined_me = 1
+        cjobs.append((pn, refs, q))
+
+    def step():
+        ctx.invalidate(-1)
+        ctx.run_batch(cjobs)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.set_profiling(True)
+    step()
+    ctx.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    return {"value": len(cjobs) / dt, "unit": "frames/s", "ms_per_step": dt * 1e3,
+            "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()}}
+
+
 def tuned_and_graph(ctx, jobs, args, eager_ms):
     """The same batch step with the setting dme_autotune_batch picks (eager
     dme_run_batch), and replayed as a captured HIP graph of that setting
@@ -606,6 +639,7 @@ def main():
     ap.add_argument("--no-imt", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 2160p large-motion leg")
+    ap.add_argument("--no-combined", action="store_true", help="skip the combined-component leg")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -712,6 +746,8 @@ def main():
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in kt.items()},
     }
     out["with_export"] = export_leg(ctx, jobs, geom, B, args, dt / args.steps)
+    if not args.no_combined:
+        out["combined_me"] = combined_leg(ctx, jobs, args)
     if not args.no_mc:
         out["motion_compensation"] = mc_leg(ctx, jobs, frames, params, geom, B, args)
     if not args.no_full:
