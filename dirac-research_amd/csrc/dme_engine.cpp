@@ -44,7 +44,8 @@ hipError_t launch_lambda(const JobDesc *jobs, const Task *tasks, int ntasks, int
 hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int level, int max_blen_x,
                            int max_blen_y, bool has_guide, bool u8, hipStream_t st);
 hipError_t launch_combine(const CombTask *ct, int nct, int total, hipStream_t st);
-hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st);
+hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                                 int max_blen_y, hipStream_t st);
 hipError_t launch_mode_tables16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, bool eighth,
                                 hipStream_t st);
 hipError_t launch_pel_wave(const JobDesc *jobs, const Task *tasks, int ntasks, int level, int max_rows, int max_blen_x,
@@ -1217,7 +1218,7 @@ static int stage_subpel(dme_ctx *ctx, std::vector<JobState *> &js, int &launches
         }));
     if (bulk16.n)
         DME_TRY(timed_launch(ctx, KK_SUBPEL_T16, launches, [&] {
-            return launch_subpel_table16(dj, dtask(ctx, bulk16), bulk16.n, bulk16.total, ctx->L->stream);
+            return launch_subpel_table16(dj, dtask(ctx, bulk16), bulk16.n, bulk16.total, bx, by, ctx->L->stream);
         }));
     if (wave.n) {
         DME_TRY(timed_launch(ctx, KK_SUBPEL_WAVE, launches,

@@ -2483,6 +2483,60 @@ __device__ int wave_bi_sad(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2)
     return wave_sum(s);
 }
 
+// The same sum for half- and quarter-pel luma jobs from the byte planes,
+// four samples per lane-op: a quarter-pel window's samples share one phase
+// (fx, fy), so each dword of four is picked out of four quarter-pel-plane
+// dwords (one per position), and |((2 p - r1) - r2) >> 1| =
+// |p - ceil((r1 + r2) / 2)| is a v_sad_u8 against the bytewise rounded-up
+// average (the bias of 128 cancels).  Lane (row, dword column) items.
+#ifndef BI_U8_ITEMS
+#define BI_U8_ITEMS 2
+#endif
+__device__ __forceinline__ uint32_t qp_phase_dword(const uint32_t *__restrict__ qp, int W, int H, int X, int Y, int fy,
+                                                   uint32_t sel) {
+    const uint32_t *__restrict__ qr = qp + (size_t)((clampi(Y, -1, H) + 1) * 4 + fy) * (W + 2);
+    uint32_t d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d[i] = qr[clampi(X + i, -1, W) + 1];
+    const uint32_t lo = __builtin_amdgcn_perm(d[1], d[0], sel), hi = __builtin_amdgcn_perm(d[3], d[2], sel);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+__device__ __forceinline__ uint32_t avg_up_u8(uint32_t a, uint32_t b);
+__device__ __forceinline__ uint32_t b8_dword(const uint8_t *__restrict__ origin, int pitch, int x, int y);
+
+template <int U = BI_U8_ITEMS>
+__device__ int wave_bi_sad_u8(const JobDesc &J, const Blk &b, dme_mv m1, dme_mv m2) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int W = J.W, H = J.H;
+    const UpStart s1 = up_start<false>(b, m1, J.step, 2 * W, 2 * H), s2 = up_start<false>(b, m2, J.step, 2 * W, 2 * H);
+    const int xq1 = 2 * s1.rsx + s1.rx, yq1 = 2 * s1.rsy + s1.ry, xq2 = 2 * s2.rsx + s2.rx, yq2 = 2 * s2.rsy + s2.ry;
+    const uint32_t sel1 = (uint32_t)(xq1 & 3) | ((uint32_t)((xq1 & 3) + 4) << 8) | 0x0c0c0000u;
+    const uint32_t sel2 = (uint32_t)(xq2 & 3) | ((uint32_t)((xq2 & 3) + 4) << 8) | 0x0c0c0000u;
+    const int nd = (b.xl + 3) >> 2, items = nd * b.yl;
+    uint32_t acc = 0;
+    // U items per lane with all their loads in flight together
+    for (int i0 = 0; i0 < items; i0 += U * WAVE) {
+        uint32_t p[U], r1[U], r2[U], mask[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = i0 + u * WAVE + lane;
+            p[u] = r1[u] = r2[u] = mask[u] = 0;
+            if (it < items) {
+                const int row = it / nd, col = it - row * nd;
+                const int valid = b.xl - 4 * col;
+                mask[u] = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+                p[u] = b8_dword(J.pic8, J.p8, b.xp + 4 * col, b.yp + row);
+                r1[u] = qp_phase_dword(J.qp[0], W, H, (xq1 >> 2) + 4 * col, (yq1 >> 2) + row, yq1 & 3, sel1);
+                r2[u] = qp_phase_dword(J.qp[1], W, H, (xq2 >> 2) + 4 * col, (yq2 >> 2) + row, yq2 & 3, sel2);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_sad_u8(p[u] & mask[u], avg_up_u8(r1[u], r2[u]) & mask[u], acc);
+    }
+    return wave_sum((int)acc);
+}
+
 // IntraBlockDiff::CalcDC (me_utils.cpp:284-299): truncating integer mean.
 __device__ int wave_block_dc(const int16_t *__restrict__ p, int W, const Blk &b) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -2546,7 +2600,8 @@ __global__ __launch_bounds__(256) void k_mode_l2(const JobDesc *__restrict__ job
             bc.mvcost = c1.mvcost + c2.mvcost;
             bc.SAD = b.empty() ? 0.0f
                                : (float)(UP8 ? up8_bi_sad(J, b, J.vec[0][k], J.vec[1][k])
-                                             : wave_bi_sad<E8, COMB>(J, b, J.vec[0][k], J.vec[1][k]));
+                                             : (E8 || COMB ? wave_bi_sad<E8, COMB>(J, b, J.vec[0][k], J.vec[1][k])
+                                                           : wave_bi_sad_u8(J, b, J.vec[0][k], J.vec[1][k])));
             bc.total = bc.SAD + loc_lambda * bc.mvcost;
             bc.total *= lf;
             uc = bc.total + mode_cost;
@@ -2629,9 +2684,7 @@ __device__ __forceinline__ dme_mv mode_pred(const dme_mv *__restrict__ V2, int X
 // ---------------------------------------------------------------------------
 
 // Packed rounded-up average of four byte pairs: (a | b) - ((a ^ b) >> 1).
-__device__ __forceinline__ uint32_t avg_up_u8(uint32_t a, uint32_t b) {
-    return (a | b) - (((a ^ b) & 0xfefefefeu) >> 1);
-}
+__device__ __forceinline__ uint32_t avg_up_u8(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) & 0xfefefefeu) >> 1); }
 
 // Per-launch LDS layout of k_mode_tables, in dwords per wave: picture rows of
 // pd dwords (prows of them), then for level 1 eight candidate slots of
@@ -2823,36 +2876,68 @@ __global__ __launch_bounds__(256) void k_mode_tables(const JobDesc *__restrict__
 // over the picture's int16 samples and the int16 up plane (up16_at), one
 // wave per block (sub-pel) or unit (mode decision).
 // ---------------------------------------------------------------------------
+// Sub-pel table: the block and the up-plane window every candidate reads
+// are staged in LDS (clamped reads, as up16_at clamps), then lane c sums
+// candidate c over the block.  LDS per wave: `words` int16 (host-sized for
+// the largest block).
+__host__ __device__ __forceinline__ int subpel16_window(int bl) { return 2 * bl + 4; }
+
 __global__ __launch_bounds__(256) void k_subpel_table16(const JobDesc *__restrict__ jobs,
-                                                        const Task *__restrict__ tasks, int ntasks, int total) {
-    const int lane = threadIdx.x & 63;
-    const int gi = xcd_block() * 4 + wave_id();
+                                                        const Task *__restrict__ tasks, int ntasks, int total,
+                                                        int words) {
+    extern __shared__ int16_t s_t16[];
+    const int wv = wave_id(), lane = threadIdx.x & 63;
+    const int gi = xcd_block() * 4 + wv;
     if (gi >= total) return;  // each wave works alone
     const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
     const JobDesc &J = jobs[tk.job];
     const int r = tk.ref, k = gi - tk.offset;
-    const int X = J.xnb, W = J.W, H = J.H;
+    const int X = J.xnb, W = J.W, H = J.H, UW = 2 * W, UH = 2 * H;
     const int xb = k % X, yb = k / X;
     const int prec = J.prec, rad = sub_radius(prec), Q = 2 * rad + 1, QS = Q * Q;
     const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], W, H, xb, yb);
     const dme_mv pel = J.vec[r][k];
     int *__restrict__ rec = J.qtab[r] + (size_t)wf_pos(xb, yb, X, J.ynb) * SUB_SLOT;
     if (!b.empty() && prec > 0) {
-        // candidate c: (pel << tq) + (c % Q - rad, c / Q - rad) in units of
-        // 1/2^tq pel, as k_subpel_bulk tabulates it
+        // candidate c: m = (pel << tq) + (c % Q - rad, c / Q - rad) in units
+        // of 1/2^tq pel (step tq), as k_subpel_bulk tabulates it; its up
+        // samples lie in columns ux0 .. ux0 + UC - 1 and rows uy0 .. of the
+        // up plane (the +1 neighbours of the quarter-pel weights included)
         const int tq = sub_unit(prec);
-        for (int c0 = 0; c0 < QS; c0 += 4) {
-            dme_mv m[4];
-            const int nc = min(4, QS - c0);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int c = c0 + (i < nc ? i : 0);
-                m[i].x = (pel.x << tq) + c % Q - rad;
-                m[i].y = (pel.y << tq) + c / Q - rad;
+        const int mx0 = (pel.x << tq) - rad, my0 = (pel.y << tq) - rad;
+        const int ux0 = 2 * b.xp + (tq == 2 ? (mx0 >> 1) : mx0), uy0 = 2 * b.yp + (tq == 2 ? (my0 >> 1) : my0);
+        const int UC = subpel16_window(b.xl), UR = subpel16_window(b.yl);
+        int16_t *__restrict__ s_pic = s_t16 + (size_t)wv * words;
+        int16_t *__restrict__ s_up = s_pic + ((b.xl * b.yl + 1) & ~1);
+        const int16_t *__restrict__ up = reinterpret_cast<const int16_t *>(J.qp[r]);
+        const int16_t *__restrict__ pic = J.pic[0];
+        for (int i = lane; i < UC * UR; i += WAVE) {
+            const int row = i / UC, col = i - row * UC;
+            s_up[i] = up[(size_t)clampi(uy0 + row, 0, UH - 1) * UW + clampi(ux0 + col, 0, UW - 1)];
+        }
+        for (int i = lane; i < b.xl * b.yl; i += WAVE) {
+            const int row = i / b.xl, col = i - row * b.xl;
+            s_pic[i] = pic[(size_t)(b.yp + row) * W + b.xp + col];
+        }
+        WAVE_SYNC();
+        if (lane < QS) {
+            dme_mv m;
+            m.x = (pel.x << tq) + lane % Q - rad;
+            m.y = (pel.y << tq) + lane / Q - rad;
+            const UpStart us = up_start<false>(b, m, tq, UW, UH);
+            const int rx = us.rx, ry = us.ry;
+            const int w0 = (2 - rx) * (2 - ry), w1 = rx * (2 - ry), w2 = (2 - rx) * ry, w3 = rx * ry;
+            const int16_t *q0 = s_up + (us.rsy - uy0) * UC + (us.rsx - ux0);
+            int sad = 0;
+            for (int dy = 0; dy < b.yl; ++dy) {
+                const int16_t *q = q0 + 2 * dy * UC;
+                const int16_t *pr = s_pic + dy * b.xl;
+                for (int dx = 0; dx < b.xl; ++dx) {
+                    const int v = w0 * q[2 * dx] + w1 * q[2 * dx + 1] + w2 * q[2 * dx + UC] + w3 * q[2 * dx + UC + 1];
+                    sad += abs(((v + 2) >> 2) - (int)pr[dx]);
+                }
             }
-            int sd[4] = {0, 0, 0, 0};
-            wave_up_sads<false, true>(J.pic[0], W, J.qp[r], 2 * W, 2 * H, b, tq, m, nc, sd);  // tq <= 2
-            if (lane < nc) rec[c0 + lane] = lane == 0 ? sd[0] : (lane == 1 ? sd[1] : (lane == 2 ? sd[2] : sd[3]));
+            rec[lane] = sad;
         }
     }
     if (lane == 0) {
@@ -2864,6 +2949,45 @@ __global__ __launch_bounds__(256) void k_subpel_table16(const JobDesc *__restric
         rec[53] = __float_as_int(pc.mvcost);
         rec[54] = __float_as_int(pc.total);
     }
+}
+
+// SAD of one candidate of a combined-component job over a unit, lanes over
+// its samples, 8 in flight: the candidate's weights are the wave's own, so
+// the loads of zero-weight up samples are skipped (one load a sample at
+// half pel, one to four at quarter and eighth pel).
+template <bool E8>
+__device__ int wave_up16_sad(const int16_t *__restrict__ pic, const int16_t *__restrict__ up, int W, int H,
+                             const Blk &b, const UpStart &s) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int UW = 2 * W, UH = 2 * H;
+    const bool e = E8 && s.e;
+    const int one = e ? 4 : 2, rx = s.rx, ry = s.ry;
+    const int w0 = (one - rx) * (one - ry), w1 = rx * (one - ry), w2 = (one - rx) * ry, w3 = rx * ry;
+    const int n = b.xl * b.yl;
+    int acc = 0;
+    for (int b0 = 0; b0 < n; b0 += 8 * WAVE) {
+        int p[8], v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = b0 + k * WAVE + lane;
+            p[k] = v[k] = 0;
+            if (i < n) {
+                const int dy = i / b.xl, dx = i - dy * b.xl;
+                const int uy = s.rsy + 2 * dy, ux = s.rsx + 2 * dx;
+                const int y0 = clampi(uy, 0, UH - 1), x0 = clampi(ux, 0, UW - 1);
+                const int y1 = clampi(uy + 1, 0, UH - 1), x1 = clampi(ux + 1, 0, UW - 1);
+                int t = w0 * up[(size_t)y0 * UW + x0];
+                if (rx) t += w1 * up[(size_t)y0 * UW + x1];
+                if (ry) t += w2 * up[(size_t)y1 * UW + x0];
+                if (rx && ry) t += w3 * up[(size_t)y1 * UW + x1];
+                v[k] = e ? (t + 8) >> 4 : (t + 2) >> 2;
+                p[k] = pic[(size_t)(b.yp + dy) * W + b.xp + dx];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += abs(v[k] - p[k]);
+    }
+    return wave_sum(acc);
 }
 
 template <bool E8>
@@ -2890,16 +3014,11 @@ __global__ __launch_bounds__(256) void k_mode_tables16(const JobDesc *__restrict
     // (candidates that no lane reads stay as they are, as k_mode_tables
     // leaves them)
     for (int r = 0; r < nr; ++r)
-        for (int q0 = 0; q0 < nq; q0 += 4) {
-            dme_mv m[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int q = q0 + i;
-                m[i] = J.vec[r][(size_t)(yu * side + q / side) * X2 + xu * side + q % side];
-            }
-            int sd[4] = {0, 0, 0, 0};
-            wave_up_sads<E8, true>(J.pic[0], W, J.qp[r], 2 * W, 2 * H, b, step, m, 4, sd);
-            if (lane < 4) out[nq * r + q0 + lane] = lane == 0 ? sd[0] : (lane == 1 ? sd[1] : (lane == 2 ? sd[2] : sd[3]));
+        for (int q = 0; q < nq; ++q) {
+            const dme_mv m = J.vec[r][(size_t)(yu * side + q / side) * X2 + xu * side + q % side];
+            const UpStart us = up_start<E8>(b, m, step, 2 * W, 2 * H);
+            const int sd = wave_up16_sad<E8>(J.pic[0], reinterpret_cast<const int16_t *>(J.qp[r]), W, H, b, us);
+            if (lane == 0) out[nq * r + q] = sd;
         }
     if (level == 1 && nr > 1) {
         // bi-prediction SADs of every pair (candidate q1 of reference 1,
@@ -3181,7 +3300,8 @@ __global__ __launch_bounds__(1024) void k_mode_wave(const JobDesc *__restrict__ 
             const dme_mv va0 = shfl_mv(mv0, 0), vb0 = shfl_mv(mv0, 1);
             dme_cost c0[2] = {shfl_cost(mc0, 0), shfl_cost(mc0, 1)};
             int bis0 = 0;
-            if (nr > 1 && !b0.empty()) bis0 = wave_bi_sad<E8, COMB, MODE_NB>(J, b0, va0, vb0);
+            if (nr > 1 && !b0.empty())
+                bis0 = E8 || COMB ? wave_bi_sad<E8, COMB, MODE_NB>(J, b0, va0, vb0) : wave_bi_sad_u8(J, b0, va0, vb0);
             int m0;
             const float sb0 = unit_decn(J, 0, lam0, c0, bis0, m0);
             if (sb0 <= sbc) {
@@ -4889,9 +5009,15 @@ hipError_t launch_subpel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks
     return hipGetLastError();
 }
 
-hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, hipStream_t st) {
+hipError_t launch_subpel_table16(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                                 int max_blen_y, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_subpel_table16, dim3((total + 3) / 4), dim3(256), 0, st, jobs, tasks, ntasks, total);
+    if (max_blen_x < 1 || max_blen_x > PEL_MAXB || max_blen_y < 1 || max_blen_y > PEL_MAXB) return hipErrorInvalidValue;
+    // per wave: the block and the window, int16, in whole dwords
+    const int words = (((max_blen_x * max_blen_y + 1) & ~1) + subpel16_window(max_blen_x) * subpel16_window(max_blen_y) +
+                       1) & ~1;
+    const size_t lds = sizeof(int16_t) * 4 * (size_t)words;  // 4 waves: at most 2 * (1024 + 68 * 68) * 4 bytes
+    hipLaunchKernelGGL(k_subpel_table16, dim3((total + 3) / 4), dim3(256), lds, st, jobs, tasks, ntasks, total, words);
     return hipGetLastError();
 }
 
