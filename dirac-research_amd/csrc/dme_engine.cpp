@@ -3303,6 +3303,24 @@ extern "C" int dme_lp_get_vectors(dme_ctx *ctx, int pic_num, int which, dme_mv *
     return dme::exception_status();
 }
 
+extern "C" int dme_lp_get_outputs(dme_ctx *ctx, int pic_num, float *out) try {
+    CTX_CHECK(ctx);
+    if (!out) return set_error(DME_ERR_INVALID, "NULL output");
+    auto it = ctx->jobs.find(pic_num);
+    if (it == ctx->jobs.end() || it->second->stage < 4 || !it->second->fv0[0].p)
+        return set_error(DME_ERR_STATE, "no fast learned estimate of picture " + std::to_string(pic_num));
+    JobState &J = *it->second;
+    const size_t B = (size_t)J.g.xnb * J.g.ynb;
+    if (ctx->lp_out.n < sizeof(float) * 32 * ((size_t)J.desc.lp_off + B))
+        return set_error(DME_ERR_STATE, "the network outputs of picture " + std::to_string(pic_num) + " are gone");
+    for (auto &ln : ctx->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
+    HIP_TRY(hipMemcpy(out, ctx->lp_out.as<float>() + (size_t)32 * J.desc.lp_off, sizeof(float) * 32 * B,
+                      hipMemcpyDeviceToHost));
+    return DME_OK;
+} catch (...) {
+    return dme::exception_status();
+}
+
 extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks) try {
     CTX_CHECK(ctx);
     long long b = ctx->lp_blocks, f = ctx->lp_fallbacks;

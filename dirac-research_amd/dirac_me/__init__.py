@@ -325,6 +325,13 @@ class Context:
         check(lib().dme_lp_get_vectors(self.h, pic_num, which, C.c_void_p(p[0]), C.c_void_p(p[1])))
         return out
 
+    def lp_outputs(self, pic_num, geom):
+        """The network's 32 outputs per block of a fast-form learned estimate
+        (valid until the context's next learned call)."""
+        out = np.zeros((geom.ynum_blocks * geom.xnum_blocks, 32), np.float32)
+        check(lib().dme_lp_get_outputs(self.h, pic_num, out.ctypes.data))
+        return out
+
     def lp_stats(self):
         b, f = C.c_longlong(), C.c_longlong()
         check(lib().dme_lp_get_stats(self.h, C.byref(b), C.byref(f)))

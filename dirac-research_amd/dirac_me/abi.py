@@ -175,6 +175,7 @@ SIGNATURES = {
     "dme_run_batch_learned": ([_P, C.POINTER(dme_job), _I, C.POINTER(dme_medata)], _I),
     "dme_lp_set_form": ([_P, _I], _I),
     "dme_lp_get_vectors": ([_P, _I, _I, _P, _P], _I),
+    "dme_lp_get_outputs": ([_P, _I, _P], _I),
     "dme_lp_get_stats": ([_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
     "dme_lp_training_pair": ([_P, _I, _P, _P], _I),
     "dme_lp_prepare": ([_P, _P, _P, _P], _I),

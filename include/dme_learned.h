@@ -71,6 +71,13 @@ int dme_lp_set_form(dme_ctx *ctx, int form);
  * array of ynum_blocks * xnum_blocks per reference (NULL skips). */
 int dme_lp_get_vectors(dme_ctx *ctx, int pic_num, int which, dme_mv *out_ref1, dme_mv *out_ref2);
 
+/* After a fast-form estimate of pic_num: the network's outputs for its
+ * blocks, ynum_blocks * xnum_blocks rows of 32 floats (per reference r and
+ * hypothesis h the offset at 8r + 2h, 8r + 2h + 1 in units of 8 pel; the
+ * predicted log(1 + SAD) of reference r at 16 + r).  Valid until the next
+ * learned call of the context; DME_ERR_STATE after that. */
+int dme_lp_get_outputs(dme_ctx *ctx, int pic_num, float *out);
+
 /* Blocks refined and blocks that took the fallback search since the
  * context was created. */
 int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fallbacks);

@@ -71,6 +71,59 @@ def test_fast_form_matches_oracle(lctx, case):
         lctx.drop(n)
 
 
+@pytest.mark.parametrize("case", ["cif2", "1080p", "720p_2416", "2160p_large_motion"])
+def test_refinement_matches_oracle(lctx, case):
+    """The exact refinement of the network's proposals (k_lp_refine) against
+    lp_ref.refine on the GPU's own network outputs: every block of a CIF
+    picture, a sample of blocks (all edge blocks among them) at larger
+    sizes.  A block whose fallback test lies within 1e-5 of its threshold is
+    skipped (expm1 of the predicted cost may round differently there)."""
+    import lp_ref
+    rng = np.random.default_rng(7)
+    if case == "cif2":
+        d = synth.sized(23, "cif", 2)
+        p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    elif case == "1080p":
+        d = synth.config1(106)
+        p = make_params(*d["block"], precision=2, depth=4, lam=10.0)
+    elif case == "2160p_large_motion":
+        d = synth.config4(107)
+        p = make_params(*d["block"], precision=2, depth=d["depth"], lam=10.0)
+    else:
+        d = synth.sized(72, "720p", 2)
+        p = make_params(24, 24, 16, 16, precision=2, depth=3, lam=10.0)
+    print("refinement case", case, flush=True)
+    got, v0, _, nums = _fast(lctx, d, p, 5300)
+    H, W = d["cur"].shape
+    g = dirac_me.geometry(p, W, H)
+    out4 = lctx.lp_outputs(5301, g)
+    _, cap, _ = O.run_oracle(p, d["cur"], d["refs"], d["u"], d["v"], stop=O.STOP_PEL, capture=True)
+    X, Y = g.xnum_blocks, g.ynum_blocks
+    if case == "cif2":
+        blocks = None
+    else:
+        edge = [(x, y) for y in range(Y) for x in range(X) if x in (0, 1, X - 2, X - 1) or y in (0, 1, Y - 2, Y - 1)]
+        inner = [(int(x), int(y)) for x, y in zip(rng.integers(0, X, 600), rng.integers(0, Y, 600))]
+        blocks = sorted(set(edge[::3] + inner))
+    nfb = nskip = ncmp = 0
+    for r in range(len(nums)):
+        want = lp_ref.refine(d["cur"], d["refs"][r], g, r, out4, cap.level_vectors[r][2], cap.level_costs[r][2],
+                             blocks)
+        for (x, y), (vx, vy, sad, fb, margin) in want.items():
+            if margin < 1e-5:
+                nskip += 1
+                continue
+            nfb += fb
+            ncmp += 1
+            gv = v0[r][y, x]
+            assert (int(gv["x"]), int(gv["y"])) == (vx, vy), "reference %d block (%d, %d): GPU (%d, %d), oracle " \
+                "(%d, %d), SAD %d, fallback %s" % (r + 1, x, y, gv["x"], gv["y"], vx, vy, sad, fb)
+    print("blocks compared", ncmp, "fallbacks", nfb, "skipped", nskip, flush=True)
+    assert nskip < 5
+    for n in [5301] + nums:
+        lctx.drop(n)
+
+
 def test_fast_form_half_pel(lctx):
     d = synth.sized(24, "cif", 2)
     p = make_params(*d["block"], precision=1, depth=d["depth"], lam=10.0)
