@@ -172,6 +172,14 @@ constexpr int DME_KIND_COMB_BELOW_EIGHTH = 4, DME_KIND_COMB_EIGHTH = 8;
 
 // One unit of a batched launch: a (job, reference) pair covering `count`
 // consecutive work items starting at `offset` of the launch's index space.
+// Refinement counters of k_lp_refine: per batch position (slot)
+// kLpStatCopies copies of the pair {blocks, fallbacks}, each on a 128-byte
+// line of its own, a wave adding to copy (its index mod kLpStatCopies), so
+// that the atomic adds of a batch spread over many lines.
+constexpr int kLpStatCopies = 64;
+constexpr int kLpStatStride = 16;  // unsigned long long per copy
+constexpr int kLpStatSlot = kLpStatCopies * kLpStatStride;
+
 struct Task {
     int job, ref, count, offset;
 };

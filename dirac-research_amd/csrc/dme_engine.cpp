@@ -2875,6 +2875,16 @@ static int lp_layout(std::vector<JobState *> &js, int base, std::vector<LpGroup>
     return P;
 }
 
+// Sums the refinement counters of the slots (host copy h of lp_stats).
+static void lp_sum_stats(const std::vector<unsigned long long> &h, size_t nslots, long long &blocks,
+                         long long &fallbacks) {
+    for (size_t i = 0; i < nslots; ++i)
+        for (int c = 0; c < kLpStatCopies; ++c) {
+            blocks += (long long)h[i * kLpStatSlot + c * kLpStatStride];
+            fallbacks += (long long)h[i * kLpStatSlot + c * kLpStatStride + 1];
+        }
+}
+
 // Proposal tensors for P positions and counters for nslots jobs (streams
 // idle).  Counters of a smaller array are added to the totals first.
 static int lp_reserve(dme_ctx *ctx, int P, size_t nslots) {
@@ -2882,16 +2892,13 @@ static int lp_reserve(dme_ctx *ctx, int P, size_t nslots) {
     DME_TRY(ctx->lp_out.ensure(sizeof(float) * 32 * (size_t)P));
     if (nslots > ctx->lp_slots) {
         if (ctx->lp_slots) {
-            std::vector<unsigned long long> h(2 * ctx->lp_slots);
+            std::vector<unsigned long long> h(kLpStatSlot * ctx->lp_slots);
             HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < ctx->lp_slots; ++i) {
-                ctx->lp_blocks += (long long)h[2 * i];
-                ctx->lp_fallbacks += (long long)h[2 * i + 1];
-            }
+            lp_sum_stats(h, ctx->lp_slots, ctx->lp_blocks, ctx->lp_fallbacks);
         }
         const size_t n = std::max(nslots, (size_t)64);
-        DME_TRY(ctx->lp_stats.ensure(sizeof(unsigned long long) * 2 * n));
-        HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(unsigned long long) * 2 * n, ctx->L->stream));
+        DME_TRY(ctx->lp_stats.ensure(sizeof(unsigned long long) * kLpStatSlot * n));
+        HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(unsigned long long) * kLpStatSlot * n, ctx->L->stream));
         HIP_TRY(hipStreamSynchronize(ctx->L->stream));
         ctx->lp_slots = n;
     }
@@ -2903,12 +2910,9 @@ static int lp_reserve(dme_ctx *ctx, int P, size_t nslots) {
 // (streams idle: every call ends with them idle).
 static int lp_fold_stats(dme_ctx *ctx) {
     if (!ctx->lp_slots || !ctx->lp_dirty) return DME_OK;
-    std::vector<unsigned long long> h(2 * ctx->lp_slots);
+    std::vector<unsigned long long> h(kLpStatSlot * ctx->lp_slots);
     HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < ctx->lp_slots; ++i) {
-        ctx->lp_blocks += (long long)h[2 * i];
-        ctx->lp_fallbacks += (long long)h[2 * i + 1];
-    }
+    lp_sum_stats(h, ctx->lp_slots, ctx->lp_blocks, ctx->lp_fallbacks);
     HIP_TRY(launch_lp_zero(ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), ctx->L->stream));
     HIP_TRY(hipStreamSynchronize(ctx->L->stream));
     ctx->lp_dirty = false;
@@ -3184,8 +3188,8 @@ static int run_jobs_learned_parts(dme_ctx *ctx, std::vector<JobState *> &js) {
         // the slots hold this batch's counts only (lp_fold_stats): the
         // redone jobs count anew
         for (auto *j : redo)
-            HIP_TRY(launch_lp_zero(ctx->lp_stats.as<unsigned long long>() + 2 * j->bpos, 2 * sizeof(unsigned long long),
-                                   ctx->L->stream));
+            HIP_TRY(launch_lp_zero(ctx->lp_stats.as<unsigned long long>() + (size_t)kLpStatSlot * j->bpos,
+                                   kLpStatSlot * sizeof(unsigned long long), ctx->L->stream));
         DME_TRY(learned_pel(ctx, redo, g, launches));
         DME_TRY(stage_lambda_host(ctx, redo, nullptr));
         DME_TRY(learned_rest(ctx, redo, launches));
@@ -3326,12 +3330,9 @@ extern "C" int dme_lp_get_stats(dme_ctx *ctx, long long *blocks, long long *fall
     long long b = ctx->lp_blocks, f = ctx->lp_fallbacks;
     if (ctx->lp_slots) {
         for (auto &ln : ctx->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
-        std::vector<unsigned long long> h(2 * ctx->lp_slots);
+        std::vector<unsigned long long> h(kLpStatSlot * ctx->lp_slots);
         HIP_TRY(hipMemcpy(h.data(), ctx->lp_stats.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < ctx->lp_slots; ++i) {
-            b += (long long)h[2 * i];
-            f += (long long)h[2 * i + 1];
-        }
+        lp_sum_stats(h, ctx->lp_slots, b, f);
     }
     if (blocks) *blocks = b;
     if (fallbacks) *fallbacks = f;

@@ -56,7 +56,6 @@ __device__ __forceinline__ int xcd_block() {
 // a loop there costs registers and occupancy).
 #define PW_MODE_L2 2
 #define PW_MODE_TABLES 4
-#define PW_LP_REFINE 2
 
 // Index of the calling wave in its workgroup, known to the compiler as
 // wave-uniform, so that what is derived from it stays in scalar registers.
@@ -3408,174 +3407,462 @@ __global__ __launch_bounds__(256) void k_lp_features(const JobDesc *__restrict__
     }
 }
 
-// Exact refinement of the proposals: one wave per (block, reference).  The
-// four hypotheses are clamped to within LP_BOX - 2 pel of 4 x the level-2
-// vector g; the picture block and the reference box g +- LP_BOX are staged
-// from the byte planes with one round of loads.  Candidates: the union of
-// the +-2 windows around the hypotheses, each position once.  If the best SAD exceeds 1.2 x the predicted SAD, the
-// +-4 window around g and the +-2 window around zero join in (zero's window
-// from a region of its own when it leaves the box) and replace the best only
-// with a smaller SAD.  Keys SAD * 1024 + raster index in the box (625 + k
-// for zero's window outside it) keep the first minimum.  Writes the level-0
-// vector and SAD; stats[lp_slot * 2 + {0, 1}] count blocks and fallbacks.
+// Exact refinement of the proposals.  Per (block, reference) item: the four
+// hypotheses are clamped to within LP_BOX - 2 pel of 4 x the level-2 vector
+// g; the candidates are the union of the +-2 windows around them; the winner
+// is the smallest key SAD * 1024 + raster index in the box g +- LP_BOX.  If
+// its SAD exceeds 1.2 x the predicted SAD, the +-4 window around g and the
+// +-2 window around zero (box raster index, or 625 + window index when zero's
+// window leaves the box) join in and replace the winner only with a smaller
+// SAD.  Keys are distinct per position, so a position evaluated twice (in two
+// overlapping windows) leaves the minimum unchanged.
+//
+// A wave takes LP_ITEMS consecutive items.  The picture block of each is
+// staged from the byte plane (realigned, zero past its width), its box as
+// whole dwords from the dword at or before the box's first column (a box
+// that leaves the plane's margin is copied sample by sample with clamped
+// coordinates).  Twenty lanes serve an item, lane (h, c) taking column c - 2
+// of hypothesis h's window: it slides down the window's rows, realigns each
+// box row it reads once and adds it to the SADs of the five candidates of its
+// column whose block rows it meets (lp_slide).  The fallback windows are
+// served the same way, a lane taking five candidate rows of one column.
+// Writes the level-0 vector and SAD; the slot's counters (kLpStatCopies copies
+// of {blocks, fallbacks}) count them.
 #define LP_BOX 12
 #define LP_BD (2 * LP_BOX + 1)
+#define LP_ITEMS 3
+#define LP_LANES 20
+#define LP_FB_TASKS 23  // fallback lanes per item: 9 columns x 2 row bands of g's window, 5 columns of zero's
 
-// LDS of one wave of k_lp_refine (dwords): picture rows of pp, the box rows
-// of rp (brows of them), zero's window rows of zp (zrows).
+// LDS of one wave of k_lp_refine (dwords): per item the picture block (prows
+// rows of pp dwords, rows 16-byte aligned), the box (brows rows of rp) and
+// zero's window (zrows rows of zp); then the items' key minima (8 ints).
 struct LpLds {
     int pp, prows, rp, brows, zp, zrows;
-    __device__ int wave() const { return pp * prows + rp * brows + zp * zrows + 100; }
+    __device__ int item() const { return (pp * prows + rp * brows + zp * zrows + 3) & ~3; }
+    __device__ int wave() const { return LP_ITEMS * item() + 8; }
 };
+
+// Index of offset (dx, dy) in win_off's order for radius 2.
+__device__ __forceinline__ int win_index2(int dx, int dy) {
+    if (dy == 0) return dx == 0 ? 0 : (dx > 0 ? 2 * dx - 1 : -2 * dx);
+    return 5 + (abs(dy) - 1) * 10 + (dx + 2) * 2 + (dy < 0 ? 1 : 0);
+}
+
+// SADs of five vertically adjacent candidates against the block whose packed
+// rows are at pic (pitch pp dwords, bytes past the block's width zero):
+// candidate dy (0..4) pairs block row y with source row row0 + dy + y, whose
+// bytes start at byte offset bo (which may include a staging shift).  Each
+// source row is read and realigned once and added to every candidate whose
+// block row it meets.  NR, ND: rows and dwords of a whole block known at
+// compile time; NR = 0 takes yl rows and nd dwords at run time, the last
+// realigned dword masked by lmask.
+template <int NR, int ND>
+__device__ __forceinline__ void lp_slide(const uint32_t *__restrict__ pic, int pp, const uint32_t *__restrict__ src,
+                                         int sp, int bo, int row0, int yl, int nd, uint32_t lmask, uint32_t acc[5]) {
+    const uint32_t *rb = src + row0 * sp + (bo >> 2);
+    const unsigned sh = bo & 3;
+    if constexpr (NR > 0) {
+        // rows stay in order: each row's loads are issued one iteration
+        // ahead and no further (left alone, the compiler issues every row's
+        // loads first and exhausts the registers)
+        uint32_t ring[5][ND], r[ND + 1];
+#pragma unroll
+        for (int d = 0; d <= ND; ++d) r[d] = rb[d];
+#pragma unroll
+        for (int j = 0; j < NR + 4; ++j) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (j < NR) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(pic + j * pp);
+                ring[j % 5][0] = q.x;
+                ring[j % 5][1] = q.y;
+                ring[j % 5][2] = q.z;
+                ring[j % 5][3] = q.w;
+                if constexpr (ND > 4) {
+                    const uint2 q2 = *reinterpret_cast<const uint2 *>(pic + j * pp + 4);
+                    ring[j % 5][4] = q2.x;
+                    ring[j % 5][5] = q2.y;
+                }
+            }
+            uint32_t a[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) a[d] = __builtin_amdgcn_alignbyte(r[d + 1], r[d], sh);
+            if (j + 1 < NR + 4) {
+#pragma unroll
+                for (int d = 0; d <= ND; ++d) r[d] = rb[(j + 1) * sp + d];
+            }
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy) {
+                const int y = j - dy;
+                if (y >= 0 && y < NR) {
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) acc[dy] = __builtin_amdgcn_sad_u8(ring[y % 5][d], a[d], acc[dy]);
+                }
+            }
+            // the row's sums are formed here, not deferred past later rows' loads
+            asm volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]));
+        }
+    } else {
+        for (int j = 0; j < yl + 4; ++j) {
+            uint32_t r[9], a[8];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) r[d] = d <= nd ? rb[j * sp + d] : 0u;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                a[d] = __builtin_amdgcn_alignbyte(r[d + 1], r[d], sh);
+                if (d == nd - 1) a[d] &= lmask;
+            }
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy) {
+                const int y = j - dy;
+                if (y >= 0 && y < yl) {
+                    const uint32_t *pr = pic + y * pp;
+#pragma unroll
+                    for (int d = 0; d < 8; ++d)
+                        if (d < nd) acc[dy] = __builtin_amdgcn_sad_u8(pr[d], a[d], acc[dy]);
+                }
+            }
+        }
+    }
+}
+
+// Smallest key of a lane's five candidates (dy < ndy count): index idx0 +
+// dy * LP_BD in the box, or, for zero's window staged apart (zx >= -2), 625 +
+// its window index of (zx, dy - 2).
+template <int NR, int ND>
+__device__ __forceinline__ int lp_column_key(const uint32_t *pic, int pp, const uint32_t *src, int sp, int bo, int row0,
+                                             int yl, int nd, uint32_t lmask, int ndy, int idx0, int zx) {
+    uint32_t acc[5] = {0u, 0u, 0u, 0u, 0u};
+    lp_slide<NR, ND>(pic, pp, src, sp, bo, row0, yl, nd, lmask, acc);
+    int key = INT_MAX;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const int idx = zx >= -2 ? LP_BD * LP_BD + win_index2(zx, dy - 2) : idx0 + dy * LP_BD;
+        if (dy < ndy) key = min(key, (int)acc[dy] * 1024 + idx);
+    }
+    return key;
+}
 
 __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
                                                    int ntasks, int total, const float *__restrict__ out4,
                                                    unsigned long long *__restrict__ stats, LpLds ly) {
     extern __shared__ uint32_t s_lp[];
-    const int wv = wave_id(), lane = threadIdx.x & 63;
-    const int g0 = (xcd_block() * 4 + wv) * PW_LP_REFINE, g1 = min(g0 + PW_LP_REFINE, total);
-    for (int gi = g0; gi < g1; ++gi) {
-        WAVE_SYNC();  // the previous block no longer reads the wave's LDS
-        uint32_t *s_pic = s_lp + wv * ly.wave();
-        uint32_t *s_box = s_pic + ly.pp * ly.prows;
-        uint32_t *s_zero = s_box + ly.rp * ly.brows;
-        int *s_list = reinterpret_cast<int *>(s_zero + ly.zp * ly.zrows);  // up to 100 union positions
-        const Task tk = tasks[find_task_wave(tasks, ntasks, gi)];
+    const int wv = wave_id(), lane = threadIdx.x & (WAVE - 1);
+    const int gbase = (xcd_block() * 4 + wv) * LP_ITEMS;
+    if (gbase >= total) return;
+    const int idw = ly.item();
+    uint32_t *s_w = s_lp + wv * ly.wave();
+    int *s_key = reinterpret_cast<int *>(s_w + LP_ITEMS * idw);  // [i] main minimum, [4 + i] fallback minimum
+    if (lane < 8) s_key[lane] = INT_MAX;
+    const int box_off = ly.pp * ly.prows, zero_off = box_off + ly.rp * ly.brows;
+
+    // The lanes of group i = min(lane / 20, 2) hold the descriptor of item
+    // gbase + i: job, reference, block index and limits, 4 x the level-2
+    // vector, the first network output of the block, the predicted SAD, the
+    // planes, and whether the box is copied as whole dwords (else sample by
+    // sample, clamped) with its staging shift.
+    const int me = min(lane / LP_LANES, LP_ITEMS - 1);
+    int d_job, d_r, d_k, d_xp, d_yp, d_xl, d_yl, d_gx, d_gy, d_o, d_fast, d_sh, d_on, d_p8, d_W, d_H;
+    const uint8_t *d_pic8, *d_ref8;
+    float d_pred;
+    {
+        int c0 = 0, c1 = 0, c2 = 0;  // tasks starting at or before each item
+        for (int j0 = 0; j0 < ntasks; j0 += WAVE) {
+            const int j = j0 + lane;
+            const int off = j < ntasks ? tasks[j].offset : INT_MAX;
+            c0 += __popcll(__ballot(off <= gbase));
+            c1 += __popcll(__ballot(off <= min(gbase + 1, total - 1)));
+            c2 += __popcll(__ballot(off <= min(gbase + 2, total - 1)));
+        }
+        const int gi = min(gbase + me, total - 1);
+        const Task tk = tasks[(me == 0 ? c0 : (me == 1 ? c1 : c2)) - 1];
         const JobDesc &J = jobs[tk.job];
         const int r = tk.ref, k = gi - tk.offset, X = J.xnb, x = k % X, y = k / X;
         const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.W, J.H, x, y);
-        dme_mv *__restrict__ V = J.vec[r];
-        dme_cost *__restrict__ C = J.pc[r];
-        if (b.empty()) {  // out of the picture: no update (pixel_match.cpp:278-281)
-            if (lane == 0) {
-                V[k].x = V[k].y = 0;
-                C[k].SAD = 10000000.0f;
-            }
-            continue;
-        }
-        const int W = J.W, H = J.H, bp = J.p8;
         dme_mv g;
         float s2;
         lp_guide(J, r, x, y, g, s2);
-        g.x *= 4;
-        g.y *= 4;
-        const float *o = out4 + ((size_t)J.lp_off + k) * 32;
-        dme_mv hyp[4];  // offsets from g
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            hyp[h].x = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h], -256.0f), 256.0f)), 2 - LP_BOX,
-                              LP_BOX - 2);
-            hyp[h].y = clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[8 * r + 2 * h + 1], -256.0f), 256.0f)), 2 - LP_BOX,
-                              LP_BOX - 2);
-        }
-        const float pred_sad = expm1f(fminf(o[16 + r], 20.0f));
-        // picture block and box in one round of loads
+        d_job = tk.job;
+        d_r = r;
+        d_k = k;
+        d_xp = b.xp;
+        d_yp = b.yp;
+        d_xl = b.xl;
+        d_yl = b.yl;
+        d_gx = 4 * g.x;
+        d_gy = 4 * g.y;
+        d_o = (J.lp_off + k) * 32;
+        d_on = gbase + me < total && !b.empty();
+        d_pred = expm1f(fminf(out4[(size_t)d_o + 16 + r], 20.0f));
+        d_p8 = J.p8;
+        d_W = J.W;
+        d_H = J.H;
+        d_pic8 = J.pic8;
+        d_ref8 = J.ref8[r];
+        const int bx0 = b.xp + d_gx - LP_BOX, by0 = b.yp + d_gy - LP_BOX, RH = b.yl + 2 * LP_BOX;
+        const int ax0 = bx0 & ~3, ncol = (b.xl + 2 * LP_BOX + (bx0 & 3) + 3) >> 2;
+        d_fast = ax0 >= -B8_MG && ax0 + 4 * ncol <= J.p8 - B8_MG && by0 >= -B8_MG && by0 + RH <= J.H + B8_MG;
+        d_sh = d_fast ? bx0 & 3 : 0;
+    }
+    // uniform copy of item i's value
+    auto U = [&](int v, int i) { return __builtin_amdgcn_readlane(v, i * LP_LANES); };
+    const unsigned long long onm = __ballot(lane % LP_LANES == 0 && d_on);
+    const int onbits = (int)((onm & 1ull) | (((onm >> 20) & 1ull) << 1) | (((onm >> 40) & 1ull) << 2));
+    // every active item a whole 16 x 16 block (the slide's unrolled form)
+    const bool full16 = __ballot(d_on && (d_xl != 16 || d_yl != 16)) == 0;
+
+    // staging: the 20 lanes of each item copy its picture block (realigned
+    // from the byte plane, zero past xl) and its box (whole dwords from the
+    // dword at or before its first column, or sample by sample with clamped
+    // coordinates), all items at once
+    if (lane < LP_ITEMS * LP_LANES && d_on) {
+        const int q = lane - me * LP_LANES, bp = d_p8, nd = (d_xl + 3) >> 2;
+        uint32_t *s_pic = s_w + me * idw, *s_box = s_pic + box_off;
         {
-            const uint8_t *const base[3] = {J.pic8, J.ref8[r], J.ref8[r]};
-            const int x0[3] = {b.xp, b.xp + g.x - LP_BOX, 0}, y0[3] = {b.yp, b.yp + g.y - LP_BOX, 0};
-            const int ppad = (b.xl + 15) & ~15, RW = b.xl + 2 * LP_BOX, RH = b.yl + 2 * LP_BOX;
-            B8Rect rc[3];
-            bool fast[3];
-            rc[0] = b8_rect(base[0], bp, s_pic, ly.pp, x0[0], y0[0], b.xl, b.yl, ppad);
-            rc[1] = b8_rect(base[1], bp, s_box, ly.rp, x0[1], y0[1], RW, RH, RW);
-            rc[2] = rc[1];
-            fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl, ppad);
-            fast[1] = b8_inside(bp, H, x0[1], y0[1], RH, RW);
-            fast[2] = false;
-            b8_stage(rc, fast, 2, base, bp, W, H, x0, y0);
+            const int n = d_yl * nd, psh = d_xp & 3;
+            const float inv = 1.0f / (float)nd;
+            const bool inside = b8_inside(bp, d_H, d_xp, d_yp, d_yl, 4 * nd);
+            const uint8_t *pb = d_pic8 + (ptrdiff_t)d_yp * bp + (d_xp - psh);
+            for (int e0 = 0; e0 < n; e0 += 4 * LP_LANES) {
+                uint32_t lo[4], hi[4];
+                int at[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = min(e0 + q + u * LP_LANES, n - 1);  // the last element repeats: no guard
+                    const int row = (int)(((float)e + 0.5f) * inv), col = e - row * nd;
+                    at[u] = row * ly.pp + col;
+                    if (inside) {
+                        const uint32_t *w = reinterpret_cast<const uint32_t *>(pb + (ptrdiff_t)row * bp) + col;
+                        lo[u] = w[0];
+                        hi[u] = w[1];
+                    } else {
+                        const uint8_t *src = d_pic8 + (ptrdiff_t)clampi(d_yp + row, 0, d_H - 1) * bp;
+                        uint32_t v = 0;
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) v |= (uint32_t)src[clampi(d_xp + 4 * col + kk, 0, d_W - 1)] << (8 * kk);
+                        lo[u] = v;
+                        hi[u] = 0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = min(e0 + q + u * LP_LANES, n - 1);
+                    const int row = (int)(((float)e + 0.5f) * inv), col = e - row * nd;
+                    const int valid = d_xl - 4 * col;
+                    const uint32_t mask = valid >= 4 ? 0xffffffffu : ((1u << (8 * valid)) - 1u);
+                    const uint32_t v = inside ? __builtin_amdgcn_alignbyte(hi[u], lo[u], psh) : lo[u];
+                    s_pic[at[u]] = v & mask;
+                }
+            }
+        }
+        {
+            const int bx0 = d_xp + d_gx - LP_BOX, by0 = d_yp + d_gy - LP_BOX;
+            const int RW = d_xl + 2 * LP_BOX, RH = d_yl + 2 * LP_BOX;
+            if (d_fast) {
+                const int ncol = (RW + d_sh + 3) >> 2, n = RH * ncol;
+                const float inv = 1.0f / (float)ncol;
+                const uint32_t *__restrict__ src =
+                    reinterpret_cast<const uint32_t *>(d_ref8 + (ptrdiff_t)by0 * bp + (bx0 - d_sh));
+                const int dp = bp >> 2;
+                for (int e0 = 0; e0 < n; e0 += 12 * LP_LANES) {
+                    uint32_t v[12];
+#pragma unroll
+                    for (int u = 0; u < 12; ++u) {
+                        const int e = min(e0 + q + u * LP_LANES, n - 1);  // the last element repeats: no guard
+                        const int row = (int)(((float)e + 0.5f) * inv), col = e - row * ncol;
+                        v[u] = src[(ptrdiff_t)row * dp + col];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 12; ++u) {
+                        const int e = min(e0 + q + u * LP_LANES, n - 1);
+                        const int row = (int)(((float)e + 0.5f) * inv), col = e - row * ncol;
+                        s_box[row * ly.rp + col] = v[u];
+                    }
+                }
+            } else {
+                const int ncol = (RW + 3) >> 2, n = RH * ncol;
+                const float inv = 1.0f / (float)ncol;
+                for (int e = q; e < n; e += LP_LANES) {
+                    const int row = (int)(((float)e + 0.5f) * inv), col = e - row * ncol;
+                    const uint8_t *src = d_ref8 + (ptrdiff_t)clampi(by0 + row, 0, d_H - 1) * bp;
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk)
+                        if (4 * col + kk < RW) v |= (uint32_t)src[clampi(bx0 + 4 * col + kk, 0, d_W - 1)] << (8 * kk);
+                    s_box[row * ly.rp + col] = v;
+                }
+            }
+        }
+    }
+    WAVE_SYNC();
+
+    // One loop serves the main windows (pass 0: lane = item x 20 + 5h + c)
+    // and, when an item falls back, its fallback windows (pass 1: task t =
+    // fallback item x 23 + q over the lanes), so that each form of lp_slide
+    // is instantiated once.
+    int best_sad[LP_ITEMS], best_idx[LP_ITEMS], nfb = 0, fbm = 0;
+    for (int pass = 0, t0 = 0;;) {
+        const int t = t0 + lane;
+        int it = LP_ITEMS, src_off = box_off, sp = ly.rp, bo = 0, row0 = 0, ndy = 5, idx0 = 0, zx = -99, slot = 0;
+        if (pass == 0) {
+            it = lane / LP_LANES;
+        } else if (t < nfb * LP_FB_TASKS) {
+            const int f = t / LP_FB_TASKS;
+            const int m1 = fbm & (fbm - 1), m2 = m1 & (m1 - 1);  // the f-th fallback item
+            it = __builtin_ctz(f == 0 ? fbm : (f == 1 ? m1 : m2));
+        }
+        // the item's fields, by lane shuffles with every lane active (a
+        // shuffle from an inactive lane does not return its value)
+        const int src_lane = min(it, LP_ITEMS - 1) * LP_LANES;
+        const int gx = __shfl(d_gx, src_lane), gy = __shfl(d_gy, src_lane), sh = __shfl(d_sh, src_lane);
+        const int xl = __shfl(d_xl, src_lane), yl = __shfl(d_yl, src_lane), on = __shfl(d_on, src_lane);
+        const int obase = __shfl(d_o, src_lane) + 8 * __shfl(d_r, src_lane);
+        if (pass == 0) {
+            if (it < LP_ITEMS && on) {
+                const int q = lane - it * LP_LANES, h = q / 5, dx = q - 5 * h - 2;
+                const float *o = out4 + (size_t)obase + 2 * h;
+                const int hx =
+                    clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[0], -256.0f), 256.0f)), 2 - LP_BOX, LP_BOX - 2);
+                const int hy =
+                    clampi((int)rintf(fminf(fmaxf(LP_SCALE_PEL * o[1], -256.0f), 256.0f)), 2 - LP_BOX, LP_BOX - 2);
+                const int px = hx + dx;
+                row0 = hy - 2 + LP_BOX;
+                bo = px + LP_BOX + sh;
+                idx0 = row0 * LP_BD + px + LP_BOX;
+                slot = it;
+            } else {
+                it = LP_ITEMS;
+            }
+        } else if (it < LP_ITEMS) {
+            const int q = t - (t / LP_FB_TASKS) * LP_FB_TASKS;
+            slot = 4 + it;
+            if (q < 18) {  // g's +-4 window: column q % 9, rows -4..0 or 1..4
+                const int band = q >= 9, px = q - 9 * band - 4;
+                row0 = (band ? 1 : -4) + LP_BOX;
+                ndy = band ? 4 : 5;
+                bo = px + LP_BOX + sh;
+                idx0 = row0 * LP_BD + px + LP_BOX;
+            } else {  // zero's +-2 window, column wx
+                const int wx = q - 20;
+                if (abs(gx) <= LP_BOX - 2 && abs(gy) <= LP_BOX - 2) {
+                    const int px = wx - gx;
+                    row0 = -2 - gy + LP_BOX;
+                    bo = px + LP_BOX + sh;
+                    idx0 = row0 * LP_BD + px + LP_BOX;
+                } else {
+                    src_off = zero_off;
+                    sp = ly.zp;
+                    bo = wx + 2;
+                    row0 = 0;
+                    zx = wx;
+                }
+            }
+        }
+        if (it < LP_ITEMS) {
+            const uint32_t *pic = s_w + it * idw;
+            const int nd = (xl + 3) >> 2;
+            const uint32_t lmask = (xl & 3) ? (1u << (8 * (xl & 3))) - 1u : 0xffffffffu;
+            const int key = full16 ? lp_column_key<16, 4>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy,
+                                                          idx0, zx)
+                                   : lp_column_key<0, 0>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy,
+                                                         idx0, zx);
+            atomicMin(&s_key[slot], key);
         }
         WAVE_SYNC();
-        const bool whole = (b.xl & 15) == 0;
-        // union of the hypothesis windows: the positions of its bounding box that
-        // lie in a window, compacted into a list (box raster indices), then one
-        // SAD per listed position
-        int ux0 = hyp[0].x, ux1 = hyp[0].x, uy0 = hyp[0].y, uy1 = hyp[0].y;
+        if (pass == 0) {
+            // the winners; items whose SAD exceeds 1.2 x the predicted SAD fall back
+            const int key = s_key[me];
+            const unsigned long long fl =
+                __ballot(lane % LP_LANES == 0 && d_on && (float)(key >> 10) > 1.2f * d_pred);
+            fbm = (int)((fl & 1ull) | (((fl >> 20) & 1ull) << 1) | (((fl >> 40) & 1ull) << 2));
+            nfb = __popc(fbm);
 #pragma unroll
-        for (int h = 1; h < 4; ++h) {
-            ux0 = min(ux0, hyp[h].x);
-            ux1 = max(ux1, hyp[h].x);
-            uy0 = min(uy0, hyp[h].y);
-            uy1 = max(uy1, hyp[h].y);
-        }
-        ux0 -= 2;
-        uy0 -= 2;
-        const int UW = ux1 + 3 - ux0, UN = UW * (uy1 + 3 - uy0);
-        int cnt = 0;
-        for (int j0 = 0; j0 < UN; j0 += WAVE) {
-            const int j = j0 + lane, py = uy0 + j / UW, px = ux0 + j % UW;
-            bool in = false;
-#pragma unroll
-            for (int h = 0; h < 4; ++h) in = in || (abs(px - hyp[h].x) <= 2 && abs(py - hyp[h].y) <= 2);
-            in = in && j < UN;
-            const unsigned long long m = __ballot(in);
-            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (in) s_list[at] = (py + LP_BOX) * LP_BD + px + LP_BOX;
-            cnt += __popcll(m);
-        }
-        WAVE_SYNC();
-        int key = INT_MAX;
-        for (int i = lane; i < cnt; i += WAVE) {
-            const int idx = s_list[i], px = idx % LP_BD, py = idx / LP_BD;
-            const int s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl)
-                                : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px, py, b.xl, b.yl);
-            key = min(key, s * 1024 + idx);
-        }
-        key = wave_min_i(key);
-        int best_sad = key >> 10;
-        dme_mv best;
-        best.x = g.x + (key & 1023) % LP_BD - LP_BOX;
-        best.y = g.y + (key & 1023) / LP_BD - LP_BOX;
-        const bool fallback = (float)best_sad > 1.2f * pred_sad;
-        if (fallback) {  // uniform
-            // zero's window: in the box when |g| <= LP_BOX - 2, else staged apart
-            const bool zin = abs(g.x) <= LP_BOX - 2 && abs(g.y) <= LP_BOX - 2;
-            if (!zin) {
+            for (int i = 0; i < LP_ITEMS; ++i) {
+                const int ki = U(key, i);
+                best_sad[i] = ki >> 10;
+                best_idx[i] = ki & 1023;
+            }
+            if (!nfb) break;
+            // zero's window outside the box: staged apart (realigned, clamped if needed)
+#pragma unroll 1
+            for (int i = 0; i < LP_ITEMS; ++i) {
+                if (!((fbm >> i) & 1) || (abs(U(d_gx, i)) <= LP_BOX - 2 && abs(U(d_gy, i)) <= LP_BOX - 2)) continue;
+                const JobDesc &J = jobs[U(d_job, i)];
+                const int bp = J.p8, r = U(d_r, i), xl = U(d_xl, i), yl = U(d_yl, i);
                 const uint8_t *const base[3] = {J.ref8[r], J.ref8[r], J.ref8[r]};
-                const int x0[3] = {b.xp - 2, 0, 0}, y0[3] = {b.yp - 2, 0, 0};
+                const int x0[3] = {U(d_xp, i) - 2, 0, 0}, y0[3] = {U(d_yp, i) - 2, 0, 0};
                 B8Rect rc[3];
                 bool fast[3];
-                rc[0] = b8_rect(base[0], bp, s_zero, ly.zp, x0[0], y0[0], b.xl + 4, b.yl + 4, b.xl + 4);
+                rc[0] = b8_rect(base[0], bp, s_w + i * idw + zero_off, ly.zp, x0[0], y0[0], xl + 4, yl + 4, xl + 4);
                 rc[1] = rc[2] = rc[0];
-                fast[0] = b8_inside(bp, H, x0[0], y0[0], b.yl + 4, b.xl + 4);
+                fast[0] = b8_inside(bp, J.H, x0[0], y0[0], yl + 4, xl + 4);
                 fast[1] = fast[2] = false;
-                b8_stage(rc, fast, 1, base, bp, W, H, x0, y0);
-                WAVE_SYNC();
+                b8_stage(rc, fast, 1, base, bp, J.W, J.H, x0, y0);
             }
-            int key2 = INT_MAX;
-            for (int i = lane; i < 106; i += WAVE) {
-                const bool gw = i < 81;
-                const dme_mv w = gw ? win_off(i, 4) : win_off(i - 81, 2);
-                int s, idx;
-                if (gw || zin) {
-                    const int px = gw ? w.x : w.x - g.x, py = gw ? w.y : w.y - g.y;  // offsets from g
-                    s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl)
-                              : packed_sad_u8<true>(s_pic, ly.pp, s_box, ly.rp, px + LP_BOX, py + LP_BOX, b.xl, b.yl);
-                    idx = (py + LP_BOX) * LP_BD + px + LP_BOX;
-                } else {
-                    s = whole ? packed_sad_u8<false>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl)
-                              : packed_sad_u8<true>(s_pic, ly.pp, s_zero, ly.zp, w.x + 2, w.y + 2, b.xl, b.yl);
-                    idx = LP_BD * LP_BD + (i - 81);
-                }
-                key2 = min(key2, s * 1024 + idx);
+            WAVE_SYNC();
+            pass = 1;
+            t0 = 0;
+        } else {
+            t0 += WAVE;
+            if (t0 >= nfb * LP_FB_TASKS) break;
+        }
+    }
+
+    // results, one item at a time; the counters are added once per run of
+    // items of one slot
+    const int key2 = s_key[4 + me];
+    unsigned long long *const cst = stats + (size_t)((xcd_block() * 4 + wv) % kLpStatCopies) * kLpStatStride;
+    int cslot = -1, cb = 0, cf = 0;
+#pragma unroll 1
+    for (int i = 0; i < LP_ITEMS; ++i) {
+        if (gbase + i >= total) break;
+        const JobDesc &J = jobs[U(d_job, i)];
+        const int r = U(d_r, i), k = U(d_k, i);
+        if (!((onbits >> i) & 1)) {  // out of the picture: no update (pixel_match.cpp:278-281)
+            if (lane == 0) {
+                J.vec[r][k].x = J.vec[r][k].y = 0;
+                J.pc[r][k].SAD = 10000000.0f;
             }
-            key2 = wave_min_i(key2);
-            if ((key2 >> 10) < best_sad) {
-                const int idx = key2 & 1023;
-                best_sad = key2 >> 10;
-                if (idx < LP_BD * LP_BD) {
-                    best.x = g.x + idx % LP_BD - LP_BOX;
-                    best.y = g.y + idx / LP_BD - LP_BOX;
-                } else {
-                    best = win_off(idx - LP_BD * LP_BD, 2);
-                }
+            continue;
+        }
+        int sad = i == 0 ? best_sad[0] : (i == 1 ? best_sad[1] : best_sad[2]);
+        int idx = i == 0 ? best_idx[0] : (i == 1 ? best_idx[1] : best_idx[2]);
+        const bool fbi = (fbm >> i) & 1;
+        bool zero_win = false;
+        if (fbi) {
+            const int k2 = U(key2, i);
+            if ((k2 >> 10) < sad) {
+                sad = k2 >> 10;
+                idx = k2 & 1023;
+                zero_win = idx >= LP_BD * LP_BD;
             }
+        }
+        dme_mv best;
+        if (zero_win) {
+            best = win_off(idx - LP_BD * LP_BD, 2);
+        } else {
+            best.x = U(d_gx, i) + idx % LP_BD - LP_BOX;
+            best.y = U(d_gy, i) + idx / LP_BD - LP_BOX;
         }
         if (lane == 0) {
-            V[k] = best;
-            C[k].SAD = (float)best_sad;
-            atomicAdd(&stats[2 * J.lp_slot], 1ull);
-            if (fallback) atomicAdd(&stats[2 * J.lp_slot + 1], 1ull);
+            J.vec[r][k] = best;
+            J.pc[r][k].SAD = (float)sad;
         }
+        const int slot = J.lp_slot;
+        if (slot != cslot) {
+            if (cslot >= 0 && lane == 0) {
+                atomicAdd(cst + (size_t)cslot * kLpStatSlot, (unsigned long long)cb);
+                if (cf) atomicAdd(cst + (size_t)cslot * kLpStatSlot + 1, (unsigned long long)cf);
+            }
+            cslot = slot;
+            cb = cf = 0;
+        }
+        ++cb;
+        cf += fbi;
+    }
+    if (cslot >= 0 && lane == 0) {
+        atomicAdd(cst + (size_t)cslot * kLpStatSlot, (unsigned long long)cb);
+        if (cf) atomicAdd(cst + (size_t)cslot * kLpStatSlot + 1, (unsigned long long)cf);
     }
 }
 
@@ -5173,16 +5460,18 @@ hipError_t launch_lp_features(const JobDesc *jobs, const Task *tasks, int ntasks
 // picture row and, from dword offset (2 * radius) / 4, one dword more of a
 // region row.
 static size_t lp_refine_lds(int max_blen_x, int max_blen_y, LpLds &ly) {
-    const int nch = ((max_blen_x + 3) / 4 + 3) / 4;
-    ly.pp = 4 * nch;
+    const int nd = (max_blen_x + 3) / 4;
+    ly.pp = (nd + 3) & ~3;
     ly.prows = max_blen_y;
-    ly.rp = max((max_blen_x + 2 * LP_BOX + 3) / 4, (2 * LP_BOX) / 4 + 4 * nch + 1) | 1;
+    // a lane reads dwords q .. q + nd of a box row, q <= (2 LP_BOX + 3) / 4
+    ly.rp = ((2 * LP_BOX + 3) / 4 + nd + 1) | 1;
     ly.brows = max_blen_y + 2 * LP_BOX;
-    ly.zp = max((max_blen_x + 4 + 3) / 4, 1 + 4 * nch + 1) | 1;
+    ly.zp = (2 + nd) | 1;
     ly.zrows = max_blen_y + 4;
-    return sizeof(uint32_t) * 4 * (size_t)(ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 100);
+    const int item = (ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 3) & ~3;
+    return sizeof(uint32_t) * 4 * (size_t)(LP_ITEMS * item + 8);
 }
-constexpr size_t kLpRefineLdsMax = 64 * 1024;
+constexpr size_t kLpRefineLdsMax = 160 * 1024;
 
 // LDS of k_fast_mode (four waves): the packed picture rows of the largest
 // unit of the launch (level-1 and level-0 lengths).
@@ -5232,7 +5521,7 @@ std::string launch_limits(const Geometry &g, bool full_search, bool learned, boo
         mode_tables_lds(g.blen[0][0], g.blen[0][1], g.blen[1][0], g.blen[1][1], ml) > kTableLdsMax)
         return "mode-decision units exceed the LDS of the unit tables";
     LpLds ll;
-    if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 64 KiB of LDS";
+    if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 160 KiB of LDS";
     int fpd;
     if (learned && fast_mode_lds(std::max(g.blen[0][0], g.blen[1][0]), std::max(g.blen[0][1], g.blen[1][1]), fpd) >
                        kFastModeLdsMax)
@@ -5251,8 +5540,8 @@ hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, 
     LpLds ly;
     const size_t lds = lp_refine_lds(max_blen_x, max_blen_y, ly);
     if (lds > kLpRefineLdsMax) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_lp_refine, dim3((total + 4 * PW_LP_REFINE - 1) / (4 * PW_LP_REFINE)), dim3(256), lds, st, jobs, tasks, ntasks, total, out4, stats,
-                       ly);
+    hipLaunchKernelGGL(k_lp_refine, dim3((total + 4 * LP_ITEMS - 1) / (4 * LP_ITEMS)), dim3(256), lds, st, jobs, tasks,
+                       ntasks, total, out4, stats, ly);
     return hipGetLastError();
 }
 
