@@ -336,6 +336,56 @@ def imt_leg(ctx, jobs, frames, geom, B, args):
     return out
 
 
+def mvdata_leg(ctx, jobs, frames, geom, B, args):
+    """The MV-data wire format of the batch's estimates (host code):
+    dme_mvdata_encode_batch over the B 1080p two-reference pictures of one
+    step on one thread and on 16, and dme_mvdata_decode on one thread.  The
+    reference's coder cannot be built here (it needs the configure-generated
+    dirac-stdint.h), so the oracle's pure-Python restatement is timed beside
+    it on a bounded sample."""
+    import dirac_me
+    from dirac_me import mvdata
+
+    est = [dirac_me.MEData(geom) for _ in frames]
+    ctx.run_batch(jobs[:len(frames)], est)
+    infos = [mvdata.info(geom, 2, 2) for _ in range(B)]
+    mes = [est[j % len(frames)] for j in range(B)]
+    mvdata.encode_batch(infos[:2], mes[:2], threads=1, cap=1 << 20)
+    out = {}
+    for th in (1, 16):
+        t0 = time.perf_counter()
+        data = mvdata.encode_batch(infos, mes, threads=th, cap=1 << 20)
+        dt = time.perf_counter() - t0
+        out["encode_%d_threads" % th] = {"value": B / dt, "unit": "pictures/s", "threads": th}
+    nbytes = sum(len(x) for x in data)
+    t0 = time.perf_counter()
+    for x in data:
+        mvdata.decode(x, geom, 2)
+    dt = time.perf_counter() - t0
+    out["decode"] = {"value": B / dt, "unit": "pictures/s", "threads": 1}
+    out["bytes_per_picture"] = nbytes / B
+    out["pictures"] = B
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import mvcodec_ref as R
+        me = est[0]
+        split = me.sb_split.tolist()
+        mode = me.mode.tolist()
+        vec = [[[(int(v["x"]), int(v["y"])) for v in row] for row in me.vectors[r]] for r in range(2)]
+        dc = [me.dc[c].tolist() for c in range(3)]
+        bp = tuple(geom.luma_blen[2]) + tuple(geom.luma_bsep[2])
+        n = 0
+        t0 = time.time()
+        while n < 4 and time.time() - t0 < args.cpu_budget / 4:
+            R.encode(split, mode, vec, dc, 2, bp, 2)
+            n += 1
+        cdt = time.time() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "pictures/s", "cores": 1, "kind": "port",
+                               "sample": "%d 1080p 2-ref pictures, the oracle's pure-Python restatement (the "
+                                         "reference's coder cannot be built here), %.1f s" % (n, cdt)}
+    return out
+
+
 def full_search_leg(ctx, jobs, frames, args):
     """Full search (the encoder's FullSearch option, range 32: a 65 x 65
     window per block and reference at temporal distance 1) over the first
@@ -637,6 +687,7 @@ def main():
     ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-imt", action="store_true")
+    ap.add_argument("--no-mvdata", action="store_true", help="skip the MV-data wire-format leg")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 2160p large-motion leg")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined-component leg")
@@ -754,6 +805,8 @@ def main():
         out["full_search"] = full_search_leg(ctx, jobs, frames, args)
     if not args.no_imt:
         out["imt"] = imt_leg(ctx, jobs, frames, geom, B, args)
+    if not args.no_mvdata:
+        out["mvdata"] = mvdata_leg(ctx, jobs, frames, geom, B, args)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, jobs, frames, geom, B, args, fps)
         out["learned"]["speedup_vs_plain_graph"] = out["learned"]["value"] / gfps

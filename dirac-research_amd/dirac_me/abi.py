@@ -83,6 +83,11 @@ class dme_mc_job(C.Structure):
                 ("mv", C.POINTER(dme_medata)), ("planes", dme_mc_plane * 3)]
 
 
+class dme_mvdata_info(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("xnum_sb", "ynum_sb", "num_refs", "xblen", "yblen", "xbsep", "ybsep",
+                                          "mv_precision", "weights_bits", "ref1_weight", "ref2_weight")]
+
+
 class dme_imt_sequence(C.Structure):
     _fields_ = [("chroma", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("source_sampling", C.c_int32),
                 ("topfieldfirst", C.c_int32), ("frame_rate_num", C.c_int32), ("frame_rate_den", C.c_int32),
@@ -128,6 +133,12 @@ SIGNATURES = {
     "dme_version": ([], _I),
     "dme_geometry_for": ([C.POINTER(dme_params), _I, _I, C.POINTER(dme_geometry)], _I),
     "dme_lambda_map_host": ([_I, _I, _I, _P, _P, C.c_float, _P], _I),
+    "dme_mvdata_encode": ([C.POINTER(dme_mvdata_info), C.POINTER(dme_medata), C.c_void_p, C.c_size_t,
+                           C.POINTER(C.c_size_t)], _I),
+    "dme_mvdata_decode": ([C.c_void_p, C.c_size_t, C.POINTER(dme_mvdata_info), C.POINTER(dme_medata),
+                           C.POINTER(C.c_size_t)], _I),
+    "dme_mvdata_encode_batch": ([_I, C.POINTER(dme_mvdata_info), C.POINTER(dme_medata), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _I], _I),
     "dme_imt_write_sequence": ([C.POINTER(dme_imt_sequence), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)], _I),
     "dme_imt_write_picture": ([C.POINTER(dme_imt_picture), C.POINTER(dme_medata), C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)], _I),

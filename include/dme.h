@@ -138,6 +138,49 @@ typedef struct dme_medata {
                                   * 300-320)                                 */
 } dme_medata;
 
+/* ---- MV data wire format (host only) ----------------------------------- *
+ * The bytes of an inter picture's motion data in the bitstream: what
+ * PictureCompressor::CodeMVData writes (picture_compress.cpp:463-530) and
+ * PictureDecompressor::DecompressMVData reads (picture_decompress.cpp:
+ * 266-357) -- MvDataByteIO's header (mvdata_byteio.cpp: block parameters,
+ * MV precision, no global motion, default prediction mode, picture
+ * weights), then the arithmetic-coded split modes, prediction modes, vector
+ * elements of reference 1 (x, y) and 2 (x, y) and DC values of Y, U, V
+ * (SplitModeCodec, PredModeCodec, VectorElementCodec, DCCodec, mv_codec.cpp:
+ * 49-641), each preceded by its byte count (mvdataelement_byteio.cpp).  The
+ * arrays are the estimator's, in the dme_medata layout; only sb_split, mode,
+ * vectors and dc are read or written.                                       */
+typedef struct dme_mvdata_info {
+    int32_t xnum_sb, ynum_sb;    /* superblock grid; blocks are 4x each way   */
+    int32_t num_refs;            /* PictureParams::NumRefs(): 1 or 2          */
+    int32_t xblen, yblen, xbsep, ybsep; /* PicturePredParams::LumaBParams(2):
+                                    dme_geometry luma_blen[2], luma_bsep[2]  */
+    int32_t mv_precision;        /* 0 pel .. 3 eighth pel                     */
+    int32_t weights_bits, ref1_weight, ref2_weight; /* picture weights; the
+                                    encoder's defaults are 1, 1, 1           */
+} dme_mvdata_info;
+
+/* Writes one picture's MV data.  Sets *len to its byte count; with buf ==
+ * NULL only sizes it, with cap < *len writes nothing and fails.
+ * Replaces PictureCompressor::CodeMVData (picture_compress.cpp:463).        */
+int dme_mvdata_encode(const dme_mvdata_info *info, const dme_medata *md,
+                      uint8_t *buf, size_t cap, size_t *len);
+/* Reads one picture's MV data from buf[0..len): info gives the grid and
+ * num_refs and receives the header's fields; md's sb_split, mode, vectors
+ * and dc receive the data, each unit's values in all its blocks and zero
+ * where nothing is coded (vectors of references a block does not use, DC of
+ * inter blocks).  *consumed (may be NULL) = bytes read.  Malformed data
+ * fails (naming the element) and leaves the arrays untouched.
+ * Replaces PictureDecompressor::DecompressMVData (picture_decompress.cpp:266). */
+int dme_mvdata_decode(const uint8_t *buf, size_t len, dme_mvdata_info *info,
+                      dme_medata *md, size_t *consumed);
+/* dme_mvdata_encode of n pictures on up to `threads` host threads (<= 0: one
+ * per core); lens[i] receives each byte count.  No counterpart: the
+ * reference codes one picture at a time.                                    */
+int dme_mvdata_encode_batch(int n, const dme_mvdata_info *info,
+                            const dme_medata *md, uint8_t *const *bufs,
+                            const size_t *caps, size_t *lens, int threads);
+
 typedef struct dme_ctx dme_ctx;
 
 /* ---- host-only helpers (no GPU needed) --------------------------------- */

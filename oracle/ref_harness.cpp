@@ -24,6 +24,7 @@
 #include <libdirac_common/motion.h>
 #include <libdirac_common/picture.h>
 #include <libdirac_common/upconvert.h>
+#include <libdirac_common/video_format_defaults.h>
 #include <libdirac_encoder/enc_picture.h>
 #include <libdirac_encoder/enc_queue.h>
 #include <libdirac_motionest/block_match.h>
@@ -580,4 +581,42 @@ extern "C" int refh_imt_read(const char *text, size_t len, int32_t hdr[11], dme_
     if (in.fail()) return -1;
     dump_medata(me, total_refs, out);
     return 0;
+}
+
+// ---- helpers of the MV-data codecs (mv_codec.cpp), compiled here from the
+// reference's motion.cpp and video_format_defaults.cpp -----------------------
+
+extern "C" unsigned refh_umean(const unsigned *v, int n) {
+    std::vector<unsigned int> a(v, v + n);
+    return dirac::GetUMean(a);
+}
+
+extern "C" int refh_smean(const int *v, int n) {
+    std::vector<int> a(v, v + n);
+    return dirac::GetSMean(a);
+}
+
+extern "C" int refh_median_list(const int *v, int n) {
+    std::vector<int> a(v, v + n);
+    return dirac::Median(a);
+}
+
+extern "C" int refh_block_params_index(int xblen, int yblen, int xbsep, int ybsep) {
+    return (int)dirac::BlockParametersIndex(dirac::OLBParams(xblen, yblen, xbsep, ybsep));
+}
+
+// SetDefaultBlockParameters(bparams, index) for index 1..4: out = xblen,
+// yblen, xbsep, ybsep
+extern "C" int refh_default_block_params(int index, int out[4]) {
+    try {
+        dirac::OLBParams b(0, 0, 0, 0);
+        dirac::SetDefaultBlockParameters(b, index);
+        out[0] = b.Xblen();
+        out[1] = b.Yblen();
+        out[2] = b.Xbsep();
+        out[3] = b.Ybsep();
+        return 0;
+    } catch (...) {
+        return -1;
+    }
 }

@@ -15,6 +15,9 @@
 //    access, overflow or other undefined behaviour aborts the run.
 // 3. Random parameter sets through dme_geometry_for, random vector fields
 //    through dme_lambda_map_host, random sizes through the generator.
+// 4. The MV-data wire format: random fields are encoded and decoded (the
+//    decoded arrays must re-encode to the same bytes), then mutated and
+//    truncated byte streams are decoded (untrusted input).
 #include "../../include/dme.h"
 
 #include <cstdint>
@@ -197,6 +200,89 @@ void synthesis() {
     (void)dme_synth_chroma(y.data(), W, H, next(), u.data(), v.data());
 }
 
+// One random MV-data field (consistent units) through the encoder and the
+// decoder, then mutations of its bytes through the decoder.
+void mvdata_round(long it) {
+    dme_mvdata_info info;
+    std::memset(&info, 0, sizeof(info));
+    info.xnum_sb = uni(1, 12);
+    info.ynum_sb = uni(1, 9);
+    info.num_refs = uni(1, 2);
+    info.xblen = uni(4, 32);
+    info.yblen = uni(4, 32);
+    info.xbsep = uni(4, 32);
+    info.ybsep = uni(4, 32);
+    info.mv_precision = uni(0, 3);
+    info.weights_bits = uni(0, 3) ? 1 : uni(0, 8);
+    info.ref1_weight = uni(0, 3) ? 1 : uni(-9, 9);
+    info.ref2_weight = uni(0, 3) ? 1 : uni(-9, 9);
+    const int S = info.xnum_sb * info.ynum_sb, X = 4 * info.xnum_sb, B = 16 * S;
+    std::vector<int32_t> split(S), mode(B);
+    std::vector<dme_mv> v1(B), v2(B);
+    std::vector<int16_t> dc[3] = {std::vector<int16_t>(B), std::vector<int16_t>(B), std::vector<int16_t>(B)};
+    const int big = it % 4 == 0 ? 1 << 20 : 64;
+    for (int s = 0; s < S; ++s) {
+        split[s] = uni(0, 2);
+        const int step = 4 >> split[s], xs = s % info.xnum_sb, ys = s / info.xnum_sb;
+        for (int y = 4 * ys; y < 4 * ys + 4; y += step)
+            for (int x = 4 * xs; x < 4 * xs + 4; x += step) {
+                const int m = uni(0, info.num_refs == 2 ? 3 : 1);
+                const dme_mv a{uni(-big, big), uni(-big, big)}, b{uni(-big, big), uni(-big, big)};
+                const int16_t d[3] = {(int16_t)uni(-32768, 32767), (int16_t)uni(-200, 200), (int16_t)uni(-9, 9)};
+                for (int j = y; j < y + step; ++j)
+                    for (int i = x; i < x + step; ++i) {
+                        mode[j * X + i] = m;
+                        v1[j * X + i] = a;
+                        v2[j * X + i] = b;
+                        for (int c = 0; c < 3; ++c) dc[c][j * X + i] = d[c];
+                    }
+            }
+    }
+    dme_medata md;
+    std::memset(&md, 0, sizeof(md));
+    md.sb_split = split.data();
+    md.mode = mode.data();
+    md.vectors[0] = v1.data();
+    md.vectors[1] = v2.data();
+    for (int c = 0; c < 3; ++c) md.dc[c] = dc[c].data();
+    size_t len = 0;
+    if (dme_mvdata_encode(&info, &md, nullptr, 0, &len) != DME_OK) std::abort();
+    std::vector<uint8_t> bytes(len);
+    if (dme_mvdata_encode(&info, &md, bytes.data(), len, &len) != DME_OK) std::abort();
+    // decode into fresh arrays and re-encode: the same bytes
+    std::vector<int32_t> split2(S), mode2(B);
+    std::vector<dme_mv> w1(B), w2(B);
+    std::vector<int16_t> e[3] = {std::vector<int16_t>(B), std::vector<int16_t>(B), std::vector<int16_t>(B)};
+    dme_medata out = md;
+    out.sb_split = split2.data();
+    out.mode = mode2.data();
+    out.vectors[0] = w1.data();
+    out.vectors[1] = w2.data();
+    for (int c = 0; c < 3; ++c) out.dc[c] = e[c].data();
+    dme_mvdata_info back;
+    std::memset(&back, 0, sizeof(back));
+    back.xnum_sb = info.xnum_sb;
+    back.ynum_sb = info.ynum_sb;
+    back.num_refs = info.num_refs;
+    size_t used = 0;
+    if (dme_mvdata_decode(bytes.data(), bytes.size(), &back, &out, &used) != DME_OK || used != bytes.size())
+        std::abort();
+    size_t len2 = 0;
+    std::vector<uint8_t> again(len + 1);
+    if (dme_mvdata_encode(&back, &out, again.data(), again.size(), &len2) != DME_OK || len2 != len ||
+        std::memcmp(again.data(), bytes.data(), len) != 0)
+        std::abort();
+    // mutated and truncated streams: refused or decoded, never out of bounds
+    for (int k = 0; k < 8; ++k) {
+        std::vector<uint8_t> m = bytes;
+        const int nmut = uni(1, 6);
+        for (int q = 0; q < nmut && !m.empty(); ++q) m[uni(0, (int)m.size() - 1)] = (uint8_t)uni(0, 255);
+        if (k & 1) m.resize(uni(0, (int)m.size()));
+        dme_mvdata_info mi = back;
+        (void)dme_mvdata_decode(m.data(), m.size(), &mi, &out, &used);
+    }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -227,8 +313,9 @@ int main(int argc, char **argv) {
         accepted += walk(mutate(text), false);
         geometry_and_lambda();
         if (i % 8 == 0) synthesis();
+        mvdata_round(i);
     }
-    std::printf("host_fuzz: %d records round-tripped; %ld mutated texts, %ld records accepted\n", records, iters,
-                accepted);
+    std::printf("host_fuzz: %d records round-tripped; %ld mutated texts, %ld records accepted; %ld MV-data fields "
+                "round-tripped\n", records, iters, accepted, iters);
     return 0;
 }
