@@ -3608,8 +3608,10 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
     auto U = [&](int v, int i) { return __builtin_amdgcn_readlane(v, i * LP_LANES); };
     const unsigned long long onm = __ballot(lane % LP_LANES == 0 && d_on);
     const int onbits = (int)((onm & 1ull) | (((onm >> 20) & 1ull) << 1) | (((onm >> 40) & 1ull) << 2));
-    // every active item a whole 16 x 16 block (the slide's unrolled form)
+    // every active item a whole 16 x 16 (24 x 24) block: the slide's
+    // unrolled forms
     const bool full16 = __ballot(d_on && (d_xl != 16 || d_yl != 16)) == 0;
+    const bool full24 = !full16 && __ballot(d_on && (d_xl != 24 || d_yl != 24)) == 0;
 
     // staging: the 20 lanes of each item copy its picture block (realigned
     // from the byte plane, zero past xl) and its box (whole dwords from the
@@ -3762,10 +3764,10 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
             const uint32_t *pic = s_w + it * idw;
             const int nd = (xl + 3) >> 2;
             const uint32_t lmask = (xl & 3) ? (1u << (8 * (xl & 3))) - 1u : 0xffffffffu;
-            const int key = full16 ? lp_column_key<16, 4>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy,
-                                                          idx0, zx)
-                                   : lp_column_key<0, 0>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy,
-                                                         idx0, zx);
+            const int key =
+                full16 ? lp_column_key<16, 4>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy, idx0, zx)
+                : full24 ? lp_column_key<24, 6>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy, idx0, zx)
+                         : lp_column_key<0, 0>(pic, ly.pp, pic + src_off, sp, bo, row0, yl, nd, lmask, ndy, idx0, zx);
             atomicMin(&s_key[slot], key);
         }
         WAVE_SYNC();
