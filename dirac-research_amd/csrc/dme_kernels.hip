@@ -3473,11 +3473,11 @@ __device__ __forceinline__ void lp_slide(const uint32_t *__restrict__ pic, int p
         for (int j = 0; j < NR + 4; ++j) {
             __builtin_amdgcn_sched_barrier(0);
             if (j < NR) {
+                // rows are 16-byte aligned and at least four dwords long
                 const uint4 q = *reinterpret_cast<const uint4 *>(pic + j * pp);
-                ring[j % 5][0] = q.x;
-                ring[j % 5][1] = q.y;
-                ring[j % 5][2] = q.z;
-                ring[j % 5][3] = q.w;
+                const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int d = 0; d < (ND < 4 ? ND : 4); ++d) ring[j % 5][d] = w[d];
                 if constexpr (ND > 4) {
                     const uint2 q2 = *reinterpret_cast<const uint2 *>(pic + j * pp + 4);
                     ring[j % 5][4] = q2.x;
@@ -3865,6 +3865,206 @@ __global__ __launch_bounds__(256) void k_lp_refine(const JobDesc *__restrict__ j
     if (cslot >= 0 && lane == 0) {
         atomicAdd(cst + (size_t)cslot * kLpStatSlot, (unsigned long long)cb);
         if (cf) atomicAdd(cst + (size_t)cslot * kLpStatSlot + 1, (unsigned long long)cf);
+    }
+}
+
+// Level 0 of the pel search for luma jobs, from the byte planes: per block
+// the SADs of the 5 x 5 candidates within R = 2 of zero and of the guide, the
+// first minima of the nine radius-1 windows centred within one pel of each
+// centre, and the block's record -- what pel_bulk_block<true> computes, laid
+// out for throughput: PB_BLOCKS consecutive blocks per wave, ten lanes per
+// block (table t, column c) each sliding down its column's five candidates
+// (lp_slide), so that a region row is read and realigned once for five SADs.
+#define PB_BLOCKS 6
+#define PB_LANES 10
+
+// LDS of one wave of k_pel_bulk8 (dwords): per block the picture block
+// (prows rows of pp dwords, rows 16-byte aligned), the zero- and
+// guide-centred regions (rrows rows of rp each), the two 5 x 5 tables and
+// the 18 window keys.
+struct Pb8Lds {
+    int pp, prows, rp, rrows;
+    int ndm, ncm;  // dwords of the widest block row and of the widest staged region row
+    __device__ int item() const { return (pp * prows + 2 * rp * rrows + 50 + 18 + 3) & ~3; }
+    __device__ int wave() const { return PB_BLOCKS * item(); }
+};
+
+__global__ __launch_bounds__(256) void k_pel_bulk8(const JobDesc *__restrict__ jobs, const Task *__restrict__ tasks,
+                                                   int ntasks, int total, Pb8Lds ly) {
+    extern __shared__ uint32_t s_pb[];
+    const int wv = wave_id(), lane = threadIdx.x & (WAVE - 1);
+    const int gbase = (xcd_block() * 4 + wv) * PB_BLOCKS;
+    if (gbase >= total) return;
+    const int idw = ly.item(), reg_off = ly.pp * ly.prows, tab_off = reg_off + 2 * ly.rp * ly.rrows;
+    uint32_t *s_w = s_pb + wv * ly.wave();
+    // the lanes of group i = min(lane / 10, 5) hold the descriptor of block gbase + i
+    const int me = min(lane / PB_LANES, PB_BLOCKS - 1), gi = min(gbase + me, total - 1);
+    int cnt = 0;
+    for (int j0 = 0; j0 < ntasks; j0 += WAVE) {
+        const int j = j0 + lane;
+        const int off = j < ntasks ? tasks[j].offset : INT_MAX;
+#pragma unroll
+        for (int i = 0; i < PB_BLOCKS; ++i) {
+            const int c = __popcll(__ballot(off <= min(gbase + i, total - 1)));
+            if (i == me) cnt += c;
+        }
+    }
+    const Task tk = tasks[cnt - 1];
+    const JobDesc &J = jobs[tk.job];
+    const int local = gi - tk.offset, X = J.lxb[0], xb = local % X, yb = local / X;
+    const Blk b = block_limits(J.blen[2], J.bsep[2], J.boff[2], J.lw[0], J.lh[0], xb, yb);
+    const int has_guide = 0 < J.depth;
+    dme_mv g;
+    g.x = g.y = 0;
+    if (has_guide) g = pel_guide(J, tk.ref, 0, local);
+    const int valid = gbase + me < total, on = valid && !b.empty();
+    auto U = [&](int v, int i) { return __builtin_amdgcn_readlane(v, i * PB_LANES); };
+    const unsigned long long onl = __ballot(lane % PB_LANES == 0 && lane < PB_BLOCKS * PB_LANES && on);
+    // the slide's unrolled forms: every active block whole at 16 x 16, 12 x 12 or 24 x 24
+    const unsigned long long odd16 = __ballot(on && (b.xl != 16 || b.yl != 16));
+    const unsigned long long odd12 = __ballot(on && (b.xl != 12 || b.yl != 12));
+    const unsigned long long odd24 = __ballot(on && (b.xl != 24 || b.yl != 24));
+
+    // staging of all blocks at once.  Task (block i, k): k < 2 ND a half
+    // (even or odd rows) of a dword column of the picture block, realigned
+    // to the block's first column and zero past its width; the other tasks a
+    // dword column of the zero- or guide-centred region, copied whole from the
+    // dword at or before the region's first column (its shift goes into the
+    // slide's byte offset).  Columns near the planes' edges are read sample by
+    // sample with clamped coordinates.
+    const int ndm = ly.ndm, ncm = ly.ncm, ntask = 2 * ndm + 2 * ncm;
+    auto region_x0 = [&](int xp, int gx, int t) { return xp - 2 + (t ? gx : 0); };
+    for (int t0 = 0; t0 < PB_BLOCKS * ntask; t0 += WAVE) {
+        const int task = t0 + lane, i = min(task / ntask, PB_BLOCKS - 1), k = task - i * ntask;
+        const int src = i * PB_LANES;
+        const int ion = (int)((onl >> src) & 1), igd = __shfl(has_guide, src);
+        const int ijob = __shfl(tk.job, src), ir = __shfl(tk.ref, src);
+        const int xp = __shfl(b.xp, src), yp = __shfl(b.yp, src), xl = __shfl(b.xl, src), yl = __shfl(b.yl, src);
+        const int gx = __shfl(g.x, src), gy = __shfl(g.y, src);
+        if (task >= PB_BLOCKS * ntask || !ion) continue;
+        const JobDesc &Ji = jobs[ijob];
+        const int bp = Ji.p8, W = Ji.lw[0], H = Ji.lh[0], nd = (xl + 3) >> 2;
+        uint32_t *s_pic = s_w + i * idw;
+        if (k < 2 * ndm) {
+            const int col = k >> 1, half = k & 1;
+            if (col >= nd) continue;
+            const int valid_b = xl - 4 * col;
+            const uint32_t mask = valid_b >= 4 ? 0xffffffffu : ((1u << (8 * valid_b)) - 1u);
+            const int sh = xp & 3;
+            const bool inside = (xp & ~3) + 4 * col + 8 <= bp - B8_MG && yp + yl <= H + B8_MG;
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(Ji.pic8 + (ptrdiff_t)yp * bp + (xp & ~3)) + col;
+            for (int r0 = half; r0 < yl; r0 += 16) {
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int row = min(r0 + 2 * u, yl - 1);  // rows past the block repeat its last row
+                    if (inside) {
+                        const uint32_t *w = q + (ptrdiff_t)row * (bp >> 2);
+                        v[u] = __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+                    } else {
+                        const uint8_t *sr = Ji.pic8 + (ptrdiff_t)clampi(yp + row, 0, H - 1) * bp;
+                        uint32_t x = 0;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) x |= (uint32_t)sr[clampi(xp + 4 * col + e, 0, W - 1)] << (8 * e);
+                        v[u] = x;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s_pic[min(r0 + 2 * u, yl - 1) * ly.pp + col] = v[u] & mask;
+            }
+        } else {
+            const int kk = k - 2 * ndm, t = kk >= ncm, col = kk - t * ncm;
+            if (t && !igd) continue;
+            const int x0 = region_x0(xp, gx, t), y0 = yp - 2 + (t ? gy : 0), RW = xl + 4, RH = yl + 4;
+            const int sh = x0 & 3, ncol = (RW + sh + 3) >> 2;
+            if (col >= ncol) continue;
+            uint32_t *dst = s_pic + reg_off + t * ly.rp * ly.rrows + col;
+            const int ax = (x0 & ~3) + 4 * col;
+            const bool inside = ax >= -B8_MG && ax + 4 <= bp - B8_MG && y0 >= -B8_MG && y0 + RH <= H + B8_MG;
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(Ji.ref8[ir] + (ptrdiff_t)y0 * bp + ax);
+            for (int r0 = 0; r0 < RH; r0 += 12) {
+                uint32_t v[12];
+#pragma unroll
+                for (int u = 0; u < 12; ++u) {
+                    const int row = min(r0 + u, RH - 1);
+                    if (inside) {
+                        v[u] = q[(ptrdiff_t)row * (bp >> 2)];
+                    } else {
+                        const uint8_t *sr = Ji.ref8[ir] + (ptrdiff_t)clampi(y0 + row, 0, H - 1) * bp;
+                        uint32_t x = 0;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) x |= (uint32_t)sr[clampi(ax + e, 0, W - 1)] << (8 * e);
+                        v[u] = x;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 12; ++u) dst[min(r0 + u, RH - 1) * ly.rp] = v[u];
+            }
+        }
+    }
+    WAVE_SYNC();
+    // the tables: lane (block, t, c) slides down column c of table t
+    {
+        const int q = lane - me * PB_LANES, t = q / 5, c = q - 5 * t;
+        if (lane < PB_BLOCKS * PB_LANES && on && (t == 0 || has_guide)) {
+            const uint32_t *pic = s_w + me * idw, *src = pic + reg_off + t * ly.rp * ly.rrows;
+            const int nd = (b.xl + 3) >> 2;
+            const uint32_t lmask = (b.xl & 3) ? (1u << (8 * (b.xl & 3))) - 1u : 0xffffffffu;
+            uint32_t acc[5] = {0u, 0u, 0u, 0u, 0u};
+            const int bo = c + ((b.xp - 2 + (t ? g.x : 0)) & 3);
+            if (!odd16) lp_slide<16, 4>(pic, ly.pp, src, ly.rp, bo, 0, b.yl, nd, lmask, acc);
+            else if (!odd12) lp_slide<12, 3>(pic, ly.pp, src, ly.rp, bo, 0, b.yl, nd, lmask, acc);
+            else if (!odd24) lp_slide<24, 6>(pic, ly.pp, src, ly.rp, bo, 0, b.yl, nd, lmask, acc);
+            else lp_slide<0, 0>(pic, ly.pp, src, ly.rp, bo, 0, b.yl, nd, lmask, acc);
+            int *tab = reinterpret_cast<int *>(s_w + me * idw + tab_off) + t * 25;
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy) tab[dy * 5 + c] = (int)acc[dy];
+        }
+    }
+    WAVE_SYNC();
+    // the window keys (SAD * 128 + enumeration index) of every block: lane
+    // task (block i, t * 9 + w) scans window w of table t
+    for (int t0 = 0; t0 < PB_BLOCKS * 18; t0 += WAVE) {
+        const int task = t0 + lane, i = min(task / 18, PB_BLOCKS - 1), q = task - i * 18;
+        const int ion = (int)((onl >> (i * PB_LANES)) & 1), igd = __shfl(has_guide, i * PB_LANES);
+        if (task < PB_BLOCKS * 18) {
+            const int t = q / 9, w = q - 9 * t;
+            int *tabs = reinterpret_cast<int *>(s_w + i * idw + tab_off);
+            int key = INT_MAX;
+            if (ion && (t == 0 || igd)) key = pel_window_min<1>(tabs + t * 25 + (w / 3 + 1) * 5 + (w % 3 + 1));
+            tabs[50 + q] = key;
+        }
+    }
+    WAVE_SYNC();
+    // the records: keys, then the winner of the zero list and of the guide
+    // list (radius 1, centred windows)
+    if (lane < PB_BLOCKS * PB_LANES && valid) {
+        const int q = lane - me * PB_LANES;
+        const int *keys = reinterpret_cast<const int *>(s_w + me * idw + tab_off) + 50;
+        int *rec = J.ptab[tk.ref] + (size_t)wf_pos(xb, yb, X, J.lyb[0]) * PEL_REC;
+        rec[8 + q] = keys[q];
+        if (q < 8) rec[18 + q] = keys[10 + q];
+        if (q == 0) {
+            int best_sad = 10000000;
+            dme_mv best_mv;
+            best_mv.x = best_mv.y = 0;
+            if (on) {
+                const int k0 = keys[4], k1 = keys[13];
+                best_sad = k0 >> 7;
+                best_mv = win_off(k0 & 127, 1);
+                if (has_guide && (k1 >> 7) < best_sad) {
+                    best_sad = k1 >> 7;
+                    best_mv = win_off(k1 & 127, 1);
+                    best_mv.x += g.x;
+                    best_mv.y += g.y;
+                }
+            }
+            rec[0] = best_sad;
+            rec[1] = best_mv.x;
+            rec[2] = best_mv.y;
+            rec[3] = g.x;
+            rec[4] = g.y;
+        }
     }
 }
 
@@ -5210,6 +5410,38 @@ static size_t pel_bulk_lds(int level, int max_blen_x, int max_blen_y, bool has_g
 }
 constexpr size_t kPelBulkLdsMax = 64 * 1024;
 
+// Dynamic LDS of k_pel_bulk8 (four waves) for blocks up to bx x by.  A
+// lane reads region dwords q .. q + nd, q <= 1; the staging writes up to ncm.
+static size_t pel_bulk8_lds(int max_blen_x, int max_blen_y, Pb8Lds &ly) {
+    const int nd = (max_blen_x + 3) / 4;
+    ly.ndm = nd;
+    ly.ncm = (max_blen_x + 4 + 3 + 3) / 4;  // region of xl + 4 samples from a shift up to 3
+    ly.pp = (nd + 3) & ~3;
+    ly.prows = max_blen_y;
+    ly.rp = std::max(ly.ncm, nd + 2) | 1;
+    ly.rrows = max_blen_y + 4;
+    const int item = (ly.pp * ly.prows + 2 * ly.rp * ly.rrows + 50 + 18 + 3) & ~3;
+    return sizeof(uint32_t) * 4 * (size_t)PB_BLOCKS * item;
+}
+constexpr size_t kPelBulk8LdsMax = 159 * 1024;
+
+static hipError_t launch_pel_bulk8(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int max_blen_x,
+                                   int max_blen_y, hipStream_t st) {
+    Pb8Lds ly;
+    const size_t lds = pel_bulk8_lds(max_blen_x, max_blen_y, ly);
+    if (lds > kPelBulk8LdsMax) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pel_bulk8, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           159 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_pel_bulk8, dim3((total + 4 * PB_BLOCKS - 1) / (4 * PB_BLOCKS)), dim3(256), lds, st, jobs,
+                       tasks, ntasks, total, ly);
+    return hipGetLastError();
+}
+
 // u8: level 0 of luma jobs (the pictures themselves, whose samples were
 // checked to 8-bit range at registration), from the byte planes; else 16-bit
 // samples (the pyramid levels, and every level of combined-component jobs).
@@ -5222,8 +5454,8 @@ hipError_t launch_pel_bulk(const JobDesc *jobs, const Task *tasks, int ntasks, i
     const size_t lds = pel_bulk_lds(level, max_blen_x, max_blen_y, has_guide, u8, ly);
     if (lds > kPelBulkLdsMax) return hipErrorInvalidValue;
     const dim3 grid((total + 4 * PW_PEL_BULK - 1) / (4 * PW_PEL_BULK));
-    if (u8) hipLaunchKernelGGL(k_pel_bulk<true>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
-    else hipLaunchKernelGGL(k_pel_bulk<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
+    if (u8) return launch_pel_bulk8(jobs, tasks, ntasks, total, max_blen_x, max_blen_y, st);
+    hipLaunchKernelGGL(k_pel_bulk<false>, grid, dim3(256), lds, st, jobs, tasks, ntasks, total, level, ly);
     return hipGetLastError();
 }
 
@@ -5473,7 +5705,7 @@ static size_t lp_refine_lds(int max_blen_x, int max_blen_y, LpLds &ly) {
     const int item = (ly.pp * ly.prows + ly.rp * ly.brows + ly.zp * ly.zrows + 3) & ~3;
     return sizeof(uint32_t) * 4 * (size_t)(LP_ITEMS * item + 8);
 }
-constexpr size_t kLpRefineLdsMax = 160 * 1024;
+constexpr size_t kLpRefineLdsMax = 159 * 1024;
 
 // LDS of k_fast_mode (four waves): the packed picture rows of the largest
 // unit of the launch (level-1 and level-0 lengths).
@@ -5523,7 +5755,7 @@ std::string launch_limits(const Geometry &g, bool full_search, bool learned, boo
         mode_tables_lds(g.blen[0][0], g.blen[0][1], g.blen[1][0], g.blen[1][1], ml) > kTableLdsMax)
         return "mode-decision units exceed the LDS of the unit tables";
     LpLds ll;
-    if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 160 KiB of LDS";
+    if (learned && lp_refine_lds(bx, by, ll) > kLpRefineLdsMax) return "learned refinement exceeds 159 KiB of LDS";
     int fpd;
     if (learned && fast_mode_lds(std::max(g.blen[0][0], g.blen[1][0]), std::max(g.blen[0][1], g.blen[1][1]), fpd) >
                        kFastModeLdsMax)
@@ -5542,6 +5774,13 @@ hipError_t launch_lp_refine(const JobDesc *jobs, const Task *tasks, int ntasks, 
     LpLds ly;
     const size_t lds = lp_refine_lds(max_blen_x, max_blen_y, ly);
     if (lds > kLpRefineLdsMax) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_lp_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           159 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
     hipLaunchKernelGGL(k_lp_refine, dim3((total + 4 * LP_ITEMS - 1) / (4 * LP_ITEMS)), dim3(256), lds, st, jobs, tasks,
                        ntasks, total, out4, stats, ly);
     return hipGetLastError();
