@@ -691,6 +691,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 2160p large-motion leg")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined-component leg")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned-proposal leg of the 1080p workload (for per-kernel profiles)")
     ap.add_argument("--workload", default="1080p", choices=("1080p", "mixed64"),
                     help="1080p: batches of 1080p B pictures (default); mixed64: the mixed-size batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -732,6 +734,10 @@ def main():
         ctx.invalidate(-1)
         ctx.run_batch(jobs)
 
+    if args.learned_only:
+        print(json.dumps({"learned": learned_leg(ctx, jobs, frames, geom, B, args, float("nan"))}))
+        ctx.close()
+        return
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
