@@ -158,10 +158,10 @@ def _mc_bytes(geom, me):
     """Algorithmic HBM bytes of compensating one picture (k_mc), counted per
     unit with no reuse between units, as the reference computes it
     (CompensateBlock per unit, mot_comp.cpp:541-603): each unit's OBMC
-    window (clipped to the picture) reads 4 up-converted int16 samples per
-    sample from each reference its mode uses (none for DC, two for
-    bi-prediction); every sample of the three arrays is read and written once
-    (int16); plus the MvData."""
+    window (clipped to the picture) reads 4 up-converted samples per sample,
+    one byte each (the planes hold samples + 128), from each reference its
+    mode uses (none for DC, two for bi-prediction); every sample of the three
+    arrays is read and written once (int16); plus the MvData."""
     import numpy as np
     total = 0.0
     xnb, ynb, xsb = geom.xnum_blocks, geom.ynum_blocks, geom.xnum_sb
@@ -185,7 +185,7 @@ def _mc_bytes(geom, me):
                 cols = min(px + k * xbs + xov, pw) - max(px, 0)
                 area_refs += rows * cols * refs_of[int(me.mode[rb, u]) & 7]
                 u += k
-        total += area_refs * 8 * (1 if c == 0 else 2) + pw * ph * 4 * (1 if c == 0 else 2)
+        total += area_refs * 4 * (1 if c == 0 else 2) + pw * ph * 4 * (1 if c == 0 else 2)
     B = xnb * ynb
     return total + B * (2 * 8 + 4 + 3 * 2) + xsb * geom.ynum_sb * 4
 

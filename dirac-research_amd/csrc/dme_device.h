@@ -136,7 +136,7 @@ constexpr int kMcSegRows = 8;
 // One component of one picture being motion compensated (k_mc / k_mc_pad).
 // Built on the host, read by every workgroup of the component's tiles.
 struct McComp {
-    const int16_t DME_G *up[2];     // up-converted reference planes, 2pw x 2ph
+    const uint8_t DME_G *up[2];     // up-converted reference planes, 2pw x 2ph bytes (samples + 128)
     int16_t DME_G *out;             // array compensated in place, lx x ly
     const int4 DME_G *unit;         // per block: {v1.x, v1.y, v2.x, v2.y}, {mode, dc Y, dc U, dc V} (k_mc_pack)
     const int DME_G *split;         // xsb x ysb superblocks

@@ -35,6 +35,7 @@ namespace dme {
 
 hipError_t launch_downconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st);
+hipError_t launch_upconvert8(const PlaneTask *pt, int npt, int total, hipStream_t st);
 hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t st);
 hipError_t launch_mc_pack(const McPackJob *jobs, int njobs, int total, hipStream_t st);
 hipError_t launch_pel_full(const JobDesc *jobs, const Task *tasks, int ntasks, int total, int pp, int rp, int rows,
@@ -168,7 +169,7 @@ struct Picture {
     int pyr_levels = 0;             // levels built so far
     DevBuf up, qp;  // up-converted plane (built on request) and the quarter-pel planes
     bool up_ready = false, qp_ready = false;
-    DevBuf mcp[3];  // up-converted Y, U, V for motion compensation (k_upconvert)
+    DevBuf mcp[3];  // up-converted Y, U, V for motion compensation (k_upconvert<true>: bytes, samples + 128)
     bool mcp_ready = false;
     DevBuf b8;      // byte plane with margins (k_bytes)
     int bp = 0;     // its row pitch
@@ -644,8 +645,9 @@ static size_t full_lds_bytes(int xl, int yl, int xr, int yr) {
 }
 
 // Chroma of a picture within 8-bit range (checked once per registration):
-// EncPicture::Combine's float squares and sums are then exact.
-static int check_chroma(dme_ctx *ctx, Picture *P) {
+// EncPicture::Combine's float squares and sums are then exact, and motion
+// compensation's up-converted planes fit in bytes.
+static int check_chroma(dme_ctx *ctx, Picture *P, const char *what = "combined-component ME") {
     if (P->chroma_ok < 0) {
         const size_t cn = (size_t)(P->W / 2) * (P->H / 2);
         DME_TRY(ctx->d_flag.ensure(sizeof(int)));
@@ -658,7 +660,7 @@ static int check_chroma(dme_ctx *ctx, Picture *P) {
         P->chroma_ok = bad ? 0 : 1;
     }
     if (!P->chroma_ok)
-        return set_error(DME_ERR_INVALID, "combined-component ME: chroma of picture " + std::to_string(P->num) +
+        return set_error(DME_ERR_INVALID, std::string(what) + ": chroma of picture " + std::to_string(P->num) +
                                               " must lie in [-128, 127] (8-bit samples minus 128)");
     return DME_OK;
 }
@@ -2508,9 +2510,11 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
         for (int r = 0; r < m.j->nrefs; ++r) {
             Picture *P = m.ref[r];
             if (P->mcp_ready || std::find(building.begin(), building.end(), P) != building.end()) continue;
+            // byte planes: the reference's chroma must be 8-bit samples (its luma was checked at registration)
+            DME_TRY(check_chroma(ctx, P, "motion compensation"));
             for (int c = 0; c < 3; ++c) {
                 const int w = c ? P->W / 2 : P->W, h = c ? P->H / 2 : P->H;
-                DME_TRY(P->mcp[c].ensure(sizeof(int16_t) * 4 * (size_t)w * h));
+                DME_TRY(P->mcp[c].ensure(4 * (size_t)w * h));
                 add_task(c == 0 ? P->y : (c == 1 ? P->u : P->v), P->mcp[c].as<int16_t>(), w, h);
             }
             building.push_back(P);
@@ -2519,7 +2523,7 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
     if (!pts.empty()) {
         DME_TRY(upload(ctx, ctx->L->d_planes, pts.data(), sizeof(PlaneTask) * pts.size()));
         DME_TRY(timed_launch(ctx, KK_UPCONVERT, launches, [&] {
-            return launch_upconvert(ctx->L->d_planes.as<PlaneTask>(), (int)pts.size(), ptotal, ctx->L->stream);
+            return launch_upconvert8(ctx->L->d_planes.as<PlaneTask>(), (int)pts.size(), ptotal, ctx->L->stream);
         }));
         for (Picture *P : building) P->mcp_ready = true;
     }
@@ -2620,8 +2624,8 @@ static int mc_run(dme_ctx *ctx, const dme_mc_job *jobs, int njobs) {
             const int xblen = c ? g.cblen[0] : g.blen[2][0], yblen = c ? g.cblen[1] : g.blen[2][1];
             const int xbsep = c ? g.cbsep[0] : g.bsep[2][0], ybsep = c ? g.cbsep[1] : g.bsep[2][1];
             const Picture *R0 = m.ref[0], *R1 = m.ref[1];
-            C.up[0] = R0->mcp[c].as<int16_t>();
-            C.up[1] = R1->mcp[c].as<int16_t>();
+            C.up[0] = R0->mcp[c].as<uint8_t>();
+            C.up[1] = R1->mcp[c].as<uint8_t>();
             C.out = j.planes[c].data;
             C.unit = pk.out;
             C.split = split;

@@ -299,7 +299,10 @@ __device__ __forceinline__ int up_tap8(const int s[8]) {
 // picture) are staged in LDS once, the vertical filter of every staged column
 // goes to LDS (4 rows per thread, the filter sliding down the column), and
 // each thread then writes the 2 x 2 quads of 4 consecutive columns of one
-// row (the filter sliding along it), 16 bytes per up row.
+// row (the filter sliding along it), 16 bytes per up row.  U8: the plane of
+// motion compensation, one byte per sample biased by 128 (every sample of a
+// picture in 8-bit range lies in [-128, 127]: the source samples and the
+// clipped taps), 8 bytes per up row.
 constexpr int UC_IW = kUpTileW + 8, UC_IH = kUpTileH + 7;  // columns x0 - 4 .. x0 + 67
 
 __device__ __forceinline__ int up_tap8_lds(const int16_t *r) {
@@ -309,6 +312,11 @@ __device__ __forceinline__ int up_tap8_lds(const int16_t *r) {
 
 __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16); }
 
+__device__ __forceinline__ uint32_t pack8x2(int a, int b, int c, int d) {
+    return (uint32_t)(a + 128) | ((uint32_t)(b + 128) << 8) | ((uint32_t)(c + 128) << 16) | ((uint32_t)(d + 128) << 24);
+}
+
+template <bool U8>
 __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__ pt, int npt, int total) {
     __shared__ int16_t s_in[UC_IH][UC_IW + 1];
     __shared__ int16_t s_vt[kUpTileH][UC_IW + 1];
@@ -347,13 +355,38 @@ __global__ __launch_bounds__(256) void k_upconvert(const PlaneTask *__restrict__
     const int row = tid >> 4, x0 = x0t + 4 * (tid & 15);
     const int y = y0 + row;
     if (y >= H || x0 >= W) return;
-    uint32_t *__restrict__ up = reinterpret_cast<uint32_t *>(p.out);  // two samples per dword, W per up row
     int16_t wi[11], wv[11];
 #pragma unroll
     for (int t = 0; t < 11; ++t) {
         wi[t] = s_in[row + 3][x0 - x0t + 1 + t];  // columns x0 - 3 .. x0 + 7
         wv[t] = s_vt[row][x0 - x0t + 1 + t];
     }
+    if (U8) {
+        // up row 2y: sample, tap, sample, tap ... of columns x0 .. x0 + 3; 2W bytes per up row
+        uint8_t *__restrict__ up8 = reinterpret_cast<uint8_t *>(p.out);
+        uint8_t *r0 = up8 + (size_t)(2 * y) * (2 * W) + 2 * x0, *r1 = r0 + 2 * W;
+        int ta[4], tb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ta[k] = up_tap8_lds(&wi[k]);
+            tb[k] = up_tap8_lds(&wv[k]);
+        }
+        if ((W & 3) == 0) {  // rows start 8-byte aligned: one 8-byte store per row
+            *reinterpret_cast<uint2 *>(r0) = make_uint2(pack8x2(wi[3], ta[0], wi[4], ta[1]), pack8x2(wi[5], ta[2], wi[6], ta[3]));
+            *reinterpret_cast<uint2 *>(r1) = make_uint2(pack8x2(wv[3], tb[0], wv[4], tb[1]), pack8x2(wv[5], tb[2], wv[6], tb[3]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (x0 + k >= W) break;
+                r0[2 * k] = (uint8_t)(wi[k + 3] + 128);
+                r0[2 * k + 1] = (uint8_t)(ta[k] + 128);
+                r1[2 * k] = (uint8_t)(wv[k + 3] + 128);
+                r1[2 * k + 1] = (uint8_t)(tb[k] + 128);
+            }
+        }
+        return;
+    }
+    uint32_t *__restrict__ up = reinterpret_cast<uint32_t *>(p.out);  // two samples per dword, W per up row
     uint32_t *u0 = up + (size_t)(2 * y) * W, *u1 = up + (size_t)(2 * y + 1) * W;
     uint32_t a[4], b[4];
 #pragma unroll
@@ -5154,25 +5187,27 @@ __device__ __forceinline__ McTap mc_tap(int prec, int x, dme_mv mv, int uw) {
 // BlockPixelPred (mot_comp.cpp:846-1230) for one sample of row y; BIL:
 // quarter or eighth pel (bilinear between four up-converted samples), else
 // one sample.  Rows clamp like columns (BChk), so any y reads inside the plane.
+// The plane's bytes are samples biased by 128: the bilinear weights sum to
+// 2^(2 sh), so the bias passes through the rounded shift whole.
 template <bool BIL>
-__device__ __forceinline__ int mc_eval(const int16_t DME_G *__restrict__ up, int uw, int ty, int sh, int y,
+__device__ __forceinline__ int mc_eval(const uint8_t DME_G *__restrict__ up, int uw, int ty, int sh, int y,
                                        const McTap &t) {
     const int r0 = mc_bchk(2 * y + t.dy, ty);
-    const int16_t DME_G *ra = up + r0 * uw;
-    if (!BIL) return ra[t.j];
+    const uint8_t DME_G *ra = up + r0 * uw;
+    if (!BIL) return (int)ra[t.j] - 128;
     const int r1 = mc_bchk(2 * y + t.dy + 1, ty);
-    const int16_t DME_G *rb = up + r1 * uw;
+    const uint8_t DME_G *rb = up + r1 * uw;
     const int one = 1 << sh;
     const int v = (one - t.fx) * (one - t.fy) * ra[t.j] + t.fx * (one - t.fy) * ra[t.x1] +
                   (one - t.fx) * t.fy * rb[t.j] + t.fx * t.fy * rb[t.x1];
-    return (v + (1 << (2 * sh - 1))) >> (2 * sh);
+    return ((v + (1 << (2 * sh - 1))) >> (2 * sh)) - 128;
 }
 
 // Adds one unit's weighted prediction to the segment's rows; rows outside the
 // unit's window have weight 0 and are skipped (a uniform branch).
 template <bool BIL, bool TWO>
-__device__ __forceinline__ void mc_rows(int (&S)[kMcSegRows], const int16_t DME_G *__restrict__ upa,
-                                        const int16_t DME_G *__restrict__ upb, int uw, int ty, int sh, int y0,
+__device__ __forceinline__ void mc_rows(int (&S)[kMcSegRows], const uint8_t DME_G *__restrict__ upa,
+                                        const uint8_t DME_G *__restrict__ upb, int uw, int ty, int sh, int y0,
                                         const int (&W)[kMcSegRows], int H, const McTap &ta, const McTap &tb) {
 #pragma unroll
     for (int i = 0; i < kMcSegRows; ++i) {
@@ -5263,7 +5298,7 @@ __global__ __launch_bounds__(256) void k_mc(const McComp *__restrict__ cs, int n
                 m1.y = rv.y >> C.shift;
                 m2.x = rv.z >> C.shift;
                 m2.y = rv.w >> C.shift;
-                const int16_t DME_G *upa = mode == DME_REF2_ONLY ? C.up[1] : C.up[0];
+                const uint8_t DME_G *upa = mode == DME_REF2_ONLY ? C.up[1] : C.up[0];
                 const McTap ta = mc_tap(C.prec, x, mode == DME_REF2_ONLY ? m2 : m1, uw);
                 const McTap tb = mc_tap(C.prec, x, m2, uw);
                 const int sh = C.prec - 1;
@@ -5374,7 +5409,14 @@ hipError_t launch_mc(const McComp *cs, int ncs, int total, bool pad, hipStream_t
 
 hipError_t launch_upconvert(const PlaneTask *pt, int npt, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_upconvert, dim3(total), dim3(256), 0, st, pt, npt, total);
+    hipLaunchKernelGGL(k_upconvert<false>, dim3(total), dim3(256), 0, st, pt, npt, total);
+    return hipGetLastError();
+}
+
+// The byte planes of motion compensation (PlaneTask::out holds 4 W H bytes).
+hipError_t launch_upconvert8(const PlaneTask *pt, int npt, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upconvert<true>, dim3(total), dim3(256), 0, st, pt, npt, total);
     return hipGetLastError();
 }
 

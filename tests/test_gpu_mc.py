@@ -154,6 +154,14 @@ def test_mc_rejects_bad_arguments(ctx):
     with pytest.raises(DmeError):  # no estimate for this picture number
         ctx.motion_compensate_batch([dict(pic_num=12345, width=352, height=288, refs=nums, direction=SUB, params=p,
                                           me=None, planes=[(x.ptr, a.shape[1], a.shape[0]) for x, a in zip(dp, planes)])])
+    # a reference whose chroma is not 8-bit samples minus 128: its byte
+    # up-converted planes could not hold it
+    y0, u0, v0 = refs[0]
+    u_hot = np.ascontiguousarray(u0.copy())
+    u_hot[5, 7] = 200
+    ctx.upload(511, y0, u_hot, v0)
+    with pytest.raises(DmeError, match="motion compensation: chroma of picture 511"):
+        ctx.motion_compensate(p, 352, 288, me, [511], SUB, planes)
     # the arrays are untouched by the failed calls
     _, _, _, fresh, _, _ = M.load_case("mc_cif_q_sub")
     for c in range(3):
