@@ -5157,30 +5157,41 @@ constexpr int MC_ROWS = kMcSegRows;
 // Where a unit's prediction from one reference reads: the clamped columns j
 // and x1 of the up-converted plane, the row offset, the bilinear fractions.
 struct McTap {
-    int j, x1, dy, fx, fy;
+    int j, x1, dy;
+    int w00, w01, w10, w11;  // bilinear weights of (row, j), (row, x1), (row + 1, j), (row + 1, x1)
 };
 
 __device__ __forceinline__ McTap mc_tap(int prec, int x, dme_mv mv, int uw) {
     McTap t;
     int dx;
+    t.w00 = 1;
+    t.w01 = t.w10 = t.w11 = 0;
     if (prec == 0) {
         dx = 2 * mv.x;
         t.dy = 2 * mv.y;
-        t.fx = t.fy = 0;
     } else if (prec == 1) {
         dx = mv.x;
         t.dy = mv.y;
-        t.fx = t.fy = 0;
     } else {
-        const int sh = prec - 1;
+        const int sh = prec - 1, one = 1 << sh;
         dx = mv.x >> sh;
         t.dy = mv.y >> sh;
-        t.fx = mv.x & ((1 << sh) - 1);
-        t.fy = mv.y & ((1 << sh) - 1);
+        const int fx = mv.x & (one - 1), fy = mv.y & (one - 1);
+        t.w00 = (one - fx) * (one - fy);
+        t.w01 = fx * (one - fy);
+        t.w10 = (one - fx) * fy;
+        t.w11 = fx * fy;
     }
     // BChk against 2 size - 1 (mot_comp.cpp:1007-1008): columns 0 .. uw - 2
     t.j = min(max(2 * x + dx, 0), uw - 2);
     t.x1 = min(max(2 * x + dx + 1, 0), uw - 2);
+    // x1 is j + 1 unless a clamp made it j: then column j takes both weights,
+    // so that the taps always read columns j and j + 1 (j + 1 <= uw - 1)
+    if (t.x1 == t.j) {
+        t.w00 += t.w01;
+        t.w10 += t.w11;
+        t.w01 = t.w11 = 0;
+    }
     return t;
 }
 
@@ -5188,33 +5199,36 @@ __device__ __forceinline__ McTap mc_tap(int prec, int x, dme_mv mv, int uw) {
 // quarter or eighth pel (bilinear between four up-converted samples), else
 // one sample.  Rows clamp like columns (BChk), so any y reads inside the plane.
 // The plane's bytes are samples biased by 128: the bilinear weights sum to
-// 2^(2 sh), so the bias passes through the rounded shift whole.
+// 2^(2 sh), so the bias passes through the rounded shift whole.  up is the
+// same for the whole wave and the offsets are unsigned 32-bit, so each load
+// adds a lane offset to a scalar base.
 template <bool BIL>
 __device__ __forceinline__ int mc_eval(const uint8_t DME_G *__restrict__ up, int uw, int ty, int sh, int y,
                                        const McTap &t) {
-    const int r0 = mc_bchk(2 * y + t.dy, ty);
-    const uint8_t DME_G *ra = up + r0 * uw;
-    if (!BIL) return (int)ra[t.j] - 128;
-    const int r1 = mc_bchk(2 * y + t.dy + 1, ty);
-    const uint8_t DME_G *rb = up + r1 * uw;
-    const int one = 1 << sh;
-    const int v = (one - t.fx) * (one - t.fy) * ra[t.j] + t.fx * (one - t.fy) * ra[t.x1] +
-                  (one - t.fx) * t.fy * rb[t.j] + t.fx * t.fy * rb[t.x1];
+    const unsigned o0 = (unsigned)(mc_bchk(2 * y + t.dy, ty) * uw) + (unsigned)t.j;
+    if (!BIL) return (int)up[o0] - 128;
+    const unsigned o1 = (unsigned)(mc_bchk(2 * y + t.dy + 1, ty) * uw) + (unsigned)t.j;
+    const int v = t.w00 * up[o0] + t.w01 * up[o0 + 1u] + t.w10 * up[o1] + t.w11 * up[o1 + 1u];
     return ((v + (1 << (2 * sh - 1))) >> (2 * sh)) - 128;
 }
 
 // Adds one unit's weighted prediction to the segment's rows; rows outside the
-// unit's window have weight 0 and are skipped (a uniform branch).
-template <bool BIL, bool TWO>
-__device__ __forceinline__ void mc_rows(int (&S)[kMcSegRows], const uint8_t DME_G *__restrict__ upa,
-                                        const uint8_t DME_G *__restrict__ upb, int uw, int ty, int sh, int y0,
-                                        const int (&W)[kMcSegRows], int H, const McTap &ta, const McTap &tb) {
+// unit's window have weight 0 and are skipped (a uniform branch).  Lanes
+// predict from reference 1 (use1, tap t1, plane up1) and / or reference 2
+// (use2, t2, up2), both planes the wave's own; bi-prediction averages the two.
+template <bool BIL>
+__device__ __forceinline__ void mc_rows(int (&S)[kMcSegRows], const uint8_t DME_G *__restrict__ up1,
+                                        const uint8_t DME_G *__restrict__ up2, int uw, int ty, int sh, int y0,
+                                        const int (&W)[kMcSegRows], int H, const McTap &t1, const McTap &t2, bool use1,
+                                        bool use2) {
 #pragma unroll
     for (int i = 0; i < kMcSegRows; ++i) {
         const int y = y0 + i;
         if (W[i] == 0) continue;
-        int val = mc_eval<BIL>(upa, uw, ty, sh, y, ta);
-        if (TWO) val = (val + mc_eval<BIL>(upb, uw, ty, sh, y, tb) + 1) >> 1;
+        int v1 = 0, v2 = 0;
+        if (use1) v1 = mc_eval<BIL>(up1, uw, ty, sh, y, t1);
+        if (use2) v2 = mc_eval<BIL>(up2, uw, ty, sh, y, t2);
+        const int val = (use1 && use2) ? (v1 + v2 + 1) >> 1 : v1 + v2;
         S[i] += val * (W[i] * H);
     }
 }
@@ -5298,21 +5312,13 @@ __global__ __launch_bounds__(256) void k_mc(const McComp *__restrict__ cs, int n
                 m1.y = rv.y >> C.shift;
                 m2.x = rv.z >> C.shift;
                 m2.y = rv.w >> C.shift;
-                const uint8_t DME_G *upa = mode == DME_REF2_ONLY ? C.up[1] : C.up[0];
-                const McTap ta = mc_tap(C.prec, x, mode == DME_REF2_ONLY ? m2 : m1, uw);
-                const McTap tb = mc_tap(C.prec, x, m2, uw);
+                const McTap t1 = mc_tap(C.prec, x, m1, uw), t2 = mc_tap(C.prec, x, m2, uw);
+                const bool use1 = mode != DME_REF2_ONLY, use2 = mode != DME_REF1_ONLY;
                 const int sh = C.prec - 1;
-                if (C.prec >= 2) {
-                    if (mode == DME_REF1AND2)
-                        mc_rows<true, true>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
-                    else
-                        mc_rows<true, false>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
-                } else {
-                    if (mode == DME_REF1AND2)
-                        mc_rows<false, true>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
-                    else
-                        mc_rows<false, false>(S, upa, C.up[1], uw, ty, sh, y0, W, H, ta, tb);
-                }
+                if (C.prec >= 2)
+                    mc_rows<true>(S, C.up[0], C.up[1], uw, ty, sh, y0, W, H, t1, t2, use1, use2);
+                else
+                    mc_rows<false>(S, C.up[0], C.up[1], uw, ty, sh, y0, W, H, t1, t2, use1, use2);
             } else {
                 const int dcv = C.comp == 0 ? rm.y : (C.comp == 1 ? rm.z : rm.w);  // DCBlock
 #pragma unroll
