@@ -129,6 +129,10 @@ inline int combine_tiles(int W, int H) { return (int)(((size_t)W * H + 255) / 25
 constexpr int kUpTileW = 64, kUpTileH = 16;
 inline int upconvert_tiles(int w, int h) { return ((w + kUpTileW - 1) / kUpTileW) * ((h + kUpTileH - 1) / kUpTileH); }
 
+// Output tile of k_downconvert (a workgroup filters kDownTileW x kDownTileH
+// samples of the coarser level).
+constexpr int kDownTileW = 64, kDownTileH = 16;
+
 // Rows of the column segment one k_mc thread owns; a k_mc tile is 64 columns
 // by 4 segments (k_mc_pad tiles are 64 x 4).
 constexpr int kMcSegRows = 8;

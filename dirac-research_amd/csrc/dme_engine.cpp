@@ -524,7 +524,7 @@ static int build_planes(dme_ctx *ctx, const PlaneNeeds &nd, int &launches) {
             t.ow = ow;
             t.oh = oh;
             t.offset = off;
-            off += ((ow + 63) / 64) * ((oh + 7) / 8);  // 64 x 8 output tiles
+            off += ((ow + kDownTileW - 1) / kDownTileW) * ((oh + kDownTileH - 1) / kDownTileH);
             all.push_back(t);
         }
         ranges.push_back(std::make_pair(first, off));

@@ -219,10 +219,10 @@ __device__ unsigned long long wave_pel_argmin(const int16_t *__restrict__ pic, c
 // ---------------------------------------------------------------------------
 __constant__ int c_down_taps[6] = {86, 46, 4, -8, -4, 4};
 
-#define DC_TW 64                  // output tile width
-#define DC_TH 8                   // output tile height
+#define DC_TW kDownTileW          // output tile width
+#define DC_TH kDownTileH          // output tile height
 #define DC_IW (2 * DC_TW + 10)    // input tile columns (138)
-#define DC_IH (2 * DC_TH + 10)    // input tile rows (26)
+#define DC_IH (2 * DC_TH + 10)    // input tile rows (42)
 
 // One launch serves every plane of a batch: tiles are mapped to planes
 // through the offset table.
